@@ -1,0 +1,131 @@
+/*
+ * pwdft_hip.h -- C-ABI of libpwdft_hip.so, the MI355X (gfx950) implementation
+ * of PWDFT's sphere-cut Gamma-point Hamiltonian application H*psi and the
+ * eigensolver filter around it.
+ *
+ * Every entry point replaces one reference interface; the citation in front
+ * of each declaration names it (paths relative to the reference tree).
+ * The C++ adapter classes in pwdft_develop_amd/host/ (Fourier, Spinor,
+ * KohnSham, EigenSolver) keep the reference's signatures and call these.
+ *
+ * Conventions
+ *   - Grids: real index r = x + n1*(y + n2*z); R2C index
+ *     g = kx + h1*(ky + n2*kz), h1 = n1/2+1 (src/SCF/fourier.cpp:169-184).
+ *   - A band block is npw x nb complex, column-major, interleaved (re, im),
+ *     G = 0 first and sqrt(2) on the other half-sphere coefficients
+ *     (src/SCF/spinor.cpp:573-595).  Inner products are Re(a^H b).
+ *   - dtype PW_F64: complex double blocks; PW_F32: complex float blocks.
+ *   - Block pointers passed to compute entry points are DEVICE pointers,
+ *     caller-owned, leading dimension npw.  Setup arrays are HOST pointers.
+ *   - Every function returns PW_OK (0) or a negative PW_ERR_* code and sets
+ *     a thread-local message readable with pw_last_error().  The C++ adapter
+ *     turns non-zero codes into the reference's ErrorHandling exception
+ *     (src/global.cpp:71-91).
+ *   - A context is bound to one device and one stream; it is not re-entrant
+ *     (like Fourier's shared scratch, include/fourier.hpp:120-125).
+ */
+#ifndef PWDFT_HIP_H
+#define PWDFT_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PW_OK 0
+#define PW_ERR_ARG (-1)      /* invalid argument */
+#define PW_ERR_HIP (-2)      /* HIP / device error */
+#define PW_ERR_DOMAIN (-3)   /* "Domain size does not match" (spinor.cpp:549-551) */
+#define PW_ERR_STATE (-4)    /* "Fourier is not prepared." / missing setup */
+#define PW_ERR_LINALG (-5)   /* non-SPD Gram, "not enough number of columns" */
+#define PW_ERR_NOMEM (-6)
+#define PW_ERR_UNSUPPORTED (-7)
+
+#define PW_F64 0
+#define PW_F32 1
+
+typedef struct pw_ctx pw_ctx;
+
+/* Thread-local text of the last error (never NULL). */
+const char* pw_last_error(void);
+
+/* ------------------------------------------------------------------ */
+/* Host-side setup: replaces Fourier::Initialize (src/SCF/fourier.cpp:97-244)
+ * and Fourier::InitializeSphere real branch (src/SCF/fourier.cpp:541-612).
+ * supercell[3*j+i] = component i of lattice vector j (bohr).
+ * First call with idx == NULL to get *npw and *npad, then with arrays. */
+int pw_sphere_build(const int grid[3], const double supercell[9], double ecut,
+                    int* npw, int* npad, int* idx, double* gkk, int* pad_src,
+                    int* pad_dst);
+
+/* Fock sphere idxFineFock (src/SCF/fourier.cpp:642-670): R2C indices with
+ * |G|^2/2 < 4*ecut, ascending.  idx == NULL returns the count only. */
+int pw_fock_sphere_build(const int grid[3], const double supercell[9],
+                         double ecut, int* nfock, int* idx, double* gkk);
+
+/* KohnSham::CalculateEkin (src/SCF/hamiltonian.cpp:2629-2667). */
+int pw_ekin_teter(int npw, const double* gkk, double* ekin, double* teter);
+
+/* Projector block vnlc_ (src/SCF/hamiltonian.cpp:693-797) for Gaussian
+ * projectors r^l exp(-r^2/(2 sigma^2)) Y_lm, l = 0..lmax, taken in G space by
+ * the closed form of the radial integral that PeriodTable::CalculateNonlocalPP
+ * tabulates (src/SCF/periodtable.cpp:1599-1849); m order s | pz px py.
+ * wgt_channel[a*(lmax+1)+l] is the coupling of channel l of atom a; the
+ * weight written is wgt*(4 pi)^2/Vol (periodtable.cpp:1661-1668).
+ * beta: npw x nbeta complex, nbeta = natom*(lmax+1)^2; lmax <= 1. */
+int pw_gaussian_projectors(const int grid[3], const double supercell[9],
+                           int npw, const int* idx, int natom,
+                           const double* pos, double sigma, int lmax,
+                           const double* wgt_channel, double* beta, double* w);
+
+/* ------------------------------------------------------------------ */
+/* Device context.  Replaces the Fourier object plus the Hamiltonian state the
+ * hot path reads (vtot_, ekin_, teter_, vnlc_).  Inputs are the reference's
+ * own arrays: idxCoarseCut (npw), idxCoarsePadding first/second (npad), ekin,
+ * teter (npw).  max_bands bounds the nb of every later call.
+ * stream: a hipStream_t (NULL = default stream). */
+int pw_ctx_create(pw_ctx** ctx, const int grid[3], const double supercell[9],
+                  int npw, const int* idx, int npad, const int* pad_src,
+                  const int* pad_dst, const double* ekin, const double* teter,
+                  int max_bands, void* stream);
+int pw_ctx_destroy(pw_ctx* ctx);
+int pw_ctx_set_stream(pw_ctx* ctx, void* stream);
+/* npw, nstick, kxmax, N; informational. */
+int pw_ctx_info(const pw_ctx* ctx, int* npw, int* nstick, int* kxmax, long* ngrid);
+
+/* Local potential vtot_[:, spinswitch_] (src/SCF/hamiltonian.cpp:3333-3334),
+ * host array of N doubles. */
+int pw_set_vtot(pw_ctx* ctx, const double* vtot);
+
+/* vnlc_[0] = (beta, w) (src/SCF/hamiltonian.cpp:693-797), host arrays. */
+int pw_set_nonlocal(pw_ctx* ctx, int nbeta, const double* beta, const double* w);
+
+/* KohnSham::MultSpinor(Spinor&, NumTns<Complex>&, Fourier&)
+ * (src/SCF/hamiltonian.cpp:3270-3452) with Spinor::AddMultSpinorFineR2C
+ * (src/SCF/spinor.cpp:527-680): a3 = (-1/2 Laplacian + V_loc) psi, a3
+ * overwritten.  No nonlocal term, as in the reference. */
+int pw_mult_spinor(pw_ctx* ctx, int dtype, int nb, const void* psi, void* a3);
+
+/* EigenSolver::NonlocalMultX (src/EigenSolver/eigensolver.cpp:471-515):
+ * VnlX = beta * (w .* Re(beta^H X)), VnlX overwritten. */
+int pw_nonlocal_multx(pw_ctx* ctx, int dtype, int nb, const void* X, void* VnlX);
+
+/* MultSpinor followed by NonlocalMultX and the add, i.e. A*X as LOBPCG
+ * assembles it (src/EigenSolver/lobpcg.cpp:203-212, 312-321). */
+int pw_apply_h(pw_ctx* ctx, int dtype, int nb, const void* X, void* HX);
+
+/* Spinor::AddTeterPrecond (src/SCF/spinor.cpp:348-374): a3 += teter .* psi */
+int pw_teter_precond(pw_ctx* ctx, int dtype, int nb, const void* psi, void* a3);
+
+/* Stage timing of pw_mult_spinor / pw_apply_h calls, measured with HIP
+ * events recorded on the context's stream around each kernel (no host
+ * synchronisation).  pw_set_timing(1) clears the totals; pw_get_timing
+ * waits for the recorded events and returns the summed milliseconds per
+ * stage: [0] sticks -> inverse z, [1] xy slab, [2] forward z -> sphere,
+ * [3] nonlocal GEMMs; *ncalls = number of timed calls (<= 1024 per read). */
+int pw_set_timing(pw_ctx* ctx, int enable);
+int pw_get_timing(pw_ctx* ctx, double* ms4, int* ncalls);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,287 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's cpu_baseline leg, never by the product package.  Arrays are
+numpy, complex blocks are complex128 (npw x nb, column-major, i.e. numpy
+arrays of shape (nb, npw) in C order).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.or_sphere_build.restype = C.c_int
+        L.or_sphere_build.argtypes = [_ip, _dp, C.c_double, _ip, _dp, C.POINTER(C.c_int), _ip, _ip]
+        L.or_fock_sphere_build.argtypes = [_ip, _dp, C.c_double, _ip, _dp]
+        L.or_ekin_teter.argtypes = [C.c_int, _dp, _dp, _dp]
+        L.or_ekin_teter.restype = None
+        L.or_fft1d.argtypes = [C.c_int, C.c_int, _dp]
+        L.or_fft_c2r.argtypes = [_ip, _dp, _dp]
+        L.or_fft_r2c.argtypes = [_ip, _dp, _dp]
+        L.or_ham_create.restype = C.c_void_p
+        L.or_ham_create.argtypes = [_ip, _dp, C.c_int, _ip, C.c_int, _ip, _ip, _dp, _dp, _dp]
+        L.or_ham_destroy.argtypes = [C.c_void_p]
+        L.or_ham_destroy.restype = None
+        L.or_ham_set_nonlocal.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.or_ham_set_exx.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int,
+                                     C.c_int, _dp, _dp, C.POINTER(C.c_double)]
+        L.or_ham_set_threads.argtypes = [C.c_void_p, C.c_int]
+        for f in ("or_mult_spinor", "or_add_exx", "or_nonlocal_multx", "or_apply_h",
+                  "or_teter_precond"):
+            getattr(L, f).argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.or_gemm_tn.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.or_gemm_tn.restype = None
+        L.or_potrf_upper.argtypes = [C.c_int, _dp, C.c_int]
+        L.or_syev.argtypes = [C.c_int, _dp, C.c_int, _dp]
+        L.or_solve_gen_eig.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.or_orthogonalize.argtypes = [C.c_int, C.c_int, _dp]
+        L.or_lobpcg.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                C.c_double, _dp, _dp, _dp, C.POINTER(C.c_int)]
+        L.or_lanczos.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.POINTER(C.c_double)]
+        L.or_cheb_filter.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_double,
+                                     C.c_double, _dp]
+        L.or_rayleigh_ritz.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
+        L.or_chebfsi_sweep.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _dp, C.c_void_p,
+                                       _dp, _dp, _dp, _dp]
+        L.or_density.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_double, _dp]
+        L.or_band_energy.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.or_band_energy.restype = C.c_double
+        L.or_nonlocal_force.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp]
+        _LIB = L
+    return _LIB
+
+
+def _r(a):
+    """complex block -> contiguous float64 view"""
+    a = np.ascontiguousarray(a, dtype=np.complex128)
+    return a.view(np.float64).reshape(-1)
+
+
+def _chk(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"oracle {what} failed with code {rc}")
+
+
+def sphere(n, supercell, ecut):
+    n = np.asarray(n, dtype=np.int32)
+    cap = (n[0] // 2 + 1) * n[1] * n[2]
+    idx = np.zeros(cap, np.int32)
+    gkk = np.zeros(cap)
+    src = np.zeros(cap, np.int32)
+    dst = np.zeros(cap, np.int32)
+    npad = C.c_int(0)
+    npw = lib().or_sphere_build(n, np.ascontiguousarray(supercell, dtype=np.float64).reshape(-1),
+                                ecut, idx, gkk, C.byref(npad), src, dst)
+    if npw <= 0:
+        raise RuntimeError("sphere build failed")
+    return idx[:npw].copy(), gkk[:npw].copy(), src[:npad.value].copy(), dst[:npad.value].copy()
+
+
+def fock_sphere(n, supercell, ecut):
+    n = np.asarray(n, dtype=np.int32)
+    cap = (n[0] // 2 + 1) * n[1] * n[2]
+    idx = np.zeros(cap, np.int32)
+    gkk = np.zeros(cap)
+    cnt = lib().or_fock_sphere_build(n, np.ascontiguousarray(supercell, dtype=np.float64).reshape(-1),
+                                     ecut, idx, gkk)
+    return idx[:cnt].copy(), gkk[:cnt].copy()
+
+
+def ekin_teter(gkk):
+    gkk = np.ascontiguousarray(gkk, dtype=np.float64)
+    e = np.zeros_like(gkk)
+    t = np.zeros_like(gkk)
+    lib().or_ekin_teter(len(gkk), gkk, e, t)
+    return e, t
+
+
+def fft1d(x, sign):
+    x = np.array(x, dtype=np.complex128)
+    buf = _r(x).copy()
+    _chk(lib().or_fft1d(len(x), sign, buf), "fft1d")
+    return buf.view(np.complex128)
+
+
+def fft_c2r(n, half):
+    """half: complex array of shape (n3, n2, h1) -> real (n3, n2, n1)"""
+    n = np.asarray(n, dtype=np.int32)
+    out = np.zeros(int(n[0]) * int(n[1]) * int(n[2]))
+    _chk(lib().or_fft_c2r(n, _r(half), out), "c2r")
+    return out.reshape(int(n[2]), int(n[1]), int(n[0]))
+
+
+def fft_r2c(n, real):
+    n = np.asarray(n, dtype=np.int32)
+    h1 = int(n[0]) // 2 + 1
+    out = np.zeros(2 * h1 * int(n[1]) * int(n[2]))
+    _chk(lib().or_fft_r2c(n, np.ascontiguousarray(real, dtype=np.float64).reshape(-1), out), "r2c")
+    return out.view(np.complex128).reshape(int(n[2]), int(n[1]), h1)
+
+
+class Hamiltonian:
+    """Oracle Hamiltonian over one problem (grid, sphere, V, projectors, EXX)."""
+
+    def __init__(self, prob, threads=0):
+        L = lib()
+        self.prob = prob
+        self.npw = prob.npw
+        self.h = L.or_ham_create(np.asarray(prob.grid, np.int32), prob.supercell.reshape(-1).copy(),
+                                 prob.npw, prob.idx, len(prob.pad_src), prob.pad_src, prob.pad_dst,
+                                 prob.ekin, prob.teter, prob.vtot.reshape(-1).copy())
+        if threads:
+            L.or_ham_set_threads(self.h, threads)
+        if prob.nbeta > 0:
+            _chk(L.or_ham_set_nonlocal(self.h, prob.nbeta, _r(prob.beta), prob.wgt), "nonlocal")
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().or_ham_destroy(self.h)
+        except Exception:
+            pass
+
+    def set_threads(self, t):
+        lib().or_ham_set_threads(self.h, int(t))
+
+    def set_exx(self, ecut, mu, frac, psi_occ, occ, divtype=1):
+        d = C.c_double(0.0)
+        psi_occ = np.ascontiguousarray(psi_occ, dtype=np.complex128)
+        _chk(lib().or_ham_set_exx(self.h, ecut, mu, frac, divtype, psi_occ.shape[0], _r(psi_occ),
+                                  np.ascontiguousarray(occ, dtype=np.float64), C.byref(d)), "set_exx")
+        return d.value
+
+    def _op(self, name, psi):
+        psi = np.ascontiguousarray(psi, dtype=np.complex128)
+        out = np.zeros_like(psi)
+        o = _r(out)
+        _chk(getattr(lib(), name)(self.h, psi.shape[0], _r(psi), o), name)
+        return o.view(np.complex128).reshape(psi.shape)
+
+    def mult_spinor(self, psi):
+        return self._op("or_mult_spinor", psi)
+
+    def nonlocal_multx(self, psi):
+        return self._op("or_nonlocal_multx", psi)
+
+    def apply_h(self, psi):
+        return self._op("or_apply_h", psi)
+
+    def add_exx(self, psi):
+        return self._op("or_add_exx", psi)
+
+    def teter(self, psi):
+        return self._op("or_teter_precond", psi)
+
+    def lobpcg(self, psi, num_eig=None, scf_iter=1, max_iter=3, tol=1e-20):
+        psi = np.ascontiguousarray(psi, dtype=np.complex128).copy()
+        nb = psi.shape[0]
+        ev = np.zeros(nb)
+        rv = np.zeros(nb)
+        it = C.c_int(0)
+        buf = _r(psi)
+        _chk(lib().or_lobpcg(self.h, nb, num_eig or nb, scf_iter, max_iter, 0.0, tol, buf, ev, rv,
+                             C.byref(it)), "lobpcg")
+        return buf.view(np.complex128).reshape(psi.shape), ev, rv, it.value
+
+    def lanczos(self, k, v0):
+        th = np.zeros(k)
+        b = C.c_double(0.0)
+        _chk(lib().or_lanczos(self.h, k, _r(v0), th, C.byref(b)), "lanczos")
+        return th, b.value
+
+    def cheb_filter(self, X, m, a, b, aL):
+        buf = _r(np.array(X, dtype=np.complex128)).copy()
+        _chk(lib().or_cheb_filter(self.h, X.shape[0], m, a, b, aL, buf), "cheb_filter")
+        return buf.view(np.complex128).reshape(X.shape)
+
+    def rayleigh_ritz(self, X):
+        buf = _r(np.array(X, dtype=np.complex128)).copy()
+        nb = X.shape[0]
+        e = np.zeros(nb)
+        r = np.zeros(nb)
+        _chk(lib().or_rayleigh_ritz(self.h, nb, buf, e, r), "rayleigh_ritz")
+        return buf.view(np.complex128).reshape(X.shape), e, r
+
+    def chebfsi_sweep(self, X, degree, k, v0, prev_eig=None):
+        buf = _r(np.array(X, dtype=np.complex128)).copy()
+        nb = X.shape[0]
+        e = np.zeros(nb)
+        r = np.zeros(nb)
+        bounds = np.zeros(3)
+        pe = None
+        if prev_eig is not None:
+            pe_arr = np.ascontiguousarray(prev_eig, dtype=np.float64)
+            pe = pe_arr.ctypes.data_as(C.c_void_p)
+        _chk(lib().or_chebfsi_sweep(self.h, nb, degree, k, _r(v0), pe, buf, e, r, bounds),
+             "chebfsi_sweep")
+        return buf.view(np.complex128).reshape(X.shape), e, r, bounds
+
+    def density(self, psi, occ, num_spin=1.0):
+        rho = np.zeros(int(np.prod(self.prob.grid)))
+        _chk(lib().or_density(self.h, psi.shape[0], _r(psi), np.ascontiguousarray(occ, np.float64),
+                              num_spin, rho), "density")
+        return rho
+
+    def band_energy(self, psi, f):
+        return lib().or_band_energy(self.h, psi.shape[0], _r(psi), np.ascontiguousarray(f, np.float64))
+
+    def nonlocal_force(self, psi, f, natom, npa):
+        out = np.zeros(3 * natom)
+        _chk(lib().or_nonlocal_force(self.h, psi.shape[0], _r(psi), np.ascontiguousarray(f, np.float64),
+                                     natom, npa, out), "nonlocal_force")
+        return out.reshape(natom, 3)
+
+
+def gemm_tn(A, B):
+    A = np.asfortranarray(A, dtype=np.float64)
+    B = np.asfortranarray(B, dtype=np.float64)
+    m, n1 = A.shape
+    n2 = B.shape[1]
+    Cm = np.zeros(n1 * n2)
+    lib().or_gemm_tn(m, n1, n2, A.reshape(-1, order="F").copy(), B.reshape(-1, order="F").copy(), Cm)
+    return Cm.reshape(n2, n1).T
+
+
+def syev(A):
+    A = np.array(A, dtype=np.float64)
+    n = A.shape[0]
+    buf = A.reshape(-1, order="F").copy()
+    w = np.zeros(n)
+    _chk(lib().or_syev(n, buf, n, w), "syev")
+    return w, buf.reshape(n, n, order="F")
+
+
+def potrf_upper(A):
+    A = np.array(A, dtype=np.float64)
+    n = A.shape[0]
+    buf = A.reshape(-1, order="F").copy()
+    info = lib().or_potrf_upper(n, buf, n)
+    return info, np.triu(buf.reshape(n, n, order="F"))
+
+
+def solve_gen_eig(A, B, width):
+    n = A.shape[0]
+    a = np.array(A, dtype=np.float64).reshape(-1, order="F").copy()
+    b = np.array(B, dtype=np.float64).reshape(-1, order="F").copy()
+    e = np.zeros(n)
+    rc = lib().or_solve_gen_eig(n, n, width, a, b, e)
+    return rc, e[:width], a.reshape(n, n, order="F")

@@ -1,0 +1,746 @@
+// H*psi on gfx950: sphere-cut Gamma-point local + kinetic operator as three
+// kernels over a stick (z-column) decomposition of the half sphere, plus the
+// nonlocal projector GEMMs.
+//
+// Replaces Spinor::AddMultSpinorFineR2C (src/SCF/spinor.cpp:527-680): per
+// band, scatter into the padded R2C array, C2R, x 1/N, x V, R2C, gather with
+// sqrt(2) and add ekin .* c.  The reference does two dense 3D FFTs per band
+// (FFTW plan built per band).  Here the transform is pruned to where the data
+// lives: only the nstick sticks of the half sphere are transformed along z
+// (kernels 1 and 3) and only kx <= kxmax columns along y (kernel 2); the x
+// transform pairs two real rows into one complex Stockham FFT.  Many bands go
+// per launch (grid.y = band).  Results equal the dense transform's up to
+// rounding: the skipped entries are exact zeros.
+//
+//   kernel 1  sticks_inverse_z : gather sphere (+ conj mirror of the (0,0)
+//             column) -> LDS pencils, inverse z FFT -> T1[band][z][stick]
+//   kernel 2  slab_xy          : per (z, band): scatter sticks (+ conj
+//             mirrors of the kx = 0 plane), inverse y FFT, paired-row x c2r,
+//             x V/N, x r2c, forward y FFT, gather sticks -> T2
+//   kernel 3  sticks_forward_z : T2 -> forward z FFT -> sphere gather with
+//             sqrt(2), + ekin .* c -> a3
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "pw_internal.h"
+
+namespace pw {
+
+template <typename T> struct cx;
+template <> struct cx<double> { typedef double2 t; };
+template <> struct cx<float> { typedef float2 t; };
+
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+  return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+template <typename C> __device__ __forceinline__ C cmulc(C a, C b) {
+  return C{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+template <typename C, typename T> __device__ __forceinline__ C cscale(C a, T s) { return C{a.x * s, a.y * s}; }
+// multiply by (sg * i)
+template <typename C, typename T> __device__ __forceinline__ C cmul_si(C a, T sg) { return C{-sg * a.y, sg * a.x}; }
+
+// In-LDS radix-R butterfly with sign sg (-1 forward, +1 backward).
+template <typename T, int R>
+__device__ __forceinline__ void dft(typename cx<T>::t* v, T sg) {
+  typedef typename cx<T>::t C;
+  if constexpr (R == 2) {
+    C a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    C t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    C t2 = cadd(v[1], v[3]), t3 = cmul_si(csub(v[1], v[3]), sg);
+    v[0] = cadd(t0, t2); v[1] = cadd(t1, t3); v[2] = csub(t0, t2); v[3] = csub(t1, t3);
+  } else if constexpr (R == 3) {
+    const T h = (T)0.86602540378443864676372317075294;
+    C a = v[0], b = v[1], c = v[2];
+    C bc = cadd(b, c);
+    C t = C{a.x - (T)0.5 * bc.x, a.y - (T)0.5 * bc.y};
+    C u = cmul_si(cscale(csub(b, c), h), sg);
+    v[0] = cadd(a, bc); v[1] = cadd(t, u); v[2] = csub(t, u);
+  } else if constexpr (R == 5) {
+    const T c1 = (T)0.30901699437494742410229341718282;
+    const T c2 = (T)-0.80901699437494742410229341718282;
+    const T s1 = (T)0.95105651629515357211643933337938;
+    const T s2 = (T)0.58778525229247312916870595463907;
+    C a = v[0];
+    C t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]), t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    C m1 = C{a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y};
+    C m2 = C{a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y};
+    C n1 = cmul_si(C{s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y}, sg);
+    C n2 = cmul_si(C{s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y}, sg);
+    v[0] = cadd(a, cadd(t1, t2)); v[1] = cadd(m1, n1); v[4] = csub(m1, n1);
+    v[2] = cadd(m2, n2); v[3] = csub(m2, n2);
+  }
+}
+
+// One Stockham pass of radix R over P pencils of length n (stride ld) held in
+// LDS: src -> dst.  tw[m] = exp(+2 pi i m / n), conjugated for sg < 0.
+template <typename T, int R>
+__device__ __forceinline__ void stockham_pass(const typename cx<T>::t* __restrict__ src,
+                                              typename cx<T>::t* __restrict__ dst, int n, int P,
+                                              int ld, int Ns, const typename cx<T>::t* tw, T sg) {
+  typedef typename cx<T>::t C;
+  const int m = n / R;
+  const int tstep = n / (Ns * R);
+  const int total = P * m;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int p = t / m;
+    const int j = t - p * m;
+    const int k = j % Ns;
+    const C* s = src + p * ld + j;
+    C v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = s[r * m];
+#pragma unroll
+    for (int r = 1; r < R; r++) {
+      C w = tw[r * k * tstep];  // r*k*tstep < n
+      w.y *= sg;
+      v[r] = cmul(v[r], w);
+    }
+    dft<T, R>(v, sg);
+    C* d = dst + p * ld + (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; r++) d[r * Ns] = v[r];
+  }
+}
+
+// Full batched 1D FFT in LDS (ping-pong a <-> b); returns the buffer holding
+// the result.  Radices are taken 4 first, then 2, 3, 5 (the order of
+// radix_plan on the host), derived from n so no table is indexed at run
+// time.  Ends with a barrier.
+template <typename T>
+__device__ typename cx<T>::t* lds_fft(typename cx<T>::t* a, typename cx<T>::t* b, int n, int P,
+                                      int ld, const typename cx<T>::t* tw, T sg) {
+  typedef typename cx<T>::t C;
+  C* src = a;
+  C* dst = b;
+  int Ns = 1;
+  int rem = n;
+  while (rem > 1) {
+    int R;
+    if (rem % 4 == 0) {
+      R = 4;
+      stockham_pass<T, 4>(src, dst, n, P, ld, Ns, tw, sg);
+    } else if (rem % 2 == 0) {
+      R = 2;
+      stockham_pass<T, 2>(src, dst, n, P, ld, Ns, tw, sg);
+    } else if (rem % 3 == 0) {
+      R = 3;
+      stockham_pass<T, 3>(src, dst, n, P, ld, Ns, tw, sg);
+    } else {
+      R = 5;
+      stockham_pass<T, 5>(src, dst, n, P, ld, Ns, tw, sg);
+    }
+    __syncthreads();
+    C* tmp = src;
+    src = dst;
+    dst = tmp;
+    Ns *= R;
+    rem /= R;
+  }
+  return src;
+}
+
+// ---------------------------------------------------------------- kernel 1
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_sticks_inverse_z(const typename cx<T>::t* __restrict__ psi, int ldpsi,
+                   typename cx<T>::t* __restrict__ t1, int n3, int nstick, int sc,
+                   const int* __restrict__ chunk_off, const int* __restrict__ pt_idx,
+                   const int* __restrict__ pt_sz, const typename cx<T>::t* __restrict__ tw3g) {
+  typedef typename cx<T>::t C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* tw = reinterpret_cast<C*>(smem_raw);
+  C* A = tw + n3;
+  C* B = A + sc * n3;
+  const int c = blockIdx.x, band = blockIdx.y;
+  const int s0 = c * sc;
+  const C* ps = psi + (size_t)band * ldpsi;
+  for (int t = threadIdx.x; t < n3; t += blockDim.x) tw[t] = tw3g[t];
+  for (int t = threadIdx.x; t < sc * n3; t += blockDim.x) A[t] = C{0, 0};
+  __syncthreads();
+  const T is2 = (T)0.70710678118654752440084436210485;
+  const int e = chunk_off[c + 1];
+  for (int t = chunk_off[c] + threadIdx.x; t < e; t += blockDim.x) {
+    const int i = pt_idx[t];
+    const int sz = pt_sz[t];
+    C v;
+    if (i >= 0) {
+      v = ps[i];
+      if (i != 0) v = cscale(v, is2);
+    } else {
+      C u = ps[-i - 1];
+      v = C{u.x * is2, -u.y * is2};
+    }
+    A[(sz & 0xffff) * n3 + (sz >> 16)] = v;
+  }
+  __syncthreads();
+  C* R = lds_fft<T>(A, B, n3, sc, n3, tw, (T)1);
+  const int nsl = min(sc, nstick - s0);
+  for (int t = threadIdx.x; t < n3 * nsl; t += blockDim.x) {
+    const int z = t / nsl, sl = t - z * nsl;
+    t1[((size_t)band * n3 + z) * nstick + s0 + sl] = R[sl * n3 + z];
+  }
+}
+
+// ---------------------------------------------------------------- kernel 2
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_slab_xy(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restrict__ t2, int n1,
+          int n2, int n3, int nstick, int kxmax, const int* __restrict__ pos,
+          const int* __restrict__ mpos, const T* __restrict__ vtot,
+          const typename cx<T>::t* __restrict__ tw1g, const typename cx<T>::t* __restrict__ tw2g,
+          int rp) {
+  typedef typename cx<T>::t C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int nk = kxmax + 1;
+  C* tw1 = reinterpret_cast<C*>(smem_raw);
+  C* tw2 = tw1 + n1;
+  C* Y0 = tw2 + n2;
+  C* Y1 = Y0 + nk * n2;
+  C* Z0 = Y1 + nk * n2;
+  C* Z1 = Z0 + rp * n1;
+  const int z = blockIdx.x, band = blockIdx.y;
+  for (int t = threadIdx.x; t < n1; t += blockDim.x) tw1[t] = tw1g[t];
+  for (int t = threadIdx.x; t < n2; t += blockDim.x) tw2[t] = tw2g[t];
+  for (int t = threadIdx.x; t < nk * n2; t += blockDim.x) Y0[t] = C{0, 0};
+  __syncthreads();
+  const C* src = t1 + ((size_t)band * n3 + z) * nstick;
+  for (int s = threadIdx.x; s < nstick; s += blockDim.x) {
+    const C v = src[s];
+    Y0[pos[s]] = v;
+    const int mp = mpos[s];
+    if (mp >= 0) Y0[mp] = C{v.x, -v.y};
+  }
+  __syncthreads();
+  // inverse y on the kx <= kxmax columns
+  C* Y = lds_fft<T>(Y0, Y1, n2, nk, n2, tw2, (T)1);
+  const int npair = (n2 + 1) / 2;
+  const T* vz = vtot + (size_t)z * n1 * n2;
+  const int h = n1 / 2;
+  for (int q0 = 0; q0 < npair; q0 += rp) {
+    const int nq = min(rp, npair - q0);
+    // Hermitian-extended spectra of rows y1 = 2q, y2 = 2q+1 packed as X1 + i X2
+    for (int t = threadIdx.x; t < nq * n1; t += blockDim.x) {
+      const int ql = t / n1, k = t - ql * n1;
+      const int y1 = 2 * (q0 + ql), y2 = y1 + 1;
+      C a1 = C{0, 0}, a2 = C{0, 0};
+      int kk = k;
+      bool cj = false;
+      if (k > h) { kk = n1 - k; cj = true; }
+      if (kk <= kxmax) {
+        a1 = Y[kk * n2 + y1];
+        if (y2 < n2) a2 = Y[kk * n2 + y2];
+        if (cj) { a1.y = -a1.y; a2.y = -a2.y; }
+        if (k == 0 || (2 * k == n1)) { a1.y = 0; a2.y = 0; }
+      }
+      Z0[ql * n1 + k] = C{a1.x - a2.y, a1.y + a2.x};
+    }
+    __syncthreads();
+    C* Zr = lds_fft<T>(Z0, Z1, n1, nq, n1, tw1, (T)1);
+    // real rows -> x V/N (V already carries the 1/N of spinor.cpp:620-621)
+    for (int t = threadIdx.x; t < nq * n1; t += blockDim.x) {
+      const int ql = t / n1, x = t - ql * n1;
+      const int y1 = 2 * (q0 + ql), y2 = y1 + 1;
+      C u = Zr[ql * n1 + x];
+      const T v1 = vz[(size_t)y1 * n1 + x];
+      const T v2 = (y2 < n2) ? vz[(size_t)y2 * n1 + x] : (T)0;
+      Zr[ql * n1 + x] = C{u.x * v1, u.y * v2};
+    }
+    __syncthreads();
+    C* Zo = (Zr == Z0) ? Z1 : Z0;
+    C* Zf = lds_fft<T>(Zr, Zo, n1, nq, n1, tw1, (T)-1);
+    // unpack the two real-row spectra for kx <= kxmax back into Y
+    for (int t = threadIdx.x; t < nq * nk; t += blockDim.x) {
+      const int ql = t / nk, k = t - ql * nk;
+      const int y1 = 2 * (q0 + ql), y2 = y1 + 1;
+      const C w = Zf[ql * n1 + k];
+      const C wc = Zf[ql * n1 + (k == 0 ? 0 : n1 - k)];
+      // U1 = (w + conj(wc))/2, U2 = (w - conj(wc))/(2i)
+      Y[k * n2 + y1] = C{(T)0.5 * (w.x + wc.x), (T)0.5 * (w.y - wc.y)};
+      if (y2 < n2) Y[k * n2 + y2] = C{(T)0.5 * (w.y + wc.y), (T)0.5 * (wc.x - w.x)};
+    }
+    __syncthreads();
+  }
+  C* Yo = (Y == Y0) ? Y1 : Y0;
+  C* Yf = lds_fft<T>(Y, Yo, n2, nk, n2, tw2, (T)-1);
+  C* dst = t2 + ((size_t)band * n3 + z) * nstick;
+  for (int s = threadIdx.x; s < nstick; s += blockDim.x) dst[s] = Yf[pos[s]];
+}
+
+// ---------------------------------------------------------------- kernel 3
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_sticks_forward_z(const typename cx<T>::t* __restrict__ t2, const typename cx<T>::t* __restrict__ psi,
+                   typename cx<T>::t* __restrict__ out, int ld, int n3, int nstick, int sc,
+                   const int* __restrict__ gchunk_off, const int* __restrict__ gpt_idx,
+                   const int* __restrict__ gpt_sz, const T* __restrict__ ekin,
+                   const typename cx<T>::t* __restrict__ tw3g) {
+  typedef typename cx<T>::t C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* tw = reinterpret_cast<C*>(smem_raw);
+  C* A = tw + n3;
+  C* B = A + sc * n3;
+  const int c = blockIdx.x, band = blockIdx.y;
+  const int s0 = c * sc;
+  const int nsl = min(sc, nstick - s0);
+  for (int t = threadIdx.x; t < n3; t += blockDim.x) tw[t] = tw3g[t];
+  for (int t = threadIdx.x; t < n3 * sc; t += blockDim.x) {
+    const int z = t / sc, sl = t - z * sc;
+    A[sl * n3 + z] = (sl < nsl) ? t2[((size_t)band * n3 + z) * nstick + s0 + sl] : C{0, 0};
+  }
+  __syncthreads();
+  C* R = lds_fft<T>(A, B, n3, sc, n3, tw, (T)-1);
+  const C* ps = psi + (size_t)band * ld;
+  C* o = out + (size_t)band * ld;
+  const T s2 = (T)1.4142135623730950488016887242097;
+  const int e = gchunk_off[c + 1];
+  for (int t = gchunk_off[c] + threadIdx.x; t < e; t += blockDim.x) {
+    const int i = gpt_idx[t];
+    const int sz = gpt_sz[t];
+    C y = R[(sz & 0xffff) * n3 + (sz >> 16)];
+    if (i != 0) y = cscale(y, s2);
+    const C cc = ps[i];
+    const T ek = ekin[i];
+    o[i] = C{ek * cc.x + y.x, ek * cc.y + y.y};
+  }
+}
+
+template <typename T>
+__global__ void k_teter(const typename cx<T>::t* __restrict__ psi, typename cx<T>::t* __restrict__ a3,
+                        const T* __restrict__ teter, int npw, long total) {
+  typedef typename cx<T>::t C;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % npw);
+    const C v = psi[t];
+    C a = a3[t];
+    a.x += v.x * teter[i];
+    a.y += v.y * teter[i];
+    a3[t] = a;
+  }
+}
+
+}  // namespace pw
+
+using namespace pw;
+
+// ----------------------------------------------------------------- host side
+
+static void build_stick_plan(pw_ctx* c, const int* idx) {
+  StickPlan& sp = c->sp;
+  const int h1 = c->h1, n2 = c->n2, n3 = c->n3, npw = c->npw;
+  // stick id per (kx, ky): sticks ordered by (ky storage index, kx)
+  std::vector<int> sid((size_t)h1 * n2, -1);
+  std::vector<char> used((size_t)h1 * n2, 0);
+  for (int i = 0; i < npw; i++) {
+    const int g = idx[i];
+    used[g % (h1 * n2)] = 1;
+  }
+  sp.kx.clear();
+  sp.ky.clear();
+  sp.kxmax = 0;
+  for (int ky = 0; ky < n2; ky++)
+    for (int kx = 0; kx < h1; kx++)
+      if (used[kx + (size_t)h1 * ky]) {
+        sid[kx + (size_t)h1 * ky] = (int)sp.kx.size();
+        sp.kx.push_back(kx);
+        sp.ky.push_back(ky);
+        if (kx > sp.kxmax) sp.kxmax = kx;
+      }
+  sp.nstick = (int)sp.kx.size();
+  sp.nchunk = (sp.nstick + sp.sc - 1) / sp.sc;
+  sp.pos.resize(sp.nstick);
+  sp.mpos.resize(sp.nstick);
+  for (int s = 0; s < sp.nstick; s++) {
+    sp.pos[s] = sp.kx[s] * n2 + sp.ky[s];
+    sp.mpos[s] = (sp.kx[s] == 0 && sp.ky[s] > 0) ? (n2 - sp.ky[s]) % n2 : -1;
+  }
+  // point lists per chunk, (z, stick)-ordered
+  struct P { int z, sl, i; };
+  std::vector<std::vector<P>> fw(sp.nchunk), inv(sp.nchunk);
+  for (int i = 0; i < npw; i++) {
+    const int g = idx[i];
+    const int col = g % (h1 * n2), kz = g / (h1 * n2);
+    const int s = sid[col];
+    const int ch = s / sp.sc, sl = s % sp.sc;
+    fw[ch].push_back({kz, sl, i});
+    inv[ch].push_back({kz, sl, i});
+    // the (0,0) column holds kz >= 0 only; its conj mirror fills -kz
+    if (sp.kx[s] == 0 && sp.ky[s] == 0 && kz != 0) inv[ch].push_back({(n3 - kz) % n3, sl, -(i + 1)});
+  }
+  auto flatten = [](std::vector<std::vector<P>>& L, std::vector<int>& off, std::vector<int>& pi,
+                    std::vector<int>& psz) {
+    off.assign(L.size() + 1, 0);
+    pi.clear();
+    psz.clear();
+    for (size_t c = 0; c < L.size(); c++) {
+      std::sort(L[c].begin(), L[c].end(), [](const P& a, const P& b) {
+        return a.z != b.z ? a.z < b.z : a.sl < b.sl;
+      });
+      for (const P& p : L[c]) {
+        pi.push_back(p.i);
+        psz.push_back(p.sl | (p.z << 16));
+      }
+      off[c + 1] = (int)pi.size();
+    }
+  };
+  flatten(inv, sp.chunk_off, sp.pt_idx, sp.pt_sz);
+  flatten(fw, sp.gchunk_off, sp.gpt_idx, sp.gpt_sz);
+}
+
+template <typename X>
+static int upload(X** d, const X* h, size_t n) {
+  PW_HIP(hipMalloc((void**)d, sizeof(X) * (n > 0 ? n : 1)));
+  if (n) PW_HIP(hipMemcpy(*d, h, sizeof(X) * n, hipMemcpyHostToDevice));
+  return PW_OK;
+}
+
+static void roots(int n, std::vector<double>& out) {
+  for (int m = 0; m < n; m++) {
+    long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
+    out.push_back((double)cosl(a));
+    out.push_back((double)sinl(a));
+  }
+}
+
+extern "C" int pw_ctx_create(pw_ctx** out, const int grid[3], const double supercell[9], int npw,
+                             const int* idx, int npad, const int* pad_src, const int* pad_dst,
+                             const double* ekin, const double* teter, int max_bands, void* stream) {
+  if (!out || !grid || !supercell || !idx || !ekin || !teter || npw <= 0 || max_bands <= 0)
+    return fail(PW_ERR_ARG, "pw_ctx_create: bad argument");
+  if (npad > 0 && (!pad_src || !pad_dst)) return fail(PW_ERR_ARG, "pw_ctx_create: null padding arrays");
+  for (int d = 0; d < 3; d++)
+    if (!smooth235(grid[d]))
+      return fail(PW_ERR_UNSUPPORTED, "pw_ctx_create: grid size %d is not 2/3/5-smooth", grid[d]);
+  if (grid[2] >= 32768) return fail(PW_ERR_UNSUPPORTED, "pw_ctx_create: n3 too large");
+  pw_ctx* c = new pw_ctx();
+  c->n1 = grid[0]; c->n2 = grid[1]; c->n3 = grid[2];
+  c->h1 = grid[0] / 2 + 1;
+  c->N = (long)grid[0] * grid[1] * grid[2];
+  c->Nh = (long)c->h1 * grid[1] * grid[2];
+  memcpy(c->supercell, supercell, sizeof(double) * 9);
+  recip_cell(supercell, c->recip);
+  c->vol = cell_volume(supercell);
+  c->npw = npw;
+  c->npad = npad;
+  c->max_bands = max_bands;
+  c->stream = (hipStream_t)stream;
+  // validate the sphere: G = 0 first, all entries in range, and the padding is
+  // exactly the conj mirror of the kx = 0 plane (fourier.cpp:580-612)
+  if (idx[0] != 0) { delete c; return fail(PW_ERR_DOMAIN, "pw_ctx_create: idx[0] must be G = 0"); }
+  for (int i = 0; i < npw; i++)
+    if (idx[i] < 0 || idx[i] >= c->Nh) { delete c; return fail(PW_ERR_DOMAIN, "pw_ctx_create: idx out of range"); }
+  {
+    std::vector<int> where((size_t)c->Nh, -1);
+    for (int i = 0; i < npw; i++) where[idx[i]] = i;
+    int expect = 0;
+    for (int i = 0; i < npw; i++) {
+      const int g = idx[i];
+      const int kx = g % c->h1, ky = (g / c->h1) % c->n2, kz = g / (c->h1 * c->n2);
+      if (kx == 0 && !(ky == 0 && kz == 0)) expect++;
+    }
+    if (expect != npad) { delete c; return fail(PW_ERR_DOMAIN, "pw_ctx_create: padding count %d != %d", npad, expect); }
+    for (int p = 0; p < npad; p++) {
+      const int s = pad_src[p], d = pad_dst[p];
+      if (s < 0 || s >= c->Nh || d < 0 || d >= c->Nh || where[s] < 0) {
+        delete c;
+        return fail(PW_ERR_DOMAIN, "pw_ctx_create: padding pair %d out of the sphere", p);
+      }
+      const int ky = (s / c->h1) % c->n2, kz = s / (c->h1 * c->n2);
+      const int mky = (c->n2 - ky) % c->n2, mkz = (c->n3 - kz) % c->n3;
+      if (s % c->h1 != 0 || d != c->h1 * (mky + c->n2 * mkz)) {
+        delete c;
+        return fail(PW_ERR_DOMAIN, "pw_ctx_create: padding pair %d is not a Hermitian mirror", p);
+      }
+    }
+  }
+  build_stick_plan(c, idx);
+  c->f1 = radix_plan(c->n1);
+  c->f2 = radix_plan(c->n2);
+  c->f3 = radix_plan(c->n3);
+  int rc;
+  PW_HIP(hipGetDevice(&c->device));
+#define UP(d, h, n)                                       \
+  if ((rc = upload(&(d), (h), (n))) != PW_OK) {           \
+    pw_ctx_destroy(c);                                    \
+    return rc;                                            \
+  }
+  UP(c->d_pt_idx, c->sp.pt_idx.data(), c->sp.pt_idx.size());
+  UP(c->d_pt_sz, c->sp.pt_sz.data(), c->sp.pt_sz.size());
+  UP(c->d_chunk_off, c->sp.chunk_off.data(), c->sp.chunk_off.size());
+  UP(c->d_gpt_idx, c->sp.gpt_idx.data(), c->sp.gpt_idx.size());
+  UP(c->d_gpt_sz, c->sp.gpt_sz.data(), c->sp.gpt_sz.size());
+  UP(c->d_gchunk_off, c->sp.gchunk_off.data(), c->sp.gchunk_off.size());
+  UP(c->d_pos, c->sp.pos.data(), c->sp.pos.size());
+  UP(c->d_mpos, c->sp.mpos.data(), c->sp.mpos.size());
+  UP(c->d_ekin, ekin, (size_t)npw);
+  UP(c->d_teter, teter, (size_t)npw);
+  {
+    std::vector<float> ef(ekin, ekin + npw), tf(teter, teter + npw);
+    UP(c->d_ekin_f, ef.data(), ef.size());
+    UP(c->d_teter_f, tf.data(), tf.size());
+    std::vector<double> tw;
+    roots(c->n1, tw);
+    roots(c->n2, tw);
+    roots(c->n3, tw);
+    std::vector<float> twf(tw.begin(), tw.end());
+    UP(c->d_tw64, tw.data(), tw.size());
+    UP(c->d_tw32, twf.data(), twf.size());
+  }
+#undef UP
+  c->tbytes = sizeof(double2) * (size_t)max_bands * c->n3 * c->sp.nstick;
+  if (hipMalloc(&c->d_t1, c->tbytes) != hipSuccess || hipMalloc(&c->d_t2, c->tbytes) != hipSuccess) {
+    pw_ctx_destroy(c);
+    return fail(PW_ERR_NOMEM, "pw_ctx_create: cannot allocate %zu bytes of stick buffers", 2 * c->tbytes);
+  }
+  if (rocblas_create_handle(&c->blas) != rocblas_status_success) {
+    pw_ctx_destroy(c);
+    return fail(PW_ERR_HIP, "pw_ctx_create: rocblas_create_handle failed");
+  }
+  rocblas_set_stream(c->blas, c->stream);
+  *out = c;
+  return PW_OK;
+}
+
+extern "C" int pw_ctx_destroy(pw_ctx* c) {
+  if (!c) return PW_OK;
+  void* ptrs[] = {c->d_pt_idx, c->d_pt_sz, c->d_chunk_off, c->d_gpt_idx, c->d_gpt_sz, c->d_gchunk_off,
+                  c->d_pos, c->d_mpos, c->d_ekin, c->d_teter, c->d_ekin_f, c->d_teter_f, c->d_tw64,
+                  c->d_tw32, c->d_vtot, c->d_vtot_f, c->d_beta, c->d_betaw, c->d_W, c->d_beta_f,
+                  c->d_betaw_f, c->d_W_f, c->d_t1, c->d_t2};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->blas) rocblas_destroy_handle(c->blas);
+  for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
+  delete c;
+  return PW_OK;
+}
+
+extern "C" int pw_ctx_set_stream(pw_ctx* c, void* stream) {
+  if (!c) return fail(PW_ERR_ARG, "null context");
+  c->stream = (hipStream_t)stream;
+  rocblas_set_stream(c->blas, c->stream);
+  return PW_OK;
+}
+
+extern "C" int pw_ctx_info(const pw_ctx* c, int* npw, int* nstick, int* kxmax, long* ngrid) {
+  if (!c) return fail(PW_ERR_ARG, "null context");
+  if (npw) *npw = c->npw;
+  if (nstick) *nstick = c->sp.nstick;
+  if (kxmax) *kxmax = c->sp.kxmax;
+  if (ngrid) *ngrid = c->N;
+  return PW_OK;
+}
+
+extern "C" int pw_set_vtot(pw_ctx* c, const double* vtot) {
+  if (!c || !vtot) return fail(PW_ERR_ARG, "pw_set_vtot: null argument");
+  std::vector<double> v(vtot, vtot + c->N);
+  const double inv = 1.0 / (double)c->N;
+  for (auto& x : v) x *= inv;  // fold the C2R normalisation of spinor.cpp:620-621
+  std::vector<float> vf(v.begin(), v.end());
+  if (!c->d_vtot) {
+    PW_HIP(hipMalloc((void**)&c->d_vtot, sizeof(double) * c->N));
+    PW_HIP(hipMalloc((void**)&c->d_vtot_f, sizeof(float) * c->N));
+  }
+  PW_HIP(hipMemcpy(c->d_vtot, v.data(), sizeof(double) * c->N, hipMemcpyHostToDevice));
+  PW_HIP(hipMemcpy(c->d_vtot_f, vf.data(), sizeof(float) * c->N, hipMemcpyHostToDevice));
+  c->have_vtot = true;
+  return PW_OK;
+}
+
+extern "C" int pw_set_nonlocal(pw_ctx* c, int nbeta, const double* beta, const double* w) {
+  if (!c || nbeta < 0 || (nbeta > 0 && (!beta || !w))) return fail(PW_ERR_ARG, "pw_set_nonlocal: bad argument");
+  void* ptrs[] = {c->d_beta, c->d_betaw, c->d_W, c->d_beta_f, c->d_betaw_f, c->d_W_f};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  c->d_beta = c->d_betaw = c->d_W = nullptr;
+  c->d_beta_f = c->d_betaw_f = c->d_W_f = nullptr;
+  c->nbeta = nbeta;
+  if (nbeta == 0) return PW_OK;
+  const size_t m = 2 * (size_t)c->npw;
+  std::vector<double> bw(beta, beta + m * nbeta);
+  for (int p = 0; p < nbeta; p++)
+    for (size_t r = 0; r < m; r++) bw[m * p + r] *= w[p];
+  std::vector<float> bf(beta, beta + m * nbeta), bwf(bw.begin(), bw.end());
+  int rc;
+  if ((rc = upload(&c->d_beta, beta, m * nbeta)) != PW_OK) return rc;
+  if ((rc = upload(&c->d_betaw, bw.data(), m * nbeta)) != PW_OK) return rc;
+  if ((rc = upload(&c->d_beta_f, bf.data(), m * nbeta)) != PW_OK) return rc;
+  if ((rc = upload(&c->d_betaw_f, bwf.data(), m * nbeta)) != PW_OK) return rc;
+  PW_HIP(hipMalloc((void**)&c->d_W, sizeof(double) * (size_t)nbeta * c->max_bands));
+  PW_HIP(hipMalloc((void**)&c->d_W_f, sizeof(float) * (size_t)nbeta * c->max_bands));
+  return PW_OK;
+}
+
+extern "C" int pw_set_timing(pw_ctx* c, int enable) {
+  if (!c) return fail(PW_ERR_ARG, "null context");
+  if (enable && c->ev_pool.empty()) {
+    c->ev_pool.resize(5 * pw_ctx::kTimedCalls);
+    for (auto& e : c->ev_pool) PW_HIP(hipEventCreate(&e));
+  }
+  c->timing = enable != 0;
+  c->ev_calls = 0;
+  for (int i = 0; i < 4; i++) c->stage_ms[i] = 0.0;
+  return PW_OK;
+}
+
+extern "C" int pw_get_timing(pw_ctx* c, double* ms, int* ncalls) {
+  if (!c || !ms) return fail(PW_ERR_ARG, "null argument");
+  const int n = std::min(c->ev_calls, pw_ctx::kTimedCalls);
+  if (n > 0) PW_HIP(hipEventSynchronize(c->ev_pool[5 * (n - 1) + 4]));
+  for (int k = 0; k < n; k++)
+    for (int i = 0; i < 4; i++) {
+      float t = 0.f;
+      PW_HIP(hipEventElapsedTime(&t, c->ev_pool[5 * k + i], c->ev_pool[5 * k + i + 1]));
+      c->stage_ms[i] += t;
+    }
+  for (int i = 0; i < 4; i++) ms[i] = c->stage_ms[i];
+  if (ncalls) *ncalls = c->ev_total + n;
+  c->ev_total += n;
+  c->ev_calls = 0;
+  return PW_OK;
+}
+
+namespace pw {
+
+// Event k of the current timed call (k = 0..4); no-op when timing is off or
+// the pool is full (the caller then sees fewer timed calls).
+static int mark(pw_ctx* c, int k) {
+  if (!c->timing || c->ev_calls >= pw_ctx::kTimedCalls) return PW_OK;
+  PW_HIP(hipEventRecord(c->ev_pool[5 * c->ev_calls + k], c->stream));
+  if (k == 4) c->ev_calls++;
+  return PW_OK;
+}
+
+static size_t slab_lds(const pw_ctx* c, int esz, int rp) {
+  const int nk = c->sp.kxmax + 1;
+  return (size_t)esz * (c->n1 + c->n2 + 2 * (size_t)nk * c->n2 + 2 * (size_t)rp * c->n1);
+}
+
+template <typename T>
+static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
+  typedef typename cx<T>::t C;
+  const int esz = sizeof(C);
+  const T* tw = (const T*)(sizeof(T) == 8 ? (const void*)c->d_tw64 : (const void*)c->d_tw32);
+  const C* tw1 = (const C*)tw;
+  const C* tw2 = tw1 + c->n1;
+  const C* tw3 = tw2 + c->n2;
+  const T* vt = (const T*)(sizeof(T) == 8 ? (const void*)c->d_vtot : (const void*)c->d_vtot_f);
+  const T* ek = (const T*)(sizeof(T) == 8 ? (const void*)c->d_ekin : (const void*)c->d_ekin_f);
+  const int sc = c->sp.sc;
+  C* t1 = (C*)c->d_t1;
+  C* t2 = (C*)c->d_t2;
+  // x-pass row-pair chunk: as many pairs as fit next to the y buffers in 64 KiB
+  int rp = 8;
+  while (rp > 1 && slab_lds(c, esz, rp) > 64 * 1024) rp /= 2;
+  const size_t lds2 = slab_lds(c, esz, rp);
+  if (lds2 > 160 * 1024)
+    return fail(PW_ERR_UNSUPPORTED, "H*psi slab needs %zu bytes of LDS (> 160 KiB)", lds2);
+  const size_t lds1 = (size_t)esz * (c->n3 + 2 * (size_t)sc * c->n3);
+  PW_TRY(mark(c, 0));
+  hipLaunchKernelGGL(k_sticks_inverse_z<T>, dim3(c->sp.nchunk, nb), dim3(256), lds1, c->stream,
+                     (const C*)psi, c->npw, t1, c->n3, c->sp.nstick, sc, c->d_chunk_off, c->d_pt_idx,
+                     c->d_pt_sz, tw3);
+  PW_HIP(hipGetLastError());
+  PW_TRY(mark(c, 1));
+  if (lds2 > 64 * 1024) PW_HIP(hipFuncSetAttribute((const void*)k_slab_xy<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
+  hipLaunchKernelGGL(k_slab_xy<T>, dim3(c->n3, nb), dim3(256), lds2, c->stream, (const C*)t1, t2, c->n1,
+                     c->n2, c->n3, c->sp.nstick, c->sp.kxmax, c->d_pos, c->d_mpos, vt, tw1, tw2, rp);
+  PW_HIP(hipGetLastError());
+  PW_TRY(mark(c, 2));
+  hipLaunchKernelGGL(k_sticks_forward_z<T>, dim3(c->sp.nchunk, nb), dim3(256), lds1, c->stream,
+                     (const C*)t2, (const C*)psi, (C*)a3, c->npw, c->n3, c->sp.nstick, sc,
+                     c->d_gchunk_off, c->d_gpt_idx, c->d_gpt_sz, ek, tw3);
+  PW_HIP(hipGetLastError());
+  PW_TRY(mark(c, 3));
+  return PW_OK;
+}
+
+// W = Re(beta^H X) on the real (2 npw) view, then Y (+)= (beta .* w) W.
+template <typename T>
+static int nonlocal(pw_ctx* c, int nb, const void* X, void* Y, bool accumulate) {
+  const int m = 2 * c->npw;
+  const T one = 1, zero = 0, b2 = accumulate ? (T)1 : (T)0;
+  if (c->nbeta == 0) {
+    if (!accumulate) PW_HIP(hipMemsetAsync(Y, 0, sizeof(T) * (size_t)m * nb, c->stream));
+    return PW_OK;
+  }
+  if constexpr (sizeof(T) == 8) {
+    PW_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->nbeta, nb, m,
+                          &one, c->d_beta, m, (const double*)X, m, &zero, c->d_W, c->nbeta));
+    PW_BLAS(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, m, nb, c->nbeta, &one,
+                          c->d_betaw, m, c->d_W, c->nbeta, &b2, (double*)Y, m));
+  } else {
+    PW_BLAS(rocblas_sgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->nbeta, nb, m,
+                          &one, c->d_beta_f, m, (const float*)X, m, &zero, c->d_W_f, c->nbeta));
+    PW_BLAS(rocblas_sgemm(c->blas, rocblas_operation_none, rocblas_operation_none, m, nb, c->nbeta, &one,
+                          c->d_betaw_f, m, c->d_W_f, c->nbeta, &b2, (float*)Y, m));
+  }
+  return PW_OK;
+}
+
+static int check_call(pw_ctx* c, int dtype, int nb, const void* a, const void* b) {
+  if (!c) return fail(PW_ERR_ARG, "null context");
+  if (dtype != PW_F64 && dtype != PW_F32) return fail(PW_ERR_ARG, "unknown dtype %d", dtype);
+  if (nb < 0 || nb > c->max_bands) return fail(PW_ERR_ARG, "nb = %d outside [0, %d]", nb, c->max_bands);
+  if (nb > 0 && (!a || !b)) return fail(PW_ERR_ARG, "null block pointer");
+  if (!c->have_vtot) return fail(PW_ERR_STATE, "vtot has not been set");
+  return PW_OK;
+}
+
+}  // namespace pw
+
+extern "C" int pw_mult_spinor(pw_ctx* c, int dtype, int nb, const void* psi, void* a3) {
+  PW_TRY(check_call(c, dtype, nb, psi, a3));
+  if (nb == 0) return PW_OK;
+  if (psi == a3) return fail(PW_ERR_ARG, "pw_mult_spinor: psi and a3 must not alias");
+  const int rc = dtype == PW_F64 ? local_kinetic<double>(c, nb, psi, a3) : local_kinetic<float>(c, nb, psi, a3);
+  if (rc == PW_OK) PW_TRY(mark(c, 4));
+  return rc;
+}
+
+extern "C" int pw_nonlocal_multx(pw_ctx* c, int dtype, int nb, const void* X, void* V) {
+  if (!c) return fail(PW_ERR_ARG, "null context");
+  if (dtype != PW_F64 && dtype != PW_F32) return fail(PW_ERR_ARG, "unknown dtype %d", dtype);
+  if (nb < 0 || nb > c->max_bands) return fail(PW_ERR_ARG, "nb = %d outside [0, %d]", nb, c->max_bands);
+  if (nb == 0) return PW_OK;
+  if (!X || !V || X == V) return fail(PW_ERR_ARG, "pw_nonlocal_multx: bad block pointers");
+  return dtype == PW_F64 ? nonlocal<double>(c, nb, X, V, false) : nonlocal<float>(c, nb, X, V, false);
+}
+
+extern "C" int pw_apply_h(pw_ctx* c, int dtype, int nb, const void* X, void* HX) {
+  PW_TRY(check_call(c, dtype, nb, X, HX));
+  if (nb == 0) return PW_OK;
+  if (X == HX) return fail(PW_ERR_ARG, "pw_apply_h: X and HX must not alias");
+  int rc = dtype == PW_F64 ? local_kinetic<double>(c, nb, X, HX) : local_kinetic<float>(c, nb, X, HX);
+  if (rc != PW_OK) return rc;
+  rc = dtype == PW_F64 ? nonlocal<double>(c, nb, X, HX, true) : nonlocal<float>(c, nb, X, HX, true);
+  if (rc != PW_OK) return rc;
+  PW_TRY(mark(c, 4));
+  return PW_OK;
+}
+
+extern "C" int pw_teter_precond(pw_ctx* c, int dtype, int nb, const void* psi, void* a3) {
+  if (!c) return fail(PW_ERR_ARG, "null context");
+  if (nb < 0 || nb > c->max_bands) return fail(PW_ERR_ARG, "nb out of range");
+  if (nb == 0) return PW_OK;
+  const long total = (long)c->npw * nb;
+  const int blocks = (int)std::min<long>((total + 255) / 256, 4096);
+  if (dtype == PW_F64)
+    hipLaunchKernelGGL(k_teter<double>, dim3(blocks), dim3(256), 0, c->stream, (const double2*)psi,
+                       (double2*)a3, c->d_teter, c->npw, total);
+  else if (dtype == PW_F32)
+    hipLaunchKernelGGL(k_teter<float>, dim3(blocks), dim3(256), 0, c->stream, (const float2*)psi,
+                       (float2*)a3, c->d_teter_f, c->npw, total);
+  else
+    return fail(PW_ERR_ARG, "unknown dtype %d", dtype);
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}

@@ -1,0 +1,112 @@
+// Internal declarations of libpwdft_hip.so (not part of the C-ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/pwdft_hip.h"
+
+namespace pw {
+
+// ---- error plumbing: a thread-local message, integer status codes ----
+void set_error(const char* fmt, ...);
+int fail(int code, const char* fmt, ...);
+
+#define PW_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return ::pw::fail(PW_ERR_HIP, "%s failed: %s (%s:%d)", #call,              \
+                        hipGetErrorString(e_), __FILE__, __LINE__);               \
+  } while (0)
+
+#define PW_BLAS(call)                                                             \
+  do {                                                                            \
+    rocblas_status s_ = (call);                                                   \
+    if (s_ != rocblas_status_success)                                             \
+      return ::pw::fail(PW_ERR_HIP, "%s failed: rocblas status %d (%s:%d)", #call, \
+                        (int)s_, __FILE__, __LINE__);                             \
+  } while (0)
+
+#define PW_TRY(call)        \
+  do {                      \
+    int rc_ = (call);       \
+    if (rc_ != PW_OK) return rc_; \
+  } while (0)
+
+// ---- grid helpers shared by host setup and context creation ----
+inline int kgrid(int i, int n) { return i <= n / 2 ? i : i - n; }
+void recip_cell(const double* supercell, double* recip);
+double cell_volume(const double* supercell);
+bool smooth235(int n);
+// Radix plan of a 2/3/5-smooth length: factors (4, 2, 3, 5 order).
+std::vector<int> radix_plan(int n);
+
+// A device buffer owned by the context.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+// Stick decomposition of the half sphere: the distinct (kx, ky) columns of
+// the sphere; each column is transformed along z, each z slab along y and x.
+struct StickPlan {
+  int nstick = 0, kxmax = 0;
+  int sc = 16;        // sticks per chunk in the z kernels
+  int nchunk = 0;
+  std::vector<int> kx, ky;        // per stick (ky = storage index 0..n2-1)
+  // per chunk point list, (z, stick) ordered: packed = sphere_idx, and
+  // (local stick, z) in a second array; mirrors (conj entries of the (0,0)
+  // stick and nothing else) flagged by negative sphere index -(i+1).
+  std::vector<int> chunk_off;     // nchunk+1
+  std::vector<int> pt_idx;        // sphere index, or -(i+1) for conj(c_i)
+  std::vector<int> pt_sz;         // local stick | (z << 16)
+  // Gather list for the forward direction (no mirrors): same layout.
+  std::vector<int> gchunk_off;
+  std::vector<int> gpt_idx;
+  std::vector<int> gpt_sz;
+  // slab positions: kx*n2 + ky, and the mirror position for kx=0,ky>0
+  std::vector<int> pos, mpos;     // mpos = -1 when no mirror
+};
+
+}  // namespace pw
+
+struct pw_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  int n1, n2, n3, h1;
+  long N, Nh;
+  double vol;
+  double supercell[9], recip[9];
+  int npw, npad, max_bands;
+  pw::StickPlan sp;
+  std::vector<int> f1, f2, f3;  // radix plans
+  // device arrays
+  int *d_pt_idx = nullptr, *d_pt_sz = nullptr, *d_chunk_off = nullptr;
+  int *d_gpt_idx = nullptr, *d_gpt_sz = nullptr, *d_gchunk_off = nullptr;
+  int *d_pos = nullptr, *d_mpos = nullptr;
+  double *d_ekin = nullptr, *d_teter = nullptr;
+  float *d_ekin_f = nullptr, *d_teter_f = nullptr;
+  double* d_tw64 = nullptr;  // roots for n1 | n2 | n3, interleaved complex
+  float* d_tw32 = nullptr;
+  double* d_vtot = nullptr;  // V / N
+  float* d_vtot_f = nullptr;
+  bool have_vtot = false;
+  int nbeta = 0;
+  double *d_beta = nullptr, *d_betaw = nullptr, *d_W = nullptr;
+  float *d_beta_f = nullptr, *d_betaw_f = nullptr, *d_W_f = nullptr;
+  void* d_t1 = nullptr;  // stick buffers: max_bands x n3 x nstick complex double
+  void* d_t2 = nullptr;
+  size_t tbytes = 0;
+  // timing
+  static constexpr int kTimedCalls = 1024;
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;  // 5 events per timed call
+  int ev_calls = 0, ev_total = 0;
+  double stage_ms[4] = {0, 0, 0, 0};
+};

@@ -1,0 +1,99 @@
+"""Python test binding over the C-ABI: a device context holding one problem.
+
+Blocks are torch tensors on the GPU, complex128 (PW_F64) or complex64
+(PW_F32), shape (nb, npw) -- i.e. column-major npw x nb, the reference's
+NumTns<Complex> layout.  torch only provides device memory and the stream;
+every operation is a call into libpwdft_hip.so.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import PW_F32, PW_F64, check
+
+
+def _dtype_code(t):
+    import torch
+    if t.dtype == torch.complex128:
+        return PW_F64
+    if t.dtype == torch.complex64:
+        return PW_F32
+    raise TypeError(f"unsupported dtype {t.dtype}")
+
+
+class Context:
+    def __init__(self, prob, max_bands=None, stream=None, device="cuda"):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("pwdft_develop_amd.ops needs a GPU (no CPU fallback)")
+        self.torch = torch
+        self.prob = prob
+        self.device = torch.device(device)
+        torch.cuda.set_device(self.device)
+        L = _lib.lib()
+        self.max_bands = int(max_bands or prob.nb)
+        h = C.c_void_p()
+        self.stream = stream if stream is not None else torch.cuda.current_stream(self.device)
+        check(L.pw_ctx_create(C.byref(h), np.asarray(prob.grid, np.int32),
+                              np.ascontiguousarray(prob.supercell, np.float64).reshape(-1),
+                              prob.npw, prob.idx, len(prob.pad_src), np.ascontiguousarray(prob.pad_src),
+                              np.ascontiguousarray(prob.pad_dst), prob.ekin, prob.teter,
+                              self.max_bands, C.c_void_p(self.stream.cuda_stream)))
+        self.h = h
+        check(L.pw_set_vtot(h, np.ascontiguousarray(prob.vtot, np.float64).reshape(-1)))
+        if prob.nbeta:
+            check(L.pw_set_nonlocal(h, prob.nbeta, np.ascontiguousarray(prob.beta).view(np.float64).reshape(-1),
+                                    np.ascontiguousarray(prob.wgt, np.float64)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib.lib().pw_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        a, b, c, d = C.c_int(), C.c_int(), C.c_int(), C.c_long()
+        check(_lib.lib().pw_ctx_info(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        return {"npw": a.value, "nstick": b.value, "kxmax": c.value, "ngrid": d.value}
+
+    def to_device(self, psi, dtype=None):
+        t = self.torch
+        dt = dtype or t.complex128
+        return t.from_numpy(np.ascontiguousarray(psi)).to(device=self.device, dtype=dt).contiguous()
+
+    def _binop(self, name, x, out=None, zero_out=False):
+        t = self.torch
+        assert x.is_contiguous() and x.dim() == 2 and x.shape[1] == self.prob.npw
+        if out is None:
+            out = t.zeros_like(x) if zero_out else t.empty_like(x)
+        check(getattr(_lib.lib(), name)(self.h, _dtype_code(x), x.shape[0], C.c_void_p(x.data_ptr()),
+                                        C.c_void_p(out.data_ptr())))
+        return out
+
+    def mult_spinor(self, x, out=None):
+        return self._binop("pw_mult_spinor", x, out)
+
+    def nonlocal_multx(self, x, out=None):
+        return self._binop("pw_nonlocal_multx", x, out)
+
+    def apply_h(self, x, out=None):
+        return self._binop("pw_apply_h", x, out)
+
+    def teter_precond(self, x, out=None):
+        return self._binop("pw_teter_precond", x, out, zero_out=True)
+
+    def set_timing(self, on=True):
+        check(_lib.lib().pw_set_timing(self.h, 1 if on else 0))
+
+    def timing(self):
+        """(summed ms per stage [4], number of timed calls) since set_timing."""
+        ms = np.zeros(4)
+        n = C.c_int(0)
+        check(_lib.lib().pw_get_timing(self.h, ms, C.byref(n)))
+        return ms, n.value

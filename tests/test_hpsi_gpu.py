@@ -1,0 +1,85 @@
+"""H*psi on the GPU through the C-ABI against the FP64 oracle and the dense-H
+golden vectors.  Tolerances: FP64 1e-11 relative (max-norm), FP32 2e-5."""
+import numpy as np
+import pytest
+
+import oracle as O
+from pwdft_develop_amd import workload as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+@pytest.fixture(scope="module")
+def ctx8(si8):
+    from pwdft_develop_amd.ops import Context
+    return Context(si8)
+
+
+def test_apply_h_fp64_matches_oracle(ctx8, si8):
+    import torch
+    print(f"case: si8_32 apply_h fp64 nb={si8.nb}", flush=True)
+    X = ctx8.to_device(si8.psi)
+    HX = ctx8.apply_h(X)
+    torch.cuda.synchronize()
+    ref = O.Hamiltonian(si8).apply_h(si8.psi)
+    assert _rel(HX.cpu().numpy(), ref) < 1e-11
+
+
+def test_apply_h_fp64_matches_dense_golden(ctx8, golden):
+    X = ctx8.to_device(golden["psi4"])
+    HX = ctx8.apply_h(X).cpu().numpy()
+    assert _rel(HX, golden["hpsi4"]) < 1e-11
+
+
+def test_mult_spinor_and_nonlocal_split(ctx8, si8):
+    H = O.Hamiltonian(si8)
+    X = ctx8.to_device(si8.psi[:7])
+    loc = ctx8.mult_spinor(X).cpu().numpy()
+    nl = ctx8.nonlocal_multx(X).cpu().numpy()
+    assert _rel(loc, H.mult_spinor(si8.psi[:7])) < 1e-11
+    assert _rel(nl, H.nonlocal_multx(si8.psi[:7])) < 1e-11
+
+
+def test_apply_h_fp32(ctx8, si8):
+    import torch
+    X = ctx8.to_device(si8.psi, torch.complex64)
+    HX = ctx8.apply_h(X).cpu().numpy()
+    ref = O.Hamiltonian(si8).apply_h(si8.psi)
+    assert _rel(HX, ref) < 2e-5
+
+
+def test_teter(ctx8, si8):
+    X = ctx8.to_device(si8.psi[:3])
+    T = ctx8.teter_precond(X).cpu().numpy()
+    ref = O.Hamiltonian(si8).teter(si8.psi[:3])
+    assert _rel(T, ref) < 1e-14
+
+
+@pytest.mark.parametrize("name", ["rand16_48", "si64_60"])
+def test_apply_h_other_grids(name):
+    import torch
+    from pwdft_develop_amd.ops import Context
+    prob = W.WORKLOADS[name](nb=8).build()
+    print(f"case: {name} apply_h fp64 nb=8", flush=True)
+    ctx = Context(prob)
+    HX = ctx.apply_h(ctx.to_device(prob.psi)).cpu().numpy()
+    ref = O.Hamiltonian(prob).apply_h(prob.psi)
+    assert _rel(HX, ref) < 1e-11
+    HX32 = ctx.apply_h(ctx.to_device(prob.psi, torch.complex64)).cpu().numpy()
+    assert _rel(HX32, ref) < 2e-5
+    ctx.close()
+
+
+def test_errors_surface_as_status(ctx8, si8):
+    from pwdft_develop_amd._lib import PwError
+    import torch
+    X = ctx8.to_device(si8.psi)
+    with pytest.raises(PwError):
+        ctx8.apply_h(X, out=X)            # aliasing is rejected
+    big = torch.zeros((si8.nb + 1, si8.npw), dtype=torch.complex128, device="cuda")
+    with pytest.raises(PwError):
+        ctx8.apply_h(big)                 # more bands than the context holds
