@@ -1,15 +1,25 @@
-"""Benchmark of the H*psi hot path on one MI355X.
+"""Benchmark of the H*psi hot path and the ChebFSI + Rayleigh-Ritz sweep on one
+MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload si64_60]
-                    [--dtype f64|f32] [--no-cpu]
+                    [--no-cpu]
 
-A step is one application of H (local + kinetic + nonlocal) to the whole band
-block of the workload (si64_60: 64-atom diamond supercell, 60^3 grid,
-Ecut 10 Ha, npw 6567, 256 bands, 256 projectors), inputs resident in HBM.
-value = band-applications per second.  The line also carries the roofline
-of the dominant kernel (the xy slab kernel, timed with HIP events on the
-launch stream inside the timed region) and the FP64 CPU oracle timed on this
-host's cores over a bounded sample of bands.
+Workload si64_60: 64-atom diamond supercell, 60^3 grid, Ecut 10 Ha,
+npw 6567, 256 bands, 256 projectors; degree-10 Chebyshev filter with bounds
+from 8 Lanczos steps, then Cholesky QR + Rayleigh-Ritz.  Inputs are resident
+in HBM.  A step is one full sweep (2824 band-applications of H: 8 Lanczos +
+10 x 256 filter + 256 Rayleigh-Ritz).
+
+W untimed FP64 sweeps warm the block (as in an SCF, where each sweep starts
+from the previous one), then K mixed sweeps (FP32 filter, FP64 Cholesky QR and
+Rayleigh-Ritz) and K FP64 sweeps are timed from that same block.
+
+value = band-applications/s of the mixed sweep.  The line also carries the
+FP64 sweep time, the mixed/FP64 speedup, the Ritz-value error of the mixed
+sweep against the FP64 sweep, plain H*psi throughput (FP64 and FP32), the
+roofline of the dominant kernel (the xy slab FFT kernel, timed with HIP
+events on its launch stream inside the timed region) and the FP64 CPU oracle
+timed on this host's cores.
 """
 import argparse
 import json
@@ -21,56 +31,42 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+DEGREE, LANCZOS = 10, 8
 
 
-def slab_bytes(prob, info, nb, p):
+def slab_bytes(prob, nstick, nb, p):
     """Algorithmic HBM bytes of one xy-slab launch over nb bands: read the
     z-transformed sticks (nb*n3*nstick complex), write them back, read V once."""
-    n3 = prob.grid[2]
-    return nb * n3 * info["nstick"] * 2 * p * 2 + prob.ngrid * p
+    return nb * prob.grid[2] * nstick * 2 * p * 2 + prob.ngrid * p
 
 
 def staged_bytes(prob, p):
-    """Staged per-band byte model p*(9 npw + 6 Nh + 5 N) of the dense-FFT
-    reference pipeline (gather, C2R, V, R2C, scatter)."""
+    """Per-band bytes of the dense-FFT staged model p*(9 npw + 6 Nh + 5 N)."""
     n1, n2, n3 = prob.grid
     Nh = (n1 // 2 + 1) * n2 * n3
     return p * (9 * prob.npw + 6 * Nh + 5 * prob.ngrid)
 
 
-def cpu_baseline(prob, nsample):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-    cores = os.cpu_count() or 1
+def host_cores():
     try:
-        cores = len(os.sched_getaffinity(0))
+        n = len(os.sched_getaffinity(0))
     except Exception:
-        pass
-    cores = min(cores, 16)
-    H = O.Hamiltonian(prob, threads=cores)
-    psi = prob.psi[:nsample]
-    H.apply_h(psi[: min(cores, nsample)])  # warm
-    t0 = time.perf_counter()
-    H.apply_h(psi)
-    dt = time.perf_counter() - t0
-    return {"value": nsample / dt, "unit": "band-applications/s", "cores": cores, "kind": "port",
-            "sample": f"FP64 oracle apply_h on {nsample} of {prob.nb} bands of {prob.name}, "
-                      f"OpenMP over bands ({dt:.2f} s)"}
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="si64_60")
-    ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
-    ap.add_argument("--cpu-sample", type=int, default=32)
+    ap.add_argument("--hpsi-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(multi-GPU sharding is described in DESIGN.md, not run)"}))
+                          "(how the path shards over GPUs is described in DESIGN.md)"}))
         sys.exit(2)
 
     import numpy as np
@@ -79,55 +75,106 @@ def main():
     from pwdft_develop_amd.ops import Context
 
     prob = W.WORKLOADS[args.workload]().build()
+    nb = prob.nb
     ctx = Context(prob)
     info = ctx.info()
-    dt = torch.complex128 if args.dtype == "f64" else torch.complex64
-    p = 8 if args.dtype == "f64" else 4
-    X = ctx.to_device(prob.psi, dt)
-    HX = torch.empty_like(X)
+    out = {}
+
+    # ---- plain H*psi throughput, FP64 and FP32 ----
+    hpsi = {}
+    for dt, p, tag in ((torch.complex128, 8, "f64"), (torch.complex64, 4, "f32")):
+        X = ctx.to_device(prob.psi, dt)
+        HX = torch.empty_like(X)
+        for _ in range(3):
+            ctx.apply_h(X, HX)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.hpsi_steps):
+            ctx.apply_h(X, HX)
+        torch.cuda.synchronize()
+        hpsi[tag] = nb * args.hpsi_steps / (time.perf_counter() - t0)
+        del X, HX
+
+    # ---- ChebFSI + RR sweeps ----
+    v0 = ctx.to_device(prob.lanczos_start())
+    X = ctx.to_device(prob.psi)
+    prev = None
     for _ in range(args.warmup):
-        ctx.apply_h(X, HX)
+        _, prev, _, _ = ctx.chebfsi_sweep(X, DEGREE, LANCZOS, v0, prev)
     torch.cuda.synchronize()
-    ctx.set_timing(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ctx.apply_h(X, HX)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    stage_ms, ncalls = ctx.timing()
-    ctx.set_timing(False)
-    ms_step = (t1 - t0) * 1e3 / args.steps
-    nb = prob.nb
-    value = nb * args.steps / (t1 - t0)
+    Xw = X.clone()
+    prev_w = prev.copy() if prev is not None else None
+    napp = LANCZOS + DEGREE * nb + nb
+
+    def timed(mixed, timing=False):
+        Xs = [Xw.clone() for _ in range(args.steps)]
+        torch.cuda.synchronize()
+        if timing:
+            ctx.set_timing(True)
+        t0 = time.perf_counter()
+        res = None
+        for i in range(args.steps):
+            res = ctx.chebfsi_sweep(Xs[i], DEGREE, LANCZOS, v0, prev_w, mixed=mixed)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        st = ctx.timing() if timing else None
+        if timing:
+            ctx.set_timing(False)
+        return (t1 - t0) / args.steps, res, st, Xs[-1]
+
+    t_mixed, (_, e_mixed, r_mixed, bounds), (stage_ms, ncalls), _ = timed(True, timing=True)
+    t_f64, (_, e_f64, r_f64, _), _, X64 = timed(False)
+    nwant = 3 * nb // 4
+    err = np.abs(e_mixed - e_f64)
+
     slab_ms = stage_ms[1] / max(ncalls, 1)
-    traffic_alg = slab_bytes(prob, info, nb, p)
-    achieved = traffic_alg / (slab_ms * 1e-3) / 1e9
-    out = {
-        "metric": "Hpsi band-applications/s (ChebFSI-RR sweep: see DESIGN.md)",
-        "value": round(value, 1),
+    achieved = slab_bytes(prob, info["nstick"], nb, 4) / (slab_ms * 1e-3) / 1e9
+    out.update({
+        "metric": "ChebFSI-RR sweep band-applications/s (mixed precision), Ritz err vs FP64",
+        "value": round(napp / t_mixed, 1),
         "unit": "band-applications/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": round(t_mixed * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": args.dtype,
+        "dtype": "f32 filter / f64 Cholesky-QR + Rayleigh-Ritz",
         "data": "synthetic (seeded Philox inputs, random-init wavefunctions)",
         "config": {"workload": prob.name, "grid": list(prob.grid), "npw": prob.npw, "nbands": nb,
-                   "nbeta": prob.nbeta, "nstick": info["nstick"], "ecut_ha": prob.ecut},
-        "stages_ms_per_step": [round(x / max(ncalls, 1), 4) for x in stage_ms],
-        "roofline": {"bound": "hbm", "kernel": "k_slab_xy", "achieved": round(achieved, 1),
+                   "nbeta": prob.nbeta, "nstick": info["nstick"], "degree": DEGREE, "lanczos": LANCZOS,
+                   "band_applications_per_sweep": napp},
+        "sweep_s": {"mixed": round(t_mixed, 6), "fp64": round(t_f64, 6)},
+        "speedup_mixed_vs_fp64": round(t_f64 / t_mixed, 3),
+        "ritz_err_ha": {"wanted_max": float(err[:nwant].max()), "all_max": float(err.max()),
+                        "nwanted": nwant},
+        "hpsi_band_applications_per_s": {k: round(v, 1) for k, v in hpsi.items()},
+        "stages_ms_per_apply_mixed": [round(x / max(ncalls, 1), 4) for x in stage_ms],
+        "roofline": {"bound": "hbm", "kernel": "k_slab_xy<float>", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None},
-        "staged_model_equiv_GBs": round(staged_bytes(prob, p) * value / 1e9, 1),
-    }
-    del X, HX
-    ctx.close()
+        "staged_model_equiv_GBs_fp64_hpsi": round(staged_bytes(prob, 8) * hpsi["f64"] / 1e9, 1),
+    })
+
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(prob, args.cpu_sample)
+        # FP64 oracle sweep on the same warm block, OpenMP over bands
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        cores = host_cores()
+        H = O.Hamiltonian(prob, threads=cores)
+        Xh = Xw.cpu().numpy()
+        v0h = prob.lanczos_start()
+        t0 = time.perf_counter()
+        _, e_o, _, _ = H.chebfsi_sweep(Xh, DEGREE, LANCZOS, v0h, prev_w)
+        t_cpu = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": round(napp / t_cpu, 1), "unit": "band-applications/s", "cores": cores, "kind": "port",
+            "sample": f"one full FP64 ChebFSI+RR sweep of {prob.name} ({nb} bands, {napp} band-applications) "
+                      f"by the C oracle, OpenMP over bands: {t_cpu:.2f} s"}
+        out["ritz_err_ha"]["fp64_gpu_vs_oracle_max"] = float(np.abs(e_f64 - e_o).max())
+        out["ritz_err_ha"]["mixed_vs_oracle_wanted_max"] = float(np.abs(e_mixed - e_o)[:nwant].max())
+    ctx.close()
     print(json.dumps(out))
 
 

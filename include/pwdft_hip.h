@@ -116,6 +116,47 @@ int pw_apply_h(pw_ctx* ctx, int dtype, int nb, const void* X, void* HX);
 /* Spinor::AddTeterPrecond (src/SCF/spinor.cpp:348-374): a3 += teter .* psi */
 int pw_teter_precond(pw_ctx* ctx, int dtype, int nb, const void* psi, void* a3);
 
+/* ------------------------------------------------------------------ */
+/* Eigensolver layer.  X blocks are device FP64 (PW_F64 layout) unless a
+ * dtype argument says otherwise.  Scratch (6 blocks of max_bands bands plus
+ * small matrices) is allocated on first use and kept until
+ * pw_free_workspace or pw_ctx_destroy. */
+
+/* EigenSolver::Orthogonalize, complex X / real Gram, non-ScaLAPACK branch
+ * (src/EigenSolver/eigensolver.cpp:386-469): S = Re(X^H X), Potrf('U'),
+ * X <- X U^{-1}.  PW_ERR_LINALG when S is not positive definite. */
+int pw_orthogonalize(pw_ctx* ctx, int nb, void* X);
+
+/* k-step Lanczos on the device vector v0 (one band, FP64): Ritz values of
+ * T_k ascending (host theta[k]) and the upper bound theta_k + |beta_k|.
+ * Spectral-bound estimator of the ChebFSI steps the reference only names
+ * (src/SCF/scf.cpp:1306-1316, src/esdf.cpp:2714-2719). */
+int pw_lanczos(pw_ctx* ctx, int k, const void* v0, double* theta, double* b_up);
+
+/* Scaled Chebyshev filter of degree m (Zhou, Saad, Tiago, Chelikowsky 2006)
+ * applied in place to X (dtype blocks): damps [a, b], scaled at aL. */
+int pw_cheb_filter(pw_ctx* ctx, int dtype, int nb, int m, double a, double b,
+                   double aL, void* X);
+
+/* Cholesky QR + Rayleigh-Ritz on the FP64 block X: X <- Ritz vectors,
+ * eig = Ritz values ascending, res = ||H x_j - theta_j x_j|| (host arrays).
+ * hdtype: precision of the H X product (PW_F64 or PW_F32). */
+int pw_rayleigh_ritz(pw_ctx* ctx, int hdtype, int nb, void* X, double* eig,
+                     double* res);
+
+/* One ChebFSI sweep -- the body of the reference's GeneralChebyStep(numEig,
+ * filterOrder) call site (src/SCF/scf.cpp:1321-1332): Lanczos(k) bounds from
+ * v0, degree-m filter in mode precision (PW_F64, or PW_F32 = mixed: FP32
+ * filter, FP64 Cholesky QR and Rayleigh-Ritz), then RR.  With prev_eig the
+ * cut is prev_eig[nb-1] and the scale point prev_eig[0]; without, the first
+ * step uses aL = theta_1, a = theta_1 + 0.1 (b_up - theta_1).  bounds (may
+ * be NULL) receives {a, b_up, aL}. */
+int pw_chebfsi_sweep(pw_ctx* ctx, int mode, int nb, int m, int k,
+                     const void* v0, const double* prev_eig, void* X,
+                     double* eig, double* res, double* bounds);
+
+int pw_free_workspace(pw_ctx* ctx);
+
 /* Stage timing of pw_mult_spinor / pw_apply_h calls, measured with HIP
  * events recorded on the context's stream around each kernel (no host
  * synchronisation).  pw_set_timing(1) clears the totals; pw_get_timing

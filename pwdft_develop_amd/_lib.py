@@ -61,6 +61,12 @@ def lib():
         "pw_nonlocal_multx": [_vp, C.c_int, C.c_int, _vp, _vp],
         "pw_apply_h": [_vp, C.c_int, C.c_int, _vp, _vp],
         "pw_teter_precond": [_vp, C.c_int, C.c_int, _vp, _vp],
+        "pw_orthogonalize": [_vp, C.c_int, _vp],
+        "pw_lanczos": [_vp, C.c_int, _vp, _dp, C.POINTER(C.c_double)],
+        "pw_cheb_filter": [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _vp],
+        "pw_rayleigh_ritz": [_vp, C.c_int, C.c_int, _vp, _dp, _dp],
+        "pw_chebfsi_sweep": [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _dp, _dp, _dp],
+        "pw_free_workspace": [_vp],
         "pw_set_timing": [_vp, C.c_int],
         "pw_get_timing": [_vp, _dp, C.POINTER(C.c_int)],
     }

@@ -1,0 +1,224 @@
+// Tall-skinny GEMMs of the hot path on MFMA (gfx950), FP64 and FP32.
+//
+//   TN:  C(n1 x n2) = A^T B, A: K x n1, B: K x n2, K = 2 npw long
+//        (Gram blocks Re(X^H Y), projections Re(beta^H X); blas.cpp:997-1029,
+//        eigensolver.cpp:471-515, lobpcg.cpp:227-419).  Split-K: each
+//        workgroup reduces one K chunk of one 64x64 tile into a partial
+//        buffer; a second kernel sums the partials in a fixed order, so the
+//        result is deterministic.
+//   NN:  Y(m x n) = alpha A(m x k) B(k x n) + beta Y  (subspace rotations,
+//        beta * W; blas.cpp:1031-1067, lobpcg.cpp:445-526).
+//
+// Both use 64x64 workgroup tiles, 4 waves of 32x32, and the 16x16x4 MFMA
+// (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).  Lane layout of the
+// 16x16x4 MFMA: A operand lane l = (row l%16, k l/16), B operand lane l =
+// (k l/16, col l%16).  Accumulator lane l register i holds column l%16 and
+// row 4(l/16)+i for the FP32 MFMA but row 4i + l/16 for the FP64 one
+// (checked on gfx950 with a known 16x16x4 product).
+#include <algorithm>
+
+#include "pw_internal.h"
+
+namespace pw {
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+template <typename T> struct acc4;
+template <> struct acc4<double> { typedef d4v t; };
+template <> struct acc4<float> { typedef f4v t; };
+
+__device__ __forceinline__ d4v mfma16(double a, double b, d4v c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4v mfma16(float a, float b, f4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+constexpr int BM = 64, BN = 64, BK = 16;
+
+template <typename T>
+__device__ __forceinline__ int acc_row(int lane, int r) {
+  return sizeof(T) == 8 ? 4 * r + (lane >> 4) : 4 * (lane >> 4) + r;
+}
+
+// Partial C tile of A^T B over K rows [kb, ke): P[split](n1 x n2, ld n1).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_gemm_tn_partial(const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb, int K, int n1, int n2,
+                  int kc, T* __restrict__ P) {
+  typedef typename acc4<T>::t V;
+  __shared__ T As[BM][BK + 1];
+  __shared__ T Bs[BN][BK + 1];
+  const int i0 = blockIdx.x * BM, j0 = blockIdx.y * BN, split = blockIdx.z;
+  const int kb = split * kc, ke = min(K, kb + kc);
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wi = (w & 1) * 32, wj = (w >> 1) * 32;
+  V acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) acc[a][b] = V{0, 0, 0, 0};
+  const int lc = t >> 2, lr = (t & 3) * 4;  // load: column lc, rows lr..lr+3
+  for (int k0 = kb; k0 < ke; k0 += BK) {
+    {
+      const int ci = i0 + lc, cj = j0 + lc;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int r = k0 + lr + q;
+        As[lc][lr + q] = (ci < n1 && r < ke) ? A[(size_t)ci * lda + r] : (T)0;
+        Bs[lc][lr + q] = (cj < n2 && r < ke) ? B[(size_t)cj * ldb + r] : (T)0;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T av[2], bv[2];
+#pragma unroll
+      for (int a = 0; a < 2; a++) av[a] = As[wi + a * 16 + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+      for (int b = 0; b < 2; b++) bv[b] = Bs[wj + b * 16 + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = mfma16(av[a], bv[b], acc[a][b]);
+    }
+    __syncthreads();
+  }
+  T* Ps = P + (size_t)split * n1 * n2;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const int col = j0 + wj + b * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = i0 + wi + a * 16 + acc_row<T>(lane, r);
+        if (row < n1 && col < n2) Ps[row + (size_t)n1 * col] = acc[a][b][r];
+      }
+    }
+}
+
+// C[i + ldc j] = rowscale[i] * sum_s P[s][i + n1 j] (+ beta C), rowscale may be null
+template <typename T>
+__global__ void k_gemm_tn_reduce(const T* __restrict__ P, int nsplit, int n1, int n2, T* __restrict__ C, int ldc,
+                                 const T* __restrict__ rowscale, T beta) {
+  const long total = (long)n1 * n2;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(e % n1), j = (int)(e / n1);
+    T s = 0;
+    for (int q = 0; q < nsplit; q++) s += P[(size_t)q * total + e];
+    if (rowscale) s *= rowscale[i];
+    T* c = C + i + (size_t)ldc * j;
+    *c = (beta == (T)0) ? s : s + beta * *c;
+  }
+}
+
+// Y = alpha A B + beta Y; A m x k (lda), B k x n (ldb), Y m x n (ldy).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_gemm_nn(const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb, T* __restrict__ Y, int ldy, int m,
+          int n, int k, T alpha, T beta) {
+  typedef typename acc4<T>::t V;
+  __shared__ T As[BK][BM + 1];
+  __shared__ T Bs[BN][BK + 1];
+  const int i0 = blockIdx.x * BM, j0 = blockIdx.y * BN;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wi = (w & 1) * 32, wj = (w >> 1) * 32;
+  V acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) acc[a][b] = V{0, 0, 0, 0};
+  const int ak = t >> 4, ai = (t & 15) * 4;   // A load: k row ak, rows ai..ai+3
+  const int bj = t >> 2, bk = (t & 3) * 4;    // B load: column bj, k bk..bk+3
+  for (int k0 = 0; k0 < k; k0 += BK) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int r = i0 + ai + q, kk = k0 + ak;
+      As[ak][ai + q] = (r < m && kk < k) ? A[(size_t)kk * lda + r] : (T)0;
+      const int cj = j0 + bj, kq = k0 + bk + q;
+      Bs[bj][bk + q] = (cj < n && kq < k) ? B[(size_t)cj * ldb + kq] : (T)0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T av[2], bv[2];
+#pragma unroll
+      for (int a = 0; a < 2; a++) av[a] = As[kk + (lane >> 4)][wi + a * 16 + (lane & 15)];
+#pragma unroll
+      for (int b = 0; b < 2; b++) bv[b] = Bs[wj + b * 16 + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = mfma16(av[a], bv[b], acc[a][b]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const int col = j0 + wj + b * 16 + (lane & 15);
+      if (col >= n) continue;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = i0 + wi + a * 16 + acc_row<T>(lane, r);
+        if (row < m) {
+          T* y = Y + row + (size_t)ldy * col;
+          const T v = alpha * acc[a][b][r];
+          *y = (beta == (T)0) ? v : v + beta * *y;
+        }
+      }
+    }
+}
+
+template <typename T>
+int gemm_tn(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, int ldb, T* C, int ldc,
+            const T* rowscale, T beta) {
+  if (n1 <= 0 || n2 <= 0) return PW_OK;
+  const int ti = (n1 + BM - 1) / BM, tj = (n2 + BN - 1) / BN;
+  const int tiles = ti * tj;
+  int nsplit = std::max(1, std::min((1024 + tiles - 1) / tiles, (K + 255) / 256));
+  int kc = (K + nsplit - 1) / nsplit;
+  kc = (kc + BK - 1) / BK * BK;
+  nsplit = (K + kc - 1) / kc;
+  const size_t need = sizeof(T) * (size_t)nsplit * n1 * n2;
+  if (need > c->gemm_ws_bytes) {
+    if (c->d_gemm_ws) (void)hipFree(c->d_gemm_ws);
+    c->d_gemm_ws = nullptr;
+    c->gemm_ws_bytes = 0;
+    PW_HIP(hipMalloc(&c->d_gemm_ws, need));
+    c->gemm_ws_bytes = need;
+  }
+  T* P = (T*)c->d_gemm_ws;
+  hipLaunchKernelGGL(k_gemm_tn_partial<T>, dim3(ti, tj, nsplit), dim3(256), 0, c->stream, A, lda, B, ldb, K, n1, n2,
+                     kc, P);
+  const long total = (long)n1 * n2;
+  const int g = (int)std::min<long>((total + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_gemm_tn_reduce<T>, dim3(g), dim3(256), 0, c->stream, (const T*)P, nsplit, n1, n2, C, ldc,
+                     rowscale, beta);
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+template <typename T>
+int gemm_nn(pw_ctx* c, int m, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T beta, T* Y,
+            int ldy) {
+  if (m <= 0 || n <= 0) return PW_OK;
+  hipLaunchKernelGGL(k_gemm_nn<T>, dim3((m + BM - 1) / BM, (n + BN - 1) / BN), dim3(256), 0, c->stream, A, lda, B,
+                     ldb, Y, ldy, m, n, k, alpha, beta);
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+template int gemm_tn<double>(pw_ctx*, int, int, int, const double*, int, const double*, int, double*, int,
+                             const double*, double);
+template int gemm_tn<float>(pw_ctx*, int, int, int, const float*, int, const float*, int, float*, int,
+                            const float*, float);
+template int gemm_nn<double>(pw_ctx*, int, int, int, double, const double*, int, const double*, int, double,
+                             double*, int);
+template int gemm_nn<float>(pw_ctx*, int, int, int, float, const float*, int, const float*, int, float, float*,
+                            int);
+
+}  // namespace pw

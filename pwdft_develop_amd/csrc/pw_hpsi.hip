@@ -511,6 +511,8 @@ extern "C" int pw_ctx_create(pw_ctx** out, const int grid[3], const double super
 
 extern "C" int pw_ctx_destroy(pw_ctx* c) {
   if (!c) return PW_OK;
+  pw_free_workspace(c);
+  if (c->d_gemm_ws) (void)hipFree(c->d_gemm_ws);
   void* ptrs[] = {c->d_pt_idx, c->d_pt_sz, c->d_chunk_off, c->d_gpt_idx, c->d_gpt_sz, c->d_gchunk_off,
                   c->d_pos, c->d_mpos, c->d_ekin, c->d_teter, c->d_ekin_f, c->d_teter_f, c->d_tw64,
                   c->d_tw32, c->d_vtot, c->d_vtot_f, c->d_beta, c->d_betaw, c->d_W, c->d_beta_f,
@@ -672,17 +674,11 @@ static int nonlocal(pw_ctx* c, int nb, const void* X, void* Y, bool accumulate) 
     if (!accumulate) PW_HIP(hipMemsetAsync(Y, 0, sizeof(T) * (size_t)m * nb, c->stream));
     return PW_OK;
   }
-  if constexpr (sizeof(T) == 8) {
-    PW_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->nbeta, nb, m,
-                          &one, c->d_beta, m, (const double*)X, m, &zero, c->d_W, c->nbeta));
-    PW_BLAS(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, m, nb, c->nbeta, &one,
-                          c->d_betaw, m, c->d_W, c->nbeta, &b2, (double*)Y, m));
-  } else {
-    PW_BLAS(rocblas_sgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->nbeta, nb, m,
-                          &one, c->d_beta_f, m, (const float*)X, m, &zero, c->d_W_f, c->nbeta));
-    PW_BLAS(rocblas_sgemm(c->blas, rocblas_operation_none, rocblas_operation_none, m, nb, c->nbeta, &one,
-                          c->d_betaw_f, m, c->d_W_f, c->nbeta, &b2, (float*)Y, m));
-  }
+  const T* beta = (const T*)(sizeof(T) == 8 ? (const void*)c->d_beta : (const void*)c->d_beta_f);
+  const T* betaw = (const T*)(sizeof(T) == 8 ? (const void*)c->d_betaw : (const void*)c->d_betaw_f);
+  T* W = (T*)(sizeof(T) == 8 ? (void*)c->d_W : (void*)c->d_W_f);
+  PW_TRY(gemm_tn<T>(c, m, c->nbeta, nb, beta, m, (const T*)X, m, W, c->nbeta, nullptr, zero));
+  PW_TRY(gemm_nn<T>(c, m, nb, c->nbeta, one, betaw, m, W, c->nbeta, b2, (T*)Y, m));
   return PW_OK;
 }
 

@@ -75,7 +75,22 @@ struct StickPlan {
 
 }  // namespace pw
 
+struct pw_ctx;
+
+namespace pw {
+// MFMA GEMMs (pw_gemm.hip).  C = A^T B (+ beta C), rows scaled by rowscale
+// when non-null; Y = alpha A B + beta Y.  Column-major, leading dims given.
+template <typename T>
+int gemm_tn(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, int ldb, T* C, int ldc,
+            const T* rowscale, T beta);
+template <typename T>
+int gemm_nn(pw_ctx* c, int m, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T beta, T* Y,
+            int ldy);
+}  // namespace pw
+
 struct pw_ctx {
+  void* d_gemm_ws = nullptr;  // split-K partial tiles
+  size_t gemm_ws_bytes = 0;
   int device = 0;
   hipStream_t stream = nullptr;
   rocblas_handle blas = nullptr;
@@ -103,6 +118,14 @@ struct pw_ctx {
   void* d_t1 = nullptr;  // stick buffers: max_bands x n3 x nstick complex double
   void* d_t2 = nullptr;
   size_t tbytes = 0;
+  // eigensolver workspace (allocated on first use): block buffers of
+  // max_bands x npw complex double, small nb x nb FP64 matrices
+  rocblas_handle solver = nullptr;  // rocSOLVER shares the rocBLAS handle type
+  void* d_ws[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  double *d_S = nullptr, *d_S2 = nullptr, *d_D = nullptr, *d_E = nullptr, *d_red = nullptr;
+  double* d_lz = nullptr;  // Lanczos vectors v, v_prev, f (3 x 2 npw)
+  int* d_info = nullptr;
+  int ws_nb = 0;
   // timing
   static constexpr int kTimedCalls = 1024;
   bool timing = false;

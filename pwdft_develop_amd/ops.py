@@ -23,7 +23,7 @@ def _dtype_code(t):
 
 
 class Context:
-    def __init__(self, prob, max_bands=None, stream=None, device="cuda"):
+    def __init__(self, prob, max_bands=None, stream=None, device="cuda:0"):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("pwdft_develop_amd.ops needs a GPU (no CPU fallback)")
@@ -87,6 +87,40 @@ class Context:
 
     def teter_precond(self, x, out=None):
         return self._binop("pw_teter_precond", x, out, zero_out=True)
+
+    # ---- eigensolver layer (X blocks FP64 on the device, in place) ----
+    def orthogonalize(self, X):
+        check(_lib.lib().pw_orthogonalize(self.h, X.shape[0], C.c_void_p(X.data_ptr())))
+        return X
+
+    def lanczos(self, k, v0):
+        th = np.zeros(k)
+        b = C.c_double(0.0)
+        check(_lib.lib().pw_lanczos(self.h, k, C.c_void_p(v0.data_ptr()), th, C.byref(b)))
+        return th, b.value
+
+    def cheb_filter(self, X, m, a, b, aL):
+        check(_lib.lib().pw_cheb_filter(self.h, _dtype_code(X), X.shape[0], m, a, b, aL,
+                                        C.c_void_p(X.data_ptr())))
+        return X
+
+    def rayleigh_ritz(self, X, hdtype=PW_F64):
+        nb = X.shape[0]
+        e, r = np.zeros(nb), np.zeros(nb)
+        check(_lib.lib().pw_rayleigh_ritz(self.h, hdtype, nb, C.c_void_p(X.data_ptr()), e, r))
+        return X, e, r
+
+    def chebfsi_sweep(self, X, degree, k, v0, prev_eig=None, mixed=False):
+        nb = X.shape[0]
+        e, r, bounds = np.zeros(nb), np.zeros(nb), np.zeros(3)
+        pe = None
+        if prev_eig is not None:
+            pe_arr = np.ascontiguousarray(prev_eig, dtype=np.float64)
+            pe = C.c_void_p(pe_arr.ctypes.data)
+        check(_lib.lib().pw_chebfsi_sweep(self.h, PW_F32 if mixed else PW_F64, nb, degree, k,
+                                          C.c_void_p(v0.data_ptr()), pe, C.c_void_p(X.data_ptr()),
+                                          e, r, bounds))
+        return X, e, r, bounds
 
     def set_timing(self, on=True):
         check(_lib.lib().pw_set_timing(self.h, 1 if on else 0))
