@@ -1,0 +1,322 @@
+"""Pins the FP64 CPU oracle (CPU only).
+
+Every oracle function is checked against something it was not written from:
+numpy/scipy FFTs and LAPACK, the dense-H golden vectors, closed forms,
+invariants and finite differences."""
+import numpy as np
+import pytest
+import scipy.linalg
+import scipy.special
+from scipy import integrate
+
+import oracle as O
+from pwdft_develop_amd import workload as W
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 6, 7, 8, 11, 15, 24, 30, 36, 40, 48, 60, 72, 90, 96, 180])
+def test_fft1d_vs_numpy(n):
+    rng = np.random.default_rng(n)
+    v = rng.normal(size=n) + 1j * rng.normal(size=n)
+    assert np.abs(O.fft1d(v, -1) - np.fft.fft(v)).max() < 1e-13 * max(1, n)
+    assert np.abs(O.fft1d(v, +1) - np.fft.ifft(v) * n).max() < 1e-13 * max(1, n)
+
+
+@pytest.mark.parametrize("grid", [(6, 10, 12), (5, 9, 4), (32, 30, 36), (16, 16, 16)])
+def test_fft3d_real_vs_numpy(grid):
+    n1, n2, n3 = grid
+    x = np.random.default_rng(1).normal(size=(n3, n2, n1))
+    r = O.fft_r2c(grid, x)
+    ref = np.fft.rfftn(x)
+    assert np.abs(r - ref).max() < 1e-12 * x.size
+    back = O.fft_c2r(grid, ref)
+    assert np.abs(back / x.size - x).max() < 1e-13
+
+
+def _brute_sphere(prob):
+    n1, n2, n3 = prob.grid
+    h1 = n1 // 2 + 1
+    B = 2 * np.pi * np.linalg.inv(prob.supercell).T
+    kz, ky, kx = np.meshgrid(np.arange(n3), np.arange(n2), np.arange(h1), indexing="ij")
+    m = np.stack([kx, np.where(ky <= n2 // 2, ky, ky - n2), np.where(kz <= n3 // 2, kz, kz - n3)], -1)
+    G = m @ B
+    g2 = 0.5 * (G ** 2).sum(-1)
+    upper = (m[..., 0] > 0) | ((m[..., 0] == 0) & ((m[..., 1] > 0) | ((m[..., 1] == 0) & (m[..., 2] >= 0))))
+    sel = (g2 < prob.ecut) & upper
+    return np.flatnonzero(sel.reshape(-1)), g2.reshape(-1)[sel.reshape(-1)]
+
+
+def test_sphere_matches_bruteforce(si8):
+    idx, gkk, src, dst = O.sphere(si8.grid, si8.supercell, si8.ecut)
+    bidx, bg = _brute_sphere(si8)
+    assert np.array_equal(idx, bidx)
+    assert np.abs(gkk - bg).max() < 1e-12
+    assert idx[0] == 0 and len(idx) == 824
+    # padding: every kx = 0 point but G = 0 has exactly one conj partner
+    n1, n2, n3 = si8.grid
+    h1 = n1 // 2 + 1
+    kx0 = [g for g in idx if g % h1 == 0 and g != 0]
+    assert len(src) == len(kx0)
+
+
+def test_sphere_roundtrip_is_hermitian(si8):
+    """c2r of the padded half sphere is real-consistent: r2c(c2r(x))/N gives
+    back exactly the sphere coefficients."""
+    p = si8
+    n1, n2, n3 = p.grid
+    h1 = n1 // 2 + 1
+    half = np.zeros(h1 * n2 * n3, complex)
+    c = p.psi[0]
+    half[p.idx[0]] = c[0]
+    half[p.idx[1:]] = c[1:] / np.sqrt(2)
+    half[p.pad_dst] = np.conj(half[p.pad_src])
+    r = O.fft_c2r(p.grid, half.reshape(n3, n2, h1))
+    back = O.fft_r2c(p.grid, r).reshape(-1) / (n1 * n2 * n3)
+    assert np.abs(back[p.idx] - half[p.idx]).max() < 1e-14
+
+
+def test_ekin_teter_closed_form(si8):
+    e, t = O.ekin_teter(si8.gkk)
+    a = 2 * si8.gkk
+    b = 27 + a * (18 + a * (12 + 8 * a))
+    assert np.allclose(e, si8.gkk, rtol=0, atol=0)
+    assert np.allclose(t, b / (b + 16 * a ** 4), rtol=1e-15)
+
+
+def test_hpsi_matches_dense_golden(si8, golden):
+    assert np.array_equal(golden["idx"], si8.idx)
+    assert abs(golden["vtot_sum"] - si8.vtot.sum()) < 1e-9
+    H = O.Hamiltonian(si8)
+    hp = H.apply_h(golden["psi4"])
+    assert np.abs(hp - golden["hpsi4"]).max() < 1e-12 * np.abs(golden["hpsi4"]).max()
+
+
+def test_plane_wave_is_kinetic_eigenvector(si8):
+    """Constant V0 and no projectors: H c = (|G|^2/2 + V0) c for a single G."""
+    import copy
+    p = copy.copy(si8)
+    p.vtot = np.full_like(si8.vtot, 0.37)
+    p.nbeta = 0
+    H = O.Hamiltonian(p)
+    c = np.zeros((3, p.npw), complex)
+    for k, i in enumerate([0, 17, 500]):
+        c[k, i] = 1.0 if i == 0 else (0.6 - 0.8j)
+    hc = H.apply_h(c)
+    expect = c * (p.ekin[None, :] + 0.37)
+    assert np.abs(hc - expect).max() < 1e-14
+
+
+def test_hermitian_in_re_inner_product(si8):
+    H = O.Hamiltonian(si8)
+    a, b = si8.psi[:5], si8.psi[5:10]
+    ha, hb = H.apply_h(a), H.apply_h(b)
+    lhs = (np.conj(a) @ hb.T).real
+    rhs = (np.conj(ha) @ b.T).real
+    assert np.abs(lhs - rhs).max() < 1e-12
+
+
+def test_gaussian_projector_radial_integral():
+    """int r^(l+2) e^{-r^2/2s^2} j_l(qr) dr = sqrt(pi/2) s^(2l+3) q^l e^{-q^2 s^2/2}."""
+    s = 0.8
+    for l in (0, 1):
+        for q in (0.3, 1.7, 4.2):
+            num, _ = integrate.quad(lambda r: r ** (l + 2) * np.exp(-r * r / (2 * s * s)) *
+                                    scipy.special.spherical_jn(l, q * r), 0, 12, limit=200)
+            closed = np.sqrt(np.pi / 2) * s ** (2 * l + 3) * q ** l * np.exp(-q * q * s * s / 2)
+            assert abs(num - closed) < 1e-10
+
+
+def test_projector_block_against_numpy(si8, golden):
+    """Independent numpy construction of the s/pz/px/py projector block."""
+    p = si8
+    n1, n2, n3 = p.grid
+    h1 = n1 // 2 + 1
+    g = p.idx
+    m = np.stack([g % h1, (g // h1) % n2, g // (h1 * n2)], 1)
+    m[:, 1] = np.where(m[:, 1] <= n2 // 2, m[:, 1], m[:, 1] - n2)
+    m[:, 2] = np.where(m[:, 2] <= n3 // 2, m[:, 2], m[:, 2] - n3)
+    G = m @ (2 * np.pi * np.linalg.inv(p.supercell).T)
+    q = np.linalg.norm(G, axis=1)
+    s = p.proj_sigma
+    gauss = np.exp(-q * q * s * s / 2)
+    cols = []
+    for R in p.atoms:
+        ph = np.exp(-1j * G @ R)
+        cols.append(np.sqrt(1 / (4 * np.pi)) * np.sqrt(np.pi / 2) * s ** 3 * gauss * ph)
+        for c in (2, 0, 1):
+            cols.append(-1j * np.sqrt(3 / (4 * np.pi)) * np.sqrt(np.pi / 2) * s ** 5 * G[:, c] * gauss * ph)
+    B = np.array(cols)
+    B[:, 1:] *= np.sqrt(2)
+    assert np.abs(B - p.beta).max() < 1e-13
+    assert np.abs(golden["proj_vals"] - p.beta[:, golden["proj_sel"]]).max() < 1e-14
+
+
+def test_syev_potrf_vs_lapack():
+    rng = np.random.default_rng(3)
+    for n in (1, 5, 64, 150):
+        A = rng.normal(size=(n, n))
+        A = A + A.T
+        w, V = O.syev(A)
+        assert np.abs(w - np.linalg.eigvalsh(A)).max() < 1e-12 * max(1, np.abs(w).max())
+        assert np.abs(A @ V - V * w).max() < 1e-11 * max(1, np.abs(w).max())
+        assert np.abs(V.T @ V - np.eye(n)).max() < 1e-12
+        S = A @ A.T + n * np.eye(n)
+        info, U = O.potrf_upper(S)
+        assert info == 0
+        assert np.abs(U - np.linalg.cholesky(S).T).max() < 1e-10
+
+
+def test_solve_gen_eig_vs_scipy():
+    rng = np.random.default_rng(4)
+    n, width = 30, 10
+    A = rng.normal(size=(n, n)); A = A + A.T
+    Bm = rng.normal(size=(n, n)); Bm = Bm @ Bm.T + n * np.eye(n)
+    rc, e, C = O.solve_gen_eig(np.triu(A), np.triu(Bm), width)
+    assert rc == 0
+    ref = scipy.linalg.eigh(A, Bm, eigvals_only=True)[:width]
+    assert np.abs(e - ref).max() < 1e-11
+    Cw = C[:, :width]
+    assert np.abs(A @ Cw - Bm @ Cw * e).max() < 1e-10
+    # rank-deficient B is truncated at 1e-8 and reports too few columns
+    Bd = np.zeros((n, n)); Bd[:5, :5] = np.eye(5)
+    rc, _, _ = O.solve_gen_eig(np.triu(A), Bd, width)
+    assert rc != 0
+
+
+def test_lobpcg_converges_to_dense(si8, golden):
+    H = O.Hamiltonian(si8)
+    X, e, r, it = H.lobpcg(si8.psi, max_iter=60, tol=1e-9)
+    assert np.abs(e[:28] - golden["evals"][:28]).max() < 1e-9
+
+
+def test_chebyshev_filter_identity(si8, golden):
+    """On an exact eigenvector v of H, the scaled filter returns
+    T_m(L(lam)) / T_m(L(aL)) v with L(x) = (x - c)/e."""
+    from tests.golden.make_golden import dense_h, full_to_half
+    Hd, m = dense_h(si8)
+    w, V = np.linalg.eigh(Hd)
+    # a real-space-real eigenvector: psi(-G) = conj(psi(G)) (Gamma symmetry)
+    v = V[:, 0] / V[0, 0] * abs(V[0, 0])
+    c = full_to_half(v, si8.npw)[None, :]
+    c[0, 0] = c[0, 0].real
+    H = O.Hamiltonian(si8)
+    a, b, aL, deg = 0.3, 12.0, -1.2, 10
+    out = H.cheb_filter(c, deg, a, b, aL)
+    e, cc = (b - a) / 2, (b + a) / 2
+    def T(k, x):
+        if abs(x) <= 1:
+            return np.cos(k * np.arccos(x))
+        return np.sign(x) ** k * np.cosh(k * np.arccosh(abs(x)))
+
+    def L(x):
+        return (x - cc) / e
+
+    fac = T(deg, L(w[0])) / T(deg, L(aL))
+    assert np.abs(out - fac * c).max() < 1e-9 * np.abs(c).max()
+    # the golden scalar table agrees with the closed forms used here
+    xs, Tt = golden["cheb_x"], golden["cheb_T"]
+    for j, x in enumerate(xs):
+        p0, p1 = 1.0, x
+        for k in range(2, 13):
+            p0, p1 = p1, 2 * x * p1 - p0
+            assert abs(p1 - Tt[k, j]) < 1e-9 * max(1, abs(Tt[k, j]))
+
+
+def test_lanczos_bounds(si8, golden):
+    H = O.Hamiltonian(si8)
+    th, bup = H.lanczos(8, si8.lanczos_start())
+    ev = golden["evals"]
+    assert bup >= ev[-1]
+    assert th.min() >= ev[0] - 1e-9 and th.max() <= ev[-1] + 1e-9
+
+
+def test_chebfsi_converges(si8, golden):
+    H = O.Hamiltonian(si8)
+    X, prev = si8.psi.copy(), None
+    v0 = si8.lanczos_start()
+    for _ in range(10):
+        X, prev, r, b = H.chebfsi_sweep(X, 10, 8, v0, prev)
+    assert np.abs(prev[:24] - golden["evals"][:24]).max() < 1e-10
+
+
+def test_density_sum_and_fd(si8):
+    """sum_r rho = sum_i f_i |c_i|^2, and rho = N dE/dV by central differences."""
+    import copy
+    H = O.Hamiltonian(si8)
+    f = np.full(si8.nb, 2.0)
+    rho = H.density(si8.psi, f)
+    assert abs(rho.sum() - f.sum()) < 1e-10
+    N = si8.ngrid
+    rng = np.random.default_rng(7)
+    for r in rng.integers(0, N, 3):
+        dv = 1e-5
+        vals = []
+        for sgn in (+1, -1):
+            p = copy.copy(si8)
+            v = si8.vtot.copy().reshape(-1)
+            v[r] += sgn * dv
+            p.vtot = v.reshape(si8.vtot.shape)
+            vals.append(O.Hamiltonian(p).band_energy(si8.psi, f))
+        fd = (vals[0] - vals[1]) / (2 * dv)
+        assert abs(fd - rho[r]) < 1e-7 * max(1, abs(rho[r]))
+
+
+def test_nonlocal_force_fd(si8):
+    import copy
+    H = O.Hamiltonian(si8)
+    f = np.full(si8.nb, 2.0)
+    F = H.nonlocal_force(si8.psi, f, len(si8.atoms), 4)
+    h = 1e-4
+    for a, d in ((0, 0), (3, 2), (5, 1)):
+        vals = []
+        for sgn in (+1, -1):
+            p = copy.copy(si8)
+            p.atoms = si8.atoms.copy()
+            p.atoms[a, d] += sgn * h
+            p.make_projectors()
+            vals.append(O.Hamiltonian(p).band_energy(si8.psi, f))
+        fd = -(vals[0] - vals[1]) / (2 * h)
+        assert abs(fd - F[a, d]) < 1e-6 * max(1, abs(F[a, d]))
+
+
+def test_exx_against_numpy_and_symmetry():
+    """Screened exchange on a small cell: the oracle's 3-FFT pair loop vs an
+    independent numpy-FFT evaluation of the same operator, Hermiticity, and
+    the kernel's closed form at G != 0 and G = 0."""
+    prob = W.Problem("exx12", (12, 12, 12), W.cubic(7.0), 5.0, W.DIAMOND[:2] * 7.0, 6).build()
+    H = O.Hamiltonian(prob)
+    occ = np.array([1.0, 1.0, 0.5, 0.25])
+    mu, frac, ecut = 0.106, 0.25, prob.ecut
+    div = H.set_exx(ecut, mu, frac, prob.psi[:4], occ)
+    psi = prob.psi
+    got = H.add_exx(psi)
+    n1, n2, n3 = prob.grid
+    N, vol = n1 * n2 * n3, prob.volume
+    h1 = n1 // 2 + 1
+
+    def to_real(c):
+        half = np.zeros(h1 * n2 * n3, complex)
+        half[prob.idx[0]] = c[0]
+        half[prob.idx[1:]] = c[1:] / np.sqrt(2)
+        half[prob.pad_dst] = np.conj(half[prob.pad_src])
+        return np.fft.irfftn(half.reshape(n3, n2, h1), s=(n3, n2, n1), axes=(0, 1, 2)) * N
+
+    fidx, fg = O.fock_sphere(prob.grid, prob.supercell, ecut)
+    g2 = 2 * fg
+    K = np.where(g2 > 1e-8, 4 * np.pi / np.where(g2 > 1e-8, g2, 1) * (1 - np.exp(-g2 / (4 * mu * mu))),
+                 -div + np.pi / mu ** 2)
+    Kfull = np.zeros(h1 * n2 * n3)
+    Kfull[fidx] = K
+    phi = [to_real(c * np.sqrt(1 / (vol * N))) for c in prob.psi[:4]]
+    ref = np.zeros_like(psi)
+    for k in range(psi.shape[0]):
+        pk = to_real(psi[k])
+        acc = np.zeros((n3, n2, n1))
+        for j in range(4):
+            rhoG = np.fft.rfftn(pk * phi[j]).reshape(-1) * Kfull / N
+            acc += -frac * occ[j] * phi[j] * np.fft.irfftn(rhoG.reshape(n3, n2, h1), s=(n3, n2, n1), axes=(0, 1, 2)) * N
+        y = np.fft.rfftn(acc).reshape(-1)
+        ref[k, 0] = y[prob.idx[0]]
+        ref[k, 1:] = np.sqrt(2) * y[prob.idx[1:]]
+    assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
+    lhs = (np.conj(psi) @ got.T).real
+    assert np.abs(lhs - lhs.T).max() < 1e-12
