@@ -157,6 +157,60 @@ int pw_chebfsi_sweep(pw_ctx* ctx, int mode, int nb, int m, int k,
 
 int pw_free_workspace(pw_ctx* ctx);
 
+/* EigenSolver::LOBPCGSolveReal(numEig, scfIter, eigMaxIter, eigMinTolerance,
+ * eigTolerance), sphere-cut branch (src/EigenSolver/lobpcg.cpp:59-576), with
+ * SolveGenEig's sequential branch (src/EigenSolver/eigensolver.cpp:224-297).
+ * X: device FP64 block (nb bands), overwritten with the eigenvectors;
+ * eigval/resval: host arrays of nb (eigVal_, resVal_); *iters = iterations
+ * done.  Scratch (7 blocks + (3nb)^2 matrices) is allocated per call. */
+int pw_lobpcg(pw_ctx* ctx, int nb, int numEig, int scfIter, int eigMaxIter,
+              double eigMinTolerance, double eigTolerance, void* X,
+              double* eigval, double* resval, int* iters);
+
+/* ------------------------------------------------------------------ */
+/* Screened exact exchange.  pw_set_exx replaces KohnSham::InitializeEXX
+ * (src/SCF/hamiltonian.cpp:3454-3551: exxDiv, kernel
+ * 4 pi/|G|^2 (1 - exp(-|G|^2/(4 mu^2))) on the Fock sphere, G = 0 term
+ * -exxDiv + pi/mu^2) plus SetPhiEXX (hamiltonian.cpp:3569-3603: phi =
+ * psi sqrt(1/(vol N))).  psi_occ: device FP64 block of nocc bands; occ:
+ * host occupations; divtype 0 or 1 (EXX_Divergence_Type).  Once set,
+ * pw_mult_spinor and pw_apply_h add V_x psi, as KohnSham::MultSpinor does
+ * when isHybrid_ && isEXXActive_ (hamiltonian.cpp:3431-3448).  nocc = 0
+ * removes the operator. */
+int pw_set_exx(pw_ctx* ctx, double ecut, double screen_mu, double exx_fraction,
+               int divtype, int nocc, const void* psi_occ, const double* occ);
+
+/* Spinor::AddMultSpinorEXX, sphere-cut branch (src/SCF/spinor.cpp:783-945):
+ * a3 += V_x psi for nb bands (dtype blocks). */
+int pw_add_exx(pw_ctx* ctx, int dtype, int nb, const void* psi, void* a3);
+
+/* exxDiv of KohnSham::InitializeEXX (hamiltonian.cpp:3473-3517), host only. */
+int pw_exx_divergence(const int grid[3], const double supercell[9], double ecut,
+                      double screen_mu, int divtype, double* exx_div);
+
+/* ------------------------------------------------------------------ */
+/* Adjoint of the band energy E = sum_i f_i Re<psi_i|H[V, R]|psi_i>.
+ *
+ * pw_density: rho(r) (+)= sum_i f_i psi_i(r)^2 / N with psi_i = C2R(c_i):
+ * the accumulation of KohnSham::CalculateDensity, sphere branch
+ * (src/SCF/hamiltonian.cpp:2509-2573, f_i = numSpin * occ_i, before the
+ * renormalisation at :2599-2600), which is also dE/dV(r).  rho: device array
+ * of N doubles; f: host.  Bands are recomputed from the sphere, never kept
+ * in real space. */
+int pw_density(pw_ctx* ctx, int dtype, int nb, const void* psi, const double* f,
+               double* rho, int accumulate);
+
+/* E = sum_i f_i Re<x_i|H x_i> (FP64 block, host f and E). */
+int pw_band_energy(pw_ctx* ctx, int nb, const void* X, const double* f, double* E);
+
+/* Nonlocal force F_a = -dE/dR_a through the projectors, d beta/dR = -iG beta:
+ * F_a = -2 sum_i f_i sum_{p in a} w_p Re<beta_p|x_i> Re<beta_p|iG x_i>, the
+ * G-space form of KohnSham::CalculateForce's nonlocal term
+ * (src/SCF/hamiltonian.cpp:2907-3006).  Projectors laid out atom-major, npa
+ * per atom (vnlc_ layout); force: host 3*natom. */
+int pw_nonlocal_force(pw_ctx* ctx, int nb, const void* X, const double* f,
+                      int natom, int npa, double* force);
+
 /* Stage timing of pw_mult_spinor / pw_apply_h calls, measured with HIP
  * events recorded on the context's stream around each kernel (no host
  * synchronisation).  pw_set_timing(1) clears the totals; pw_get_timing

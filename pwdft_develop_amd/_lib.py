@@ -67,6 +67,14 @@ def lib():
         "pw_rayleigh_ritz": [_vp, C.c_int, C.c_int, _vp, _dp, _dp],
         "pw_chebfsi_sweep": [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _dp, _dp, _dp],
         "pw_free_workspace": [_vp],
+        "pw_density": [_vp, C.c_int, C.c_int, _vp, _dp, _vp, C.c_int],
+        "pw_band_energy": [_vp, C.c_int, _vp, _dp, C.POINTER(C.c_double)],
+        "pw_nonlocal_force": [_vp, C.c_int, _vp, _dp, C.c_int, C.c_int, _dp],
+        "pw_set_exx": [_vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _vp, _dp],
+        "pw_add_exx": [_vp, C.c_int, C.c_int, _vp, _vp],
+        "pw_exx_divergence": [_ip, _dp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double)],
+        "pw_lobpcg": [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _vp, _dp, _dp,
+                      C.POINTER(C.c_int)],
         "pw_set_timing": [_vp, C.c_int],
         "pw_get_timing": [_vp, _dp, C.POINTER(C.c_int)],
     }

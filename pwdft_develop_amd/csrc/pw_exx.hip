@@ -1,0 +1,736 @@
+// Screened exact exchange on gfx950: Spinor::AddMultSpinorEXX, sphere-cut
+// branch (src/SCF/spinor.cpp:783-945), with the kernel of
+// KohnSham::InitializeEXX (src/SCF/hamiltonian.cpp:3454-3551) and the phi
+// scaling of KohnSham::SetPhiEXX (hamiltonian.cpp:3569-3603).
+//
+// The reference runs 3 FFTs per (band k, occupied j) pair: R2C of the pair
+// density psi_k phi_j, C2R of the kernel-weighted spectrum, and an R2C of
+// phi_j times the result that is gathered into a3_k.  The last transform is
+// linear, so it is taken once per band on the accumulated real-space sum:
+// 2 FFTs per pair + 1 per band, equal up to rounding.  All transforms are
+// batched over pairs; the Fock sphere (|G|^2/2 < 4 Ecut) gets its own stick
+// plan, so its z transforms touch only the Fock columns.
+//
+//   k_exx_slab_inv   wavefunction sticks -> y IFFT -> x c2r -> real grid (psi_k, phi_j)
+//   k_exx_pair_fwd   psi_k(r) phi_j(r) -> x r2c -> y FFT -> Fock sticks
+//   k_exx_fock_z     Fock sticks: z FFT, x K(G)/N, z IFFT (in place)
+//   k_exx_pair_acc   Fock sticks -> y IFFT -> x c2r -> acc_k += f_j phi_j u_kj, over j
+//   k_exx_slab_fwd   acc_k(r) -> x r2c -> y FFT -> wavefunction sticks -> a3 += gather
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "pw_fft.h"
+#include "pw_internal.h"
+
+namespace pw {
+
+struct ExxState {
+  double ecut = 0, mu = 0, frac = 0, div = 0;
+  int divtype = 1, nocc = 0;
+  int nfock = 0;
+  // Fock stick plan
+  int nstick = 0, kxmax = 0, sc = 16, nchunk = 0;
+  int* d_pos = nullptr;          // kx*n2 + ky per Fock stick
+  double* d_K64 = nullptr;       // K(G)/N per (stick, z)
+  float* d_K32 = nullptr;
+  double* d_phi64 = nullptr;     // phi_j(r), nocc x N
+  float* d_phi32 = nullptr;
+  double* d_fac64 = nullptr;     // -frac * occ_j
+  float* d_fac32 = nullptr;
+  std::vector<double> occ;
+  // scratch
+  void* d_psir = nullptr;        // max_bands x N (double-sized)
+  void* d_acc = nullptr;         // max_bands x N
+  void* d_fst = nullptr;         // pair Fock sticks: npair_max x n3 x nstick complex double
+  int npair_max = 0;
+};
+
+// ------------------------------------------------------------ x-pass helpers
+// Rows y1 = 2q, y2 = 2q+1 (q in [q0, q0+nq)) of the y-transformed spectrum
+// Y[kx * n2 + y] (kx <= kxmax) -> real rows packed as Z = u1 + i u2.
+template <typename T>
+__device__ typename cx<T>::t* x_c2r_rows(const typename cx<T>::t* Y, int kxmax, int n1, int n2, int q0, int nq,
+                                         typename cx<T>::t* Z0, typename cx<T>::t* Z1,
+                                         const typename cx<T>::t* tw1) {
+  typedef typename cx<T>::t C;
+  const int h = n1 / 2;
+  for (int t = threadIdx.x; t < nq * n1; t += blockDim.x) {
+    const int ql = t / n1, k = t - ql * n1;
+    const int y1 = 2 * (q0 + ql), y2 = y1 + 1;
+    C a1 = C{0, 0}, a2 = C{0, 0};
+    int kk = k;
+    bool cj = false;
+    if (k > h) { kk = n1 - k; cj = true; }
+    if (kk <= kxmax) {
+      a1 = Y[kk * n2 + y1];
+      if (y2 < n2) a2 = Y[kk * n2 + y2];
+      if (cj) { a1.y = -a1.y; a2.y = -a2.y; }
+      if (k == 0 || (2 * k == n1)) { a1.y = 0; a2.y = 0; }
+    }
+    Z0[ql * n1 + k] = C{a1.x - a2.y, a1.y + a2.x};
+  }
+  __syncthreads();
+  return lds_fft<T>(Z0, Z1, n1, nq, n1, tw1, (T)1);
+}
+
+// Real rows packed in Zr -> forward x FFT -> spectra of both rows for
+// kx <= kxmax written to Y[kx * n2 + y].
+template <typename T>
+__device__ void x_r2c_rows(typename cx<T>::t* Zr, typename cx<T>::t* Zo, typename cx<T>::t* Y, int kxmax, int n1,
+                           int n2, int q0, int nq, const typename cx<T>::t* tw1) {
+  typedef typename cx<T>::t C;
+  const int nk = kxmax + 1;
+  C* Zf = lds_fft<T>(Zr, Zo, n1, nq, n1, tw1, (T)-1);
+  for (int t = threadIdx.x; t < nq * nk; t += blockDim.x) {
+    const int ql = t / nk, k = t - ql * nk;
+    const int y1 = 2 * (q0 + ql), y2 = y1 + 1;
+    const C w = Zf[ql * n1 + k];
+    const C wc = Zf[ql * n1 + (k == 0 ? 0 : n1 - k)];
+    Y[k * n2 + y1] = C{(T)0.5 * (w.x + wc.x), (T)0.5 * (w.y - wc.y)};
+    if (y2 < n2) Y[k * n2 + y2] = C{(T)0.5 * (w.y + wc.y), (T)0.5 * (wc.x - w.x)};
+  }
+  __syncthreads();
+}
+
+struct SlabGeom {
+  int n1, n2, n3, nstick, kxmax, rp;
+};
+
+template <typename T>
+__device__ void load_tw(typename cx<T>::t* tw1, typename cx<T>::t* tw2, const typename cx<T>::t* g1,
+                        const typename cx<T>::t* g2, int n1, int n2) {
+  for (int t = threadIdx.x; t < n1; t += blockDim.x) tw1[t] = g1[t];
+  for (int t = threadIdx.x; t < n2; t += blockDim.x) tw2[t] = g2[t];
+}
+
+// sticks (with optional conj mirrors) -> y IFFT; returns the y-space buffer
+template <typename T>
+__device__ typename cx<T>::t* sticks_to_yspace(const typename cx<T>::t* src, const int* pos, const int* mpos,
+                                               const SlabGeom& g, typename cx<T>::t* Y0, typename cx<T>::t* Y1,
+                                               const typename cx<T>::t* tw2) {
+  typedef typename cx<T>::t C;
+  const int nk = g.kxmax + 1;
+  for (int t = threadIdx.x; t < nk * g.n2; t += blockDim.x) Y0[t] = C{0, 0};
+  __syncthreads();
+  for (int s = threadIdx.x; s < g.nstick; s += blockDim.x) {
+    const C v = src[s];
+    Y0[pos[s]] = v;
+    if (mpos) {
+      const int mp = mpos[s];
+      if (mp >= 0) Y0[mp] = C{v.x, -v.y};
+    }
+  }
+  __syncthreads();
+  return lds_fft<T>(Y0, Y1, g.n2, nk, g.n2, tw2, (T)1);
+}
+
+#define SLAB_LDS_SETUP                                      \
+  typedef typename cx<T>::t C;                              \
+  extern __shared__ __align__(16) unsigned char smem_raw[]; \
+  const int nk = g.kxmax + 1;                               \
+  C* tw1 = reinterpret_cast<C*>(smem_raw);                  \
+  C* tw2 = tw1 + g.n1;                                      \
+  C* Y0 = tw2 + g.n2;                                       \
+  C* Y1 = Y0 + nk * g.n2;                                   \
+  C* Z0 = Y1 + nk * g.n2;                                   \
+  C* Z1 = Z0 + g.rp * g.n1;                                 \
+  (void)Z1;
+
+// wavefunction sticks -> real grid, scaled
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_exx_slab_inv(const typename cx<T>::t* __restrict__ st, T* __restrict__ out, const int* __restrict__ pos,
+               const int* __restrict__ mpos, SlabGeom g, T scale, const typename cx<T>::t* tw1g,
+               const typename cx<T>::t* tw2g) {
+  SLAB_LDS_SETUP
+  const int z = blockIdx.x, b = blockIdx.y;
+  load_tw<T>(tw1, tw2, tw1g, tw2g, g.n1, g.n2);
+  C* Y = sticks_to_yspace<T>(st + ((size_t)b * g.n3 + z) * g.nstick, pos, mpos, g, Y0, Y1, tw2);
+  T* o = out + ((size_t)b * g.n3 + z) * g.n1 * g.n2;
+  const int npair = (g.n2 + 1) / 2;
+  for (int q0 = 0; q0 < npair; q0 += g.rp) {
+    const int nq = min(g.rp, npair - q0);
+    C* Zr = x_c2r_rows<T>(Y, g.kxmax, g.n1, g.n2, q0, nq, Z0, Z1, tw1);
+    for (int t = threadIdx.x; t < nq * g.n1; t += blockDim.x) {
+      const int ql = t / g.n1, x = t - ql * g.n1;
+      const int y1 = 2 * (q0 + ql), y2 = y1 + 1;
+      const C u = Zr[ql * g.n1 + x];
+      o[(size_t)y1 * g.n1 + x] = u.x * scale;
+      if (y2 < g.n2) o[(size_t)y2 * g.n1 + x] = u.y * scale;
+    }
+    __syncthreads();
+  }
+}
+
+// pair densities psi_k phi_j -> Fock sticks; pair p = k + nb * jl
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_exx_pair_fwd(const T* __restrict__ psir, const T* __restrict__ phir, int nb, int j0,
+               typename cx<T>::t* __restrict__ fst, const int* __restrict__ pos, SlabGeom g,
+               const typename cx<T>::t* tw1g, const typename cx<T>::t* tw2g) {
+  SLAB_LDS_SETUP
+  const int z = blockIdx.x, p = blockIdx.y;
+  const int k = p % nb, j = j0 + p / nb;
+  const size_t plane = (size_t)g.n1 * g.n2;
+  const T* a = psir + ((size_t)k * g.n3 + z) * plane;
+  const T* bph = phir + ((size_t)j * g.n3 + z) * plane;
+  load_tw<T>(tw1, tw2, tw1g, tw2g, g.n1, g.n2);
+  __syncthreads();
+  const int npair = (g.n2 + 1) / 2;
+  for (int q0 = 0; q0 < npair; q0 += g.rp) {
+    const int nq = min(g.rp, npair - q0);
+    for (int t = threadIdx.x; t < nq * g.n1; t += blockDim.x) {
+      const int ql = t / g.n1, x = t - ql * g.n1;
+      const int y1 = 2 * (q0 + ql), y2 = y1 + 1;
+      const T u1 = a[(size_t)y1 * g.n1 + x] * bph[(size_t)y1 * g.n1 + x];
+      const T u2 = (y2 < g.n2) ? a[(size_t)y2 * g.n1 + x] * bph[(size_t)y2 * g.n1 + x] : (T)0;
+      Z0[ql * g.n1 + x] = C{u1, u2};
+    }
+    __syncthreads();
+    x_r2c_rows<T>(Z0, Z1, Y0, g.kxmax, g.n1, g.n2, q0, nq, tw1);
+  }
+  C* Yf = lds_fft<T>(Y0, Y1, g.n2, nk, g.n2, tw2, (T)-1);
+  C* dst = fst + ((size_t)p * g.n3 + z) * g.nstick;
+  for (int s = threadIdx.x; s < g.nstick; s += blockDim.x) dst[s] = Yf[pos[s]];
+}
+
+// Fock sticks: forward z, x K/N, inverse z (in place); grid (chunk, pair)
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_exx_fock_z(typename cx<T>::t* __restrict__ fst, const T* __restrict__ K, int n3, int nstick, int sc,
+             const typename cx<T>::t* __restrict__ tw3g) {
+  typedef typename cx<T>::t C;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* tw = reinterpret_cast<C*>(smem_raw);
+  C* A = tw + n3;
+  C* B = A + sc * n3;
+  const int c = blockIdx.x, p = blockIdx.y;
+  const int s0 = c * sc;
+  const int nsl = min(sc, nstick - s0);
+  for (int t = threadIdx.x; t < n3; t += blockDim.x) tw[t] = tw3g[t];
+  C* base = fst + (size_t)p * n3 * nstick;
+  for (int t = threadIdx.x; t < n3 * sc; t += blockDim.x) {
+    const int z = t / sc, sl = t - z * sc;
+    A[sl * n3 + z] = (sl < nsl) ? base[(size_t)z * nstick + s0 + sl] : C{0, 0};
+  }
+  __syncthreads();
+  C* R = lds_fft<T>(A, B, n3, sc, n3, tw, (T)-1);
+  for (int t = threadIdx.x; t < n3 * nsl; t += blockDim.x) {
+    const int sl = t / n3, z = t - sl * n3;
+    R[sl * n3 + z] = cscale(R[sl * n3 + z], K[(size_t)(s0 + sl) * n3 + z]);
+  }
+  __syncthreads();
+  C* O = (R == A) ? B : A;
+  C* Q = lds_fft<T>(R, O, n3, sc, n3, tw, (T)1);
+  for (int t = threadIdx.x; t < n3 * nsl; t += blockDim.x) {
+    const int z = t / nsl, sl = t - z * nsl;
+    base[(size_t)z * nstick + s0 + sl] = Q[sl * n3 + z];
+  }
+}
+
+// acc_k(r) += sum_jl fac_j phi_j(r) c2r(Fock sticks of pair (k, jl))(r)
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_exx_pair_acc(const typename cx<T>::t* __restrict__ fst, const T* __restrict__ phir, const T* __restrict__ fac,
+               int nb, int j0, int nj, T* __restrict__ acc, const int* __restrict__ pos, SlabGeom g,
+               const typename cx<T>::t* tw1g, const typename cx<T>::t* tw2g) {
+  SLAB_LDS_SETUP
+  const int z = blockIdx.x, k = blockIdx.y;
+  const size_t plane = (size_t)g.n1 * g.n2;
+  T* ak = acc + ((size_t)k * g.n3 + z) * plane;
+  load_tw<T>(tw1, tw2, tw1g, tw2g, g.n1, g.n2);
+  const int npair = (g.n2 + 1) / 2;
+  for (int jl = 0; jl < nj; jl++) {
+    const int j = j0 + jl;
+    const T fj = fac[j];
+    const T* ph = phir + ((size_t)j * g.n3 + z) * plane;
+    const int p = k + nb * jl;
+    C* Y = sticks_to_yspace<T>(fst + ((size_t)p * g.n3 + z) * g.nstick, pos, nullptr, g, Y0, Y1, tw2);
+    for (int q0 = 0; q0 < npair; q0 += g.rp) {
+      const int nq = min(g.rp, npair - q0);
+      C* Zr = x_c2r_rows<T>(Y, g.kxmax, g.n1, g.n2, q0, nq, Z0, Z1, tw1);
+      for (int t = threadIdx.x; t < nq * g.n1; t += blockDim.x) {
+        const int ql = t / g.n1, x = t - ql * g.n1;
+        const int y1 = 2 * (q0 + ql), y2 = y1 + 1;
+        const C u = Zr[ql * g.n1 + x];
+        ak[(size_t)y1 * g.n1 + x] += fj * ph[(size_t)y1 * g.n1 + x] * u.x;
+        if (y2 < g.n2) ak[(size_t)y2 * g.n1 + x] += fj * ph[(size_t)y2 * g.n1 + x] * u.y;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// real grid acc_k -> x r2c -> y FFT -> wavefunction sticks
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_exx_slab_fwd(const T* __restrict__ in, typename cx<T>::t* __restrict__ st, const int* __restrict__ pos,
+               SlabGeom g, const typename cx<T>::t* tw1g, const typename cx<T>::t* tw2g) {
+  SLAB_LDS_SETUP
+  const int z = blockIdx.x, b = blockIdx.y;
+  const T* a = in + ((size_t)b * g.n3 + z) * g.n1 * g.n2;
+  load_tw<T>(tw1, tw2, tw1g, tw2g, g.n1, g.n2);
+  __syncthreads();
+  const int npair = (g.n2 + 1) / 2;
+  for (int q0 = 0; q0 < npair; q0 += g.rp) {
+    const int nq = min(g.rp, npair - q0);
+    for (int t = threadIdx.x; t < nq * g.n1; t += blockDim.x) {
+      const int ql = t / g.n1, x = t - ql * g.n1;
+      const int y1 = 2 * (q0 + ql), y2 = y1 + 1;
+      Z0[ql * g.n1 + x] = C{a[(size_t)y1 * g.n1 + x], (y2 < g.n2) ? a[(size_t)y2 * g.n1 + x] : (T)0};
+    }
+    __syncthreads();
+    x_r2c_rows<T>(Z0, Z1, Y0, g.kxmax, g.n1, g.n2, q0, nq, tw1);
+  }
+  C* Yf = lds_fft<T>(Y0, Y1, g.n2, nk, g.n2, tw2, (T)-1);
+  C* dst = st + ((size_t)b * g.n3 + z) * g.nstick;
+  for (int s = threadIdx.x; s < g.nstick; s += blockDim.x) dst[s] = Yf[pos[s]];
+}
+
+static size_t slab_bytes(const pw_ctx* c, int esz, int kxmax, int rp) {
+  return (size_t)esz * (c->n1 + c->n2 + 2 * (size_t)(kxmax + 1) * c->n2 + 2 * (size_t)rp * c->n1);
+}
+
+static int pick_rp(const pw_ctx* c, int esz, int kxmax) {
+  int rp = 8;
+  while (rp > 1 && slab_bytes(c, esz, kxmax, rp) > 96 * 1024) rp /= 2;
+  return rp;
+}
+
+template <typename K>
+static int set_lds(K kern, size_t bytes) {
+  if (bytes > 160 * 1024) return fail(PW_ERR_UNSUPPORTED, "exchange slab needs %zu bytes of LDS (> 160 KiB)", bytes);
+  if (bytes > 64 * 1024)
+    PW_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return PW_OK;
+}
+
+// psi (sphere, dtype T) -> real grid (scale) for nb bands, via the wavefunction sticks
+template <typename T>
+static int sphere_to_real(pw_ctx* c, int nb, const void* psi, T* out, T scale) {
+  typedef typename cx<T>::t C;
+  PW_TRY(sphere_to_sticks<T>(c, nb, psi, c->d_t1));
+  const C* tw = (const C*)(sizeof(T) == 8 ? (const void*)c->d_tw64 : (const void*)c->d_tw32);
+  SlabGeom g{c->n1, c->n2, c->n3, c->sp.nstick, c->sp.kxmax, 0};
+  g.rp = pick_rp(c, sizeof(C), g.kxmax);
+  const size_t lds = slab_bytes(c, sizeof(C), g.kxmax, g.rp);
+  PW_TRY(set_lds(k_exx_slab_inv<T>, lds));
+  hipLaunchKernelGGL(k_exx_slab_inv<T>, dim3(c->n3, nb), dim3(256), lds, c->stream, (const C*)c->d_t1, out,
+                     c->d_pos, c->d_mpos, g, scale, tw, tw + c->n1);
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+template <typename T>
+int exx_add(pw_ctx* c, int nb, const void* psi, void* a3) {
+  typedef typename cx<T>::t C;
+  ExxState* E = c->exx;
+  if (!E || E->nocc == 0 || nb == 0) return PW_OK;
+  const long N = c->N;
+  T* psir = (T*)E->d_psir;
+  T* acc = (T*)E->d_acc;
+  const T* phir = (const T*)(sizeof(T) == 8 ? (const void*)E->d_phi64 : (const void*)E->d_phi32);
+  const T* fac = (const T*)(sizeof(T) == 8 ? (const void*)E->d_fac64 : (const void*)E->d_fac32);
+  const T* K = (const T*)(sizeof(T) == 8 ? (const void*)E->d_K64 : (const void*)E->d_K32);
+  const C* tw = (const C*)(sizeof(T) == 8 ? (const void*)c->d_tw64 : (const void*)c->d_tw32);
+  PW_TRY(sphere_to_real<T>(c, nb, psi, psir, (T)1));
+  PW_HIP(hipMemsetAsync(acc, 0, sizeof(T) * N * nb, c->stream));
+  SlabGeom gf{c->n1, c->n2, c->n3, E->nstick, E->kxmax, 0};
+  gf.rp = pick_rp(c, sizeof(C), gf.kxmax);
+  const size_t ldsf = slab_bytes(c, sizeof(C), gf.kxmax, gf.rp);
+  PW_TRY(set_lds(k_exx_pair_fwd<T>, ldsf));
+  PW_TRY(set_lds(k_exx_pair_acc<T>, ldsf));
+  const int J = std::max(1, std::min(E->nocc, E->npair_max / nb));
+  const size_t ldsz = sizeof(C) * (c->n3 + 2 * (size_t)E->sc * c->n3);
+  for (int j0 = 0; j0 < E->nocc; j0 += J) {
+    const int nj = std::min(J, E->nocc - j0);
+    hipLaunchKernelGGL(k_exx_pair_fwd<T>, dim3(c->n3, nb * nj), dim3(256), ldsf, c->stream, (const T*)psir, phir,
+                       nb, j0, (C*)E->d_fst, E->d_pos, gf, tw, tw + c->n1);
+    PW_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_exx_fock_z<T>, dim3(E->nchunk, nb * nj), dim3(256), ldsz, c->stream, (C*)E->d_fst, K,
+                       c->n3, E->nstick, E->sc, tw + c->n1 + c->n2);
+    PW_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_exx_pair_acc<T>, dim3(c->n3, nb), dim3(256), ldsf, c->stream, (const C*)E->d_fst, phir,
+                       fac, nb, j0, nj, acc, E->d_pos, gf, tw, tw + c->n1);
+    PW_HIP(hipGetLastError());
+  }
+  SlabGeom g{c->n1, c->n2, c->n3, c->sp.nstick, c->sp.kxmax, 0};
+  g.rp = pick_rp(c, sizeof(C), g.kxmax);
+  const size_t lds = slab_bytes(c, sizeof(C), g.kxmax, g.rp);
+  PW_TRY(set_lds(k_exx_slab_fwd<T>, lds));
+  hipLaunchKernelGGL(k_exx_slab_fwd<T>, dim3(c->n3, nb), dim3(256), lds, c->stream, (const T*)acc, (C*)c->d_t2,
+                     c->d_pos, g, tw, tw + c->n1);
+  PW_HIP(hipGetLastError());
+  return sticks_to_sphere_add<T>(c, nb, c->d_t2, a3);
+}
+
+template int exx_add<double>(pw_ctx*, int, const void*, void*);
+template int exx_add<float>(pw_ctx*, int, const void*, void*);
+
+// ------------------------------------------------------------------ adjoint
+// dE/dV(r) of E = sum_i f_i <psi_i|H|psi_i>: sum_i f_i psi_i(r)^2 / N with
+// psi_i(r) = C2R(c_i) -- the accumulation of KohnSham::CalculateDensity
+// (src/SCF/hamiltonian.cpp:2509-2573) with f_i = numSpin occ_i.  Bands are
+// recomputed from their sticks chunk by chunk; real-space psi is never
+// stored.  Block (z, split) sums the bands b = split, split + nsplit, ...
+// of the chunk into its own partial plane; partials are reduced in order.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_density_acc(const typename cx<T>::t* __restrict__ st, const double* __restrict__ f, int b0, int nbc, int nsplit,
+              double* __restrict__ part, const int* __restrict__ pos, const int* __restrict__ mpos, SlabGeom g,
+              double invN, const typename cx<T>::t* tw1g, const typename cx<T>::t* tw2g) {
+  SLAB_LDS_SETUP
+  const int z = blockIdx.x, sp = blockIdx.y;
+  const size_t plane = (size_t)g.n1 * g.n2;
+  double* acc = part + ((size_t)sp * g.n3 + z) * plane;
+  load_tw<T>(tw1, tw2, tw1g, tw2g, g.n1, g.n2);
+  const int npair = (g.n2 + 1) / 2;
+  for (int bl = sp; bl < nbc; bl += nsplit) {
+    const double fb = f[b0 + bl] * invN;
+    C* Y = sticks_to_yspace<T>(st + ((size_t)bl * g.n3 + z) * g.nstick, pos, mpos, g, Y0, Y1, tw2);
+    for (int q0 = 0; q0 < npair; q0 += g.rp) {
+      const int nq = min(g.rp, npair - q0);
+      C* Zr = x_c2r_rows<T>(Y, g.kxmax, g.n1, g.n2, q0, nq, Z0, Z1, tw1);
+      for (int t = threadIdx.x; t < nq * g.n1; t += blockDim.x) {
+        const int ql = t / g.n1, x = t - ql * g.n1;
+        const int y1 = 2 * (q0 + ql), y2 = y1 + 1;
+        const C u = Zr[ql * g.n1 + x];
+        acc[(size_t)y1 * g.n1 + x] += fb * (double)u.x * (double)u.x;
+        if (y2 < g.n2) acc[(size_t)y2 * g.n1 + x] += fb * (double)u.y * (double)u.y;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__global__ void k_sum_partials(const double* __restrict__ part, int nsplit, long n, double* __restrict__ out,
+                               int accumulate) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    double s = accumulate ? out[i] : 0.0;
+    for (int q = 0; q < nsplit; q++) s += part[(size_t)q * n + i];
+    out[i] = s;
+  }
+}
+
+// F[a][d] = -2 sum_i f_i sum_{p in a} w_p W[p,i] D_d[p,i]; one block per (a, d)
+__global__ void k_force_reduce(const double* __restrict__ W, const double* __restrict__ D, const double* __restrict__ w,
+                               const double* __restrict__ f, int nbeta, int nb, int npa, double* __restrict__ F) {
+  const int a = blockIdx.x, d = blockIdx.y;
+  const double* Dd = D + (size_t)d * nbeta * nb;
+  double s = 0.0;
+  for (int t = threadIdx.x; t < npa * nb; t += blockDim.x) {
+    const int q = t % npa, i = t / npa;
+    const int p = a * npa + q;
+    s += f[i] * w[p] * W[p + (size_t)nbeta * i] * Dd[p + (size_t)nbeta * i];
+  }
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) F[3 * a + d] = -2.0 * red[0];
+}
+
+// E = sum_i f_i Re<x_i| h_i>: per-band dots reduced on the host in FP64
+__global__ void k_band_dots(const double* __restrict__ X, const double* __restrict__ HX, long m, double* __restrict__ out) {
+  const int j = blockIdx.x;
+  const double* x = X + (size_t)j * m;
+  const double* h = HX + (size_t)j * m;
+  double s = 0.0;
+  for (long i = threadIdx.x; i < m; i += blockDim.x) s += x[i] * h[i];
+  __shared__ double red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = red[0];
+}
+
+void exx_free(pw_ctx* c) {
+  ExxState* E = c->exx;
+  if (!E) return;
+  void* p[] = {E->d_pos, E->d_K64, E->d_K32, E->d_phi64, E->d_phi32, E->d_fac64, E->d_fac32,
+               E->d_psir, E->d_acc, E->d_fst};
+  for (void* q : p)
+    if (q) (void)hipFree(q);
+  delete E;
+  c->exx = nullptr;
+}
+
+template <typename X>
+static int up(X** d, const X* h, size_t n) {
+  PW_HIP(hipMalloc((void**)d, sizeof(X) * (n ? n : 1)));
+  if (n) PW_HIP(hipMemcpy(*d, h, sizeof(X) * n, hipMemcpyHostToDevice));
+  return PW_OK;
+}
+
+}  // namespace pw
+
+using namespace pw;
+
+extern "C" int pw_exx_divergence(const int grid[3], const double supercell[9], double ecut, double mu,
+                                 int divtype, double* div) {
+  if (!grid || !supercell || !div) return fail(PW_ERR_ARG, "pw_exx_divergence: null argument");
+  const int n1 = grid[0], n2 = grid[1], n3 = grid[2];
+  double b[9];
+  recip_cell(supercell, b);
+  const double epsDiv = 1e-8, alpha = 10.0 / (ecut * 2.0);
+  double d = 0.0;
+  if (divtype == 1) {
+    for (int k = 0; k < n3; k++)
+      for (int j = 0; j < n2; j++)
+        for (int i = 0; i < n1; i++) {
+          const int m0 = kgrid(i, n1), m1 = kgrid(j, n2), m2 = kgrid(k, n3);
+          double g[3];
+          for (int q = 0; q < 3; q++) g[q] = b[q] * m0 + b[3 + q] * m1 + b[6 + q] * m2;
+          const double g2 = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+          if (g2 > epsDiv && g2 < ecut * 4.0)
+            d += mu > 0.0 ? std::exp(-alpha * g2) / g2 * (1.0 - std::exp(-g2 / (4.0 * mu * mu)))
+                          : std::exp(-alpha * g2) / g2;
+        }
+    d += mu > 0.0 ? 1.0 / (4.0 * mu * mu) : -alpha;
+    d *= 4.0 * M_PI;
+    const double aa = mu > 0.0 ? 1.0 / std::sqrt(alpha * M_PI) - 1.0 / std::sqrt(M_PI * (alpha + 1.0 / (4.0 * mu * mu)))
+                               : 1.0 / std::sqrt(alpha * M_PI);
+    d -= cell_volume(supercell) * aa;
+  } else if (divtype != 0) {
+    return fail(PW_ERR_UNSUPPORTED, "EXX divergence type %d", divtype);
+  }
+  *div = d;
+  return PW_OK;
+}
+
+extern "C" int pw_set_exx(pw_ctx* c, double ecut, double mu, double exx_fraction, int divtype, int nocc,
+                          const void* psi_occ, const double* occ) {
+  if (!c || nocc < 0 || (nocc > 0 && (!psi_occ || !occ))) return fail(PW_ERR_ARG, "pw_set_exx: bad argument");
+  if (nocc > c->max_bands) return fail(PW_ERR_ARG, "pw_set_exx: nocc > max_bands");
+  exx_free(c);
+  if (nocc == 0) return PW_OK;
+  ExxState* E = new ExxState();
+  c->exx = E;
+  E->ecut = ecut; E->mu = mu; E->frac = exx_fraction; E->divtype = divtype; E->nocc = nocc;
+  E->occ.assign(occ, occ + nocc);
+  PW_TRY(pw_exx_divergence(&c->n1, c->supercell, ecut, mu, divtype, &E->div));
+  // Fock sphere and its sticks; kernel K(G)/N (InitializeEXX folded with the
+  // vol/N and 1/vol factors of FFTWExecute)
+  const int n1 = c->n1, n2 = c->n2, n3 = c->n3, h1 = c->h1;
+  int nf = 0;
+  PW_TRY(pw_fock_sphere_build(&c->n1, c->supercell, ecut, &nf, nullptr, nullptr));
+  std::vector<int> fidx(nf);
+  std::vector<double> fg(nf);
+  PW_TRY(pw_fock_sphere_build(&c->n1, c->supercell, ecut, &nf, fidx.data(), fg.data()));
+  E->nfock = nf;
+  std::vector<int> sid((size_t)h1 * n2, -1);
+  std::vector<int> pos;
+  int kxmax = 0;
+  for (int i = 0; i < nf; i++) sid[fidx[i] % (h1 * n2)] = 1;
+  for (int ky = 0; ky < n2; ky++)
+    for (int kx = 0; kx < h1; kx++)
+      if (sid[kx + (size_t)h1 * ky] > 0) {
+        sid[kx + (size_t)h1 * ky] = (int)pos.size();
+        pos.push_back(kx * n2 + ky);
+        kxmax = std::max(kxmax, kx);
+      }
+  E->nstick = (int)pos.size();
+  E->kxmax = kxmax;
+  E->nchunk = (E->nstick + E->sc - 1) / E->sc;
+  std::vector<double> K((size_t)E->nstick * n3, 0.0);
+  const double inv = 1.0 / (double)c->N;
+  for (int i = 0; i < nf; i++) {
+    const double g2 = fg[i] * 2.0;
+    double kv;
+    if (g2 > 1e-8) kv = mu > 0 ? 4.0 * M_PI / g2 * (1.0 - std::exp(-g2 / (4.0 * mu * mu))) : 4.0 * M_PI / g2;
+    else kv = -E->div + (mu > 0 ? M_PI / (mu * mu) : 0.0);
+    const int col = fidx[i] % (h1 * n2), kz = fidx[i] / (h1 * n2);
+    K[(size_t)sid[col] * n3 + kz] = kv * inv;
+  }
+  std::vector<float> Kf(K.begin(), K.end());
+  PW_TRY(up(&E->d_pos, pos.data(), pos.size()));
+  PW_TRY(up(&E->d_K64, K.data(), K.size()));
+  PW_TRY(up(&E->d_K32, Kf.data(), Kf.size()));
+  std::vector<double> fac(nocc);
+  for (int j = 0; j < nocc; j++) fac[j] = -exx_fraction * occ[j];
+  std::vector<float> facf(fac.begin(), fac.end());
+  PW_TRY(up(&E->d_fac64, fac.data(), fac.size()));
+  PW_TRY(up(&E->d_fac32, facf.data(), facf.size()));
+  const size_t N = (size_t)c->N;
+  PW_HIP(hipMalloc(&E->d_psir, sizeof(double) * N * c->max_bands));
+  PW_HIP(hipMalloc(&E->d_acc, sizeof(double) * N * c->max_bands));
+  PW_HIP(hipMalloc((void**)&E->d_phi64, sizeof(double) * N * nocc));
+  PW_HIP(hipMalloc((void**)&E->d_phi32, sizeof(float) * N * nocc));
+  // pair stick buffer: up to ~1 GiB, at least one pair per band
+  const size_t per_pair = sizeof(double) * 2 * (size_t)n3 * E->nstick;
+  size_t npm = std::max<size_t>((size_t)c->max_bands, (size_t)(1ull << 30) / per_pair);
+  npm = std::min<size_t>(npm, (size_t)c->max_bands * nocc);
+  E->npair_max = (int)npm;
+  PW_HIP(hipMalloc(&E->d_fst, per_pair * npm));
+  // phi_j(r) = C2R(psi_occ_j * sqrt(1/(vol N))) in both precisions
+  const double fphi = std::sqrt(1.0 / c->vol / (double)c->N);
+  PW_TRY(sphere_to_real<double>(c, nocc, psi_occ, E->d_phi64, fphi));
+  {
+    float* tmp = E->d_phi32;
+    const long n = (long)N * nocc;
+    // FP32 copy of the FP64 real-space phi
+    std::vector<double> h(0);
+    (void)h;
+    PW_HIP(hipStreamSynchronize(c->stream));
+    std::vector<double> host((size_t)n);
+    PW_HIP(hipMemcpy(host.data(), E->d_phi64, sizeof(double) * n, hipMemcpyDeviceToHost));
+    std::vector<float> hf(host.begin(), host.end());
+    PW_HIP(hipMemcpy(tmp, hf.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+  }
+  return PW_OK;
+}
+
+extern "C" int pw_add_exx(pw_ctx* c, int dtype, int nb, const void* psi, void* a3) {
+  if (!c) return fail(PW_ERR_ARG, "null context");
+  if (!c->exx) return fail(PW_ERR_STATE, "pw_add_exx: no exchange operator set (pw_set_exx)");
+  if (nb < 0 || nb > c->max_bands) return fail(PW_ERR_ARG, "nb out of range");
+  if (nb == 0) return PW_OK;
+  if (!psi || !a3 || psi == a3) return fail(PW_ERR_ARG, "pw_add_exx: bad block pointers");
+  if (dtype == PW_F64) return exx_add<double>(c, nb, psi, a3);
+  if (dtype == PW_F32) return exx_add<float>(c, nb, psi, a3);
+  return fail(PW_ERR_ARG, "unknown dtype %d", dtype);
+}
+
+// ------------------------------------------------------------ adjoint entries
+
+namespace pw {
+
+template <typename T>
+static int density_impl(pw_ctx* c, int nb, const void* psi, const double* f, double* rho, int accumulate) {
+  typedef typename cx<T>::t C;
+  const long N = c->N;
+  const int nsplit = 16;
+  double *df = nullptr, *part = nullptr;
+  auto done = [&](int rc) {
+    if (df) (void)hipFree(df);
+    if (part) (void)hipFree(part);
+    return rc;
+  };
+  if (hipMalloc((void**)&df, sizeof(double) * nb) != hipSuccess ||
+      hipMalloc((void**)&part, sizeof(double) * N * nsplit) != hipSuccess)
+    return done(fail(PW_ERR_NOMEM, "pw_density: scratch allocation failed"));
+  if (hipMemcpy(df, f, sizeof(double) * nb, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemsetAsync(part, 0, sizeof(double) * N * nsplit, c->stream) != hipSuccess)
+    return done(fail(PW_ERR_HIP, "pw_density: copy failed"));
+  int rc = sphere_to_sticks<T>(c, nb, psi, c->d_t1);
+  if (rc) return done(rc);
+  const C* tw = (const C*)(sizeof(T) == 8 ? (const void*)c->d_tw64 : (const void*)c->d_tw32);
+  SlabGeom g{c->n1, c->n2, c->n3, c->sp.nstick, c->sp.kxmax, 0};
+  g.rp = pick_rp(c, sizeof(C), g.kxmax);
+  const size_t lds = slab_bytes(c, sizeof(C), g.kxmax, g.rp);
+  if ((rc = set_lds(k_density_acc<T>, lds)) != PW_OK) return done(rc);
+  hipLaunchKernelGGL(k_density_acc<T>, dim3(c->n3, nsplit), dim3(256), lds, c->stream, (const C*)c->d_t1,
+                     (const double*)df, 0, nb, nsplit, part, c->d_pos, c->d_mpos, g, 1.0 / (double)N, tw,
+                     tw + c->n1);
+  hipLaunchKernelGGL(k_sum_partials, dim3((int)std::min<long>((N + 255) / 256, 4096)), dim3(256), 0, c->stream,
+                     (const double*)part, nsplit, N, rho, accumulate);
+  if (hipGetLastError() != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+    return done(fail(PW_ERR_HIP, "pw_density: kernel failed"));
+  return done(PW_OK);
+}
+
+__global__ void k_make_dbeta(const double2* __restrict__ beta, const double* __restrict__ G, int npw, int nbeta,
+                             double2* __restrict__ out) {
+  const long total = (long)npw * nbeta;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(e % npw);
+    const double2 b = beta[e];
+    for (int d = 0; d < 3; d++) {
+      const double gd = G[3 * i + d];
+      out[(size_t)d * total + e] = double2{gd * b.y, -gd * b.x};  // -i G_d beta
+    }
+  }
+}
+
+}  // namespace pw
+
+extern "C" int pw_density(pw_ctx* c, int dtype, int nb, const void* psi, const double* f, double* rho,
+                          int accumulate) {
+  if (!c || !psi || !f || !rho || nb < 1 || nb > c->max_bands) return fail(PW_ERR_ARG, "pw_density: bad argument");
+  if (dtype == PW_F64) return density_impl<double>(c, nb, psi, f, rho, accumulate);
+  if (dtype == PW_F32) return density_impl<float>(c, nb, psi, f, rho, accumulate);
+  return fail(PW_ERR_ARG, "unknown dtype %d", dtype);
+}
+
+extern "C" int pw_band_energy(pw_ctx* c, int nb, const void* X, const double* f, double* E) {
+  if (!c || !X || !f || !E || nb < 1 || nb > c->max_bands) return fail(PW_ERR_ARG, "pw_band_energy: bad argument");
+  const long m = 2L * c->npw;
+  double *HX = nullptr, *dots = nullptr;
+  if (hipMalloc((void**)&HX, sizeof(double) * m * nb) != hipSuccess ||
+      hipMalloc((void**)&dots, sizeof(double) * nb) != hipSuccess) {
+    if (HX) (void)hipFree(HX);
+    return fail(PW_ERR_NOMEM, "pw_band_energy: scratch allocation failed");
+  }
+  int rc = pw_apply_h(c, PW_F64, nb, X, HX);
+  std::vector<double> h(nb);
+  if (rc == PW_OK) {
+    hipLaunchKernelGGL(k_band_dots, dim3(nb), dim3(256), 0, c->stream, (const double*)X, (const double*)HX, m, dots);
+    if (hipMemcpyAsync(h.data(), dots, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+      rc = fail(PW_ERR_HIP, "pw_band_energy: reduction failed");
+  }
+  (void)hipFree(HX);
+  (void)hipFree(dots);
+  if (rc != PW_OK) return rc;
+  double e = 0.0;
+  for (int i = 0; i < nb; i++) e += f[i] * h[i];
+  *E = e;
+  return PW_OK;
+}
+
+extern "C" int pw_nonlocal_force(pw_ctx* c, int nb, const void* X, const double* f, int natom, int npa,
+                                 double* force) {
+  if (!c || !X || !f || !force || nb < 1 || nb > c->max_bands) return fail(PW_ERR_ARG, "pw_nonlocal_force: bad argument");
+  if (natom * npa != c->nbeta) return fail(PW_ERR_ARG, "pw_nonlocal_force: natom*npa = %d != nbeta = %d", natom * npa, c->nbeta);
+  const int npw = c->npw, nbeta = c->nbeta, m = 2 * npw;
+  // G vectors of the sphere
+  std::vector<double> G(3 * (size_t)npw);
+  for (int i = 0; i < npw; i++) {
+    const int g = c->h_idx[i];
+    const int kx = g % c->h1, ky = (g / c->h1) % c->n2, kz = g / (c->h1 * c->n2);
+    const int m0 = kgrid(kx, c->n1), m1 = kgrid(ky, c->n2), m2 = kgrid(kz, c->n3);
+    for (int d = 0; d < 3; d++) G[3 * i + d] = c->recip[d] * m0 + c->recip[3 + d] * m1 + c->recip[6 + d] * m2;
+  }
+  std::vector<double> wh(nbeta);
+  double *dG = nullptr, *dB = nullptr, *dW = nullptr, *dD = nullptr, *dw = nullptr, *dfv = nullptr, *dF = nullptr;
+  auto done = [&](int rc) {
+    void* p[] = {dG, dB, dW, dD, dw, dfv, dF};
+    for (void* q : p)
+      if (q) (void)hipFree(q);
+    return rc;
+  };
+  const size_t blk = (size_t)m * nbeta;
+  if (hipMalloc((void**)&dG, sizeof(double) * 3 * npw) != hipSuccess ||
+      hipMalloc((void**)&dB, sizeof(double) * 3 * blk) != hipSuccess ||
+      hipMalloc((void**)&dW, sizeof(double) * (size_t)nbeta * nb) != hipSuccess ||
+      hipMalloc((void**)&dD, sizeof(double) * 3 * (size_t)nbeta * nb) != hipSuccess ||
+      hipMalloc((void**)&dw, sizeof(double) * nbeta) != hipSuccess ||
+      hipMalloc((void**)&dfv, sizeof(double) * nb) != hipSuccess ||
+      hipMalloc((void**)&dF, sizeof(double) * 3 * natom) != hipSuccess)
+    return done(fail(PW_ERR_NOMEM, "pw_nonlocal_force: scratch allocation failed"));
+  // weights: betaw = beta * w column-wise -> recover w from the FP64 copies on the host side
+  if (hipMemcpy(dG, G.data(), sizeof(double) * 3 * npw, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dw, c->h_wgt.data(), sizeof(double) * nbeta, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dfv, f, sizeof(double) * nb, hipMemcpyHostToDevice) != hipSuccess)
+    return done(fail(PW_ERR_HIP, "pw_nonlocal_force: upload failed"));
+  hipLaunchKernelGGL(k_make_dbeta, dim3(4096), dim3(256), 0, c->stream, (const double2*)c->d_beta,
+                     (const double*)dG, npw, nbeta, (double2*)dB);
+  int rc = gemm_tn<double>(c, m, nbeta, nb, c->d_beta, m, (const double*)X, m, dW, nbeta, nullptr, 0.0);
+  for (int d = 0; d < 3 && rc == PW_OK; d++)
+    rc = gemm_tn<double>(c, m, nbeta, nb, dB + d * blk, m, (const double*)X, m, dD + (size_t)d * nbeta * nb, nbeta,
+                         nullptr, 0.0);
+  if (rc != PW_OK) return done(rc);
+  hipLaunchKernelGGL(k_force_reduce, dim3(natom, 3), dim3(256), 0, c->stream, (const double*)dW, (const double*)dD,
+                     (const double*)dw, (const double*)dfv, nbeta, nb, npa, dF);
+  if (hipMemcpyAsync(force, dF, sizeof(double) * 3 * natom, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+      hipStreamSynchronize(c->stream) != hipSuccess)
+    return done(fail(PW_ERR_HIP, "pw_nonlocal_force: reduction failed"));
+  return done(PW_OK);
+}

@@ -1,0 +1,131 @@
+// Batched Stockham FFT engine in LDS for gfx950 (radix 4, 2, 3, 5), shared
+// by the H*psi and exact-exchange kernels.  Pencils of length n with stride
+// ld live in LDS; one pass reads R values per butterfly, applies directly
+// computed twiddles tw[m] = exp(+2 pi i m / n) and writes the autosorted
+// result to the other buffer.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pw {
+
+template <typename T> struct cx;
+template <> struct cx<double> { typedef double2 t; };
+template <> struct cx<float> { typedef float2 t; };
+
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
+  return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+template <typename C> __device__ __forceinline__ C cmulc(C a, C b) {
+  return C{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+template <typename C, typename T> __device__ __forceinline__ C cscale(C a, T s) { return C{a.x * s, a.y * s}; }
+// multiply by (sg * i)
+template <typename C, typename T> __device__ __forceinline__ C cmul_si(C a, T sg) { return C{-sg * a.y, sg * a.x}; }
+
+// In-LDS radix-R butterfly with sign sg (-1 forward, +1 backward).
+template <typename T, int R>
+__device__ __forceinline__ void dft(typename cx<T>::t* v, T sg) {
+  typedef typename cx<T>::t C;
+  if constexpr (R == 2) {
+    C a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    C t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    C t2 = cadd(v[1], v[3]), t3 = cmul_si(csub(v[1], v[3]), sg);
+    v[0] = cadd(t0, t2); v[1] = cadd(t1, t3); v[2] = csub(t0, t2); v[3] = csub(t1, t3);
+  } else if constexpr (R == 3) {
+    const T h = (T)0.86602540378443864676372317075294;
+    C a = v[0], b = v[1], c = v[2];
+    C bc = cadd(b, c);
+    C t = C{a.x - (T)0.5 * bc.x, a.y - (T)0.5 * bc.y};
+    C u = cmul_si(cscale(csub(b, c), h), sg);
+    v[0] = cadd(a, bc); v[1] = cadd(t, u); v[2] = csub(t, u);
+  } else if constexpr (R == 5) {
+    const T c1 = (T)0.30901699437494742410229341718282;
+    const T c2 = (T)-0.80901699437494742410229341718282;
+    const T s1 = (T)0.95105651629515357211643933337938;
+    const T s2 = (T)0.58778525229247312916870595463907;
+    C a = v[0];
+    C t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]), t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+    C m1 = C{a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y};
+    C m2 = C{a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y};
+    C n1 = cmul_si(C{s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y}, sg);
+    C n2 = cmul_si(C{s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y}, sg);
+    v[0] = cadd(a, cadd(t1, t2)); v[1] = cadd(m1, n1); v[4] = csub(m1, n1);
+    v[2] = cadd(m2, n2); v[3] = csub(m2, n2);
+  }
+}
+
+// One Stockham pass of radix R over P pencils of length n (stride ld) held in
+// LDS: src -> dst.  tw[m] = exp(+2 pi i m / n), conjugated for sg < 0.
+template <typename T, int R>
+__device__ __forceinline__ void stockham_pass(const typename cx<T>::t* __restrict__ src,
+                                              typename cx<T>::t* __restrict__ dst, int n, int P,
+                                              int ld, int Ns, const typename cx<T>::t* tw, T sg) {
+  typedef typename cx<T>::t C;
+  const int m = n / R;
+  const int tstep = n / (Ns * R);
+  const int total = P * m;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int p = t / m;
+    const int j = t - p * m;
+    const int k = j % Ns;
+    const C* s = src + p * ld + j;
+    C v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = s[r * m];
+#pragma unroll
+    for (int r = 1; r < R; r++) {
+      C w = tw[r * k * tstep];  // r*k*tstep < n
+      w.y *= sg;
+      v[r] = cmul(v[r], w);
+    }
+    dft<T, R>(v, sg);
+    C* d = dst + p * ld + (j / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; r++) d[r * Ns] = v[r];
+  }
+}
+
+// Full batched 1D FFT in LDS (ping-pong a <-> b); returns the buffer holding
+// the result.  Radices are taken 4 first, then 2, 3, 5 (the order of
+// radix_plan on the host), derived from n so no table is indexed at run
+// time.  Ends with a barrier.
+template <typename T>
+__device__ typename cx<T>::t* lds_fft(typename cx<T>::t* a, typename cx<T>::t* b, int n, int P,
+                                      int ld, const typename cx<T>::t* tw, T sg) {
+  typedef typename cx<T>::t C;
+  C* src = a;
+  C* dst = b;
+  int Ns = 1;
+  int rem = n;
+  while (rem > 1) {
+    int R;
+    if (rem % 4 == 0) {
+      R = 4;
+      stockham_pass<T, 4>(src, dst, n, P, ld, Ns, tw, sg);
+    } else if (rem % 2 == 0) {
+      R = 2;
+      stockham_pass<T, 2>(src, dst, n, P, ld, Ns, tw, sg);
+    } else if (rem % 3 == 0) {
+      R = 3;
+      stockham_pass<T, 3>(src, dst, n, P, ld, Ns, tw, sg);
+    } else {
+      R = 5;
+      stockham_pass<T, 5>(src, dst, n, P, ld, Ns, tw, sg);
+    }
+    __syncthreads();
+    C* tmp = src;
+    src = dst;
+    dst = tmp;
+    Ns *= R;
+    rem /= R;
+  }
+  return src;
+}
+
+}  // namespace pw

@@ -24,129 +24,10 @@
 #include <cstring>
 #include <vector>
 
+#include "pw_fft.h"
 #include "pw_internal.h"
 
 namespace pw {
-
-template <typename T> struct cx;
-template <> struct cx<double> { typedef double2 t; };
-template <> struct cx<float> { typedef float2 t; };
-
-template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return C{a.x + b.x, a.y + b.y}; }
-template <typename C> __device__ __forceinline__ C csub(C a, C b) { return C{a.x - b.x, a.y - b.y}; }
-template <typename C> __device__ __forceinline__ C cmul(C a, C b) {
-  return C{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
-}
-// a * conj(b)
-template <typename C> __device__ __forceinline__ C cmulc(C a, C b) {
-  return C{a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
-}
-template <typename C, typename T> __device__ __forceinline__ C cscale(C a, T s) { return C{a.x * s, a.y * s}; }
-// multiply by (sg * i)
-template <typename C, typename T> __device__ __forceinline__ C cmul_si(C a, T sg) { return C{-sg * a.y, sg * a.x}; }
-
-// In-LDS radix-R butterfly with sign sg (-1 forward, +1 backward).
-template <typename T, int R>
-__device__ __forceinline__ void dft(typename cx<T>::t* v, T sg) {
-  typedef typename cx<T>::t C;
-  if constexpr (R == 2) {
-    C a = v[0], b = v[1];
-    v[0] = cadd(a, b);
-    v[1] = csub(a, b);
-  } else if constexpr (R == 4) {
-    C t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-    C t2 = cadd(v[1], v[3]), t3 = cmul_si(csub(v[1], v[3]), sg);
-    v[0] = cadd(t0, t2); v[1] = cadd(t1, t3); v[2] = csub(t0, t2); v[3] = csub(t1, t3);
-  } else if constexpr (R == 3) {
-    const T h = (T)0.86602540378443864676372317075294;
-    C a = v[0], b = v[1], c = v[2];
-    C bc = cadd(b, c);
-    C t = C{a.x - (T)0.5 * bc.x, a.y - (T)0.5 * bc.y};
-    C u = cmul_si(cscale(csub(b, c), h), sg);
-    v[0] = cadd(a, bc); v[1] = cadd(t, u); v[2] = csub(t, u);
-  } else if constexpr (R == 5) {
-    const T c1 = (T)0.30901699437494742410229341718282;
-    const T c2 = (T)-0.80901699437494742410229341718282;
-    const T s1 = (T)0.95105651629515357211643933337938;
-    const T s2 = (T)0.58778525229247312916870595463907;
-    C a = v[0];
-    C t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]), t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
-    C m1 = C{a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y};
-    C m2 = C{a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y};
-    C n1 = cmul_si(C{s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y}, sg);
-    C n2 = cmul_si(C{s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y}, sg);
-    v[0] = cadd(a, cadd(t1, t2)); v[1] = cadd(m1, n1); v[4] = csub(m1, n1);
-    v[2] = cadd(m2, n2); v[3] = csub(m2, n2);
-  }
-}
-
-// One Stockham pass of radix R over P pencils of length n (stride ld) held in
-// LDS: src -> dst.  tw[m] = exp(+2 pi i m / n), conjugated for sg < 0.
-template <typename T, int R>
-__device__ __forceinline__ void stockham_pass(const typename cx<T>::t* __restrict__ src,
-                                              typename cx<T>::t* __restrict__ dst, int n, int P,
-                                              int ld, int Ns, const typename cx<T>::t* tw, T sg) {
-  typedef typename cx<T>::t C;
-  const int m = n / R;
-  const int tstep = n / (Ns * R);
-  const int total = P * m;
-  for (int t = threadIdx.x; t < total; t += blockDim.x) {
-    const int p = t / m;
-    const int j = t - p * m;
-    const int k = j % Ns;
-    const C* s = src + p * ld + j;
-    C v[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) v[r] = s[r * m];
-#pragma unroll
-    for (int r = 1; r < R; r++) {
-      C w = tw[r * k * tstep];  // r*k*tstep < n
-      w.y *= sg;
-      v[r] = cmul(v[r], w);
-    }
-    dft<T, R>(v, sg);
-    C* d = dst + p * ld + (j / Ns) * Ns * R + k;
-#pragma unroll
-    for (int r = 0; r < R; r++) d[r * Ns] = v[r];
-  }
-}
-
-// Full batched 1D FFT in LDS (ping-pong a <-> b); returns the buffer holding
-// the result.  Radices are taken 4 first, then 2, 3, 5 (the order of
-// radix_plan on the host), derived from n so no table is indexed at run
-// time.  Ends with a barrier.
-template <typename T>
-__device__ typename cx<T>::t* lds_fft(typename cx<T>::t* a, typename cx<T>::t* b, int n, int P,
-                                      int ld, const typename cx<T>::t* tw, T sg) {
-  typedef typename cx<T>::t C;
-  C* src = a;
-  C* dst = b;
-  int Ns = 1;
-  int rem = n;
-  while (rem > 1) {
-    int R;
-    if (rem % 4 == 0) {
-      R = 4;
-      stockham_pass<T, 4>(src, dst, n, P, ld, Ns, tw, sg);
-    } else if (rem % 2 == 0) {
-      R = 2;
-      stockham_pass<T, 2>(src, dst, n, P, ld, Ns, tw, sg);
-    } else if (rem % 3 == 0) {
-      R = 3;
-      stockham_pass<T, 3>(src, dst, n, P, ld, Ns, tw, sg);
-    } else {
-      R = 5;
-      stockham_pass<T, 5>(src, dst, n, P, ld, Ns, tw, sg);
-    }
-    __syncthreads();
-    C* tmp = src;
-    src = dst;
-    dst = tmp;
-    Ns *= R;
-    rem /= R;
-  }
-  return src;
-}
 
 // ---------------------------------------------------------------- kernel 1
 template <typename T>
@@ -307,9 +188,14 @@ k_sticks_forward_z(const typename cx<T>::t* __restrict__ t2, const typename cx<T
     const int sz = gpt_sz[t];
     C y = R[(sz & 0xffff) * n3 + (sz >> 16)];
     if (i != 0) y = cscale(y, s2);
-    const C cc = ps[i];
-    const T ek = ekin[i];
-    o[i] = C{ek * cc.x + y.x, ek * cc.y + y.y};
+    if (ekin) {
+      const C cc = ps[i];
+      const T ek = ekin[i];
+      o[i] = C{ek * cc.x + y.x, ek * cc.y + y.y};
+    } else {  // accumulate: a3 += gather (spinor.cpp:933-935)
+      const C a = o[i];
+      o[i] = C{a.x + y.x, a.y + y.y};
+    }
   }
 }
 
@@ -462,6 +348,7 @@ extern "C" int pw_ctx_create(pw_ctx** out, const int grid[3], const double super
     }
   }
   build_stick_plan(c, idx);
+  c->h_idx.assign(idx, idx + npw);
   c->f1 = radix_plan(c->n1);
   c->f2 = radix_plan(c->n2);
   c->f3 = radix_plan(c->n3);
@@ -512,6 +399,7 @@ extern "C" int pw_ctx_create(pw_ctx** out, const int grid[3], const double super
 extern "C" int pw_ctx_destroy(pw_ctx* c) {
   if (!c) return PW_OK;
   pw_free_workspace(c);
+  exx_free(c);
   if (c->d_gemm_ws) (void)hipFree(c->d_gemm_ws);
   void* ptrs[] = {c->d_pt_idx, c->d_pt_sz, c->d_chunk_off, c->d_gpt_idx, c->d_gpt_sz, c->d_gchunk_off,
                   c->d_pos, c->d_mpos, c->d_ekin, c->d_teter, c->d_ekin_f, c->d_teter_f, c->d_tw64,
@@ -565,6 +453,7 @@ extern "C" int pw_set_nonlocal(pw_ctx* c, int nbeta, const double* beta, const d
   c->d_beta = c->d_betaw = c->d_W = nullptr;
   c->d_beta_f = c->d_betaw_f = c->d_W_f = nullptr;
   c->nbeta = nbeta;
+  c->h_wgt.assign(w ? w : (const double*)nullptr, w ? w + nbeta : (const double*)nullptr);
   if (nbeta == 0) return PW_OK;
   const size_t m = 2 * (size_t)c->npw;
   std::vector<double> bw(beta, beta + m * nbeta);
@@ -665,6 +554,37 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
   return PW_OK;
 }
 
+// Sphere coefficients -> z-transformed sticks T[band][z][stick] (kernel 1).
+template <typename T>
+int sphere_to_sticks(pw_ctx* c, int nb, const void* psi, void* t1) {
+  typedef typename cx<T>::t C;
+  const C* tw3 = (const C*)(sizeof(T) == 8 ? (const void*)c->d_tw64 : (const void*)c->d_tw32) + c->n1 + c->n2;
+  const size_t lds1 = sizeof(C) * (c->n3 + 2 * (size_t)c->sp.sc * c->n3);
+  hipLaunchKernelGGL(k_sticks_inverse_z<T>, dim3(c->sp.nchunk, nb), dim3(256), lds1, c->stream,
+                     (const C*)psi, c->npw, (C*)t1, c->n3, c->sp.nstick, c->sp.sc, c->d_chunk_off, c->d_pt_idx,
+                     c->d_pt_sz, tw3);
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+// Sticks -> forward z transform -> out += sqrt(2)-scaled sphere gather.
+template <typename T>
+int sticks_to_sphere_add(pw_ctx* c, int nb, const void* t2, void* out) {
+  typedef typename cx<T>::t C;
+  const C* tw3 = (const C*)(sizeof(T) == 8 ? (const void*)c->d_tw64 : (const void*)c->d_tw32) + c->n1 + c->n2;
+  const size_t lds1 = sizeof(C) * (c->n3 + 2 * (size_t)c->sp.sc * c->n3);
+  hipLaunchKernelGGL(k_sticks_forward_z<T>, dim3(c->sp.nchunk, nb), dim3(256), lds1, c->stream, (const C*)t2,
+                     (const C*)nullptr, (C*)out, c->npw, c->n3, c->sp.nstick, c->sp.sc, c->d_gchunk_off,
+                     c->d_gpt_idx, c->d_gpt_sz, (const T*)nullptr, tw3);
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+template int sphere_to_sticks<double>(pw_ctx*, int, const void*, void*);
+template int sphere_to_sticks<float>(pw_ctx*, int, const void*, void*);
+template int sticks_to_sphere_add<double>(pw_ctx*, int, const void*, void*);
+template int sticks_to_sphere_add<float>(pw_ctx*, int, const void*, void*);
+
 // W = Re(beta^H X) on the real (2 npw) view, then Y (+)= (beta .* w) W.
 template <typename T>
 static int nonlocal(pw_ctx* c, int nb, const void* X, void* Y, bool accumulate) {
@@ -697,7 +617,9 @@ extern "C" int pw_mult_spinor(pw_ctx* c, int dtype, int nb, const void* psi, voi
   PW_TRY(check_call(c, dtype, nb, psi, a3));
   if (nb == 0) return PW_OK;
   if (psi == a3) return fail(PW_ERR_ARG, "pw_mult_spinor: psi and a3 must not alias");
-  const int rc = dtype == PW_F64 ? local_kinetic<double>(c, nb, psi, a3) : local_kinetic<float>(c, nb, psi, a3);
+  int rc = dtype == PW_F64 ? local_kinetic<double>(c, nb, psi, a3) : local_kinetic<float>(c, nb, psi, a3);
+  // hybrid functional with EXX active: + V_x psi (hamiltonian.cpp:3431-3448)
+  if (rc == PW_OK && c->exx) rc = dtype == PW_F64 ? exx_add<double>(c, nb, psi, a3) : exx_add<float>(c, nb, psi, a3);
   if (rc == PW_OK) PW_TRY(mark(c, 4));
   return rc;
 }
@@ -716,6 +638,7 @@ extern "C" int pw_apply_h(pw_ctx* c, int dtype, int nb, const void* X, void* HX)
   if (nb == 0) return PW_OK;
   if (X == HX) return fail(PW_ERR_ARG, "pw_apply_h: X and HX must not alias");
   int rc = dtype == PW_F64 ? local_kinetic<double>(c, nb, X, HX) : local_kinetic<float>(c, nb, X, HX);
+  if (rc == PW_OK && c->exx) rc = dtype == PW_F64 ? exx_add<double>(c, nb, X, HX) : exx_add<float>(c, nb, X, HX);
   if (rc != PW_OK) return rc;
   rc = dtype == PW_F64 ? nonlocal<double>(c, nb, X, HX, true) : nonlocal<float>(c, nb, X, HX, true);
   if (rc != PW_OK) return rc;

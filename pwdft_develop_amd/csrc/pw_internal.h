@@ -86,9 +86,21 @@ int gemm_tn(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, i
 template <typename T>
 int gemm_nn(pw_ctx* c, int m, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T beta, T* Y,
             int ldy);
+// Stick transforms of the wavefunction sphere (pw_hpsi.hip).
+template <typename T>
+int sphere_to_sticks(pw_ctx* c, int nb, const void* psi, void* t1);
+template <typename T>
+int sticks_to_sphere_add(pw_ctx* c, int nb, const void* t2, void* out);
+// Exact exchange (pw_exx.hip): a3 += V_x psi when the context holds an
+// exchange operator; state freed with the context.
+struct ExxState;
+template <typename T>
+int exx_add(pw_ctx* c, int nb, const void* psi, void* a3);
+void exx_free(pw_ctx* c);
 }  // namespace pw
 
 struct pw_ctx {
+  pw::ExxState* exx = nullptr;  // screened exchange operator, when set
   void* d_gemm_ws = nullptr;  // split-K partial tiles
   size_t gemm_ws_bytes = 0;
   int device = 0;
@@ -100,6 +112,8 @@ struct pw_ctx {
   double supercell[9], recip[9];
   int npw, npad, max_bands;
   pw::StickPlan sp;
+  std::vector<int> h_idx;       // idxCoarseCut, host copy
+  std::vector<double> h_wgt;    // projector weights, host copy
   std::vector<int> f1, f2, f3;  // radix plans
   // device arrays
   int *d_pt_idx = nullptr, *d_pt_sz = nullptr, *d_chunk_off = nullptr;

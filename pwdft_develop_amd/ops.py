@@ -122,6 +122,63 @@ class Context:
                                           e, r, bounds))
         return X, e, r, bounds
 
+    def lobpcg(self, X, num_eig=None, scf_iter=1, max_iter=3, tol=1e-20):
+        nb = X.shape[0]
+        e, r = np.zeros(nb), np.zeros(nb)
+        it = C.c_int(0)
+        check(_lib.lib().pw_lobpcg(self.h, nb, num_eig or nb, scf_iter, max_iter, 0.0, tol,
+                                   C.c_void_p(X.data_ptr()), e, r, C.byref(it)))
+        return X, e, r, it.value
+
+    def set_exx(self, ecut, mu, frac, psi_occ, occ, divtype=1):
+        """psi_occ: device FP64 block (nocc, npw)."""
+        n = 0 if psi_occ is None else psi_occ.shape[0]
+        ptr = C.c_void_p(psi_occ.data_ptr()) if n else None
+        check(_lib.lib().pw_set_exx(self.h, ecut, mu, frac, divtype, n, ptr,
+                                    np.ascontiguousarray(occ if n else np.zeros(1), np.float64)))
+
+    def add_exx(self, x, out=None):
+        return self._binop("pw_add_exx", x, out, zero_out=True)
+
+    # ---- inputs that the adjoint differentiates ----
+    def set_vtot(self, vtot):
+        v = np.ascontiguousarray(np.asarray(vtot, dtype=np.float64).reshape(-1))
+        check(_lib.lib().pw_set_vtot(self.h, v))
+
+    def set_atoms(self, pos):
+        """Rebuild the Gaussian projector block at new atom positions (same
+        recipe and couplings as the problem) and load it."""
+        p = self.prob
+        pos = np.ascontiguousarray(np.asarray(pos, dtype=np.float64).reshape(-1, 3))
+        beta = np.zeros(2 * p.npw * p.nbeta)
+        w = np.zeros(p.nbeta)
+        check(_lib.lib().pw_gaussian_projectors(
+            np.asarray(p.grid, np.int32), np.ascontiguousarray(p.supercell, np.float64).reshape(-1), p.npw, p.idx,
+            len(pos), pos.reshape(-1), p.proj_sigma, p.lmax, np.ascontiguousarray(p.wgt_channel.reshape(-1)),
+            beta, w))
+        check(_lib.lib().pw_set_nonlocal(self.h, p.nbeta, beta, w))
+
+    def density(self, x, f, out=None, accumulate=False):
+        t = self.torch
+        if out is None:
+            out = t.zeros(int(np.prod(self.prob.grid)), dtype=t.float64, device=self.device)
+        check(_lib.lib().pw_density(self.h, _dtype_code(x), x.shape[0], C.c_void_p(x.data_ptr()),
+                                    np.ascontiguousarray(f, np.float64), C.c_void_p(out.data_ptr()),
+                                    1 if accumulate else 0))
+        return out
+
+    def band_energy(self, x, f):
+        e = C.c_double()
+        check(_lib.lib().pw_band_energy(self.h, x.shape[0], C.c_void_p(x.data_ptr()),
+                                        np.ascontiguousarray(f, np.float64), C.byref(e)))
+        return e.value
+
+    def nonlocal_force(self, x, f, natom, npa):
+        F = np.zeros(3 * natom)
+        check(_lib.lib().pw_nonlocal_force(self.h, x.shape[0], C.c_void_p(x.data_ptr()),
+                                           np.ascontiguousarray(f, np.float64), natom, npa, F))
+        return F.reshape(natom, 3)
+
     def set_timing(self, on=True):
         check(_lib.lib().pw_set_timing(self.h, 1 if on else 0))
 
