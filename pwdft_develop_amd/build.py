@@ -63,6 +63,17 @@ def build(force=False, jobs=4):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link of libpwdft_hip.so failed")
     os.replace(tmp, LIB)
+    # C++ adapter driver used by the adapter tests (links the library above)
+    drv_src = os.path.join(ROOT, "tests", "cpp", "adapter_driver.cpp")
+    if os.path.exists(drv_src):
+        drv = os.path.join(ROOT, "tests", "cpp", "adapter_driver")
+        cmd = [HIPCC, "-O2", "-std=c++17", f"-I{ROCM}/include", drv_src, "-o", drv, f"-L{HERE}", "-lpwdft_hip",
+               f"-L{ROCM}/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN/../../pwdft_develop_amd",
+               f"-Wl,-rpath,{ROCM}/lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("adapter driver build failed")
     return LIB
 
 
