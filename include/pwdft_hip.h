@@ -157,6 +157,13 @@ int pw_chebfsi_sweep(pw_ctx* ctx, int mode, int nb, int m, int k,
 
 int pw_free_workspace(pw_ctx* ctx);
 
+/* Graph capture of the Chebyshev filter (default on): the first call of a
+ * (dtype, nb, degree) shape runs eagerly, the second is captured into a
+ * hipGraph, and later calls replay it on a private stream ordered after the
+ * context's stream.  enable = 0 runs every filter eagerly.  No counterpart in
+ * the reference (a launch-overhead device for small cells). */
+int pw_set_graphs(pw_ctx* ctx, int enable);
+
 /* EigenSolver::LOBPCGSolveReal(numEig, scfIter, eigMaxIter, eigMinTolerance,
  * eigTolerance), sphere-cut branch (src/EigenSolver/lobpcg.cpp:59-576), with
  * SolveGenEig's sequential branch (src/EigenSolver/eigensolver.cpp:224-297).

@@ -288,6 +288,11 @@ k_exx_slab_fwd(const T* __restrict__ in, typename cx<T>::t* __restrict__ st, con
   for (int s = threadIdx.x; s < g.nstick; s += blockDim.x) dst[s] = Yf[pos[s]];
 }
 
+template <typename T>
+__global__ void k_fill_zero(T* __restrict__ p, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = (T)0;
+}
+
 static size_t slab_bytes(const pw_ctx* c, int esz, int kxmax, int rp) {
   return (size_t)esz * (c->n1 + c->n2 + 2 * (size_t)(kxmax + 1) * c->n2 + 2 * (size_t)rp * c->n1);
 }
@@ -301,9 +306,7 @@ static int pick_rp(const pw_ctx* c, int esz, int kxmax) {
 template <typename K>
 static int set_lds(K kern, size_t bytes) {
   if (bytes > 160 * 1024) return fail(PW_ERR_UNSUPPORTED, "exchange slab needs %zu bytes of LDS (> 160 KiB)", bytes);
-  if (bytes > 64 * 1024)
-    PW_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-  return PW_OK;
+  return ensure_lds((const void*)kern, bytes);
 }
 
 // psi (sphere, dtype T) -> real grid (scale) for nb bands, via the wavefunction sticks
@@ -335,7 +338,10 @@ int exx_add(pw_ctx* c, int nb, const void* psi, void* a3) {
   const T* K = (const T*)(sizeof(T) == 8 ? (const void*)E->d_K64 : (const void*)E->d_K32);
   const C* tw = (const C*)(sizeof(T) == 8 ? (const void*)c->d_tw64 : (const void*)c->d_tw32);
   PW_TRY(sphere_to_real<T>(c, nb, psi, psir, (T)1));
-  PW_HIP(hipMemsetAsync(acc, 0, sizeof(T) * N * nb, c->stream));
+  // zero fill by a kernel, not a memset node: this runs inside captured
+  // filter graphs
+  hipLaunchKernelGGL(k_fill_zero<T>, dim3((int)std::min<long>((N * nb + 255) / 256, 8192)), dim3(256), 0, c->stream,
+                     acc, N * (long)nb);
   SlabGeom gf{c->n1, c->n2, c->n3, E->nstick, E->kxmax, 0};
   gf.rp = pick_rp(c, sizeof(C), gf.kxmax);
   const size_t ldsf = slab_bytes(c, sizeof(C), gf.kxmax, gf.rp);
@@ -454,6 +460,7 @@ __global__ void k_band_dots(const double* __restrict__ X, const double* __restri
 void exx_free(pw_ctx* c) {
   ExxState* E = c->exx;
   if (!E) return;
+  invalidate_graph(c);
   void* p[] = {E->d_pos, E->d_K64, E->d_K32, E->d_phi64, E->d_phi32, E->d_fac64, E->d_fac32,
                E->d_psir, E->d_acc, E->d_fst};
   for (void* q : p)
@@ -510,6 +517,7 @@ extern "C" int pw_set_exx(pw_ctx* c, double ecut, double mu, double exx_fraction
   if (!c || nocc < 0 || (nocc > 0 && (!psi_occ || !occ))) return fail(PW_ERR_ARG, "pw_set_exx: bad argument");
   if (nocc > c->max_bands) return fail(PW_ERR_ARG, "pw_set_exx: nocc > max_bands");
   exx_free(c);
+  invalidate_graph(c);
   if (nocc == 0) return PW_OK;
   ExxState* E = new ExxState();
   c->exx = E;

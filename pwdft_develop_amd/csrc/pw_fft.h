@@ -128,4 +128,38 @@ __device__ typename cx<T>::t* lds_fft(typename cx<T>::t* a, typename cx<T>::t* b
   return src;
 }
 
+// Compile-time-length version: the radix sequence (4s, then 2, 3s, 5s) is
+// unrolled at compile time, so every stride, twiddle step and index division
+// inside the passes is a constant.
+template <int REM>
+struct first_radix {
+  static constexpr int value = (REM % 4 == 0) ? 4 : (REM % 2 == 0) ? 2 : (REM % 3 == 0) ? 3 : 5;
+};
+
+template <typename T, int N, int NS, int REM>
+__device__ __forceinline__ typename cx<T>::t* lds_fft_ct_rec(typename cx<T>::t* src, typename cx<T>::t* dst, int P,
+                                                              int ld, const typename cx<T>::t* tw, T sg) {
+  if constexpr (REM <= 1) {
+    (void)dst; (void)P; (void)ld; (void)tw; (void)sg;
+    return src;
+  } else {
+    constexpr int R = first_radix<REM>::value;
+    stockham_pass<T, R>(src, dst, N, P, ld, NS, tw, sg);
+    __syncthreads();
+    return lds_fft_ct_rec<T, N, NS * R, REM / R>(dst, src, P, ld, tw, sg);
+  }
+}
+
+// N > 0: compile-time length; N == 0: the runtime engine with length n.
+template <typename T, int N>
+__device__ __forceinline__ typename cx<T>::t* lds_fft_n(typename cx<T>::t* a, typename cx<T>::t* b, int n, int P,
+                                                         int ld, const typename cx<T>::t* tw, T sg) {
+  if constexpr (N > 0) {
+    (void)n;
+    return lds_fft_ct_rec<T, N, 1, N>(a, b, P, ld, tw, sg);
+  } else {
+    return lds_fft<T>(a, b, n, P, ld, tw, sg);
+  }
+}
+
 }  // namespace pw

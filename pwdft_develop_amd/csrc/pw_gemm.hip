@@ -185,6 +185,7 @@ int gemm_tn(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, i
   nsplit = (K + kc - 1) / kc;
   const size_t need = sizeof(T) * (size_t)nsplit * n1 * n2;
   if (need > c->gemm_ws_bytes) {
+    invalidate_graph(c);
     if (c->d_gemm_ws) (void)hipFree(c->d_gemm_ws);
     c->d_gemm_ws = nullptr;
     c->gemm_ws_bytes = 0;

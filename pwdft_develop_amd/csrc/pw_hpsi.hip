@@ -30,13 +30,14 @@
 namespace pw {
 
 // ---------------------------------------------------------------- kernel 1
-template <typename T>
+template <typename T, int N3>
 __global__ void __launch_bounds__(256)
 k_sticks_inverse_z(const typename cx<T>::t* __restrict__ psi, int ldpsi,
-                   typename cx<T>::t* __restrict__ t1, int n3, int nstick, int sc,
+                   typename cx<T>::t* __restrict__ t1, int n3_, int nstick, int sc,
                    const int* __restrict__ chunk_off, const int* __restrict__ pt_idx,
                    const int* __restrict__ pt_sz, const typename cx<T>::t* __restrict__ tw3g) {
   typedef typename cx<T>::t C;
+  const int n3 = N3 > 0 ? N3 : n3_;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* tw = reinterpret_cast<C*>(smem_raw);
   C* A = tw + n3;
@@ -63,7 +64,7 @@ k_sticks_inverse_z(const typename cx<T>::t* __restrict__ psi, int ldpsi,
     A[(sz & 0xffff) * n3 + (sz >> 16)] = v;
   }
   __syncthreads();
-  C* R = lds_fft<T>(A, B, n3, sc, n3, tw, (T)1);
+  C* R = lds_fft_n<T, N3>(A, B, n3, sc, n3, tw, (T)1);
   const int nsl = min(sc, nstick - s0);
   for (int t = threadIdx.x; t < n3 * nsl; t += blockDim.x) {
     const int z = t / nsl, sl = t - z * nsl;
@@ -72,14 +73,16 @@ k_sticks_inverse_z(const typename cx<T>::t* __restrict__ psi, int ldpsi,
 }
 
 // ---------------------------------------------------------------- kernel 2
-template <typename T>
+template <typename T, int N1, int N2>
 __global__ void __launch_bounds__(256)
-k_slab_xy(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restrict__ t2, int n1,
-          int n2, int n3, int nstick, int kxmax, const int* __restrict__ pos,
+k_slab_xy(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restrict__ t2, int n1_,
+          int n2_, int n3, int nstick, int kxmax, const int* __restrict__ pos,
           const int* __restrict__ mpos, const T* __restrict__ vtot,
           const typename cx<T>::t* __restrict__ tw1g, const typename cx<T>::t* __restrict__ tw2g,
           int rp) {
   typedef typename cx<T>::t C;
+  const int n1 = N1 > 0 ? N1 : n1_;  // compile-time when the grid is one of the dispatched sizes
+  const int n2 = N2 > 0 ? N2 : n2_;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int nk = kxmax + 1;
   C* tw1 = reinterpret_cast<C*>(smem_raw);
@@ -102,7 +105,7 @@ k_slab_xy(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restric
   }
   __syncthreads();
   // inverse y on the kx <= kxmax columns
-  C* Y = lds_fft<T>(Y0, Y1, n2, nk, n2, tw2, (T)1);
+  C* Y = lds_fft_n<T, N2>(Y0, Y1, n2, nk, n2, tw2, (T)1);
   const int npair = (n2 + 1) / 2;
   const T* vz = vtot + (size_t)z * n1 * n2;
   const int h = n1 / 2;
@@ -125,7 +128,7 @@ k_slab_xy(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restric
       Z0[ql * n1 + k] = C{a1.x - a2.y, a1.y + a2.x};
     }
     __syncthreads();
-    C* Zr = lds_fft<T>(Z0, Z1, n1, nq, n1, tw1, (T)1);
+    C* Zr = lds_fft_n<T, N1>(Z0, Z1, n1, nq, n1, tw1, (T)1);
     // real rows -> x V/N (V already carries the 1/N of spinor.cpp:620-621)
     for (int t = threadIdx.x; t < nq * n1; t += blockDim.x) {
       const int ql = t / n1, x = t - ql * n1;
@@ -137,7 +140,7 @@ k_slab_xy(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restric
     }
     __syncthreads();
     C* Zo = (Zr == Z0) ? Z1 : Z0;
-    C* Zf = lds_fft<T>(Zr, Zo, n1, nq, n1, tw1, (T)-1);
+    C* Zf = lds_fft_n<T, N1>(Zr, Zo, n1, nq, n1, tw1, (T)-1);
     // unpack the two real-row spectra for kx <= kxmax back into Y
     for (int t = threadIdx.x; t < nq * nk; t += blockDim.x) {
       const int ql = t / nk, k = t - ql * nk;
@@ -151,20 +154,21 @@ k_slab_xy(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restric
     __syncthreads();
   }
   C* Yo = (Y == Y0) ? Y1 : Y0;
-  C* Yf = lds_fft<T>(Y, Yo, n2, nk, n2, tw2, (T)-1);
+  C* Yf = lds_fft_n<T, N2>(Y, Yo, n2, nk, n2, tw2, (T)-1);
   C* dst = t2 + ((size_t)band * n3 + z) * nstick;
   for (int s = threadIdx.x; s < nstick; s += blockDim.x) dst[s] = Yf[pos[s]];
 }
 
 // ---------------------------------------------------------------- kernel 3
-template <typename T>
+template <typename T, int N3>
 __global__ void __launch_bounds__(256)
 k_sticks_forward_z(const typename cx<T>::t* __restrict__ t2, const typename cx<T>::t* __restrict__ psi,
-                   typename cx<T>::t* __restrict__ out, int ld, int n3, int nstick, int sc,
+                   typename cx<T>::t* __restrict__ out, int ld, int n3_, int nstick, int sc,
                    const int* __restrict__ gchunk_off, const int* __restrict__ gpt_idx,
                    const int* __restrict__ gpt_sz, const T* __restrict__ ekin,
                    const typename cx<T>::t* __restrict__ tw3g) {
   typedef typename cx<T>::t C;
+  const int n3 = N3 > 0 ? N3 : n3_;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* tw = reinterpret_cast<C*>(smem_raw);
   C* A = tw + n3;
@@ -178,7 +182,7 @@ k_sticks_forward_z(const typename cx<T>::t* __restrict__ t2, const typename cx<T
     A[sl * n3 + z] = (sl < nsl) ? t2[((size_t)band * n3 + z) * nstick + s0 + sl] : C{0, 0};
   }
   __syncthreads();
-  C* R = lds_fft<T>(A, B, n3, sc, n3, tw, (T)-1);
+  C* R = lds_fft_n<T, N3>(A, B, n3, sc, n3, tw, (T)-1);
   const C* ps = psi + (size_t)band * ld;
   C* o = out + (size_t)band * ld;
   const T s2 = (T)1.4142135623730950488016887242097;
@@ -196,6 +200,52 @@ k_sticks_forward_z(const typename cx<T>::t* __restrict__ t2, const typename cx<T
       const C a = o[i];
       o[i] = C{a.x + y.x, a.y + y.y};
     }
+  }
+}
+
+// Grid sizes whose FFT lengths are compiled in (all lengths of the built
+// workloads); other 2/3/5-smooth sizes use the runtime-length engine.
+#define PW_FFT_SIZES(X) X(24) X(30) X(32) X(36) X(40) X(48) X(60) X(72) X(96) X(180)
+
+template <typename T>
+auto slab_kernel(int n1, int n2) -> decltype(&k_slab_xy<T, 0, 0>) {
+  if (n1 == n2) {
+    switch (n1) {
+#define PW_CASE(n) \
+  case n:          \
+    return &k_slab_xy<T, n, n>;
+      PW_FFT_SIZES(PW_CASE)
+#undef PW_CASE
+      default:
+        break;
+    }
+  }
+  return &k_slab_xy<T, 0, 0>;
+}
+
+template <typename T>
+auto inv_z_kernel(int n3) -> decltype(&k_sticks_inverse_z<T, 0>) {
+  switch (n3) {
+#define PW_CASE(n) \
+  case n:          \
+    return &k_sticks_inverse_z<T, n>;
+    PW_FFT_SIZES(PW_CASE)
+#undef PW_CASE
+    default:
+      return &k_sticks_inverse_z<T, 0>;
+  }
+}
+
+template <typename T>
+auto fwd_z_kernel(int n3) -> decltype(&k_sticks_forward_z<T, 0>) {
+  switch (n3) {
+#define PW_CASE(n) \
+  case n:          \
+    return &k_sticks_forward_z<T, n>;
+    PW_FFT_SIZES(PW_CASE)
+#undef PW_CASE
+    default:
+      return &k_sticks_forward_z<T, 0>;
   }
 }
 
@@ -436,6 +486,7 @@ extern "C" int pw_set_vtot(pw_ctx* c, const double* vtot) {
   for (auto& x : v) x *= inv;  // fold the C2R normalisation of spinor.cpp:620-621
   std::vector<float> vf(v.begin(), v.end());
   if (!c->d_vtot) {
+    invalidate_graph(c);
     PW_HIP(hipMalloc((void**)&c->d_vtot, sizeof(double) * c->N));
     PW_HIP(hipMalloc((void**)&c->d_vtot_f, sizeof(float) * c->N));
   }
@@ -447,6 +498,7 @@ extern "C" int pw_set_vtot(pw_ctx* c, const double* vtot) {
 
 extern "C" int pw_set_nonlocal(pw_ctx* c, int nbeta, const double* beta, const double* w) {
   if (!c || nbeta < 0 || (nbeta > 0 && (!beta || !w))) return fail(PW_ERR_ARG, "pw_set_nonlocal: bad argument");
+  invalidate_graph(c);
   void* ptrs[] = {c->d_beta, c->d_betaw, c->d_W, c->d_beta_f, c->d_betaw_f, c->d_W_f};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -536,17 +588,20 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
     return fail(PW_ERR_UNSUPPORTED, "H*psi slab needs %zu bytes of LDS (> 160 KiB)", lds2);
   const size_t lds1 = (size_t)esz * (c->n3 + 2 * (size_t)sc * c->n3);
   PW_TRY(mark(c, 0));
-  hipLaunchKernelGGL(k_sticks_inverse_z<T>, dim3(c->sp.nchunk, nb), dim3(256), lds1, c->stream,
+  auto kin = inv_z_kernel<T>(c->n3);
+  hipLaunchKernelGGL(kin, dim3(c->sp.nchunk, nb), dim3(256), lds1, c->stream,
                      (const C*)psi, c->npw, t1, c->n3, c->sp.nstick, sc, c->d_chunk_off, c->d_pt_idx,
                      c->d_pt_sz, tw3);
   PW_HIP(hipGetLastError());
   PW_TRY(mark(c, 1));
-  if (lds2 > 64 * 1024) PW_HIP(hipFuncSetAttribute((const void*)k_slab_xy<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds2));
-  hipLaunchKernelGGL(k_slab_xy<T>, dim3(c->n3, nb), dim3(256), lds2, c->stream, (const C*)t1, t2, c->n1,
+  auto kslab = slab_kernel<T>(c->n1, c->n2);
+  PW_TRY(ensure_lds((const void*)kslab, lds2));
+  hipLaunchKernelGGL(kslab, dim3(c->n3, nb), dim3(256), lds2, c->stream, (const C*)t1, t2, c->n1,
                      c->n2, c->n3, c->sp.nstick, c->sp.kxmax, c->d_pos, c->d_mpos, vt, tw1, tw2, rp);
   PW_HIP(hipGetLastError());
   PW_TRY(mark(c, 2));
-  hipLaunchKernelGGL(k_sticks_forward_z<T>, dim3(c->sp.nchunk, nb), dim3(256), lds1, c->stream,
+  auto kfw = fwd_z_kernel<T>(c->n3);
+  hipLaunchKernelGGL(kfw, dim3(c->sp.nchunk, nb), dim3(256), lds1, c->stream,
                      (const C*)t2, (const C*)psi, (C*)a3, c->npw, c->n3, c->sp.nstick, sc,
                      c->d_gchunk_off, c->d_gpt_idx, c->d_gpt_sz, ek, tw3);
   PW_HIP(hipGetLastError());
@@ -560,7 +615,8 @@ int sphere_to_sticks(pw_ctx* c, int nb, const void* psi, void* t1) {
   typedef typename cx<T>::t C;
   const C* tw3 = (const C*)(sizeof(T) == 8 ? (const void*)c->d_tw64 : (const void*)c->d_tw32) + c->n1 + c->n2;
   const size_t lds1 = sizeof(C) * (c->n3 + 2 * (size_t)c->sp.sc * c->n3);
-  hipLaunchKernelGGL(k_sticks_inverse_z<T>, dim3(c->sp.nchunk, nb), dim3(256), lds1, c->stream,
+  auto kin = inv_z_kernel<T>(c->n3);
+  hipLaunchKernelGGL(kin, dim3(c->sp.nchunk, nb), dim3(256), lds1, c->stream,
                      (const C*)psi, c->npw, (C*)t1, c->n3, c->sp.nstick, c->sp.sc, c->d_chunk_off, c->d_pt_idx,
                      c->d_pt_sz, tw3);
   PW_HIP(hipGetLastError());
@@ -573,7 +629,8 @@ int sticks_to_sphere_add(pw_ctx* c, int nb, const void* t2, void* out) {
   typedef typename cx<T>::t C;
   const C* tw3 = (const C*)(sizeof(T) == 8 ? (const void*)c->d_tw64 : (const void*)c->d_tw32) + c->n1 + c->n2;
   const size_t lds1 = sizeof(C) * (c->n3 + 2 * (size_t)c->sp.sc * c->n3);
-  hipLaunchKernelGGL(k_sticks_forward_z<T>, dim3(c->sp.nchunk, nb), dim3(256), lds1, c->stream, (const C*)t2,
+  auto kfw = fwd_z_kernel<T>(c->n3);
+  hipLaunchKernelGGL(kfw, dim3(c->sp.nchunk, nb), dim3(256), lds1, c->stream, (const C*)t2,
                      (const C*)nullptr, (C*)out, c->npw, c->n3, c->sp.nstick, c->sp.sc, c->d_gchunk_off,
                      c->d_gpt_idx, c->d_gpt_sz, (const T*)nullptr, tw3);
   PW_HIP(hipGetLastError());

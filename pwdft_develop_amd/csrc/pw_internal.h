@@ -45,6 +45,10 @@ double cell_volume(const double* supercell);
 bool smooth235(int n);
 // Radix plan of a 2/3/5-smooth length: factors (4, 2, 3, 5 order).
 std::vector<int> radix_plan(int n);
+// Raise a kernel's dynamic LDS limit (above the 64 KiB default, at most
+// 160 KiB).  Made once per kernel and size: the call is not stream ordered,
+// so a graph capture (which always follows an eager call) never makes it.
+int ensure_lds(const void* kern, size_t bytes);
 
 // A device buffer owned by the context.
 struct DevBuf {
@@ -86,6 +90,11 @@ int gemm_tn(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, i
 template <typename T>
 int gemm_nn(pw_ctx* c, int m, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T beta, T* Y,
             int ldy);
+// Dense symmetric eigensolvers (pw_eig.hip, pw_solver.hip): eig_sym writes
+// eigenvalues ascending to w (host) and eigenvectors to Z; sym_eig works in
+// place (A <- vectors, w device) with the sign convention of the oracle.
+int eig_sym(pw_ctx* c, int n, const double* A, int lda, double* w, double* Z, int ldz);
+int sym_eig(pw_ctx* c, int n, double* A, int lda, double* w, double* work, int* info);
 // Stick transforms of the wavefunction sphere (pw_hpsi.hip).
 template <typename T>
 int sphere_to_sticks(pw_ctx* c, int nb, const void* psi, void* t1);
@@ -140,6 +149,18 @@ struct pw_ctx {
   double* d_lz = nullptr;  // Lanczos vectors v, v_prev, f (3 x 2 npw)
   int* d_info = nullptr;
   int ws_nb = 0;
+  // Chebyshev recurrence coefficients (3 per step, degree <= 64) and the
+  // captured filter graph, replayed on a private stream
+  double h_coef[192];
+  float h_coef_f[192];
+  double* d_coef = nullptr;
+  float* d_coef_f = nullptr;
+  bool use_graphs = true;
+  hipStream_t gstream = nullptr;
+  hipEvent_t gev[2] = {nullptr, nullptr};
+  hipGraphExec_t fg_exec = nullptr;
+  long fg_key = -1, fg_warm = -1;
+  void* fg_result = nullptr;
   // timing
   static constexpr int kTimedCalls = 1024;
   bool timing = false;
@@ -147,3 +168,10 @@ struct pw_ctx {
   int ev_calls = 0, ev_total = 0;
   double stage_ms[4] = {0, 0, 0, 0};
 };
+
+namespace pw {
+// A captured filter graph holds raw device pointers and the shape of the
+// operator (potential, projectors, exchange): it is dropped whenever any of
+// them is reallocated or switched on or off.
+inline void invalidate_graph(pw_ctx* c) { c->fg_key = c->fg_warm = -1; }
+}  // namespace pw

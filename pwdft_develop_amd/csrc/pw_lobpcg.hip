@@ -112,13 +112,7 @@ struct LobpcgWs {
 };
 
 static int syevd_fixed(pw_ctx* c, int n, double* A, int ld, double* w, double* e, int* info) {
-  PW_BLAS(rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_upper, n, A, ld, w, e, info));
-  int h = 0;
-  PW_HIP(hipMemcpyAsync(&h, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  PW_HIP(hipStreamSynchronize(c->stream));
-  if (h != 0) return fail(PW_ERR_LINALG, "Syevd failed (info = %d)", h);
-  hipLaunchKernelGGL(k_sign_fix2, dim3(n), dim3(256), 0, c->stream, A, n, ld);
-  return PW_OK;
+  return sym_eig(c, n, A, ld, w, e, info);
 }
 
 // SolveGenEig sequential branch on the device; eigenvectors overwrite AMat

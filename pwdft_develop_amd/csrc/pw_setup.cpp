@@ -6,6 +6,8 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include "pw_internal.h"
 
@@ -54,6 +56,19 @@ bool smooth235(int n) {
   for (int p : {2, 3, 5})
     while (n % p == 0) n /= p;
   return n == 1;
+}
+
+int ensure_lds(const void* kern, size_t bytes) {
+  static std::mutex mu;
+  static std::map<const void*, size_t> done;
+  if (bytes <= 64 * 1024) return PW_OK;
+  if (bytes > 160 * 1024) return fail(PW_ERR_UNSUPPORTED, "kernel needs %zu bytes of LDS (> 160 KiB)", bytes);
+  std::lock_guard<std::mutex> lock(mu);
+  size_t& have = done[kern];
+  if (have >= bytes) return PW_OK;
+  PW_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  have = bytes;
+  return PW_OK;
 }
 
 std::vector<int> radix_plan(int n) {

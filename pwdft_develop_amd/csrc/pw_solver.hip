@@ -118,6 +118,13 @@ __global__ void k_lanczos_update(double* __restrict__ f, const double* __restric
     f[i] -= a * v[i] + b * vp[i];
 }
 
+// hist = (alpha, ||f||^2); ab[1] = ||f|| becomes the next step's beta
+__global__ void k_lanczos_record(double* __restrict__ ab, double* __restrict__ hist) {
+  hist[0] = ab[0];
+  hist[1] = ab[2];
+  ab[1] = sqrt(ab[2]);
+}
+
 __global__ void k_scale_to(double* __restrict__ out, const double* __restrict__ in, const double* __restrict__ nrm2,
                            long n) {
   const double s = 1.0 / sqrt(nrm2[0]);
@@ -126,6 +133,41 @@ __global__ void k_scale_to(double* __restrict__ out, const double* __restrict__ 
 }
 
 static inline int grid_for(long n) { return (int)std::min<long>((n + 255) / 256, 8192); }
+
+// Symmetric eigen-decomposition in place: A (n x n, upper triangle, lda)
+// <- eigenvectors (largest |component| positive), w (device) <- eigenvalues
+// ascending.  Block one-sided Jacobi (pw_eig.hip) by default; rocSOLVER
+// dsyevd when PW_EIGEN=rocsolver.
+int sym_eig(pw_ctx* c, int n, double* A, int lda, double* w, double* work, int* info) {
+  hipStream_t s = c->stream;
+  static const bool use_rocsolver = [] {
+    const char* e = getenv("PW_EIGEN");
+    return e && std::string(e) == "rocsolver";
+  }();
+  if (use_rocsolver) {
+    PW_BLAS(rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_upper, n, A, lda, w, work, info));
+    int h = 0;
+    PW_HIP(hipMemcpyAsync(&h, info, sizeof(int), hipMemcpyDeviceToHost, s));
+    PW_HIP(hipStreamSynchronize(s));
+    if (h != 0) return fail(PW_ERR_LINALG, "Syevd failed (info = %d)", h);
+  } else {
+    std::vector<double> wh(n);
+    double* Z = nullptr;
+    PW_HIP(hipMalloc((void**)&Z, sizeof(double) * (size_t)n * n));
+    int rc = eig_sym(c, n, A, lda, wh.data(), Z, n);
+    if (rc == PW_OK) {
+      if (hipMemcpy2DAsync(A, sizeof(double) * lda, Z, sizeof(double) * n, sizeof(double) * n, n,
+                           hipMemcpyDeviceToDevice, s) != hipSuccess ||
+          hipMemcpyAsync(w, wh.data(), sizeof(double) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess)
+        rc = fail(PW_ERR_HIP, "sym_eig: copy failed");
+    }
+    (void)hipFree(Z);
+    if (rc != PW_OK) return rc;
+  }
+  hipLaunchKernelGGL(k_sign_fix, dim3(n), dim3(256), 0, s, A, n, lda);
+  return PW_OK;
+}
 
 static int ensure_ws(pw_ctx* c) {
   if (c->ws_nb) return PW_OK;
@@ -138,9 +180,11 @@ static int ensure_ws(pw_ctx* c) {
   PW_HIP(hipMalloc((void**)&c->d_S2, sizeof(double) * nn));
   PW_HIP(hipMalloc((void**)&c->d_D, sizeof(double) * c->max_bands));
   PW_HIP(hipMalloc((void**)&c->d_E, sizeof(double) * c->max_bands));
-  PW_HIP(hipMalloc((void**)&c->d_red, sizeof(double) * (c->max_bands + 8)));
+  PW_HIP(hipMalloc((void**)&c->d_red, sizeof(double) * (std::max(c->max_bands, 128) + 8)));
   PW_HIP(hipMalloc((void**)&c->d_info, sizeof(int) * 4));
   PW_HIP(hipMalloc((void**)&c->d_lz, sizeof(double) * 3 * 2 * (size_t)c->npw));
+  PW_HIP(hipMalloc((void**)&c->d_coef, sizeof(double) * 192));
+  PW_HIP(hipMalloc((void**)&c->d_coef_f, sizeof(float) * 192));
   c->ws_nb = c->max_bands;
   return PW_OK;
 }
@@ -153,12 +197,22 @@ extern "C" int pw_free_workspace(pw_ctx* c) {
   if (!c) return fail(PW_ERR_ARG, "null context");
   for (int i = 0; i < 6; i++)
     if (c->d_ws[i]) { (void)hipFree(c->d_ws[i]); c->d_ws[i] = nullptr; }
-  void* p[] = {c->d_S, c->d_S2, c->d_D, c->d_E, c->d_red, c->d_info, c->d_lz};
+  void* p[] = {c->d_S, c->d_S2, c->d_D, c->d_E, c->d_red, c->d_info, c->d_lz, c->d_coef, c->d_coef_f};
   for (void* q : p)
     if (q) (void)hipFree(q);
   c->d_S = c->d_S2 = c->d_D = c->d_E = c->d_red = c->d_lz = nullptr;
+  c->d_coef = nullptr;
+  c->d_coef_f = nullptr;
   c->d_info = nullptr;
   c->ws_nb = 0;
+  // the captured filter graph refers to the workspace: drop it too
+  if (c->fg_exec) (void)hipGraphExecDestroy(c->fg_exec);
+  c->fg_exec = nullptr;
+  c->fg_key = c->fg_warm = -1;
+  if (c->gstream) (void)hipStreamDestroy(c->gstream);
+  c->gstream = nullptr;
+  for (auto& e : c->gev)
+    if (e) { (void)hipEventDestroy(e); e = nullptr; }
   return PW_OK;
 }
 
@@ -176,6 +230,8 @@ extern "C" int pw_lanczos(pw_ctx* c, int k, const void* v0, double* theta, doubl
   PW_HIP(hipMemsetAsync(ab, 0, sizeof(double) * 4, s));
   hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, (const double*)v0, (const double*)v0, m, ab + 2);
   hipLaunchKernelGGL(k_scale_to, dim3(grid_for(m)), dim3(256), 0, s, v, (const double*)v0, ab + 2, m);
+  // alpha_j, ||f_j||^2 recorded on the device; one readback at the end
+  double* hist = ab + 4;  // 2k <= 128 entries; d_red holds max(max_bands, 128) + 8
   std::vector<double> al(k), be(k);
   for (int j = 0; j < k; j++) {
     PW_TRY(pw_apply_h(c, PW_F64, 1, v, f));
@@ -183,19 +239,19 @@ extern "C" int pw_lanczos(pw_ctx* c, int k, const void* v0, double* theta, doubl
     hipLaunchKernelGGL(k_lanczos_update, dim3(grid_for(m)), dim3(256), 0, s, f, (const double*)v,
                        (const double*)vp, (const double*)ab, m);
     hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, (const double*)f, (const double*)f, m, ab + 2);
-    double h[3];
-    PW_HIP(hipMemcpyAsync(h, ab, sizeof(double) * 3, hipMemcpyDeviceToHost, s));
-    PW_HIP(hipStreamSynchronize(s));
-    al[j] = h[0];
-    be[j] = std::sqrt(h[2]);
+    hipLaunchKernelGGL(k_lanczos_record, dim3(1), dim3(1), 0, s, ab, hist + 2 * j);
     if (j + 1 < k) {
-      // vp <- v ; v <- f / beta ; next beta in ab[1]
+      // vp <- v ; v <- f / beta ; beta (for the next step) = ||f|| stays on the device
       PW_HIP(hipMemcpyAsync(vp, v, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
       hipLaunchKernelGGL(k_scale_to, dim3(grid_for(m)), dim3(256), 0, s, v, (const double*)f, ab + 2, m);
-      const double bnext = be[j];
-      PW_HIP(hipMemcpyAsync(ab + 1, &bnext, sizeof(double), hipMemcpyHostToDevice, s));
-      PW_HIP(hipStreamSynchronize(s));
     }
+  }
+  std::vector<double> h(2 * k);
+  PW_HIP(hipMemcpyAsync(h.data(), hist, sizeof(double) * 2 * k, hipMemcpyDeviceToHost, s));
+  PW_HIP(hipStreamSynchronize(s));
+  for (int j = 0; j < k; j++) {
+    al[j] = h[2 * j];
+    be[j] = std::sqrt(h[2 * j + 1]);
   }
   // Ritz values of T_k (symmetric tridiagonal, k <= 64): Jacobi rotations on host
   std::vector<double> T((size_t)k * k, 0.0);
@@ -234,50 +290,165 @@ extern "C" int pw_lanczos(pw_ctx* c, int k, const void* v0, double* theta, doubl
   return PW_OK;
 }
 
-// Chebyshev filter of degree m on X (dtype blocks, in place).
-extern "C" int pw_cheb_filter(pw_ctx* c, int dtype, int nb, int m, double a, double b, double aL, void* X) {
-  if (!c || !X || nb < 1 || nb > c->max_bands || m < 1) return fail(PW_ERR_ARG, "pw_cheb_filter: bad argument");
-  if (dtype != PW_F64 && dtype != PW_F32) return fail(PW_ERR_ARG, "unknown dtype %d", dtype);
-  if (!(b > a) || !(aL < a)) return fail(PW_ERR_ARG, "pw_cheb_filter: need aL < a < b");
-  PW_TRY(ensure_ws(c));
+// Y = (HX - c X) s with (c, s) = coef[0..1]
+template <typename T>
+__global__ void k_cheb_first_c(T* __restrict__ Y, const T* __restrict__ HX, const T* __restrict__ X,
+                               const T* __restrict__ coef, long n) {
+  const T c = coef[0], s = coef[1];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    Y[i] = (HX[i] - c * X[i]) * s;
+}
+
+// Yn = (HY - c Y) s1 - s2 Xp with (c, s1, s2) = coef[3 step ..]
+template <typename T>
+__global__ void k_cheb_step_c(T* __restrict__ Yn, const T* __restrict__ HY, const T* __restrict__ Y,
+                              const T* __restrict__ Xp, const T* __restrict__ coef, int step, long n) {
+  const T c = coef[3 * step], s1 = coef[3 * step + 1], s2 = coef[3 * step + 2];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    Yn[i] = (HY[i] - c * Y[i]) * s1 - s2 * Xp[i];
+}
+
+// Filter body on the fixed buffers: input d_ws[0], three rotating buffers
+// d_ws[1..3], H Y in d_ws[4]; *result = buffer holding p_m(H) X.  Only
+// stream-ordered work: safe to capture.
+static int cheb_body(pw_ctx* c, int dtype, int nb, int m, void** result) {
   const long n = 2L * c->npw * nb;
-  const size_t bytes = (dtype == PW_F64 ? 8 : 4) * (size_t)n;
-  const double e = (b - a) / 2.0, cc = (b + a) / 2.0;
-  double sigma = e / (aL - cc);
-  const double tau = 2.0 / sigma;
   void* bufs[3] = {c->d_ws[1], c->d_ws[2], c->d_ws[3]};
+  void* X = c->d_ws[0];
   void* HY = c->d_ws[4];
   hipStream_t s = c->stream;
-  void* Xp = X;      // X_{k-1}
-  void* Y = bufs[0]; // X_k
+  void* Xp = X;       // X_{k-1}
+  void* Y = bufs[0];  // X_k
   PW_TRY(pw_apply_h(c, dtype, nb, X, HY));
   if (dtype == PW_F64)
-    hipLaunchKernelGGL(k_cheb_first<double>, dim3(grid_for(n)), dim3(256), 0, s, (double*)Y, (const double*)HY,
-                       (const double*)X, cc, sigma / e, n);
+    hipLaunchKernelGGL(k_cheb_first_c<double>, dim3(grid_for(n)), dim3(256), 0, s, (double*)Y, (const double*)HY,
+                       (const double*)X, (const double*)c->d_coef, n);
   else
-    hipLaunchKernelGGL(k_cheb_first<float>, dim3(grid_for(n)), dim3(256), 0, s, (float*)Y, (const float*)HY,
-                       (const float*)X, (float)cc, (float)(sigma / e), n);
+    hipLaunchKernelGGL(k_cheb_first_c<float>, dim3(grid_for(n)), dim3(256), 0, s, (float*)Y, (const float*)HY,
+                       (const float*)X, (const float*)c->d_coef_f, n);
   int nxt = 1;
   for (int it = 2; it <= m; it++) {
-    const double sn = 1.0 / (tau - sigma);
     void* Yn = bufs[nxt];
     PW_TRY(pw_apply_h(c, dtype, nb, Y, HY));
     if (dtype == PW_F64)
-      hipLaunchKernelGGL(k_cheb_step<double>, dim3(grid_for(n)), dim3(256), 0, s, (double*)Yn, (const double*)HY,
-                         (const double*)Y, (const double*)Xp, cc, 2.0 * sn / e, sigma * sn, n);
+      hipLaunchKernelGGL(k_cheb_step_c<double>, dim3(grid_for(n)), dim3(256), 0, s, (double*)Yn, (const double*)HY,
+                         (const double*)Y, (const double*)Xp, (const double*)c->d_coef, it - 1, n);
     else
-      hipLaunchKernelGGL(k_cheb_step<float>, dim3(grid_for(n)), dim3(256), 0, s, (float*)Yn, (const float*)HY,
-                         (const float*)Y, (const float*)Xp, (float)cc, (float)(2.0 * sn / e), (float)(sigma * sn), n);
-    // rotate: Xp <- Y, Y <- Yn; the three scratch buffers cycle, X is never overwritten mid-way
+      hipLaunchKernelGGL(k_cheb_step_c<float>, dim3(grid_for(n)), dim3(256), 0, s, (float*)Yn, (const float*)HY,
+                         (const float*)Y, (const float*)Xp, (const float*)c->d_coef_f, it - 1, n);
+    // rotate: Xp <- Y, Y <- Yn over the three scratch buffers
     Xp = Y;
     Y = Yn;
     nxt = (nxt + 1) % 3;
     if (bufs[nxt] == Xp) nxt = (nxt + 1) % 3;
     if (bufs[nxt] == Y) nxt = (nxt + 1) % 3;
-    sigma = sn;
   }
-  PW_HIP(hipMemcpyAsync(X, Y, bytes, hipMemcpyDeviceToDevice, s));
   PW_HIP(hipGetLastError());
+  *result = Y;
+  return PW_OK;
+}
+
+// Capture the filter body on the context's private stream into a graph.
+static int graph_capture(pw_ctx* c, int dtype, int nb, int m) {
+  if (c->fg_exec) {
+    (void)hipGraphExecDestroy(c->fg_exec);
+    c->fg_exec = nullptr;
+  }
+  if (!c->gstream) PW_HIP(hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking));
+  if (!c->gev[0]) {
+    PW_HIP(hipEventCreateWithFlags(&c->gev[0], hipEventDisableTiming));
+    PW_HIP(hipEventCreateWithFlags(&c->gev[1], hipEventDisableTiming));
+  }
+  hipStream_t user = c->stream;
+  const bool timing = c->timing;
+  c->timing = false;  // no event nodes in the graph
+  c->stream = c->gstream;
+  hipGraph_t g = nullptr;
+  int rc = PW_OK;
+  if (hipStreamBeginCapture(c->gstream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    rc = fail(PW_ERR_HIP, "hipStreamBeginCapture failed");
+  } else {
+    void* res = nullptr;
+    rc = cheb_body(c, dtype, nb, m, &res);
+    const hipError_t e = hipStreamEndCapture(c->gstream, &g);
+    if (rc == PW_OK && e != hipSuccess) rc = fail(PW_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    if (rc == PW_OK && hipGraphInstantiate(&c->fg_exec, g, nullptr, nullptr, 0) != hipSuccess)
+      rc = fail(PW_ERR_HIP, "hipGraphInstantiate failed");
+    c->fg_result = res;
+  }
+  if (g) (void)hipGraphDestroy(g);
+  c->stream = user;
+  c->timing = timing;
+  return rc;
+}
+
+// Replay: ordered after the work already queued on the context's stream,
+// and the stream waits for the replay.
+static int graph_launch(pw_ctx* c) {
+  PW_HIP(hipEventRecord(c->gev[0], c->stream));
+  PW_HIP(hipStreamWaitEvent(c->gstream, c->gev[0], 0));
+  PW_HIP(hipGraphLaunch(c->fg_exec, c->gstream));
+  PW_HIP(hipEventRecord(c->gev[1], c->gstream));
+  PW_HIP(hipStreamWaitEvent(c->stream, c->gev[1], 0));
+  return PW_OK;
+}
+
+// Chebyshev filter of degree m on X (dtype blocks, in place).
+extern "C" int pw_cheb_filter(pw_ctx* c, int dtype, int nb, int m, double a, double b, double aL, void* X) {
+  if (!c || !X || nb < 1 || nb > c->max_bands || m < 1) return fail(PW_ERR_ARG, "pw_cheb_filter: bad argument");
+  if (dtype != PW_F64 && dtype != PW_F32) return fail(PW_ERR_ARG, "unknown dtype %d", dtype);
+  if (!(b > a) || !(aL < a)) return fail(PW_ERR_ARG, "pw_cheb_filter: need aL < a < b");
+  if (m > 64) return fail(PW_ERR_ARG, "pw_cheb_filter: degree > 64");
+  PW_TRY(ensure_ws(c));
+  const long n = 2L * c->npw * nb;
+  const size_t bytes = (dtype == PW_F64 ? 8 : 4) * (size_t)n;
+  // scaled-recurrence coefficients (c, s1, s2) per step, uploaded before the
+  // (possibly graph-replayed) filter body reads them
+  {
+    const double e = (b - a) / 2.0, cc = (b + a) / 2.0;
+    double sigma = e / (aL - cc);
+    const double tau = 2.0 / sigma;
+    c->h_coef[0] = cc; c->h_coef[1] = sigma / e; c->h_coef[2] = 0.0;
+    for (int it = 2; it <= m; it++) {
+      const double sn = 1.0 / (tau - sigma);
+      c->h_coef[3 * (it - 1)] = cc;
+      c->h_coef[3 * (it - 1) + 1] = 2.0 * sn / e;
+      c->h_coef[3 * (it - 1) + 2] = sigma * sn;
+      sigma = sn;
+    }
+    for (int i = 0; i < 3 * m; i++) c->h_coef_f[i] = (float)c->h_coef[i];
+  }
+  hipStream_t s = c->stream;
+  PW_HIP(hipMemcpyAsync(c->d_coef, c->h_coef, sizeof(double) * 3 * m, hipMemcpyHostToDevice, s));
+  PW_HIP(hipMemcpyAsync(c->d_coef_f, c->h_coef_f, sizeof(float) * 3 * m, hipMemcpyHostToDevice, s));
+  void* X0 = c->d_ws[0];
+  if (X != X0) PW_HIP(hipMemcpyAsync(X0, X, bytes, hipMemcpyDeviceToDevice, s));
+  void* result = nullptr;
+  const long key = ((long)dtype << 40) | ((long)m << 32) | (long)nb;
+  if (!c->use_graphs) {
+    PW_TRY(cheb_body(c, dtype, nb, m, &result));
+  } else if (c->fg_exec && c->fg_key == key) {
+    PW_TRY(graph_launch(c));
+    result = c->fg_result;
+  } else if (c->fg_warm == key) {
+    PW_TRY(graph_capture(c, dtype, nb, m));
+    c->fg_key = key;
+    PW_TRY(graph_launch(c));
+    result = c->fg_result;
+  } else {
+    // first call of this shape runs eagerly (allocating every lazily sized
+    // scratch); the next one is captured and replayed from then on
+    PW_TRY(cheb_body(c, dtype, nb, m, &result));
+    c->fg_warm = key;
+  }
+  if (X != result) PW_HIP(hipMemcpyAsync(X, result, bytes, hipMemcpyDeviceToDevice, s));
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+extern "C" int pw_set_graphs(pw_ctx* c, int enable) {
+  if (!c) return fail(PW_ERR_ARG, "null context");
+  c->use_graphs = enable != 0;
   return PW_OK;
 }
 
@@ -320,13 +491,7 @@ extern "C" int pw_rayleigh_ritz(pw_ctx* c, int hdtype, int nb, void* X, double* 
     hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n)), dim3(256), 0, s, HX, (const float*)HXf, n);
   }
   PW_TRY(gemm_tn<double>(c, m, nb, nb, (const double*)X, m, HX, m, c->d_S, nb, nullptr, zero));
-  PW_BLAS(rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_upper, nb, c->d_S, nb, c->d_D,
-                           c->d_E, c->d_info));
-  int info = 0;
-  PW_HIP(hipMemcpyAsync(&info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, s));
-  PW_HIP(hipStreamSynchronize(s));
-  if (info != 0) return fail(PW_ERR_LINALG, "Syevd failed (info = %d)", info);
-  hipLaunchKernelGGL(k_sign_fix, dim3(nb), dim3(256), 0, s, c->d_S, nb, nb);
+  PW_TRY(sym_eig(c, nb, c->d_S, nb, c->d_D, c->d_E, c->d_info));
   PW_TRY(gemm_nn<double>(c, m, nb, nb, one, (const double*)X, m, c->d_S, nb, zero, T, m));
   PW_HIP(hipMemcpyAsync(X, T, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
   PW_TRY(gemm_nn<double>(c, m, nb, nb, one, HX, m, c->d_S, nb, zero, T, m));

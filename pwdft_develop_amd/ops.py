@@ -179,6 +179,10 @@ class Context:
                                            np.ascontiguousarray(f, np.float64), natom, npa, F))
         return F.reshape(natom, 3)
 
+    def set_graphs(self, on=True):
+        """Capture/replay the Chebyshev filter as a hipGraph (default on)."""
+        check(_lib.lib().pw_set_graphs(self.h, 1 if on else 0))
+
     def set_timing(self, on=True):
         check(_lib.lib().pw_set_timing(self.h, 1 if on else 0))
 
