@@ -191,6 +191,16 @@ int pw_set_exx(pw_ctx* ctx, double ecut, double screen_mu, double exx_fraction,
  * a3 += V_x psi for nb bands (dtype blocks). */
 int pw_add_exx(pw_ctx* ctx, int dtype, int nb, const void* psi, void* a3);
 
+/* Adaptively compressed exchange: replaces KohnSham::CalculateVexxACE,
+ * sphere branch (src/SCF/hamiltonian.cpp:3685-3924).  From the FP64 device
+ * block psi (nb bands): W = V_x psi, M = -Re(psi^H W) = L L^T, xi = W L^{-T}.
+ * Afterwards pw_mult_spinor and pw_apply_h use a3 -= xi Re(xi^H psi)
+ * (hamiltonian.cpp:3347-3430, the isHybridACE branch) in place of the full
+ * exchange, in either dtype; pw_add_exx still applies the full V_x.  nb = 0
+ * drops the compressed operator; pw_set_exx drops it too.  PW_ERR_LINALG
+ * when M is not positive definite (the reference's Potrf failure). */
+int pw_build_ace(pw_ctx* ctx, int nb, const void* psi);
+
 /* exxDiv of KohnSham::InitializeEXX (hamiltonian.cpp:3473-3517), host only. */
 int pw_exx_divergence(const int grid[3], const double supercell[9], double ecut,
                       double screen_mu, int divtype, double* exx_div);

@@ -10,6 +10,8 @@
  *   or_teter_precond     spinor.cpp:348-374
  *   or_ham_set_exx       hamiltonian.cpp:3454-3551 (kernel), 3553-3603 (phi)
  *   or_add_exx           spinor.cpp:783-945
+ *   or_ham_build_ace     hamiltonian.cpp:3685-3924 (sphere branch); applied in
+ *                        or_mult_spinor as hamiltonian.cpp:3347-3430
  *   or_density           hamiltonian.cpp:2509-2573
  *   or_nonlocal_force    hamiltonian.cpp:2907-3006 (in G-space form)
  */
@@ -161,6 +163,9 @@ struct or_ham {
   double exx_fraction;
   double* phi_rs; /* N x nocc real-space phi */
   double* occ;
+  /* ACE: xi (2npw x nace real view); MultSpinor uses it instead of V_x */
+  int nace;
+  double* xi;
   int nthreads;
 };
 
@@ -199,6 +204,7 @@ void or_ham_destroy(or_ham* H) {
   free(H->ekin); free(H->teter); free(H->vtot);
   free(H->beta); free(H->wgt);
   free(H->fock_idx); free(H->exx_kernel); free(H->phi_rs); free(H->occ);
+  free(H->xi);
   free(H);
 }
 
@@ -269,6 +275,16 @@ int or_mult_spinor(or_ham* H, int nb, const double* psid, double* a3d) {
     free(r);
     or_plan3_delete(P);
   }
+  if (H->nocc > 0 && H->nace > 0) {
+    /* ACE apply (hamiltonian.cpp:3347-3430): M = Re(xi^H psi) (real
+     * nace x nb, blas.cpp:997-1029), a3 -= xi M (blas.cpp:1031-1067) */
+    const int m = 2 * H->npw;
+    double* M = (double*)malloc(sizeof(double) * (size_t)H->nace * nb);
+    or_gemm_tn(m, H->nace, nb, H->xi, psid, M);
+    or_gemm_nn(m, nb, H->nace, -1.0, H->xi, m, M, H->nace, 1.0, a3d, m);
+    free(M);
+    return 0;
+  }
   if (H->nocc > 0) return or_add_exx(H, nb, psid, a3d);
   return 0;
 }
@@ -332,6 +348,9 @@ int or_ham_set_exx(or_ham* H, double ecut, double mu, double exx_fraction, int d
   const int n1 = H->n[0], n2 = H->n[1], n3 = H->n[2];
   const double epsDiv = 1e-8;
   free(H->fock_idx); free(H->exx_kernel); free(H->phi_rs); free(H->occ);
+  free(H->xi); /* a new exchange operator drops the compressed one */
+  H->xi = NULL;
+  H->nace = 0;
   H->fock_idx = (int*)malloc(sizeof(int) * H->Nh);
   double* gf = (double*)malloc(sizeof(double) * H->Nh);
   H->nfock = or_fock_sphere_build(H->n, H->supercell, ecut, H->fock_idx, gf);
@@ -428,6 +447,36 @@ int or_add_exx(or_ham* H, int nb, const double* psid, double* a3d) {
     free(half); free(prod); free(psr); free(r);
     or_plan3_delete(P);
   }
+  return 0;
+}
+
+/* KohnSham::CalculateVexxACE, sphere branch (hamiltonian.cpp:3685-3924):
+ * W = V_x psi (AddMultSpinorEXX into a zero block), M = -Re(psi^H W),
+ * Potrf('L') M = L L^T, xi = W L^{-T} (Trsm 'R','L','T','N').  With
+ * L = U^T this is the upper factor M = U^T U and xi = W U^{-1}.  nb = 0
+ * drops the operator.  Returns -4 when M is not positive definite. */
+int or_ham_build_ace(or_ham* H, int nb, const double* psi) {
+  free(H->xi);
+  H->xi = NULL;
+  H->nace = 0;
+  if (nb == 0) return 0;
+  if (H->nocc <= 0) return -3;
+  const int m = 2 * H->npw;
+  double* W = (double*)calloc((size_t)m * nb, sizeof(double));
+  double* M = (double*)malloc(sizeof(double) * (size_t)nb * nb);
+  or_add_exx(H, nb, psi, W);
+  or_gemm_tn(m, nb, nb, psi, W, M);
+  for (size_t i = 0; i < (size_t)nb * nb; i++) M[i] = -M[i];
+  int info = or_potrf_upper(nb, M, nb);
+  if (info != 0) {
+    free(W);
+    free(M);
+    return -4;
+  }
+  or_trsm_right_upper(m, nb, M, nb, W, m);
+  free(M);
+  H->xi = W;
+  H->nace = nb;
   return 0;
 }
 

@@ -74,6 +74,12 @@ int or_ham_set_exx(or_ham* H, double ecut, double screen_mu, double exx_fraction
                    const double* occ, double* exx_div_out);
 int or_ham_set_threads(or_ham* H, int nthreads);
 
+/* Adaptively compressed exchange from the block psi (npw x nb):
+ * KohnSham::CalculateVexxACE sphere branch (hamiltonian.cpp:3685-3924).
+ * After it or_mult_spinor applies -xi Re(xi^H psi) in place of V_x
+ * (hamiltonian.cpp:3347-3430).  nb = 0 drops it; or_ham_set_exx drops it. */
+int or_ham_build_ace(or_ham* H, int nb, const double* psi);
+
 /* KohnSham::MultSpinor (hamiltonian.cpp:3270-3452): a3 = (T + V_loc [+ V_x]) psi */
 int or_mult_spinor(or_ham* H, int nb, const double* psi, double* a3);
 /* Spinor::AddMultSpinorEXX (spinor.cpp:783-945): a3 += V_x psi */

@@ -45,6 +45,7 @@ def lib():
         L.or_ham_set_exx.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int,
                                      C.c_int, _dp, _dp, C.POINTER(C.c_double)]
         L.or_ham_set_threads.argtypes = [C.c_void_p, C.c_int]
+        L.or_ham_build_ace.argtypes = [C.c_void_p, C.c_int, _dp]
         for f in ("or_mult_spinor", "or_add_exx", "or_nonlocal_multx", "or_apply_h",
                   "or_teter_precond"):
             getattr(L, f).argtypes = [C.c_void_p, C.c_int, _dp, _dp]
@@ -168,6 +169,14 @@ class Hamiltonian:
         _chk(lib().or_ham_set_exx(self.h, ecut, mu, frac, divtype, psi_occ.shape[0], _r(psi_occ),
                                   np.ascontiguousarray(occ, dtype=np.float64), C.byref(d)), "set_exx")
         return d.value
+
+    def build_ace(self, psi):
+        """KohnSham::CalculateVexxACE (hamiltonian.cpp:3685-3924) from the
+        block psi; afterwards mult_spinor/apply_h use the compressed exchange.
+        An empty block drops it."""
+        psi = np.ascontiguousarray(psi, dtype=np.complex128)
+        buf = _r(psi) if psi.size else np.zeros(1)
+        _chk(lib().or_ham_build_ace(self.h, psi.shape[0], buf), "build_ace")
 
     def _op(self, name, psi):
         psi = np.ascontiguousarray(psi, dtype=np.complex128)

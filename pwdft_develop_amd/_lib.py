@@ -73,6 +73,7 @@ def lib():
         "pw_nonlocal_force": [_vp, C.c_int, _vp, _dp, C.c_int, C.c_int, _dp],
         "pw_set_exx": [_vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _vp, _dp],
         "pw_add_exx": [_vp, C.c_int, C.c_int, _vp, _vp],
+        "pw_build_ace": [_vp, C.c_int, _vp],
         "pw_exx_divergence": [_ip, _dp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double)],
         "pw_lobpcg": [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _vp, _dp, _dp,
                       C.POINTER(C.c_int)],

@@ -20,6 +20,8 @@
 #include <cmath>
 #include <vector>
 
+#include <rocsolver/rocsolver.h>
+
 #include "pw_fft.h"
 #include "pw_internal.h"
 
@@ -44,7 +46,38 @@ struct ExxState {
   void* d_acc = nullptr;         // max_bands x N
   void* d_fst = nullptr;         // pair Fock sticks: npair_max x n3 x nstick complex double
   int npair_max = 0;
+  // adaptively compressed exchange: xi (2 npw x nace real view) and the
+  // projection scratch M = Re(xi^H psi) (nace x max_bands)
+  int nace = 0;
+  double* d_xi64 = nullptr;
+  float* d_xi32 = nullptr;
+  double* d_M64 = nullptr;
+  float* d_M32 = nullptr;
 };
+
+static void ace_free(ExxState* E) {
+  void* p[] = {E->d_xi64, E->d_xi32, E->d_M64, E->d_M32};
+  for (void* q : p)
+    if (q) (void)hipFree(q);
+  E->d_xi64 = nullptr;
+  E->d_xi32 = nullptr;
+  E->d_M64 = nullptr;
+  E->d_M32 = nullptr;
+  E->nace = 0;
+}
+
+template <typename S, typename D>
+__global__ void k_exx_convert(D* __restrict__ out, const S* __restrict__ in, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = (D)in[i];
+}
+
+template <typename T>
+__global__ void k_exx_negate(T* __restrict__ p, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = -p[i];
+}
+
+static inline int grid_of(long n) { return (int)std::max<long>(1, std::min<long>((n + 255) / 256, 8192)); }
 
 // ------------------------------------------------------------ x-pass helpers
 // Rows y1 = 2q, y2 = 2q+1 (q in [q0, q0+nq)) of the y-transformed spectrum
@@ -465,6 +498,7 @@ void exx_free(pw_ctx* c) {
                E->d_psir, E->d_acc, E->d_fst};
   for (void* q : p)
     if (q) (void)hipFree(q);
+  ace_free(E);
   delete E;
   c->exx = nullptr;
 }
@@ -581,18 +615,99 @@ extern "C" int pw_set_exx(pw_ctx* c, double ecut, double mu, double exx_fraction
   const double fphi = std::sqrt(1.0 / c->vol / (double)c->N);
   PW_TRY(sphere_to_real<double>(c, nocc, psi_occ, E->d_phi64, fphi));
   {
-    float* tmp = E->d_phi32;
-    const long n = (long)N * nocc;
     // FP32 copy of the FP64 real-space phi
-    std::vector<double> h(0);
-    (void)h;
-    PW_HIP(hipStreamSynchronize(c->stream));
-    std::vector<double> host((size_t)n);
-    PW_HIP(hipMemcpy(host.data(), E->d_phi64, sizeof(double) * n, hipMemcpyDeviceToHost));
-    std::vector<float> hf(host.begin(), host.end());
-    PW_HIP(hipMemcpy(tmp, hf.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    const long n = (long)N * nocc;
+    hipLaunchKernelGGL((k_exx_convert<double, float>), dim3(grid_of(n)), dim3(256), 0, c->stream, E->d_phi32,
+                       (const double*)E->d_phi64, n);
+    PW_HIP(hipGetLastError());
   }
   return PW_OK;
+}
+
+// ---------------------------------------------- adaptively compressed exchange
+
+namespace pw {
+
+// MultSpinor's exchange term (hamiltonian.cpp:3343-3449): the compressed
+// operator a3 -= xi Re(xi^H psi) when one is built (:3347-3430), else the
+// full pair-density apply.
+template <typename T>
+int exchange_add(pw_ctx* c, int nb, const void* psi, void* a3) {
+  ExxState* E = c->exx;
+  if (E->nace == 0) return exx_add<T>(c, nb, psi, a3);
+  const int m = 2 * c->npw;
+  const T* xi = sizeof(T) == 8 ? (const T*)(const void*)E->d_xi64 : (const T*)(const void*)E->d_xi32;
+  T* M = sizeof(T) == 8 ? (T*)(void*)E->d_M64 : (T*)(void*)E->d_M32;
+  PW_TRY(gemm_tn<T>(c, m, E->nace, nb, xi, m, (const T*)psi, m, M, E->nace, nullptr, (T)0));
+  PW_TRY(gemm_nn<T>(c, m, nb, E->nace, (T)-1, xi, m, M, E->nace, (T)1, (T*)a3, m));
+  return PW_OK;
+}
+template int exchange_add<double>(pw_ctx*, int, const void*, void*);
+template int exchange_add<float>(pw_ctx*, int, const void*, void*);
+
+}  // namespace pw
+
+// KohnSham::CalculateVexxACE, sphere branch (hamiltonian.cpp:3685-3924):
+// W = V_x psi, M = -Re(psi^H W), M = U^T U (the reference's Potrf('L') with
+// L = U^T), xi = W U^{-1} (its Trsm('R','L','T','N')).  FP64 throughout; an
+// FP32 copy of xi serves FP32 applies.
+extern "C" int pw_build_ace(pw_ctx* c, int nb, const void* psi) {
+  if (!c) return fail(PW_ERR_ARG, "null context");
+  if (!c->exx) return fail(PW_ERR_STATE, "pw_build_ace: no exchange operator set (pw_set_exx)");
+  if (nb < 0 || nb > c->max_bands) return fail(PW_ERR_ARG, "pw_build_ace: nb = %d outside [0, %d]", nb, c->max_bands);
+  ExxState* E = c->exx;
+  invalidate_graph(c);
+  PW_HIP(hipStreamSynchronize(c->stream));  // earlier applies may still read xi
+  ace_free(E);
+  if (nb == 0) return PW_OK;
+  if (!psi) return fail(PW_ERR_ARG, "pw_build_ace: null block");
+  const int m = 2 * c->npw;
+  const long nw = (long)m * nb;
+  double* S = nullptr;
+  int* info = nullptr;
+  if (hipMalloc((void**)&E->d_xi64, sizeof(double) * nw) != hipSuccess ||
+      hipMalloc((void**)&E->d_xi32, sizeof(float) * nw) != hipSuccess ||
+      hipMalloc((void**)&E->d_M64, sizeof(double) * (size_t)nb * c->max_bands) != hipSuccess ||
+      hipMalloc((void**)&E->d_M32, sizeof(float) * (size_t)nb * c->max_bands) != hipSuccess ||
+      hipMalloc((void**)&S, sizeof(double) * (size_t)nb * nb) != hipSuccess ||
+      hipMalloc((void**)&info, sizeof(int)) != hipSuccess) {
+    if (S) (void)hipFree(S);
+    if (info) (void)hipFree(info);
+    ace_free(E);
+    return fail(PW_ERR_NOMEM, "pw_build_ace: cannot allocate the compressed operator (%d bands)", nb);
+  }
+  auto cleanup = [&](int rc) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(S);
+    (void)hipFree(info);
+    if (rc != PW_OK) ace_free(E);
+    return rc;
+  };
+  hipStream_t s = c->stream;
+  const double one = 1.0;
+  int rc = PW_OK;
+  if (hipMemsetAsync(E->d_xi64, 0, sizeof(double) * nw, s) != hipSuccess)
+    return cleanup(fail(PW_ERR_HIP, "pw_build_ace: memset failed"));
+  if ((rc = exx_add<double>(c, nb, psi, E->d_xi64)) != PW_OK) return cleanup(rc);
+  if ((rc = gemm_tn<double>(c, m, nb, nb, (const double*)psi, m, E->d_xi64, m, S, nb, nullptr, 0.0)) != PW_OK)
+    return cleanup(rc);
+  hipLaunchKernelGGL(k_exx_negate<double>, dim3(grid_of((long)nb * nb)), dim3(256), 0, s, S, (long)nb * nb);
+  if (rocsolver_dpotrf(c->blas, rocblas_fill_upper, nb, S, nb, info) != rocblas_status_success)
+    return cleanup(fail(PW_ERR_HIP, "pw_build_ace: rocsolver_dpotrf failed"));
+  int hinfo = 0;
+  if (hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return cleanup(fail(PW_ERR_HIP, "pw_build_ace: reading the Cholesky status failed"));
+  if (hinfo != 0)
+    return cleanup(fail(PW_ERR_LINALG, "Potrf: -psi^T V_x psi is not positive definite (info = %d)", hinfo));
+  if (rocblas_dtrsm(c->blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                    rocblas_diagonal_non_unit, m, nb, &one, S, nb, E->d_xi64, m) != rocblas_status_success)
+    return cleanup(fail(PW_ERR_HIP, "pw_build_ace: rocblas_dtrsm failed"));
+  hipLaunchKernelGGL((k_exx_convert<double, float>), dim3(grid_of(nw)), dim3(256), 0, s, E->d_xi32,
+                     (const double*)E->d_xi64, nw);
+  if (hipGetLastError() != hipSuccess) return cleanup(fail(PW_ERR_HIP, "pw_build_ace: kernel launch failed"));
+  E->nace = nb;
+  return cleanup(PW_OK);
 }
 
 extern "C" int pw_add_exx(pw_ctx* c, int dtype, int nb, const void* psi, void* a3) {

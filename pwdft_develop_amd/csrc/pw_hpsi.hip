@@ -676,7 +676,7 @@ extern "C" int pw_mult_spinor(pw_ctx* c, int dtype, int nb, const void* psi, voi
   if (psi == a3) return fail(PW_ERR_ARG, "pw_mult_spinor: psi and a3 must not alias");
   int rc = dtype == PW_F64 ? local_kinetic<double>(c, nb, psi, a3) : local_kinetic<float>(c, nb, psi, a3);
   // hybrid functional with EXX active: + V_x psi (hamiltonian.cpp:3431-3448)
-  if (rc == PW_OK && c->exx) rc = dtype == PW_F64 ? exx_add<double>(c, nb, psi, a3) : exx_add<float>(c, nb, psi, a3);
+  if (rc == PW_OK && c->exx) rc = dtype == PW_F64 ? exchange_add<double>(c, nb, psi, a3) : exchange_add<float>(c, nb, psi, a3);
   if (rc == PW_OK) PW_TRY(mark(c, 4));
   return rc;
 }
@@ -695,7 +695,7 @@ extern "C" int pw_apply_h(pw_ctx* c, int dtype, int nb, const void* X, void* HX)
   if (nb == 0) return PW_OK;
   if (X == HX) return fail(PW_ERR_ARG, "pw_apply_h: X and HX must not alias");
   int rc = dtype == PW_F64 ? local_kinetic<double>(c, nb, X, HX) : local_kinetic<float>(c, nb, X, HX);
-  if (rc == PW_OK && c->exx) rc = dtype == PW_F64 ? exx_add<double>(c, nb, X, HX) : exx_add<float>(c, nb, X, HX);
+  if (rc == PW_OK && c->exx) rc = dtype == PW_F64 ? exchange_add<double>(c, nb, X, HX) : exchange_add<float>(c, nb, X, HX);
   if (rc != PW_OK) return rc;
   rc = dtype == PW_F64 ? nonlocal<double>(c, nb, X, HX, true) : nonlocal<float>(c, nb, X, HX, true);
   if (rc != PW_OK) return rc;

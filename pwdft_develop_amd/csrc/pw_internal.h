@@ -105,6 +105,10 @@ int sticks_to_sphere_add(pw_ctx* c, int nb, const void* t2, void* out);
 struct ExxState;
 template <typename T>
 int exx_add(pw_ctx* c, int nb, const void* psi, void* a3);
+// The exchange term of MultSpinor: the compressed operator when built,
+// else exx_add.
+template <typename T>
+int exchange_add(pw_ctx* c, int nb, const void* psi, void* a3);
 void exx_free(pw_ctx* c);
 }  // namespace pw
 

@@ -140,6 +140,12 @@ class Context:
     def add_exx(self, x, out=None):
         return self._binop("pw_add_exx", x, out, zero_out=True)
 
+    def build_ace(self, psi=None):
+        """Compress the exchange operator on the FP64 device block psi
+        (None drops it); apply_h/mult_spinor then use the compressed form."""
+        n = 0 if psi is None else psi.shape[0]
+        check(_lib.lib().pw_build_ace(self.h, n, C.c_void_p(psi.data_ptr()) if n else None))
+
     # ---- inputs that the adjoint differentiates ----
     def set_vtot(self, vtot):
         v = np.ascontiguousarray(np.asarray(vtot, dtype=np.float64).reshape(-1))

@@ -320,3 +320,29 @@ def test_exx_against_numpy_and_symmetry():
     assert np.abs(got - ref).max() < 1e-12 * np.abs(ref).max()
     lhs = (np.conj(psi) @ got.T).real
     assert np.abs(lhs - lhs.T).max() < 1e-12
+
+
+def test_ace_reproduces_exchange_on_its_block():
+    """ACE (hamiltonian.cpp:3685-3924): -xi xi^T psi = V_x psi for every
+    vector in the span of the block it was built from, so MultSpinor with ACE
+    equals MultSpinor with the full exchange there; outside the span it
+    differs.  Building from an empty block drops the operator."""
+    prob = W.Problem("exx12", (12, 12, 12), W.cubic(7.0), 5.0, W.DIAMOND[:2] * 7.0, 6).build()
+    H = O.Hamiltonian(prob)
+    occ = np.array([1.0, 1.0, 0.5, 0.25])
+    H.set_exx(prob.ecut, 0.106, 0.25, prob.psi[:4], occ)
+    full = H.mult_spinor(prob.psi)
+    rng = np.random.default_rng(3)
+    comb = rng.standard_normal((3, prob.nb)) @ prob.psi
+    full_comb = H.mult_spinor(comb)
+    outside = rng.standard_normal((2, prob.npw)) + 1j * rng.standard_normal((2, prob.npw))
+    outside[:, 0] = outside[:, 0].real
+    full_out = H.mult_spinor(outside)
+    H.build_ace(prob.psi)
+    ace = H.mult_spinor(prob.psi)
+    scale = np.abs(full).max()
+    assert np.abs(ace - full).max() < 1e-12 * scale
+    assert np.abs(H.mult_spinor(comb) - full_comb).max() < 1e-11 * np.abs(full_comb).max()
+    assert np.abs(H.mult_spinor(outside) - full_out).max() > 1e-6 * np.abs(full_out).max()
+    H.build_ace(np.zeros((0, prob.npw), complex))
+    assert np.abs(H.mult_spinor(prob.psi) - full).max() < 1e-13 * scale
