@@ -217,6 +217,15 @@ int pw_exx_divergence(const int grid[3], const double supercell[9], double ecut,
 int pw_density(pw_ctx* ctx, int dtype, int nb, const void* psi, const double* f,
                double* rho, int accumulate);
 
+/* KohnSham::CalculateDensity, sphere branch, one density component
+ * (hamiltonian.cpp:2453-2627): the accumulation above with
+ * f_i = num_spin * occ_i, then the renormalisation
+ * rho *= num_spin * num_occupied * N / (vol * sum(rho)) (:2599-2600).
+ * *val (may be NULL) = sum(rho) vol / N after it, the reference's val
+ * (SCF's totalCharge_).  occ: host, nb values; rho: device, N doubles. */
+int pw_calculate_density(pw_ctx* ctx, int dtype, int nb, const void* psi, const double* occ,
+                         int num_spin, int num_occupied, double* rho, double* val);
+
 /* E = sum_i f_i Re<x_i|H x_i> (FP64 block, host f and E). */
 int pw_band_energy(pw_ctx* ctx, int nb, const void* X, const double* f, double* E);
 

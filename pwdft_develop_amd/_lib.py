@@ -70,6 +70,8 @@ def lib():
         "pw_set_graphs": [_vp, C.c_int],
         "pw_density": [_vp, C.c_int, C.c_int, _vp, _dp, _vp, C.c_int],
         "pw_band_energy": [_vp, C.c_int, _vp, _dp, C.POINTER(C.c_double)],
+        "pw_calculate_density": [_vp, C.c_int, C.c_int, _vp, _dp, C.c_int, C.c_int, _vp,
+                                 C.POINTER(C.c_double)],
         "pw_nonlocal_force": [_vp, C.c_int, _vp, _dp, C.c_int, C.c_int, _dp],
         "pw_set_exx": [_vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _vp, _dp],
         "pw_add_exx": [_vp, C.c_int, C.c_int, _vp, _vp],

@@ -452,6 +452,25 @@ __global__ void k_sum_partials(const double* __restrict__ part, int nsplit, long
   }
 }
 
+// Per-block partial sums of x (fixed order: the host adds the partials in
+// block order, so the total is reproducible run to run).
+__global__ void k_block_sums(const double* __restrict__ x, long n, double* __restrict__ part) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) s += x[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = blockDim.x / 2; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_scale(double* __restrict__ x, long n, double s) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] *= s;
+}
+
 // F[a][d] = -2 sum_i f_i sum_{p in a} w_p W[p,i] D_d[p,i]; one block per (a, d)
 __global__ void k_force_reduce(const double* __restrict__ W, const double* __restrict__ D, const double* __restrict__ w,
                                const double* __restrict__ f, int nbeta, int nb, int npa, double* __restrict__ F) {
@@ -780,6 +799,49 @@ extern "C" int pw_density(pw_ctx* c, int dtype, int nb, const void* psi, const d
   if (dtype == PW_F64) return density_impl<double>(c, nb, psi, f, rho, accumulate);
   if (dtype == PW_F32) return density_impl<float>(c, nb, psi, f, rho, accumulate);
   return fail(PW_ERR_ARG, "unknown dtype %d", dtype);
+}
+
+namespace pw {
+static int device_sum(pw_ctx* c, const double* x, long n, double* out) {
+  const int nblk = 256;
+  double* part = nullptr;
+  if (hipMalloc((void**)&part, sizeof(double) * nblk) != hipSuccess)
+    return fail(PW_ERR_NOMEM, "device_sum: scratch allocation failed");
+  hipLaunchKernelGGL(k_block_sums, dim3(nblk), dim3(256), 0, c->stream, x, n, part);
+  double h[nblk];
+  const bool ok = hipGetLastError() == hipSuccess &&
+                  hipMemcpyAsync(h, part, sizeof(h), hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+                  hipStreamSynchronize(c->stream) == hipSuccess;
+  (void)hipFree(part);
+  if (!ok) return fail(PW_ERR_HIP, "device_sum: reduction failed");
+  double s = 0.0;
+  for (int i = 0; i < nblk; i++) s += h[i];
+  *out = s;
+  return PW_OK;
+}
+}  // namespace pw
+
+// KohnSham::CalculateDensity, sphere branch with one density component
+// (src/SCF/hamiltonian.cpp:2453-2627): rho = sum_i numSpin occ_i
+// (C2R(c_i)/sqrt(N))^2, then rho *= numSpin nocc N / (vol sum(rho))
+// (:2599-2600); *val = sum(rho) vol / N after the scaling (:2604-2607).
+extern "C" int pw_calculate_density(pw_ctx* c, int dtype, int nb, const void* psi, const double* occ, int num_spin,
+                                    int num_occupied, double* rho, double* val) {
+  if (!c || !psi || !occ || !rho || nb < 1 || nb > c->max_bands || num_spin < 1 || num_occupied < 0)
+    return fail(PW_ERR_ARG, "pw_calculate_density: bad argument");
+  std::vector<double> f(occ, occ + nb);
+  for (auto& x : f) x *= num_spin;
+  PW_TRY(pw_density(c, dtype, nb, psi, f.data(), rho, 0));
+  double s = 0.0;
+  PW_TRY(device_sum(c, rho, c->N, &s));
+  if (!(s > 0.0)) return fail(PW_ERR_DOMAIN, "CalculateDensity: the density sums to %g", s);
+  const double scale = (double)num_spin * (double)num_occupied * (double)c->N / (c->vol * s);
+  hipLaunchKernelGGL(k_scale, dim3((int)std::min<long>((c->N + 255) / 256, 4096)), dim3(256), 0, c->stream, rho,
+                     c->N, scale);
+  PW_HIP(hipGetLastError());
+  PW_TRY(device_sum(c, rho, c->N, &s));
+  if (val) *val = s * c->vol / (double)c->N;
+  return PW_OK;
 }
 
 extern "C" int pw_band_energy(pw_ctx* c, int nb, const void* X, const double* f, double* E) {

@@ -173,6 +173,18 @@ class Context:
                                     1 if accumulate else 0))
         return out
 
+    def calculate_density(self, x, occ, num_spin=2, num_occupied=None):
+        """KohnSham::CalculateDensity: renormalised density (device, N) and
+        its integral sum(rho) vol / N."""
+        t = self.torch
+        nocc = x.shape[0] if num_occupied is None else num_occupied
+        out = t.empty(int(np.prod(self.prob.grid)), dtype=t.float64, device=self.device)
+        val = C.c_double()
+        check(_lib.lib().pw_calculate_density(self.h, _dtype_code(x), x.shape[0], C.c_void_p(x.data_ptr()),
+                                              np.ascontiguousarray(occ, np.float64), num_spin, nocc,
+                                              C.c_void_p(out.data_ptr()), C.byref(val)))
+        return out, val.value
+
     def band_energy(self, x, f):
         e = C.c_double()
         check(_lib.lib().pw_band_energy(self.h, x.shape[0], C.c_void_p(x.data_ptr()),

@@ -71,3 +71,23 @@ def test_autograd_against_finite_differences(r16):
         fd = (vals[0] - vals[1]) / (2 * hr)
         assert abs(fd - gR[a, d]) < 1e-6 * max(1.0, abs(gR[a, d])), (fd, gR[a, d])
     ctx.close()
+
+
+def test_calculate_density_normalisation(r16):
+    """KohnSham::CalculateDensity (hamiltonian.cpp:2453-2627): the GPU density
+    is the oracle's accumulation rescaled so that sum(rho) vol / N equals
+    numSpin * numOccupiedState (the reference's val), for uneven
+    occupations."""
+    from pwdft_develop_amd.ops import Context
+    print("case: rand16_48 calculate_density", flush=True)
+    occ = np.linspace(1.0, 0.0, r16.nb)
+    ctx = Context(r16)
+    rho, val = ctx.calculate_density(ctx.to_device(r16.psi), occ, num_spin=2, num_occupied=64)
+    rho = rho.cpu().numpy()
+    N, vol = r16.ngrid, r16.volume
+    assert abs(val - 2 * 64) < 1e-10 * 128
+    assert abs(rho.sum() * vol / N - 128) < 1e-10 * 128
+    raw = O.Hamiltonian(r16).density(r16.psi, 2 * occ)
+    ref = raw * (2 * 64 * N / (vol * raw.sum()))
+    assert np.abs(rho - ref).max() < 1e-11 * np.abs(ref).max()
+    ctx.close()
