@@ -127,6 +127,15 @@ int pw_teter_precond(pw_ctx* ctx, int dtype, int nb, const void* psi, void* a3);
  * X <- X U^{-1}.  PW_ERR_LINALG when S is not positive definite. */
 int pw_orthogonalize(pw_ctx* ctx, int nb, void* X);
 
+/* lapack::Syevd('V', 'U', n, A, lda, w) (include/lapack.hpp:107-108) on the
+ * device: the dense symmetric eigensolver every Rayleigh-Ritz and
+ * SolveGenEig call uses.  A (device FP64, upper triangle read) <- the
+ * eigenvectors, each with its largest-magnitude component positive (first
+ * index on ties); w (device, n) <- eigenvalues ascending.  Householder
+ * tridiagonalisation, multisection and inverse iteration for n <= 1024,
+ * rocSOLVER's dsyevd beyond. */
+int pw_syevd(pw_ctx* ctx, int n, double* A, int lda, double* w);
+
 /* k-step Lanczos on the device vector v0 (one band, FP64): Ritz values of
  * T_k ascending (host theta[k]) and the upper bound theta_k + |beta_k|.
  * Spectral-bound estimator of the ChebFSI steps the reference only names

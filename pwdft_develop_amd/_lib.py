@@ -68,6 +68,7 @@ def lib():
         "pw_chebfsi_sweep": [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _dp, _dp, _dp],
         "pw_free_workspace": [_vp],
         "pw_set_graphs": [_vp, C.c_int],
+        "pw_syevd": [_vp, C.c_int, _vp, C.c_int, _vp],
         "pw_density": [_vp, C.c_int, C.c_int, _vp, _dp, _vp, C.c_int],
         "pw_band_energy": [_vp, C.c_int, _vp, _dp, C.POINTER(C.c_double)],
         "pw_calculate_density": [_vp, C.c_int, C.c_int, _vp, _dp, C.c_int, C.c_int, _vp,

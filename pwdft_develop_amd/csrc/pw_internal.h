@@ -90,10 +90,11 @@ int gemm_tn(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, i
 template <typename T>
 int gemm_nn(pw_ctx* c, int m, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T beta, T* Y,
             int ldy);
-// Dense symmetric eigensolvers (pw_eig.hip, pw_solver.hip): eig_sym writes
-// eigenvalues ascending to w (host) and eigenvectors to Z; sym_eig works in
-// place (A <- vectors, w device) with the sign convention of the oracle.
-int eig_sym(pw_ctx* c, int n, const double* A, int lda, double* w, double* Z, int ldz);
+// Dense symmetric eigensolvers (pw_eig.hip, pw_solver.hip), in place on the
+// upper triangle of A (A <- eigenvectors, w (device) <- eigenvalues
+// ascending); sym_eig adds the sign convention of the oracle and takes
+// rocSOLVER beyond eig_sym's n <= 1024.
+int eig_sym(pw_ctx* c, int n, double* A, int lda, double* w);
 int sym_eig(pw_ctx* c, int n, double* A, int lda, double* w, double* work, int* info);
 // Stick transforms of the wavefunction sphere (pw_hpsi.hip).
 template <typename T>
@@ -151,6 +152,8 @@ struct pw_ctx {
   void* d_ws[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   double *d_S = nullptr, *d_S2 = nullptr, *d_D = nullptr, *d_E = nullptr, *d_red = nullptr;
   double* d_lz = nullptr;  // Lanczos vectors v, v_prev, f (3 x 2 npw)
+  void* d_eig_ws = nullptr;  // dense eigensolver scratch, grown on demand
+  size_t eig_ws_bytes = 0;
   int* d_info = nullptr;
   int ws_nb = 0;
   // Chebyshev recurrence coefficients (3 per step, degree <= 64) and the

@@ -144,26 +144,14 @@ int sym_eig(pw_ctx* c, int n, double* A, int lda, double* w, double* work, int* 
     const char* e = getenv("PW_EIGEN");
     return e && std::string(e) == "rocsolver";
   }();
-  if (use_rocsolver) {
+  if (use_rocsolver || n > 1024) {
     PW_BLAS(rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_upper, n, A, lda, w, work, info));
     int h = 0;
     PW_HIP(hipMemcpyAsync(&h, info, sizeof(int), hipMemcpyDeviceToHost, s));
     PW_HIP(hipStreamSynchronize(s));
     if (h != 0) return fail(PW_ERR_LINALG, "Syevd failed (info = %d)", h);
   } else {
-    std::vector<double> wh(n);
-    double* Z = nullptr;
-    PW_HIP(hipMalloc((void**)&Z, sizeof(double) * (size_t)n * n));
-    int rc = eig_sym(c, n, A, lda, wh.data(), Z, n);
-    if (rc == PW_OK) {
-      if (hipMemcpy2DAsync(A, sizeof(double) * lda, Z, sizeof(double) * n, sizeof(double) * n, n,
-                           hipMemcpyDeviceToDevice, s) != hipSuccess ||
-          hipMemcpyAsync(w, wh.data(), sizeof(double) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
-          hipStreamSynchronize(s) != hipSuccess)
-        rc = fail(PW_ERR_HIP, "sym_eig: copy failed");
-    }
-    (void)hipFree(Z);
-    if (rc != PW_OK) return rc;
+    PW_TRY(eig_sym(c, n, A, lda, w));
   }
   hipLaunchKernelGGL(k_sign_fix, dim3(n), dim3(256), 0, s, A, n, lda);
   return PW_OK;
@@ -205,6 +193,9 @@ extern "C" int pw_free_workspace(pw_ctx* c) {
   c->d_coef_f = nullptr;
   c->d_info = nullptr;
   c->ws_nb = 0;
+  if (c->d_eig_ws) (void)hipFree(c->d_eig_ws);
+  c->d_eig_ws = nullptr;
+  c->eig_ws_bytes = 0;
   // the captured filter graph refers to the workspace: drop it too
   if (c->fg_exec) (void)hipGraphExecDestroy(c->fg_exec);
   c->fg_exec = nullptr;
@@ -444,6 +435,21 @@ extern "C" int pw_cheb_filter(pw_ctx* c, int dtype, int nb, int m, double a, dou
   if (X != result) PW_HIP(hipMemcpyAsync(X, result, bytes, hipMemcpyDeviceToDevice, s));
   PW_HIP(hipGetLastError());
   return PW_OK;
+}
+
+extern "C" int pw_syevd(pw_ctx* c, int n, double* A, int lda, double* w) {
+  if (!c || !A || !w || n < 1 || lda < n) return fail(PW_ERR_ARG, "pw_syevd: bad argument");
+  double* work = nullptr;
+  int* info = nullptr;
+  if (hipMalloc((void**)&work, sizeof(double) * n) != hipSuccess || hipMalloc((void**)&info, sizeof(int)) != hipSuccess) {
+    if (work) (void)hipFree(work);
+    return fail(PW_ERR_NOMEM, "pw_syevd: scratch allocation failed");
+  }
+  int rc = sym_eig(c, n, A, lda, w, work, info);
+  if (hipStreamSynchronize(c->stream) != hipSuccess && rc == PW_OK) rc = fail(PW_ERR_HIP, "pw_syevd: kernel failed");
+  (void)hipFree(work);
+  (void)hipFree(info);
+  return rc;
 }
 
 extern "C" int pw_set_graphs(pw_ctx* c, int enable) {

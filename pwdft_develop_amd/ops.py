@@ -197,6 +197,17 @@ class Context:
                                            np.ascontiguousarray(f, np.float64), natom, npa, F))
         return F.reshape(natom, 3)
 
+    def syevd(self, A):
+        """lapack::Syevd('V', 'U') on a device FP64 square matrix (column-
+        major, i.e. A[j, i] in torch's row-major view is element (i, j)).
+        Returns (eigenvalues ascending, eigenvector matrix) as new tensors."""
+        t = self.torch
+        n = A.shape[0]
+        Z = A.detach().clone().contiguous()
+        w = t.empty(n, dtype=t.float64, device=self.device)
+        check(_lib.lib().pw_syevd(self.h, n, C.c_void_p(Z.data_ptr()), n, C.c_void_p(w.data_ptr())))
+        return w, Z
+
     def set_graphs(self, on=True):
         """Capture/replay the Chebyshev filter as a hipGraph (default on)."""
         check(_lib.lib().pw_set_graphs(self.h, 1 if on else 0))
