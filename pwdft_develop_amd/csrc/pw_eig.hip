@@ -8,21 +8,24 @@
 // runs inside one workgroup and everything else is parallel:
 //
 //   k_sym_from_upper  mirror the upper triangle (Syevd 'U' reads only it)
-//   k_sytrd           Householder tridiagonalisation A = Q T Q^T in the
-//                     order of LAPACK dsytd2 'L': n-2 steps of reflector,
-//                     symmetric matrix-vector product and rank-2 update,
-//                     one 1024-thread workgroup on the L2-resident matrix
+//   k_sytrd           Householder tridiagonalisation A = Q T Q^T, blocked
+//                     as LAPACK dsytrd/dlatrd 'L': per column a reflector
+//                     and a read-only symmetric product with the L2-resident
+//                     trailing matrix, per panel one rank-2 nbk update; one
+//                     1024-thread workgroup
 //   k_stebz           all eigenvalues of T: 64-way multisection on Sturm
 //                     counts, one wavefront per eigenvalue (LAPACK dstebz
 //                     tolerances)
 //   k_stein           eigenvectors of T: inverse iteration on LU with
-//                     partial pivoting of T - lambda I, modified
-//                     Gram-Schmidt within clusters of eigenvalues closer than
-//                     1e-3 ||T|| (LAPACK dstein's rule), one thread per
-//                     cluster, scratch interleaved by thread so that lanes
+//                     partial pivoting of T - lambda I, one thread per
+//                     eigenvalue; modified Gram-Schmidt within tight clusters
+//                     (gaps <= 1e-5 ||T||; LAPACK dstein uses 1e-3, which
+//                     chains the dense Rayleigh-Ritz spectra into long serial
+//                     runs); scratch interleaved by thread so that lanes
 //                     stepping through the same row coalesce
-//   k_ormtr           Z <- Q Z_T: every workgroup applies all reflectors to
-//                     its own block of 8 columns held in LDS
+//   k_ormtr           Z <- Q Z_T in compact-WY chunks (dlarft + dlarfb):
+//                     every workgroup applies all chunks to its own block of
+//                     8 columns held in LDS
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -37,15 +40,21 @@ constexpr int kTrdWaves = kTrdThreads / 64;
 constexpr int kOrmCols = 8;         // columns per k_ormtr workgroup
 constexpr int kEigMaxN = 1024;      // LDS bound of k_sytrd
 
+// Workgroup barrier for LDS traffic only: waits for this wavefront's LDS
+// operations but leaves its global loads and stores in flight (a full
+// __syncthreads also drains those).  Only for phases whose cross-wavefront
+// data goes through LDS.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // Block-wide sum, identical order on every call (deterministic), returned to
 // all threads.  sred holds one double per wavefront.
 template <int NT>
 __device__ __forceinline__ double block_sum(double v, double* sred) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  __syncthreads();
+  lds_barrier();
   if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = v;
-  __syncthreads();
+  lds_barrier();
   double s = 0.0;
 #pragma unroll
   for (int w = 0; w < NT / 64; w++) s += sred[w];
@@ -61,80 +70,178 @@ __global__ void k_sym_from_upper(double* __restrict__ A, int n, int lda) {
 
 // A (full symmetric storage) -> T = Q^T A Q: d (n), e (n-1), tau (n-1); the
 // reflector v_k (v_k[0] = 1) overwrites A[k+1:n, k].  H_k = I - tau_k v v^T.
+//
+// Blocked as LAPACK dsytrd/dlatrd ('L'), panels of nbk columns: inside a
+// panel the trailing matrix is only read.  Column k is first brought up to
+// date with the panel's earlier reflectors, then p = A22 v - V (W^T v) -
+// W (V^T v) from the stale A22 and the panel's V, W (kept in LDS), and the
+// rank-2 nbk update A22 -= V W^T + W V^T runs once per panel.  The symmetric
+// product streams A22 from L2 in batches of 16 independent loads per lane;
+// its waves split rows in 64-row chunks and columns in G groups with
+// G * chunks <= 16, so the partial sums fit 1024 doubles.
 __global__ void __launch_bounds__(kTrdThreads)
-k_sytrd(double* __restrict__ A, int n, int lda, double* __restrict__ d, double* __restrict__ e,
+k_sytrd(double* __restrict__ A, int n, int lda, int nbk, double* __restrict__ d, double* __restrict__ e,
         double* __restrict__ tau) {
   extern __shared__ double sm[];
-  double* sv = sm;           // v, n
-  double* sw = sv + n;       // p then w, n
-  double* red = sw + n;      // per-wave partial products, kTrdWaves x n
+  double* Vp = sm;                       // nbk columns of n (global row index)
+  double* Wp = Vp + (size_t)nbk * n;
+  double* sv = Wp + (size_t)nbk * n;    // v, n (local row index i = r - k - 1)
+  double* red = sv + n;                  // 1024 partial sums
+  double* al = red + 1024;               // W^T v, nbk
+  double* be = al + nbk;                 // V^T v, nbk
   __shared__ double sred[kTrdWaves];
+  __shared__ double sbc[2];              // alpha, diagonal of column k
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  for (int k = 0; k < n - 2; k++) {
-    const int m = n - k - 1;
-    double* col = A + (size_t)lda * k + (k + 1);
-    double* A22 = A + (size_t)lda * (k + 1) + (k + 1);
-    double s = 0.0;
-    for (int i = 1 + t; i < m; i += kTrdThreads) {
-      const double x = col[i];
-      s += x * x;
-    }
-    s = block_sum<kTrdThreads>(s, sred);
-    const double alpha = col[0];
-    double tk = 0.0, beta = alpha, scal = 0.0;
-    if (s > 0.0) {
-      beta = -copysign(sqrt(alpha * alpha + s), alpha);
-      tk = (beta - alpha) / beta;
-      scal = 1.0 / (alpha - beta);
-    }
-    __syncthreads();  // every thread has read col[0] before it is overwritten
-    for (int i = t; i < m; i += kTrdThreads) {
-      const double vi = (i == 0) ? 1.0 : col[i] * scal;
-      sv[i] = vi;
-      col[i] = vi;
-    }
-    if (t == 0) {
-      d[k] = A[(size_t)lda * k + k];
-      e[k] = beta;
-      tau[k] = tk;
-    }
-    __syncthreads();
-    if (tk == 0.0) continue;  // block-uniform
-    // p = tau A22 v: wave wv sums columns j = wv (mod waves), lanes run down rows
-    for (int c0 = 0; c0 < m; c0 += 64) {
-      const int i = c0 + lane;
-      double acc = 0.0;
-      if (i < m) {
-#pragma unroll 4
-        for (int j = wv; j < m; j += kTrdWaves) acc += A22[(size_t)lda * j + i] * sv[j];
-        red[wv * n + i] = acc;
+  for (int p0 = 0; p0 < n - 2; p0 += nbk) {
+    const int nbp = min(nbk, n - 2 - p0);
+    // column p0 (rows p0.. n-1) for the first step; later columns are
+    // prefetched one step ahead (stale within the panel, so the early copy is
+    // the value the step needs).  n <= 1024 rows: one per thread.
+    double xpre = (p0 + t < n) ? A[(size_t)lda * p0 + p0 + t] : 0.0;
+    for (int jj = 0; jj < nbp; jj++) {
+      const int k = p0 + jj, m = n - k - 1;
+      double* colk = A + (size_t)lda * k;
+      // column k (rows k..n-1) brought up to date, and ||x[1:]||^2 of the
+      // reflector input x = A[k+1:, k].  Thread t owns row k + t in every
+      // phase of the step, so its global writes are read back only by itself.
+      double s = 0.0;
+      {
+        const int r = k + t;
+        double x = xpre;
+        if (r < n) {
+          if (jj > 0) {
+            double u = 0.0;
+            for (int q = 0; q < jj; q++) u += Vp[q * n + r] * Wp[q * n + k] + Wp[q * n + r] * Vp[q * n + k];
+            x -= u;
+          }
+          if (r >= k + 2) s += x * x;
+          if (r == k) sbc[1] = x;
+          if (r == k + 1) sbc[0] = x;
+        }
+        xpre = x;  // the updated entry, reused below by the same thread
       }
-    }
-    __syncthreads();
-    double pv = 0.0;
-    for (int i = t; i < m; i += kTrdThreads) {
-      double p = 0.0;
-#pragma unroll
-      for (int w2 = 0; w2 < kTrdWaves; w2++) p += red[w2 * n + i];
-      p *= tk;
-      sw[i] = p;
-      pv += p * sv[i];
-    }
-    pv = block_sum<kTrdThreads>(pv, sred);
-    const double h = 0.5 * tk * pv;
-    for (int i = t; i < m; i += kTrdThreads) sw[i] -= h * sv[i];
-    __syncthreads();
-    // A22 -= v w^T + w v^T
-    for (int c0 = 0; c0 < m; c0 += 64) {
-      const int i = c0 + lane;
-      if (i < m) {
-        const double vi = sv[i], wi = sw[i];
-#pragma unroll 4
-        for (int j = wv; j < m; j += kTrdWaves) {
-          double* a = A22 + (size_t)lda * j + i;
-          *a -= vi * sw[j] + wi * sv[j];
+      s = block_sum<kTrdThreads>(s, sred);
+      const double alpha = sbc[0];
+      double tk = 0.0, beta = alpha, scal = 0.0;
+      if (s > 0.0) {
+        beta = -copysign(sqrt(alpha * alpha + s), alpha);
+        tk = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+      }
+      if (t == 0) {
+        d[k] = sbc[1];
+        e[k] = beta;
+        tau[k] = tk;
+      }
+      {
+        const int r = k + t;
+        if (r > k && r < n) {
+          const double vi = (r == k + 1) ? 1.0 : xpre * scal;
+          sv[r - k - 1] = vi;
+          colk[r] = vi;  // reflector storage, read only by the back-transform
+          Vp[jj * n + r] = vi;
         }
       }
+      lds_barrier();
+      if (tk == 0.0) {  // block-uniform: H_k = I, w = 0
+        for (int r = k + 1 + t; r < n; r += kTrdThreads) Wp[jj * n + r] = 0.0;
+        if (jj + 1 < nbp) xpre = (k + 1 + t < n) ? A[(size_t)lda * (k + 1) + k + 1 + t] : 0.0;
+        lds_barrier();
+        continue;
+      }
+      // partial sums of A22 v (stale trailing block) and the panel dots
+      const int nrc = (m + 63) >> 6;
+      const int G = max(1, kTrdWaves / nrc);
+      const double* A22 = A + (size_t)lda * (k + 1) + (k + 1);
+      if (wv < nrc * G) {
+        const int rc = wv / G, g = wv - rc * G;
+        const int i = rc * 64 + lane;
+        double acc = 0.0;
+        if (i < m) {
+          for (int j = g; j < m; j += 16 * G) {
+            double a[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+              const int jc = j + u * G;
+              a[u] = jc < m ? A22[(size_t)lda * jc + i] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+              const int jc = j + u * G;
+              if (jc < m) acc += a[u] * sv[jc];
+            }
+          }
+        }
+        red[g * (nrc * 64) + i] = acc;
+      }
+      // next column, issued after this step's matrix loads have been consumed
+      if (jj + 1 < nbp) xpre = (k + 1 + t < n) ? A[(size_t)lda * (k + 1) + k + 1 + t] : 0.0;
+      for (int q = wv; q < jj; q += kTrdWaves) {
+        double sa = 0.0, sb = 0.0;
+        for (int i = lane; i < m; i += 64) {
+          const double vi = sv[i];
+          sa += Wp[q * n + k + 1 + i] * vi;
+          sb += Vp[q * n + k + 1 + i] * vi;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+          sa += __shfl_xor(sa, o, 64);
+          sb += __shfl_xor(sb, o, 64);
+        }
+        if (lane == 0) {
+          al[q] = sa;
+          be[q] = sb;
+        }
+      }
+      lds_barrier();
+      // p = tau (A22 v - V al - W be), w = p - (tau/2)(p.v) v
+      double pv = 0.0;
+      double pl[(1024 + kTrdThreads - 1) / kTrdThreads];
+      for (int i = t, c = 0; i < m; i += kTrdThreads, c++) {
+        double p = 0.0;
+        for (int g2 = 0; g2 < G; g2++) p += red[g2 * (nrc * 64) + i];
+        const int r = k + 1 + i;
+        for (int q = 0; q < jj; q++) p -= Vp[q * n + r] * al[q] + Wp[q * n + r] * be[q];
+        p *= tk;
+        pl[c] = p;
+        pv += p * sv[i];
+      }
+      pv = block_sum<kTrdThreads>(pv, sred);
+      const double h = 0.5 * tk * pv;
+      for (int i = t, c = 0; i < m; i += kTrdThreads, c++) Wp[jj * n + k + 1 + i] = pl[c] - h * sv[i];
+      lds_barrier();
+    }
+    // the trailing update below reads V, W from LDS and rewrites A in global
+    // memory; the full barrier after it publishes A to every wavefront
+    // trailing update of rows and columns >= p1 in 4 x 4 register tiles
+    const int p1 = p0 + nbp, mt = n - p1, nt = (mt + 3) >> 2;
+    for (int tile = t; tile < nt * nt; tile += kTrdThreads) {
+      const int r0 = p1 + 4 * (tile % nt), c0 = p1 + 4 * (tile / nt);
+      double acc[4][4];
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = 0.0;
+      for (int q = 0; q < nbp; q++) {
+        double vr[4], wr[4], vc[4], wc[4];
+#pragma unroll
+        for (int a = 0; a < 4; a++) {
+          const bool okr = r0 + a < n, okc = c0 + a < n;
+          vr[a] = okr ? Vp[q * n + r0 + a] : 0.0;
+          wr[a] = okr ? Wp[q * n + r0 + a] : 0.0;
+          vc[a] = okc ? Vp[q * n + c0 + a] : 0.0;
+          wc[a] = okc ? Wp[q * n + c0 + a] : 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+#pragma unroll
+          for (int b = 0; b < 4; b++) acc[a][b] += vr[a] * wc[b] + wr[a] * vc[b];
+      }
+#pragma unroll
+      for (int b = 0; b < 4; b++)
+#pragma unroll
+        for (int a = 0; a < 4; a++)
+          if (r0 + a < n && c0 + b < n) A[(size_t)lda * (c0 + b) + r0 + a] -= acc[a][b];
     }
     __syncthreads();
   }
@@ -249,137 +356,277 @@ __device__ __forceinline__ double start_value(int j, int i) {
   return (double)h * (2.0 / 4294967296.0) - 1.0;
 }
 
-// Eigenvectors of T (column jj of Zt, ld n) for the cluster that starts at
-// thread j.  Scratch: 6 arrays of n rows, entry (a, i) of thread q at
-// S[(a * n + i) * nthr + q].
+// Per-thread rows of the inverse-iteration scratch: array a of thread q at
+// S + (a * n) * nthr, row i at [i * nthr + q] (lanes of a wavefront stepping
+// through the same row touch consecutive addresses).
+struct SteinRows {
+  double* __restrict__ r0;   // 1 / pivot
+  double* __restrict__ u1;   // first superdiagonal of U
+  double* __restrict__ u2;   // second superdiagonal of U (row interchanges)
+  double* __restrict__ ml;   // multiplier
+  double* __restrict__ pv;   // 1 where rows i, i+1 were interchanged
+  double* __restrict__ x;    // iterate
+  double* __restrict__ y;    // forward-eliminated right-hand side
+};
+
+// LU with partial pivoting of T - lam I (dlagtf's elimination order).
+// Branch-free (selects), so that unrolled iterations can issue their
+// memory operations early.
+__device__ void stein_factor(const double* d, const double* e, int n, double lam, double pert, SteinRows R,
+                             int nthr, int q) {
+  double c = d[0] - lam, s1 = n > 1 ? e[0] : 0.0;
+#pragma unroll 4
+  for (int i = 0; i < n - 1; i++) {
+    const double sub = e[i], dn = d[i + 1] - lam, sup = (i + 1 < n - 1) ? e[i + 1] : 0.0;
+    const size_t o = (size_t)i * nthr + q;
+    const bool sw = fabs(sub) > fabs(c);
+    // without interchange the pivot c is kept away from zero
+    const double cp = (!sw && fabs(c) < pert) ? (c >= 0.0 ? pert : -pert) : c;
+    const double piv = sw ? sub : cp;
+    const double mlt = (sw ? cp : sub) / piv;
+    R.r0[o] = 1.0 / piv;
+    R.u1[o] = sw ? dn : s1;
+    R.u2[o] = sw ? sup : 0.0;
+    R.ml[o] = mlt;
+    R.pv[o] = sw ? 1.0 : 0.0;
+    const double cn = sw ? s1 - mlt * dn : dn - mlt * s1;
+    s1 = sw ? -mlt * sup : sup;
+    c = cn;
+  }
+  if (fabs(c) < pert) c = c >= 0.0 ? pert : -pert;
+  const size_t o = (size_t)(n - 1) * nthr + q;
+  R.r0[o] = 1.0 / c; R.u1[o] = 0.0; R.u2[o] = 0.0;
+}
+
+// x <- (T - lam I)^{-1} (x * scale); returns ||x||^2 of the result.  Rows
+// go in batches of kSteinB: all loads of a batch are issued before any of
+// its stores, so a batch costs one memory round trip (the compiler cannot
+// reorder across the previous batch's stores by itself).
+constexpr int kSteinB = 16;
+__device__ double stein_solve(int n, double scale, SteinRows R, int nthr, int q) {
+  double y0 = R.x[q] * scale;
+  for (int i0 = 0; i0 < n - 1; i0 += kSteinB) {
+    double xb[kSteinB], mb[kSteinB], pb[kSteinB];
+#pragma unroll
+    for (int u = 0; u < kSteinB; u++) {
+      const int i = i0 + u;
+      const size_t o = (size_t)i * nthr + q;
+      const bool ok = i < n - 1;
+      xb[u] = ok ? R.x[o + nthr] : 0.0;
+      mb[u] = ok ? R.ml[o] : 0.0;
+      pb[u] = ok ? R.pv[o] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSteinB; u++) {
+      const int i = i0 + u;
+      if (i < n - 1) {
+        const double y1 = xb[u] * scale, mlt = mb[u];
+        const bool sw = pb[u] != 0.0;
+        R.y[(size_t)i * nthr + q] = sw ? y1 : y0;
+        y0 = sw ? y0 - mlt * y1 : y1 - mlt * y0;
+      }
+    }
+  }
+  R.y[(size_t)(n - 1) * nthr + q] = y0;
+  double xn1 = 0.0, xn2 = 0.0, ss = 0.0;
+  for (int i0 = n - 1; i0 >= 0; i0 -= kSteinB) {
+    double yb[kSteinB], ab[kSteinB], bb[kSteinB], rb[kSteinB];
+#pragma unroll
+    for (int u = 0; u < kSteinB; u++) {
+      const int i = i0 - u;
+      const size_t o = (size_t)(i >= 0 ? i : 0) * nthr + q;
+      const bool ok = i >= 0;
+      yb[u] = ok ? R.y[o] : 0.0;
+      ab[u] = ok ? R.u1[o] : 0.0;
+      bb[u] = ok ? R.u2[o] : 0.0;
+      rb[u] = ok ? R.r0[o] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSteinB; u++) {
+      const int i = i0 - u;
+      if (i >= 0) {
+        const double xi = (yb[u] - ab[u] * xn1 - bb[u] * xn2) * rb[u];
+        R.x[(size_t)i * nthr + q] = xi;
+        ss += xi * xi;
+        xn2 = xn1;
+        xn1 = xi;
+      }
+    }
+  }
+  return ss;
+}
+
+// Eigenvectors of T into the columns of Zt (ld n).  One thread per
+// eigenvalue j: two inverse iterations from a pseudo-random start.  Tight
+// clusters (consecutive gaps <= 1e-5 ||T||, which includes exact
+// degeneracies) are taken by the thread of their first member, sequentially
+// with modified Gram-Schmidt against the earlier members on every iteration
+// (LAPACK dstein's scheme).  Outside them the vectors are orthogonal to
+// ~eps ||T|| / gap <= 1e-10.  blockDim = 64.
 __global__ void __launch_bounds__(64)
-k_stein(const double* __restrict__ d, const double* __restrict__ e, const double* __restrict__ w, int n,
+k_stein(const double* __restrict__ dg, const double* __restrict__ eg, const double* __restrict__ w, int n,
         double* __restrict__ Zt, double* __restrict__ S) {
+  extern __shared__ double sde[];
+  double* d = sde;
+  double* e = d + n;
   __shared__ double sred[16];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    d[i] = dg[i];
+    if (i < n - 1) e[i] = eg[i];
+  }
   double gl, gu, nrm, em;
-  tri_bounds(d, e, n, sred, &gl, &gu, &nrm, &em);
+  tri_bounds(dg, eg, n, sred, &gl, &gu, &nrm, &em);  // ends with a barrier
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const double eps = DBL_EPSILON;
-  const double ortol = 1e-3 * nrm;
+  const double ctol = 1e-5 * nrm;
   const double pert = eps * fmax(nrm, DBL_MIN);
-  if (j > 0 && w[j] - w[j - 1] <= ortol) return;  // not a cluster start
+  if (j > 0 && w[j] - w[j - 1] <= ctol) return;  // member of an earlier thread's cluster
   int j1 = j;
-  while (j1 + 1 < n && w[j1 + 1] - w[j1] <= ortol) j1++;
+  while (j1 + 1 < n && w[j1 + 1] - w[j1] <= ctol) j1++;
   const int nthr = n;
-  auto at = [&](int a, int i) -> double& { return S[((size_t)a * n + i) * nthr + j]; };
-  // arrays: 0 u0 (pivot), 1 u1, 2 u2, 3 multiplier, 4 pivoting flag, 5 x
+  const size_t an = (size_t)n * nthr;
+  SteinRows R{S, S + an, S + 2 * an, S + 3 * an, S + 4 * an, S + 5 * an, S + 6 * an};
   double lamprev = 0.0;
   for (int jj = j; jj <= j1; jj++) {
     double lam = w[jj];
     if (jj > j && lam - lamprev < 10.0 * eps * fabs(lam)) lam = lamprev + 10.0 * eps * fabs(lam);
     lamprev = lam;
-    // LU with partial pivoting of T - lam I
-    double c = d[0] - lam, s1 = n > 1 ? e[0] : 0.0;
-    for (int i = 0; i < n - 1; i++) {
-      const double sub = e[i], dn = d[i + 1] - lam, sup = (i + 1 < n - 1) ? e[i + 1] : 0.0;
-      if (fabs(sub) > fabs(c)) {
-        const double mlt = c / sub;
-        at(0, i) = sub; at(1, i) = dn; at(2, i) = sup; at(3, i) = mlt; at(4, i) = 1.0;
-        c = s1 - mlt * dn;
-        s1 = -mlt * sup;
-      } else {
-        if (fabs(c) < pert) c = c >= 0.0 ? pert : -pert;
-        const double mlt = sub / c;
-        at(0, i) = c; at(1, i) = s1; at(2, i) = 0.0; at(3, i) = mlt; at(4, i) = 0.0;
-        c = dn - mlt * s1;
-        s1 = sup;
-      }
-    }
-    if (fabs(c) < pert) c = c >= 0.0 ? pert : -pert;
-    at(0, n - 1) = c; at(1, n - 1) = 0.0; at(2, n - 1) = 0.0;
-    for (int i = 0; i < n; i++) at(5, i) = start_value(jj, i);
-    const int iters = (j1 > j) ? 5 : 3;
+    stein_factor(d, e, n, lam, pert, R, nthr, j);
+    for (int i = 0; i < n; i++) R.x[(size_t)i * nthr + j] = start_value(jj, i);
+    const int iters = (j1 > j) ? 4 : 2;
+    double scale = 1.0;
     for (int it = 0; it < iters; it++) {
-      // scale to unit max norm, forward elimination, back substitution
-      double xm = 0.0;
-      for (int i = 0; i < n; i++) xm = fmax(xm, fabs(at(5, i)));
-      const double inv = xm > 0.0 ? 1.0 / xm : 1.0;
-      double y0 = at(5, 0) * inv;
-      for (int i = 0; i < n - 1; i++) {
-        const double y1 = at(5, i + 1) * inv, mlt = at(3, i);
-        if (at(4, i) != 0.0) {
-          at(5, i) = y1;
-          y0 = y0 - mlt * y1;
-        } else {
-          at(5, i) = y0;
-          y0 = y1 - mlt * y0;
+      double ss = stein_solve(n, scale, R, nthr, j);
+      if (jj > j) {
+        // orthogonalise against the cluster's earlier vectors (unit norm)
+        for (int kk = j; kk < jj; kk++) {
+          const double* zk = Zt + (size_t)n * kk;
+          double dot = 0.0;
+          for (int i = 0; i < n; i++) dot += zk[i] * R.x[(size_t)i * nthr + j];
+          for (int i = 0; i < n; i++) R.x[(size_t)i * nthr + j] -= dot * zk[i];
+        }
+        ss = 0.0;
+        for (int i = 0; i < n; i++) {
+          const double v = R.x[(size_t)i * nthr + j];
+          ss += v * v;
         }
       }
-      at(5, n - 1) = y0;
-      double xn1 = 0.0, xn2 = 0.0;
-      for (int i = n - 1; i >= 0; i--) {
-        const double xi = (at(5, i) - at(1, i) * xn1 - at(2, i) * xn2) / at(0, i);
-        at(5, i) = xi;
-        xn2 = xn1;
-        xn1 = xi;
-      }
-      // orthogonalise against the cluster's earlier vectors
-      for (int kk = j; kk < jj; kk++) {
-        const double* zk = Zt + (size_t)n * kk;
-        double dot = 0.0;
-        for (int i = 0; i < n; i++) dot += zk[i] * at(5, i);
-        for (int i = 0; i < n; i++) at(5, i) -= dot * zk[i];
-      }
-      double nn2 = 0.0;
-      for (int i = 0; i < n; i++) nn2 += at(5, i) * at(5, i);
-      const double sc = nn2 > 0.0 ? 1.0 / sqrt(nn2) : 0.0;
-      for (int i = 0; i < n; i++) at(5, i) *= sc;
+      scale = ss > 0.0 ? 1.0 / sqrt(ss) : 1.0;
     }
     double* zj = Zt + (size_t)n * jj;
-    for (int i = 0; i < n; i++) zj[i] = at(5, i);
+    for (int i0 = 0; i0 < n; i0 += kSteinB) {
+      double xb[kSteinB];
+#pragma unroll
+      for (int u = 0; u < kSteinB; u++) xb[u] = (i0 + u < n) ? R.x[(size_t)(i0 + u) * nthr + j] : 0.0;
+#pragma unroll
+      for (int u = 0; u < kSteinB; u++)
+        if (i0 + u < n) zj[i0 + u] = xb[u] * scale;
+    }
   }
 }
 
-// Z <- H_0 H_1 ... H_{n-3} Z for the kOrmCols columns of this workgroup.
+// Z <- H_0 H_1 ... H_{n-3} Z for the kOrmCols columns of this workgroup, in
+// compact-WY chunks of nbw reflectors (LAPACK dlarft 'F','C' + dlarfb): for
+// the chunk H_kb ... H_{ke-1} = I - V T V^T,  Z <- Z - V (T (V^T Z)), chunks
+// from the last to the first.  V (n x nbw), T, V^T V and the 8-column block
+// of Z live in LDS; no per-reflector reductions.  blockDim = 256.
 __global__ void __launch_bounds__(256)
-k_ormtr(const double* __restrict__ V, int ldv, const double* __restrict__ tau, int n, double* __restrict__ Z,
-        int ldz) {
-  extern __shared__ double zs[];  // n x kOrmCols, row-major
-  __shared__ double sred[4][kOrmCols];
-  __shared__ double dots[kOrmCols];
+k_ormtr(const double* __restrict__ A, int lda, const double* __restrict__ tau, int n, int nbw,
+        double* __restrict__ Z, int ldz) {
+  extern __shared__ double sm[];
+  double* zs = sm;                                 // n x kOrmCols, row-major
+  const int lvs = nbw + 1;                         // padded row stride of Vs
+  double* Vs = zs + (size_t)n * kOrmCols;          // n x nbw, row-major (row r, reflector j)
+  double* Gs = Vs + (size_t)n * lvs;               // nbw x nbw: V^T V, then T
+  double* Ts = Gs + (size_t)nbw * nbw;
+  double* W1 = Ts + (size_t)nbw * nbw;             // nbw x kOrmCols
   const int c0 = blockIdx.x * kOrmCols;
   const int nc = min(kOrmCols, n - c0);
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  for (int e = t; e < n * kOrmCols; e += blockDim.x) {
-    const int r = e / kOrmCols, c = e % kOrmCols;
-    zs[e] = (c < nc) ? Z[r + (size_t)ldz * (c0 + c)] : 0.0;
+  const int t = threadIdx.x;
+  // staged 8 loads per thread at a time (one memory round trip per batch)
+  for (int e0 = 0; e0 < n * kOrmCols; e0 += 8 * 256) {
+    double b8[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int e = e0 + u * 256 + t, r = e / kOrmCols, c = e % kOrmCols;
+      b8[u] = (e < n * kOrmCols && c < nc) ? Z[r + (size_t)ldz * (c0 + c)] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++)
+      if (e0 + u * 256 + t < n * kOrmCols) zs[e0 + u * 256 + t] = b8[u];
+  }
+  const int nref = n - 2;  // reflectors 0..n-3 (tau[n-2] = 0)
+  const int nchunk = (nref + nbw - 1) / nbw;
+  for (int ch = nchunk - 1; ch >= 0; ch--) {
+    const int kb = ch * nbw, ke = min(nref, kb + nbw), b = ke - kb;
+    // V rows kb+1 .. n-1 (row kb+1+j holds 1 for reflector j, zeros above)
+    __syncthreads();
+    // consecutive threads walk down one reflector column (coalesced loads),
+    // 8 loads in flight per thread; the padded stride keeps the transposed
+    // LDS stores conflict-free
+    const int nr = n - kb, tot = nr * nbw;
+    for (int e0 = 0; e0 < tot; e0 += 8 * 256) {
+      double b8[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int e = e0 + u * 256 + t;
+        const int j = e / nr, r = kb + e % nr, k = kb + j;
+        b8[u] = (e < tot && j < b) ? ((r == k + 1) ? 1.0 : (r > k + 1 ? A[(size_t)lda * k + r] : 0.0)) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int e = e0 + u * 256 + t;
+        if (e < tot) Vs[(size_t)(kb + e % nr) * lvs + e / nr] = b8[u];
+      }
+    }
+    __syncthreads();
+    // G = V^T V (upper part used) and W1 = V^T Z
+    for (int e = t; e < nbw * nbw; e += blockDim.x) {
+      const int i = e / nbw, j = e % nbw;
+      double g = 0.0;
+      if (i < j && j < b)
+        for (int r = kb + 1 + j; r < n; r++) g += Vs[(size_t)r * lvs + i] * Vs[(size_t)r * lvs + j];
+      Gs[e] = g;
+    }
+    for (int e = t; e < nbw * kOrmCols; e += blockDim.x) {
+      const int j = e / kOrmCols, c = e % kOrmCols;
+      double w = 0.0;
+      if (j < b)
+        for (int r = kb + 1 + j; r < n; r++) w += Vs[(size_t)r * lvs + j] * zs[r * kOrmCols + c];
+      W1[e] = w;
+    }
+    __syncthreads();
+    // T (upper triangular): T_jj = tau_j, T[0:j, j] = -tau_j T[0:j, 0:j] G[0:j, j].
+    // Thread i owns row i of T and reads only that row, so the recurrence
+    // needs no synchronisation.
+    if (t < nbw) {
+      for (int j = 0; j < nbw; j++) {
+        double x = 0.0;
+        if (t < j && j < b)
+          for (int q = t; q < j; q++) x += Ts[t * nbw + q] * Gs[q * nbw + j];
+        Ts[t * nbw + j] = (j >= b || t > j) ? 0.0 : (t == j ? tau[kb + j] : -tau[kb + j] * x);
+      }
+    }
+    __syncthreads();
+    // W2 = T W1 (into Gs, no longer needed)
+    for (int e = t; e < nbw * kOrmCols; e += blockDim.x) {
+      const int j = e / kOrmCols, c = e % kOrmCols;
+      double w = 0.0;
+      if (j < b)
+        for (int i = j; i < b; i++) w += Ts[j * nbw + i] * W1[i * kOrmCols + c];
+      Gs[e] = w;
+    }
+    __syncthreads();
+    // Z -= V W2 on rows kb+1 .. n-1
+    for (int e = t; e < (n - kb - 1) * kOrmCols; e += blockDim.x) {
+      const int r = kb + 1 + e / kOrmCols, c = e % kOrmCols;
+      double u = 0.0;
+      for (int j = 0; j < b; j++) u += Vs[(size_t)r * lvs + j] * Gs[j * kOrmCols + c];
+      zs[r * kOrmCols + c] -= u;
+    }
   }
   __syncthreads();
-  for (int k = n - 3; k >= 0; k--) {
-    const double tk = tau[k];
-    if (tk == 0.0) continue;  // block-uniform
-    const double* v = V + (size_t)ldv * k;  // v[r] for r in k+1..n-1, v[k+1] = 1
-    double part[kOrmCols];
-#pragma unroll
-    for (int c = 0; c < kOrmCols; c++) part[c] = 0.0;
-    for (int r = k + 1 + t; r < n; r += blockDim.x) {
-      const double vr = v[r];
-#pragma unroll
-      for (int c = 0; c < kOrmCols; c++) part[c] += vr * zs[r * kOrmCols + c];
-    }
-#pragma unroll
-    for (int c = 0; c < kOrmCols; c++) {
-      double x = part[c];
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-      if (lane == 0) sred[wv][c] = x;
-    }
-    __syncthreads();
-    if (t < kOrmCols) dots[t] = tk * (sred[0][t] + sred[1][t] + sred[2][t] + sred[3][t]);
-    __syncthreads();
-    for (int r = k + 1 + t; r < n; r += blockDim.x) {
-      const double vr = v[r];
-#pragma unroll
-      for (int c = 0; c < kOrmCols; c++) zs[r * kOrmCols + c] -= vr * dots[c];
-    }
-    __syncthreads();
-  }
   for (int e = t; e < n * kOrmCols; e += blockDim.x) {
     const int r = e / kOrmCols, c = e % kOrmCols;
     if (c < nc) Z[r + (size_t)ldz * (c0 + c)] = zs[e];
@@ -399,7 +646,7 @@ int eig_sym(pw_ctx* c, int n, double* A, int lda, double* w) {
     return PW_OK;
   }
   const size_t nn = (size_t)n * n;
-  const size_t need = sizeof(double) * (3 * (size_t)n + nn + 6 * nn);
+  const size_t need = sizeof(double) * (3 * (size_t)n + nn + 7 * nn);
   if (need > c->eig_ws_bytes) {
     if (c->d_eig_ws) (void)hipFree(c->d_eig_ws);
     c->d_eig_ws = nullptr;
@@ -414,17 +661,24 @@ int eig_sym(pw_ctx* c, int n, double* A, int lda, double* w) {
   double* S = Zt + nn;
   const int g1 = (int)std::min<size_t>((nn + 255) / 256, 4096);
   hipLaunchKernelGGL(k_sym_from_upper, dim3(g1), dim3(256), 0, s, A, n, lda);
-  const size_t lds_trd = sizeof(double) * (size_t)(2 + kTrdWaves) * n;
+  // panel width: the V and W panels (2 nbk n doubles) fit LDS next to v and
+  // the partial sums (n = 256: 32; 512: 16; 1024: 8)
+  const int nbk = std::max(1, std::min(32, 8192 / n));
+  const size_t lds_trd = sizeof(double) * ((size_t)2 * nbk * n + n + 1024 + 2 * (size_t)nbk);
   PW_TRY(ensure_lds((const void*)k_sytrd, lds_trd + 1024));  // + its static LDS
-  hipLaunchKernelGGL(k_sytrd, dim3(1), dim3(kTrdThreads), lds_trd, s, A, n, lda, d, e, tau);
+  hipLaunchKernelGGL(k_sytrd, dim3(1), dim3(kTrdThreads), lds_trd, s, A, n, lda, nbk, d, e, tau);
   hipLaunchKernelGGL(k_stebz, dim3((n + 3) / 4), dim3(256), sizeof(double) * 2 * n, s, (const double*)d,
                      (const double*)e, n, w);
-  hipLaunchKernelGGL(k_stein, dim3((n + 63) / 64), dim3(64), 0, s, (const double*)d, (const double*)e,
+  hipLaunchKernelGGL(k_stein, dim3((n + 63) / 64), dim3(64), sizeof(double) * 2 * n, s, (const double*)d,
+                     (const double*)e,
                      (const double*)w, n, Zt, S);
-  const size_t lds_orm = sizeof(double) * (size_t)n * kOrmCols;
-  PW_TRY(ensure_lds((const void*)k_ormtr, lds_orm + 1024));  // + its static LDS
+  // compact-WY chunk: V (n x nbw) next to the Z block in LDS
+  const int nbw = std::max(1, std::min(32, 8192 / n));
+  const size_t lds_orm = sizeof(double) * ((size_t)n * kOrmCols + (size_t)n * (nbw + 1) + 2 * (size_t)nbw * nbw +
+                                           (size_t)nbw * kOrmCols);
+  PW_TRY(ensure_lds((const void*)k_ormtr, lds_orm + 1024));
   hipLaunchKernelGGL(k_ormtr, dim3((n + kOrmCols - 1) / kOrmCols), dim3(256), lds_orm, s, (const double*)A, lda,
-                     (const double*)tau, n, Zt, n);
+                     (const double*)tau, n, nbw, Zt, n);
   PW_HIP(hipGetLastError());
   PW_HIP(hipMemcpy2DAsync(A, sizeof(double) * lda, Zt, sizeof(double) * n, sizeof(double) * n, n,
                           hipMemcpyDeviceToDevice, s));

@@ -530,6 +530,7 @@ extern "C" int pw_set_timing(pw_ctx* c, int enable) {
   }
   c->timing = enable != 0;
   c->ev_calls = 0;
+  c->ev_total = 0;  // a new timing session counts its own calls
   for (int i = 0; i < 4; i++) c->stage_ms[i] = 0.0;
   return PW_OK;
 }
