@@ -2,7 +2,7 @@
 MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload si64_60]
-                    [--no-cpu]
+                    [--no-cpu] [--copy-probe]
 
 Workload si64_60: 64-atom diamond supercell, 60^3 grid, Ecut 10 Ha,
 npw 6567, 256 bands, 256 projectors; degree-10 Chebyshev filter with bounds
@@ -47,6 +47,18 @@ def staged_bytes(prob, p):
     return p * (9 * prob.npw + 6 * Nh + 5 * prob.ngrid)
 
 
+def pmc_traffic(path, kernel):
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 --pmc
+    summary (a line 'traffic <kernel...> bytes_per_launch=<B>'), or None."""
+    try:
+        for line in open(path):
+            if line.startswith("traffic ") and kernel in line and "bytes_per_launch=" in line:
+                return int(float(line.split("bytes_per_launch=")[1].split()[0]))
+    except OSError:
+        pass
+    return None
+
+
 def host_cores():
     try:
         n = len(os.sched_getaffinity(0))
@@ -63,10 +75,13 @@ def main():
     ap.add_argument("--workload", default="si64_60")
     ap.add_argument("--hpsi-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="end with 3 device copies of 256 MiB (the reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(how the path shards over GPUs is described in DESIGN.md)"}))
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
+                          "(the band-sharded multi-GPU layout is a design, not built)"}))
         sys.exit(2)
 
     import numpy as np
@@ -126,19 +141,23 @@ def main():
     nwant = 3 * nb // 4
     err = np.abs(e_mixed - e_f64)
 
-    # Roofline of the dominant kernel group of the mixed sweep, the FP32 local
-    # H*psi pipeline (sticks -> inverse z, xy slab, forward z -> sphere): the
-    # staged bytes p (9 npw + 6 Nh + 5 N) per band-application times nb bands
-    # per call, over the summed HIP-event durations of the three launches in
-    # the FP32 H*psi timed loop above (same shape as the filter's calls).
-    def roof(tag, p):
-        local_ms = sum(stages[tag][:3])
-        ach = staged_bytes(prob, p) * nb / (local_ms * 1e-3) / 1e9
-        own = slab_bytes(prob, info["nstick"], nb, p) / (stages[tag][1] * 1e-3) / 1e9
-        return ach, own, local_ms
+    # Roofline of the dominant H*psi kernel, the FP32 xy slab kernel (10 of
+    # the 11 H*psi calls of a mixed sweep): its algorithmic bytes per launch
+    # (sticks read and written once, V read once) over its average launch
+    # duration from HIP events recorded on its launch stream in the FP32 H*psi
+    # loop above (same shape as the filter's calls).  The staged dense-FFT
+    # model over the three local launches is reported beside it.
+    slab_ms = stages["f32"][1]
+    slab_b = slab_bytes(prob, info["nstick"], nb, 4)
+    achieved = slab_b / (slab_ms * 1e-3) / 1e9
+    traffic = pmc_traffic(os.path.join(ROOT, "profiles", f"pmc_traffic_{prob.name}.md"), "k_slab_xy<float")
 
-    achieved, slab_own, local_ms = roof("f32", 4)
-    achieved64, slab_own64, local_ms64 = roof("f64", 8)
+    def staged(tag, p):
+        local_ms = sum(stages[tag][:3])
+        return staged_bytes(prob, p) * nb / (local_ms * 1e-3) / 1e9, local_ms
+
+    st32, local_ms = staged("f32", 4)
+    st64, local_ms64 = staged("f64", 8)
     out.update({
         "metric": "ChebFSI-RR sweep band-applications/s (mixed precision), Ritz err vs FP64",
         "value": round(napp / t_mixed, 1),
@@ -161,16 +180,17 @@ def main():
                         "nwanted": nwant},
         "hpsi_band_applications_per_s": {k: round(v, 1) for k, v in hpsi.items()},
         "stages_ms_per_apply": {k: [round(x, 4) for x in v] for k, v in stages.items()},
-        "roofline": {"bound": "hbm", "kernel": "k_sticks_inverse_z + k_slab_xy + k_sticks_forward_z <float>",
+        "roofline": {"bound": "hbm", "kernel": f"k_slab_xy<float, {prob.grid[0]}, {prob.grid[1]}>",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "model": "staged bytes 4*(9 npw + 6 Nh + 5 N) per band-application",
-                     "launch_ms": round(local_ms, 4),
-                     "fp64": {"achieved": round(achieved64, 1), "frac": round(achieved64 / HBM_PEAK_GBS, 4),
-                              "launch_ms": round(local_ms64, 4)},
-                     "slab_kernel_own_bytes_frac": {"f32": round(slab_own / HBM_PEAK_GBS, 4),
-                                                    "f64": round(slab_own64 / HBM_PEAK_GBS, 4)}},
-        "staged_model_equiv_GBs_fp64_hpsi": round(staged_bytes(prob, 8) * hpsi["f64"] / 1e9, 1),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_launch": slab_b, "launch_ms": round(slab_ms, 4),
+                     "model": "per launch: nb x (n3 x nstick complex sticks read + written) + V read"},
+        "staged_model": {"kernels": "k_sticks_inverse_z + k_slab_xy + k_sticks_forward_z",
+                         "bytes_per_band": "p (9 npw + 6 Nh + 5 N)",
+                         "f32": {"GBs": round(st32, 1), "frac": round(st32 / HBM_PEAK_GBS, 4),
+                                 "launch_ms": round(local_ms, 4)},
+                         "f64": {"GBs": round(st64, 1), "frac": round(st64 / HBM_PEAK_GBS, 4),
+                                 "launch_ms": round(local_ms64, 4)}},
     })
 
     if not args.no_cpu:
@@ -191,6 +211,14 @@ def main():
         out["ritz_err_ha"]["fp64_gpu_vs_oracle_max"] = float(np.abs(e_f64 - e_o).max())
         out["ritz_err_ha"]["mixed_vs_oracle_wanted_max"] = float(np.abs(e_mixed - e_o)[:nwant].max())
     ctx.close()
+    if args.copy_probe:
+        # plain device copies of a known size: the scale for FETCH_SIZE / WRITE_SIZE
+        src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_probe_bytes"] = 256 << 20
     print(json.dumps(out))
 
 
