@@ -40,12 +40,6 @@ constexpr int kTrdWaves = kTrdThreads / 64;
 constexpr int kOrmCols = 8;         // columns per k_ormtr workgroup
 constexpr int kEigMaxN = 1024;      // LDS bound of k_sytrd
 
-// Workgroup barrier for LDS traffic only: waits for this wavefront's LDS
-// operations but leaves its global loads and stores in flight (a full
-// __syncthreads also drains those).  Only for phases whose cross-wavefront
-// data goes through LDS.
-__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
-
 // Block-wide sum, identical order on every call (deterministic), returned to
 // all threads.  sred holds one double per wavefront.
 template <int NT>

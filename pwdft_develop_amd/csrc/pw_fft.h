@@ -6,6 +6,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 namespace pw {
 
 template <typename T> struct cx;
@@ -160,6 +162,99 @@ __device__ __forceinline__ typename cx<T>::t* lds_fft_n(typename cx<T>::t* a, ty
   } else {
     return lds_fft<T>(a, b, n, P, ld, tw, sg);
   }
+}
+
+}  // namespace pw
+
+namespace pw {
+
+// Compile-time twiddles: cos and sin of 2 pi m / n.  The quadrant is taken
+// exactly in integers (rotations by pi/2 are exact), the remaining angle
+// a in [0, pi/2) goes through Taylor series to x^33 (truncation < 1e-20),
+// so the constants are correctly rounded to within about an ulp.
+namespace ctw {
+constexpr double kHalfPi = 1.57079632679489661923132169163975144;
+constexpr double sin_t(double x) {
+  double x2 = x * x, term = x, sum = x;
+  for (int k = 1; k <= 16; k++) {
+    term *= -x2 / ((2.0 * k) * (2.0 * k + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double cos_t(double x) {
+  double x2 = x * x, term = 1.0, sum = 1.0;
+  for (int k = 1; k <= 16; k++) {
+    term *= -x2 / ((2.0 * k - 1.0) * (2.0 * k));
+    sum += term;
+  }
+  return sum;
+}
+constexpr void cs2pi(long m, long n, double& c, double& s) {
+  m %= n;
+  if (m < 0) m += n;
+  const long q = (4 * m) / n, rem = 4 * m - q * n;
+  const double a = (double)rem / (double)n * kHalfPi;
+  const double ca = cos_t(a), sa = sin_t(a);
+  if (q == 0) { c = ca; s = sa; }
+  else if (q == 1) { c = -sa; s = ca; }
+  else if (q == 2) { c = -ca; s = -sa; }
+  else { c = sa; s = -ca; }
+}
+// exp(2 pi i r STEP / N) for r = 0 .. R-1
+template <int N, int R, int STEP>
+struct TwR {
+  double c[R], s[R];
+  constexpr TwR() : c(), s() {
+    for (int r = 0; r < R; r++) cs2pi((long)r * STEP, N, c[r], s[r]);
+  }
+};
+}  // namespace ctw
+
+// In-register FFT of compile-time length N held by one thread: the Stockham
+// passes of lds_fft (radix 4, 2, 3, 5 order) with the butterfly index J, the
+// stride and the twiddles all compile-time constants, so the data stays in
+// registers and every twiddle is an immediate.  SG = +1 backward (exp(+i)),
+// -1 forward.
+template <typename T, int N, int NS, int R, int SG, int J>
+__device__ __forceinline__ void reg_bfly(const typename cx<T>::t (&a)[N], typename cx<T>::t (&b)[N]) {
+  typedef typename cx<T>::t C;
+  constexpr int M = N / R, K = J % NS, TSTEP = N / (NS * R);
+  C v[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) v[r] = a[J + r * M];
+  if constexpr (K != 0) {
+    constexpr ctw::TwR<N, R, K * TSTEP> W{};
+#pragma unroll
+    for (int r = 1; r < R; r++) v[r] = cmul(v[r], C{(T)W.c[r], (T)(SG * W.s[r])});
+  }
+  dft<T, R>(v, (T)SG);
+#pragma unroll
+  for (int r = 0; r < R; r++) b[(J / NS) * NS * R + K + r * NS] = v[r];
+}
+
+template <typename T, int N, int NS, int R, int SG, int... J>
+__device__ __forceinline__ void reg_pass(const typename cx<T>::t (&a)[N], typename cx<T>::t (&b)[N],
+                                         std::integer_sequence<int, J...>) {
+  (reg_bfly<T, N, NS, R, SG, J>(a, b), ...);
+}
+
+template <typename T, int N, int NS, int REM, int SG>
+__device__ __forceinline__ void reg_fft_rec(typename cx<T>::t (&a)[N]) {
+  if constexpr (REM > 1) {
+    typedef typename cx<T>::t C;
+    constexpr int R = first_radix<REM>::value;
+    C b[N];
+    reg_pass<T, N, NS, R, SG>(a, b, std::make_integer_sequence<int, N / R>{});
+#pragma unroll
+    for (int i = 0; i < N; i++) a[i] = b[i];
+    reg_fft_rec<T, N, NS * R, REM / R, SG>(a);
+  }
+}
+
+template <typename T, int N, int SG>
+__device__ __forceinline__ void reg_fft(typename cx<T>::t (&a)[N]) {
+  reg_fft_rec<T, N, 1, N, SG>(a);
 }
 
 }  // namespace pw

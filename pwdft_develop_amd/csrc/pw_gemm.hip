@@ -35,7 +35,7 @@ __device__ __forceinline__ f4v mfma16(float a, float b, f4v c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-constexpr int BM = 64, BN = 64, BK = 16;
+constexpr int BM = 64, BN = 64, BK = 32;
 
 template <typename T>
 __device__ __forceinline__ int acc_row(int lane, int r) {
@@ -43,6 +43,10 @@ __device__ __forceinline__ int acc_row(int lane, int r) {
 }
 
 // Partial C tile of A^T B over K rows [kb, ke): P[split](n1 x n2, ld n1).
+// Software-pipelined: the next K chunk is loaded into registers while the
+// MFMAs run on the current one in LDS.  Chunk loads are column runs: load q
+// of wave w reads column 16 w + 2 q + lane/32, rows lane%32 (one 32-row
+// segment per half wave), for A and for B.
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_gemm_tn_partial(const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb, int K, int n1, int n2,
@@ -59,18 +63,26 @@ k_gemm_tn_partial(const T* __restrict__ A, int lda, const T* __restrict__ B, int
   for (int a = 0; a < 2; a++)
 #pragma unroll
     for (int b = 0; b < 2; b++) acc[a][b] = V{0, 0, 0, 0};
-  const int lc = t >> 2, lr = (t & 3) * 4;  // load: column lc, rows lr..lr+3
-  for (int k0 = kb; k0 < ke; k0 += BK) {
-    {
-      const int ci = i0 + lc, cj = j0 + lc;
+  const int lr = lane & 31, lcol = 16 * w + (lane >> 5);  // column of load q: lcol + 2 q
+  T ra[8], rb[8];
+  auto load = [&](int k0) {
+    const int r = k0 + lr;
 #pragma unroll
-      for (int q = 0; q < 4; q++) {
-        const int r = k0 + lr + q;
-        As[lc][lr + q] = (ci < n1 && r < ke) ? A[(size_t)ci * lda + r] : (T)0;
-        Bs[lc][lr + q] = (cj < n2 && r < ke) ? B[(size_t)cj * ldb + r] : (T)0;
-      }
+    for (int q = 0; q < 8; q++) {
+      const int ci = i0 + lcol + 2 * q, cj = j0 + lcol + 2 * q;
+      ra[q] = (ci < n1 && r < ke) ? A[(size_t)ci * lda + r] : (T)0;
+      rb[q] = (cj < n2 && r < ke) ? B[(size_t)cj * ldb + r] : (T)0;
     }
-    __syncthreads();
+  };
+  load(kb);
+  for (int k0 = kb; k0 < ke; k0 += BK) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      As[lcol + 2 * q][lr] = ra[q];
+      Bs[lcol + 2 * q][lr] = rb[q];
+    }
+    lds_barrier();
+    if (k0 + BK < ke) load(k0 + BK);  // in flight during the MFMAs below and the barrier after them
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       T av[2], bv[2];
@@ -83,7 +95,7 @@ k_gemm_tn_partial(const T* __restrict__ A, int lda, const T* __restrict__ B, int
 #pragma unroll
         for (int b = 0; b < 2; b++) acc[a][b] = mfma16(av[a], bv[b], acc[a][b]);
     }
-    __syncthreads();
+    lds_barrier();
   }
   T* Ps = P + (size_t)split * n1 * n2;
 #pragma unroll
@@ -99,7 +111,9 @@ k_gemm_tn_partial(const T* __restrict__ A, int lda, const T* __restrict__ B, int
     }
 }
 
-// C[i + ldc j] = rowscale[i] * sum_s P[s][i + n1 j] (+ beta C), rowscale may be null
+// C[i + ldc j] = rowscale[i] * sum_s P[s][i + n1 j] (+ beta C), rowscale may
+// be null.  The partials are summed in split order (deterministic), eight
+// loads in flight at a time.
 template <typename T>
 __global__ void k_gemm_tn_reduce(const T* __restrict__ P, int nsplit, int n1, int n2, T* __restrict__ C, int ldc,
                                  const T* __restrict__ rowscale, T beta) {
@@ -107,7 +121,13 @@ __global__ void k_gemm_tn_reduce(const T* __restrict__ P, int nsplit, int n1, in
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
     const int i = (int)(e % n1), j = (int)(e / n1);
     T s = 0;
-    for (int q = 0; q < nsplit; q++) s += P[(size_t)q * total + e];
+    for (int q0 = 0; q0 < nsplit; q0 += 8) {
+      T v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = (q0 + u < nsplit) ? P[(size_t)(q0 + u) * total + e] : (T)0;
+#pragma unroll
+      for (int u = 0; u < 8; u++) s += v[u];
+    }
     if (rowscale) s *= rowscale[i];
     T* c = C + i + (size_t)ldc * j;
     *c = (beta == (T)0) ? s : s + beta * *c;
@@ -115,6 +135,9 @@ __global__ void k_gemm_tn_reduce(const T* __restrict__ P, int nsplit, int n1, in
 }
 
 // Y = alpha A B + beta Y; A m x k (lda), B k x n (ldb), Y m x n (ldy).
+// Pipelined like k_gemm_tn_partial.  A chunk (64 rows x 32 k): load q of
+// wave w reads k = 8 w + q, rows lane (one 64-row run per wave); B chunk
+// (32 k x 64 columns): column 16 w + 2 q + lane/32, k lane%32.
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_gemm_nn(const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb, T* __restrict__ Y, int ldy, int m,
@@ -130,17 +153,27 @@ k_gemm_nn(const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb, T*
   for (int a = 0; a < 2; a++)
 #pragma unroll
     for (int b = 0; b < 2; b++) acc[a][b] = V{0, 0, 0, 0};
-  const int ak = t >> 4, ai = (t & 15) * 4;   // A load: k row ak, rows ai..ai+3
-  const int bj = t >> 2, bk = (t & 3) * 4;    // B load: column bj, k bk..bk+3
+  const int bk = lane & 31, bcol = 16 * w + (lane >> 5);
+  const int ar = i0 + lane;
+  T ra[8], rb[8];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int kk = k0 + 8 * w + q;
+      ra[q] = (ar < m && kk < k) ? A[(size_t)kk * lda + ar] : (T)0;
+      const int cj = j0 + bcol + 2 * q, kq = k0 + bk;
+      rb[q] = (cj < n && kq < k) ? B[(size_t)cj * ldb + kq] : (T)0;
+    }
+  };
+  load(0);
   for (int k0 = 0; k0 < k; k0 += BK) {
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int r = i0 + ai + q, kk = k0 + ak;
-      As[ak][ai + q] = (r < m && kk < k) ? A[(size_t)kk * lda + r] : (T)0;
-      const int cj = j0 + bj, kq = k0 + bk + q;
-      Bs[bj][bk + q] = (cj < n && kq < k) ? B[(size_t)cj * ldb + kq] : (T)0;
+    for (int q = 0; q < 8; q++) {
+      As[8 * w + q][lane] = ra[q];
+      Bs[bcol + 2 * q][bk] = rb[q];
     }
-    __syncthreads();
+    lds_barrier();
+    if (k0 + BK < k) load(k0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       T av[2], bv[2];
@@ -153,7 +186,7 @@ k_gemm_nn(const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb, T*
 #pragma unroll
         for (int b = 0; b < 2; b++) acc[a][b] = mfma16(av[a], bv[b], acc[a][b]);
     }
-    __syncthreads();
+    lds_barrier();
   }
 #pragma unroll
   for (int a = 0; a < 2; a++)

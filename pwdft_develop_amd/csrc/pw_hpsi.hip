@@ -159,6 +159,154 @@ k_slab_xy(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restric
   for (int s = threadIdx.x; s < nstick; s += blockDim.x) dst[s] = Yf[pos[s]];
 }
 
+// ------------------------------------------------ kernel 2, register variant
+// The same slab transform for compile-time N1 = N2 with every 1D FFT held by
+// one thread in registers (reg_fft: constant twiddles).  A workgroup takes one z plane of SB
+// bands: V's plane is staged once in LDS (rows padded to N1 + 1 floats, so
+// the row-pair reads of neighbouring threads hit distinct banks), each band's
+// y-space lives in LDS, and the five phases (scatter, y IFFT per column, x
+// c2r / V / r2c per row pair, y FFT per column, gather) are separated by one
+// barrier each.  Same arithmetic, in the same order, as k_slab_xy.
+template <typename T, int N1, int N2, int SB>
+__global__ void __launch_bounds__(256, 2)  // two waves per SIMD: <= 256 VGPRs
+k_slab_xy_reg(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restrict__ t2, int n3, int nstick,
+              int kxmax, int nb, const int* __restrict__ pos, const int* __restrict__ mpos,
+              const T* __restrict__ vtot) {
+  typedef typename cx<T>::t C;
+  constexpr int H1 = N1 / 2, NPAIR = (N2 + 1) / 2, LV = N1 + 1;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* Y = reinterpret_cast<C*>(smem_raw);           // SB x nk x N2
+  const int nk = kxmax + 1;
+  T* vs = reinterpret_cast<T*>(Y + (size_t)SB * nk * N2);  // N2 x LV
+  int* spos = reinterpret_cast<int*>(vs + (size_t)N2 * LV);  // nstick
+  int* smpos = spos + nstick;                                // nstick
+  const int z = blockIdx.x, b0 = blockIdx.y * SB;
+  const int ns = min(SB, nb - b0);
+  const int t = threadIdx.x;
+  // Staging loads go in batches issued before any use (one memory round trip
+  // per batch; a plain strided loop waits for each load in turn).
+  const T* vz = vtot + (size_t)z * N1 * N2;
+  constexpr int VB = (N1 * N2 + 255) / 256;
+  {
+    T vb[VB];
+#pragma unroll
+    for (int u = 0; u < VB; u++) {
+      const int e = t + 256 * u;
+      vb[u] = e < N1 * N2 ? vz[e] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < VB; u++) {
+      const int e = t + 256 * u;
+      if (e < N1 * N2) vs[(e / N1) * LV + (e % N1)] = vb[u];
+    }
+  }
+  for (int e0 = 0; e0 < nstick; e0 += 4 * 256) {
+    int pb[4], mb[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int e = e0 + t + 256 * u;
+      pb[u] = e < nstick ? pos[e] : 0;
+      mb[u] = e < nstick ? mpos[e] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int e = e0 + t + 256 * u;
+      if (e < nstick) { spos[e] = pb[u]; smpos[e] = mb[u]; }
+    }
+  }
+  for (int e = t; e < SB * nk * N2; e += blockDim.x) Y[e] = C{0, 0};
+  __syncthreads();
+  const int tot = ns * nstick;
+  for (int e0 = 0; e0 < tot; e0 += 8 * 256) {
+    C vb[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int e = e0 + t + 256 * u;
+      const int s = e / nstick, st = e - s * nstick;
+      vb[u] = e < tot ? t1[((size_t)(b0 + s) * n3 + z) * nstick + st] : C{0, 0};
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int e = e0 + t + 256 * u;
+      if (e < tot) {
+        const int s = e / nstick, st = e - s * nstick;
+        C* Ys = Y + (size_t)s * nk * N2;
+        Ys[spos[st]] = vb[u];
+        const int mp = smpos[st];
+        if (mp >= 0) Ys[mp] = C{vb[u].x, -vb[u].y};
+      }
+    }
+  }
+  __syncthreads();
+  // inverse y on the kx <= kxmax columns
+  for (int c = t; c < ns * nk; c += blockDim.x) {
+    C* col = Y + (size_t)c * N2;
+    C a[N2];
+#pragma unroll
+    for (int i = 0; i < N2; i++) a[i] = col[i];
+    reg_fft<T, N2, 1>(a);
+#pragma unroll
+    for (int i = 0; i < N2; i++) col[i] = a[i];
+  }
+  __syncthreads();
+  // per row pair: Hermitian-extended spectra packed as X1 + i X2, c2r, x V/N,
+  // r2c, unpack kx <= kxmax; a thread reads and writes only its two rows
+  for (int c = t; c < ns * NPAIR; c += blockDim.x) {
+    const int s = c / NPAIR, q = c - s * NPAIR;
+    const int y1 = 2 * q, y2 = y1 + 1;
+    const bool has2 = y2 < N2;
+    C* Ys = Y + (size_t)s * nk * N2;
+    C a[N1];
+#pragma unroll
+    for (int k = 0; k < N1; k++) {
+      const int kk = (k > H1) ? N1 - k : k;
+      C a1 = C{0, 0}, a2 = C{0, 0};
+      if (kk <= kxmax) {
+        a1 = Ys[kk * N2 + y1];
+        if (has2) a2 = Ys[kk * N2 + y2];
+        if (k > H1) { a1.y = -a1.y; a2.y = -a2.y; }
+        if (k == 0 || 2 * k == N1) { a1.y = 0; a2.y = 0; }
+      }
+      a[k] = C{a1.x - a2.y, a1.y + a2.x};
+    }
+    reg_fft<T, N1, 1>(a);
+    // keep the V reads next to their use (hoisting all of them above the FFT
+    // would hold 2 N1 more registers)
+    __builtin_amdgcn_sched_barrier(0);
+    const T* v1 = vs + y1 * LV;
+    const T* v2 = vs + (has2 ? y2 : y1) * LV;
+#pragma unroll
+    for (int x = 0; x < N1; x++) a[x] = C{a[x].x * v1[x], has2 ? a[x].y * v2[x] : (T)0};
+    __builtin_amdgcn_sched_barrier(0);
+    reg_fft<T, N1, -1>(a);
+#pragma unroll
+    for (int k = 0; k <= H1; k++) {
+      if (k <= kxmax) {
+        const C w = a[k];
+        const C wc = a[k == 0 ? 0 : N1 - k];
+        Ys[k * N2 + y1] = C{(T)0.5 * (w.x + wc.x), (T)0.5 * (w.y - wc.y)};
+        if (has2) Ys[k * N2 + y2] = C{(T)0.5 * (w.y + wc.y), (T)0.5 * (wc.x - w.x)};
+      }
+    }
+  }
+  __syncthreads();
+  // forward y
+  for (int c = t; c < ns * nk; c += blockDim.x) {
+    C* col = Y + (size_t)c * N2;
+    C a[N2];
+#pragma unroll
+    for (int i = 0; i < N2; i++) a[i] = col[i];
+    reg_fft<T, N2, -1>(a);
+#pragma unroll
+    for (int i = 0; i < N2; i++) col[i] = a[i];
+  }
+  __syncthreads();
+  for (int e = t; e < tot; e += blockDim.x) {
+    const int s = e / nstick, st = e - s * nstick;
+    t2[((size_t)(b0 + s) * n3 + z) * nstick + st] = Y[(size_t)s * nk * N2 + spos[st]];
+  }
+}
+
 // ---------------------------------------------------------------- kernel 3
 template <typename T, int N3>
 __global__ void __launch_bounds__(256)
@@ -221,6 +369,24 @@ auto slab_kernel(int n1, int n2) -> decltype(&k_slab_xy<T, 0, 0>) {
     }
   }
   return &k_slab_xy<T, 0, 0>;
+}
+
+// Register slab kernel: FP32 square planes of these sizes (a row of N1
+// complex floats per thread stays within the register budget up to 72).
+constexpr int kSlabSB = 8;
+#define PW_REG_SIZES(X) X(24) X(30) X(32) X(36) X(40) X(48) X(60) X(64) X(72)
+typedef decltype(&k_slab_xy_reg<float, 24, 24, kSlabSB>) slab_reg_fn;
+static slab_reg_fn slab_reg_kernel(int n1, int n2) {
+  if (n1 != n2) return nullptr;
+  switch (n1) {
+#define PW_CASE(n) \
+  case n:          \
+    return &k_slab_xy_reg<float, n, n, kSlabSB>;
+    PW_REG_SIZES(PW_CASE)
+#undef PW_CASE
+    default:
+      return nullptr;
+  }
 }
 
 template <typename T>
@@ -357,6 +523,10 @@ extern "C" int pw_ctx_create(pw_ctx** out, const int grid[3], const double super
       return fail(PW_ERR_UNSUPPORTED, "pw_ctx_create: grid size %d is not 2/3/5-smooth", grid[d]);
   if (grid[2] >= 32768) return fail(PW_ERR_UNSUPPORTED, "pw_ctx_create: n3 too large");
   pw_ctx* c = new pw_ctx();
+  {
+    const char* sl = getenv("PW_SLAB");
+    c->use_reg_slab = !(sl && std::string(sl) == "lds");
+  }
   c->n1 = grid[0]; c->n2 = grid[1]; c->n3 = grid[2];
   c->h1 = grid[0] / 2 + 1;
   c->N = (long)grid[0] * grid[1] * grid[2];
@@ -595,10 +765,21 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
                      c->d_pt_sz, tw3);
   PW_HIP(hipGetLastError());
   PW_TRY(mark(c, 1));
-  auto kslab = slab_kernel<T>(c->n1, c->n2);
-  PW_TRY(ensure_lds((const void*)kslab, lds2));
-  hipLaunchKernelGGL(kslab, dim3(c->n3, nb), dim3(256), lds2, c->stream, (const C*)t1, t2, c->n1,
-                     c->n2, c->n3, c->sp.nstick, c->sp.kxmax, c->d_pos, c->d_mpos, vt, tw1, tw2, rp);
+  slab_reg_fn kreg = (sizeof(T) == 4 && c->use_reg_slab) ? slab_reg_kernel(c->n1, c->n2) : nullptr;
+  if (kreg) {
+    // one z plane of kSlabSB bands per workgroup, V plane + y-space in LDS
+    const size_t ldsr = sizeof(float2) * ((size_t)kSlabSB * (c->sp.kxmax + 1) * c->n2) +
+                        sizeof(float) * (size_t)c->n2 * (c->n1 + 1) + sizeof(int) * 2 * (size_t)c->sp.nstick;
+    PW_TRY(ensure_lds((const void*)kreg, ldsr));
+    hipLaunchKernelGGL(kreg, dim3(c->n3, (nb + kSlabSB - 1) / kSlabSB), dim3(256), ldsr, c->stream,
+                       (const float2*)t1, (float2*)t2, c->n3, c->sp.nstick, c->sp.kxmax, nb, c->d_pos, c->d_mpos,
+                       (const float*)vt);
+  } else {
+    auto kslab = slab_kernel<T>(c->n1, c->n2);
+    PW_TRY(ensure_lds((const void*)kslab, lds2));
+    hipLaunchKernelGGL(kslab, dim3(c->n3, nb), dim3(256), lds2, c->stream, (const C*)t1, t2, c->n1,
+                       c->n2, c->n3, c->sp.nstick, c->sp.kxmax, c->d_pos, c->d_mpos, vt, tw1, tw2, rp);
+  }
   PW_HIP(hipGetLastError());
   PW_TRY(mark(c, 2));
   auto kfw = fwd_z_kernel<T>(c->n3);

@@ -12,6 +12,12 @@
 
 namespace pw {
 
+// Workgroup barrier for LDS traffic only: waits for this wavefront's LDS
+// operations but leaves its global loads and stores in flight (a full
+// __syncthreads also drains those).  Only for phases whose cross-wavefront
+// data goes through LDS.
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
 // ---- error plumbing: a thread-local message, integer status codes ----
 void set_error(const char* fmt, ...);
 int fail(int code, const char* fmt, ...);
@@ -163,6 +169,7 @@ struct pw_ctx {
   double* d_coef = nullptr;
   float* d_coef_f = nullptr;
   bool use_graphs = true;
+  bool use_reg_slab = true;  // FP32 register slab kernel where compiled (env PW_SLAB=lds: the LDS kernel)
   hipStream_t gstream = nullptr;
   hipEvent_t gev[2] = {nullptr, nullptr};
   hipGraphExec_t fg_exec = nullptr;

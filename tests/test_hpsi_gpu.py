@@ -52,6 +52,29 @@ def test_apply_h_fp32(ctx8, si8):
     assert _rel(HX, ref) < 2e-5
 
 
+@pytest.mark.parametrize("name,nb", [("si8_32", 13), ("rand16_48", 8), ("si64_60", 24)])
+def test_fp32_register_slab_kernel(name, nb, monkeypatch):
+    """The FP32 register slab kernel (one z plane of 8 bands per workgroup,
+    ragged last group) against the oracle and against the LDS slab kernel
+    (same arithmetic; agreement to FP32 rounding)."""
+    import torch
+    from pwdft_develop_amd.ops import Context
+    prob = W.WORKLOADS[name](nb=nb).build()
+    print(f"case: {name} fp32 slab kernels nb={nb}", flush=True)
+    ref = O.Hamiltonian(prob).apply_h(prob.psi)
+    monkeypatch.delenv("PW_SLAB", raising=False)
+    creg = Context(prob)
+    monkeypatch.setenv("PW_SLAB", "lds")
+    clds = Context(prob)
+    X = creg.to_device(prob.psi, torch.complex64)
+    a = creg.apply_h(X).cpu().numpy()
+    b = clds.apply_h(X).cpu().numpy()
+    assert _rel(a, ref) < 2e-5
+    assert _rel(a, b) < 2e-6
+    creg.close()
+    clds.close()
+
+
 def test_teter(ctx8, si8):
     X = ctx8.to_device(si8.psi[:3])
     T = ctx8.teter_precond(X).cpu().numpy()
