@@ -44,8 +44,7 @@ constexpr int kEigMaxN = 1024;      // LDS bound of k_sytrd
 // all threads.  sred holds one double per wavefront.
 template <int NT>
 __device__ __forceinline__ double block_sum(double v, double* sred) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  v = wave_sum(v);
   lds_barrier();
   if ((threadIdx.x & 63) == 0) sred[threadIdx.x >> 6] = v;
   lds_barrier();
@@ -105,6 +104,7 @@ k_sytrd(double* __restrict__ A, int n, int lda, int nbk, double* __restrict__ d,
         if (r < n) {
           if (jj > 0) {
             double u = 0.0;
+#pragma unroll 8
             for (int q = 0; q < jj; q++) u += Vp[q * n + r] * Wp[q * n + k] + Wp[q * n + r] * Vp[q * n + k];
             x -= u;
           }
@@ -177,11 +177,8 @@ k_sytrd(double* __restrict__ A, int n, int lda, int nbk, double* __restrict__ d,
           sa += Wp[q * n + k + 1 + i] * vi;
           sb += Vp[q * n + k + 1 + i] * vi;
         }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-          sa += __shfl_xor(sa, o, 64);
-          sb += __shfl_xor(sb, o, 64);
-        }
+        sa = wave_sum(sa);
+        sb = wave_sum(sb);
         if (lane == 0) {
           al[q] = sa;
           be[q] = sb;
@@ -195,6 +192,7 @@ k_sytrd(double* __restrict__ A, int n, int lda, int nbk, double* __restrict__ d,
         double p = 0.0;
         for (int g2 = 0; g2 < G; g2++) p += red[g2 * (nrc * 64) + i];
         const int r = k + 1 + i;
+#pragma unroll 8
         for (int q = 0; q < jj; q++) p -= Vp[q * n + r] * al[q] + Wp[q * n + r] * be[q];
         p *= tk;
         pl[c] = p;

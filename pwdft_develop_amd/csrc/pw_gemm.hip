@@ -206,10 +206,97 @@ k_gemm_nn(const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb, T*
     }
 }
 
+// Single-column products (the one-band H x of Lanczos): GEMV kernels.
+// C(n1) = A^T b over K: workgroup (i, split) sums column i of A against b
+// over one K range, 256 threads with 8 loads in flight each; partials go
+// through k_gemm_tn_reduce like the split-K GEMM.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_gemv_t_partial(const T* __restrict__ A, int lda, const T* __restrict__ b, int K, int kc, int n1,
+                 T* __restrict__ P) {
+  const int i = blockIdx.x, split = blockIdx.y;
+  const int kb = split * kc, ke = min(K, kb + kc);
+  const T* a = A + (size_t)i * lda;
+  T s = 0;
+  for (int r0 = kb + threadIdx.x; r0 < ke; r0 += 8 * 256) {
+    T x[8], y[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int r = r0 + 256 * u;
+      x[u] = r < ke ? a[r] : (T)0;
+      y[u] = r < ke ? b[r] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) s += x[u] * y[u];
+  }
+  __shared__ T red[256];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) P[(size_t)split * n1 + i] = red[0];
+}
+
+// y = alpha A x + beta y, A m x k: workgroup of 256 threads per 64 rows; wave
+// w sums columns j = w (mod 4) for its 64 rows (16 loads in flight per lane),
+// the four partial rows meet in LDS in a fixed order.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_gemv_n(const T* __restrict__ A, int lda, const T* __restrict__ x, T* __restrict__ y, int m, int k, T alpha,
+         T beta) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int row = blockIdx.x * 64 + lane;
+  T s = 0;
+  if (row < m) {
+    for (int j0 = w; j0 < k; j0 += 16 * 4) {
+      T a[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const int j = j0 + 4 * u;
+        a[u] = j < k ? A[(size_t)j * lda + row] : (T)0;
+      }
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const int j = j0 + 4 * u;
+        if (j < k) s += a[u] * x[j];
+      }
+    }
+  }
+  __shared__ T red[4][64];
+  red[w][lane] = s;
+  __syncthreads();
+  if (w == 0 && row < m) {
+    const T v = alpha * (red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane]);
+    y[row] = (beta == (T)0) ? v : v + beta * y[row];
+  }
+}
+
 template <typename T>
 int gemm_tn(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, int ldb, T* C, int ldc,
             const T* rowscale, T beta) {
   if (n1 <= 0 || n2 <= 0) return PW_OK;
+  if (n2 == 1) {
+    const int nsplit = std::max(1, std::min(64, (K + 2047) / 2048));
+    const int kc = (K + nsplit - 1) / nsplit;
+    const size_t need = sizeof(T) * (size_t)nsplit * n1;
+    if (need > c->gemm_ws_bytes) {
+      invalidate_graph(c);
+      if (c->d_gemm_ws) (void)hipFree(c->d_gemm_ws);
+      c->d_gemm_ws = nullptr;
+      c->gemm_ws_bytes = 0;
+      PW_HIP(hipMalloc(&c->d_gemm_ws, need));
+      c->gemm_ws_bytes = need;
+    }
+    T* P = (T*)c->d_gemm_ws;
+    hipLaunchKernelGGL(k_gemv_t_partial<T>, dim3(n1, nsplit), dim3(256), 0, c->stream, A, lda, B, K, kc, n1, P);
+    const int g = std::max(1, std::min((n1 + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_gemm_tn_reduce<T>, dim3(g), dim3(256), 0, c->stream, (const T*)P, nsplit, n1, 1, C, ldc,
+                       rowscale, beta);
+    PW_HIP(hipGetLastError());
+    return PW_OK;
+  }
   const int ti = (n1 + BM - 1) / BM, tj = (n2 + BN - 1) / BN;
   const int tiles = ti * tj;
   int nsplit = std::max(1, std::min((1024 + tiles - 1) / tiles, (K + 255) / 256));
@@ -240,6 +327,11 @@ template <typename T>
 int gemm_nn(pw_ctx* c, int m, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T beta, T* Y,
             int ldy) {
   if (m <= 0 || n <= 0) return PW_OK;
+  if (n == 1) {
+    hipLaunchKernelGGL(k_gemv_n<T>, dim3((m + 63) / 64), dim3(256), 0, c->stream, A, lda, B, Y, m, k, alpha, beta);
+    PW_HIP(hipGetLastError());
+    return PW_OK;
+  }
   hipLaunchKernelGGL(k_gemm_nn<T>, dim3((m + BM - 1) / BM, (n + BN - 1) / BN), dim3(256), 0, c->stream, A, lda, B,
                      ldb, Y, ldy, m, n, k, alpha, beta);
   PW_HIP(hipGetLastError());

@@ -18,22 +18,6 @@
 
 namespace pw {
 
-// Y = (HX - c X) * s     (n reals)
-template <typename T>
-__global__ void k_cheb_first(T* __restrict__ Y, const T* __restrict__ HX, const T* __restrict__ X, T c,
-                             T s, long n) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    Y[i] = (HX[i] - c * X[i]) * s;
-}
-
-// Yn = (HY - c Y) * s1 - s2 * Xp
-template <typename T>
-__global__ void k_cheb_step(T* __restrict__ Yn, const T* __restrict__ HY, const T* __restrict__ Y,
-                            const T* __restrict__ Xp, T c, T s1, T s2, long n) {
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    Yn[i] = (HY[i] - c * Y[i]) * s1 - s2 * Xp[i];
-}
-
 template <typename A, typename B>
 __global__ void k_convert(B* __restrict__ out, const A* __restrict__ in, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -96,11 +80,24 @@ __global__ void k_sign_fix(double* __restrict__ Q, int n, int ld) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) q[i] = -q[i];
 }
 
-// a = dot(x, y) over n reals into out[0] (one block)
-__global__ void k_dot(const double* __restrict__ x, const double* __restrict__ y, long n, double* out) {
+// a = dot(x, y) over n reals into out[0]: one 1024-thread block, each
+// thread streaming its elements in batches of 8 independent loads, then a
+// fixed-order tree (deterministic).
+__global__ void __launch_bounds__(1024) k_dot(const double* __restrict__ x, const double* __restrict__ y, long n,
+                                              double* out) {
   double s = 0.0;
-  for (long i = threadIdx.x; i < n; i += blockDim.x) s += x[i] * y[i];
-  __shared__ double red[256];
+  for (long i0 = threadIdx.x; i0 < n; i0 += 8L * blockDim.x) {
+    double a[8], b[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const long i = i0 + (long)u * blockDim.x;
+      a[u] = i < n ? x[i] : 0.0;
+      b[u] = i < n ? y[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) s += a[u] * b[u];
+  }
+  __shared__ double red[1024];
   red[threadIdx.x] = s;
   __syncthreads();
   for (int w = blockDim.x / 2; w > 0; w >>= 1) {
@@ -136,8 +133,8 @@ static inline int grid_for(long n) { return (int)std::min<long>((n + 255) / 256,
 
 // Symmetric eigen-decomposition in place: A (n x n, upper triangle, lda)
 // <- eigenvectors (largest |component| positive), w (device) <- eigenvalues
-// ascending.  Block one-sided Jacobi (pw_eig.hip) by default; rocSOLVER
-// dsyevd when PW_EIGEN=rocsolver.
+// ascending.  The tridiagonal solver of pw_eig.hip up to n = 1024; rocSOLVER
+// dsyevd beyond, or when PW_EIGEN=rocsolver.
 int sym_eig(pw_ctx* c, int n, double* A, int lda, double* w, double* work, int* info) {
   hipStream_t s = c->stream;
   static const bool use_rocsolver = [] {
@@ -219,17 +216,17 @@ extern "C" int pw_lanczos(pw_ctx* c, int k, const void* v0, double* theta, doubl
   hipStream_t s = c->stream;
   PW_HIP(hipMemsetAsync(vp, 0, sizeof(double) * m, s));
   PW_HIP(hipMemsetAsync(ab, 0, sizeof(double) * 4, s));
-  hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, (const double*)v0, (const double*)v0, m, ab + 2);
+  hipLaunchKernelGGL(k_dot, dim3(1), dim3(1024), 0, s, (const double*)v0, (const double*)v0, m, ab + 2);
   hipLaunchKernelGGL(k_scale_to, dim3(grid_for(m)), dim3(256), 0, s, v, (const double*)v0, ab + 2, m);
   // alpha_j, ||f_j||^2 recorded on the device; one readback at the end
   double* hist = ab + 4;  // 2k <= 128 entries; d_red holds max(max_bands, 128) + 8
   std::vector<double> al(k), be(k);
   for (int j = 0; j < k; j++) {
     PW_TRY(pw_apply_h(c, PW_F64, 1, v, f));
-    hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, (const double*)v, (const double*)f, m, ab);
+    hipLaunchKernelGGL(k_dot, dim3(1), dim3(1024), 0, s, (const double*)v, (const double*)f, m, ab);
     hipLaunchKernelGGL(k_lanczos_update, dim3(grid_for(m)), dim3(256), 0, s, f, (const double*)v,
                        (const double*)vp, (const double*)ab, m);
-    hipLaunchKernelGGL(k_dot, dim3(1), dim3(256), 0, s, (const double*)f, (const double*)f, m, ab + 2);
+    hipLaunchKernelGGL(k_dot, dim3(1), dim3(1024), 0, s, (const double*)f, (const double*)f, m, ab + 2);
     hipLaunchKernelGGL(k_lanczos_record, dim3(1), dim3(1), 0, s, ab, hist + 2 * j);
     if (j + 1 < k) {
       // vp <- v ; v <- f / beta ; beta (for the next step) = ||f|| stays on the device
