@@ -183,6 +183,15 @@ int pw_lobpcg(pw_ctx* ctx, int nb, int numEig, int scfIter, int eigMaxIter,
               double eigMinTolerance, double eigTolerance, void* X,
               double* eigval, double* resval, int* iters);
 
+/* EigenSolver::PPCGSolveReal(numEig, scfIter, eigMaxIter, eigMinTolerance,
+ * eigTolerance), sphere-cut branch (src/EigenSolver/ppcg.cpp:59-576), with
+ * sbSize = esdfParam.PPCGsbSize (the sub-block of the small Sygvd problems).
+ * X, eigval, resval, iters as pw_lobpcg.  PW_ERR_LINALG carries the
+ * reference's "sygvd returned with info" / Potrf failures. */
+int pw_ppcg(pw_ctx* ctx, int nb, int numEig, int scfIter, int eigMaxIter,
+            double eigMinTolerance, double eigTolerance, int sbSize, void* X,
+            double* eigval, double* resval, int* iters);
+
 /* ------------------------------------------------------------------ */
 /* Screened exact exchange.  pw_set_exx replaces KohnSham::InitializeEXX
  * (src/SCF/hamiltonian.cpp:3454-3551: exxDiv, kernel

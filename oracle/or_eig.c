@@ -4,6 +4,7 @@
  *   or_lobpcg        EigenSolver::LOBPCGSolveReal, sphere branch
  *                    (lobpcg.cpp:119-576), one rank: the Alltoall transposes
  *                    and MPI reductions are identities.
+ *   or_ppcg          EigenSolver::PPCGSolveReal, sphere branch (ppcg.cpp:119-576).
  *   or_lanczos       k-step Lanczos bound of Zhou & Li, Linear Algebra Appl.
  *                    435 (2011) 480: b_up = theta_k + |beta_k|.
  *   or_cheb_filter   scaled Chebyshev filter of Zhou, Saad, Tiago &
@@ -271,4 +272,169 @@ int or_chebfsi_sweep(or_ham* H, int nb, int deg, int k, const double* v0, const 
   rc = or_cheb_filter(H, nb, deg, a, bup, aL, X);
   if (rc) return rc;
   return or_rayleigh_ritz(H, nb, X, eig, res);
+}
+
+/* EigenSolver::PPCGSolveReal, sphere branch (ppcg.cpp:119-576), one rank:
+ * no locking; numSet 2 on the first iteration, 3 after; the block of
+ * sbSize columns k solves Sygvd on its (x | w | p) triple and updates
+ * p <- w cw (+ p cp), x <- x cx + p (and the A-images alike); Cholesky QR
+ * (X and AX by U^{-1}) closes every iteration; post-processing Syevd of
+ * X^T A X, rotation and residual norms. */
+static void pp_block_gram(int m, int sb, const double* a, const double* b, double* M, int ldm, int r0, int c0) {
+  double* T = (double*)malloc(sizeof(double) * (size_t)sb * sb);
+  or_gemm_tn(m, sb, sb, a, b, T);
+  put_block(M, ldm, r0, c0, sb, sb, T);
+  free(T);
+}
+
+int or_ppcg(or_ham* H, int nb, int numEig, int scfIter, int eigMaxIter, double eigMinTolerance,
+            double eigTolerance, int sbSize, double* psi, double* eigval, double* resval, int* iters_out) {
+  (void)eigMinTolerance;
+  extern int or_ham_npw(const or_ham*);
+  const int npw = or_ham_npw(H);
+  const int m = 2 * npw, width = nb;
+  if (numEig > width) return -5; /* ppcg.cpp:105-112 */
+  if (sbSize < 1) return -1;
+  const size_t blk = (size_t)m * width;
+  double* S = (double*)calloc(3 * blk, sizeof(double));
+  double* AS = (double*)calloc(3 * blk, sizeof(double));
+  double *X = S, *W = S + blk, *P = S + 2 * blk;
+  double *AX = AS, *AW = AS + blk, *AP = AS + 2 * blk;
+  double* XTX = (double*)calloc((size_t)width * width, sizeof(double));
+  double* Xtemp = (double*)calloc(blk, sizeof(double));
+  double* tmp = (double*)calloc(blk, sizeof(double));
+  double* resNorm = (double*)calloc(width, sizeof(double));
+  double* eigValS = (double*)calloc(3 * width, sizeof(double));
+  const int sb3 = 3 * sbSize;
+  double* AMat = (double*)calloc((size_t)sb3 * sb3, sizeof(double));
+  double* BMat = (double*)calloc((size_t)sb3 * sb3, sizeof(double));
+  double* eigs = (double*)calloc(sb3, sizeof(double));
+  int rc = 0, iter = 0, numSet = 2;
+  double resMax = 0.0;
+
+  memcpy(X, psi, sizeof(double) * blk);
+  if (scfIter == 1) {
+    rc = or_orthogonalize(npw, width, X);
+    if (rc) goto done;
+  }
+  or_apply_h(H, width, X, AX);
+  do {
+    iter++;
+    numSet = (iter == 1) ? 2 : 3;
+    or_gemm_tn(m, width, width, X, AX, XTX);
+    memcpy(Xtemp, AX, sizeof(double) * blk);
+    or_gemm_nn(m, width, width, -1.0, X, m, XTX, width, 1.0, Xtemp, m);
+    for (int k = 0; k < width; k++) {
+      const double r = sqrt(energy(m, Xtemp + (size_t)m * k));
+      const double d = fabs(XTX[k + (size_t)width * k]);
+      resNorm[k] = r / (d > 1.0 ? d : 1.0);
+    }
+    resMax = resNorm[0];
+    for (int k = 1; k < numEig; k++)
+      if (resNorm[k] > resMax) resMax = resNorm[k];
+    /* W = T R, AW = A W */
+    memset(W, 0, sizeof(double) * blk);
+    or_teter_precond(H, width, Xtemp, W);
+    or_apply_h(H, width, W, AW);
+    /* W -= X (X'W), AW -= AX (X'W); normalise */
+    or_gemm_tn(m, width, width, X, W, XTX);
+    or_gemm_nn(m, width, width, -1.0, X, m, XTX, width, 1.0, W, m);
+    or_gemm_nn(m, width, width, -1.0, AX, m, XTX, width, 1.0, AW, m);
+    for (int k = 0; k < width; k++) {
+      const double s = 1.0 / sqrt(energy(m, W + (size_t)m * k));
+      for (int i = 0; i < m; i++) {
+        W[(size_t)m * k + i] *= s;
+        AW[(size_t)m * k + i] *= s;
+      }
+    }
+    if (numSet == 3) {
+      or_gemm_tn(m, width, width, X, P, XTX);
+      or_gemm_nn(m, width, width, -1.0, X, m, XTX, width, 1.0, P, m);
+      or_gemm_nn(m, width, width, -1.0, AX, m, XTX, width, 1.0, AP, m);
+      for (int k = 0; k < width; k++) {
+        const double s = 1.0 / sqrt(energy(m, P + (size_t)m * k));
+        for (int i = 0; i < m; i++) {
+          P[(size_t)m * k + i] *= s;
+          AP[(size_t)m * k + i] *= s;
+        }
+      }
+    }
+    const int nsb = (width + sbSize - 1) / sbSize;
+    for (int k = 0; k < nsb; k++) {
+      const int sb = (k == nsb - 1 && width % sbSize) ? width % sbSize : sbSize;
+      const int ld = 3 * sb;
+      const size_t off = (size_t)m * sbSize * k;
+      double *x = X + off, *w = W + off, *p = P + off, *ax = AX + off, *aw = AW + off, *ap = AP + off;
+      memset(AMat, 0, sizeof(double) * (size_t)sb3 * sb3);
+      memset(BMat, 0, sizeof(double) * (size_t)sb3 * sb3);
+      pp_block_gram(m, sb, x, ax, AMat, ld, 0, 0);
+      pp_block_gram(m, sb, w, aw, AMat, ld, sb, sb);
+      pp_block_gram(m, sb, x, aw, AMat, ld, 0, sb);
+      pp_block_gram(m, sb, x, x, BMat, ld, 0, 0);
+      pp_block_gram(m, sb, w, w, BMat, ld, sb, sb);
+      pp_block_gram(m, sb, x, w, BMat, ld, 0, sb);
+      if (numSet == 3) {
+        pp_block_gram(m, sb, p, ap, AMat, ld, 2 * sb, 2 * sb);
+        pp_block_gram(m, sb, x, ap, AMat, ld, 0, 2 * sb);
+        pp_block_gram(m, sb, w, ap, AMat, ld, sb, 2 * sb);
+        pp_block_gram(m, sb, p, p, BMat, ld, 2 * sb, 2 * sb);
+        pp_block_gram(m, sb, x, p, BMat, ld, 0, 2 * sb);
+        pp_block_gram(m, sb, w, p, BMat, ld, sb, 2 * sb);
+      }
+      const int dim = (numSet == 3) ? 3 * sb : 2 * sb;
+      rc = or_sygvd(dim, AMat, ld, BMat, ld, eigs);
+      if (rc) goto done;
+      const double* cx = AMat;
+      const double* cw = AMat + sb;
+      const double* cp = AMat + 2 * sb;
+      if (numSet == 3) {
+        or_gemm_nn(m, sb, sb, 1.0, p, m, cp, ld, 0.0, tmp, m);
+        or_gemm_nn(m, sb, sb, 1.0, w, m, cw, ld, 1.0, tmp, m);
+        memcpy(p, tmp, sizeof(double) * (size_t)m * sb);
+        or_gemm_nn(m, sb, sb, 1.0, ap, m, cp, ld, 0.0, tmp, m);
+        or_gemm_nn(m, sb, sb, 1.0, aw, m, cw, ld, 1.0, tmp, m);
+        memcpy(ap, tmp, sizeof(double) * (size_t)m * sb);
+      } else {
+        or_gemm_nn(m, sb, sb, 1.0, w, m, cw, ld, 0.0, p, m);
+        or_gemm_nn(m, sb, sb, 1.0, aw, m, cw, ld, 0.0, ap, m);
+      }
+      memcpy(tmp, p, sizeof(double) * (size_t)m * sb);
+      or_gemm_nn(m, sb, sb, 1.0, x, m, cx, ld, 1.0, tmp, m);
+      memcpy(x, tmp, sizeof(double) * (size_t)m * sb);
+      memcpy(tmp, ap, sizeof(double) * (size_t)m * sb);
+      or_gemm_nn(m, sb, sb, 1.0, ax, m, cx, ld, 1.0, tmp, m);
+      memcpy(ax, tmp, sizeof(double) * (size_t)m * sb);
+    }
+    /* Cholesky QR of X; AX transformed by the same U^{-1} */
+    or_gemm_tn(m, width, width, X, X, XTX);
+    if (or_potrf_upper(width, XTX, width) != 0) { rc = -4; goto done; }
+    or_trsm_right_upper(m, width, XTX, width, X, m);
+    or_trsm_right_upper(m, width, XTX, width, AX, m);
+  } while (iter < eigMaxIter && resMax > eigTolerance);
+
+  /* post-processing (ppcg.cpp:530-576) */
+  or_gemm_tn(m, width, width, X, AX, XTX);
+  rc = or_syev(width, XTX, width, eigValS);
+  if (rc) goto done;
+  or_gemm_nn(m, width, width, 1.0, X, m, XTX, width, 0.0, Xtemp, m);
+  memcpy(X, Xtemp, sizeof(double) * blk);
+  or_gemm_nn(m, width, width, 1.0, AX, m, XTX, width, 0.0, Xtemp, m);
+  memcpy(AX, Xtemp, sizeof(double) * blk);
+  for (int k = 0; k < width; k++) {
+    double s = 0.0;
+    for (int i = 0; i < m; i++) {
+      const double r = AX[(size_t)m * k + i] - X[(size_t)m * k + i] * eigValS[k];
+      s += r * r;
+    }
+    const double d = fabs(eigValS[k]);
+    resNorm[k] = sqrt(s) / (d > 1.0 ? d : 1.0);
+  }
+  memcpy(psi, X, sizeof(double) * blk);
+  memcpy(eigval, eigValS, sizeof(double) * width);
+  memcpy(resval, resNorm, sizeof(double) * width);
+done:
+  if (iters_out) *iters_out = iter;
+  free(S); free(AS); free(XTX); free(Xtemp); free(tmp); free(resNorm); free(eigValS);
+  free(AMat); free(BMat); free(eigs);
+  return rc;
 }

@@ -280,3 +280,44 @@ int or_orthogonalize(int npw, int nb, double* X) {
   free(S);
   return info == 0 ? 0 : -4;
 }
+
+/* lapack::Sygvd(1, 'V', 'U', n, A, lda, B, ldb, w) (include/lapack.hpp):
+ * A x = lambda B x with B = U^T U (Potrf 'U'), C = U^{-T} A U^{-1} (upper
+ * triangles read), C = Y diag(w) Y^T, x = U^{-1} y; A <- X (B-orthonormal
+ * columns, each with its largest-magnitude component positive), B <- U.
+ * Returns 0, or -4 when B is not positive definite. */
+int or_sygvd(int n, double* A, int lda, double* B, int ldb, double* w) {
+  if (n <= 0) return 0;
+  for (int j = 0; j < n; j++)
+    for (int i = j + 1; i < n; i++) B[i + (size_t)ldb * j] = 0.0;
+  if (or_potrf_upper(n, B, ldb) != 0) return -4;
+  double* C = (double*)malloc(sizeof(double) * (size_t)n * n);
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) C[i + (size_t)n * j] = (i <= j) ? A[i + (size_t)lda * j] : A[j + (size_t)lda * i];
+  /* C <- C U^{-1} (rows), then C <- U^{-T} C */
+  or_trsm_right_upper(n, n, B, ldb, C, n);
+  for (int j = 0; j < n; j++)      /* solve U^T Z = C column by column (forward substitution) */
+    for (int i = 0; i < n; i++) {
+      double s = C[i + (size_t)n * j];
+      for (int k = 0; k < i; k++) s -= B[k + (size_t)ldb * i] * C[k + (size_t)n * j];
+      C[i + (size_t)n * j] = s / B[i + (size_t)ldb * i];
+    }
+  int rc = or_syev(n, C, n, w);
+  /* X = U^{-1} Y: back substitution per column */
+  for (int j = 0; j < n; j++)
+    for (int i = n - 1; i >= 0; i--) {
+      double s = C[i + (size_t)n * j];
+      for (int k = i + 1; k < n; k++) s -= B[i + (size_t)ldb * k] * C[k + (size_t)n * j];
+      C[i + (size_t)n * j] = s / B[i + (size_t)ldb * i];
+    }
+  for (int j = 0; j < n; j++) {
+    const double* z = C + (size_t)n * j;
+    int im = 0;
+    for (int i = 1; i < n; i++)
+      if (fabs(z[i]) > fabs(z[im])) im = i;
+    const double sg = z[im] < 0 ? -1.0 : 1.0;
+    for (int i = 0; i < n; i++) A[i + (size_t)lda * j] = sg * z[i];
+  }
+  free(C);
+  return rc ? -2 : 0;
+}

@@ -104,6 +104,10 @@ void or_trsm_right_upper(int m, int n, const double* U, int ldu, double* X, int 
 /* Symmetric eigensolver (upper triangle used; vectors overwrite A),
  * eigenvalues ascending (lapack Syevd 'V','U'). */
 int or_syev(int n, double* A, int lda, double* w);
+/* lapack::Sygvd(1, 'V', 'U') (include/lapack.hpp): A <- B-orthonormal
+ * eigenvectors (largest-magnitude component positive), w ascending,
+ * B <- its Cholesky factor U.  -4 when B is not positive definite. */
+int or_sygvd(int n, double* A, int lda, double* B, int ldb, double* w);
 /* EigenSolver::SolveGenEig sequential branch (eigensolver.cpp:224-297). */
 int or_solve_gen_eig(int lda, int numCol, int width, double* A, double* B, double* eig);
 /* EigenSolver::Orthogonalize complex-X branch (eigensolver.cpp:386-469),
@@ -115,6 +119,11 @@ int or_orthogonalize(int npw, int nb, double* X);
 int or_lobpcg(or_ham* H, int nb, int numEig, int scfIter, int eigMaxIter,
               double eigMinTolerance, double eigTolerance, double* psi,
               double* eigval, double* resval, int* iters_out);
+
+/* EigenSolver::PPCGSolveReal sphere branch (ppcg.cpp:119-576). */
+int or_ppcg(or_ham* H, int nb, int numEig, int scfIter, int eigMaxIter,
+            double eigMinTolerance, double eigTolerance, int sbSize, double* psi,
+            double* eigval, double* resval, int* iters_out);
 
 /* k-step Lanczos on v0 (normalised internally): Ritz values of T_k and the
  * upper bound b_up = theta_k + |beta_k| (Zhou & Li 2011). */

@@ -46,6 +46,9 @@ def lib():
                                      C.c_int, _dp, _dp, C.POINTER(C.c_double)]
         L.or_ham_set_threads.argtypes = [C.c_void_p, C.c_int]
         L.or_ham_build_ace.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.or_ppcg.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                              C.c_int, _dp, _dp, _dp, C.POINTER(C.c_int)]
+        L.or_sygvd.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, _dp]
         for f in ("or_mult_spinor", "or_add_exx", "or_nonlocal_multx", "or_apply_h",
                   "or_teter_precond"):
             getattr(L, f).argtypes = [C.c_void_p, C.c_int, _dp, _dp]
@@ -211,6 +214,18 @@ class Hamiltonian:
                              C.byref(it)), "lobpcg")
         return buf.view(np.complex128).reshape(psi.shape), ev, rv, it.value
 
+    def ppcg(self, psi, num_eig=None, scf_iter=1, max_iter=3, tol=1e-20, sb_size=32):
+        """EigenSolver::PPCGSolveReal (ppcg.cpp:119-576)."""
+        psi = np.ascontiguousarray(psi, dtype=np.complex128).copy()
+        nb = psi.shape[0]
+        ev = np.zeros(nb)
+        rv = np.zeros(nb)
+        it = C.c_int(0)
+        buf = _r(psi)
+        _chk(lib().or_ppcg(self.h, nb, num_eig or nb, scf_iter, max_iter, 0.0, tol, sb_size, buf, ev, rv,
+                           C.byref(it)), "ppcg")
+        return buf.view(np.complex128).reshape(psi.shape), ev, rv, it.value
+
     def lanczos(self, k, v0):
         th = np.zeros(k)
         b = C.c_double(0.0)
@@ -277,6 +292,17 @@ def syev(A):
     w = np.zeros(n)
     _chk(lib().or_syev(n, buf, n, w), "syev")
     return w, buf.reshape(n, n, order="F")
+
+
+def sygvd(A, B):
+    """lapack::Sygvd(1, 'V', 'U'): (eigenvalues, B-orthonormal eigenvectors)."""
+    A = np.array(A, dtype=np.float64)
+    n = A.shape[0]
+    a = A.reshape(-1, order="F").copy()
+    b = np.array(B, dtype=np.float64).reshape(-1, order="F").copy()
+    w = np.zeros(n)
+    _chk(lib().or_sygvd(n, a, n, b, n, w), "sygvd")
+    return w, a.reshape(n, n, order="F")
 
 
 def potrf_upper(A):

@@ -80,6 +80,8 @@ def lib():
         "pw_exx_divergence": [_ip, _dp, C.c_double, C.c_double, C.c_int, C.POINTER(C.c_double)],
         "pw_lobpcg": [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, _vp, _dp, _dp,
                       C.POINTER(C.c_int)],
+        "pw_ppcg": [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _vp, _dp, _dp,
+                    C.POINTER(C.c_int)],
         "pw_set_timing": [_vp, C.c_int],
         "pw_get_timing": [_vp, _dp, C.POINTER(C.c_int)],
     }

@@ -130,6 +130,14 @@ class Context:
                                    C.c_void_p(X.data_ptr()), e, r, C.byref(it)))
         return X, e, r, it.value
 
+    def ppcg(self, X, num_eig=None, scf_iter=1, max_iter=3, tol=1e-20, sb_size=32):
+        nb = X.shape[0]
+        e, r = np.zeros(nb), np.zeros(nb)
+        it = C.c_int(0)
+        check(_lib.lib().pw_ppcg(self.h, nb, num_eig or nb, scf_iter, max_iter, 0.0, tol, sb_size,
+                                 C.c_void_p(X.data_ptr()), e, r, C.byref(it)))
+        return X, e, r, it.value
+
     def set_exx(self, ecut, mu, frac, psi_occ, occ, divtype=1):
         """psi_occ: device FP64 block (nocc, npw)."""
         n = 0 if psi_occ is None else psi_occ.shape[0]
