@@ -150,7 +150,12 @@ def main():
     slab_ms = stages["f32"][1]
     slab_b = slab_bytes(prob, info["nstick"], nb, 4)
     achieved = slab_b / (slab_ms * 1e-3) / 1e9
-    traffic = pmc_traffic(os.path.join(ROOT, "profiles", f"pmc_traffic_{prob.name}.md"), "k_slab_xy<float")
+    # FP32 square grids of the register-FFT sizes take k_slab_xy_reg (8 bands
+    # per z plane), others the LDS kernel k_slab_xy
+    reg = prob.grid[0] == prob.grid[1] and prob.grid[0] in (24, 30, 32, 36, 40, 48, 60, 64, 72)
+    kname = (f"k_slab_xy_reg<float, {prob.grid[0]}, {prob.grid[1]}, 8>" if reg
+             else f"k_slab_xy<float, {prob.grid[0]}, {prob.grid[1]}>")
+    traffic = pmc_traffic(os.path.join(ROOT, "profiles", f"pmc_traffic_{prob.name}.md"), kname)
 
     def staged(tag, p):
         local_ms = sum(stages[tag][:3])
@@ -180,12 +185,12 @@ def main():
                         "nwanted": nwant},
         "hpsi_band_applications_per_s": {k: round(v, 1) for k, v in hpsi.items()},
         "stages_ms_per_apply": {k: [round(x, 4) for x in v] for k, v in stages.items()},
-        "roofline": {"bound": "hbm", "kernel": f"k_slab_xy<float, {prob.grid[0]}, {prob.grid[1]}>",
+        "roofline": {"bound": "hbm", "kernel": kname,
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": slab_b, "launch_ms": round(slab_ms, 4),
                      "model": "per launch: nb x (n3 x nstick complex sticks read + written) + V read"},
-        "staged_model": {"kernels": "k_sticks_inverse_z + k_slab_xy + k_sticks_forward_z",
+        "staged_model": {"kernels": f"k_sticks_inverse_z + {kname.split('<')[0]} + k_sticks_forward_z",
                          "bytes_per_band": "p (9 npw + 6 Nh + 5 N)",
                          "f32": {"GBs": round(st32, 1), "frac": round(st32 / HBM_PEAK_GBS, 4),
                                  "launch_ms": round(local_ms, 4)},

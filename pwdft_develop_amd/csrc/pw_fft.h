@@ -257,4 +257,16 @@ __device__ __forceinline__ void reg_fft(typename cx<T>::t (&a)[N]) {
   reg_fft_rec<T, N, 1, N, SG>(a);
 }
 
+// Ordering point for a register-held array: every element must be final
+// here (each is an operand of an empty volatile asm), and no memory access
+// moves across.  Without it the scheduler hoists the loads that follow an
+// in-register FFT (e.g. the potential it multiplies by) above the FFT and
+// keeps both live at once, which spills at two waves per SIMD.
+template <typename C, int N>
+__device__ __forceinline__ void pin_regs(C (&a)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; i++) asm volatile("" : "+v"(a[i].x), "+v"(a[i].y));
+  asm volatile("" ::: "memory");
+}
+
 }  // namespace pw

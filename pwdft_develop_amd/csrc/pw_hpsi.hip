@@ -162,23 +162,29 @@ k_slab_xy(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restric
 // ------------------------------------------------ kernel 2, register variant
 // The same slab transform for compile-time N1 = N2 with every 1D FFT held by
 // one thread in registers (reg_fft: constant twiddles).  A workgroup takes one z plane of SB
-// bands: V's plane is staged once in LDS (rows padded to N1 + 1 floats, so
-// the row-pair reads of neighbouring threads hit distinct banks), each band's
-// y-space lives in LDS, and the five phases (scatter, y IFFT per column, x
-// c2r / V / r2c per row pair, y FFT per column, gather) are separated by one
-// barrier each.  Same arithmetic, in the same order, as k_slab_xy.
+// bands: V's plane is staged once in LDS as (row 2q, row 2q + 1) pairs, one
+// 64-bit word per x, pair rows padded to N1 + 1 words so that neighbouring
+// threads hit distinct banks; each band's y-space lives in LDS, and the five
+// phases (scatter, y IFFT per column, x c2r / V / r2c per row pair, y FFT per
+// column, gather) are separated by one barrier each.  In the row-pair phase a
+// thread holds N1 complex values; its LDS reads are issued in chunks fenced
+// by scheduling barriers so that the compiler does not hoist all of them next
+// to the spectrum (that spills past the 256 VGPRs of two waves per SIMD).
+// Same arithmetic, in the same order, as k_slab_xy.
 template <typename T, int N1, int N2, int SB>
 __global__ void __launch_bounds__(256, 2)  // two waves per SIMD: <= 256 VGPRs
 k_slab_xy_reg(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restrict__ t2, int n3, int nstick,
               int kxmax, int nb, const int* __restrict__ pos, const int* __restrict__ mpos,
               const T* __restrict__ vtot) {
+  static_assert(sizeof(T) == 4 && N1 % 2 == 0 && N2 % 2 == 0, "FP32, even N1 and N2");
   typedef typename cx<T>::t C;
-  constexpr int H1 = N1 / 2, NPAIR = (N2 + 1) / 2, LV = N1 + 1;
+  constexpr int H1 = N1 / 2, NPAIR = N2 / 2, LV = N1 + 1, KC = 8;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   C* Y = reinterpret_cast<C*>(smem_raw);           // SB x nk x N2
   const int nk = kxmax + 1;
-  T* vs = reinterpret_cast<T*>(Y + (size_t)SB * nk * N2);  // N2 x LV
-  int* spos = reinterpret_cast<int*>(vs + (size_t)N2 * LV);  // nstick
+  C* vp = Y + (size_t)SB * nk * N2;                // NPAIR x LV pairs (v[2q][x], v[2q+1][x])
+  T* vs = reinterpret_cast<T*>(vp);
+  int* spos = reinterpret_cast<int*>(vp + (size_t)NPAIR * LV);  // nstick
   int* smpos = spos + nstick;                                // nstick
   const int z = blockIdx.x, b0 = blockIdx.y * SB;
   const int ns = min(SB, nb - b0);
@@ -197,7 +203,10 @@ k_slab_xy_reg(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __res
 #pragma unroll
     for (int u = 0; u < VB; u++) {
       const int e = t + 256 * u;
-      if (e < N1 * N2) vs[(e / N1) * LV + (e % N1)] = vb[u];
+      if (e < N1 * N2) {
+        const int y = e / N1, x = e - y * N1;
+        vs[2 * ((y >> 1) * LV + x) + (y & 1)] = vb[u];
+      }
     }
   }
   for (int e0 = 0; e0 < nstick; e0 += 4 * 256) {
@@ -250,42 +259,46 @@ k_slab_xy_reg(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __res
   }
   __syncthreads();
   // per row pair: Hermitian-extended spectra packed as X1 + i X2, c2r, x V/N,
-  // r2c, unpack kx <= kxmax; a thread reads and writes only its two rows
+  // r2c, unpack kx <= kxmax; a thread reads and writes only its two rows,
+  // which sit side by side in LDS (one 128-bit access per kx)
   for (int c = t; c < ns * NPAIR; c += blockDim.x) {
     const int s = c / NPAIR, q = c - s * NPAIR;
-    const int y1 = 2 * q, y2 = y1 + 1;
-    const bool has2 = y2 < N2;
+    const int y1 = 2 * q;
     C* Ys = Y + (size_t)s * nk * N2;
     C a[N1];
 #pragma unroll
-    for (int k = 0; k < N1; k++) {
-      const int kk = (k > H1) ? N1 - k : k;
-      C a1 = C{0, 0}, a2 = C{0, 0};
-      if (kk <= kxmax) {
-        a1 = Ys[kk * N2 + y1];
-        if (has2) a2 = Ys[kk * N2 + y2];
-        if (k > H1) { a1.y = -a1.y; a2.y = -a2.y; }
-        if (k == 0 || 2 * k == N1) { a1.y = 0; a2.y = 0; }
-      }
-      a[k] = C{a1.x - a2.y, a1.y + a2.x};
-    }
-    reg_fft<T, N1, 1>(a);
-    // keep the V reads next to their use (hoisting all of them above the FFT
-    // would hold 2 N1 more registers)
-    __builtin_amdgcn_sched_barrier(0);
-    const T* v1 = vs + y1 * LV;
-    const T* v2 = vs + (has2 ? y2 : y1) * LV;
+    for (int k0 = 0; k0 <= H1; k0 += KC) {
 #pragma unroll
-    for (int x = 0; x < N1; x++) a[x] = C{a[x].x * v1[x], has2 ? a[x].y * v2[x] : (T)0};
-    __builtin_amdgcn_sched_barrier(0);
+      for (int kk = k0; kk < k0 + KC; kk++) {
+        if (kk > H1) continue;
+        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (kk <= kxmax) p = *reinterpret_cast<const float4*>(Ys + kk * N2 + y1);
+        if (kk == 0 || kk == H1) { p.y = 0.f; p.w = 0.f; }
+        // row y1 spectrum (p.x, p.y), row y2 (p.z, p.w); the kx > H1 half is
+        // their conjugates
+        a[kk] = C{p.x - p.w, p.y + p.z};
+        if (kk != 0 && kk != H1) a[N1 - kk] = C{p.x + p.w, p.z - p.y};
+      }
+    }
+    pin_regs(a);
+    reg_fft<T, N1, 1>(a);
+    pin_regs(a);
+    const C* vq = vp + q * LV;
+#pragma unroll
+    for (int x = 0; x < N1; x++) {
+      const C v = vq[x];
+      a[x] = C{a[x].x * v.x, a[x].y * v.y};
+    }
+    pin_regs(a);
     reg_fft<T, N1, -1>(a);
+    pin_regs(a);
 #pragma unroll
     for (int k = 0; k <= H1; k++) {
       if (k <= kxmax) {
         const C w = a[k];
         const C wc = a[k == 0 ? 0 : N1 - k];
-        Ys[k * N2 + y1] = C{(T)0.5 * (w.x + wc.x), (T)0.5 * (w.y - wc.y)};
-        if (has2) Ys[k * N2 + y2] = C{(T)0.5 * (w.y + wc.y), (T)0.5 * (wc.x - w.x)};
+        *reinterpret_cast<float4*>(Ys + k * N2 + y1) =
+            make_float4(0.5f * (w.x + wc.x), 0.5f * (w.y - wc.y), 0.5f * (w.y + wc.y), 0.5f * (wc.x - w.x));
       }
     }
   }
@@ -769,7 +782,7 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
   if (kreg) {
     // one z plane of kSlabSB bands per workgroup, V plane + y-space in LDS
     const size_t ldsr = sizeof(float2) * ((size_t)kSlabSB * (c->sp.kxmax + 1) * c->n2) +
-                        sizeof(float) * (size_t)c->n2 * (c->n1 + 1) + sizeof(int) * 2 * (size_t)c->sp.nstick;
+                        sizeof(float2) * (size_t)(c->n2 / 2) * (c->n1 + 1) + sizeof(int) * 2 * (size_t)c->sp.nstick;
     PW_TRY(ensure_lds((const void*)kreg, ldsr));
     hipLaunchKernelGGL(kreg, dim3(c->n3, (nb + kSlabSB - 1) / kSlabSB), dim3(256), ldsr, c->stream,
                        (const float2*)t1, (float2*)t2, c->n3, c->sp.nstick, c->sp.kxmax, nb, c->d_pos, c->d_mpos,
