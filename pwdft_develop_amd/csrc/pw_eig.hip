@@ -23,9 +23,11 @@
 //                     chains the dense Rayleigh-Ritz spectra into long serial
 //                     runs); scratch interleaved by thread so that lanes
 //                     stepping through the same row coalesce
-//   k_ormtr           Z <- Q Z_T in compact-WY chunks (dlarft + dlarfb):
-//                     every workgroup applies all chunks to its own block of
-//                     8 columns held in LDS
+//   k_larft           the triangular factor T of every compact-WY chunk of
+//                     reflectors (dlarft), one workgroup per chunk
+//   k_ormtr           Z <- Q Z_T chunk by chunk (dlarfb): every workgroup
+//                     applies all chunks to its own block of 8 columns held
+//                     in LDS
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -518,21 +520,97 @@ k_stein(const double* __restrict__ dg, const double* __restrict__ eg, const doub
   }
 }
 
-// Z <- H_0 H_1 ... H_{n-3} Z for the kOrmCols columns of this workgroup, in
-// compact-WY chunks of nbw reflectors (LAPACK dlarft 'F','C' + dlarfb): for
-// the chunk H_kb ... H_{ke-1} = I - V T V^T,  Z <- Z - V (T (V^T Z)), chunks
-// from the last to the first.  V (n x nbw), T, V^T V and the 8-column block
-// of Z live in LDS; no per-reflector reductions.  blockDim = 256.
+// V of the chunk of reflectors kb .. kb + b - 1 into LDS, row-major with the
+// padded stride nbw + 1 (row r, reflector j; 1 on row kb + 1 + j, zeros
+// above).  Consecutive threads walk down one reflector column (coalesced
+// loads), 8 loads in flight per thread; the padded stride keeps the
+// transposed LDS stores conflict-free.  blockDim = 256.
+__device__ __forceinline__ void stage_v_chunk(const double* __restrict__ A, int lda, int n, int nbw, int kb, int b,
+                                              double* Vs) {
+  const int lvs = nbw + 1, t = threadIdx.x;
+  const int nr = n - kb, tot = nr * nbw;
+  for (int e0 = 0; e0 < tot; e0 += 8 * 256) {
+    double b8[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int e = e0 + u * 256 + t;
+      const int j = e / nr, r = kb + e % nr, k = kb + j;
+      b8[u] = (e < tot && j < b) ? ((r == k + 1) ? 1.0 : (r > k + 1 ? A[(size_t)lda * k + r] : 0.0)) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int e = e0 + u * 256 + t;
+      if (e < tot) Vs[(size_t)(kb + e % nr) * lvs + e / nr] = b8[u];
+    }
+  }
+}
+
+// Triangular factor T of each compact-WY chunk (LAPACK dlarft 'F', 'C'):
+// H_kb ... H_{kb+b-1} = I - V T V^T with T_jj = tau_j and
+// T[0:j, j] = -tau_j T[0:j, 0:j] (V^T V)[0:j, j].  One workgroup per chunk,
+// all chunks at once; T (nbw x nbw, row-major) to Tg + chunk * nbw^2.
+// blockDim = 256.
 __global__ void __launch_bounds__(256)
-k_ormtr(const double* __restrict__ A, int lda, const double* __restrict__ tau, int n, int nbw,
+k_larft(const double* __restrict__ A, int lda, const double* __restrict__ tau, int n, int nbw,
+        double* __restrict__ Tg) {
+  extern __shared__ double sm[];
+  const int lvs = nbw + 1;
+  double* Vs = sm;                                 // n x nbw (rows kb.. used)
+  double* Gs = Vs + (size_t)n * lvs;               // nbw x nbw: V^T V
+  double* Ts = Gs + (size_t)nbw * nbw;
+  const int nref = n - 2;
+  const int kb = blockIdx.x * nbw, b = min(nref, kb + nbw) - kb;
+  const int t = threadIdx.x;
+  stage_v_chunk(A, lda, n, nbw, kb, b, Vs);
+  __syncthreads();
+  // G = V^T V, strictly upper part (rows of V below kb + 1 + j are zero in
+  // column j): 4 independent partial sums per entry
+  for (int e = t; e < nbw * nbw; e += blockDim.x) {
+    const int i = e / nbw, j = e % nbw;
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+    if (i < j && j < b) {
+      int r = kb + 1 + j;
+      for (; r + 3 < n; r += 4) {
+        g0 += Vs[(size_t)r * lvs + i] * Vs[(size_t)r * lvs + j];
+        g1 += Vs[(size_t)(r + 1) * lvs + i] * Vs[(size_t)(r + 1) * lvs + j];
+        g2 += Vs[(size_t)(r + 2) * lvs + i] * Vs[(size_t)(r + 2) * lvs + j];
+        g3 += Vs[(size_t)(r + 3) * lvs + i] * Vs[(size_t)(r + 3) * lvs + j];
+      }
+      for (; r < n; r++) g0 += Vs[(size_t)r * lvs + i] * Vs[(size_t)r * lvs + j];
+    }
+    Gs[e] = (g0 + g1) + (g2 + g3);
+  }
+  __syncthreads();
+  // Thread i owns row i of T and reads only that row, so the recurrence
+  // needs no synchronisation.
+  if (t < nbw) {
+    for (int j = 0; j < nbw; j++) {
+      double x = 0.0;
+      if (t < j && j < b)
+        for (int q = t; q < j; q++) x += Ts[t * nbw + q] * Gs[q * nbw + j];
+      Ts[t * nbw + j] = (j >= b || t > j) ? 0.0 : (t == j ? tau[kb + j] : -tau[kb + j] * x);
+    }
+  }
+  __syncthreads();
+  double* Tc = Tg + (size_t)blockIdx.x * nbw * nbw;
+  for (int e = t; e < nbw * nbw; e += blockDim.x) Tc[e] = Ts[e];
+}
+
+// Z <- H_0 H_1 ... H_{n-3} Z for the kOrmCols columns of this workgroup, in
+// compact-WY chunks of nbw reflectors (LAPACK dlarfb with the T of k_larft):
+// for the chunk H_kb ... H_{ke-1} = I - V T V^T,  Z <- Z - V (T (V^T Z)),
+// chunks from the last to the first.  V, T and the 8-column block of Z live
+// in LDS; no per-reflector reductions.  blockDim = 256.
+__global__ void __launch_bounds__(256)
+k_ormtr(const double* __restrict__ A, int lda, const double* __restrict__ Tg, int n, int nbw,
         double* __restrict__ Z, int ldz) {
   extern __shared__ double sm[];
   double* zs = sm;                                 // n x kOrmCols, row-major
   const int lvs = nbw + 1;                         // padded row stride of Vs
   double* Vs = zs + (size_t)n * kOrmCols;          // n x nbw, row-major (row r, reflector j)
-  double* Gs = Vs + (size_t)n * lvs;               // nbw x nbw: V^T V, then T
-  double* Ts = Gs + (size_t)nbw * nbw;
-  double* W1 = Ts + (size_t)nbw * nbw;             // nbw x kOrmCols
+  double* Ts = Vs + (size_t)n * lvs;               // nbw x nbw
+  double* W1 = Ts + (size_t)nbw * nbw;             // nbw x kOrmCols: V^T Z
+  double* W2 = W1 + (size_t)nbw * kOrmCols;        // nbw x kOrmCols: T W1
   const int c0 = blockIdx.x * kOrmCols;
   const int nc = min(kOrmCols, n - c0);
   const int t = threadIdx.x;
@@ -552,69 +630,52 @@ k_ormtr(const double* __restrict__ A, int lda, const double* __restrict__ tau, i
   const int nchunk = (nref + nbw - 1) / nbw;
   for (int ch = nchunk - 1; ch >= 0; ch--) {
     const int kb = ch * nbw, ke = min(nref, kb + nbw), b = ke - kb;
-    // V rows kb+1 .. n-1 (row kb+1+j holds 1 for reflector j, zeros above)
     __syncthreads();
-    // consecutive threads walk down one reflector column (coalesced loads),
-    // 8 loads in flight per thread; the padded stride keeps the transposed
-    // LDS stores conflict-free
-    const int nr = n - kb, tot = nr * nbw;
-    for (int e0 = 0; e0 < tot; e0 += 8 * 256) {
-      double b8[8];
+    {
+      const double* Tc = Tg + (size_t)ch * nbw * nbw;
+      double tb[4];
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const int e = e0 + u * 256 + t;
-        const int j = e / nr, r = kb + e % nr, k = kb + j;
-        b8[u] = (e < tot && j < b) ? ((r == k + 1) ? 1.0 : (r > k + 1 ? A[(size_t)lda * k + r] : 0.0)) : 0.0;
+      for (int u = 0; u < 4; u++) {
+        const int e = t + 256 * u;
+        tb[u] = e < nbw * nbw ? Tc[e] : 0.0;
       }
+      stage_v_chunk(A, lda, n, nbw, kb, b, Vs);
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const int e = e0 + u * 256 + t;
-        if (e < tot) Vs[(size_t)(kb + e % nr) * lvs + e / nr] = b8[u];
+      for (int u = 0; u < 4; u++) {
+        const int e = t + 256 * u;
+        if (e < nbw * nbw) Ts[e] = tb[u];
       }
     }
     __syncthreads();
-    // G = V^T V (upper part used) and W1 = V^T Z
-    for (int e = t; e < nbw * nbw; e += blockDim.x) {
-      const int i = e / nbw, j = e % nbw;
-      double g = 0.0;
-      if (i < j && j < b)
-        for (int r = kb + 1 + j; r < n; r++) g += Vs[(size_t)r * lvs + i] * Vs[(size_t)r * lvs + j];
-      Gs[e] = g;
-    }
+    // W1 = V^T Z (rows of V below kb + 1 + j are zero in column j)
     for (int e = t; e < nbw * kOrmCols; e += blockDim.x) {
       const int j = e / kOrmCols, c = e % kOrmCols;
-      double w = 0.0;
-      if (j < b)
-        for (int r = kb + 1 + j; r < n; r++) w += Vs[(size_t)r * lvs + j] * zs[r * kOrmCols + c];
-      W1[e] = w;
-    }
-    __syncthreads();
-    // T (upper triangular): T_jj = tau_j, T[0:j, j] = -tau_j T[0:j, 0:j] G[0:j, j].
-    // Thread i owns row i of T and reads only that row, so the recurrence
-    // needs no synchronisation.
-    if (t < nbw) {
-      for (int j = 0; j < nbw; j++) {
-        double x = 0.0;
-        if (t < j && j < b)
-          for (int q = t; q < j; q++) x += Ts[t * nbw + q] * Gs[q * nbw + j];
-        Ts[t * nbw + j] = (j >= b || t > j) ? 0.0 : (t == j ? tau[kb + j] : -tau[kb + j] * x);
+      double w0 = 0.0, w1 = 0.0;
+      if (j < b) {
+        int r = kb + 1 + j;
+        for (; r + 1 < n; r += 2) {
+          w0 += Vs[(size_t)r * lvs + j] * zs[r * kOrmCols + c];
+          w1 += Vs[(size_t)(r + 1) * lvs + j] * zs[(r + 1) * kOrmCols + c];
+        }
+        if (r < n) w0 += Vs[(size_t)r * lvs + j] * zs[r * kOrmCols + c];
       }
+      W1[e] = w0 + w1;
     }
     __syncthreads();
-    // W2 = T W1 (into Gs, no longer needed)
+    // W2 = T W1
     for (int e = t; e < nbw * kOrmCols; e += blockDim.x) {
       const int j = e / kOrmCols, c = e % kOrmCols;
       double w = 0.0;
       if (j < b)
         for (int i = j; i < b; i++) w += Ts[j * nbw + i] * W1[i * kOrmCols + c];
-      Gs[e] = w;
+      W2[e] = w;
     }
     __syncthreads();
     // Z -= V W2 on rows kb+1 .. n-1
     for (int e = t; e < (n - kb - 1) * kOrmCols; e += blockDim.x) {
       const int r = kb + 1 + e / kOrmCols, c = e % kOrmCols;
       double u = 0.0;
-      for (int j = 0; j < b; j++) u += Vs[(size_t)r * lvs + j] * Gs[j * kOrmCols + c];
+      for (int j = 0; j < b; j++) u += Vs[(size_t)r * lvs + j] * W2[j * kOrmCols + c];
       zs[r * kOrmCols + c] -= u;
     }
   }
@@ -638,7 +699,10 @@ int eig_sym(pw_ctx* c, int n, double* A, int lda, double* w) {
     return PW_OK;
   }
   const size_t nn = (size_t)n * n;
-  const size_t need = sizeof(double) * (3 * (size_t)n + nn + 7 * nn);
+  // compact-WY chunks of the back-transform: nbw reflectors each
+  const int nbw = std::max(1, std::min(32, 8192 / n));
+  const int nchunk = (n - 2 + nbw - 1) / nbw;
+  const size_t need = sizeof(double) * (3 * (size_t)n + nn + 7 * nn + (size_t)nchunk * nbw * nbw);
   if (need > c->eig_ws_bytes) {
     if (c->d_eig_ws) (void)hipFree(c->d_eig_ws);
     c->d_eig_ws = nullptr;
@@ -651,6 +715,7 @@ int eig_sym(pw_ctx* c, int n, double* A, int lda, double* w) {
   double* tau = e + n;
   double* Zt = tau + n;
   double* S = Zt + nn;
+  double* Tw = S + 7 * nn;
   const int g1 = (int)std::min<size_t>((nn + 255) / 256, 4096);
   hipLaunchKernelGGL(k_sym_from_upper, dim3(g1), dim3(256), 0, s, A, n, lda);
   // panel width: the V and W panels (2 nbk n doubles) fit LDS next to v and
@@ -659,18 +724,23 @@ int eig_sym(pw_ctx* c, int n, double* A, int lda, double* w) {
   const size_t lds_trd = sizeof(double) * ((size_t)2 * nbk * n + n + 1024 + 2 * (size_t)nbk);
   PW_TRY(ensure_lds((const void*)k_sytrd, lds_trd + 1024));  // + its static LDS
   hipLaunchKernelGGL(k_sytrd, dim3(1), dim3(kTrdThreads), lds_trd, s, A, n, lda, nbk, d, e, tau);
+  if (nchunk > 0) {
+    const size_t lds_t = sizeof(double) * ((size_t)n * (nbw + 1) + 2 * (size_t)nbw * nbw);
+    PW_TRY(ensure_lds((const void*)k_larft, lds_t));
+    hipLaunchKernelGGL(k_larft, dim3(nchunk), dim3(256), lds_t, s, (const double*)A, lda, (const double*)tau, n,
+                       nbw, Tw);
+  }
   hipLaunchKernelGGL(k_stebz, dim3((n + 3) / 4), dim3(256), sizeof(double) * 2 * n, s, (const double*)d,
                      (const double*)e, n, w);
   hipLaunchKernelGGL(k_stein, dim3((n + 63) / 64), dim3(64), sizeof(double) * 2 * n, s, (const double*)d,
                      (const double*)e,
                      (const double*)w, n, Zt, S);
-  // compact-WY chunk: V (n x nbw) next to the Z block in LDS
-  const int nbw = std::max(1, std::min(32, 8192 / n));
-  const size_t lds_orm = sizeof(double) * ((size_t)n * kOrmCols + (size_t)n * (nbw + 1) + 2 * (size_t)nbw * nbw +
-                                           (size_t)nbw * kOrmCols);
-  PW_TRY(ensure_lds((const void*)k_ormtr, lds_orm + 1024));
+  // compact-WY chunk: V (n x nbw) and T next to the Z block in LDS
+  const size_t lds_orm = sizeof(double) * ((size_t)n * kOrmCols + (size_t)n * (nbw + 1) + (size_t)nbw * nbw +
+                                           2 * (size_t)nbw * kOrmCols);
+  PW_TRY(ensure_lds((const void*)k_ormtr, lds_orm));
   hipLaunchKernelGGL(k_ormtr, dim3((n + kOrmCols - 1) / kOrmCols), dim3(256), lds_orm, s, (const double*)A, lda,
-                     (const double*)tau, n, nbw, Zt, n);
+                     (const double*)Tw, n, nbw, Zt, n);
   PW_HIP(hipGetLastError());
   PW_HIP(hipMemcpy2DAsync(A, sizeof(double) * lda, Zt, sizeof(double) * n, sizeof(double) * n, n,
                           hipMemcpyDeviceToDevice, s));
