@@ -18,6 +18,13 @@ namespace pw {
 // data goes through LDS.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
+// Data exchanged through LDS between lanes of one wavefront: this
+// wavefront's LDS operations issued so far have completed, and the compiler
+// moves no memory access across (without it, a lane's load of a word another
+// lane stored may be hoisted above the store, since the compiler models
+// lanes as independent threads).
+__device__ __forceinline__ void wave_lds_sync() { asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory"); }
+
 // Sum of a double over the 64 lanes of a wavefront, returned to every lane,
 // with DPP lane moves (no LDS round trips): row_shr 1, 2, 4, 8 leave each
 // 16-lane row's sum in its lane 15, row_bcast 15 and 31 carry the row sums
@@ -123,6 +130,10 @@ int gemm_nn(pw_ctx* c, int m, int n, int k, T alpha, const T* A, int lda, const 
 // ascending); sym_eig adds the sign convention of the oracle and takes
 // rocSOLVER beyond eig_sym's n <= 1024.
 int eig_sym(pw_ctx* c, int n, double* A, int lda, double* w);
+// S = U^T U (U into S's upper triangle) and W = U^{-1}; device info word 0
+// or the failing column; n <= chol_inv_max_n()
+int chol_upper_inv(pw_ctx* c, int n, double* S, int ld, double* W, int ldw, int* info);
+int chol_inv_max_n();
 int sym_eig(pw_ctx* c, int n, double* A, int lda, double* w, double* work, int* info);
 // Stick transforms of the wavefunction sphere (pw_hpsi.hip).
 template <typename T>

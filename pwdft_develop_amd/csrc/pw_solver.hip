@@ -456,12 +456,27 @@ extern "C" int pw_set_graphs(pw_ctx* c, int enable) {
 }
 
 // Cholesky QR of X (FP64, in place): S = X^T X, S = U^T U, X <- X U^{-1}.
+// Up to 256 bands the factor and its inverse come from one single-workgroup
+// kernel and X U^{-1} is an MFMA GEMM; beyond, rocSOLVER potrf + rocBLAS trsm.
+// When the overlap matrix is not positive definite the status is
+// PW_ERR_LINALG and X is unspecified.
 extern "C" int pw_orthogonalize(pw_ctx* c, int nb, void* X) {
   if (!c || !X || nb < 1 || nb > c->max_bands) return fail(PW_ERR_ARG, "pw_orthogonalize: bad argument");
   PW_TRY(ensure_ws(c));
   const int m = 2 * c->npw;
   const double one = 1.0, zero = 0.0;
   PW_TRY(gemm_tn<double>(c, m, nb, nb, (const double*)X, m, (const double*)X, m, c->d_S, nb, nullptr, zero));
+  if (nb <= chol_inv_max_n()) {
+    PW_TRY(chol_upper_inv(c, nb, c->d_S, nb, c->d_S2, nb, c->d_info));
+    double* T = (double*)c->d_ws[5];
+    PW_TRY(gemm_nn<double>(c, m, nb, nb, one, (const double*)X, m, c->d_S2, nb, zero, T, m));
+    PW_HIP(hipMemcpyAsync(X, T, sizeof(double) * m * (size_t)nb, hipMemcpyDeviceToDevice, c->stream));
+    int info = 0;
+    PW_HIP(hipMemcpyAsync(&info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    PW_HIP(hipStreamSynchronize(c->stream));
+    if (info != 0) return fail(PW_ERR_LINALG, "Potrf: overlap matrix is not positive definite (info = %d)", info);
+    return PW_OK;
+  }
   PW_BLAS(rocsolver_dpotrf(c->blas, rocblas_fill_upper, nb, c->d_S, nb, c->d_info));
   int info = 0;
   PW_HIP(hipMemcpyAsync(&info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
