@@ -130,10 +130,13 @@ int gemm_nn(pw_ctx* c, int m, int n, int k, T alpha, const T* A, int lda, const 
 // ascending); sym_eig adds the sign convention of the oracle and takes
 // rocSOLVER beyond eig_sym's n <= 1024.
 int eig_sym(pw_ctx* c, int n, double* A, int lda, double* w);
-// S = U^T U (U into S's upper triangle) and W = U^{-1}; device info word 0
-// or the failing column; n <= chol_inv_max_n()
-int chol_upper_inv(pw_ctx* c, int n, double* S, int ld, double* W, int ldw, int* info);
+// S = U^T U (U into S's upper triangle) and the inverses of U's 32 x 32
+// diagonal blocks into Dinv (chol_dinv_doubles(n)); device info word 0 or
+// the failing column; then X (m x n) <- X U^{-1}.  n <= chol_inv_max_n().
+int chol_upper(pw_ctx* c, int n, double* S, int ld, double* Dinv, int* info);
+int trsm_right_upper(pw_ctx* c, int m, int n, double* X, int ldx, const double* U, int ldu, const double* Dinv);
 int chol_inv_max_n();
+size_t chol_dinv_doubles(int n);
 int sym_eig(pw_ctx* c, int n, double* A, int lda, double* w, double* work, int* info);
 // Stick transforms of the wavefunction sphere (pw_hpsi.hip).
 template <typename T>
@@ -194,6 +197,7 @@ struct pw_ctx {
   void* d_eig_ws = nullptr;  // dense eigensolver scratch, grown on demand
   size_t eig_ws_bytes = 0;
   int* d_info = nullptr;
+  double* d_chol = nullptr;  // inverses of the Cholesky factor's diagonal blocks
   int ws_nb = 0;
   // Chebyshev recurrence coefficients (3 per step, degree <= 64) and the
   // captured filter graph, replayed on a private stream

@@ -167,6 +167,7 @@ static int ensure_ws(pw_ctx* c) {
   PW_HIP(hipMalloc((void**)&c->d_E, sizeof(double) * c->max_bands));
   PW_HIP(hipMalloc((void**)&c->d_red, sizeof(double) * (std::max(c->max_bands, 128) + 8)));
   PW_HIP(hipMalloc((void**)&c->d_info, sizeof(int) * 4));
+  PW_HIP(hipMalloc((void**)&c->d_chol, sizeof(double) * chol_dinv_doubles(chol_inv_max_n())));
   PW_HIP(hipMalloc((void**)&c->d_lz, sizeof(double) * 3 * 2 * (size_t)c->npw));
   PW_HIP(hipMalloc((void**)&c->d_coef, sizeof(double) * 192));
   PW_HIP(hipMalloc((void**)&c->d_coef_f, sizeof(float) * 192));
@@ -182,10 +183,10 @@ extern "C" int pw_free_workspace(pw_ctx* c) {
   if (!c) return fail(PW_ERR_ARG, "null context");
   for (int i = 0; i < 6; i++)
     if (c->d_ws[i]) { (void)hipFree(c->d_ws[i]); c->d_ws[i] = nullptr; }
-  void* p[] = {c->d_S, c->d_S2, c->d_D, c->d_E, c->d_red, c->d_info, c->d_lz, c->d_coef, c->d_coef_f};
+  void* p[] = {c->d_S, c->d_S2, c->d_D, c->d_E, c->d_red, c->d_info, c->d_lz, c->d_coef, c->d_coef_f, c->d_chol};
   for (void* q : p)
     if (q) (void)hipFree(q);
-  c->d_S = c->d_S2 = c->d_D = c->d_E = c->d_red = c->d_lz = nullptr;
+  c->d_S = c->d_S2 = c->d_D = c->d_E = c->d_red = c->d_lz = c->d_chol = nullptr;
   c->d_coef = nullptr;
   c->d_coef_f = nullptr;
   c->d_info = nullptr;
@@ -456,8 +457,9 @@ extern "C" int pw_set_graphs(pw_ctx* c, int enable) {
 }
 
 // Cholesky QR of X (FP64, in place): S = X^T X, S = U^T U, X <- X U^{-1}.
-// Up to 256 bands the factor and its inverse come from one single-workgroup
-// kernel and X U^{-1} is an MFMA GEMM; beyond, rocSOLVER potrf + rocBLAS trsm.
+// Up to 256 bands the factor comes from one single-workgroup kernel and
+// X U^{-1} from a row-parallel block substitution; beyond, rocSOLVER potrf +
+// rocBLAS trsm.
 // When the overlap matrix is not positive definite the status is
 // PW_ERR_LINALG and X is unspecified.
 extern "C" int pw_orthogonalize(pw_ctx* c, int nb, void* X) {
@@ -467,10 +469,8 @@ extern "C" int pw_orthogonalize(pw_ctx* c, int nb, void* X) {
   const double one = 1.0, zero = 0.0;
   PW_TRY(gemm_tn<double>(c, m, nb, nb, (const double*)X, m, (const double*)X, m, c->d_S, nb, nullptr, zero));
   if (nb <= chol_inv_max_n()) {
-    PW_TRY(chol_upper_inv(c, nb, c->d_S, nb, c->d_S2, nb, c->d_info));
-    double* T = (double*)c->d_ws[5];
-    PW_TRY(gemm_nn<double>(c, m, nb, nb, one, (const double*)X, m, c->d_S2, nb, zero, T, m));
-    PW_HIP(hipMemcpyAsync(X, T, sizeof(double) * m * (size_t)nb, hipMemcpyDeviceToDevice, c->stream));
+    PW_TRY(chol_upper(c, nb, c->d_S, nb, c->d_chol, c->d_info));
+    PW_TRY(trsm_right_upper(c, m, nb, (double*)X, m, c->d_S, nb, c->d_chol));
     int info = 0;
     PW_HIP(hipMemcpyAsync(&info, c->d_info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     PW_HIP(hipStreamSynchronize(c->stream));
