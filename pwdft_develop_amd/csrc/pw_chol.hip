@@ -21,10 +21,9 @@
 // The two branches meet at the same three barriers per block column.  A
 // non-positive pivot stops with info = its 1-based column (LAPACK's info).
 #include "pw_internal.h"
+#include "pw_mfma.h"
 
 namespace pw {
-
-typedef double d4v __attribute__((ext_vector_type(4)));
 
 constexpr int kChB = 32;                                   // block size
 constexpr int kChMaxNB = 8;                                // n <= 256
@@ -34,16 +33,6 @@ constexpr int kChSlots = (kChMaxNB * (kChMaxNB - 1) / 2 + kChOwners - 1) / kChOw
 constexpr int kChLdp = kChB * kChMaxNB + 1;                // row stride of the LDS panel
 constexpr int kChLdd = kChB + 1;                           // row stride of a 32 x 32 LDS block
 
-__device__ __forceinline__ d4v ch_mfma(double a, double b, d4v c) {
-  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-
-__device__ __forceinline__ double lane_bcast(double v, int src) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
-  return __hiloint2double(hi, lo);
-}
-
 // block b of the strictly upper block triangle, numbered column by column
 // (b = J (J - 1) / 2 + I, I < J)
 __device__ __forceinline__ void ch_block(int b, int& I, int& J) {
@@ -51,22 +40,6 @@ __device__ __forceinline__ void ch_block(int b, int& I, int& J) {
   while ((j + 1) * j / 2 <= b) j++;
   J = j;
   I = b - j * (j - 1) / 2;
-}
-
-// Accumulator layout of v_mfma_f64_16x16x4f64: tile q = ti + 2 tj of a
-// 32 x 32 block, lane l, register i holds element (16 ti + 4 i + l / 16,
-// 16 tj + l % 16).  Row-major LDS copies of a block:
-__device__ __forceinline__ void ch_to_lds(const d4v (&a)[4], double* M, int ldm, int lane) {
-#pragma unroll
-  for (int q = 0; q < 4; q++)
-#pragma unroll
-    for (int i = 0; i < 4; i++) M[(16 * (q & 1) + 4 * i + (lane >> 4)) * ldm + 16 * (q >> 1) + (lane & 15)] = a[q][i];
-}
-__device__ __forceinline__ void ch_from_lds(d4v (&a)[4], const double* M, int ldm, int lane) {
-#pragma unroll
-  for (int q = 0; q < 4; q++)
-#pragma unroll
-    for (int i = 0; i < 4; i++) a[q][i] = M[(16 * (q & 1) + 4 * i + (lane >> 4)) * ldm + 16 * (q >> 1) + (lane & 15)];
 }
 
 // S (n x n, upper triangle read, ld) -> U in its upper triangle; Dinv
@@ -207,7 +180,7 @@ k_potrf(double* __restrict__ S, int n, int ld, double* __restrict__ Dinv, int* _
 #pragma unroll
       for (int u = 0; u < 16; u++) stg[(lane & 31) * kChLdd + 2 * u + (lane >> 5)] = acc[s][u >> 2][u & 3];
       wave_lds_sync();
-      ch_from_lds(acc[s], stg, kChLdd, lane);
+      blk_from_lds(acc[s], stg, kChLdd, lane);
       wave_lds_sync();
     }
     for (int P = 0; P < NB; P++) {
@@ -223,7 +196,7 @@ k_potrf(double* __restrict__ S, int n, int ld, double* __restrict__ Dinv, int* _
         ch_block(b, I, J);
         if (I != P) continue;
         double* PJ = Pn + kChB * J;
-        ch_to_lds(acc[s], PJ, kChLdp, lane);
+        blk_to_lds(acc[s], PJ, kChLdp, lane);
         wave_lds_sync();
 #pragma unroll
         for (int q = 0; q < 4; q++) acc[s][q] = d4v{0.0, 0.0, 0.0, 0.0};
@@ -236,10 +209,10 @@ k_potrf(double* __restrict__ S, int n, int ld, double* __restrict__ Dinv, int* _
             bb[u] = PJ[(k0 + lk) * kChLdp + 16 * u + lr];
           }
 #pragma unroll
-          for (int q = 0; q < 4; q++) acc[s][q] = ch_mfma(a[q & 1], bb[q >> 1], acc[s][q]);
+          for (int q = 0; q < 4; q++) acc[s][q] = mfma_f64(a[q & 1], bb[q >> 1], acc[s][q]);
         }
         wave_lds_sync();  // every lane's operand reads of A_PJ are done
-        ch_to_lds(acc[s], PJ, kChLdp, lane);
+        blk_to_lds(acc[s], PJ, kChLdp, lane);
       }
       __syncthreads();  // block row P of U in the panel
       // trailing blocks: A_IJ -= U_PI^T U_PJ
@@ -259,7 +232,7 @@ k_potrf(double* __restrict__ S, int n, int ld, double* __restrict__ Dinv, int* _
             bb[u] = Pn[(k0 + lk) * kChLdp + kChB * J + 16 * u + lr];
           }
 #pragma unroll
-          for (int q = 0; q < 4; q++) acc[s][q] = ch_mfma(a[q & 1], bb[q >> 1], acc[s][q]);
+          for (int q = 0; q < 4; q++) acc[s][q] = mfma_f64(a[q & 1], bb[q >> 1], acc[s][q]);
         }
       }
       // diagonal blocks I > P owned by this wave, through LDS
@@ -267,7 +240,7 @@ k_potrf(double* __restrict__ S, int n, int ld, double* __restrict__ Dinv, int* _
         if (I % kChOwners != wo) continue;
         double* DI = Dg + (size_t)I * kChB * kChLdd;
         d4v dacc[4];
-        ch_from_lds(dacc, DI, kChLdd, lane);
+        blk_from_lds(dacc, DI, kChLdd, lane);
 #pragma unroll
         for (int k0 = 0; k0 < kChB; k0 += 4) {
           double a[2], bb[2];
@@ -277,9 +250,9 @@ k_potrf(double* __restrict__ S, int n, int ld, double* __restrict__ Dinv, int* _
             bb[u] = Pn[(k0 + lk) * kChLdp + kChB * I + 16 * u + lr];
           }
 #pragma unroll
-          for (int q = 0; q < 4; q++) dacc[q] = ch_mfma(a[q & 1], bb[q >> 1], dacc[q]);
+          for (int q = 0; q < 4; q++) dacc[q] = mfma_f64(a[q & 1], bb[q >> 1], dacc[q]);
         }
-        ch_to_lds(dacc, DI, kChLdd, lane);
+        blk_to_lds(dacc, DI, kChLdd, lane);
       }
     }
     // off-diagonal blocks of U into S
@@ -366,7 +339,7 @@ k_trsm_right_upper(double* __restrict__ X, int m, int ldx, int n, const double* 
       for (int kq = 0; kq < 8; kq++) {
 #pragma unroll
         for (int ti = 0; ti < 2; ti++)
-          c[ti] = ch_mfma(-Us[(kChB * K + 4 * kq + lk) * kChLdd + 16 * ti + lr], yt[K][kq >> 2][kq & 3], c[ti]);
+          c[ti] = mfma_f64(-Us[(kChB * K + 4 * kq + lk) * kChLdd + 16 * ti + lr], yt[K][kq >> 2][kq & 3], c[ti]);
       }
     // Y_J^T = Dinv_J^T C
     d4v o[2] = {d4v{0.0, 0.0, 0.0, 0.0}, d4v{0.0, 0.0, 0.0, 0.0}};
@@ -374,7 +347,7 @@ k_trsm_right_upper(double* __restrict__ X, int m, int ldx, int n, const double* 
     for (int kq = 0; kq < 8; kq++)
 #pragma unroll
       for (int ti = 0; ti < 2; ti++)
-        o[ti] = ch_mfma(Ds[(4 * kq + lk) * kChLdd + 16 * ti + lr], c[kq >> 2][kq & 3], o[ti]);
+        o[ti] = mfma_f64(Ds[(4 * kq + lk) * kChLdd + 16 * ti + lr], c[kq >> 2][kq & 3], o[ti]);
     yt[J][0] = o[0];
     yt[J][1] = o[1];
   }

@@ -34,6 +34,7 @@
 #include <cstdint>
 
 #include "pw_internal.h"
+#include "pw_mfma.h"
 
 namespace pw {
 
@@ -243,6 +244,279 @@ k_sytrd(double* __restrict__ A, int n, int lda, int nbk, double* __restrict__ d,
     d[n - 2] = A[(size_t)lda * (n - 2) + (n - 2)];
     d[n - 1] = A[(size_t)lda * (n - 1) + (n - 1)];
     e[n - 2] = A[(size_t)lda * (n - 2) + (n - 1)];
+    tau[n - 2] = 0.0;
+  }
+}
+
+// ------------------------------------------------------------------------
+// n <= 256: the same reduction unblocked (LAPACK dsytd2 'L' order) with the
+// matrix on chip.  The strictly lower 32 x 32 blocks (at most 28) sit in the
+// MFMA accumulator layout of eight wavefronts (block b, numbered row by row,
+// belongs to wave b % 8); diagonal block I lives in LDS (full, kept
+// symmetric) and is handled by wave I.  Per column k:
+//   (a) column k goes to LDS from its owners,
+//   (b) wave 0 forms the reflector v (zero outside k+1 .. n-1) and tau,
+//   (c) every wave sums A v over its blocks that meet the trailing matrix:
+//       row sums across the 16 lanes of a DPP row, column sums (the
+//       transposed half of an off-diagonal block) across lanes l, l+16,
+//       l+32, l+48; per-wave partials in LDS, added in a fixed order,
+//   (d) wave 0: w = p - (tau/2)(p.v) v with p = tau A v,
+//   (e) A -= v w^T + w v^T: one MFMA per 16 x 16 register tile (K = 2 of
+//       the 4), read-modify-write on the LDS diagonal blocks.
+// No global memory traffic after the load except the reflector and d, e,
+// tau.  Same arithmetic as k_sytrd up to the summation order.
+constexpr int kTr2MaxNB = 8;
+constexpr int kTr2Waves = 8;  // = kTr2MaxNB: wave I owns diagonal block I
+constexpr int kTr2Slots = (kTr2MaxNB * (kTr2MaxNB - 1) / 2 + kTr2Waves - 1) / kTr2Waves;  // 4
+constexpr int kTr2Ld = 33;    // row stride of an LDS diagonal block
+
+// strictly lower block b (I > J), numbered row by row: b = I (I - 1) / 2 + J
+__device__ __forceinline__ void tr_block(int b, int& I, int& J) {
+  int i = 1;
+  while ((i + 1) * i / 2 <= b) i++;
+  I = i;
+  J = b - i * (i - 1) / 2;
+}
+
+// sum over the 16 lanes of each DPP row (inclusive prefix): lane 15 of the
+// row holds the total
+__device__ __forceinline__ double row16_sum(double v) {
+  v += dpp_f64<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_f64<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_f64<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_f64<0x118, 0xf>(v);  // row_shr:8
+  return v;
+}
+
+// Column c, rows r0 .. n-1, of the on-chip matrix into col[].
+__device__ __forceinline__ void tr_column(const d4v (&a)[kTr2Slots][4], const double* Dg, int w, int lane, int nblk,
+                                          int n, int c, int r0, double* col) {
+  const int lr = lane & 15, lk = lane >> 4;
+  const int K = c >> 5, kt = (c >> 4) & 1, kl = c & 15;
+  if (w == K && lane < 32) {
+    const int row = 32 * K + lane;
+    if (row >= r0 && row < n) col[row] = Dg[(size_t)K * 32 * kTr2Ld + lane * kTr2Ld + (c & 31)];
+  }
+#pragma unroll
+  for (int s = 0; s < kTr2Slots; s++) {
+    const int b = w + kTr2Waves * s;
+    if (b >= nblk) continue;
+    int I, J;
+    tr_block(b, I, J);
+    if (J != K || 32 * I + 31 < r0 || lr != kl) continue;
+#pragma unroll
+    for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int row = 32 * I + 16 * ti + 4 * i + lk;
+        const double val = kt ? a[s][ti + 2][i] : a[s][ti][i];
+        if (row >= r0 && row < n) col[row] = val;
+      }
+  }
+}
+
+__global__ void __launch_bounds__(512)
+k_sytrd_reg(double* __restrict__ A, int n, int lda, double* __restrict__ d, double* __restrict__ e,
+            double* __restrict__ tau) {
+  extern __shared__ double sm[];
+  double* xk = sm;                             // 256: column k
+  double* sv = xk + 256;                       // 256: reflector v, zero outside k+1 .. n-1
+  double* sw = sv + 256;                       // 256: w
+  double* pw = sw + 256;                       // kTr2Waves x 256: per-wave partial sums of A v
+  double* Dg = pw + kTr2Waves * 256;           // kTr2MaxNB x 32 x kTr2Ld: diagonal blocks
+  double* stg = Dg + kTr2MaxNB * 32 * kTr2Ld;  // load phase: kTr2Waves x 32 x 33 staging
+  __shared__ double stau;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int NB = (n + 31) / 32, nblk = NB * (NB - 1) / 2;
+  double* Dw = Dg + (size_t)w * 32 * kTr2Ld;   // this wave's diagonal block
+  // diagonal block w (zero beyond n): lane = row, two columns per load
+  if (w < NB) {
+    double v16[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int row = 32 * w + (lane & 31), col = 32 * w + 2 * u + (lane >> 5);
+      v16[u] = (row < n && col < n) ? A[row + (size_t)lda * col] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++) Dw[(lane & 31) * kTr2Ld + 2 * u + (lane >> 5)] = v16[u];
+  }
+  // off-diagonal blocks: the same loads, then through this wave's staging
+  // area into the accumulator layout
+  d4v a[kTr2Slots][4];
+  double* st = stg + w * 32 * 33;
+#pragma unroll
+  for (int s = 0; s < kTr2Slots; s++) {
+    const int b = w + kTr2Waves * s;
+    int I = 1, J = 0;
+    if (b < nblk) tr_block(b, I, J);
+    double v16[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int row = 32 * I + (lane & 31), col = 32 * J + 2 * u + (lane >> 5);
+      v16[u] = (b < nblk && row < n && col < n) ? A[row + (size_t)lda * col] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++) st[(lane & 31) * 33 + 2 * u + (lane >> 5)] = v16[u];
+    wave_lds_sync();
+    blk_from_lds(a[s], st, 33, lane);
+    wave_lds_sync();
+  }
+  __syncthreads();
+  for (int k = 0; k < n - 2; k++) {
+    // (a) column k, rows >= k
+    tr_column(a, Dg, w, lane, nblk, n, k, k, xk);
+    __syncthreads();
+    // (b) reflector (wave 0)
+    if (w == 0) {
+      double ss = 0.0;
+      for (int r = k + 2 + lane; r < n; r += 64) ss += xk[r] * xk[r];
+      ss = wave_sum(ss);
+      const double alpha = xk[k + 1];
+      double tk = 0.0, beta = alpha, scal = 0.0;
+      if (ss > 0.0) {
+        beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+        tk = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+      }
+      if (lane == 0) {
+        d[k] = xk[k];
+        e[k] = beta;
+        tau[k] = tk;
+        stau = tk;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int r = lane + 64 * u;
+        double v = 0.0;
+        if (r == k + 1)
+          v = 1.0;
+        else if (r > k + 1 && r < n)
+          v = xk[r] * scal;
+        sv[r] = v;
+        if (r > k && r < n) A[r + (size_t)lda * k] = v;  // reflector storage for the back-transform
+      }
+    }
+    __syncthreads();
+    // (c) per-wave partial sums of A v over the blocks meeting the trailing matrix
+    const double tk = stau;
+#pragma unroll
+    for (int u = 0; u < 4; u++) pw[w * 256 + lane + 64 * u] = 0.0;
+    wave_lds_sync();
+    if (tk != 0.0) {
+      if (w < NB && 32 * w + 31 > k) {
+        // diagonal block: lane (row, half) sums 16 columns
+        const int r = lane & 31, h = lane >> 5;
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 16; j++) acc += Dw[r * kTr2Ld + 16 * h + j] * sv[32 * w + 16 * h + j];
+        acc += __shfl_xor(acc, 32);
+        if (h == 0) pw[w * 256 + 32 * w + r] += acc;
+        wave_lds_sync();
+      }
+#pragma unroll
+      for (int s = 0; s < kTr2Slots; s++) {
+        const int b = w + kTr2Waves * s;
+        if (b >= nblk) continue;
+        int I, J;
+        tr_block(b, I, J);
+        if (32 * J + 31 <= k) continue;  // v is zero on all its columns
+        const double vc0 = sv[32 * J + lr], vc1 = sv[32 * J + 16 + lr];
+        double rp[2][4];
+#pragma unroll
+        for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+          for (int i = 0; i < 4; i++) rp[ti][i] = row16_sum(a[s][ti][i] * vc0 + a[s][ti + 2][i] * vc1);
+        if (lr == 15) {
+#pragma unroll
+          for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) pw[w * 256 + 32 * I + 16 * ti + 4 * i + lk] += rp[ti][i];
+        }
+        double cp0 = 0.0, cp1 = 0.0;
+#pragma unroll
+        for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            const double vr = sv[32 * I + 16 * ti + 4 * i + lk];
+            cp0 += a[s][ti][i] * vr;
+            cp1 += a[s][ti + 2][i] * vr;
+          }
+        cp0 += __shfl_xor(cp0, 16);
+        cp1 += __shfl_xor(cp1, 16);
+        cp0 += __shfl_xor(cp0, 32);
+        cp1 += __shfl_xor(cp1, 32);
+        if (lk == 0) {
+          pw[w * 256 + 32 * J + lr] += cp0;
+          pw[w * 256 + 32 * J + 16 + lr] += cp1;
+        }
+        wave_lds_sync();  // this block's additions land before the next block's
+      }
+    }
+    __syncthreads();
+    // (d) w = p - (tau/2)(p.v) v, p = tau A v (wave 0)
+    if (w == 0) {
+      double pl[4], pv = 0.0;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int r = lane + 64 * u;
+        double p = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < kTr2Waves; ww++) p += pw[ww * 256 + r];
+        p *= tk;
+        pl[u] = p;
+        pv += p * sv[r];
+      }
+      pv = wave_sum(pv);
+      const double h = 0.5 * tk * pv;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int r = lane + 64 * u;
+        sw[r] = (r > k && r < n) ? pl[u] - h * sv[r] : 0.0;
+      }
+    }
+    __syncthreads();
+    // (e) A -= v w^T + w v^T
+    if (tk != 0.0) {
+      if (w < NB && 32 * w + 31 > k) {
+        const int r = lane & 31, h = lane >> 5;
+        const double vr = sv[32 * w + r], wr = sw[32 * w + r];
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+          const int c = 16 * h + j;
+          Dw[r * kTr2Ld + c] -= vr * sw[32 * w + c] + wr * sv[32 * w + c];
+        }
+      }
+      // register tiles: A operand (row, kk) = (v, w, 0, 0)[kk], B operand
+      // (kk, col) = (-w, -v, 0, 0)[kk]
+#pragma unroll
+      for (int s = 0; s < kTr2Slots; s++) {
+        const int b = w + kTr2Waves * s;
+        if (b >= nblk) continue;
+        int I, J;
+        tr_block(b, I, J);
+        if (32 * J + 31 <= k) continue;
+        double ao[2], bo[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+          const int r = 32 * I + 16 * u + lr, c = 32 * J + 16 * u + lr;
+          ao[u] = lk == 0 ? sv[r] : (lk == 1 ? sw[r] : 0.0);
+          bo[u] = lk == 0 ? -sw[c] : (lk == 1 ? -sv[c] : 0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) a[s][q] = mfma_f64(ao[q & 1], bo[q >> 1], a[s][q]);
+      }
+    }
+    wave_lds_sync();  // this wave's diagonal-block writes before the next column read
+  }
+  // the last 2 x 2: columns n - 2 (rows n-2, n-1) and n - 1
+  tr_column(a, Dg, w, lane, nblk, n, n - 2, n - 2, xk);
+  tr_column(a, Dg, w, lane, nblk, n, n - 1, n - 1, sw);
+  __syncthreads();
+  if (t == 0) {
+    d[n - 2] = xk[n - 2];
+    d[n - 1] = sw[n - 1];
+    e[n - 2] = xk[n - 1];
     tau[n - 2] = 0.0;
   }
 }
@@ -720,10 +994,17 @@ int eig_sym(pw_ctx* c, int n, double* A, int lda, double* w) {
   hipLaunchKernelGGL(k_sym_from_upper, dim3(g1), dim3(256), 0, s, A, n, lda);
   // panel width: the V and W panels (2 nbk n doubles) fit LDS next to v and
   // the partial sums (n = 256: 32; 512: 16; 1024: 8)
-  const int nbk = std::max(1, std::min(32, 8192 / n));
-  const size_t lds_trd = sizeof(double) * ((size_t)2 * nbk * n + n + 1024 + 2 * (size_t)nbk);
-  PW_TRY(ensure_lds((const void*)k_sytrd, lds_trd + 1024));  // + its static LDS
-  hipLaunchKernelGGL(k_sytrd, dim3(1), dim3(kTrdThreads), lds_trd, s, A, n, lda, nbk, d, e, tau);
+  if (n <= 32 * kTr2MaxNB) {
+    const size_t lds_reg = sizeof(double) * (3 * 256 + (size_t)kTr2Waves * 256 + (size_t)kTr2MaxNB * 32 * kTr2Ld +
+                                             (size_t)kTr2Waves * 32 * 33);
+    PW_TRY(ensure_lds((const void*)k_sytrd_reg, lds_reg + 256));
+    hipLaunchKernelGGL(k_sytrd_reg, dim3(1), dim3(64 * kTr2Waves), lds_reg, s, A, n, lda, d, e, tau);
+  } else {
+    const int nbk = std::max(1, std::min(32, 8192 / n));
+    const size_t lds_trd = sizeof(double) * ((size_t)2 * nbk * n + n + 1024 + 2 * (size_t)nbk);
+    PW_TRY(ensure_lds((const void*)k_sytrd, lds_trd + 1024));  // + its static LDS
+    hipLaunchKernelGGL(k_sytrd, dim3(1), dim3(kTrdThreads), lds_trd, s, A, n, lda, nbk, d, e, tau);
+  }
   if (nchunk > 0) {
     const size_t lds_t = sizeof(double) * ((size_t)n * (nbw + 1) + 2 * (size_t)nbw * nbw);
     PW_TRY(ensure_lds((const void*)k_larft, lds_t));
