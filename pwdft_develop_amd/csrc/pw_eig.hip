@@ -278,19 +278,35 @@ __device__ __forceinline__ void tr_block(int b, int& I, int& J) {
   J = b - i * (i - 1) / 2;
 }
 
-// sum over the 16 lanes of each DPP row (inclusive prefix): lane 15 of the
-// row holds the total
-__device__ __forceinline__ double row16_sum(double v) {
-  v += dpp_f64<0x111, 0xf>(v);  // row_shr:1
-  v += dpp_f64<0x112, 0xf>(v);  // row_shr:2
-  v += dpp_f64<0x114, 0xf>(v);  // row_shr:4
-  v += dpp_f64<0x118, 0xf>(v);  // row_shr:8
-  return v;
+// Sums over the 16 lanes of each DPP row of eight values r[0..7] per lane,
+// by a butterfly that halves the values at every stage: lanes l, l ^ 8
+// (row_ror 8), then l, 7 - l within each half row (row_half_mirror), then
+// l ^ 2 and l ^ 1 (quad_perm).  Lanes 2 j and 2 j + 1 of the row end with
+// the total of r[j], j = (l % 16) / 2.
+__device__ __forceinline__ double row16_sum8(const double (&r)[8], int lr) {
+  const bool b3 = lr & 8, b2 = lr & 4, b1 = lr & 2;
+  double s1[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const double keep = b3 ? r[4 + j] : r[j], send = b3 ? r[j] : r[4 + j];
+    s1[j] = keep + dpp_f64<0x128, 0xf>(send);  // row_ror:8
+  }
+  double s2[2];
+#pragma unroll
+  for (int j = 0; j < 2; j++) {
+    const double keep = b2 ? s1[2 + j] : s1[j], send = b2 ? s1[j] : s1[2 + j];
+    s2[j] = keep + dpp_f64<0x141, 0xf>(send);  // row_half_mirror
+  }
+  const double keep = b1 ? s2[1] : s2[0], send = b1 ? s2[0] : s2[1];
+  double s3 = keep + dpp_f64<0x4E, 0xf>(send);  // quad_perm [2,3,0,1]
+  return s3 + dpp_f64<0xB1, 0xf>(s3);            // quad_perm [1,0,3,2]
 }
 
-// Column c, rows r0 .. n-1, of the on-chip matrix into col[].
-__device__ __forceinline__ void tr_column(const d4v (&a)[kTr2Slots][4], const double* Dg, int w, int lane, int nblk,
-                                          int n, int c, int r0, double* col) {
+// Column c, rows r0 .. n-1, of the on-chip matrix into col[] (slot s of
+// this wave holds block (bI[s], bJ[s]), bJ[s] < 0 for none).
+__device__ __forceinline__ void tr_column(const d4v (&a)[kTr2Slots][4], const int (&bI)[kTr2Slots],
+                                          const int (&bJ)[kTr2Slots], const double* Dg, int w, int lane, int n,
+                                          int c, int r0, double* col) {
   const int lr = lane & 15, lk = lane >> 4;
   const int K = c >> 5, kt = (c >> 4) & 1, kl = c & 15;
   if (w == K && lane < 32) {
@@ -299,11 +315,8 @@ __device__ __forceinline__ void tr_column(const d4v (&a)[kTr2Slots][4], const do
   }
 #pragma unroll
   for (int s = 0; s < kTr2Slots; s++) {
-    const int b = w + kTr2Waves * s;
-    if (b >= nblk) continue;
-    int I, J;
-    tr_block(b, I, J);
-    if (J != K || 32 * I + 31 < r0 || lr != kl) continue;
+    const int I = bI[s];
+    if (bJ[s] != K || 32 * I + 31 < r0 || lr != kl) continue;
 #pragma unroll
     for (int ti = 0; ti < 2; ti++)
 #pragma unroll
@@ -342,19 +355,27 @@ k_sytrd_reg(double* __restrict__ A, int n, int lda, double* __restrict__ d, doub
     for (int u = 0; u < 16; u++) Dw[(lane & 31) * kTr2Ld + 2 * u + (lane >> 5)] = v16[u];
   }
   // off-diagonal blocks: the same loads, then through this wave's staging
-  // area into the accumulator layout
+  // area into the accumulator layout; block coordinates per slot (wave
+  // uniform, scalar registers)
+  int bI[kTr2Slots], bJ[kTr2Slots];
+#pragma unroll
+  for (int s = 0; s < kTr2Slots; s++) {
+    const int b = __builtin_amdgcn_readfirstlane(w) + kTr2Waves * s;
+    int I = 1, J = -1;
+    if (b < nblk) tr_block(b, I, J);
+    bI[s] = I;
+    bJ[s] = J;
+  }
   d4v a[kTr2Slots][4];
   double* st = stg + w * 32 * 33;
 #pragma unroll
   for (int s = 0; s < kTr2Slots; s++) {
-    const int b = w + kTr2Waves * s;
-    int I = 1, J = 0;
-    if (b < nblk) tr_block(b, I, J);
+    const int I = bI[s], J = bJ[s];
     double v16[16];
 #pragma unroll
     for (int u = 0; u < 16; u++) {
       const int row = 32 * I + (lane & 31), col = 32 * J + 2 * u + (lane >> 5);
-      v16[u] = (b < nblk && row < n && col < n) ? A[row + (size_t)lda * col] : 0.0;
+      v16[u] = (J >= 0 && row < n && col < n) ? A[row + (size_t)lda * col] : 0.0;
     }
 #pragma unroll
     for (int u = 0; u < 16; u++) st[(lane & 31) * 33 + 2 * u + (lane >> 5)] = v16[u];
@@ -365,12 +386,17 @@ k_sytrd_reg(double* __restrict__ A, int n, int lda, double* __restrict__ d, doub
   __syncthreads();
   for (int k = 0; k < n - 2; k++) {
     // (a) column k, rows >= k
-    tr_column(a, Dg, w, lane, nblk, n, k, k, xk);
+    tr_column(a, bI, bJ, Dg, w, lane, n, k, k, xk);
     __syncthreads();
     // (b) reflector (wave 0)
     if (w == 0) {
       double ss = 0.0;
-      for (int r = k + 2 + lane; r < n; r += 64) ss += xk[r] * xk[r];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int r = k + 2 + lane + 64 * u;
+        const double x = r < n ? xk[r] : 0.0;
+        ss += x * x;
+      }
       ss = wave_sum(ss);
       const double alpha = xk[k + 1];
       double tk = 0.0, beta = alpha, scal = 0.0;
@@ -416,22 +442,18 @@ k_sytrd_reg(double* __restrict__ A, int n, int lda, double* __restrict__ d, doub
       }
 #pragma unroll
       for (int s = 0; s < kTr2Slots; s++) {
-        const int b = w + kTr2Waves * s;
-        if (b >= nblk) continue;
-        int I, J;
-        tr_block(b, I, J);
-        if (32 * J + 31 <= k) continue;  // v is zero on all its columns
+        const int I = bI[s], J = bJ[s];
+        if (J < 0 || 32 * J + 31 <= k) continue;  // no block, or v is zero on all its columns
         const double vc0 = sv[32 * J + lr], vc1 = sv[32 * J + 16 + lr];
-        double rp[2][4];
+        double rp[8];
 #pragma unroll
         for (int ti = 0; ti < 2; ti++)
 #pragma unroll
-          for (int i = 0; i < 4; i++) rp[ti][i] = row16_sum(a[s][ti][i] * vc0 + a[s][ti + 2][i] * vc1);
-        if (lr == 15) {
-#pragma unroll
-          for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-            for (int i = 0; i < 4; i++) pw[w * 256 + 32 * I + 16 * ti + 4 * i + lk] += rp[ti][i];
+          for (int i = 0; i < 4; i++) rp[4 * ti + i] = a[s][ti][i] * vc0 + a[s][ti + 2][i] * vc1;
+        const double rs = row16_sum8(rp, lr);
+        if ((lr & 1) == 0) {
+          const int j = lr >> 1;  // = 4 ti + i
+          pw[w * 256 + 32 * I + 16 * (j >> 2) + 4 * (j & 3) + lk] += rs;
         }
         double cp0 = 0.0, cp1 = 0.0;
 #pragma unroll
@@ -451,6 +473,9 @@ k_sytrd_reg(double* __restrict__ A, int n, int lda, double* __restrict__ d, doub
           pw[w * 256 + 32 * J + 16 + lr] += cp1;
         }
         wave_lds_sync();  // this block's additions land before the next block's
+        // one block at a time: keeps the scheduler from overlapping blocks
+#pragma unroll
+        for (int q = 0; q < 4; q++) asm volatile("" : "+v"(a[s][q]));
       }
     }
     __syncthreads();
@@ -491,11 +516,8 @@ k_sytrd_reg(double* __restrict__ A, int n, int lda, double* __restrict__ d, doub
       // (kk, col) = (-w, -v, 0, 0)[kk]
 #pragma unroll
       for (int s = 0; s < kTr2Slots; s++) {
-        const int b = w + kTr2Waves * s;
-        if (b >= nblk) continue;
-        int I, J;
-        tr_block(b, I, J);
-        if (32 * J + 31 <= k) continue;
+        const int I = bI[s], J = bJ[s];
+        if (J < 0 || 32 * J + 31 <= k) continue;
         double ao[2], bo[2];
 #pragma unroll
         for (int u = 0; u < 2; u++) {
@@ -510,8 +532,8 @@ k_sytrd_reg(double* __restrict__ A, int n, int lda, double* __restrict__ d, doub
     wave_lds_sync();  // this wave's diagonal-block writes before the next column read
   }
   // the last 2 x 2: columns n - 2 (rows n-2, n-1) and n - 1
-  tr_column(a, Dg, w, lane, nblk, n, n - 2, n - 2, xk);
-  tr_column(a, Dg, w, lane, nblk, n, n - 1, n - 1, sw);
+  tr_column(a, bI, bJ, Dg, w, lane, n, n - 2, n - 2, xk);
+  tr_column(a, bI, bJ, Dg, w, lane, n, n - 1, n - 1, sw);
   __syncthreads();
   if (t == 0) {
     d[n - 2] = xk[n - 2];
