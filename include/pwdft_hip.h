@@ -155,8 +155,9 @@ int pw_rayleigh_ritz(pw_ctx* ctx, int hdtype, int nb, void* X, double* eig,
 
 /* One ChebFSI sweep -- the body of the reference's GeneralChebyStep(numEig,
  * filterOrder) call site (src/SCF/scf.cpp:1321-1332): Lanczos(k) bounds from
- * v0, degree-m filter in mode precision (PW_F64, or PW_F32 = mixed: FP32
- * filter, FP64 Cholesky QR and Rayleigh-Ritz), then RR.  With prev_eig the
+ * v0 and the degree-m filter in mode precision (PW_F64, or PW_F32 = mixed:
+ * FP32 Lanczos and filter, FP64 Cholesky QR and Rayleigh-Ritz), then RR.
+ * With prev_eig the
  * cut is prev_eig[nb-1] and the scale point prev_eig[0]; without, the first
  * step uses aL = theta_1, a = theta_1 + 0.1 (b_up - theta_1).  bounds (may
  * be NULL) receives {a, b_up, aL}. */
@@ -166,11 +167,12 @@ int pw_chebfsi_sweep(pw_ctx* ctx, int mode, int nb, int m, int k,
 
 int pw_free_workspace(pw_ctx* ctx);
 
-/* Graph capture of the Chebyshev filter (default on): the first call of a
- * (dtype, nb, degree) shape runs eagerly, the second is captured into a
- * hipGraph, and later calls replay it on a private stream ordered after the
- * context's stream.  enable = 0 runs every filter eagerly.  No counterpart in
- * the reference (a launch-overhead device for small cells). */
+/* Graph capture of the Chebyshev filter and of the Lanczos steps (default
+ * on): the first call of a shape ((dtype, nb, degree), resp. (dtype, k))
+ * runs eagerly, the second is captured into a hipGraph, and later calls
+ * replay it on a private stream ordered after the context's stream.
+ * enable = 0 runs everything eagerly.  No counterpart in the reference (a
+ * launch-overhead device for small cells). */
 int pw_set_graphs(pw_ctx* ctx, int enable);
 
 /* EigenSolver::LOBPCGSolveReal(numEig, scfIter, eigMaxIter, eigMinTolerance,

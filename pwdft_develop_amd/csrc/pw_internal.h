@@ -212,6 +212,8 @@ struct pw_ctx {
   hipGraphExec_t fg_exec = nullptr;
   long fg_key = -1, fg_warm = -1;
   void* fg_result = nullptr;
+  hipGraphExec_t lz_exec = nullptr;  // Lanczos steps, keyed by their count
+  long lz_key = -1, lz_warm = -1;
   // timing
   static constexpr int kTimedCalls = 1024;
   bool timing = false;
@@ -224,5 +226,5 @@ namespace pw {
 // A captured filter graph holds raw device pointers and the shape of the
 // operator (potential, projectors, exchange): it is dropped whenever any of
 // them is reallocated or switched on or off.
-inline void invalidate_graph(pw_ctx* c) { c->fg_key = c->fg_warm = -1; }
+inline void invalidate_graph(pw_ctx* c) { c->fg_key = c->fg_warm = c->lz_key = c->lz_warm = -1; }
 }  // namespace pw

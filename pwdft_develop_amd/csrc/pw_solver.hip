@@ -129,6 +129,121 @@ __global__ void k_scale_to(double* __restrict__ out, const double* __restrict__ 
     out[i] = in[i] * s;
 }
 
+// FP32 vectors of the mixed-precision Lanczos bound: the same three
+// operations, dot products accumulated in FP64.
+__global__ void __launch_bounds__(1024) k_dot_f(const float* __restrict__ x, const float* __restrict__ y, long n,
+                                                double* out) {
+  double s = 0.0;
+  for (long i0 = threadIdx.x; i0 < n; i0 += 8L * blockDim.x) {
+    float a[8], b[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const long i = i0 + (long)u * blockDim.x;
+      a[u] = i < n ? x[i] : 0.f;
+      b[u] = i < n ? y[i] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) s += (double)a[u] * (double)b[u];
+  }
+  __shared__ double red[1024];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+__global__ void k_lanczos_update_f(float* __restrict__ f, const float* __restrict__ v, const float* __restrict__ vp,
+                                   const double* __restrict__ ab, long n) {
+  const float a = (float)ab[0], b = (float)ab[1];
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    f[i] -= a * v[i] + b * vp[i];
+}
+
+__global__ void k_scale_to_f(float* __restrict__ out, const float* __restrict__ in, const double* __restrict__ nrm2,
+                             long n) {
+  const float s = (float)(1.0 / sqrt(nrm2[0]));
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    out[i] = in[i] * s;
+}
+
+// One FP32 Lanczos vector step in a single workgroup, the vectors held in
+// registers (n <= kLzMaxPer * 1024): alpha = v.f, f <- f - alpha v - beta
+// v_prev (beta = ab[1]), hist = (alpha, ||f||^2), ab[1] = ||f||, and unless
+// this is the last step v_prev <- f / ||f|| (the next v).  Fixed-order
+// block sums (deterministic).
+constexpr int kLzMaxPer = 16;
+__device__ __forceinline__ double lz_block_sum(double v, double* red) {
+  v = wave_sum(v);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int w = 0; w < 16; w++) s += red[w];
+  return s;
+}
+template <typename T>
+__global__ void __launch_bounds__(1024)
+k_lanczos_step(const T* __restrict__ v, const T* __restrict__ f, T* __restrict__ vp, double* __restrict__ ab,
+               double* __restrict__ hist, long n, int last) {
+  __shared__ double red[16];
+  const int t = threadIdx.x;
+  T xv[kLzMaxPer], xf[kLzMaxPer];
+#pragma unroll
+  for (int u = 0; u < kLzMaxPer; u++) {
+    const long i = t + 1024L * u;
+    xv[u] = i < n ? v[i] : (T)0;
+    xf[u] = i < n ? f[i] : (T)0;
+  }
+  double sa = 0.0;
+#pragma unroll
+  for (int u = 0; u < kLzMaxPer; u++) sa += (double)xv[u] * (double)xf[u];
+  const double alpha = lz_block_sum(sa, red);
+  const T a = (T)alpha, b = (T)ab[1];
+  double sf = 0.0;
+  // v_prev in batches of four loads (registers for three full vectors would
+  // not fit 1024 threads in FP64)
+#pragma unroll
+  for (int u0 = 0; u0 < kLzMaxPer; u0 += 4) {
+    asm volatile("" ::: "memory");  // one batch of loads at a time
+    T xp[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const long i = t + 1024L * (u0 + u);
+      xp[u] = i < n ? vp[i] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      xf[u0 + u] -= a * xv[u0 + u] + b * xp[u];
+      sf += (double)xf[u0 + u] * (double)xf[u0 + u];
+    }
+  }
+  const double nf = lz_block_sum(sf, red);
+  if (t == 0) {
+    hist[0] = alpha;
+    hist[1] = nf;
+    ab[1] = sqrt(nf);
+  }
+  if (!last) {
+    const T sc = (T)(1.0 / sqrt(nf));
+#pragma unroll
+    for (int u = 0; u < kLzMaxPer; u++) {
+      const long i = t + 1024L * u;
+      if (i < n) vp[i] = xf[u] * sc;
+    }
+  }
+}
+
+static inline void launch_dot(bool f32, const void* x, const void* y, long n, double* out, hipStream_t s) {
+  if (f32)
+    hipLaunchKernelGGL(k_dot_f, dim3(1), dim3(1024), 0, s, (const float*)x, (const float*)y, n, out);
+  else
+    hipLaunchKernelGGL(k_dot, dim3(1), dim3(1024), 0, s, (const double*)x, (const double*)y, n, out);
+}
+
 static inline int grid_for(long n) { return (int)std::min<long>((n + 255) / 256, 8192); }
 
 // Symmetric eigen-decomposition in place: A (n x n, upper triangle, lda)
@@ -168,7 +283,7 @@ static int ensure_ws(pw_ctx* c) {
   PW_HIP(hipMalloc((void**)&c->d_red, sizeof(double) * (std::max(c->max_bands, 128) + 8)));
   PW_HIP(hipMalloc((void**)&c->d_info, sizeof(int) * 4));
   PW_HIP(hipMalloc((void**)&c->d_chol, sizeof(double) * chol_dinv_doubles(chol_inv_max_n())));
-  PW_HIP(hipMalloc((void**)&c->d_lz, sizeof(double) * 3 * 2 * (size_t)c->npw));
+  PW_HIP(hipMalloc((void**)&c->d_lz, sizeof(double) * 4 * 2 * (size_t)c->npw));
   PW_HIP(hipMalloc((void**)&c->d_coef, sizeof(double) * 192));
   PW_HIP(hipMalloc((void**)&c->d_coef_f, sizeof(float) * 192));
   c->ws_nb = c->max_bands;
@@ -197,6 +312,9 @@ extern "C" int pw_free_workspace(pw_ctx* c) {
   // the captured filter graph refers to the workspace: drop it too
   if (c->fg_exec) (void)hipGraphExecDestroy(c->fg_exec);
   c->fg_exec = nullptr;
+  if (c->lz_exec) (void)hipGraphExecDestroy(c->lz_exec);
+  c->lz_exec = nullptr;
+  c->lz_key = c->lz_warm = -1;
   c->fg_key = c->fg_warm = -1;
   if (c->gstream) (void)hipStreamDestroy(c->gstream);
   c->gstream = nullptr;
@@ -205,38 +323,144 @@ extern "C" int pw_free_workspace(pw_ctx* c) {
   return PW_OK;
 }
 
-extern "C" int pw_lanczos(pw_ctx* c, int k, const void* v0, double* theta, double* b_up) {
+// Capture body() (stream-ordered work on c->stream) on the context's
+// private stream into *exec, replacing what it held.
+template <typename Body>
+static int graph_capture(pw_ctx* c, hipGraphExec_t* exec, Body body) {
+  if (*exec) {
+    (void)hipGraphExecDestroy(*exec);
+    *exec = nullptr;
+  }
+  if (!c->gstream) PW_HIP(hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking));
+  if (!c->gev[0]) {
+    PW_HIP(hipEventCreateWithFlags(&c->gev[0], hipEventDisableTiming));
+    PW_HIP(hipEventCreateWithFlags(&c->gev[1], hipEventDisableTiming));
+  }
+  hipStream_t user = c->stream;
+  const bool timing = c->timing;
+  c->timing = false;  // no event nodes in the graph
+  c->stream = c->gstream;
+  hipGraph_t g = nullptr;
+  int rc = PW_OK;
+  if (hipStreamBeginCapture(c->gstream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    rc = fail(PW_ERR_HIP, "hipStreamBeginCapture failed");
+  } else {
+    rc = body();
+    const hipError_t e = hipStreamEndCapture(c->gstream, &g);
+    if (rc == PW_OK && e != hipSuccess) rc = fail(PW_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    if (rc == PW_OK && hipGraphInstantiate(exec, g, nullptr, nullptr, 0) != hipSuccess)
+      rc = fail(PW_ERR_HIP, "hipGraphInstantiate failed");
+  }
+  if (g) (void)hipGraphDestroy(g);
+  c->stream = user;
+  c->timing = timing;
+  return rc;
+}
+
+// Replay: ordered after the work already queued on the context's stream,
+// and the stream waits for the replay.
+static int graph_launch(pw_ctx* c, hipGraphExec_t exec) {
+  PW_HIP(hipEventRecord(c->gev[0], c->stream));
+  PW_HIP(hipStreamWaitEvent(c->gstream, c->gev[0], 0));
+  PW_HIP(hipGraphLaunch(exec, c->gstream));
+  PW_HIP(hipEventRecord(c->gev[1], c->gstream));
+  PW_HIP(hipStreamWaitEvent(c->stream, c->gev[1], 0));
+  return PW_OK;
+}
+
+__global__ void k_zero_d(double* __restrict__ p, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+// k Lanczos steps (vectors in dtype) from the normalised copy of v0 held at
+// d_lz + 3 m; alpha_j and ||f_j||^2 to hist = d_red + 4.  Three rotating
+// vectors (v, v_prev, f), no copies, zeroing by kernel: only stream-ordered
+// kernel work, safe to capture.
+static int lanczos_body(pw_ctx* c, int dtype, int k) {
+  const bool f32 = dtype == PW_F32;
+  const long m = 2L * c->npw;
+  const size_t es = f32 ? sizeof(float) : sizeof(double);
+  char* base = (char*)c->d_lz;
+  void* B[3] = {base, base + es * m, base + 2 * es * m};
+  const void* v0 = base + 3 * es * m;
+  double* ab = c->d_red;  // ab[0] = alpha, ab[1] = beta, ab[2] = ||f||^2
+  double* hist = ab + 4;  // 2k <= 128 entries; d_red holds max(max_bands, 128) + 8
+  hipStream_t s = c->stream;
+  const dim3 g(grid_for(m)), b(256);
+  // v_prev = 0 (m = 2 npw is even: m / 2 doubles hold m floats)
+  hipLaunchKernelGGL(k_zero_d, dim3(grid_for(m)), dim3(256), 0, s, (double*)B[1], f32 ? m / 2 : m);
+  hipLaunchKernelGGL(k_zero_d, dim3(1), dim3(64), 0, s, ab, 4L);
+  launch_dot(f32, v0, v0, m, ab + 2, s);
+  if (f32)
+    hipLaunchKernelGGL(k_scale_to_f, g, b, 0, s, (float*)B[0], (const float*)v0, (const double*)(ab + 2), m);
+  else
+    hipLaunchKernelGGL(k_scale_to, g, b, 0, s, (double*)B[0], (const double*)v0, (const double*)(ab + 2), m);
+  int iv = 0, ivp = 1;
+  void* f = B[2];
+  for (int j = 0; j < k; j++) {
+    void* v = B[iv];
+    PW_TRY(pw_apply_h(c, dtype, 1, v, f));
+    if (f32 && m <= 1024L * kLzMaxPer) {
+      // FP32: the vector step fused into one launch (three vectors fit one
+      // workgroup's registers); the next v goes over v_prev
+      hipLaunchKernelGGL(k_lanczos_step<float>, dim3(1), dim3(1024), 0, s, (const float*)v, (const float*)f,
+                         (float*)B[ivp], ab, hist + 2 * j, m, j + 1 == k ? 1 : 0);
+      std::swap(iv, ivp);
+      continue;
+    }
+    launch_dot(f32, v, f, m, ab, s);
+    if (f32)
+      hipLaunchKernelGGL(k_lanczos_update_f, g, b, 0, s, (float*)f, (const float*)v, (const float*)B[ivp],
+                         (const double*)ab, m);
+    else
+      hipLaunchKernelGGL(k_lanczos_update, g, b, 0, s, (double*)f, (const double*)v, (const double*)B[ivp],
+                         (const double*)ab, m);
+    launch_dot(f32, f, f, m, ab + 2, s);
+    hipLaunchKernelGGL(k_lanczos_record, dim3(1), dim3(1), 0, s, ab, hist + 2 * j);
+    if (j + 1 < k) {
+      // v_prev <- v ; v <- f / beta, written over the old v_prev
+      if (f32)
+        hipLaunchKernelGGL(k_scale_to_f, g, b, 0, s, (float*)B[ivp], (const float*)f, (const double*)(ab + 2), m);
+      else
+        hipLaunchKernelGGL(k_scale_to, g, b, 0, s, (double*)B[ivp], (const double*)f, (const double*)(ab + 2), m);
+      std::swap(iv, ivp);
+    }
+  }
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+// Lanczos bound in dtype (PW_F64 for the C-ABI entry point; the mixed sweep
+// runs it in PW_F32): theta (k Ritz values of T_k, ascending) and the upper
+// bound theta_k + |beta_k|.
+static int lanczos_run(pw_ctx* c, int dtype, int k, const void* v0, double* theta, double* b_up) {
   if (!c || k < 1 || !v0 || !theta || !b_up) return fail(PW_ERR_ARG, "pw_lanczos: bad argument");
   if (k > 64) return fail(PW_ERR_ARG, "pw_lanczos: k > 64");
   PW_TRY(ensure_ws(c));
   const long m = 2L * c->npw;
-  double* v = c->d_lz;
-  double* vp = v + m;
-  double* f = vp + m;
-  double* ab = c->d_red;  // ab[0] = alpha, ab[1] = beta, ab[2] = ||f||^2
   hipStream_t s = c->stream;
-  PW_HIP(hipMemsetAsync(vp, 0, sizeof(double) * m, s));
-  PW_HIP(hipMemsetAsync(ab, 0, sizeof(double) * 4, s));
-  hipLaunchKernelGGL(k_dot, dim3(1), dim3(1024), 0, s, (const double*)v0, (const double*)v0, m, ab + 2);
-  hipLaunchKernelGGL(k_scale_to, dim3(grid_for(m)), dim3(256), 0, s, v, (const double*)v0, ab + 2, m);
-  // alpha_j, ||f_j||^2 recorded on the device; one readback at the end
-  double* hist = ab + 4;  // 2k <= 128 entries; d_red holds max(max_bands, 128) + 8
-  std::vector<double> al(k), be(k);
-  for (int j = 0; j < k; j++) {
-    PW_TRY(pw_apply_h(c, PW_F64, 1, v, f));
-    hipLaunchKernelGGL(k_dot, dim3(1), dim3(1024), 0, s, (const double*)v, (const double*)f, m, ab);
-    hipLaunchKernelGGL(k_lanczos_update, dim3(grid_for(m)), dim3(256), 0, s, f, (const double*)v,
-                       (const double*)vp, (const double*)ab, m);
-    hipLaunchKernelGGL(k_dot, dim3(1), dim3(1024), 0, s, (const double*)f, (const double*)f, m, ab + 2);
-    hipLaunchKernelGGL(k_lanczos_record, dim3(1), dim3(1), 0, s, ab, hist + 2 * j);
-    if (j + 1 < k) {
-      // vp <- v ; v <- f / beta ; beta (for the next step) = ||f|| stays on the device
-      PW_HIP(hipMemcpyAsync(vp, v, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
-      hipLaunchKernelGGL(k_scale_to, dim3(grid_for(m)), dim3(256), 0, s, v, (const double*)f, ab + 2, m);
-    }
+  if (dtype == PW_F32)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(m)), dim3(256), 0, s, (float*)c->d_lz + 3 * m,
+                       (const double*)v0, m);
+  else
+    PW_HIP(hipMemcpyAsync(c->d_lz + 3 * m, v0, sizeof(double) * m, hipMemcpyDeviceToDevice, s));
+  // the steps are replayed from a graph after one eager call of this shape
+  const long key = 100L * dtype + k;
+  if (!c->use_graphs) {
+    PW_TRY(lanczos_body(c, dtype, k));
+  } else if (c->lz_exec && c->lz_key == key) {
+    PW_TRY(graph_launch(c, c->lz_exec));
+  } else if (c->lz_warm == key) {
+    PW_TRY(graph_capture(c, &c->lz_exec, [&] { return lanczos_body(c, dtype, k); }));
+    c->lz_key = key;
+    PW_TRY(graph_launch(c, c->lz_exec));
+  } else {
+    PW_TRY(lanczos_body(c, dtype, k));
+    c->lz_warm = key;
   }
+  std::vector<double> al(k), be(k);
   std::vector<double> h(2 * k);
-  PW_HIP(hipMemcpyAsync(h.data(), hist, sizeof(double) * 2 * k, hipMemcpyDeviceToHost, s));
+  PW_HIP(hipMemcpyAsync(h.data(), c->d_red + 4, sizeof(double) * 2 * k, hipMemcpyDeviceToHost, s));
   PW_HIP(hipStreamSynchronize(s));
   for (int j = 0; j < k; j++) {
     al[j] = h[2 * j];
@@ -277,6 +501,10 @@ extern "C" int pw_lanczos(pw_ctx* c, int k, const void* v0, double* theta, doubl
   std::sort(theta, theta + k);
   *b_up = theta[k - 1] + std::fabs(be[k - 1]);
   return PW_OK;
+}
+
+extern "C" int pw_lanczos(pw_ctx* c, int k, const void* v0, double* theta, double* b_up) {
+  return lanczos_run(c, PW_F64, k, v0, theta, b_up);
 }
 
 // Y = (HX - c X) s with (c, s) = coef[0..1]
@@ -337,51 +565,6 @@ static int cheb_body(pw_ctx* c, int dtype, int nb, int m, void** result) {
   return PW_OK;
 }
 
-// Capture the filter body on the context's private stream into a graph.
-static int graph_capture(pw_ctx* c, int dtype, int nb, int m) {
-  if (c->fg_exec) {
-    (void)hipGraphExecDestroy(c->fg_exec);
-    c->fg_exec = nullptr;
-  }
-  if (!c->gstream) PW_HIP(hipStreamCreateWithFlags(&c->gstream, hipStreamNonBlocking));
-  if (!c->gev[0]) {
-    PW_HIP(hipEventCreateWithFlags(&c->gev[0], hipEventDisableTiming));
-    PW_HIP(hipEventCreateWithFlags(&c->gev[1], hipEventDisableTiming));
-  }
-  hipStream_t user = c->stream;
-  const bool timing = c->timing;
-  c->timing = false;  // no event nodes in the graph
-  c->stream = c->gstream;
-  hipGraph_t g = nullptr;
-  int rc = PW_OK;
-  if (hipStreamBeginCapture(c->gstream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-    rc = fail(PW_ERR_HIP, "hipStreamBeginCapture failed");
-  } else {
-    void* res = nullptr;
-    rc = cheb_body(c, dtype, nb, m, &res);
-    const hipError_t e = hipStreamEndCapture(c->gstream, &g);
-    if (rc == PW_OK && e != hipSuccess) rc = fail(PW_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-    if (rc == PW_OK && hipGraphInstantiate(&c->fg_exec, g, nullptr, nullptr, 0) != hipSuccess)
-      rc = fail(PW_ERR_HIP, "hipGraphInstantiate failed");
-    c->fg_result = res;
-  }
-  if (g) (void)hipGraphDestroy(g);
-  c->stream = user;
-  c->timing = timing;
-  return rc;
-}
-
-// Replay: ordered after the work already queued on the context's stream,
-// and the stream waits for the replay.
-static int graph_launch(pw_ctx* c) {
-  PW_HIP(hipEventRecord(c->gev[0], c->stream));
-  PW_HIP(hipStreamWaitEvent(c->gstream, c->gev[0], 0));
-  PW_HIP(hipGraphLaunch(c->fg_exec, c->gstream));
-  PW_HIP(hipEventRecord(c->gev[1], c->gstream));
-  PW_HIP(hipStreamWaitEvent(c->stream, c->gev[1], 0));
-  return PW_OK;
-}
-
 // Chebyshev filter of degree m on X (dtype blocks, in place).
 extern "C" int pw_cheb_filter(pw_ctx* c, int dtype, int nb, int m, double a, double b, double aL, void* X) {
   if (!c || !X || nb < 1 || nb > c->max_bands || m < 1) return fail(PW_ERR_ARG, "pw_cheb_filter: bad argument");
@@ -417,12 +600,12 @@ extern "C" int pw_cheb_filter(pw_ctx* c, int dtype, int nb, int m, double a, dou
   if (!c->use_graphs) {
     PW_TRY(cheb_body(c, dtype, nb, m, &result));
   } else if (c->fg_exec && c->fg_key == key) {
-    PW_TRY(graph_launch(c));
+    PW_TRY(graph_launch(c, c->fg_exec));
     result = c->fg_result;
   } else if (c->fg_warm == key) {
-    PW_TRY(graph_capture(c, dtype, nb, m));
+    PW_TRY(graph_capture(c, &c->fg_exec, [&] { return cheb_body(c, dtype, nb, m, &c->fg_result); }));
     c->fg_key = key;
-    PW_TRY(graph_launch(c));
+    PW_TRY(graph_launch(c, c->fg_exec));
     result = c->fg_result;
   } else {
     // first call of this shape runs eagerly (allocating every lazily sized
@@ -523,15 +706,16 @@ extern "C" int pw_rayleigh_ritz(pw_ctx* c, int hdtype, int nb, void* X, double* 
   return PW_OK;
 }
 
-// One ChebFSI sweep on an FP64 block X: Lanczos(k) bounds from v0, degree-m
-// filter (FP64 or FP32 per mode), Cholesky QR + Rayleigh-Ritz (FP64 H X).
+// One ChebFSI sweep on an FP64 block X: Lanczos(k) bounds from v0 and the
+// degree-m filter, both in the precision of mode, then Cholesky QR +
+// Rayleigh-Ritz (FP64 H X).
 extern "C" int pw_chebfsi_sweep(pw_ctx* c, int mode, int nb, int m, int k, const void* v0,
                                 const double* prev_eig, void* X, double* eig, double* res, double* bounds) {
   if (!c || !X || !v0 || !eig || !res) return fail(PW_ERR_ARG, "pw_chebfsi_sweep: null argument");
   if (mode != PW_F64 && mode != PW_F32) return fail(PW_ERR_ARG, "pw_chebfsi_sweep: unknown mode %d", mode);
   std::vector<double> theta(k > 0 ? k : 1);
   double bup = 0.0;
-  PW_TRY(pw_lanczos(c, k, v0, theta.data(), &bup));
+  PW_TRY(lanczos_run(c, mode, k, v0, theta.data(), &bup));
   double a, aL;
   if (prev_eig) {
     a = prev_eig[nb - 1];

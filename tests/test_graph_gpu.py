@@ -73,3 +73,28 @@ def test_filter_graph_dropped_on_new_projectors(si8):
         assert np.array_equal(Xg.cpu().numpy(), Xe.cpu().numpy())
     cg.close()
     ce.close()
+
+
+def test_lanczos_graph_replays_match_eager(si8):
+    """The Lanczos steps replayed from a graph (after one eager call and one
+    capture of the same step count) equal the eager steps bit for bit, with
+    a different start vector buffer on every call, eager filter work in
+    between, and a new step count captured anew."""
+    import torch
+    from pwdft_develop_amd.ops import Context
+    print("case: si8_32 lanczos graph replays", flush=True)
+    cg = Context(si8)
+    ce = Context(si8)
+    ce.set_graphs(False)
+    rng = np.random.default_rng(11)
+    X = cg.to_device(si8.psi)
+    for k in (8, 8, 8, 8, 5, 5, 5, 8):
+        v = si8.lanczos_start() + 0.1 * (rng.standard_normal(si8.npw) + 1j * rng.standard_normal(si8.npw))
+        v[0] = v[0].real
+        tg, bg = cg.lanczos(k, cg.to_device(v))
+        te, be = ce.lanczos(k, ce.to_device(v))
+        assert np.array_equal(tg, te) and bg == be, (k, np.abs(tg - te).max(), bg - be)
+        cg.cheb_filter(X, 4, 0.3, 12.0, -1.0)  # eager or replayed filter work between calls
+        torch.cuda.synchronize()
+    cg.close()
+    ce.close()
