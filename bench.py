@@ -153,7 +153,7 @@ def main():
     # FP32 square grids of the register-FFT sizes take k_slab_xy_reg (8 bands
     # per z plane), others the LDS kernel k_slab_xy
     reg = prob.grid[0] == prob.grid[1] and prob.grid[0] in (24, 30, 32, 36, 40, 48, 60, 64, 72)
-    kname = (f"k_slab_xy_reg<float, {prob.grid[0]}, {prob.grid[1]}, 8>" if reg
+    kname = (f"k_slab_xy_reg<float, {prob.grid[0]}, {prob.grid[1]}, 8, 2>" if reg
              else f"k_slab_xy<float, {prob.grid[0]}, {prob.grid[1]}>")
     traffic = pmc_traffic(os.path.join(ROOT, "profiles", f"pmc_traffic_{prob.name}.md"), kname)
 

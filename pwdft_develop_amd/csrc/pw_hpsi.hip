@@ -171,12 +171,12 @@ k_slab_xy(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restric
 // by scheduling barriers so that the compiler does not hoist all of them next
 // to the spectrum (that spills past the 256 VGPRs of two waves per SIMD).
 // Same arithmetic, in the same order, as k_slab_xy.
-template <typename T, int N1, int N2, int SB>
-__global__ void __launch_bounds__(256, 2)  // two waves per SIMD: <= 256 VGPRs
+template <typename T, int N1, int N2, int SB, int LB>
+__global__ void __launch_bounds__(256, LB)  // LB waves per SIMD: <= 512 / LB registers
 k_slab_xy_reg(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restrict__ t2, int n3, int nstick,
               int kxmax, int nb, const int* __restrict__ pos, const int* __restrict__ mpos,
               const T* __restrict__ vtot) {
-  static_assert(sizeof(T) == 4 && N1 % 2 == 0 && N2 % 2 == 0, "FP32, even N1 and N2");
+  static_assert(N1 % 2 == 0 && N2 % 2 == 0, "even N1 and N2");
   typedef typename cx<T>::t C;
   constexpr int H1 = N1 / 2, NPAIR = N2 / 2, LV = N1 + 1, KC = 8;
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -271,9 +271,17 @@ k_slab_xy_reg(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __res
 #pragma unroll
       for (int kk = k0; kk < k0 + KC; kk++) {
         if (kk > H1) continue;
-        float4 p = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (kk <= kxmax) p = *reinterpret_cast<const float4*>(Ys + kk * N2 + y1);
-        if (kk == 0 || kk == H1) { p.y = 0.f; p.w = 0.f; }
+        struct { T x, y, z, w; } p = {0, 0, 0, 0};
+        if (kk <= kxmax) {
+          if constexpr (sizeof(T) == 4) {
+            const float4 q = *reinterpret_cast<const float4*>(Ys + kk * N2 + y1);
+            p = {q.x, q.y, q.z, q.w};
+          } else {
+            const C q1 = Ys[kk * N2 + y1], q2 = Ys[kk * N2 + y1 + 1];
+            p = {q1.x, q1.y, q2.x, q2.y};
+          }
+        }
+        if (kk == 0 || kk == H1) { p.y = 0; p.w = 0; }
         // row y1 spectrum (p.x, p.y), row y2 (p.z, p.w); the kx > H1 half is
         // their conjugates
         a[kk] = C{p.x - p.w, p.y + p.z};
@@ -297,8 +305,13 @@ k_slab_xy_reg(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __res
       if (k <= kxmax) {
         const C w = a[k];
         const C wc = a[k == 0 ? 0 : N1 - k];
-        *reinterpret_cast<float4*>(Ys + k * N2 + y1) =
-            make_float4(0.5f * (w.x + wc.x), 0.5f * (w.y - wc.y), 0.5f * (w.y + wc.y), 0.5f * (wc.x - w.x));
+        const C o1{(T)0.5 * (w.x + wc.x), (T)0.5 * (w.y - wc.y)}, o2{(T)0.5 * (w.y + wc.y), (T)0.5 * (wc.x - w.x)};
+        if constexpr (sizeof(T) == 4) {
+          *reinterpret_cast<float4*>(Ys + k * N2 + y1) = make_float4(o1.x, o1.y, o2.x, o2.y);
+        } else {
+          Ys[k * N2 + y1] = o1;
+          Ys[k * N2 + y1 + 1] = o2;
+        }
       }
     }
   }
@@ -386,15 +399,23 @@ auto slab_kernel(int n1, int n2) -> decltype(&k_slab_xy<T, 0, 0>) {
 
 // Register slab kernel: FP32 square planes of these sizes (a row of N1
 // complex floats per thread stays within the register budget up to 72).
-constexpr int kSlabSB = 8;
+// bands per z plane and waves per SIMD of the register slab kernel: FP32
+// 8 bands, two waves; FP64 4 bands (the y-space of 8 would not leave LDS
+// for a second workgroup either), two waves up to 48 points, one beyond
+// (the 60-point double FFT pair needs about 350 registers)
+template <typename T> struct SlabCfg { static constexpr int SB = 8; };
+template <> struct SlabCfg<double> { static constexpr int SB = 4; };
+template <typename T, int N> constexpr int slab_lb() { return (sizeof(T) == 4 || N <= 48) ? 2 : 1; }
 #define PW_REG_SIZES(X) X(24) X(30) X(32) X(36) X(40) X(48) X(60) X(64) X(72)
-typedef decltype(&k_slab_xy_reg<float, 24, 24, kSlabSB>) slab_reg_fn;
-static slab_reg_fn slab_reg_kernel(int n1, int n2) {
+template <typename T>
+using slab_reg_fn = decltype(&k_slab_xy_reg<T, 24, 24, SlabCfg<T>::SB, 2>);
+template <typename T>
+static slab_reg_fn<T> slab_reg_kernel(int n1, int n2) {
   if (n1 != n2) return nullptr;
   switch (n1) {
 #define PW_CASE(n) \
   case n:          \
-    return &k_slab_xy_reg<float, n, n, kSlabSB>;
+    return &k_slab_xy_reg<T, n, n, SlabCfg<T>::SB, slab_lb<T, n>()>;
     PW_REG_SIZES(PW_CASE)
 #undef PW_CASE
     default:
@@ -778,15 +799,15 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
                      c->d_pt_sz, tw3);
   PW_HIP(hipGetLastError());
   PW_TRY(mark(c, 1));
-  slab_reg_fn kreg = (sizeof(T) == 4 && c->use_reg_slab) ? slab_reg_kernel(c->n1, c->n2) : nullptr;
+  slab_reg_fn<T> kreg = c->use_reg_slab ? slab_reg_kernel<T>(c->n1, c->n2) : nullptr;
   if (kreg) {
-    // one z plane of kSlabSB bands per workgroup, V plane + y-space in LDS
-    const size_t ldsr = sizeof(float2) * ((size_t)kSlabSB * (c->sp.kxmax + 1) * c->n2) +
-                        sizeof(float2) * (size_t)(c->n2 / 2) * (c->n1 + 1) + sizeof(int) * 2 * (size_t)c->sp.nstick;
+    // one z plane of SB bands per workgroup, V plane + y-space in LDS
+    constexpr int SB = SlabCfg<T>::SB;
+    const size_t ldsr = sizeof(C) * ((size_t)SB * (c->sp.kxmax + 1) * c->n2) +
+                        sizeof(C) * (size_t)(c->n2 / 2) * (c->n1 + 1) + sizeof(int) * 2 * (size_t)c->sp.nstick;
     PW_TRY(ensure_lds((const void*)kreg, ldsr));
-    hipLaunchKernelGGL(kreg, dim3(c->n3, (nb + kSlabSB - 1) / kSlabSB), dim3(256), ldsr, c->stream,
-                       (const float2*)t1, (float2*)t2, c->n3, c->sp.nstick, c->sp.kxmax, nb, c->d_pos, c->d_mpos,
-                       (const float*)vt);
+    hipLaunchKernelGGL(kreg, dim3(c->n3, (nb + SB - 1) / SB), dim3(256), ldsr, c->stream, (const C*)t1, (C*)t2,
+                       c->n3, c->sp.nstick, c->sp.kxmax, nb, c->d_pos, c->d_mpos, (const T*)vt);
   } else {
     auto kslab = slab_kernel<T>(c->n1, c->n2);
     PW_TRY(ensure_lds((const void*)kslab, lds2));

@@ -75,6 +75,28 @@ def test_fp32_register_slab_kernel(name, nb, monkeypatch):
     clds.close()
 
 
+@pytest.mark.parametrize("name,nb", [("si8_32", 32), ("si8_32", 7), ("rand16_48", 9), ("si64_60", 5)])
+def test_fp64_register_slab_kernel(name, nb, monkeypatch):
+    """The FP64 register slab kernel (one z plane of 4 bands per workgroup,
+    ragged last group; one wave per SIMD at 60 points) against the oracle
+    (FP64 parity) and against the LDS slab kernel (same arithmetic)."""
+    from pwdft_develop_amd.ops import Context
+    prob = W.WORKLOADS[name](nb=nb).build()
+    print(f"case: {name} fp64 slab kernels nb={nb}", flush=True)
+    ref = O.Hamiltonian(prob).apply_h(prob.psi)
+    monkeypatch.delenv("PW_SLAB", raising=False)
+    creg = Context(prob)
+    monkeypatch.setenv("PW_SLAB", "lds")
+    clds = Context(prob)
+    X = creg.to_device(prob.psi)
+    a = creg.apply_h(X).cpu().numpy()
+    b = clds.apply_h(X).cpu().numpy()
+    assert _rel(a, ref) < 1e-12
+    assert _rel(a, b) < 1e-13
+    creg.close()
+    clds.close()
+
+
 def test_teter(ctx8, si8):
     X = ctx8.to_device(si8.psi[:3])
     T = ctx8.teter_precond(X).cpu().numpy()
