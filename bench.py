@@ -48,15 +48,17 @@ def staged_bytes(prob, p):
 
 
 def pmc_traffic(path, kernel):
-    """HBM bytes per launch of `kernel` from a committed rocprofv3 --pmc
-    summary (a line 'traffic <kernel...> bytes_per_launch=<B>'), or None."""
+    """(HBM bytes per launch, full kernel name) of the kernel whose name
+    starts with `kernel`, from a committed rocprofv3 --pmc summary (a line
+    'traffic <kernel name> bytes_per_launch=<B>'), or (None, None)."""
     try:
         for line in open(path):
-            if line.startswith("traffic ") and kernel in line and "bytes_per_launch=" in line:
-                return int(float(line.split("bytes_per_launch=")[1].split()[0]))
+            if line.startswith("traffic " + kernel) and "bytes_per_launch=" in line:
+                name = line[len("traffic "):line.index(" bytes_per_launch=")]
+                return int(float(line.split("bytes_per_launch=")[1].split()[0])), name
     except OSError:
         pass
-    return None
+    return None, None
 
 
 def host_cores():
@@ -153,9 +155,10 @@ def main():
     # FP32 square grids of the register-FFT sizes take k_slab_xy_reg (8 bands
     # per z plane), others the LDS kernel k_slab_xy
     reg = prob.grid[0] == prob.grid[1] and prob.grid[0] in (24, 30, 32, 36, 40, 48, 60, 64, 72)
-    kname = (f"k_slab_xy_reg<float, {prob.grid[0]}, {prob.grid[1]}, 8, 2>" if reg
+    kname = (f"k_slab_xy_reg<float, {prob.grid[0]}, {prob.grid[1]}," if reg
              else f"k_slab_xy<float, {prob.grid[0]}, {prob.grid[1]}>")
-    traffic = pmc_traffic(os.path.join(ROOT, "profiles", f"pmc_traffic_{prob.name}.md"), kname)
+    traffic, full = pmc_traffic(os.path.join(ROOT, "profiles", f"pmc_traffic_{prob.name}.md"), kname)
+    kname = full or (kname + " ...>" if reg else kname)
 
     def staged(tag, p):
         local_ms = sum(stages[tag][:3])

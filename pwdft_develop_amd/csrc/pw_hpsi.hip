@@ -400,9 +400,10 @@ auto slab_kernel(int n1, int n2) -> decltype(&k_slab_xy<T, 0, 0>) {
 // Register slab kernel: FP32 square planes of these sizes (a row of N1
 // complex floats per thread stays within the register budget up to 72).
 // bands per z plane and waves per SIMD of the register slab kernel: FP32
-// 8 bands, two waves; FP64 4 bands (the y-space of 8 would not leave LDS
-// for a second workgroup either), two waves up to 48 points, one beyond
-// (the 60-point double FFT pair needs about 350 registers)
+// 8 bands, two waves (4 bands at three waves measured slower: 116 against
+// 98 us at 60^3); FP64 4 bands (the y-space of 8 would not leave LDS for a
+// second workgroup either), two waves up to 48 points, one beyond (the
+// 60-point double FFT pair needs about 350 registers)
 template <typename T> struct SlabCfg { static constexpr int SB = 8; };
 template <> struct SlabCfg<double> { static constexpr int SB = 4; };
 template <typename T, int N> constexpr int slab_lb() { return (sizeof(T) == 4 || N <= 48) ? 2 : 1; }
