@@ -266,6 +266,45 @@ int pw_nonlocal_force(pw_ctx* ctx, int nb, const void* X, const double* f,
 int pw_set_timing(pw_ctx* ctx, int enable);
 int pw_get_timing(pw_ctx* ctx, double* ms4, int* ncalls);
 
+/* ------------------------------------------------------------------ */
+/* Real-space dual-grid Gamma-point path (Use_Real_Space = 1, FFT_Number_Type
+ * "even" or "power"): wavefunctions are real on the coarse grid (ntot =
+ * n1*n2*n3 values per band, band-major, DEVICE pointers), the local
+ * potential and the projectors live on the fine grid (numGridFine).  The
+ * handle plays the part of Fourier's R2C and R2CFine plans and buffers
+ * (src/SCF/fourier.cpp:166-183, 246-330); grids must be even and the fine
+ * grid at least the coarse one in every dimension.
+ * ekin: host array on the coarse R2C grid ((n1/2+1)*n2*n3), the reference's
+ * ekin_ without sphere cut (src/SCF/hamiltonian.cpp:2629-2667).
+ * max_bands caps the bands transformed per batch (the library also keeps
+ * its batch buffers within about 4 GiB); any nb is accepted later. */
+typedef struct pw_rs pw_rs;
+int pw_rs_create(pw_rs** rs, const int grid[3], const int grid_fine[3], double vol,
+                 const double* ekin, int max_bands, void* stream);
+int pw_rs_destroy(pw_rs* rs);
+int pw_rs_set_stream(pw_rs* rs, void* stream);
+/* ntot, ntotFine and the bands per batch; informational. */
+int pw_rs_info(const pw_rs* rs, long* ntot, long* ntot_fine, int* chunk);
+
+/* vtot_[:, spinswitch_] on the fine grid (src/SCF/hamiltonian.cpp:3120-3122),
+ * host array of ntotFine doubles. */
+int pw_rs_set_vtot(pw_rs* rs, const double* vtot);
+
+/* pseudo_[a].vnlList (src/SCF/periodtable.cpp:1042-1589), host arrays, per
+ * atom a: npts[a] fine-grid indices (the shared iv of the atom's SparseVecs),
+ * nproj[a] projectors (<= 32) whose VAL columns form an npts[a] x nproj[a]
+ * column-major block of val, and one weight each in wgt.  natom = 0 removes
+ * the nonlocal term. */
+int pw_rs_set_nonlocal(pw_rs* rs, int natom, const int* npts, const int* nproj,
+                       const int* idx, const double* val, const double* wgt);
+
+/* KohnSham::MultSpinor(Spinor&, NumTns<Real>&, Fourier&)
+ * (src/SCF/hamiltonian.cpp:3058-3125) with the real Spinor::
+ * AddMultSpinorFineR2C (src/SCF/spinor.cpp:376-525): a3 = H psi (kinetic on
+ * the coarse grid, local and nonlocal terms on the fine grid through IP_c2f /
+ * IP_f2c, src/utility.cpp:3150-3276), a3 overwritten; psi is not modified. */
+int pw_rs_mult_spinor(pw_rs* rs, int nb, const double* psi, double* a3);
+
 #ifdef __cplusplus
 }
 #endif

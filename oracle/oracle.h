@@ -58,6 +58,14 @@ int or_fft_c2r(const int* n, const double* in, double* out);
 /* Unnormalised 3D r2c: in n1*n2*n3 real -> out h1*n2*n3 complex. */
 int or_fft_r2c(const int* n, const double* in, double* out);
 
+/* ---------- real-space dual-grid H*psi (or_rs.c) ---------- */
+void or_ip_c2f(const int* Nc, const int* Nf, const double* tilde, double* tildef);
+void or_ip_f2c(const int* Nc, const int* Nf, double* tilde, const double* tildef);
+int or_rs_mult_spinor(const int* Nc, const int* Nf, double vol, const double* ekin,
+                      const double* vtot, int natom, const int* npts, const int* nproj,
+                      const int* idx, const double* val, const double* wgt, int nb,
+                      const double* psi, double* a3);
+
 /* ---------- Hamiltonian ---------- */
 typedef struct or_ham or_ham;
 

@@ -70,6 +70,12 @@ def lib():
         L.or_band_energy.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.or_band_energy.restype = C.c_double
         L.or_nonlocal_force.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp]
+        L.or_ip_c2f.argtypes = [_ip, _ip, _dp, _dp]
+        L.or_ip_c2f.restype = None
+        L.or_ip_f2c.argtypes = [_ip, _ip, _dp, _dp]
+        L.or_ip_f2c.restype = None
+        L.or_rs_mult_spinor.argtypes = [_ip, _ip, C.c_double, _dp, _dp, C.c_int, _ip, _ip, _ip, _dp,
+                                        _dp, C.c_int, _dp, _dp]
         _LIB = L
     return _LIB
 
@@ -320,3 +326,58 @@ def solve_gen_eig(A, B, width):
     e = np.zeros(n)
     rc = lib().or_solve_gen_eig(n, n, width, a, b, e)
     return rc, e[:width], a.reshape(n, n, order="F")
+
+
+# ---------------------------------------------------------------- real space
+def _half_shape(n):
+    return (int(n[2]), int(n[1]), int(n[0]) // 2 + 1)
+
+
+def ip_c2f(nc, nf, coarse):
+    """IP_c2f (utility.cpp:3150-3212): coarse half spectrum (Nc2, Nc1, Nc0/2+1)
+    -> fine half spectrum (Nf2, Nf1, Nf0/2+1), zeros where nothing lands."""
+    nc, nf = np.asarray(nc, np.int32), np.asarray(nf, np.int32)
+    out = np.zeros(_half_shape(nf), np.complex128)
+    src = np.ascontiguousarray(coarse, np.complex128)
+    assert src.shape == _half_shape(nc)
+    ov = out.reshape(-1).view(np.float64)
+    lib().or_ip_c2f(nc, nf, src.reshape(-1).view(np.float64), ov)
+    return out
+
+
+def ip_f2c(nc, nf, fine, coarse=None):
+    """IP_f2c (utility.cpp:3215-3276): returns coarse + restriction(fine)."""
+    nc, nf = np.asarray(nc, np.int32), np.asarray(nf, np.int32)
+    out = np.zeros(_half_shape(nc), np.complex128) if coarse is None else \
+        np.array(coarse, np.complex128, copy=True)
+    src = np.ascontiguousarray(fine, np.complex128)
+    assert src.shape == _half_shape(nf)
+    lib().or_ip_f2c(nc, nf, out.reshape(-1).view(np.float64), src.reshape(-1).view(np.float64))
+    return out
+
+
+def rs_mult_spinor(nc, nf, vol, ekin, vtot, npts, nproj, idx, val, wgt, psi):
+    """Real-space dual-grid a3 = H psi (spinor.cpp:376-525).  psi: (nb, ntot)
+    real, band-major; ekin: coarse R2C grid; vtot: fine grid; projectors per
+    atom (npts, nproj, idx, val column-major per atom, wgt)."""
+    nc, nf = np.asarray(nc, np.int32), np.asarray(nf, np.int32)
+    psi = np.ascontiguousarray(psi, np.float64)
+    nb = psi.shape[0]
+    out = np.zeros_like(psi)
+    npts = np.ascontiguousarray(npts, np.int32)
+    nproj = np.ascontiguousarray(nproj, np.int32)
+    idx = np.ascontiguousarray(idx, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    wgt = np.ascontiguousarray(wgt, np.float64)
+    if idx.size == 0:
+        idx = np.zeros(1, np.int32)
+    if val.size == 0:
+        val = np.zeros(1)
+    if wgt.size == 0:
+        wgt = np.zeros(1)
+    _chk(lib().or_rs_mult_spinor(nc, nf, float(vol), np.ascontiguousarray(ekin, np.float64).reshape(-1),
+                                 np.ascontiguousarray(vtot, np.float64).reshape(-1), len(npts),
+                                 npts if npts.size else np.zeros(1, np.int32),
+                                 nproj if nproj.size else np.zeros(1, np.int32), idx, val, wgt, nb,
+                                 psi.reshape(-1), out.reshape(-1)), "rs_mult_spinor")
+    return out

@@ -84,6 +84,13 @@ def lib():
                     C.POINTER(C.c_int)],
         "pw_set_timing": [_vp, C.c_int],
         "pw_get_timing": [_vp, _dp, C.POINTER(C.c_int)],
+        "pw_rs_create": [C.POINTER(_vp), _ip, _ip, C.c_double, _dp, C.c_int, _vp],
+        "pw_rs_destroy": [_vp],
+        "pw_rs_set_stream": [_vp, _vp],
+        "pw_rs_info": [_vp, C.POINTER(C.c_long), C.POINTER(C.c_long), C.POINTER(C.c_int)],
+        "pw_rs_set_vtot": [_vp, _dp],
+        "pw_rs_set_nonlocal": [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp],
+        "pw_rs_mult_spinor": [_vp, C.c_int, _vp, _vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)

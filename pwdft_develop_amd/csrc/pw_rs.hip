@@ -1,0 +1,601 @@
+// Real-space dual-grid Gamma-point H*psi (Use_Real_Space = 1, FFT_Number_Type
+// "even"/"power"): KohnSham::MultSpinor(Spinor&, NumTns<Real>&, Fourier&)
+// (src/SCF/hamiltonian.cpp:3058-3125) with the real-a3 Spinor::
+// AddMultSpinorFineR2C (src/SCF/spinor.cpp:376-525), IP_c2f / IP_f2c
+// (src/utility.cpp:3150-3276) and the sparse real-space projectors of
+// PeriodTable::CalculateNonlocalPP (src/SCF/periodtable.cpp:1042-1589).
+//
+// Per chunk of bands (all on the handle's stream):
+//   1. coarse R2C of psi                                   (rocFFT)
+//   2. k_rs_c2f: coarse spectrum -> fine spectrum, written whole (zeros
+//      included) as a gather, so no memset and no scatter conflicts
+//   3. fine C2R                                            (rocFFT)
+//   4. k_rs_weights: w[p, b] = vol/Nf * wgt_p * sum_i val_p[i] psi_b[idx[i]]
+//   5. k_rs_local: u = V psi + sum over the projectors touching each point,
+//      in place, through a point-major index of the projector entries
+//   6. fine R2C                                            (rocFFT)
+//   7. k_rs_f2c_ekin: ekin .* coarse spectrum + restriction of the fine one
+//   8. coarse C2R straight into a3                         (rocFFT)
+// The FFTs are plain full-grid transforms of any length (the reference's own
+// decks have prime factors 31 and 17), hence the library; every step between
+// them is a kernel here.  FFTWExecute's vol/N and 1/vol factors
+// (src/SCF/fourier.cpp:695-760) cancel pairwise and are folded into steps 2
+// and 7 as 1/Nc and 1/Nf.
+#include <rocfft/rocfft.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+
+#include "pw_internal.h"
+
+using namespace pw;
+
+#define PW_FFT(call)                                                                   \
+  do {                                                                                 \
+    rocfft_status s_ = (call);                                                         \
+    if (s_ != rocfft_status_success)                                                   \
+      return ::pw::fail(PW_ERR_HIP, "%s failed: rocfft status %d (%s:%d)", #call,      \
+                        (int)s_, __FILE__, __LINE__);                                  \
+  } while (0)
+
+namespace {
+
+struct RsPlans {
+  rocfft_plan c_fwd = nullptr, f_inv = nullptr, f_fwd = nullptr, c_inv = nullptr;
+};
+
+// Per-atom projector table: points, first global projector, projector count,
+// offset of the atom's npts x nproj value block.
+struct AtomRec {
+  int npts, pg0, np, vo;
+};
+
+constexpr int kRowThreads = 64;
+
+}  // namespace
+
+struct pw_rs {
+  int nc[3], nf[3];
+  int hc0, hf0;             // n0/2 + 1 of the coarse and the fine grid
+  long Nc, Nf, Hc, Hf;      // real and half-spectrum sizes per band
+  double vol;
+  int chunk;
+  hipStream_t stream;
+  int device;
+  double* d_ekin = nullptr;
+  double* d_vtot = nullptr;
+  // projectors
+  int natom = 0, nproj_tot = 0, npmax = 0;
+  AtomRec* d_atab = nullptr;
+  int* d_apt_off = nullptr;   // natom + 1 offsets into d_idx
+  int* d_idx = nullptr;
+  double* d_val = nullptr;
+  double* d_wgt = nullptr;
+  int* d_rowptr = nullptr;    // Nf + 1
+  int2* d_ent = nullptr;      // (atom, local point) sorted by point, then atom
+  // chunk buffers
+  double2* d_xc = nullptr;
+  double2* d_ff = nullptr;
+  double* d_pf = nullptr;
+  double* d_coef = nullptr;
+  void* d_work = nullptr;
+  size_t work_bytes = 0;
+  rocfft_execution_info info = nullptr;
+  std::map<int, RsPlans> plans;
+};
+
+// ---------------------------------------------------------------- kernels
+
+// Step 2: fine half spectrum from the coarse one (IP_c2f, utility.cpp:
+// 3150-3212, as a gather).  Fine row (jF, kF) takes coarse row (j, k) where
+// jF is j's primary image (j <= Nc1/2 -> j, else Nf1 - Nc1 + j) or, for the
+// Nyquist j = Nc1/2, its duplicate cjF = Nf1 - Nc1/2; the reference's branch
+// chain decides the value: primaries carry t/2 if any of i, j, k is a Nyquist
+// index (else t); a j-duplicate carries t/2 only for "j Nyquist alone", a
+// k-duplicate only for "k alone", a (j, k)-duplicate only for "j and k, i
+// not".  One wavefront per fine row; grid (Nf1 * Nf2, bands).
+__global__ __launch_bounds__(kRowThreads) void k_rs_c2f(const double2* __restrict__ xc,
+                                                       double2* __restrict__ ff, int nc0, int nc1,
+                                                       int nc2, int nf1, int nf2, int hc0,
+                                                       int hf0, long Hc, long Hf, double scale) {
+  const int row = blockIdx.x;
+  const int jF = row % nf1, kF = row / nf1;
+  const long b = blockIdx.y;
+  double2* out = ff + b * Hf + (long)row * hf0;
+  const int Jn = nc1 / 2, Kn = nc2 / 2, In = nc0 / 2;
+  // source row in j: dj = 0 primary, 1 duplicate, -1 none
+  int j, dj;
+  if (jF <= Jn) { j = jF; dj = 0; }
+  else if (jF > nf1 - Jn) { j = jF - nf1 + nc1; dj = 0; }
+  else if (jF == nf1 - Jn) { j = Jn; dj = 1; }
+  else { j = 0; dj = -1; }
+  int k, dk;
+  if (kF <= Kn) { k = kF; dk = 0; }
+  else if (kF > nf2 - Kn) { k = kF - nf2 + nc2; dk = 0; }
+  else if (kF == nf2 - Kn) { k = Kn; dk = 1; }
+  else { k = 0; dk = -1; }
+  const bool nj = (j == Jn), nk = (k == Kn);
+  const double2* src = xc + b * Hc + ((long)k * nc1 + j) * hc0;
+  const bool row_live = dj >= 0 && dk >= 0;
+  for (int i = threadIdx.x; i < hf0; i += kRowThreads) {
+    double2 v = make_double2(0.0, 0.0);
+    if (row_live && i <= In) {
+      const bool ni = (i == In);
+      double f;
+      if (dj == 0 && dk == 0) f = (ni || nj || nk) ? 0.5 : 1.0;
+      else if (dj == 1 && dk == 0) f = (!ni && !nk) ? 0.5 : 0.0;
+      else if (dj == 0) f = (!ni && !nj) ? 0.5 : 0.0;
+      else f = !ni ? 0.5 : 0.0;
+      if (f != 0.0) {
+        const double2 t = src[i];
+        v = make_double2(t.x * (f * scale), t.y * (f * scale));
+      }
+    }
+    out[i] = v;
+  }
+}
+
+// Step 7: xc <- ekin .* xc * sc + restriction(ff) * sf (IP_f2c, utility.cpp:
+// 3215-3276, and the kinetic multiply of spinor.cpp:469-473).  One wavefront
+// per coarse row; grid (Nc1 * Nc2, bands).
+__global__ __launch_bounds__(kRowThreads) void k_rs_f2c_ekin(double2* __restrict__ xc,
+                                                            const double2* __restrict__ ff,
+                                                            const double* __restrict__ ekin, int nc0,
+                                                            int nc1, int nc2, int nf1, int nf2,
+                                                            int hc0, int hf0, long Hc, long Hf,
+                                                            double sc, double sf) {
+  const int row = blockIdx.x;
+  const int j = row % nc1, k = row / nc1;
+  const long b = blockIdx.y;
+  const int Jn = nc1 / 2, Kn = nc2 / 2, In = nc0 / 2;
+  const int jF = (j <= Jn) ? j : nf1 - nc1 + j;
+  const int kF = (k <= Kn) ? k : nf2 - nc2 + k;
+  const int cjF = nf1 - Jn, ckF = nf2 - Kn;
+  const int mjF = (nf1 - jF) % nf1, mkF = (nf2 - kF) % nf2;
+  const bool nj = (j == Jn), nk = (k == Kn);
+  const double2* fb = ff + b * Hf;
+  const double2* f1row = fb + ((long)kF * nf1 + jF) * hf0;
+  // second row for the i < Nc0/2 entries (plain sum)
+  const double2* f2row = nullptr;
+  if (nj && nk) f2row = fb + ((long)ckF * nf1 + cjF) * hf0;
+  else if (nj) f2row = fb + ((long)kF * nf1 + cjF) * hf0;
+  else if (nk) f2row = fb + ((long)ckF * nf1 + jF) * hf0;
+  // second row for the i = Nc0/2 entry (conjugate sum)
+  const double2* g2row;
+  if (nj) g2row = fb + ((long)mkF * nf1 + jF) * hf0;
+  else if (nk) g2row = fb + ((long)kF * nf1 + mjF) * hf0;
+  else g2row = fb + ((long)mkF * nf1 + mjF) * hf0;
+  double2* crow = xc + b * Hc + (long)row * hc0;
+  const double* erow = ekin + (long)row * hc0;
+  for (int i = threadIdx.x; i < hc0; i += kRowThreads) {
+    const double2 f1 = f1row[i];
+    double2 acc;
+    if (i == In) {
+      if (nj && nk) {
+        acc = make_double2(f1.x, 0.0);
+      } else {
+        const double2 f2 = g2row[i];
+        acc = make_double2(0.5 * (f1.x + f2.x), 0.5 * (f1.y - f2.y));
+      }
+    } else if (f2row) {
+      const double2 f2 = f2row[i];
+      acc = make_double2(0.5 * (f1.x + f2.x), 0.5 * (f1.y + f2.y));
+    } else {
+      acc = f1;
+    }
+    const double2 x = crow[i];
+    const double e = erow[i] * sc;
+    crow[i] = make_double2(fma(e, x.x, acc.x * sf), fma(e, x.y, acc.y * sf));
+  }
+}
+
+// Step 4: projector weights of one atom for BB bands.  Grid (natom, band
+// groups), 256 threads striding over the atom's points; NPMAX x BB partial
+// sums per thread, reduced across each wavefront with DPP and across the
+// four wavefronts through LDS.  coef[p * nbc + b].
+template <int NPMAX, int BB>
+__global__ __launch_bounds__(256) void k_rs_weights(const double* __restrict__ pf,
+                                                   const AtomRec* __restrict__ atab,
+                                                   const int* __restrict__ apt_off,
+                                                   const int* __restrict__ idx,
+                                                   const double* __restrict__ val,
+                                                   const double* __restrict__ wgt,
+                                                   double* __restrict__ coef, int nbc, long Nf,
+                                                   double scale) {
+  __shared__ double part[4][NPMAX * BB];
+  const int a = blockIdx.x;
+  const int b0 = blockIdx.y * BB;
+  const int nbb = min(BB, nbc - b0);
+  const AtomRec rec = atab[a];
+  const int* ia = idx + apt_off[a];
+  const double* va = val + rec.vo;
+  double acc[NPMAX][BB];
+#pragma unroll
+  for (int p = 0; p < NPMAX; p++)
+#pragma unroll
+    for (int q = 0; q < BB; q++) acc[p][q] = 0.0;
+  for (int i = threadIdx.x; i < rec.npts; i += 256) {
+    const long r = ia[i];
+    double ps[BB];
+#pragma unroll
+    for (int q = 0; q < BB; q++) ps[q] = (q < nbb) ? pf[(long)(b0 + q) * Nf + r] : 0.0;
+#pragma unroll
+    for (int p = 0; p < NPMAX; p++) {
+      if (p < rec.np) {
+        const double v = va[(long)p * rec.npts + i];
+#pragma unroll
+        for (int q = 0; q < BB; q++) acc[p][q] = fma(v, ps[q], acc[p][q]);
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int p = 0; p < NPMAX; p++)
+#pragma unroll
+    for (int q = 0; q < BB; q++) {
+      const double s = wave_sum(acc[p][q]);
+      if (lane == 0) part[wv][p * BB + q] = s;
+    }
+  __syncthreads();
+  for (int t = threadIdx.x; t < rec.np * BB; t += 256) {
+    const int p = t / BB, q = t % BB;
+    if (q < nbb) {
+      const double s = (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]);
+      coef[(long)(rec.pg0 + p) * nbc + b0 + q] = s * (scale * wgt[rec.pg0 + p]);
+    }
+  }
+}
+
+// Step 5: pf <- vtot .* pf + sum over the (atom, point) entries of each grid
+// point of val_p * coef[p, b], in the reference's order (local term first,
+// then atom by atom, projector by projector; spinor.cpp:437-467).  One thread
+// per fine point, bands in groups of 8.
+__global__ __launch_bounds__(256) void k_rs_local(double* __restrict__ pf,
+                                                 const double* __restrict__ vtot,
+                                                 const int* __restrict__ rowptr,
+                                                 const int2* __restrict__ ent,
+                                                 const AtomRec* __restrict__ atab,
+                                                 const double* __restrict__ val,
+                                                 const double* __restrict__ coef, int nbc, long Nf,
+                                                 int has_nl) {
+  const long r = (long)blockIdx.x * 256 + threadIdx.x;
+  if (r >= Nf) return;
+  const double v = vtot[r];
+  const int e0 = has_nl ? rowptr[r] : 0, e1 = has_nl ? rowptr[r + 1] : 0;
+  constexpr int G = 8;
+  for (int b0 = 0; b0 < nbc; b0 += G) {
+    const int nbb = min(G, nbc - b0);
+    double u[G];
+#pragma unroll
+    for (int q = 0; q < G; q++) u[q] = (q < nbb) ? v * pf[(long)(b0 + q) * Nf + r] : 0.0;
+    for (int e = e0; e < e1; e++) {
+      const int2 en = ent[e];
+      const AtomRec rec = atab[en.x];
+      const double* vp = val + rec.vo + en.y;
+      for (int p = 0; p < rec.np; p++) {
+        const double dv = vp[(long)p * rec.npts];
+        const double* cp = coef + (long)(rec.pg0 + p) * nbc + b0;
+#pragma unroll
+        for (int q = 0; q < G; q++)
+          if (q < nbb) u[q] = fma(dv, cp[q], u[q]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < G; q++)
+      if (q < nbb) pf[(long)(b0 + q) * Nf + r] = u[q];
+  }
+}
+
+// ---------------------------------------------------------------- host
+
+namespace {
+
+std::once_flag g_rocfft_once;
+rocfft_status g_rocfft_setup = rocfft_status_success;
+
+int make_plan(rocfft_plan* plan, rocfft_transform_type type, const int* n, int batch) {
+  const size_t len[3] = {(size_t)n[0], (size_t)n[1], (size_t)n[2]};
+  const size_t h0 = (size_t)n[0] / 2 + 1;
+  const size_t rstr[3] = {1, len[0], len[0] * len[1]};
+  const size_t cstr[3] = {1, h0, h0 * len[1]};
+  const size_t rdist = len[0] * len[1] * len[2], cdist = h0 * len[1] * len[2];
+  rocfft_plan_description desc = nullptr;
+  PW_FFT(rocfft_plan_description_create(&desc));
+  rocfft_status s;
+  if (type == rocfft_transform_type_real_forward)
+    s = rocfft_plan_description_set_data_layout(desc, rocfft_array_type_real,
+                                                rocfft_array_type_hermitian_interleaved, nullptr,
+                                                nullptr, 3, rstr, rdist, 3, cstr, cdist);
+  else
+    s = rocfft_plan_description_set_data_layout(desc, rocfft_array_type_hermitian_interleaved,
+                                                rocfft_array_type_real, nullptr, nullptr, 3, cstr,
+                                                cdist, 3, rstr, rdist);
+  if (s == rocfft_status_success)
+    s = rocfft_plan_create(plan, rocfft_placement_notinplace, type, rocfft_precision_double, 3, len,
+                           (size_t)batch, desc);
+  rocfft_plan_description_destroy(desc);
+  if (s != rocfft_status_success)
+    return fail(PW_ERR_HIP, "rocfft_plan_create (%d x %d x %d, batch %d) failed: status %d", n[0],
+                n[1], n[2], batch, (int)s);
+  return PW_OK;
+}
+
+int get_plans(pw_rs* rs, int batch, RsPlans** out) {
+  auto it = rs->plans.find(batch);
+  if (it != rs->plans.end()) {
+    *out = &it->second;
+    return PW_OK;
+  }
+  RsPlans p;
+  int rc;
+  if ((rc = make_plan(&p.c_fwd, rocfft_transform_type_real_forward, rs->nc, batch)) != PW_OK ||
+      (rc = make_plan(&p.f_inv, rocfft_transform_type_real_inverse, rs->nf, batch)) != PW_OK ||
+      (rc = make_plan(&p.f_fwd, rocfft_transform_type_real_forward, rs->nf, batch)) != PW_OK ||
+      (rc = make_plan(&p.c_inv, rocfft_transform_type_real_inverse, rs->nc, batch)) != PW_OK) {
+    for (rocfft_plan q : {p.c_fwd, p.f_inv, p.f_fwd, p.c_inv})
+      if (q) rocfft_plan_destroy(q);
+    return rc;
+  }
+  size_t need = 0;
+  for (rocfft_plan q : {p.c_fwd, p.f_inv, p.f_fwd, p.c_inv}) {
+    size_t w = 0;
+    PW_FFT(rocfft_plan_get_work_buffer_size(q, &w));
+    need = std::max(need, w);
+  }
+  if (need > rs->work_bytes) {
+    // the old buffer may still be read by queued transforms
+    PW_HIP(hipStreamSynchronize(rs->stream));
+    if (rs->d_work) PW_HIP(hipFree(rs->d_work));
+    rs->d_work = nullptr;
+    PW_HIP(hipMalloc(&rs->d_work, need));
+    rs->work_bytes = need;
+    PW_FFT(rocfft_execution_info_set_work_buffer(rs->info, rs->d_work, rs->work_bytes));
+  }
+  rs->plans[batch] = p;
+  *out = &rs->plans[batch];
+  return PW_OK;
+}
+
+int exec(pw_rs* rs, rocfft_plan plan, const void* in, void* out) {
+  void* ib[1] = {const_cast<void*>(in)};
+  void* ob[1] = {out};
+  PW_FFT(rocfft_execute(plan, ib, ob, rs->info));
+  return PW_OK;
+}
+
+template <typename X>
+int upload_rs(X** d, const X* h, size_t n) {
+  PW_HIP(hipMalloc((void**)d, sizeof(X) * (n > 0 ? n : 1)));
+  if (n) PW_HIP(hipMemcpy(*d, h, sizeof(X) * n, hipMemcpyHostToDevice));
+  return PW_OK;
+}
+
+void free_nonlocal(pw_rs* rs) {
+  hipFree(rs->d_atab);
+  hipFree(rs->d_apt_off);
+  hipFree(rs->d_idx);
+  hipFree(rs->d_val);
+  hipFree(rs->d_wgt);
+  hipFree(rs->d_rowptr);
+  hipFree(rs->d_ent);
+  hipFree(rs->d_coef);
+  rs->d_atab = nullptr;
+  rs->d_apt_off = rs->d_idx = rs->d_rowptr = nullptr;
+  rs->d_val = rs->d_wgt = rs->d_coef = nullptr;
+  rs->d_ent = nullptr;
+  rs->natom = rs->nproj_tot = rs->npmax = 0;
+}
+
+bool even_grid(const int* n) { return n[0] > 0 && n[1] > 0 && n[2] > 0 && n[0] % 2 == 0 && n[1] % 2 == 0 && n[2] % 2 == 0; }
+
+}  // namespace
+
+extern "C" int pw_rs_create(pw_rs** out, const int grid[3], const int grid_fine[3], double vol,
+                            const double* ekin, int max_bands, void* stream) {
+  if (!out || !grid || !grid_fine || !ekin || max_bands <= 0 || !(vol > 0.0))
+    return fail(PW_ERR_ARG, "pw_rs_create: bad argument");
+  *out = nullptr;
+  if (!even_grid(grid) || !even_grid(grid_fine))
+    return fail(PW_ERR_UNSUPPORTED,
+                "pw_rs_create: grids must be even in every dimension (FFT_Number_Type even/power)");
+  for (int d = 0; d < 3; d++)
+    if (grid_fine[d] < grid[d]) return fail(PW_ERR_ARG, "pw_rs_create: fine grid smaller than the coarse grid");
+  std::call_once(g_rocfft_once, [] { g_rocfft_setup = rocfft_setup(); });
+  if (g_rocfft_setup != rocfft_status_success)
+    return fail(PW_ERR_HIP, "rocfft_setup failed: status %d", (int)g_rocfft_setup);
+  pw_rs* rs = new pw_rs();
+  for (int d = 0; d < 3; d++) {
+    rs->nc[d] = grid[d];
+    rs->nf[d] = grid_fine[d];
+  }
+  rs->hc0 = grid[0] / 2 + 1;
+  rs->hf0 = grid_fine[0] / 2 + 1;
+  rs->Nc = (long)grid[0] * grid[1] * grid[2];
+  rs->Nf = (long)grid_fine[0] * grid_fine[1] * grid_fine[2];
+  rs->Hc = (long)rs->hc0 * grid[1] * grid[2];
+  rs->Hf = (long)rs->hf0 * grid_fine[1] * grid_fine[2];
+  if (rs->Nf >= (1L << 31) - 1) {
+    delete rs;
+    return fail(PW_ERR_UNSUPPORTED, "pw_rs_create: fine grid too large");
+  }
+  rs->vol = vol;
+  rs->stream = (hipStream_t)stream;
+  // bands per chunk: chunk buffers within about 4 GiB
+  const double per_band = 16.0 * rs->Hc + 16.0 * rs->Hf + 8.0 * rs->Nf;
+  rs->chunk = (int)std::max(1.0, std::min((double)max_bands, 4294967296.0 / per_band));
+  int rc = PW_OK;
+  auto bail = [&](int code) {
+    pw_rs_destroy(rs);
+    return code;
+  };
+  if (hipGetDevice(&rs->device) != hipSuccess) return bail(fail(PW_ERR_HIP, "pw_rs_create: hipGetDevice failed"));
+  if ((rc = upload_rs(&rs->d_ekin, ekin, (size_t)rs->Hc)) != PW_OK) return bail(rc);
+  if (hipMalloc((void**)&rs->d_xc, sizeof(double2) * rs->Hc * rs->chunk) != hipSuccess ||
+      hipMalloc((void**)&rs->d_ff, sizeof(double2) * rs->Hf * rs->chunk) != hipSuccess ||
+      hipMalloc((void**)&rs->d_pf, sizeof(double) * rs->Nf * rs->chunk) != hipSuccess)
+    return bail(fail(PW_ERR_NOMEM, "pw_rs_create: cannot allocate %d-band chunk buffers", rs->chunk));
+  if (rocfft_execution_info_create(&rs->info) != rocfft_status_success)
+    return bail(fail(PW_ERR_HIP, "pw_rs_create: rocfft_execution_info_create failed"));
+  if (rocfft_execution_info_set_stream(rs->info, rs->stream) != rocfft_status_success)
+    return bail(fail(PW_ERR_HIP, "pw_rs_create: rocfft_execution_info_set_stream failed"));
+  RsPlans* p;
+  if ((rc = get_plans(rs, rs->chunk, &p)) != PW_OK) return bail(rc);
+  *out = rs;
+  return PW_OK;
+}
+
+extern "C" int pw_rs_destroy(pw_rs* rs) {
+  if (!rs) return PW_OK;
+  hipStreamSynchronize(rs->stream);
+  for (auto& kv : rs->plans)
+    for (rocfft_plan q : {kv.second.c_fwd, kv.second.f_inv, kv.second.f_fwd, kv.second.c_inv})
+      if (q) rocfft_plan_destroy(q);
+  if (rs->info) rocfft_execution_info_destroy(rs->info);
+  free_nonlocal(rs);
+  hipFree(rs->d_ekin);
+  hipFree(rs->d_vtot);
+  hipFree(rs->d_xc);
+  hipFree(rs->d_ff);
+  hipFree(rs->d_pf);
+  hipFree(rs->d_work);
+  delete rs;
+  return PW_OK;
+}
+
+extern "C" int pw_rs_set_stream(pw_rs* rs, void* stream) {
+  if (!rs) return fail(PW_ERR_ARG, "pw_rs_set_stream: null handle");
+  PW_HIP(hipStreamSynchronize(rs->stream));
+  rs->stream = (hipStream_t)stream;
+  PW_FFT(rocfft_execution_info_set_stream(rs->info, rs->stream));
+  return PW_OK;
+}
+
+extern "C" int pw_rs_info(const pw_rs* rs, long* ntot, long* ntot_fine, int* chunk) {
+  if (!rs) return fail(PW_ERR_ARG, "pw_rs_info: null handle");
+  if (ntot) *ntot = rs->Nc;
+  if (ntot_fine) *ntot_fine = rs->Nf;
+  if (chunk) *chunk = rs->chunk;
+  return PW_OK;
+}
+
+extern "C" int pw_rs_set_vtot(pw_rs* rs, const double* vtot) {
+  if (!rs || !vtot) return fail(PW_ERR_ARG, "pw_rs_set_vtot: bad argument");
+  PW_HIP(hipStreamSynchronize(rs->stream));
+  if (!rs->d_vtot) PW_HIP(hipMalloc((void**)&rs->d_vtot, sizeof(double) * rs->Nf));
+  PW_HIP(hipMemcpy(rs->d_vtot, vtot, sizeof(double) * rs->Nf, hipMemcpyHostToDevice));
+  return PW_OK;
+}
+
+extern "C" int pw_rs_set_nonlocal(pw_rs* rs, int natom, const int* npts, const int* nproj,
+                                  const int* idx, const double* val, const double* wgt) {
+  if (!rs || natom < 0) return fail(PW_ERR_ARG, "pw_rs_set_nonlocal: bad argument");
+  if (natom > 0 && (!npts || !nproj)) return fail(PW_ERR_ARG, "pw_rs_set_nonlocal: null counts");
+  std::vector<AtomRec> atab((size_t)natom);
+  std::vector<int> apt_off((size_t)natom + 1, 0);
+  long tot_pts = 0, tot_val = 0;
+  int tot_proj = 0, npmax = 0;
+  for (int a = 0; a < natom; a++) {
+    if (npts[a] < 0 || nproj[a] < 0) return fail(PW_ERR_ARG, "pw_rs_set_nonlocal: negative count for atom %d", a);
+    if (nproj[a] > 32)
+      return fail(PW_ERR_UNSUPPORTED, "pw_rs_set_nonlocal: atom %d has %d projectors (at most 32)", a, nproj[a]);
+    atab[a] = AtomRec{npts[a], tot_proj, nproj[a], (int)tot_val};
+    apt_off[a] = (int)tot_pts;
+    tot_pts += npts[a];
+    tot_val += (long)npts[a] * nproj[a];
+    tot_proj += nproj[a];
+    npmax = std::max(npmax, nproj[a]);
+    if (tot_pts >= (1L << 31) || tot_val >= (1L << 31))
+      return fail(PW_ERR_UNSUPPORTED, "pw_rs_set_nonlocal: too many projector entries");
+  }
+  apt_off[natom] = (int)tot_pts;
+  if ((tot_pts > 0 && !idx) || (tot_val > 0 && !val) || (tot_proj > 0 && !wgt))
+    return fail(PW_ERR_ARG, "pw_rs_set_nonlocal: null arrays");
+  for (long i = 0; i < tot_pts; i++)
+    if (idx[i] < 0 || idx[i] >= rs->Nf)
+      return fail(PW_ERR_DOMAIN, "pw_rs_set_nonlocal: grid index %d out of the fine grid", idx[i]);
+  // point-major index of (atom, local point), atoms ascending per point
+  std::vector<int> rowptr((size_t)rs->Nf + 1, 0);
+  for (long i = 0; i < tot_pts; i++) rowptr[(size_t)idx[i] + 1]++;
+  for (long r = 0; r < rs->Nf; r++) rowptr[r + 1] += rowptr[r];
+  std::vector<int2> ent((size_t)tot_pts);
+  {
+    std::vector<int> fill(rowptr.begin(), rowptr.end() - 1);
+    for (int a = 0; a < natom; a++)
+      for (int i = 0; i < npts[a]; i++) {
+        const int r = idx[apt_off[a] + i];
+        ent[(size_t)fill[r]++] = make_int2(a, i);
+      }
+  }
+  PW_HIP(hipStreamSynchronize(rs->stream));
+  free_nonlocal(rs);
+  int rc;
+  if ((rc = upload_rs(&rs->d_atab, atab.data(), atab.size())) != PW_OK ||
+      (rc = upload_rs(&rs->d_apt_off, apt_off.data(), apt_off.size())) != PW_OK ||
+      (rc = upload_rs(&rs->d_idx, idx, (size_t)tot_pts)) != PW_OK ||
+      (rc = upload_rs(&rs->d_val, val, (size_t)tot_val)) != PW_OK ||
+      (rc = upload_rs(&rs->d_wgt, wgt, (size_t)tot_proj)) != PW_OK ||
+      (rc = upload_rs(&rs->d_rowptr, rowptr.data(), rowptr.size())) != PW_OK ||
+      (rc = upload_rs(&rs->d_ent, ent.data(), ent.size())) != PW_OK) {
+    free_nonlocal(rs);
+    return rc;
+  }
+  if (tot_proj > 0 &&
+      hipMalloc((void**)&rs->d_coef, sizeof(double) * (size_t)tot_proj * rs->chunk) != hipSuccess) {
+    free_nonlocal(rs);
+    return fail(PW_ERR_NOMEM, "pw_rs_set_nonlocal: cannot allocate projector weights");
+  }
+  rs->natom = natom;
+  rs->nproj_tot = tot_proj;
+  rs->npmax = npmax;
+  return PW_OK;
+}
+
+extern "C" int pw_rs_mult_spinor(pw_rs* rs, int nb, const double* psi, double* a3) {
+  if (!rs || nb < 0 || (nb > 0 && (!psi || !a3))) return fail(PW_ERR_ARG, "pw_rs_mult_spinor: bad argument");
+  if (!rs->d_vtot) return fail(PW_ERR_STATE, "pw_rs_mult_spinor: local potential not set");
+  int dev;
+  PW_HIP(hipGetDevice(&dev));
+  if (dev != rs->device) return fail(PW_ERR_STATE, "pw_rs_mult_spinor: handle belongs to device %d", rs->device);
+  const int* nc = rs->nc;
+  const int* nf = rs->nf;
+  const bool nl = rs->natom > 0 && rs->nproj_tot > 0;
+  for (int b0 = 0; b0 < nb; b0 += rs->chunk) {
+    const int nbc = std::min(rs->chunk, nb - b0);
+    RsPlans* p;
+    PW_TRY(get_plans(rs, nbc, &p));
+    PW_TRY(exec(rs, p->c_fwd, psi + (long)b0 * rs->Nc, rs->d_xc));
+    hipLaunchKernelGGL(k_rs_c2f, dim3(nf[1] * nf[2], nbc), dim3(kRowThreads), 0, rs->stream, rs->d_xc,
+                       rs->d_ff, nc[0], nc[1], nc[2], nf[1], nf[2], rs->hc0, rs->hf0, rs->Hc, rs->Hf,
+                       1.0 / (double)rs->Nc);
+    PW_HIP(hipGetLastError());
+    PW_TRY(exec(rs, p->f_inv, rs->d_ff, rs->d_pf));
+    if (nl) {
+      const double scale = rs->vol / (double)rs->Nf;
+      if (rs->npmax <= 8)
+        hipLaunchKernelGGL((k_rs_weights<8, 8>), dim3(rs->natom, (nbc + 7) / 8), dim3(256), 0, rs->stream,
+                           rs->d_pf, rs->d_atab, rs->d_apt_off, rs->d_idx, rs->d_val, rs->d_wgt, rs->d_coef,
+                           nbc, rs->Nf, scale);
+      else if (rs->npmax <= 16)
+        hipLaunchKernelGGL((k_rs_weights<16, 4>), dim3(rs->natom, (nbc + 3) / 4), dim3(256), 0, rs->stream,
+                           rs->d_pf, rs->d_atab, rs->d_apt_off, rs->d_idx, rs->d_val, rs->d_wgt, rs->d_coef,
+                           nbc, rs->Nf, scale);
+      else
+        hipLaunchKernelGGL((k_rs_weights<32, 2>), dim3(rs->natom, (nbc + 1) / 2), dim3(256), 0, rs->stream,
+                           rs->d_pf, rs->d_atab, rs->d_apt_off, rs->d_idx, rs->d_val, rs->d_wgt, rs->d_coef,
+                           nbc, rs->Nf, scale);
+      PW_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_rs_local, dim3((unsigned)((rs->Nf + 255) / 256)), dim3(256), 0, rs->stream, rs->d_pf,
+                       rs->d_vtot, rs->d_rowptr, rs->d_ent, rs->d_atab, rs->d_val, rs->d_coef, nbc, rs->Nf,
+                       nl ? 1 : 0);
+    PW_HIP(hipGetLastError());
+    PW_TRY(exec(rs, p->f_fwd, rs->d_pf, rs->d_ff));
+    hipLaunchKernelGGL(k_rs_f2c_ekin, dim3(nc[1] * nc[2], nbc), dim3(kRowThreads), 0, rs->stream, rs->d_xc,
+                       rs->d_ff, rs->d_ekin, nc[0], nc[1], nc[2], nf[1], nf[2], rs->hc0, rs->hf0, rs->Hc,
+                       rs->Hf, 1.0 / (double)rs->Nc, 1.0 / (double)rs->Nf);
+    PW_HIP(hipGetLastError());
+    PW_TRY(exec(rs, p->c_inv, rs->d_xc, a3 + (long)b0 * rs->Nc));
+  }
+  return PW_OK;
+}
