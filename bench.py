@@ -2,7 +2,7 @@
 MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload si64_60]
-                    [--no-cpu] [--copy-probe]
+                    [--no-cpu] [--copy-probe] [--realspace | --realspace-only]
 
 Workload si64_60: 64-atom diamond supercell, 60^3 grid, Ecut 10 Ha,
 npw 6567, 256 bands, 256 projectors; degree-10 Chebyshev filter with bounds
@@ -61,6 +61,65 @@ def pmc_traffic(path, kernel):
     return None, None
 
 
+def realspace_bench(args):
+    """Real-space dual-grid H*psi (pw_rs_mult_spinor) on the SiC_large deck's
+    shape: 62^3 coarse / 124^3 fine grid, 216 atoms x 8 projectors, 442
+    bands resident in HBM.  A step is one H*psi over all bands.  The roofline
+    line compares the whole apply with a streaming model (every stage reads
+    and writes its arrays once)."""
+    import numpy as np
+    import torch
+    from pwdft_develop_amd import realspace as R
+    prob = R.sic216_rs().build()
+    nb = prob.nb
+    ctx = R.RealSpaceContext(prob)
+    x = torch.from_numpy(prob.psi).to("cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(max(args.warmup, 1)):
+        ctx.mult_spinor(x, y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    steps = max(args.steps, 1)
+    for _ in range(steps):
+        ctx.mult_spinor(x, y)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    Nc, Nf = prob.ntot, prob.ntot_fine
+    Hc = (prob.grid[0] // 2 + 1) * prob.grid[1] * prob.grid[2]
+    Hf = (prob.grid_fine[0] // 2 + 1) * prob.grid_fine[1] * prob.grid_fine[2]
+    spts = int(prob.npts.sum())
+    # psi in, a3 out, two coarse and two fine transforms (each one read and
+    # one write), c2f / f2c, V multiply and the projector gather / update
+    model = (8 * Nc * 2 + 2 * (8 * Nc + 16 * Hc) + 2 * (8 * Nf + 16 * Hf) + (16 * Hc + 16 * Hf)
+             + 48 * Hc + 16 * Nf + 8 * spts + 16 * spts)
+    achieved = model * nb / t / 1e9
+    res = {
+        "metric": "real-space dual-grid H*psi band-applications/s",
+        "value": round(nb / t, 1), "unit": "band-applications/s", "ms_per_step": round(t * 1e3, 3),
+        "steps": steps, "dtype": "f64",
+        "config": {"workload": prob.name, "grid": list(prob.grid), "grid_fine": list(prob.grid_fine),
+                   "nbands": nb, "natom": len(prob.atoms), "nproj": int(prob.nproj.sum()),
+                   "projector_points": spts, "bands_per_batch": ctx.info()["chunk"]},
+        "roofline": {"bound": "hbm", "kernel": "whole apply (streaming model)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "model_bytes_per_band": model},
+    }
+    ctx.close()
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        t0 = time.perf_counter()
+        O.rs_mult_spinor(prob.grid, prob.grid_fine, prob.volume, prob.ekin, prob.vtot, prob.npts, prob.nproj,
+                         prob.idx, prob.val, prob.wgt, prob.psi[:2])
+        tc = (time.perf_counter() - t0) / 2
+        res["cpu_baseline"] = {"value": round(1.0 / tc, 3), "unit": "band-applications/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"2 bands of {prob.name} through the C oracle, one thread: "
+                                         f"{tc:.2f} s per band"}
+    return res
+
+
 def host_cores():
     try:
         n = len(os.sched_getaffinity(0))
@@ -77,6 +136,10 @@ def main():
     ap.add_argument("--workload", default="si64_60")
     ap.add_argument("--hpsi-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--realspace", action="store_true",
+                    help="also time the real-space dual-grid H*psi on the SiC_large shape (sic216_rs)")
+    ap.add_argument("--realspace-only", action="store_true",
+                    help="time only the real-space dual-grid H*psi and print its line")
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with 3 device copies of 256 MiB (the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -90,6 +153,10 @@ def main():
     import torch
     from pwdft_develop_amd import workload as W
     from pwdft_develop_amd.ops import Context
+
+    if args.realspace_only:
+        print(json.dumps({"realspace": realspace_bench(args)}))
+        return
 
     prob = W.WORKLOADS[args.workload]().build()
     nb = prob.nb
@@ -219,6 +286,8 @@ def main():
         out["ritz_err_ha"]["fp64_gpu_vs_oracle_max"] = float(np.abs(e_f64 - e_o).max())
         out["ritz_err_ha"]["mixed_vs_oracle_wanted_max"] = float(np.abs(e_mixed - e_o)[:nwant].max())
     ctx.close()
+    if args.realspace:
+        out["realspace"] = realspace_bench(args)
     if args.copy_probe:
         # plain device copies of a known size: the scale for FETCH_SIZE / WRITE_SIZE
         src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda:0")

@@ -11,8 +11,10 @@
 //      included) as a gather, so no memset and no scatter conflicts
 //   3. fine C2R                                            (rocFFT)
 //   4. k_rs_weights: w[p, b] = vol/Nf * wgt_p * sum_i val_p[i] psi_b[idx[i]]
-//   5. k_rs_local: u = V psi + sum over the projectors touching each point,
-//      in place, through a point-major index of the projector entries
+//   5. k_rs_vmul (u = V psi, in place), then k_rs_nl_apply once per colour
+//      of a colouring of the atoms by shared points (u += projector terms);
+//      k_rs_local does both through a point-major index of the projector
+//      entries when an atom lists a point twice
 //   6. fine R2C                                            (rocFFT)
 //   7. k_rs_f2c_ekin: ekin .* coarse spectrum + restriction of the fine one
 //   8. coarse C2R straight into a3                         (rocFFT)
@@ -74,6 +76,11 @@ struct pw_rs {
   double* d_wgt = nullptr;
   int* d_rowptr = nullptr;    // Nf + 1
   int2* d_ent = nullptr;      // (atom, local point) sorted by point, then atom
+  // atoms in colour order: no two atoms of one colour share a grid point
+  // and no atom lists a point twice (else ncolor = 0 and k_rs_local runs)
+  int ncolor = 0;
+  std::vector<int> color_off;  // ncolor + 1 offsets into d_clist
+  int* d_clist = nullptr;
   // chunk buffers
   double2* d_xc = nullptr;
   double2* d_ff = nullptr;
@@ -247,7 +254,8 @@ __global__ __launch_bounds__(256) void k_rs_weights(const double* __restrict__ p
   }
 }
 
-// Step 5: pf <- vtot .* pf + sum over the (atom, point) entries of each grid
+// Step 5 (general form, used when some atom lists a grid point twice): pf <-
+// vtot .* pf + sum over the (atom, point) entries of each grid
 // point of val_p * coef[p, b], in the reference's order (local term first,
 // then atom by atom, projector by projector; spinor.cpp:437-467).  One thread
 // per fine point, bands in groups of 8.
@@ -284,6 +292,83 @@ __global__ __launch_bounds__(256) void k_rs_local(double* __restrict__ pf,
 #pragma unroll
     for (int q = 0; q < G; q++)
       if (q < nbb) pf[(long)(b0 + q) * Nf + r] = u[q];
+  }
+}
+
+// Step 5, split for speed when the atoms colour cleanly: pf <- vtot .* pf
+// over the whole fine grid (two points and eight bands per thread) ...
+__global__ __launch_bounds__(256) void k_rs_vmul(double* __restrict__ pf, const double* __restrict__ vtot,
+                                                int nbc, long Nf) {
+  const long r = 2 * ((long)blockIdx.x * 256 + threadIdx.x);
+  if (r >= Nf) return;
+  const int b0 = blockIdx.y * 8;
+  const int nbb = min(8, nbc - b0);
+  if (r + 1 < Nf) {
+    const double2 v = *reinterpret_cast<const double2*>(vtot + r);
+    double2 x[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      if (q < nbb) x[q] = *reinterpret_cast<const double2*>(pf + (long)(b0 + q) * Nf + r);
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      if (q < nbb) {
+        x[q].x *= v.x;
+        x[q].y *= v.y;
+        *reinterpret_cast<double2*>(pf + (long)(b0 + q) * Nf + r) = x[q];
+      }
+  } else {
+    for (int q = 0; q < nbb; q++) pf[(long)(b0 + q) * Nf + r] *= vtot[r];
+  }
+}
+
+// ... then, one launch per colour, each atom's projectors added at its own
+// points: pf[b, idx_i] += sum_p val[i, p] coef[p, b].  Atoms of one colour
+// share no point, so the read-modify-writes never meet.  Grid (atoms of the
+// colour, groups of 32 bands); the atom's coefficient block sits in LDS.
+template <int NPMAX>
+__global__ __launch_bounds__(256) void k_rs_nl_apply(double* __restrict__ pf, const int* __restrict__ clist,
+                                                    const AtomRec* __restrict__ atab,
+                                                    const int* __restrict__ apt_off,
+                                                    const int* __restrict__ idx,
+                                                    const double* __restrict__ val,
+                                                    const double* __restrict__ coef, int nbc, long Nf) {
+  constexpr int BG = 32;
+  __shared__ double cl[NPMAX * BG];
+  const int a = clist[blockIdx.x];
+  const AtomRec rec = atab[a];
+  const int b0 = blockIdx.y * BG;
+  const int nbb = min(BG, nbc - b0);
+  for (int t = threadIdx.x; t < rec.np * BG; t += 256) {
+    const int p = t / BG, q = t % BG;
+    cl[t] = (q < nbb) ? coef[(long)(rec.pg0 + p) * nbc + b0 + q] : 0.0;
+  }
+  __syncthreads();
+  const int* ia = idx + apt_off[a];
+  const double* va = val + rec.vo;
+  for (int i = threadIdx.x; i < rec.npts; i += 256) {
+    const long r = ia[i];
+    double v[NPMAX];
+#pragma unroll
+    for (int p = 0; p < NPMAX; p++) v[p] = (p < rec.np) ? va[(long)p * rec.npts + i] : 0.0;
+    // eight bands at a time: all loads issued before the first store (a
+    // store may alias a later load of the same pointer, so a load-add-store
+    // loop would run one round trip at a time)
+    for (int q0 = 0; q0 < nbb; q0 += 8) {
+      double x[8];
+#pragma unroll
+      for (int q = 0; q < 8; q++) x[q] = (q0 + q < nbb) ? pf[(long)(b0 + q0 + q) * Nf + r] : 0.0;
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        double u = 0.0;
+#pragma unroll
+        for (int p = 0; p < NPMAX; p++)
+          if (p < rec.np) u = fma(v[p], cl[p * BG + q0 + q], u);
+        x[q] += u;
+      }
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        if (q0 + q < nbb) pf[(long)(b0 + q0 + q) * Nf + r] = x[q];
+    }
   }
 }
 
@@ -379,11 +464,15 @@ void free_nonlocal(pw_rs* rs) {
   hipFree(rs->d_wgt);
   hipFree(rs->d_rowptr);
   hipFree(rs->d_ent);
+  hipFree(rs->d_clist);
   hipFree(rs->d_coef);
   rs->d_atab = nullptr;
   rs->d_apt_off = rs->d_idx = rs->d_rowptr = nullptr;
   rs->d_val = rs->d_wgt = rs->d_coef = nullptr;
   rs->d_ent = nullptr;
+  rs->d_clist = nullptr;
+  rs->ncolor = 0;
+  rs->color_off.clear();
   rs->natom = rs->nproj_tot = rs->npmax = 0;
 }
 
@@ -527,6 +616,40 @@ extern "C" int pw_rs_set_nonlocal(pw_rs* rs, int natom, const int* npts, const i
         ent[(size_t)fill[r]++] = make_int2(a, i);
       }
   }
+  // greedy colouring of the atoms by shared grid points (bit c of mask[r]:
+  // an atom of colour c owns r); an atom that lists a point twice, or more
+  // than 32 colours, leaves the point-major kernel in charge
+  int ncolor = 0;
+  std::vector<int> color_of((size_t)natom, 0);
+  {
+    std::vector<uint32_t> mask((size_t)rs->Nf, 0u);
+    std::vector<int> stamp((size_t)rs->Nf, -1);
+    bool ok = true;
+    for (int a = 0; a < natom && ok; a++) {
+      uint32_t used = 0;
+      for (int i = 0; i < npts[a]; i++) {
+        const int r = idx[apt_off[a] + i];
+        if (stamp[r] == a) { ok = false; break; }
+        stamp[r] = a;
+        used |= mask[r];
+      }
+      if (!ok || used == 0xffffffffu) { ok = false; break; }
+      const int c = __builtin_ctz(~used);
+      color_of[a] = c;
+      ncolor = std::max(ncolor, c + 1);
+      for (int i = 0; i < npts[a]; i++) mask[idx[apt_off[a] + i]] |= 1u << c;
+    }
+    if (!ok) ncolor = 0;
+  }
+  std::vector<int> clist, color_off;
+  if (ncolor > 0) {
+    color_off.assign((size_t)ncolor + 1, 0);
+    for (int a = 0; a < natom; a++) color_off[color_of[a] + 1]++;
+    for (int c = 0; c < ncolor; c++) color_off[c + 1] += color_off[c];
+    clist.resize((size_t)natom);
+    std::vector<int> fill(color_off.begin(), color_off.end() - 1);
+    for (int a = 0; a < natom; a++) clist[(size_t)fill[color_of[a]]++] = a;
+  }
   PW_HIP(hipStreamSynchronize(rs->stream));
   free_nonlocal(rs);
   int rc;
@@ -536,7 +659,8 @@ extern "C" int pw_rs_set_nonlocal(pw_rs* rs, int natom, const int* npts, const i
       (rc = upload_rs(&rs->d_val, val, (size_t)tot_val)) != PW_OK ||
       (rc = upload_rs(&rs->d_wgt, wgt, (size_t)tot_proj)) != PW_OK ||
       (rc = upload_rs(&rs->d_rowptr, rowptr.data(), rowptr.size())) != PW_OK ||
-      (rc = upload_rs(&rs->d_ent, ent.data(), ent.size())) != PW_OK) {
+      (rc = upload_rs(&rs->d_ent, ent.data(), ent.size())) != PW_OK ||
+      (rc = upload_rs(&rs->d_clist, clist.data(), clist.size())) != PW_OK) {
     free_nonlocal(rs);
     return rc;
   }
@@ -548,6 +672,8 @@ extern "C" int pw_rs_set_nonlocal(pw_rs* rs, int natom, const int* npts, const i
   rs->natom = natom;
   rs->nproj_tot = tot_proj;
   rs->npmax = npmax;
+  rs->ncolor = ncolor;
+  rs->color_off = color_off;
   return PW_OK;
 }
 
@@ -586,10 +712,32 @@ extern "C" int pw_rs_mult_spinor(pw_rs* rs, int nb, const double* psi, double* a
                            nbc, rs->Nf, scale);
       PW_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_rs_local, dim3((unsigned)((rs->Nf + 255) / 256)), dim3(256), 0, rs->stream, rs->d_pf,
-                       rs->d_vtot, rs->d_rowptr, rs->d_ent, rs->d_atab, rs->d_val, rs->d_coef, nbc, rs->Nf,
-                       nl ? 1 : 0);
-    PW_HIP(hipGetLastError());
+    if (!nl || rs->ncolor > 0) {
+      // Nf is even (even grids), so every band row starts 16-byte aligned
+      const unsigned nblk = (unsigned)((rs->Nf / 2 + 255) / 256);
+      hipLaunchKernelGGL(k_rs_vmul, dim3(nblk, (nbc + 7) / 8), dim3(256), 0, rs->stream, rs->d_pf, rs->d_vtot,
+                         nbc, rs->Nf);
+      PW_HIP(hipGetLastError());
+      for (int c = 0; nl && c < rs->ncolor; c++) {
+        const int na = rs->color_off[c + 1] - rs->color_off[c];
+        const int* cl = rs->d_clist + rs->color_off[c];
+        const dim3 g(na, (nbc + 31) / 32);
+        if (rs->npmax <= 8)
+          hipLaunchKernelGGL(k_rs_nl_apply<8>, g, dim3(256), 0, rs->stream, rs->d_pf, cl, rs->d_atab, rs->d_apt_off,
+                             rs->d_idx, rs->d_val, rs->d_coef, nbc, rs->Nf);
+        else if (rs->npmax <= 16)
+          hipLaunchKernelGGL(k_rs_nl_apply<16>, g, dim3(256), 0, rs->stream, rs->d_pf, cl, rs->d_atab,
+                             rs->d_apt_off, rs->d_idx, rs->d_val, rs->d_coef, nbc, rs->Nf);
+        else
+          hipLaunchKernelGGL(k_rs_nl_apply<32>, g, dim3(256), 0, rs->stream, rs->d_pf, cl, rs->d_atab,
+                             rs->d_apt_off, rs->d_idx, rs->d_val, rs->d_coef, nbc, rs->Nf);
+        PW_HIP(hipGetLastError());
+      }
+    } else {
+      hipLaunchKernelGGL(k_rs_local, dim3((unsigned)((rs->Nf + 255) / 256)), dim3(256), 0, rs->stream, rs->d_pf,
+                         rs->d_vtot, rs->d_rowptr, rs->d_ent, rs->d_atab, rs->d_val, rs->d_coef, nbc, rs->Nf, 1);
+      PW_HIP(hipGetLastError());
+    }
     PW_TRY(exec(rs, p->f_fwd, rs->d_pf, rs->d_ff));
     hipLaunchKernelGGL(k_rs_f2c_ekin, dim3(nc[1] * nc[2], nbc), dim3(kRowThreads), 0, rs->stream, rs->d_xc,
                        rs->d_ff, rs->d_ekin, nc[0], nc[1], nc[2], nf[1], nf[2], rs->hc0, rs->hf0, rs->Hc,

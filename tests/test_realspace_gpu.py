@@ -73,6 +73,33 @@ def test_rs_local_terms_only():
     assert _rel(got, ref) < TOL
 
 
+def test_rs_repeated_points_in_one_atom():
+    """An atom whose sphere lists grid points twice (a cell smaller than the
+    projector box, periodtable.cpp:1063-1101 wraps without a check): the
+    point-major kernel takes over from the colour-batched one."""
+    p = _prob("box_rs", 5)
+    offs = np.concatenate([[0], np.cumsum(p.npts)])
+    voffs = np.concatenate([[0], np.cumsum(p.npts.astype(np.int64) * p.nproj)])
+    idx, val, npts = [], [], p.npts.copy()
+    for a in range(len(p.npts)):
+        ia = p.idx[offs[a]:offs[a + 1]]
+        va = p.val[voffs[a]:voffs[a + 1]].reshape(p.npts[a], p.nproj[a], order="F")
+        if a == 0:
+            ia = np.concatenate([ia, ia[:5]])
+            va = np.vstack([va, 0.5 * va[:5]])
+            npts[a] += 5
+        idx.append(ia)
+        val.append(va.reshape(-1, order="F"))
+    idx, val = np.concatenate(idx).astype(np.int32), np.concatenate(val)
+    ref = O.rs_mult_spinor(p.grid, p.grid_fine, p.volume, p.ekin, p.vtot, npts, p.nproj, idx, val, p.wgt, p.psi)
+    import torch
+    ctx = R.RealSpaceContext(p, nonlocal_=False)
+    ctx.set_nonlocal(npts, p.nproj, idx, val, p.wgt)
+    got = ctx.mult_spinor(torch.from_numpy(p.psi).to("cuda:0")).cpu().numpy()
+    ctx.close()
+    assert _rel(got, ref) < TOL
+
+
 def test_rs_sic_large_one_band_matches_oracle():
     print("case sic216_rs one band", flush=True)
     p = _prob("sic216_rs", 2)
