@@ -29,6 +29,7 @@
 #include <map>
 #include <mutex>
 
+#include "pw_fft.h"
 #include "pw_internal.h"
 
 using namespace pw;
@@ -88,6 +89,20 @@ struct pw_rs {
   double* d_coef = nullptr;
   void* d_work = nullptr;
   size_t work_bytes = 0;
+  // own dual-grid transforms (every length factors into 2, 3, 5, 7 ... 31)
+  bool own_fft = false;
+  int ny = 0, nz = 0;           // live fine rows / planes
+  double2* d_tz = nullptr;      // [z][l][i]  f2 x ny x hc
+  double2* d_tf = nullptr;      // [z][y][i]  f2 x f1 x hc
+  double2* d_u0 = nullptr;      // [m][l][i]  nz x ny x hc
+  double2* d_cv = nullptr;      // coarse [z][y][i] scratch
+  double2* d_cw = nullptr;
+  int* d_ymap = nullptr;        // fine y -> live row or -1 (f1)
+  int* d_zmap = nullptr;        // fine z -> live plane or -1 (f2)
+  int* d_lmap = nullptr;        // live row -> fine y (ny)
+  int* d_mmap = nullptr;        // live plane -> fine z (nz)
+  double* d_cs = nullptr;       // prime-radix cos / sin table
+  std::map<int, double2*> d_tw; // exp(+2 pi i m / n), per length
   rocfft_execution_info info = nullptr;
   std::map<int, RsPlans> plans;
 };
@@ -372,6 +387,382 @@ __global__ __launch_bounds__(256) void k_rs_nl_apply(double* __restrict__ pf, co
   }
 }
 
+// ------------------------------------------------- dual-grid transforms
+//
+// Batched 1D transforms in LDS along one axis of a [band][a2][a1][a0]
+// array, P pencils per workgroup (row stride n + 1), radix 4, 2, 3, 5 from
+// pw_fft.h and unrolled odd-prime radices 7..31 (the decks' 62 = 2*31 and
+// 124 = 4*31 points).  The fine transforms are pruned to the coarse band:
+// on the way in only the kx <= Nc0/2 half-columns, the coarse rows (plus the
+// Nyquist duplicate row) and planes are nonzero; on the way out only those
+// are kept.  Live fine rows l = 0..ny-1 are the coarse rows j = l (fine
+// index j or Nf1 - Nc1 + j) and, when Nf1 > Nc1, l = Nc1 for cjF = Nf1 -
+// Nc1/2; live planes likewise.
+
+__host__ __device__ constexpr int cs_off(int R) {
+  return R == 7 ? 0 : R == 11 ? 14 : R == 13 ? 36 : R == 17 ? 62 : R == 19 ? 96 : R == 23 ? 134 : R == 29 ? 180 : 238;
+}
+constexpr int kCsLen = 300;   // cos, sin of 2 pi m / R, m < R, for the eight primes
+
+// DFT of odd prime length R on twiddled inputs v (destroyed), sign sg,
+// outputs stored at d[q * ns]: sums and differences of the mirrored inputs
+// halve the multiplies (X[q] and X[R-q] share cos and sin sums).  cs holds
+// cos, sin of 2 pi m / R; the indices are compile-time constants, so the
+// table is read through scalar loads.
+template <int R>
+__device__ __forceinline__ void rs_dft_prime(double2* v, double sg, const double* __restrict__ cs, double2* d,
+                                             int ns) {
+  constexpr int h = (R - 1) / 2;
+  const double2 x0 = v[0];
+  double2 sum = v[0];
+#pragma unroll
+  for (int r = 1; r <= h; r++) {
+    const double2 a = v[r], b = v[R - r];
+    v[r] = make_double2(a.x + b.x, a.y + b.y);
+    v[R - r] = make_double2(a.x - b.x, a.y - b.y);
+    sum.x += v[r].x;
+    sum.y += v[r].y;
+  }
+  d[0] = sum;
+#pragma unroll
+  for (int q = 1; q <= h; q++) {
+    double ax = x0.x, ay = x0.y, bx = 0.0, by = 0.0;
+#pragma unroll
+    for (int r = 1; r <= h; r++) {
+      const int m = (r * q) % R;
+      const double c = cs[2 * m], sn = cs[2 * m + 1];
+      ax = fma(c, v[r].x, ax);
+      ay = fma(c, v[r].y, ay);
+      bx = fma(sn, v[R - r].x, bx);
+      by = fma(sn, v[R - r].y, by);
+    }
+    d[q * ns] = make_double2(ax - sg * by, ay + sg * bx);
+    d[(R - q) * ns] = make_double2(ax + sg * by, ay - sg * bx);
+  }
+}
+
+// One Stockham pass of radix R over P pencils of length n (row stride ld).
+template <int R>
+__device__ __forceinline__ void rs_pass(const double2* __restrict__ src, double2* __restrict__ dst, int n, int P,
+                                        int ld, int Ns, const double2* tw, double sg,
+                                        const double* __restrict__ cs) {
+  const int m = n / R;
+  const int tstep = n / (Ns * R);
+  const int total = P * m;
+  for (int t = threadIdx.x; t < total; t += blockDim.x) {
+    const int p = t / m;
+    const int j = t - p * m;
+    const int k = j % Ns;
+    const double2* s = src + p * ld + j;
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) v[r] = s[r * m];
+#pragma unroll
+    for (int r = 1; r < R; r++) {
+      double2 w = tw[r * k * tstep];
+      w.y *= sg;
+      v[r] = cmul(v[r], w);
+    }
+    double2* d = dst + p * ld + (j / Ns) * Ns * R + k;
+    if constexpr (R <= 5) {
+      dft<double, R>(v, sg);
+#pragma unroll
+      for (int r = 0; r < R; r++) d[r * Ns] = v[r];
+    } else {
+      rs_dft_prime<R>(v, sg, cs + cs_off(R), d, Ns);
+    }
+  }
+}
+
+// Full transform (ping-pong a <-> b) with radices 4, 2, 3, 5 and the odd
+// prime PR (0: none); returns the buffer holding the result.  Ends with a
+// barrier.  One prime per instantiation keeps each kernel's registers to
+// what its largest butterfly needs.
+template <int PR>
+__device__ __forceinline__ double2* lds_fft_any(double2* a, double2* b, int n, int P, int ld, const double2* tw,
+                                                double sg, const double* __restrict__ cs) {
+  double2* src = a;
+  double2* dst = b;
+  int Ns = 1, rem = n;
+  while (rem > 1) {
+    int R;
+    if (rem % 4 == 0) {
+      R = 4;
+      rs_pass<4>(src, dst, n, P, ld, Ns, tw, sg, cs);
+    } else if (rem % 2 == 0) {
+      R = 2;
+      rs_pass<2>(src, dst, n, P, ld, Ns, tw, sg, cs);
+    } else if (rem % 3 == 0) {
+      R = 3;
+      rs_pass<3>(src, dst, n, P, ld, Ns, tw, sg, cs);
+    } else if (rem % 5 == 0) {
+      R = 5;
+      rs_pass<5>(src, dst, n, P, ld, Ns, tw, sg, cs);
+    } else {
+      if constexpr (PR > 0) {
+        R = PR;
+        rs_pass<PR>(src, dst, n, P, ld, Ns, tw, sg, cs);
+      } else {
+        R = rem;  // lengths are checked on the host: never reached
+      }
+    }
+    __syncthreads();
+    double2* tmp = src;
+    src = dst;
+    dst = tmp;
+    Ns *= R;
+    rem /= R;
+  }
+  return src;
+}
+
+__device__ __forceinline__ void load_tw(double2* tw, const double2* __restrict__ g, int n) {
+  for (int t = threadIdx.x; t < n; t += blockDim.x) tw[t] = g[t];
+}
+
+// Pass A: inverse z transform of the fine spectrum's live columns, the
+// columns gathered from the coarse spectrum by IP_c2f's rules (utility.cpp:
+// 3150-3212) and scaled by 1/Nc.  Pencil g = l * hc + i; out tz[z][l][i].
+template <int PR>
+__global__ __launch_bounds__(256) void k_rs_zinv_c2f(const double2* __restrict__ xc, double2* __restrict__ tz,
+                                                    const double2* __restrict__ twg,
+                                                    const double* __restrict__ cs, int c0, int c1, int c2,
+                                                    int f2, int hc, int ny, int P, long Hc, long Tzb,
+                                                    double scale) {
+  extern __shared__ double2 sm[];
+  const int n = f2, ld = n + 1;
+  double2* tw = sm;
+  double2* A = sm + n;
+  double2* B = A + P * ld;
+  load_tw(tw, twg, n);
+  const long b = blockIdx.y;
+  const int g0 = blockIdx.x * P;
+  const int npen = ny * hc;
+  const int In = c0 / 2, Jn = c1 / 2, Kn = c2 / 2;
+  for (int t = threadIdx.x; t < P * n; t += blockDim.x) {
+    const int p = t % P, e = t / P, g = g0 + p;
+    double2 v = make_double2(0.0, 0.0);
+    if (g < npen) {
+      const int l = g / hc, i = g - l * hc;
+      const int j = l < c1 ? l : Jn;
+      const int dj = l < c1 ? 0 : 1;
+      int k, dk;
+      if (e <= Kn) { k = e; dk = 0; }
+      else if (e > f2 - Kn) { k = e - f2 + c2; dk = 0; }
+      else if (e == f2 - Kn) { k = Kn; dk = 1; }
+      else { k = 0; dk = -1; }
+      if (dk >= 0) {
+        const bool ni = (i == In), nj = (j == Jn), nk = (k == Kn);
+        double f;
+        if (dj == 0 && dk == 0) f = (ni || nj || nk) ? 0.5 : 1.0;
+        else if (dj == 1 && dk == 0) f = (!ni && !nk) ? 0.5 : 0.0;
+        else if (dj == 0) f = (!ni && !nj) ? 0.5 : 0.0;
+        else f = !ni ? 0.5 : 0.0;
+        if (f != 0.0) {
+          const double2 x = xc[b * Hc + ((long)k * c1 + j) * hc + i];
+          v = make_double2(x.x * (f * scale), x.y * (f * scale));
+        }
+      }
+    }
+    A[p * ld + e] = v;
+  }
+  __syncthreads();
+  const double2* res = lds_fft_any<PR>(A, B, n, P, ld, tw, 1.0, cs);
+  for (int t = threadIdx.x; t < P * n; t += blockDim.x) {
+    const int p = t % P, e = t / P, g = g0 + p;
+    if (g < npen) tz[b * Tzb + (long)e * npen + g] = res[p * ld + e];
+  }
+}
+
+// A complex pass along one axis: pencil g = u * hc + i, element e at
+// in[b*in_bs + u*in_su + in_map[e]*in_se + i] (in_map NULL: e; -1: zero),
+// output element o <- transform element out_map[o] (NULL: o).
+struct PenArgs {
+  const double2* in;
+  long in_bs, in_su, in_se;
+  const int* in_map;
+  double2* out;
+  long out_bs, out_su, out_se;
+  const int* out_map;
+  int nout, U, hc, n, P;
+  double sg;
+  const double2* tw;
+};
+
+template <int PR>
+__global__ __launch_bounds__(256) void k_rs_pencil(PenArgs a, const double* __restrict__ cs) {
+  extern __shared__ double2 sm[];
+  const int n = a.n, ld = n + 1, P = a.P;
+  double2* tw = sm;
+  double2* A = sm + n;
+  double2* B = A + P * ld;
+  load_tw(tw, a.tw, n);
+  const long b = blockIdx.y;
+  const int g0 = blockIdx.x * P;
+  const int npen = a.U * a.hc;
+  for (int t = threadIdx.x; t < P * n; t += blockDim.x) {
+    const int p = t % P, e = t / P, g = g0 + p;
+    double2 v = make_double2(0.0, 0.0);
+    if (g < npen) {
+      const int u = g / a.hc, i = g - u * a.hc;
+      const int s = a.in_map ? a.in_map[e] : e;
+      if (s >= 0) v = a.in[b * a.in_bs + u * a.in_su + s * a.in_se + i];
+    }
+    A[p * ld + e] = v;
+  }
+  __syncthreads();
+  const double2* res = lds_fft_any<PR>(A, B, n, P, ld, tw, a.sg, cs);
+  for (int t = threadIdx.x; t < P * a.nout; t += blockDim.x) {
+    const int p = t % P, o = t / P, g = g0 + p;
+    if (g < npen) {
+      const int u = g / a.hc, i = g - u * a.hc;
+      const int e = a.out_map ? a.out_map[o] : o;
+      a.out[b * a.out_bs + u * a.out_su + o * a.out_se + i] = res[p * ld + e];
+    }
+  }
+}
+
+// Real rows along x, two rows per complex transform (row r0 in the real
+// part, r1 = r0 + 1 in the imaginary part).  C2R: input rows hold hw
+// half-spectrum entries (zero beyond), Hermitian-extended to length n.
+template <int PR>
+__global__ __launch_bounds__(256) void k_rs_x_c2r(const double2* __restrict__ in, double* __restrict__ out,
+                                                 const double2* __restrict__ twg, const double* __restrict__ cs,
+                                                 int n, int hw, int npair, long in_bs, long out_bs, int P) {
+  extern __shared__ double2 sm[];
+  const int ld = n + 1;
+  double2* tw = sm;
+  double2* A = sm + n;
+  double2* B = A + P * ld;
+  load_tw(tw, twg, n);
+  const long b = blockIdx.y;
+  const int g0 = blockIdx.x * P;
+  for (int t = threadIdx.x; t < P * n; t += blockDim.x) {
+    const int e = t % n, p = t / n, g = g0 + p;
+    double2 z = make_double2(0.0, 0.0);
+    if (g < npair) {
+      const double2* r0 = in + b * in_bs + (long)(2 * g) * hw;
+      const double2* r1 = r0 + hw;
+      double2 z1 = make_double2(0.0, 0.0), z2 = z1;
+      if (e < hw) {
+        z1 = r0[e];
+        z2 = r1[e];
+      } else if (2 * e > n && n - e < hw) {
+        const double2 a0 = r0[n - e], a1 = r1[n - e];
+        z1 = make_double2(a0.x, -a0.y);
+        z2 = make_double2(a1.x, -a1.y);
+      }
+      z = make_double2(z1.x - z2.y, z1.y + z2.x);
+    }
+    A[p * ld + e] = z;
+  }
+  __syncthreads();
+  const double2* res = lds_fft_any<PR>(A, B, n, P, ld, tw, 1.0, cs);
+  for (int t = threadIdx.x; t < P * n; t += blockDim.x) {
+    const int e = t % n, p = t / n, g = g0 + p;
+    if (g < npair) {
+      const double2 z = res[p * ld + e];
+      double* o = out + b * out_bs + (long)(2 * g) * n;
+      o[e] = z.x;
+      o[n + e] = z.y;
+    }
+  }
+}
+
+// R2C of real rows along x, two rows per complex transform, keeping the
+// first hw entries of each row's half spectrum.
+template <int PR>
+__global__ __launch_bounds__(256) void k_rs_x_r2c(const double* __restrict__ in, double2* __restrict__ out,
+                                                 const double2* __restrict__ twg, const double* __restrict__ cs,
+                                                 int n, int hw, int npair, long in_bs, long out_bs, int P) {
+  extern __shared__ double2 sm[];
+  const int ld = n + 1;
+  double2* tw = sm;
+  double2* A = sm + n;
+  double2* B = A + P * ld;
+  load_tw(tw, twg, n);
+  const long b = blockIdx.y;
+  const int g0 = blockIdx.x * P;
+  for (int t = threadIdx.x; t < P * n; t += blockDim.x) {
+    const int e = t % n, p = t / n, g = g0 + p;
+    double2 z = make_double2(0.0, 0.0);
+    if (g < npair) {
+      const double* r0 = in + b * in_bs + (long)(2 * g) * n;
+      z = make_double2(r0[e], r0[n + e]);
+    }
+    A[p * ld + e] = z;
+  }
+  __syncthreads();
+  const double2* res = lds_fft_any<PR>(A, B, n, P, ld, tw, -1.0, cs);
+  for (int t = threadIdx.x; t < P * hw; t += blockDim.x) {
+    const int k = t % hw, p = t / hw, g = g0 + p;
+    if (g < npair) {
+      const double2 zk = res[p * ld + k];
+      const double2 zn = res[p * ld + (k == 0 ? 0 : n - k)];
+      double2* o = out + b * out_bs + (long)(2 * g) * hw;
+      o[k] = make_double2(0.5 * (zk.x + zn.x), 0.5 * (zk.y - zn.y));
+      o[hw + k] = make_double2(0.5 * (zk.y + zn.y), -0.5 * (zk.x - zn.x));
+    }
+  }
+}
+
+// Pass G: k_rs_f2c_ekin on the pruned fine spectrum u0[m][l][i] (live
+// planes m, live rows l); ymap / zmap send fine indices to live ones.
+__global__ __launch_bounds__(kRowThreads) void k_rs_f2c_ekin_p(double2* __restrict__ xc,
+                                                              const double2* __restrict__ u0,
+                                                              const double* __restrict__ ekin,
+                                                              const int* __restrict__ ymap,
+                                                              const int* __restrict__ zmap, int c0, int c1,
+                                                              int c2, int f1, int f2, int hc, int ny, long Hc,
+                                                              long U0b, double sc, double sf) {
+  const int row = blockIdx.x;
+  const int j = row % c1, k = row / c1;
+  const long b = blockIdx.y;
+  const int Jn = c1 / 2, Kn = c2 / 2, In = c0 / 2;
+  const int jF = (j <= Jn) ? j : f1 - c1 + j;
+  const int kF = (k <= Kn) ? k : f2 - c2 + k;
+  const int cjF = f1 - Jn, ckF = f2 - Kn;
+  const int mjF = (f1 - jF) % f1, mkF = (f2 - kF) % f2;
+  const bool nj = (j == Jn), nk = (k == Kn);
+  const double2* fb = u0 + b * U0b;
+  auto rowp = [&](int kk, int jj) -> const double2* {
+    const int m = zmap[kk], l = ymap[jj];
+    return (m >= 0 && l >= 0) ? fb + ((long)m * ny + l) * hc : nullptr;
+  };
+  const double2* f1row = rowp(kF, jF);
+  const double2* f2row = nullptr;
+  if (nj && nk) f2row = rowp(ckF, cjF);
+  else if (nj) f2row = rowp(kF, cjF);
+  else if (nk) f2row = rowp(ckF, jF);
+  const double2* g2row;
+  if (nj) g2row = rowp(mkF, jF);
+  else if (nk) g2row = rowp(kF, mjF);
+  else g2row = rowp(mkF, mjF);
+  const double2 zero = make_double2(0.0, 0.0);
+  double2* crow = xc + b * Hc + (long)row * hc;
+  const double* erow = ekin + (long)row * hc;
+  for (int i = threadIdx.x; i < hc; i += kRowThreads) {
+    const double2 f1v = f1row ? f1row[i] : zero;
+    double2 acc;
+    if (i == In) {
+      if (nj && nk) {
+        acc = make_double2(f1v.x, 0.0);
+      } else {
+        const double2 f2v = g2row ? g2row[i] : zero;
+        acc = make_double2(0.5 * (f1v.x + f2v.x), 0.5 * (f1v.y - f2v.y));
+      }
+    } else if (nj || nk) {
+      const double2 f2v = f2row ? f2row[i] : zero;
+      acc = make_double2(0.5 * (f1v.x + f2v.x), 0.5 * (f1v.y + f2v.y));
+    } else {
+      acc = f1v;
+    }
+    const double2 x = crow[i];
+    const double e = erow[i] * sc;
+    crow[i] = make_double2(fma(e, x.x, acc.x * sf), fma(e, x.y, acc.y * sf));
+  }
+}
+
 // ---------------------------------------------------------------- host
 
 namespace {
@@ -476,6 +867,96 @@ void free_nonlocal(pw_rs* rs) {
   rs->natom = rs->nproj_tot = rs->npmax = 0;
 }
 
+// The odd prime >= 7 in n (0 if n is 2, 3, 5-smooth), or -1 if n has a
+// prime factor above 31 or two different ones >= 7.
+int big_prime(int n) {
+  for (int f : {2, 3, 5})
+    while (n % f == 0) n /= f;
+  int pr = 0;
+  for (int f : {7, 11, 13, 17, 19, 23, 29, 31})
+    if (n % f == 0) {
+      if (pr) return -1;
+      pr = f;
+      while (n % f == 0) n /= f;
+    }
+  return n == 1 ? pr : -1;
+}
+
+bool own_length(int n) { return big_prime(n) >= 0; }
+
+#define RS_PR_SWITCH(PRV, CALL)                                                              \
+  switch (PRV) {                                                                             \
+    case 0: CALL(0); break;                                                                  \
+    case 7: CALL(7); break;                                                                  \
+    case 11: CALL(11); break;                                                                \
+    case 13: CALL(13); break;                                                                \
+    case 17: CALL(17); break;                                                                \
+    case 19: CALL(19); break;                                                                \
+    case 23: CALL(23); break;                                                                \
+    case 29: CALL(29); break;                                                                \
+    case 31: CALL(31); break;                                                                \
+    default: return fail(PW_ERR_UNSUPPORTED, "dual-grid transform: length without a kernel"); \
+  }
+
+// pencils per workgroup: two P x (n + 1) buffers and the twiddles in 64 KiB
+int pencils(int n) { return std::max(1, std::min(64, (4096 - n) / (2 * (n + 1)))); }
+
+size_t pencil_lds(int n, int P) { return sizeof(double2) * ((size_t)n + 2 * (size_t)P * (n + 1)); }
+
+void live_map(int c, int f, std::vector<int>& live, std::vector<int>& fmap) {
+  // live index -> fine index, and fine index -> live index (or -1)
+  const int h = c / 2;
+  live.clear();
+  for (int l = 0; l < c; l++) live.push_back(l <= h ? l : f - c + l);
+  if (f > c) live.push_back(f - h);
+  fmap.assign((size_t)f, -1);
+  for (size_t l = 0; l < live.size(); l++) fmap[(size_t)live[l]] = (int)l;
+}
+
+int setup_own_fft(pw_rs* rs) {
+  const int hc = rs->hc0;
+  std::vector<int> ly, yf, lz, zf;
+  live_map(rs->nc[1], rs->nf[1], ly, yf);
+  live_map(rs->nc[2], rs->nf[2], lz, zf);
+  rs->ny = (int)ly.size();
+  rs->nz = (int)lz.size();
+  int rc;
+  if ((rc = upload_rs(&rs->d_ymap, yf.data(), yf.size())) != PW_OK ||
+      (rc = upload_rs(&rs->d_zmap, zf.data(), zf.size())) != PW_OK ||
+      (rc = upload_rs(&rs->d_lmap, ly.data(), ly.size())) != PW_OK ||
+      (rc = upload_rs(&rs->d_mmap, lz.data(), lz.size())) != PW_OK)
+    return rc;
+  std::vector<double> cs(kCsLen);
+  for (int R : {7, 11, 13, 17, 19, 23, 29, 31})
+    for (int m = 0; m < R; m++) {
+      const long double a = 2.0L * 3.14159265358979323846264338327950288L * m / R;
+      cs[cs_off(R) + 2 * m] = (double)cosl(a);
+      cs[cs_off(R) + 2 * m + 1] = (double)sinl(a);
+    }
+  if ((rc = upload_rs(&rs->d_cs, cs.data(), cs.size())) != PW_OK) return rc;
+  for (int n : {rs->nc[0], rs->nc[1], rs->nc[2], rs->nf[0], rs->nf[1], rs->nf[2]}) {
+    if (rs->d_tw.count(n)) continue;
+    std::vector<double2> tw((size_t)n);
+    for (int m = 0; m < n; m++) {
+      const long double a = 2.0L * 3.14159265358979323846264338327950288L * m / n;
+      tw[(size_t)m] = make_double2((double)cosl(a), (double)sinl(a));
+    }
+    double2* d = nullptr;
+    if ((rc = upload_rs(&d, tw.data(), tw.size())) != PW_OK) return rc;
+    rs->d_tw[n] = d;
+  }
+  const size_t ch = (size_t)rs->chunk;
+  const size_t tz = (size_t)rs->nf[2] * rs->ny * hc, tf = (size_t)rs->nf[2] * rs->nf[1] * hc;
+  const size_t u0 = (size_t)rs->nz * rs->ny * hc;
+  if (hipMalloc((void**)&rs->d_tz, sizeof(double2) * tz * ch) != hipSuccess ||
+      hipMalloc((void**)&rs->d_tf, sizeof(double2) * tf * ch) != hipSuccess ||
+      hipMalloc((void**)&rs->d_u0, sizeof(double2) * u0 * ch) != hipSuccess ||
+      hipMalloc((void**)&rs->d_cv, sizeof(double2) * rs->Hc * ch) != hipSuccess ||
+      hipMalloc((void**)&rs->d_cw, sizeof(double2) * rs->Hc * ch) != hipSuccess)
+    return fail(PW_ERR_NOMEM, "pw_rs_create: cannot allocate %d-band transform buffers", rs->chunk);
+  return PW_OK;
+}
+
 bool even_grid(const int* n) { return n[0] > 0 && n[1] > 0 && n[2] > 0 && n[0] % 2 == 0 && n[1] % 2 == 0 && n[2] % 2 == 0; }
 
 }  // namespace
@@ -490,10 +971,22 @@ extern "C" int pw_rs_create(pw_rs** out, const int grid[3], const int grid_fine[
                 "pw_rs_create: grids must be even in every dimension (FFT_Number_Type even/power)");
   for (int d = 0; d < 3; d++)
     if (grid_fine[d] < grid[d]) return fail(PW_ERR_ARG, "pw_rs_create: fine grid smaller than the coarse grid");
-  std::call_once(g_rocfft_once, [] { g_rocfft_setup = rocfft_setup(); });
-  if (g_rocfft_setup != rocfft_status_success)
-    return fail(PW_ERR_HIP, "rocfft_setup failed: status %d", (int)g_rocfft_setup);
   pw_rs* rs = new pw_rs();
+  {
+    // PW_RS_FFT=rocfft forces the library transforms (comparison runs)
+    const char* e = getenv("PW_RS_FFT");
+    bool own = !(e && std::string(e) == "rocfft");
+    for (int d = 0; d < 3; d++)
+      own = own && own_length(grid[d]) && own_length(grid_fine[d]) && grid_fine[d] <= 1024;
+    rs->own_fft = own;
+  }
+  if (!rs->own_fft) {
+    std::call_once(g_rocfft_once, [] { g_rocfft_setup = rocfft_setup(); });
+    if (g_rocfft_setup != rocfft_status_success) {
+      delete rs;
+      return fail(PW_ERR_HIP, "rocfft_setup failed: status %d", (int)g_rocfft_setup);
+    }
+  }
   for (int d = 0; d < 3; d++) {
     rs->nc[d] = grid[d];
     rs->nf[d] = grid_fine[d];
@@ -511,7 +1004,12 @@ extern "C" int pw_rs_create(pw_rs** out, const int grid[3], const int grid_fine[
   rs->vol = vol;
   rs->stream = (hipStream_t)stream;
   // bands per chunk: chunk buffers within about 4 GiB
-  const double per_band = 16.0 * rs->Hc + 16.0 * rs->Hf + 8.0 * rs->Nf;
+  double per_band = 16.0 * rs->Hc + 16.0 * rs->Hf + 8.0 * rs->Nf;
+  if (rs->own_fft) {
+    const double hc = rs->hc0, ny = grid[1] + 1, nz = grid[2] + 1;
+    per_band = 48.0 * rs->Hc + 8.0 * rs->Nf +
+               16.0 * hc * ((double)grid_fine[2] * ny + (double)grid_fine[2] * grid_fine[1] + nz * ny);
+  }
   rs->chunk = (int)std::max(1.0, std::min((double)max_bands, 4294967296.0 / per_band));
   int rc = PW_OK;
   auto bail = [&](int code) {
@@ -521,8 +1019,14 @@ extern "C" int pw_rs_create(pw_rs** out, const int grid[3], const int grid_fine[
   if (hipGetDevice(&rs->device) != hipSuccess) return bail(fail(PW_ERR_HIP, "pw_rs_create: hipGetDevice failed"));
   if ((rc = upload_rs(&rs->d_ekin, ekin, (size_t)rs->Hc)) != PW_OK) return bail(rc);
   if (hipMalloc((void**)&rs->d_xc, sizeof(double2) * rs->Hc * rs->chunk) != hipSuccess ||
-      hipMalloc((void**)&rs->d_ff, sizeof(double2) * rs->Hf * rs->chunk) != hipSuccess ||
       hipMalloc((void**)&rs->d_pf, sizeof(double) * rs->Nf * rs->chunk) != hipSuccess)
+    return bail(fail(PW_ERR_NOMEM, "pw_rs_create: cannot allocate %d-band chunk buffers", rs->chunk));
+  if (rs->own_fft) {
+    if ((rc = setup_own_fft(rs)) != PW_OK) return bail(rc);
+    *out = rs;
+    return PW_OK;
+  }
+  if (hipMalloc((void**)&rs->d_ff, sizeof(double2) * rs->Hf * rs->chunk) != hipSuccess)
     return bail(fail(PW_ERR_NOMEM, "pw_rs_create: cannot allocate %d-band chunk buffers", rs->chunk));
   if (rocfft_execution_info_create(&rs->info) != rocfft_status_success)
     return bail(fail(PW_ERR_HIP, "pw_rs_create: rocfft_execution_info_create failed"));
@@ -548,6 +1052,17 @@ extern "C" int pw_rs_destroy(pw_rs* rs) {
   hipFree(rs->d_ff);
   hipFree(rs->d_pf);
   hipFree(rs->d_work);
+  hipFree(rs->d_tz);
+  hipFree(rs->d_tf);
+  hipFree(rs->d_u0);
+  hipFree(rs->d_cv);
+  hipFree(rs->d_cw);
+  hipFree(rs->d_ymap);
+  hipFree(rs->d_zmap);
+  hipFree(rs->d_lmap);
+  hipFree(rs->d_mmap);
+  hipFree(rs->d_cs);
+  for (auto& kv : rs->d_tw) hipFree(kv.second);
   delete rs;
   return PW_OK;
 }
@@ -556,7 +1071,7 @@ extern "C" int pw_rs_set_stream(pw_rs* rs, void* stream) {
   if (!rs) return fail(PW_ERR_ARG, "pw_rs_set_stream: null handle");
   PW_HIP(hipStreamSynchronize(rs->stream));
   rs->stream = (hipStream_t)stream;
-  PW_FFT(rocfft_execution_info_set_stream(rs->info, rs->stream));
+  if (rs->info) PW_FFT(rocfft_execution_info_set_stream(rs->info, rs->stream));
   return PW_OK;
 }
 
@@ -677,6 +1192,157 @@ extern "C" int pw_rs_set_nonlocal(pw_rs* rs, int natom, const int* npts, const i
   return PW_OK;
 }
 
+namespace {
+
+int launch_pencil(pw_rs* rs, PenArgs a, int nbc) {
+  a.P = pencils(a.n);
+  a.tw = rs->d_tw[a.n];
+  const int npen = a.U * a.hc;
+  const dim3 g((npen + a.P - 1) / a.P, nbc);
+#define RS_LAUNCH(PRV) \
+  hipLaunchKernelGGL(k_rs_pencil<PRV>, g, dim3(256), pencil_lds(a.n, a.P), rs->stream, a, (const double*)rs->d_cs)
+  RS_PR_SWITCH(big_prime(a.n), RS_LAUNCH)
+#undef RS_LAUNCH
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+int launch_x(pw_rs* rs, bool c2r, const void* in, void* out, int n, int hw, int nrows, long in_bs, long out_bs,
+             int nbc) {
+  const int P = pencils(n);
+  const int npair = nrows / 2;
+  const dim3 g((npair + P - 1) / P, nbc);
+  const double2* tw = rs->d_tw[n];
+  const double* cs = rs->d_cs;
+  if (c2r) {
+#define RS_LAUNCH(PRV)                                                                                    \
+  hipLaunchKernelGGL(k_rs_x_c2r<PRV>, g, dim3(256), pencil_lds(n, P), rs->stream, (const double2*)in, \
+                     (double*)out, tw, cs, n, hw, npair, in_bs, out_bs, P)
+    RS_PR_SWITCH(big_prime(n), RS_LAUNCH)
+#undef RS_LAUNCH
+  } else {
+#define RS_LAUNCH(PRV)                                                                                   \
+  hipLaunchKernelGGL(k_rs_x_r2c<PRV>, g, dim3(256), pencil_lds(n, P), rs->stream, (const double*)in, \
+                     (double2*)out, tw, cs, n, hw, npair, in_bs, out_bs, P)
+    RS_PR_SWITCH(big_prime(n), RS_LAUNCH)
+#undef RS_LAUNCH
+  }
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+// coarse psi -> xc (coarse spectrum) -> pf (psi on the fine grid)
+int own_to_fine(pw_rs* rs, const double* psi, int nbc) {
+  const int *nc = rs->nc, *nf = rs->nf;
+  const int hc = rs->hc0;
+  const long Hc = rs->Hc;
+  PW_TRY(launch_x(rs, false, psi, rs->d_cv, nc[0], hc, nc[1] * nc[2], rs->Nc, Hc, nbc));
+  PenArgs y{rs->d_cv, Hc, (long)nc[1] * hc, hc, nullptr, rs->d_cw, Hc, (long)nc[1] * hc, hc, nullptr,
+            nc[1], nc[2], hc, nc[1], 0, -1.0, nullptr};
+  PW_TRY(launch_pencil(rs, y, nbc));
+  PenArgs z{rs->d_cw, Hc, hc, (long)nc[1] * hc, nullptr, rs->d_xc, Hc, hc, (long)nc[1] * hc, nullptr,
+            nc[2], nc[1], hc, nc[2], 0, -1.0, nullptr};
+  PW_TRY(launch_pencil(rs, z, nbc));
+  // fine inverse: z (with the coarse-to-fine gather), y, x
+  const long Tzb = (long)nf[2] * rs->ny * hc, Tfb = (long)nf[2] * nf[1] * hc;
+  {
+    const int P = pencils(nf[2]);
+    const dim3 g((rs->ny * hc + P - 1) / P, nbc);
+    const double2* tw = rs->d_tw[nf[2]];
+    const double* cs = rs->d_cs;
+#define RS_LAUNCH(PRV)                                                                                     \
+  hipLaunchKernelGGL(k_rs_zinv_c2f<PRV>, g, dim3(256), pencil_lds(nf[2], P), rs->stream, rs->d_xc, rs->d_tz, \
+                     tw, cs, nc[0], nc[1], nc[2], nf[2], hc, rs->ny, P, Hc, Tzb, 1.0 / (double)rs->Nc)
+    RS_PR_SWITCH(big_prime(nf[2]), RS_LAUNCH)
+#undef RS_LAUNCH
+    PW_HIP(hipGetLastError());
+  }
+  PenArgs yi{rs->d_tz, Tzb, (long)rs->ny * hc, hc, rs->d_ymap, rs->d_tf, Tfb, (long)nf[1] * hc, hc, nullptr,
+             nf[1], nf[2], hc, nf[1], 0, 1.0, nullptr};
+  PW_TRY(launch_pencil(rs, yi, nbc));
+  PW_TRY(launch_x(rs, true, rs->d_tf, rs->d_pf, nf[0], hc, nf[1] * nf[2], Tfb, rs->Nf, nbc));
+  return PW_OK;
+}
+
+// pf (H psi terms on the fine grid) -> restriction + kinetic -> a3
+int own_to_coarse(pw_rs* rs, double* a3, int nbc) {
+  const int *nc = rs->nc, *nf = rs->nf;
+  const int hc = rs->hc0;
+  const long Hc = rs->Hc;
+  const long Tzb = (long)nf[2] * rs->ny * hc, Tfb = (long)nf[2] * nf[1] * hc;
+  const long U0b = (long)rs->nz * rs->ny * hc;
+  PW_TRY(launch_x(rs, false, rs->d_pf, rs->d_tf, nf[0], hc, nf[1] * nf[2], rs->Nf, Tfb, nbc));
+  PenArgs yf{rs->d_tf, Tfb, (long)nf[1] * hc, hc, nullptr, rs->d_tz, Tzb, (long)rs->ny * hc, hc, rs->d_lmap,
+             rs->ny, nf[2], hc, nf[1], 0, -1.0, nullptr};
+  PW_TRY(launch_pencil(rs, yf, nbc));
+  PenArgs zf{rs->d_tz, Tzb, hc, (long)rs->ny * hc, nullptr, rs->d_u0, U0b, hc, (long)rs->ny * hc, rs->d_mmap,
+             rs->nz, rs->ny, hc, nf[2], 0, -1.0, nullptr};
+  PW_TRY(launch_pencil(rs, zf, nbc));
+  hipLaunchKernelGGL(k_rs_f2c_ekin_p, dim3(nc[1] * nc[2], nbc), dim3(kRowThreads), 0, rs->stream, rs->d_xc,
+                     rs->d_u0, rs->d_ekin, rs->d_ymap, rs->d_zmap, nc[0], nc[1], nc[2], nf[1], nf[2], hc, rs->ny,
+                     Hc, U0b, 1.0 / (double)rs->Nc, 1.0 / (double)rs->Nf);
+  PW_HIP(hipGetLastError());
+  // coarse inverse: z, y, x
+  PenArgs z{rs->d_xc, Hc, hc, (long)nc[1] * hc, nullptr, rs->d_cv, Hc, hc, (long)nc[1] * hc, nullptr,
+            nc[2], nc[1], hc, nc[2], 0, 1.0, nullptr};
+  PW_TRY(launch_pencil(rs, z, nbc));
+  PenArgs y{rs->d_cv, Hc, (long)nc[1] * hc, hc, nullptr, rs->d_cw, Hc, (long)nc[1] * hc, hc, nullptr,
+            nc[1], nc[2], hc, nc[1], 0, 1.0, nullptr};
+  PW_TRY(launch_pencil(rs, y, nbc));
+  PW_TRY(launch_x(rs, true, rs->d_cw, a3, nc[0], hc, nc[1] * nc[2], Hc, rs->Nc, nbc));
+  return PW_OK;
+}
+
+// projector weights, V psi and the projector terms on pf
+int local_terms(pw_rs* rs, int nbc) {
+  const bool nl = rs->natom > 0 && rs->nproj_tot > 0;
+  if (nl) {
+    const double scale = rs->vol / (double)rs->Nf;
+    if (rs->npmax <= 8)
+      hipLaunchKernelGGL((k_rs_weights<8, 8>), dim3(rs->natom, (nbc + 7) / 8), dim3(256), 0, rs->stream,
+                         rs->d_pf, rs->d_atab, rs->d_apt_off, rs->d_idx, rs->d_val, rs->d_wgt, rs->d_coef,
+                         nbc, rs->Nf, scale);
+    else if (rs->npmax <= 16)
+      hipLaunchKernelGGL((k_rs_weights<16, 4>), dim3(rs->natom, (nbc + 3) / 4), dim3(256), 0, rs->stream,
+                         rs->d_pf, rs->d_atab, rs->d_apt_off, rs->d_idx, rs->d_val, rs->d_wgt, rs->d_coef,
+                         nbc, rs->Nf, scale);
+    else
+      hipLaunchKernelGGL((k_rs_weights<32, 2>), dim3(rs->natom, (nbc + 1) / 2), dim3(256), 0, rs->stream,
+                         rs->d_pf, rs->d_atab, rs->d_apt_off, rs->d_idx, rs->d_val, rs->d_wgt, rs->d_coef,
+                         nbc, rs->Nf, scale);
+    PW_HIP(hipGetLastError());
+  }
+  if (!nl || rs->ncolor > 0) {
+    // Nf is even (even grids), so every band row starts 16-byte aligned
+    const unsigned nblk = (unsigned)((rs->Nf / 2 + 255) / 256);
+    hipLaunchKernelGGL(k_rs_vmul, dim3(nblk, (nbc + 7) / 8), dim3(256), 0, rs->stream, rs->d_pf, rs->d_vtot,
+                       nbc, rs->Nf);
+    PW_HIP(hipGetLastError());
+    for (int c = 0; nl && c < rs->ncolor; c++) {
+      const int na = rs->color_off[c + 1] - rs->color_off[c];
+      const int* cl = rs->d_clist + rs->color_off[c];
+      const dim3 g(na, (nbc + 31) / 32);
+      if (rs->npmax <= 8)
+        hipLaunchKernelGGL(k_rs_nl_apply<8>, g, dim3(256), 0, rs->stream, rs->d_pf, cl, rs->d_atab, rs->d_apt_off,
+                           rs->d_idx, rs->d_val, rs->d_coef, nbc, rs->Nf);
+      else if (rs->npmax <= 16)
+        hipLaunchKernelGGL(k_rs_nl_apply<16>, g, dim3(256), 0, rs->stream, rs->d_pf, cl, rs->d_atab,
+                           rs->d_apt_off, rs->d_idx, rs->d_val, rs->d_coef, nbc, rs->Nf);
+      else
+        hipLaunchKernelGGL(k_rs_nl_apply<32>, g, dim3(256), 0, rs->stream, rs->d_pf, cl, rs->d_atab,
+                           rs->d_apt_off, rs->d_idx, rs->d_val, rs->d_coef, nbc, rs->Nf);
+      PW_HIP(hipGetLastError());
+    }
+  } else {
+    hipLaunchKernelGGL(k_rs_local, dim3((unsigned)((rs->Nf + 255) / 256)), dim3(256), 0, rs->stream, rs->d_pf,
+                       rs->d_vtot, rs->d_rowptr, rs->d_ent, rs->d_atab, rs->d_val, rs->d_coef, nbc, rs->Nf, 1);
+    PW_HIP(hipGetLastError());
+  }
+  return PW_OK;
+}
+
+}  // namespace
+
 extern "C" int pw_rs_mult_spinor(pw_rs* rs, int nb, const double* psi, double* a3) {
   if (!rs || nb < 0 || (nb > 0 && (!psi || !a3))) return fail(PW_ERR_ARG, "pw_rs_mult_spinor: bad argument");
   if (!rs->d_vtot) return fail(PW_ERR_STATE, "pw_rs_mult_spinor: local potential not set");
@@ -685,9 +1351,14 @@ extern "C" int pw_rs_mult_spinor(pw_rs* rs, int nb, const double* psi, double* a
   if (dev != rs->device) return fail(PW_ERR_STATE, "pw_rs_mult_spinor: handle belongs to device %d", rs->device);
   const int* nc = rs->nc;
   const int* nf = rs->nf;
-  const bool nl = rs->natom > 0 && rs->nproj_tot > 0;
   for (int b0 = 0; b0 < nb; b0 += rs->chunk) {
     const int nbc = std::min(rs->chunk, nb - b0);
+    if (rs->own_fft) {
+      PW_TRY(own_to_fine(rs, psi + (long)b0 * rs->Nc, nbc));
+      PW_TRY(local_terms(rs, nbc));
+      PW_TRY(own_to_coarse(rs, a3 + (long)b0 * rs->Nc, nbc));
+      continue;
+    }
     RsPlans* p;
     PW_TRY(get_plans(rs, nbc, &p));
     PW_TRY(exec(rs, p->c_fwd, psi + (long)b0 * rs->Nc, rs->d_xc));
@@ -696,48 +1367,7 @@ extern "C" int pw_rs_mult_spinor(pw_rs* rs, int nb, const double* psi, double* a
                        1.0 / (double)rs->Nc);
     PW_HIP(hipGetLastError());
     PW_TRY(exec(rs, p->f_inv, rs->d_ff, rs->d_pf));
-    if (nl) {
-      const double scale = rs->vol / (double)rs->Nf;
-      if (rs->npmax <= 8)
-        hipLaunchKernelGGL((k_rs_weights<8, 8>), dim3(rs->natom, (nbc + 7) / 8), dim3(256), 0, rs->stream,
-                           rs->d_pf, rs->d_atab, rs->d_apt_off, rs->d_idx, rs->d_val, rs->d_wgt, rs->d_coef,
-                           nbc, rs->Nf, scale);
-      else if (rs->npmax <= 16)
-        hipLaunchKernelGGL((k_rs_weights<16, 4>), dim3(rs->natom, (nbc + 3) / 4), dim3(256), 0, rs->stream,
-                           rs->d_pf, rs->d_atab, rs->d_apt_off, rs->d_idx, rs->d_val, rs->d_wgt, rs->d_coef,
-                           nbc, rs->Nf, scale);
-      else
-        hipLaunchKernelGGL((k_rs_weights<32, 2>), dim3(rs->natom, (nbc + 1) / 2), dim3(256), 0, rs->stream,
-                           rs->d_pf, rs->d_atab, rs->d_apt_off, rs->d_idx, rs->d_val, rs->d_wgt, rs->d_coef,
-                           nbc, rs->Nf, scale);
-      PW_HIP(hipGetLastError());
-    }
-    if (!nl || rs->ncolor > 0) {
-      // Nf is even (even grids), so every band row starts 16-byte aligned
-      const unsigned nblk = (unsigned)((rs->Nf / 2 + 255) / 256);
-      hipLaunchKernelGGL(k_rs_vmul, dim3(nblk, (nbc + 7) / 8), dim3(256), 0, rs->stream, rs->d_pf, rs->d_vtot,
-                         nbc, rs->Nf);
-      PW_HIP(hipGetLastError());
-      for (int c = 0; nl && c < rs->ncolor; c++) {
-        const int na = rs->color_off[c + 1] - rs->color_off[c];
-        const int* cl = rs->d_clist + rs->color_off[c];
-        const dim3 g(na, (nbc + 31) / 32);
-        if (rs->npmax <= 8)
-          hipLaunchKernelGGL(k_rs_nl_apply<8>, g, dim3(256), 0, rs->stream, rs->d_pf, cl, rs->d_atab, rs->d_apt_off,
-                             rs->d_idx, rs->d_val, rs->d_coef, nbc, rs->Nf);
-        else if (rs->npmax <= 16)
-          hipLaunchKernelGGL(k_rs_nl_apply<16>, g, dim3(256), 0, rs->stream, rs->d_pf, cl, rs->d_atab,
-                             rs->d_apt_off, rs->d_idx, rs->d_val, rs->d_coef, nbc, rs->Nf);
-        else
-          hipLaunchKernelGGL(k_rs_nl_apply<32>, g, dim3(256), 0, rs->stream, rs->d_pf, cl, rs->d_atab,
-                             rs->d_apt_off, rs->d_idx, rs->d_val, rs->d_coef, nbc, rs->Nf);
-        PW_HIP(hipGetLastError());
-      }
-    } else {
-      hipLaunchKernelGGL(k_rs_local, dim3((unsigned)((rs->Nf + 255) / 256)), dim3(256), 0, rs->stream, rs->d_pf,
-                         rs->d_vtot, rs->d_rowptr, rs->d_ent, rs->d_atab, rs->d_val, rs->d_coef, nbc, rs->Nf, 1);
-      PW_HIP(hipGetLastError());
-    }
+    PW_TRY(local_terms(rs, nbc));
     PW_TRY(exec(rs, p->f_fwd, rs->d_pf, rs->d_ff));
     hipLaunchKernelGGL(k_rs_f2c_ekin, dim3(nc[1] * nc[2], nbc), dim3(kRowThreads), 0, rs->stream, rs->d_xc,
                        rs->d_ff, rs->d_ekin, nc[0], nc[1], nc[2], nf[1], nf[2], rs->hc0, rs->hf0, rs->Hc,

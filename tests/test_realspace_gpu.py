@@ -66,6 +66,17 @@ def test_rs_mult_spinor_matches_oracle(name, nb, max_bands):
     assert _rel(got, ref) < TOL
 
 
+@pytest.mark.parametrize("name,nb", [("box_rs", 4), ("sic8_rs", 3)])
+def test_rs_rocfft_transforms_match_oracle(name, nb, monkeypatch):
+    """The library-FFT variant (taken for lengths with prime factors above
+    31, forced here) agrees with the oracle as the LDS transforms do."""
+    monkeypatch.setenv("PW_RS_FFT", "rocfft")
+    p = _prob(name, nb)
+    ref = _oracle(p, p.psi)
+    got = _gpu(p, p.psi)
+    assert _rel(got, ref) < TOL
+
+
 def test_rs_local_terms_only():
     p = _prob("sic8_rs", 4)
     ref = _oracle(p, p.psi, nonlocal_=False)
