@@ -404,28 +404,17 @@ __host__ __device__ constexpr int cs_off(int R) {
 }
 constexpr int kCsLen = 300;   // cos, sin of 2 pi m / R, m < R, for the eight primes
 
-// DFT of odd prime length R on twiddled inputs v (destroyed), sign sg,
-// outputs stored at d[q * ns]: sums and differences of the mirrored inputs
-// halve the multiplies (X[q] and X[R-q] share cos and sin sums).  cs holds
-// cos, sin of 2 pi m / R; the indices are compile-time constants, so the
-// table is read through scalar loads.
-template <int R>
-__device__ __forceinline__ void rs_dft_prime(double2* v, double sg, const double* __restrict__ cs, double2* d,
-                                             int ns) {
+// Output pairs q = Q0..Q1 of an odd-prime DFT of length R: v[r] holds
+// v_r + v_{R-r} and v[R-r] holds v_r - v_{R-r} (r = 1..(R-1)/2), x0 = v_0.
+// X[q] and X[R-q] share the cos and sin sums, which halves the multiplies.
+// cs holds cos, sin of 2 pi m / R; the indices are compile-time constants,
+// so the table is read through scalar loads.
+template <int R, int Q0, int Q1>
+__device__ __forceinline__ void rs_prime_pairs(const double2* v, double2 x0, double sg,
+                                               const double* __restrict__ cs, double2* d, int ns) {
   constexpr int h = (R - 1) / 2;
-  const double2 x0 = v[0];
-  double2 sum = v[0];
 #pragma unroll
-  for (int r = 1; r <= h; r++) {
-    const double2 a = v[r], b = v[R - r];
-    v[r] = make_double2(a.x + b.x, a.y + b.y);
-    v[R - r] = make_double2(a.x - b.x, a.y - b.y);
-    sum.x += v[r].x;
-    sum.y += v[r].y;
-  }
-  d[0] = sum;
-#pragma unroll
-  for (int q = 1; q <= h; q++) {
+  for (int q = Q0; q <= Q1; q++) {
     double ax = x0.x, ay = x0.y, bx = 0.0, by = 0.0;
 #pragma unroll
     for (int r = 1; r <= h; r++) {
@@ -441,17 +430,23 @@ __device__ __forceinline__ void rs_dft_prime(double2* v, double sg, const double
   }
 }
 
-// One Stockham pass of radix R over P pencils of length n (row stride ld).
+// Stockham pass of odd prime radix R >= 7: every butterfly is shared by the
+// four wavefronts of the (256-thread) workgroup, wavefront w computing the
+// output pairs of its quarter of q = 1..(R-1)/2 (lane = butterfly), so all
+// four wavefronts work even when a workgroup holds only ~64 butterflies.
 template <int R>
-__device__ __forceinline__ void rs_pass(const double2* __restrict__ src, double2* __restrict__ dst, int n, int P,
-                                        int ld, int Ns, const double2* tw, double sg,
-                                        const double* __restrict__ cs) {
+__device__ __forceinline__ void rs_pass_prime(const double2* __restrict__ src, double2* __restrict__ dst, int n,
+                                              int P, int ld, int Ns, const double2* tw, double sg,
+                                              const double* __restrict__ cs) {
+  constexpr int h = (R - 1) / 2;
+  constexpr int QS = (h + 3) / 4;
   const int m = n / R;
   const int tstep = n / (Ns * R);
   const int total = P * m;
-  for (int t = threadIdx.x; t < total; t += blockDim.x) {
-    const int p = t / m;
-    const int j = t - p * m;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  for (int u = threadIdx.x & 63; u < total; u += 64) {
+    const int p = u / m;
+    const int j = u - p * m;
     const int k = j % Ns;
     const double2* s = src + p * ld + j;
     double2 v[R];
@@ -459,17 +454,69 @@ __device__ __forceinline__ void rs_pass(const double2* __restrict__ src, double2
     for (int r = 0; r < R; r++) v[r] = s[r * m];
 #pragma unroll
     for (int r = 1; r < R; r++) {
-      double2 w = tw[r * k * tstep];
-      w.y *= sg;
-      v[r] = cmul(v[r], w);
+      double2 t = tw[r * k * tstep];
+      t.y *= sg;
+      v[r] = cmul(v[r], t);
+    }
+    const double2 x0 = v[0];
+    double2 sum = v[0];
+#pragma unroll
+    for (int r = 1; r <= h; r++) {
+      const double2 a = v[r], b = v[R - r];
+      v[r] = make_double2(a.x + b.x, a.y + b.y);
+      v[R - r] = make_double2(a.x - b.x, a.y - b.y);
+      sum.x += v[r].x;
+      sum.y += v[r].y;
     }
     double2* d = dst + p * ld + (j / Ns) * Ns * R + k;
-    if constexpr (R <= 5) {
+    switch (w) {
+      case 0:
+        d[0] = sum;
+        rs_prime_pairs<R, 1, (QS < h ? QS : h)>(v, x0, sg, cs, d, Ns);
+        break;
+      case 1:
+        rs_prime_pairs<R, QS + 1, (2 * QS < h ? 2 * QS : h)>(v, x0, sg, cs, d, Ns);
+        break;
+      case 2:
+        rs_prime_pairs<R, 2 * QS + 1, (3 * QS < h ? 3 * QS : h)>(v, x0, sg, cs, d, Ns);
+        break;
+      default:
+        rs_prime_pairs<R, 3 * QS + 1, h>(v, x0, sg, cs, d, Ns);
+        break;
+    }
+  }
+}
+
+// One Stockham pass of radix R (2, 3, 4, 5) over P pencils of length n (row
+// stride ld); odd primes go to rs_pass_prime.
+template <int R>
+__device__ __forceinline__ void rs_pass(const double2* __restrict__ src, double2* __restrict__ dst, int n, int P,
+                                        int ld, int Ns, const double2* tw, double sg,
+                                        const double* __restrict__ cs) {
+  if constexpr (R > 5) {
+    rs_pass_prime<R>(src, dst, n, P, ld, Ns, tw, sg, cs + cs_off(R));
+  } else {
+    const int m = n / R;
+    const int tstep = n / (Ns * R);
+    const int total = P * m;
+    for (int t = threadIdx.x; t < total; t += blockDim.x) {
+      const int p = t / m;
+      const int j = t - p * m;
+      const int k = j % Ns;
+      const double2* s = src + p * ld + j;
+      double2 v[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) v[r] = s[r * m];
+#pragma unroll
+      for (int r = 1; r < R; r++) {
+        double2 t2 = tw[r * k * tstep];
+        t2.y *= sg;
+        v[r] = cmul(v[r], t2);
+      }
       dft<double, R>(v, sg);
+      double2* d = dst + p * ld + (j / Ns) * Ns * R + k;
 #pragma unroll
       for (int r = 0; r < R; r++) d[r * Ns] = v[r];
-    } else {
-      rs_dft_prime<R>(v, sg, cs + cs_off(R), d, Ns);
     }
   }
 }
@@ -539,10 +586,15 @@ __global__ __launch_bounds__(256) void k_rs_zinv_c2f(const double2* __restrict__
   const int g0 = blockIdx.x * P;
   const int npen = ny * hc;
   const int In = c0 / 2, Jn = c1 / 2, Kn = c2 / 2;
-  for (int t = threadIdx.x; t < P * n; t += blockDim.x) {
+  // all global loads of the phase issued before the first LDS store
+  constexpr int EPT = 8;  // P * n < 2048 by the choice of P
+  double2 vt[EPT];
+#pragma unroll
+  for (int it = 0; it < EPT; it++) {
+    const int t = threadIdx.x + it * 256;
     const int p = t % P, e = t / P, g = g0 + p;
     double2 v = make_double2(0.0, 0.0);
-    if (g < npen) {
+    if (t < P * n && g < npen) {
       const int l = g / hc, i = g - l * hc;
       const int j = l < c1 ? l : Jn;
       const int dj = l < c1 ? 0 : 1;
@@ -564,7 +616,12 @@ __global__ __launch_bounds__(256) void k_rs_zinv_c2f(const double2* __restrict__
         }
       }
     }
-    A[p * ld + e] = v;
+    vt[it] = v;
+  }
+#pragma unroll
+  for (int it = 0; it < EPT; it++) {
+    const int t = threadIdx.x + it * 256;
+    if (t < P * n) A[(t % P) * ld + t / P] = vt[it];
   }
   __syncthreads();
   const double2* res = lds_fft_any<PR>(A, B, n, P, ld, tw, 1.0, cs);
@@ -600,15 +657,24 @@ __global__ __launch_bounds__(256) void k_rs_pencil(PenArgs a, const double* __re
   const long b = blockIdx.y;
   const int g0 = blockIdx.x * P;
   const int npen = a.U * a.hc;
-  for (int t = threadIdx.x; t < P * n; t += blockDim.x) {
+  constexpr int EPT = 8;  // P * n < 2048 by the choice of P
+  double2 vt[EPT];
+#pragma unroll
+  for (int it = 0; it < EPT; it++) {
+    const int t = threadIdx.x + it * 256;
     const int p = t % P, e = t / P, g = g0 + p;
     double2 v = make_double2(0.0, 0.0);
-    if (g < npen) {
+    if (t < P * n && g < npen) {
       const int u = g / a.hc, i = g - u * a.hc;
       const int s = a.in_map ? a.in_map[e] : e;
       if (s >= 0) v = a.in[b * a.in_bs + u * a.in_su + s * a.in_se + i];
     }
-    A[p * ld + e] = v;
+    vt[it] = v;
+  }
+#pragma unroll
+  for (int it = 0; it < EPT; it++) {
+    const int t = threadIdx.x + it * 256;
+    if (t < P * n) A[(t % P) * ld + t / P] = vt[it];
   }
   __syncthreads();
   const double2* res = lds_fft_any<PR>(A, B, n, P, ld, tw, a.sg, cs);
@@ -637,10 +703,14 @@ __global__ __launch_bounds__(256) void k_rs_x_c2r(const double2* __restrict__ in
   load_tw(tw, twg, n);
   const long b = blockIdx.y;
   const int g0 = blockIdx.x * P;
-  for (int t = threadIdx.x; t < P * n; t += blockDim.x) {
+  constexpr int EPT = 8;  // P * n < 2048 by the choice of P
+  double2 vt[EPT];
+#pragma unroll
+  for (int it = 0; it < EPT; it++) {
+    const int t = threadIdx.x + it * 256;
     const int e = t % n, p = t / n, g = g0 + p;
     double2 z = make_double2(0.0, 0.0);
-    if (g < npair) {
+    if (t < P * n && g < npair) {
       const double2* r0 = in + b * in_bs + (long)(2 * g) * hw;
       const double2* r1 = r0 + hw;
       double2 z1 = make_double2(0.0, 0.0), z2 = z1;
@@ -654,17 +724,25 @@ __global__ __launch_bounds__(256) void k_rs_x_c2r(const double2* __restrict__ in
       }
       z = make_double2(z1.x - z2.y, z1.y + z2.x);
     }
-    A[p * ld + e] = z;
+    vt[it] = z;
+  }
+#pragma unroll
+  for (int it = 0; it < EPT; it++) {
+    const int t = threadIdx.x + it * 256;
+    if (t < P * n) A[(t / n) * ld + t % n] = vt[it];
   }
   __syncthreads();
   const double2* res = lds_fft_any<PR>(A, B, n, P, ld, tw, 1.0, cs);
-  for (int t = threadIdx.x; t < P * n; t += blockDim.x) {
-    const int e = t % n, p = t / n, g = g0 + p;
+  // two neighbouring x per lane: 16-byte stores (n even, so rows are
+  // 16-byte aligned)
+  const int h = n / 2;
+  for (int t = threadIdx.x; t < P * h; t += blockDim.x) {
+    const int e = 2 * (t % h), p = t / h, g = g0 + p;
     if (g < npair) {
-      const double2 z = res[p * ld + e];
-      double* o = out + b * out_bs + (long)(2 * g) * n;
-      o[e] = z.x;
-      o[n + e] = z.y;
+      const double2 z0 = res[p * ld + e], z1 = res[p * ld + e + 1];
+      double2* o = reinterpret_cast<double2*>(out + b * out_bs + (long)(2 * g) * n + e);
+      o[0] = make_double2(z0.x, z1.x);
+      o[h] = make_double2(z0.y, z1.y);
     }
   }
 }
@@ -683,14 +761,23 @@ __global__ __launch_bounds__(256) void k_rs_x_r2c(const double* __restrict__ in,
   load_tw(tw, twg, n);
   const long b = blockIdx.y;
   const int g0 = blockIdx.x * P;
-  for (int t = threadIdx.x; t < P * n; t += blockDim.x) {
+  constexpr int EPT = 8;  // P * n < 2048 by the choice of P
+  double2 vt[EPT];
+#pragma unroll
+  for (int it = 0; it < EPT; it++) {
+    const int t = threadIdx.x + it * 256;
     const int e = t % n, p = t / n, g = g0 + p;
     double2 z = make_double2(0.0, 0.0);
-    if (g < npair) {
+    if (t < P * n && g < npair) {
       const double* r0 = in + b * in_bs + (long)(2 * g) * n;
-      z = make_double2(r0[e], r0[n + e]);
+      z = make_double2(__builtin_nontemporal_load(r0 + e), __builtin_nontemporal_load(r0 + n + e));
     }
-    A[p * ld + e] = z;
+    vt[it] = z;
+  }
+#pragma unroll
+  for (int it = 0; it < EPT; it++) {
+    const int t = threadIdx.x + it * 256;
+    if (t < P * n) A[(t / n) * ld + t % n] = vt[it];
   }
   __syncthreads();
   const double2* res = lds_fft_any<PR>(A, B, n, P, ld, tw, -1.0, cs);
