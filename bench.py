@@ -2,7 +2,7 @@
 MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload si64_60]
-                    [--no-cpu] [--copy-probe] [--realspace | --realspace-only]
+                    [--no-cpu] [--copy-probe] [--no-realspace | --realspace-only]
 
 Workload si64_60: 64-atom diamond supercell, 60^3 grid, Ecut 10 Ha,
 npw 6567, 256 bands, 256 projectors; degree-10 Chebyshev filter with bounds
@@ -19,7 +19,9 @@ FP64 sweep time, the mixed/FP64 speedup, the Ritz-value error of the mixed
 sweep against the FP64 sweep, plain H*psi throughput (FP64 and FP32), the
 roofline of the dominant kernel (the xy slab FFT kernel, timed with HIP
 events on its launch stream inside the timed region) and the FP64 CPU oracle
-timed on this host's cores.
+timed on this host's cores.  A "realspace" object carries the same kind
+of line for the real-space dual-grid H*psi on the SiC_large deck's shape
+(62^3 / 124^3 grids, 216 atoms, 442 bands).
 """
 import argparse
 import json
@@ -136,8 +138,8 @@ def main():
     ap.add_argument("--workload", default="si64_60")
     ap.add_argument("--hpsi-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--realspace", action="store_true",
-                    help="also time the real-space dual-grid H*psi on the SiC_large shape (sic216_rs)")
+    ap.add_argument("--no-realspace", action="store_true",
+                    help="skip the real-space dual-grid H*psi line (sic216_rs, SiC_large deck shape)")
     ap.add_argument("--realspace-only", action="store_true",
                     help="time only the real-space dual-grid H*psi and print its line")
     ap.add_argument("--copy-probe", action="store_true",
@@ -286,7 +288,7 @@ def main():
         out["ritz_err_ha"]["fp64_gpu_vs_oracle_max"] = float(np.abs(e_f64 - e_o).max())
         out["ritz_err_ha"]["mixed_vs_oracle_wanted_max"] = float(np.abs(e_mixed - e_o)[:nwant].max())
     ctx.close()
-    if args.realspace:
+    if not args.no_realspace:
         out["realspace"] = realspace_bench(args)
     if args.copy_probe:
         # plain device copies of a known size: the scale for FETCH_SIZE / WRITE_SIZE

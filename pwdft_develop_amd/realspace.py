@@ -26,7 +26,7 @@ from . import _lib
 from ._lib import check
 from .workload import MASTER_SEED, Problem, random_normal, random_uniform
 
-STREAM_RS_PSI, STREAM_RS_WGT = 10, 11
+STREAM_RS_PSI, STREAM_RS_WGT, STREAM_RS_ATOMS = 10, 11, 12
 MIN_RADIAL = 1e-10
 
 
@@ -199,6 +199,25 @@ def sic216_rs(nb=442):
     return zincblende(3, 24.660926953004 / 3, nb, "sic216_rs")
 
 
+def mos2_rs(nb=4):
+    """The MoS2_large deck's cell and grids (tests/MoS2_large/config.yaml):
+    30.64 x 35.39 x 27.90 bohr, Ecut 40 Ha, Density_Grid_Factor 2: coarse
+    88 x 102 x 80 (factors 11 and 17), fine 176 x 204 x 160; 108 synthetic
+    atoms (minimum separation 3.5 bohr) with the two species above."""
+    L = np.array([30.64420245852073, 35.385090888733515, 27.897585104690275])
+    grid = tuple(even_grid(40.0, l) for l in L)
+    fine = tuple(int(math.ceil(n * 2.0 / 2.0)) * 2 for n in grid)
+    u = random_uniform(MASTER_SEED, STREAM_RS_ATOMS, 3 * 20000).reshape(-1, 3) * L
+    atoms = []
+    for p in u:
+        if all(np.linalg.norm((p - q) - L * np.round((p - q) / L)) >= 3.5 for q in atoms):
+            atoms.append(p)
+            if len(atoms) == 108:
+                break
+    return RealSpaceProblem("mos2_rs", grid, fine, np.diag(L), np.array(atoms),
+                            [SI if i < 36 else CARBON for i in range(108)], nb)
+
+
 def box_rs(nb=6):
     """Orthorhombic 7 x 9 x 6 bohr box, 3 atoms, coarse 12 x 16 x 10, fine
     18 x 32 x 20 (unequal refinement per axis)."""
@@ -207,7 +226,7 @@ def box_rs(nb=6):
     return RealSpaceProblem("box_rs", (12, 16, 10), (18, 32, 20), sc, atoms, [SI, CARBON, SI], nb)
 
 
-RS_WORKLOADS = {"sic8_rs": sic8_rs, "sic216_rs": sic216_rs, "box_rs": box_rs}
+RS_WORKLOADS = {"sic8_rs": sic8_rs, "sic216_rs": sic216_rs, "mos2_rs": mos2_rs, "box_rs": box_rs}
 
 
 class RealSpaceContext:

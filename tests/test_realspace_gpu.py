@@ -119,6 +119,16 @@ def test_rs_sic_large_one_band_matches_oracle():
     assert _rel(got, ref) < TOL
 
 
+def test_rs_mos2_large_shape_matches_oracle():
+    """The MoS2_large deck's grids: 88 x 102 x 80 coarse (primes 11 and 17),
+    176 x 204 x 160 fine."""
+    print("case mos2_rs two bands", flush=True)
+    p = _prob("mos2_rs", 2)
+    ref = _oracle(p, p.psi)
+    got = _gpu(p, p.psi)
+    assert _rel(got, ref) < TOL
+
+
 def test_rs_sic_large_all_bands_linear():
     import torch
     print("case sic216_rs 442 bands linearity", flush=True)

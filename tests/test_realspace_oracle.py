@@ -159,3 +159,5 @@ def test_workload_shapes():
     assert len(p.atoms) == 216
     q = R.sic8_rs()
     assert q.grid == (22, 22, 22) and q.grid_fine == (44, 44, 44)
+    m = R.mos2_rs()
+    assert m.grid == (88, 102, 80) and m.grid_fine == (176, 204, 160) and len(m.atoms) == 108
