@@ -14,6 +14,12 @@
 //                     NonlocalMultX, FirstChebyStep, GeneralChebyStep, EigVal,
 //                     ResVal)
 //   ErrorHandling     src/global.cpp:71-91: throws std::runtime_error
+//   SparseVec, NonlocalPP, PseudoPot   include/utility.hpp:120-152 (the
+//                     real-space projectors, vnlList only)
+// The real-space dual-grid path (Use_Real_Space = 1) adds Domain::numGridFine,
+// Spinor::Wavefun (real coarse-grid values), KohnSham::CalculateEkin over the
+// whole R2C grid, the fine-grid vtot and pseudo_, and the NumTns<Real>
+// MultSpinor overload (hamiltonian.cpp:3058-3125).
 //
 // Host arrays stay host arrays (as in the reference); each call moves the
 // band block to the device, runs the kernels there and copies results back.
@@ -49,6 +55,18 @@ class NumTns {
  public:
   NumTns() = default;
   NumTns(Int m, Int n, Int p, bool owndata = true, F* data = nullptr) { Resize(m, n, p, owndata, data); }
+  // copies own their data when the source does (a deep copy), else share
+  // the wrapped memory, as the reference's containers do
+  NumTns(const NumTns& o) { *this = o; }
+  NumTns(NumTns&&) = default;
+  NumTns& operator=(NumTns&&) = default;
+  NumTns& operator=(const NumTns& o) {
+    if (this == &o) return *this;
+    m_ = o.m_; n_ = o.n_; p_ = o.p_; owndata_ = o.owndata_;
+    store_ = o.store_;
+    data_ = o.owndata_ ? store_.data() : o.data_;
+    return *this;
+  }
   void Resize(Int m, Int n, Int p, bool owndata = true, F* data = nullptr) {
     m_ = m; n_ = n; p_ = p; owndata_ = owndata;
     if (owndata) { store_.assign((size_t)m * n * p, F(0)); data_ = store_.data(); if (data) std::memcpy(data_, data, sizeof(F) * store_.size()); }
@@ -74,8 +92,18 @@ class NumMat {
  public:
   NumMat() = default;
   NumMat(Int m, Int n, bool owndata = true, F* data = nullptr) { Resize(m, n, owndata, data); }
+  NumMat(const NumMat& o) { *this = o; }
+  NumMat(NumMat&&) = default;
+  NumMat& operator=(NumMat&&) = default;
+  NumMat& operator=(const NumMat& o) {
+    if (this == &o) return *this;
+    m_ = o.m_; n_ = o.n_; owndata_ = o.owndata_;
+    store_ = o.store_;
+    data_ = o.owndata_ ? store_.data() : o.data_;
+    return *this;
+  }
   void Resize(Int m, Int n, bool owndata = true, F* data = nullptr) {
-    m_ = m; n_ = n;
+    m_ = m; n_ = n; owndata_ = owndata;
     if (owndata) { store_.assign((size_t)m * n, F(0)); data_ = store_.data(); if (data) std::memcpy(data_, data, sizeof(F) * store_.size()); }
     else data_ = data;
   }
@@ -87,6 +115,7 @@ class NumMat {
 
  private:
   Int m_ = 0, n_ = 0;
+  bool owndata_ = true;
   F* data_ = nullptr;
   std::vector<F> store_;
 };
@@ -115,10 +144,19 @@ typedef NumVec<Real> DblNumVec;
 typedef NumMat<Real> DblNumMat;
 typedef NumMat<Complex> CpxNumMat;
 
+// include/utility.hpp:120-152 (real-space projectors; column VAL = 0 of dv)
+typedef std::pair<IntNumVec, DblNumMat> SparseVec;
+typedef std::pair<SparseVec, Real> NonlocalPP;
+struct PseudoPot {
+  std::vector<NonlocalPP> vnlList;
+};
+enum { VAL = 0, DX = 1, DY = 2, DZ = 3 };
+
 // ---- Domain (include/domain.hpp) ----
 struct Domain {
   DblNumMat supercell{3, 3};  // supercell(i, j): component j of lattice vector i
   Int numGrid[3] = {0, 0, 0};
+  Int numGridFine[3] = {0, 0, 0};
   Int numGridSphere = 0;
   Real Volume() const {
     const Real* s = supercell.Data();
@@ -126,6 +164,7 @@ struct Domain {
                     s[6] * (s[1] * s[5] - s[2] * s[4]));
   }
   Int NumGridTotal() const { return numGrid[0] * numGrid[1] * numGrid[2]; }
+  Int NumGridTotalFine() const { return numGridFine[0] * numGridFine[1] * numGridFine[2]; }
   // row-major lattice (vector j in row j) for the C-ABI
   void Lattice(double* out) const {
     for (int j = 0; j < 3; j++)
@@ -154,6 +193,14 @@ class DeviceBlock {
   void Download(Complex* h, size_t n) const {
     if (hipMemcpy(h, p_, sizeof(Complex) * n, hipMemcpyDeviceToHost) != hipSuccess) ErrorHandling("D2H copy failed");
   }
+  // real arrays: n doubles (the block is sized in complex slots)
+  void UploadReal(const Real* h, size_t n) {
+    Alloc((n + 1) / 2);
+    if (hipMemcpy(p_, h, sizeof(Real) * n, hipMemcpyHostToDevice) != hipSuccess) ErrorHandling("H2D copy failed");
+  }
+  void DownloadReal(Real* h, size_t n) const {
+    if (hipMemcpy(h, p_, sizeof(Real) * n, hipMemcpyDeviceToHost) != hipSuccess) ErrorHandling("D2H copy failed");
+  }
   void* Ptr() const { return p_; }
 
  private:
@@ -172,7 +219,10 @@ class Fourier {
   DblNumVec gkkSphere;  // |G|^2/2 on the sphere
   Real ecutWavefunction = 0;
 
-  ~Fourier() { if (ctx_) pw_ctx_destroy(ctx_); }
+  ~Fourier() {
+    if (ctx_) pw_ctx_destroy(ctx_);
+    if (rs_) pw_rs_destroy(rs_);
+  }
 
   // Initialize + InitializeSphere (fourier.cpp:97-244, 541-672)
   void Initialize(const Domain& dm, Real ecut, Int maxBands) {
@@ -212,9 +262,26 @@ class Fourier {
     if (!ctx_) ErrorHandling("Fourier is not prepared.");
     return ctx_;
   }
+  // Real-space dual-grid handle (the R2C / R2CFine plans, fourier.cpp:
+  // 166-183, 246-330), created on first use with ekin on the R2C grid.
+  pw_rs* RsCtx(const DblNumVec& ekinR2C) {
+    if (!isInitialized) ErrorHandling("Fourier is not prepared.");
+    if (!rs_) {
+      if (domain.NumGridTotalFine() <= 0) ErrorHandling("numGridFine is not set.");
+      if (ekinR2C.m() != numGridTotalR2C) ErrorHandling("Domain size does not match.");
+      Check(pw_rs_create(&rs_, domain.numGrid, domain.numGridFine, domain.Volume(), ekinR2C.Data(), maxBands_,
+                         nullptr));
+    }
+    return rs_;
+  }
+  pw_rs* RsCtx() const {
+    if (!rs_) ErrorHandling("Fourier is not prepared.");
+    return rs_;
+  }
 
  private:
   pw_ctx* ctx_ = nullptr;
+  pw_rs* rs_ = nullptr;
   Int maxBands_ = 0;
 };
 
@@ -232,6 +299,16 @@ class Spinor {
     numStateTotal_ = numStateTotal;
     wavefunG_.Resize(dm.numGridSphere, numComponent, numStateTotal, owndata, data);
   }
+  // Real-space Gamma wavefunctions on the coarse grid (spinor.cpp:85-112):
+  // ntot x numComponent x numState, wrapping caller memory when owndata ==
+  // false.
+  void SetupReal(const Domain& dm, Int numComponent, Int numStateTotal, bool owndata, Real* data) {
+    domain_ = dm;
+    numComponent_ = numComponent;
+    numStateTotal_ = numStateTotal;
+    wavefun_.Resize(dm.NumGridTotal(), numComponent, numStateTotal, owndata, data);
+  }
+  NumTns<Real>& Wavefun() { return wavefun_; }
   NumTns<Complex>& WavefunG() { return wavefunG_; }
   Int NumGridTotal() const { return wavefunG_.m(); }
   Int NumComponent() const { return numComponent_; }
@@ -256,6 +333,7 @@ class Spinor {
   Domain domain_;
   Int numComponent_ = 1, numStateTotal_ = 0;
   NumTns<Complex> wavefunG_;
+  NumTns<Real> wavefun_;
 };
 
 // ---- Hamiltonian / KohnSham (include/hamiltonian.hpp) ----
@@ -263,6 +341,7 @@ class Hamiltonian {
  public:
   virtual ~Hamiltonian() = default;
   virtual void MultSpinor(Spinor& psi, NumTns<Complex>& a3, Fourier& fft) = 0;
+  virtual void MultSpinor(Spinor& psi, NumTns<Real>& a3, Fourier& fft) = 0;
 };
 
 class KohnSham : public Hamiltonian {
@@ -300,6 +379,85 @@ class KohnSham : public Hamiltonian {
     Check(pw_mult_spinor(ctx_, PW_F64, nb, x.Ptr(), y.Ptr()));
     y.Download(a3.Data(), (size_t)ntot * nb);
   }
+  // CalculateEkin without sphere cut (hamiltonian.cpp:2629-2667 with
+  // isUseSphereCut false): |G|^2/2 and Teter over the coarse R2C grid, G from
+  // the k-grid of fourier.cpp:125-131 and the reciprocal lattice.
+  void CalculateEkinR2C(Fourier& fft) {
+    const Domain& dm = fft.domain;
+    const Int n1 = dm.numGrid[0], n2 = dm.numGrid[1], n3 = dm.numGrid[2], h1 = n1 / 2 + 1;
+    double A[9], B[9];
+    dm.Lattice(A);
+    const double det = A[0] * (A[4] * A[8] - A[5] * A[7]) - A[1] * (A[3] * A[8] - A[5] * A[6]) +
+                       A[2] * (A[3] * A[7] - A[4] * A[6]);
+    if (det == 0.0) ErrorHandling("singular supercell");
+    // rows of B = 2 pi (A^{-1})^T: b_j . a_i = 2 pi delta_ij
+    const double c = 2.0 * 3.14159265358979323846 / det;
+    B[0] = c * (A[4] * A[8] - A[5] * A[7]); B[1] = c * (A[5] * A[6] - A[3] * A[8]); B[2] = c * (A[3] * A[7] - A[4] * A[6]);
+    B[3] = c * (A[2] * A[7] - A[1] * A[8]); B[4] = c * (A[0] * A[8] - A[2] * A[6]); B[5] = c * (A[1] * A[6] - A[0] * A[7]);
+    B[6] = c * (A[1] * A[5] - A[2] * A[4]); B[7] = c * (A[2] * A[3] - A[0] * A[5]); B[8] = c * (A[0] * A[4] - A[1] * A[3]);
+    ekinR2C_.Resize(h1 * n2 * n3);
+    teterR2C_.Resize(h1 * n2 * n3);
+    Int g = 0;
+    for (Int k = 0; k < n3; k++)
+      for (Int j = 0; j < n2; j++)
+        for (Int i = 0; i < h1; i++, g++) {
+          const double m[3] = {(double)i, (double)(j <= n2 / 2 ? j : j - n2), (double)(k <= n3 / 2 ? k : k - n3)};
+          double G[3] = {0, 0, 0};
+          for (int d = 0; d < 3; d++) G[d] = m[0] * B[d] + m[1] * B[3 + d] + m[2] * B[6 + d];
+          ekinR2C_[g] = (G[0] * G[0] + G[1] * G[1] + G[2] * G[2]) / 2;
+          const double a = ekinR2C_[g] * 2.0, b = 27.0 + a * (18.0 + a * (12.0 + a * 8.0));
+          teterR2C_[g] = b / (b + 16.0 * a * a * a * a);
+        }
+    rs_ = fft.RsCtx(ekinR2C_);
+  }
+  // vtot_[:, spinswitch_] on the fine grid (hamiltonian.cpp:3120-3122)
+  void SetVtotFine(const DblNumVec& vtotFine) {
+    if (!rs_) ErrorHandling("Fourier is not prepared.");
+    Check(pw_rs_set_vtot(rs_, vtotFine.Data()));
+  }
+  // pseudo_ (hamiltonian.cpp:325-470): every projector of an atom is a
+  // SparseVec on the atom's index list (periodtable.cpp:1063-1589)
+  void SetPseudo(const std::vector<PseudoPot>& pseudo) {
+    if (!rs_) ErrorHandling("Fourier is not prepared.");
+    std::vector<int> npts, nproj, idx;
+    std::vector<double> val, wgt;
+    for (const PseudoPot& pp : pseudo) {
+      const Int np = (Int)pp.vnlList.size();
+      const Int m = np ? pp.vnlList[0].first.first.m() : 0;
+      for (const NonlocalPP& v : pp.vnlList) {
+        const IntNumVec& iv = v.first.first;
+        if (iv.m() != m || std::memcmp(iv.Data(), pp.vnlList[0].first.first.Data(), sizeof(Int) * m) != 0)
+          ErrorHandling("projectors of one atom must share their grid points");
+        if (v.first.second.m() != m) ErrorHandling("Domain size does not match.");
+      }
+      npts.push_back(m);
+      nproj.push_back(np);
+      if (np) idx.insert(idx.end(), pp.vnlList[0].first.first.Data(), pp.vnlList[0].first.first.Data() + m);
+      for (const NonlocalPP& v : pp.vnlList) {
+        const DblNumMat& dv = v.first.second;
+        for (Int i = 0; i < m; i++) val.push_back(dv(i, VAL));
+        wgt.push_back(v.second);
+      }
+    }
+    Check(pw_rs_set_nonlocal(rs_, (int)pseudo.size(), npts.data(), nproj.data(), idx.data(), val.data(),
+                             wgt.data()));
+  }
+  // MultSpinor(Spinor&, NumTns<Real>&, Fourier&) (hamiltonian.cpp:3058-3125)
+  // with AddMultSpinorFineR2C (spinor.cpp:376-525): a3 = H psi.
+  void MultSpinor(Spinor& psi, NumTns<Real>& a3, Fourier& fft) override {
+    if (!fft.isInitialized) ErrorHandling("Fourier is not prepared.");
+    const Int ntot = psi.Wavefun().m(), nb = psi.Wavefun().p();
+    if (fft.domain.NumGridTotal() != ntot) ErrorHandling("Domain size does not match3.");
+    if (psi.Wavefun().n() != 1)
+      ErrorHandling("Component number of spinor should be one in spin-restricted and spin-unrestricted calculations.");
+    pw_rs* rs = fft.RsCtx();
+    DeviceBlock x, y;
+    x.UploadReal(psi.Wavefun().Data(), (size_t)ntot * nb);
+    y.Alloc(((size_t)ntot * nb + 1) / 2);
+    Check(pw_rs_mult_spinor(rs, nb, static_cast<const double*>(x.Ptr()), static_cast<double*>(y.Ptr())));
+    y.DownloadReal(a3.Data(), (size_t)ntot * nb);
+  }
+  DblNumVec& EkinR2C() { return ekinR2C_; }
   DblNumVec& Teter() { return teter_; }
   DblNumVec& Ekin() { return ekin_; }
   std::pair<CpxNumMat, DblNumVec>& Vnlc() { return vnlc_; }
@@ -308,7 +466,8 @@ class KohnSham : public Hamiltonian {
 
  private:
   pw_ctx* ctx_ = nullptr;
-  DblNumVec ekin_, teter_, vtot_, eigVal_;
+  pw_rs* rs_ = nullptr;
+  DblNumVec ekin_, teter_, vtot_, eigVal_, ekinR2C_, teterR2C_;
   std::pair<CpxNumMat, DblNumVec> vnlc_;
 };
 

@@ -11,6 +11,7 @@
 //         eigenvalues[nb] (3 iterations), ChebFSI eigenvalues[nb] (one
 //         GeneralChebyStep after the LOBPCG, degree 10, 8 Lanczos steps)
 #include <cstdio>
+#include <string>
 #include <vector>
 
 #include "../../pwdft_develop_amd/host/pwdft_adapter.hpp"
@@ -22,7 +23,77 @@ static void rd(FILE* f, T* p, size_t n) {
   if (fread(p, sizeof(T), n, f) != n) ErrorHandling("short read");
 }
 
+// adapter_driver rs <input.bin> <output.bin>: the real-space dual-grid path
+// input: int32 n[3] nf[3] nb natom, then per atom npts nproj; float64
+//        supercell[9]; vtotFine[Nf]; per atom int32 idx[npts], float64
+//        val[npts nproj] (column-major), wgt[nproj]; psi[ntot nb]
+// output: float64 a3[ntot nb], ekin on the R2C grid
+static int run_realspace(const char* in, const char* outp) {
+  FILE* f = fopen(in, "rb");
+  if (!f) ErrorHandling("cannot open input");
+  int hdr[8];
+  rd(f, hdr, 8);
+  const int nb = hdr[6], natom = hdr[7];
+  std::vector<int> cnt(2 * (size_t)natom);
+  rd(f, cnt.data(), cnt.size());
+  Domain dm;
+  for (int d = 0; d < 3; d++) {
+    dm.numGrid[d] = hdr[d];
+    dm.numGridFine[d] = hdr[3 + d];
+  }
+  double sc[9];
+  rd(f, sc, 9);
+  for (int j = 0; j < 3; j++)
+    for (int i = 0; i < 3; i++) dm.supercell(j, i) = sc[3 * j + i];
+  DblNumVec vtot(dm.NumGridTotalFine());
+  rd(f, vtot.Data(), vtot.m());
+  std::vector<PseudoPot> pseudo((size_t)natom);
+  for (int a = 0; a < natom; a++) {
+    const int m = cnt[2 * a], np = cnt[2 * a + 1];
+    IntNumVec iv(m);
+    rd(f, iv.Data(), m);
+    std::vector<double> val((size_t)m * np), w(np);
+    rd(f, val.data(), val.size());
+    rd(f, w.data(), w.size());
+    for (int p = 0; p < np; p++) {
+      DblNumMat dv(m, 4);
+      for (int i = 0; i < m; i++) dv(i, VAL) = val[(size_t)p * m + i];
+      pseudo[a].vnlList.push_back(NonlocalPP(SparseVec(iv, dv), w[p]));
+    }
+  }
+  const int ntot = dm.NumGridTotal();
+  std::vector<double> psi((size_t)ntot * nb);
+  rd(f, psi.data(), psi.size());
+  fclose(f);
+
+  Fourier fft;
+  fft.Initialize(dm, 1.0, nb);
+  KohnSham ham;
+  ham.CalculateEkinR2C(fft);
+  ham.SetVtotFine(vtot);
+  ham.SetPseudo(pseudo);
+  Spinor spn;
+  spn.SetupReal(fft.domain, 1, nb, false, psi.data());
+  NumTns<Real> a3(ntot, 1, nb);
+  ham.MultSpinor(spn, a3, fft);
+  FILE* g = fopen(outp, "wb");
+  const size_t n3 = (size_t)ntot * nb, ne = (size_t)ham.EkinR2C().m();
+  if (!g || fwrite(a3.Data(), sizeof(double), n3, g) != n3 ||
+      fwrite(ham.EkinR2C().Data(), sizeof(double), ne, g) != ne)
+    ErrorHandling("cannot write output");
+  fclose(g);
+  return 0;
+}
+
 int main(int argc, char** argv) {
+  if (argc == 4 && std::string(argv[1]) == "rs") {
+    try {
+      return run_realspace(argv[2], argv[3]);
+    } catch (const std::exception& e) {
+      fprintf(stderr, "ERROR: %s\n", e.what());
+      return 1;
+    }
+  }
   if (argc != 3) {
     fprintf(stderr, "usage: %s in.bin out.bin\n", argv[0]);
     return 2;
