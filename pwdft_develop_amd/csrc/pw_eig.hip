@@ -16,9 +16,10 @@
 //   k_stebz           all eigenvalues of T: 64-way multisection on Sturm
 //                     counts, one wavefront per eigenvalue (LAPACK dstebz
 //                     tolerances)
-//   k_stein           eigenvectors of T: inverse iteration on LU with
-//                     partial pivoting of T - lambda I, one thread per
-//                     eigenvalue; modified Gram-Schmidt within tight clusters
+//   k_stein_lds       eigenvectors of T: inverse iteration on LU with
+//   (k_stein)         partial pivoting of T - lambda I, one workgroup (lane
+//                     0, scratch in LDS) per eigenvalue up to n = 768, one
+//                     thread per eigenvalue beyond; modified Gram-Schmidt within tight clusters
 //                     (gaps <= 1e-5 ||T||; LAPACK dstein uses 1e-3, which
 //                     chains the dense Rayleigh-Ritz spectra into long serial
 //                     runs); scratch interleaved by thread so that lanes
@@ -42,6 +43,7 @@ constexpr int kTrdThreads = 1024;  // k_sytrd workgroup
 constexpr int kTrdWaves = kTrdThreads / 64;
 constexpr int kOrmCols = 8;         // columns per k_ormtr workgroup
 constexpr int kEigMaxN = 1024;      // LDS bound of k_sytrd
+constexpr int kSteinLdsMaxN = 768;  // 9 n doubles of k_stein_lds within 64 KiB
 
 // Block-wide sum, identical order on every call (deterministic), returned to
 // all threads.  sred holds one double per wavefront.
@@ -746,13 +748,63 @@ __device__ double stein_solve(int n, double scale, SteinRows R, int nthr, int q)
   return ss;
 }
 
-// Eigenvectors of T into the columns of Zt (ld n).  One thread per
-// eigenvalue j: two inverse iterations from a pseudo-random start.  Tight
-// clusters (consecutive gaps <= 1e-5 ||T||, which includes exact
-// degeneracies) are taken by the thread of their first member, sequentially
-// with modified Gram-Schmidt against the earlier members on every iteration
-// (LAPACK dstein's scheme).  Outside them the vectors are orthogonal to
-// ~eps ||T|| / gap <= 1e-10.  blockDim = 64.
+// Eigenvector(s) of T for eigenvalue j into the columns of Zt (ld n): two
+// inverse iterations from a pseudo-random start.  Tight clusters
+// (consecutive gaps <= 1e-5 ||T||, which includes exact degeneracies) are
+// taken by the caller of their first member, sequentially with modified
+// Gram-Schmidt against the earlier members on every iteration (LAPACK
+// dstein's scheme).  Outside them the vectors are orthogonal to ~eps ||T|| /
+// gap <= 1e-10.  R holds the caller's scratch rows (stride nthr, column q).
+__device__ void stein_cluster(const double* d, const double* e, const double* __restrict__ w, int n, int j,
+                              double nrm, double* __restrict__ Zt, SteinRows R, int nthr, int q) {
+  const double eps = DBL_EPSILON;
+  const double ctol = 1e-5 * nrm;
+  const double pert = eps * fmax(nrm, DBL_MIN);
+  if (j > 0 && w[j] - w[j - 1] <= ctol) return;  // member of an earlier caller's cluster
+  int j1 = j;
+  while (j1 + 1 < n && w[j1 + 1] - w[j1] <= ctol) j1++;
+  double lamprev = 0.0;
+  for (int jj = j; jj <= j1; jj++) {
+    double lam = w[jj];
+    if (jj > j && lam - lamprev < 10.0 * eps * fabs(lam)) lam = lamprev + 10.0 * eps * fabs(lam);
+    lamprev = lam;
+    stein_factor(d, e, n, lam, pert, R, nthr, q);
+    for (int i = 0; i < n; i++) R.x[(size_t)i * nthr + q] = start_value(jj, i);
+    const int iters = (j1 > j) ? 4 : 2;
+    double scale = 1.0;
+    for (int it = 0; it < iters; it++) {
+      double ss = stein_solve(n, scale, R, nthr, q);
+      if (jj > j) {
+        // orthogonalise against the cluster's earlier vectors (unit norm)
+        for (int kk = j; kk < jj; kk++) {
+          const double* zk = Zt + (size_t)n * kk;
+          double dot = 0.0;
+          for (int i = 0; i < n; i++) dot += zk[i] * R.x[(size_t)i * nthr + q];
+          for (int i = 0; i < n; i++) R.x[(size_t)i * nthr + q] -= dot * zk[i];
+        }
+        ss = 0.0;
+        for (int i = 0; i < n; i++) {
+          const double v = R.x[(size_t)i * nthr + q];
+          ss += v * v;
+        }
+      }
+      scale = ss > 0.0 ? 1.0 / sqrt(ss) : 1.0;
+    }
+    double* zj = Zt + (size_t)n * jj;
+    for (int i0 = 0; i0 < n; i0 += kSteinB) {
+      double xb[kSteinB];
+#pragma unroll
+      for (int u = 0; u < kSteinB; u++) xb[u] = (i0 + u < n) ? R.x[(size_t)(i0 + u) * nthr + q] : 0.0;
+#pragma unroll
+      for (int u = 0; u < kSteinB; u++)
+        if (i0 + u < n) zj[i0 + u] = xb[u] * scale;
+    }
+  }
+}
+
+// One thread per eigenvalue, scratch rows in global memory S (7 n^2
+// doubles, interleaved by thread so that lanes stepping through the same
+// row coalesce).  blockDim = 64.  For n beyond the LDS form below.
 __global__ void __launch_bounds__(64)
 k_stein(const double* __restrict__ dg, const double* __restrict__ eg, const double* __restrict__ w, int n,
         double* __restrict__ Zt, double* __restrict__ S) {
@@ -768,52 +820,32 @@ k_stein(const double* __restrict__ dg, const double* __restrict__ eg, const doub
   tri_bounds(dg, eg, n, sred, &gl, &gu, &nrm, &em);  // ends with a barrier
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  const double eps = DBL_EPSILON;
-  const double ctol = 1e-5 * nrm;
-  const double pert = eps * fmax(nrm, DBL_MIN);
-  if (j > 0 && w[j] - w[j - 1] <= ctol) return;  // member of an earlier thread's cluster
-  int j1 = j;
-  while (j1 + 1 < n && w[j1 + 1] - w[j1] <= ctol) j1++;
-  const int nthr = n;
-  const size_t an = (size_t)n * nthr;
+  const size_t an = (size_t)n * n;
   SteinRows R{S, S + an, S + 2 * an, S + 3 * an, S + 4 * an, S + 5 * an, S + 6 * an};
-  double lamprev = 0.0;
-  for (int jj = j; jj <= j1; jj++) {
-    double lam = w[jj];
-    if (jj > j && lam - lamprev < 10.0 * eps * fabs(lam)) lam = lamprev + 10.0 * eps * fabs(lam);
-    lamprev = lam;
-    stein_factor(d, e, n, lam, pert, R, nthr, j);
-    for (int i = 0; i < n; i++) R.x[(size_t)i * nthr + j] = start_value(jj, i);
-    const int iters = (j1 > j) ? 4 : 2;
-    double scale = 1.0;
-    for (int it = 0; it < iters; it++) {
-      double ss = stein_solve(n, scale, R, nthr, j);
-      if (jj > j) {
-        // orthogonalise against the cluster's earlier vectors (unit norm)
-        for (int kk = j; kk < jj; kk++) {
-          const double* zk = Zt + (size_t)n * kk;
-          double dot = 0.0;
-          for (int i = 0; i < n; i++) dot += zk[i] * R.x[(size_t)i * nthr + j];
-          for (int i = 0; i < n; i++) R.x[(size_t)i * nthr + j] -= dot * zk[i];
-        }
-        ss = 0.0;
-        for (int i = 0; i < n; i++) {
-          const double v = R.x[(size_t)i * nthr + j];
-          ss += v * v;
-        }
-      }
-      scale = ss > 0.0 ? 1.0 / sqrt(ss) : 1.0;
-    }
-    double* zj = Zt + (size_t)n * jj;
-    for (int i0 = 0; i0 < n; i0 += kSteinB) {
-      double xb[kSteinB];
-#pragma unroll
-      for (int u = 0; u < kSteinB; u++) xb[u] = (i0 + u < n) ? R.x[(size_t)(i0 + u) * nthr + j] : 0.0;
-#pragma unroll
-      for (int u = 0; u < kSteinB; u++)
-        if (i0 + u < n) zj[i0 + u] = xb[u] * scale;
-    }
+  stein_cluster(d, e, w, n, j, nrm, Zt, R, n, j);
+}
+
+// One eigenvalue per workgroup (one wavefront; lane 0 iterates), its seven
+// scratch rows in LDS next to d and e: every step of the serial
+// factorisation and solves is an LDS round trip instead of an L2 one, and
+// the n workgroups spread over the CUs.  9 n doubles of LDS.
+__global__ void __launch_bounds__(64)
+k_stein_lds(const double* __restrict__ dg, const double* __restrict__ eg, const double* __restrict__ w, int n,
+            double* __restrict__ Zt) {
+  extern __shared__ double sde[];
+  double* d = sde;
+  double* e = d + n;
+  double* S = e + n;
+  __shared__ double sred[16];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    d[i] = dg[i];
+    if (i < n - 1) e[i] = eg[i];
   }
+  double gl, gu, nrm, em;
+  tri_bounds(dg, eg, n, sred, &gl, &gu, &nrm, &em);  // ends with a barrier
+  if (threadIdx.x != 0) return;
+  SteinRows R{S, S + n, S + 2 * n, S + 3 * n, S + 4 * n, S + 5 * n, S + 6 * n};
+  stein_cluster(d, e, w, n, blockIdx.x, nrm, Zt, R, 1, 0);
 }
 
 // V of the chunk of reflectors kb .. kb + b - 1 into LDS, row-major with the
@@ -1035,9 +1067,12 @@ int eig_sym(pw_ctx* c, int n, double* A, int lda, double* w) {
   }
   hipLaunchKernelGGL(k_stebz, dim3((n + 3) / 4), dim3(256), sizeof(double) * 2 * n, s, (const double*)d,
                      (const double*)e, n, w);
-  hipLaunchKernelGGL(k_stein, dim3((n + 63) / 64), dim3(64), sizeof(double) * 2 * n, s, (const double*)d,
-                     (const double*)e,
-                     (const double*)w, n, Zt, S);
+  if (n <= kSteinLdsMaxN)
+    hipLaunchKernelGGL(k_stein_lds, dim3(n), dim3(64), sizeof(double) * 9 * n, s, (const double*)d,
+                       (const double*)e, (const double*)w, n, Zt);
+  else
+    hipLaunchKernelGGL(k_stein, dim3((n + 63) / 64), dim3(64), sizeof(double) * 2 * n, s, (const double*)d,
+                       (const double*)e, (const double*)w, n, Zt, S);
   // compact-WY chunk: V (n x nbw) and T next to the Z block in LDS
   const size_t lds_orm = sizeof(double) * ((size_t)n * kOrmCols + (size_t)n * (nbw + 1) + (size_t)nbw * nbw +
                                            2 * (size_t)nbw * kOrmCols);
