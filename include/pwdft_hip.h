@@ -277,7 +277,7 @@ int pw_get_timing(pw_ctx* ctx, double* ms4, int* ncalls);
  * ekin: host array on the coarse R2C grid ((n1/2+1)*n2*n3), the reference's
  * ekin_ without sphere cut (src/SCF/hamiltonian.cpp:2629-2667).
  * max_bands caps the bands transformed per batch (the library also keeps
- * its batch buffers within about 4 GiB); any nb is accepted later. */
+ * its batch buffers within about 8 GiB); any nb is accepted later. */
 typedef struct pw_rs pw_rs;
 int pw_rs_create(pw_rs** rs, const int grid[3], const int grid_fine[3], double vol,
                  const double* ekin, int max_bands, void* stream);

@@ -97,6 +97,7 @@ struct pw_rs {
   double2* d_u0 = nullptr;      // [m][l][i]  nz x ny x hc
   double2* d_cv = nullptr;      // coarse [z][y][i] scratch
   double2* d_cw = nullptr;
+  double* d_pv = nullptr;       // V psi on the fine grid (fused local term)
   int* d_ymap = nullptr;        // fine y -> live row or -1 (f1)
   int* d_zmap = nullptr;        // fine z -> live plane or -1 (f2)
   int* d_lmap = nullptr;        // live row -> fine y (ny)
@@ -694,7 +695,8 @@ __global__ __launch_bounds__(256) void k_rs_pencil(PenArgs a, const double* __re
 template <int PR>
 __global__ __launch_bounds__(256) void k_rs_x_c2r(const double2* __restrict__ in, double* __restrict__ out,
                                                  const double2* __restrict__ twg, const double* __restrict__ cs,
-                                                 int n, int hw, int npair, long in_bs, long out_bs, int P) {
+                                                 int n, int hw, int npair, long in_bs, long out_bs, int P,
+                                                 const double* __restrict__ vloc, double* __restrict__ out2) {
   extern __shared__ double2 sm[];
   const int ld = n + 1;
   double2* tw = sm;
@@ -740,9 +742,18 @@ __global__ __launch_bounds__(256) void k_rs_x_c2r(const double2* __restrict__ in
     const int e = 2 * (t % h), p = t / h, g = g0 + p;
     if (g < npair) {
       const double2 z0 = res[p * ld + e], z1 = res[p * ld + e + 1];
-      double2* o = reinterpret_cast<double2*>(out + b * out_bs + (long)(2 * g) * n + e);
+      const long off = (long)(2 * g) * n + e;
+      double2* o = reinterpret_cast<double2*>(out + b * out_bs + off);
       o[0] = make_double2(z0.x, z1.x);
       o[h] = make_double2(z0.y, z1.y);
+      if (vloc) {
+        // V psi beside psi (the local-potential term of the fine grid)
+        const double2 v0 = *reinterpret_cast<const double2*>(vloc + off);
+        const double2 v1 = *reinterpret_cast<const double2*>(vloc + off + n);
+        double2* o2 = reinterpret_cast<double2*>(out2 + b * out_bs + off);
+        o2[0] = make_double2(v0.x * z0.x, v0.y * z1.x);
+        o2[h] = make_double2(v1.x * z0.y, v1.y * z1.y);
+      }
     }
   }
 }
@@ -1039,7 +1050,8 @@ int setup_own_fft(pw_rs* rs) {
       hipMalloc((void**)&rs->d_tf, sizeof(double2) * tf * ch) != hipSuccess ||
       hipMalloc((void**)&rs->d_u0, sizeof(double2) * u0 * ch) != hipSuccess ||
       hipMalloc((void**)&rs->d_cv, sizeof(double2) * rs->Hc * ch) != hipSuccess ||
-      hipMalloc((void**)&rs->d_cw, sizeof(double2) * rs->Hc * ch) != hipSuccess)
+      hipMalloc((void**)&rs->d_cw, sizeof(double2) * rs->Hc * ch) != hipSuccess ||
+      hipMalloc((void**)&rs->d_pv, sizeof(double) * rs->Nf * ch) != hipSuccess)
     return fail(PW_ERR_NOMEM, "pw_rs_create: cannot allocate %d-band transform buffers", rs->chunk);
   return PW_OK;
 }
@@ -1090,14 +1102,14 @@ extern "C" int pw_rs_create(pw_rs** out, const int grid[3], const int grid_fine[
   }
   rs->vol = vol;
   rs->stream = (hipStream_t)stream;
-  // bands per chunk: chunk buffers within about 4 GiB
+  // bands per chunk: chunk buffers within about 8 GiB of the 288 GB HBM
   double per_band = 16.0 * rs->Hc + 16.0 * rs->Hf + 8.0 * rs->Nf;
   if (rs->own_fft) {
     const double hc = rs->hc0, ny = grid[1] + 1, nz = grid[2] + 1;
-    per_band = 48.0 * rs->Hc + 8.0 * rs->Nf +
+    per_band = 48.0 * rs->Hc + 16.0 * rs->Nf +
                16.0 * hc * ((double)grid_fine[2] * ny + (double)grid_fine[2] * grid_fine[1] + nz * ny);
   }
-  rs->chunk = (int)std::max(1.0, std::min((double)max_bands, 4294967296.0 / per_band));
+  rs->chunk = (int)std::max(1.0, std::min((double)max_bands, 8589934592.0 / per_band));
   int rc = PW_OK;
   auto bail = [&](int code) {
     pw_rs_destroy(rs);
@@ -1144,6 +1156,7 @@ extern "C" int pw_rs_destroy(pw_rs* rs) {
   hipFree(rs->d_u0);
   hipFree(rs->d_cv);
   hipFree(rs->d_cw);
+  hipFree(rs->d_pv);
   hipFree(rs->d_ymap);
   hipFree(rs->d_zmap);
   hipFree(rs->d_lmap);
@@ -1295,7 +1308,7 @@ int launch_pencil(pw_rs* rs, PenArgs a, int nbc) {
 }
 
 int launch_x(pw_rs* rs, bool c2r, const void* in, void* out, int n, int hw, int nrows, long in_bs, long out_bs,
-             int nbc) {
+             int nbc, const double* vt = nullptr, double* out2 = nullptr) {
   const int P = pencils(n);
   const int npair = nrows / 2;
   const dim3 g((npair + P - 1) / P, nbc);
@@ -1304,7 +1317,7 @@ int launch_x(pw_rs* rs, bool c2r, const void* in, void* out, int n, int hw, int 
   if (c2r) {
 #define RS_LAUNCH(PRV)                                                                                    \
   hipLaunchKernelGGL(k_rs_x_c2r<PRV>, g, dim3(256), pencil_lds(n, P), rs->stream, (const double2*)in, \
-                     (double*)out, tw, cs, n, hw, npair, in_bs, out_bs, P)
+                     (double*)out, tw, cs, n, hw, npair, in_bs, out_bs, P, vt, out2)
     RS_PR_SWITCH(big_prime(n), RS_LAUNCH)
 #undef RS_LAUNCH
   } else {
@@ -1319,7 +1332,7 @@ int launch_x(pw_rs* rs, bool c2r, const void* in, void* out, int n, int hw, int 
 }
 
 // coarse psi -> xc (coarse spectrum) -> pf (psi on the fine grid)
-int own_to_fine(pw_rs* rs, const double* psi, int nbc) {
+int own_to_fine(pw_rs* rs, const double* psi, int nbc, bool fuse_v) {
   const int *nc = rs->nc, *nf = rs->nf;
   const int hc = rs->hc0;
   const long Hc = rs->Hc;
@@ -1347,18 +1360,19 @@ int own_to_fine(pw_rs* rs, const double* psi, int nbc) {
   PenArgs yi{rs->d_tz, Tzb, (long)rs->ny * hc, hc, rs->d_ymap, rs->d_tf, Tfb, (long)nf[1] * hc, hc, nullptr,
              nf[1], nf[2], hc, nf[1], 0, 1.0, nullptr};
   PW_TRY(launch_pencil(rs, yi, nbc));
-  PW_TRY(launch_x(rs, true, rs->d_tf, rs->d_pf, nf[0], hc, nf[1] * nf[2], Tfb, rs->Nf, nbc));
+  PW_TRY(launch_x(rs, true, rs->d_tf, rs->d_pf, nf[0], hc, nf[1] * nf[2], Tfb, rs->Nf, nbc,
+                  fuse_v ? rs->d_vtot : nullptr, rs->d_pv));
   return PW_OK;
 }
 
 // pf (H psi terms on the fine grid) -> restriction + kinetic -> a3
-int own_to_coarse(pw_rs* rs, double* a3, int nbc) {
+int own_to_coarse(pw_rs* rs, const double* fine, double* a3, int nbc) {
   const int *nc = rs->nc, *nf = rs->nf;
   const int hc = rs->hc0;
   const long Hc = rs->Hc;
   const long Tzb = (long)nf[2] * rs->ny * hc, Tfb = (long)nf[2] * nf[1] * hc;
   const long U0b = (long)rs->nz * rs->ny * hc;
-  PW_TRY(launch_x(rs, false, rs->d_pf, rs->d_tf, nf[0], hc, nf[1] * nf[2], rs->Nf, Tfb, nbc));
+  PW_TRY(launch_x(rs, false, fine, rs->d_tf, nf[0], hc, nf[1] * nf[2], rs->Nf, Tfb, nbc));
   PenArgs yf{rs->d_tf, Tfb, (long)nf[1] * hc, hc, nullptr, rs->d_tz, Tzb, (long)rs->ny * hc, hc, rs->d_lmap,
              rs->ny, nf[2], hc, nf[1], 0, -1.0, nullptr};
   PW_TRY(launch_pencil(rs, yf, nbc));
@@ -1380,9 +1394,12 @@ int own_to_coarse(pw_rs* rs, double* a3, int nbc) {
   return PW_OK;
 }
 
-// projector weights, V psi and the projector terms on pf
-int local_terms(pw_rs* rs, int nbc) {
+// Projector weights (from psi on the fine grid, pf), then V psi and the
+// projector terms.  fused: V psi is already in d_pv (written by the C2R
+// row pass) and the projector terms go there; else in place on pf.
+int local_terms(pw_rs* rs, int nbc, bool fused) {
   const bool nl = rs->natom > 0 && rs->nproj_tot > 0;
+  double* u = fused ? rs->d_pv : rs->d_pf;
   if (nl) {
     const double scale = rs->vol / (double)rs->Nf;
     if (rs->npmax <= 8)
@@ -1400,23 +1417,25 @@ int local_terms(pw_rs* rs, int nbc) {
     PW_HIP(hipGetLastError());
   }
   if (!nl || rs->ncolor > 0) {
-    // Nf is even (even grids), so every band row starts 16-byte aligned
-    const unsigned nblk = (unsigned)((rs->Nf / 2 + 255) / 256);
-    hipLaunchKernelGGL(k_rs_vmul, dim3(nblk, (nbc + 7) / 8), dim3(256), 0, rs->stream, rs->d_pf, rs->d_vtot,
-                       nbc, rs->Nf);
-    PW_HIP(hipGetLastError());
+    if (!fused) {
+      // Nf is even (even grids), so every band row starts 16-byte aligned
+      const unsigned nblk = (unsigned)((rs->Nf / 2 + 255) / 256);
+      hipLaunchKernelGGL(k_rs_vmul, dim3(nblk, (nbc + 7) / 8), dim3(256), 0, rs->stream, rs->d_pf, rs->d_vtot,
+                         nbc, rs->Nf);
+      PW_HIP(hipGetLastError());
+    }
     for (int c = 0; nl && c < rs->ncolor; c++) {
       const int na = rs->color_off[c + 1] - rs->color_off[c];
       const int* cl = rs->d_clist + rs->color_off[c];
       const dim3 g(na, (nbc + 31) / 32);
       if (rs->npmax <= 8)
-        hipLaunchKernelGGL(k_rs_nl_apply<8>, g, dim3(256), 0, rs->stream, rs->d_pf, cl, rs->d_atab, rs->d_apt_off,
+        hipLaunchKernelGGL(k_rs_nl_apply<8>, g, dim3(256), 0, rs->stream, u, cl, rs->d_atab, rs->d_apt_off,
                            rs->d_idx, rs->d_val, rs->d_coef, nbc, rs->Nf);
       else if (rs->npmax <= 16)
-        hipLaunchKernelGGL(k_rs_nl_apply<16>, g, dim3(256), 0, rs->stream, rs->d_pf, cl, rs->d_atab,
+        hipLaunchKernelGGL(k_rs_nl_apply<16>, g, dim3(256), 0, rs->stream, u, cl, rs->d_atab,
                            rs->d_apt_off, rs->d_idx, rs->d_val, rs->d_coef, nbc, rs->Nf);
       else
-        hipLaunchKernelGGL(k_rs_nl_apply<32>, g, dim3(256), 0, rs->stream, rs->d_pf, cl, rs->d_atab,
+        hipLaunchKernelGGL(k_rs_nl_apply<32>, g, dim3(256), 0, rs->stream, u, cl, rs->d_atab,
                            rs->d_apt_off, rs->d_idx, rs->d_val, rs->d_coef, nbc, rs->Nf);
       PW_HIP(hipGetLastError());
     }
@@ -1441,9 +1460,11 @@ extern "C" int pw_rs_mult_spinor(pw_rs* rs, int nb, const double* psi, double* a
   for (int b0 = 0; b0 < nb; b0 += rs->chunk) {
     const int nbc = std::min(rs->chunk, nb - b0);
     if (rs->own_fft) {
-      PW_TRY(own_to_fine(rs, psi + (long)b0 * rs->Nc, nbc));
-      PW_TRY(local_terms(rs, nbc));
-      PW_TRY(own_to_coarse(rs, a3 + (long)b0 * rs->Nc, nbc));
+      const bool nl = rs->natom > 0 && rs->nproj_tot > 0;
+      const bool fuse_v = !nl || rs->ncolor > 0;
+      PW_TRY(own_to_fine(rs, psi + (long)b0 * rs->Nc, nbc, fuse_v));
+      PW_TRY(local_terms(rs, nbc, fuse_v));
+      PW_TRY(own_to_coarse(rs, fuse_v ? rs->d_pv : rs->d_pf, a3 + (long)b0 * rs->Nc, nbc));
       continue;
     }
     RsPlans* p;
@@ -1454,7 +1475,7 @@ extern "C" int pw_rs_mult_spinor(pw_rs* rs, int nb, const double* psi, double* a
                        1.0 / (double)rs->Nc);
     PW_HIP(hipGetLastError());
     PW_TRY(exec(rs, p->f_inv, rs->d_ff, rs->d_pf));
-    PW_TRY(local_terms(rs, nbc));
+    PW_TRY(local_terms(rs, nbc, false));
     PW_TRY(exec(rs, p->f_fwd, rs->d_pf, rs->d_ff));
     hipLaunchKernelGGL(k_rs_f2c_ekin, dim3(nc[1] * nc[2], nbc), dim3(kRowThreads), 0, rs->stream, rs->d_xc,
                        rs->d_ff, rs->d_ekin, nc[0], nc[1], nc[2], nf[1], nf[2], rs->hc0, rs->hf0, rs->Hc,
