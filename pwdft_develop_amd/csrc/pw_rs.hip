@@ -715,16 +715,13 @@ __global__ __launch_bounds__(256) void k_rs_x_c2r(const double2* __restrict__ in
     if (t < P * n && g < npair) {
       const double2* r0 = in + b * in_bs + (long)(2 * g) * hw;
       const double2* r1 = r0 + hw;
-      double2 z1 = make_double2(0.0, 0.0), z2 = z1;
-      if (e < hw) {
-        z1 = r0[e];
-        z2 = r1[e];
-      } else if (2 * e > n && n - e < hw) {
-        const double2 a0 = r0[n - e], a1 = r1[n - e];
-        z1 = make_double2(a0.x, -a0.y);
-        z2 = make_double2(a1.x, -a1.y);
+      // one load path: entry k of each half row, conjugated above n/2
+      const int k = e < hw ? e : ((2 * e > n && n - e < hw) ? n - e : -1);
+      if (k >= 0) {
+        const double sg = e < hw ? 1.0 : -1.0;
+        const double2 a0 = r0[k], a1 = r1[k];
+        z = make_double2(a0.x - sg * a1.y, sg * a0.y + a1.x);
       }
-      z = make_double2(z1.x - z2.y, z1.y + z2.x);
     }
     vt[it] = z;
   }
