@@ -5,24 +5,27 @@
 // (src/utility.cpp:3150-3276) and the sparse real-space projectors of
 // PeriodTable::CalculateNonlocalPP (src/SCF/periodtable.cpp:1042-1589).
 //
-// Per chunk of bands (all on the handle's stream):
-//   1. coarse R2C of psi                                   (rocFFT)
-//   2. k_rs_c2f: coarse spectrum -> fine spectrum, written whole (zeros
-//      included) as a gather, so no memset and no scatter conflicts
-//   3. fine C2R                                            (rocFFT)
-//   4. k_rs_weights: w[p, b] = vol/Nf * wgt_p * sum_i val_p[i] psi_b[idx[i]]
-//   5. k_rs_vmul (u = V psi, in place), then k_rs_nl_apply once per colour
-//      of a colouring of the atoms by shared points (u += projector terms);
-//      k_rs_local does both through a point-major index of the projector
-//      entries when an atom lists a point twice
-//   6. fine R2C                                            (rocFFT)
-//   7. k_rs_f2c_ekin: ekin .* coarse spectrum + restriction of the fine one
-//   8. coarse C2R straight into a3                         (rocFFT)
-// The FFTs are plain full-grid transforms of any length (the reference's own
-// decks have prime factors 31 and 17), hence the library; every step between
-// them is a kernel here.  FFTWExecute's vol/N and 1/vol factors
-// (src/SCF/fourier.cpp:695-760) cancel pairwise and are folded into steps 2
-// and 7 as 1/Nc and 1/Nf.
+// Per batch of bands (all on the handle's stream), when every grid length
+// factors into 2, 3, 5 and at most one prime 7..31 (the reference's decks:
+// 62 = 2*31, 124 = 4*31, 88 = 8*11, 102 = 2*3*17):
+//   1. coarse R2C of psi: k_rs_x_r2c (rows), k_rs_pencil along y and z
+//   2. fine C2R restricted to the coarse band: k_rs_zinv_c2f (inverse z of
+//      the live columns, gathered from the coarse spectrum by IP_c2f's
+//      rules), k_rs_pencil along y, k_rs_x_c2r (writing psi and V psi)
+//   3. k_rs_weights: w[p, b] = vol/Nf * wgt_p * sum_i val_p[i] psi_b[idx[i]]
+//   4. k_rs_nl_apply once per colour of a colouring of the atoms by shared
+//      points (V psi += projector terms); k_rs_vmul + k_rs_local (V psi and
+//      the projector terms through a point-major index of the entries) when
+//      an atom lists a point twice
+//   5. fine R2C keeping the coarse band: k_rs_x_r2c, k_rs_pencil along y and
+//      z (live rows and planes only)
+//   6. k_rs_f2c_ekin_p: IP_f2c as a gather on that pruned spectrum plus the
+//      kinetic term
+//   7. coarse C2R: k_rs_pencil along z and y, k_rs_x_c2r straight into a3.
+// Other lengths take rocFFT's full-grid transforms around the unpruned
+// gathers k_rs_c2f / k_rs_f2c_ekin.  FFTWExecute's vol/N and 1/vol factors
+// (src/SCF/fourier.cpp:695-760) cancel pairwise and are folded into the
+// gathers as 1/Nc and 1/Nf.
 #include <rocfft/rocfft.h>
 
 #include <algorithm>
