@@ -305,6 +305,13 @@ int pw_rs_set_nonlocal(pw_rs* rs, int natom, const int* npts, const int* nproj,
  * IP_f2c, src/utility.cpp:3150-3276), a3 overwritten; psi is not modified. */
 int pw_rs_mult_spinor(pw_rs* rs, int nb, const double* psi, double* a3);
 
+/* The wavefunction ecut filter of that MultSpinor (set_wfn_filter,
+ * include/hamiltonian.hpp:347-352; applied to psi on the first call,
+ * src/SCF/hamiltonian.cpp:3090-3116, and to a3 on every call, :3240-3265):
+ * x <- C2R(mask .* R2C(x)) in place, zeroing the R2C entries whose |G|^2/2
+ * (the handle's ekin, i.e. Fourier::gkkR2C) exceeds cutoff. */
+int pw_rs_filter(pw_rs* rs, double cutoff, int nb, double* x);
+
 #ifdef __cplusplus
 }
 #endif

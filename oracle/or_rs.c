@@ -11,6 +11,9 @@
  *                              Fourier&), src/SCF/hamiltonian.cpp:3058-3125,
  *                              with Spinor::AddMultSpinorFineR2C (real a3),
  *                              src/SCF/spinor.cpp:376-525
+ *   or_rs_filter               the wavefunction ecut filter of that
+ *                              MultSpinor (hamiltonian.cpp:3090-3116,
+ *                              3240-3265)
  *
  * Scalings are FFTWExecute's (src/SCF/fourier.cpp:695-760): forward
  * transforms are multiplied by vol/N, backward ones by 1/vol.
@@ -187,5 +190,33 @@ int or_rs_mult_spinor(const int* Nc, const int* Nf, double vol, const double* ek
     for (long r = 0; r < ntot; r++) y[r] = tmp[r] * (1.0 / vol);
   }
   free(outc); free(outf); free(psiF); free(upd); free(tmp);
+  return rc;
+}
+
+/* Wavefunction ecut filter of the real MultSpinor (hamiltonian.cpp:3090-3116
+ * on psi, :3240-3265 on a3): R2C (x vol/ntot), zero every entry with
+ * gkkR2C > cutoff, C2R (x 1/vol), in place on nb bands. */
+int or_rs_filter(const int* Nc, double vol, const double* gkk, double cutoff, int nb, double* x) {
+  const long ntot = (long)Nc[0] * Nc[1] * Nc[2];
+  const long nr2c = (long)(Nc[0] / 2 + 1) * Nc[1] * Nc[2];
+  double* outc = malloc(sizeof(double) * 2 * nr2c);
+  double* tmp = malloc(sizeof(double) * ntot);
+  if (!outc || !tmp) {
+    free(outc); free(tmp);
+    return -1;
+  }
+  int rc = 0;
+  for (int b = 0; b < nb && rc == 0; b++) {
+    double* y = x + (long)b * ntot;
+    rc |= or_fft_r2c(Nc, y, outc);
+    for (long g = 0; g < nr2c; g++) {
+      const double f = gkk[g] > cutoff ? 0.0 : vol / (double)ntot;
+      outc[2 * g] *= f;
+      outc[2 * g + 1] *= f;
+    }
+    rc |= or_fft_c2r(Nc, outc, tmp);
+    for (long r = 0; r < ntot; r++) y[r] = tmp[r] * (1.0 / vol);
+  }
+  free(outc); free(tmp);
   return rc;
 }

@@ -65,6 +65,7 @@ int or_rs_mult_spinor(const int* Nc, const int* Nf, double vol, const double* ek
                       const double* vtot, int natom, const int* npts, const int* nproj,
                       const int* idx, const double* val, const double* wgt, int nb,
                       const double* psi, double* a3);
+int or_rs_filter(const int* Nc, double vol, const double* gkk, double cutoff, int nb, double* x);
 
 /* ---------- Hamiltonian ---------- */
 typedef struct or_ham or_ham;

@@ -76,6 +76,7 @@ def lib():
         L.or_ip_f2c.restype = None
         L.or_rs_mult_spinor.argtypes = [_ip, _ip, C.c_double, _dp, _dp, C.c_int, _ip, _ip, _ip, _dp,
                                         _dp, C.c_int, _dp, _dp]
+        L.or_rs_filter.argtypes = [_ip, C.c_double, _dp, C.c_double, C.c_int, _dp]
         _LIB = L
     return _LIB
 
@@ -380,4 +381,14 @@ def rs_mult_spinor(nc, nf, vol, ekin, vtot, npts, nproj, idx, val, wgt, psi):
                                  npts if npts.size else np.zeros(1, np.int32),
                                  nproj if nproj.size else np.zeros(1, np.int32), idx, val, wgt, nb,
                                  psi.reshape(-1), out.reshape(-1)), "rs_mult_spinor")
+    return out
+
+
+def rs_filter(nc, vol, gkk, cutoff, psi):
+    """Wavefunction ecut filter of the real MultSpinor (hamiltonian.cpp:
+    3090-3116, 3240-3265) on (nb, ntot) real bands; returns a filtered copy."""
+    nc = np.asarray(nc, np.int32)
+    out = np.array(psi, np.float64, copy=True, order="C")
+    _chk(lib().or_rs_filter(nc, float(vol), np.ascontiguousarray(gkk, np.float64), float(cutoff),
+                            out.shape[0], out.reshape(-1)), "rs_filter")
     return out

@@ -91,6 +91,7 @@ def lib():
         "pw_rs_set_vtot": [_vp, _dp],
         "pw_rs_set_nonlocal": [_vp, C.c_int, _vp, _vp, _vp, _vp, _vp],
         "pw_rs_mult_spinor": [_vp, C.c_int, _vp, _vp],
+        "pw_rs_filter": [_vp, C.c_double, C.c_int, _vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)

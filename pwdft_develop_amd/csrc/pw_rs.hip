@@ -861,6 +861,21 @@ __global__ __launch_bounds__(kRowThreads) void k_rs_f2c_ekin_p(double2* __restri
   }
 }
 
+// Wavefunction ecut filter on the coarse spectrum (hamiltonian.cpp:3104-3109
+// and :3254-3259): entries with |G|^2/2 (ekin) above the cutoff are zeroed,
+// the rest scaled by 1/Nc (the forward vol/N and backward 1/vol factors).
+__global__ __launch_bounds__(256) void k_rs_cutmask(double2* __restrict__ xc, const double* __restrict__ ekin,
+                                                   double cutoff, long Hc, double scale) {
+  const long g = (long)blockIdx.x * 256 + threadIdx.x;
+  if (g >= Hc) return;
+  const long b = blockIdx.y;
+  const double f = ekin[g] > cutoff ? 0.0 : scale;
+  double2 v = xc[b * Hc + g];
+  v.x *= f;
+  v.y *= f;
+  xc[b * Hc + g] = v;
+}
+
 // ---------------------------------------------------------------- host
 
 namespace {
@@ -1331,9 +1346,9 @@ int launch_x(pw_rs* rs, bool c2r, const void* in, void* out, int n, int hw, int 
   return PW_OK;
 }
 
-// coarse psi -> xc (coarse spectrum) -> pf (psi on the fine grid)
-int own_to_fine(pw_rs* rs, const double* psi, int nbc, bool fuse_v) {
-  const int *nc = rs->nc, *nf = rs->nf;
+// coarse psi -> xc (unscaled coarse spectrum)
+int own_coarse_fwd(pw_rs* rs, const double* psi, int nbc) {
+  const int* nc = rs->nc;
   const int hc = rs->hc0;
   const long Hc = rs->Hc;
   PW_TRY(launch_x(rs, false, psi, rs->d_cv, nc[0], hc, nc[1] * nc[2], rs->Nc, Hc, nbc));
@@ -1343,6 +1358,30 @@ int own_to_fine(pw_rs* rs, const double* psi, int nbc, bool fuse_v) {
   PenArgs z{rs->d_cw, Hc, hc, (long)nc[1] * hc, nullptr, rs->d_xc, Hc, hc, (long)nc[1] * hc, nullptr,
             nc[2], nc[1], hc, nc[2], 0, -1.0, nullptr};
   PW_TRY(launch_pencil(rs, z, nbc));
+  return PW_OK;
+}
+
+// xc -> real coarse bands (unscaled inverse)
+int own_coarse_inv(pw_rs* rs, double* a3, int nbc) {
+  const int* nc = rs->nc;
+  const int hc = rs->hc0;
+  const long Hc = rs->Hc;
+  PenArgs z{rs->d_xc, Hc, hc, (long)nc[1] * hc, nullptr, rs->d_cv, Hc, hc, (long)nc[1] * hc, nullptr,
+            nc[2], nc[1], hc, nc[2], 0, 1.0, nullptr};
+  PW_TRY(launch_pencil(rs, z, nbc));
+  PenArgs y{rs->d_cv, Hc, (long)nc[1] * hc, hc, nullptr, rs->d_cw, Hc, (long)nc[1] * hc, hc, nullptr,
+            nc[1], nc[2], hc, nc[1], 0, 1.0, nullptr};
+  PW_TRY(launch_pencil(rs, y, nbc));
+  PW_TRY(launch_x(rs, true, rs->d_cw, a3, nc[0], hc, nc[1] * nc[2], Hc, rs->Nc, nbc));
+  return PW_OK;
+}
+
+// coarse psi -> xc (coarse spectrum) -> pf (psi on the fine grid)
+int own_to_fine(pw_rs* rs, const double* psi, int nbc, bool fuse_v) {
+  const int *nc = rs->nc, *nf = rs->nf;
+  const int hc = rs->hc0;
+  const long Hc = rs->Hc;
+  PW_TRY(own_coarse_fwd(rs, psi, nbc));
   // fine inverse: z (with the coarse-to-fine gather), y, x
   const long Tzb = (long)nf[2] * rs->ny * hc, Tfb = (long)nf[2] * nf[1] * hc;
   {
@@ -1383,15 +1422,7 @@ int own_to_coarse(pw_rs* rs, const double* fine, double* a3, int nbc) {
                      rs->d_u0, rs->d_ekin, rs->d_ymap, rs->d_zmap, nc[0], nc[1], nc[2], nf[1], nf[2], hc, rs->ny,
                      Hc, U0b, 1.0 / (double)rs->Nc, 1.0 / (double)rs->Nf);
   PW_HIP(hipGetLastError());
-  // coarse inverse: z, y, x
-  PenArgs z{rs->d_xc, Hc, hc, (long)nc[1] * hc, nullptr, rs->d_cv, Hc, hc, (long)nc[1] * hc, nullptr,
-            nc[2], nc[1], hc, nc[2], 0, 1.0, nullptr};
-  PW_TRY(launch_pencil(rs, z, nbc));
-  PenArgs y{rs->d_cv, Hc, (long)nc[1] * hc, hc, nullptr, rs->d_cw, Hc, (long)nc[1] * hc, hc, nullptr,
-            nc[1], nc[2], hc, nc[1], 0, 1.0, nullptr};
-  PW_TRY(launch_pencil(rs, y, nbc));
-  PW_TRY(launch_x(rs, true, rs->d_cw, a3, nc[0], hc, nc[1] * nc[2], Hc, rs->Nc, nbc));
-  return PW_OK;
+  return own_coarse_inv(rs, a3, nbc);
 }
 
 // Projector weights (from psi on the fine grid, pf), then V psi and the
@@ -1482,6 +1513,33 @@ extern "C" int pw_rs_mult_spinor(pw_rs* rs, int nb, const double* psi, double* a
                        rs->Hf, 1.0 / (double)rs->Nc, 1.0 / (double)rs->Nf);
     PW_HIP(hipGetLastError());
     PW_TRY(exec(rs, p->c_inv, rs->d_xc, a3 + (long)b0 * rs->Nc));
+  }
+  return PW_OK;
+}
+
+extern "C" int pw_rs_filter(pw_rs* rs, double cutoff, int nb, double* x) {
+  if (!rs || nb < 0 || (nb > 0 && !x)) return fail(PW_ERR_ARG, "pw_rs_filter: bad argument");
+  int dev;
+  PW_HIP(hipGetDevice(&dev));
+  if (dev != rs->device) return fail(PW_ERR_STATE, "pw_rs_filter: handle belongs to device %d", rs->device);
+  for (int b0 = 0; b0 < nb; b0 += rs->chunk) {
+    const int nbc = std::min(rs->chunk, nb - b0);
+    double* xb = x + (long)b0 * rs->Nc;
+    RsPlans* p = nullptr;
+    if (rs->own_fft) {
+      PW_TRY(own_coarse_fwd(rs, xb, nbc));
+    } else {
+      PW_TRY(get_plans(rs, nbc, &p));
+      PW_TRY(exec(rs, p->c_fwd, xb, rs->d_xc));
+    }
+    hipLaunchKernelGGL(k_rs_cutmask, dim3((unsigned)((rs->Hc + 255) / 256), nbc), dim3(256), 0, rs->stream,
+                       rs->d_xc, rs->d_ekin, cutoff, rs->Hc, 1.0 / (double)rs->Nc);
+    PW_HIP(hipGetLastError());
+    if (rs->own_fft) {
+      PW_TRY(own_coarse_inv(rs, xb, nbc));
+    } else {
+      PW_TRY(exec(rs, p->c_inv, rs->d_xc, xb));
+    }
   }
   return PW_OK;
 }

@@ -442,8 +442,17 @@ class KohnSham : public Hamiltonian {
     Check(pw_rs_set_nonlocal(rs_, (int)pseudo.size(), npts.data(), nproj.data(), idx.data(), val.data(),
                              wgt.data()));
   }
-  // MultSpinor(Spinor&, NumTns<Real>&, Fourier&) (hamiltonian.cpp:3058-3125)
-  // with AddMultSpinorFineR2C (spinor.cpp:376-525): a3 = H psi.
+  // set_wfn_filter (include/hamiltonian.hpp:347-356)
+  void set_wfn_filter(int apply_filter, int apply_first, Real wfn_cutoff) {
+    apply_filter_ = apply_filter;
+    apply_first_ = apply_first;
+    wfn_cutoff_ = wfn_cutoff;
+  }
+  void set_wfn_filter(int apply_first) { apply_first_ = apply_first; }
+  // MultSpinor(Spinor&, NumTns<Real>&, Fourier&) (hamiltonian.cpp:3058-3125,
+  // 3240-3265) with AddMultSpinorFineR2C (spinor.cpp:376-525): a3 = H psi;
+  // with the wavefunction filter on, psi is filtered in place on the first
+  // call and a3 on every call.
   void MultSpinor(Spinor& psi, NumTns<Real>& a3, Fourier& fft) override {
     if (!fft.isInitialized) ErrorHandling("Fourier is not prepared.");
     const Int ntot = psi.Wavefun().m(), nb = psi.Wavefun().p();
@@ -453,8 +462,14 @@ class KohnSham : public Hamiltonian {
     pw_rs* rs = fft.RsCtx();
     DeviceBlock x, y;
     x.UploadReal(psi.Wavefun().Data(), (size_t)ntot * nb);
+    if (apply_filter_ && apply_first_) {
+      apply_first_ = 0;
+      Check(pw_rs_filter(rs, wfn_cutoff_, nb, static_cast<double*>(x.Ptr())));
+      x.DownloadReal(psi.Wavefun().Data(), (size_t)ntot * nb);
+    }
     y.Alloc(((size_t)ntot * nb + 1) / 2);
     Check(pw_rs_mult_spinor(rs, nb, static_cast<const double*>(x.Ptr()), static_cast<double*>(y.Ptr())));
+    if (apply_filter_) Check(pw_rs_filter(rs, wfn_cutoff_, nb, static_cast<double*>(y.Ptr())));
     y.DownloadReal(a3.Data(), (size_t)ntot * nb);
   }
   DblNumVec& EkinR2C() { return ekinR2C_; }
@@ -467,6 +482,8 @@ class KohnSham : public Hamiltonian {
  private:
   pw_ctx* ctx_ = nullptr;
   pw_rs* rs_ = nullptr;
+  bool apply_filter_ = false, apply_first_ = false;
+  Real wfn_cutoff_ = 0.0;
   DblNumVec ekin_, teter_, vtot_, eigVal_, ekinR2C_, teterR2C_;
   std::pair<CpxNumMat, DblNumVec> vnlc_;
 };

@@ -274,6 +274,13 @@ class RealSpaceContext:
                                            C.c_void_p(out.data_ptr())))
         return out
 
+    def filter(self, x, cutoff):
+        """In-place wavefunction ecut filter (pw_rs_filter) on (nb, ntot)."""
+        if x.dtype != self.torch.float64 or x.device != self.device or not x.is_contiguous():
+            raise ValueError("x must be a contiguous float64 tensor on the handle's device")
+        check(_lib.lib().pw_rs_filter(self.h, float(cutoff), x.shape[0], C.c_void_p(x.data_ptr())))
+        return x
+
     def close(self):
         if getattr(self, "h", None):
             _lib.lib().pw_rs_destroy(self.h)

@@ -11,6 +11,7 @@
 //         eigenvalues[nb] (3 iterations), ChebFSI eigenvalues[nb] (one
 //         GeneralChebyStep after the LOBPCG, degree 10, 8 Lanczos steps)
 #include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -23,12 +24,15 @@ static void rd(FILE* f, T* p, size_t n) {
   if (fread(p, sizeof(T), n, f) != n) ErrorHandling("short read");
 }
 
-// adapter_driver rs <input.bin> <output.bin>: the real-space dual-grid path
+// adapter_driver rs <input.bin> <output.bin> [cutoff]: the real-space
+// dual-grid path, MultSpinor called twice (with the wavefunction filter at
+// cutoff when given)
 // input: int32 n[3] nf[3] nb natom, then per atom npts nproj; float64
 //        supercell[9]; vtotFine[Nf]; per atom int32 idx[npts], float64
 //        val[npts nproj] (column-major), wgt[nproj]; psi[ntot nb]
-// output: float64 a3[ntot nb], ekin on the R2C grid
-static int run_realspace(const char* in, const char* outp) {
+// output: float64 a3[ntot nb] (first call), a3 (second call), psi after the
+//         calls, ekin on the R2C grid
+static int run_realspace(const char* in, const char* outp, double cutoff) {
   FILE* f = fopen(in, "rb");
   if (!f) ErrorHandling("cannot open input");
   int hdr[8];
@@ -74,21 +78,23 @@ static int run_realspace(const char* in, const char* outp) {
   ham.SetPseudo(pseudo);
   Spinor spn;
   spn.SetupReal(fft.domain, 1, nb, false, psi.data());
-  NumTns<Real> a3(ntot, 1, nb);
+  if (cutoff > 0.0) ham.set_wfn_filter(1, 1, cutoff);
+  NumTns<Real> a3(ntot, 1, nb), a3b(ntot, 1, nb);
   ham.MultSpinor(spn, a3, fft);
+  ham.MultSpinor(spn, a3b, fft);  // second call: psi is filtered only once
   FILE* g = fopen(outp, "wb");
   const size_t n3 = (size_t)ntot * nb, ne = (size_t)ham.EkinR2C().m();
-  if (!g || fwrite(a3.Data(), sizeof(double), n3, g) != n3 ||
-      fwrite(ham.EkinR2C().Data(), sizeof(double), ne, g) != ne)
+  if (!g || fwrite(a3.Data(), sizeof(double), n3, g) != n3 || fwrite(a3b.Data(), sizeof(double), n3, g) != n3 ||
+      fwrite(psi.data(), sizeof(double), n3, g) != n3 || fwrite(ham.EkinR2C().Data(), sizeof(double), ne, g) != ne)
     ErrorHandling("cannot write output");
   fclose(g);
   return 0;
 }
 
 int main(int argc, char** argv) {
-  if (argc == 4 && std::string(argv[1]) == "rs") {
+  if ((argc == 4 || argc == 5) && std::string(argv[1]) == "rs") {
     try {
-      return run_realspace(argv[2], argv[3]);
+      return run_realspace(argv[2], argv[3], argc == 5 ? atof(argv[4]) : 0.0);
     } catch (const std::exception& e) {
       fprintf(stderr, "ERROR: %s\n", e.what());
       return 1;

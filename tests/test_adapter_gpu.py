@@ -54,10 +54,12 @@ def test_adapter_driver_reports_errors(tmp_path):
     assert r.returncode == 1 and "ERROR" in r.stderr
 
 
-@pytest.mark.parametrize("name", ["box_rs", "sic8_rs"])
-def test_adapter_driver_realspace(tmp_path, name):
+@pytest.mark.parametrize("name,cutoff", [("box_rs", 0.0), ("sic8_rs", 0.0), ("sic8_rs", 12.0)])
+def test_adapter_driver_realspace(tmp_path, name, cutoff):
     """Spinor::SetupReal, KohnSham::CalculateEkinR2C / SetVtotFine / SetPseudo
-    and the NumTns<Real> MultSpinor overload, from C++, against the oracle."""
+    and the NumTns<Real> MultSpinor overload, from C++, against the oracle;
+    with set_wfn_filter on, psi is filtered on the first call only and a3 on
+    every call (hamiltonian.cpp:3090-3116, 3240-3265)."""
     p = R.RS_WORKLOADS[name](nb=3).build()
     offs = np.concatenate([[0], np.cumsum(p.npts)])
     voffs = np.concatenate([[0], np.cumsum(p.npts.astype(np.int64) * p.nproj)])
@@ -75,12 +77,21 @@ def test_adapter_driver_realspace(tmp_path, name):
         p.psi.astype(np.float64).tofile(fh)
     out = tmp_path / "rs_out.bin"
     print(f"case: adapter driver realspace {name}", flush=True)
-    r = subprocess.run([DRIVER, "rs", str(inp), str(out)], capture_output=True, text=True, timeout=120)
+    args = [DRIVER, "rs", str(inp), str(out)] + ([repr(cutoff)] if cutoff > 0 else [])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     res = np.fromfile(out, np.float64)
     n = p.nb * p.ntot
-    a3, ekin = res[:n].reshape(p.nb, p.ntot), res[n:]
+    a3, a3b = res[:n].reshape(p.nb, p.ntot), res[n:2 * n].reshape(p.nb, p.ntot)
+    psi_after, ekin = res[2 * n:3 * n].reshape(p.nb, p.ntot), res[3 * n:]
     assert np.abs(ekin - p.ekin).max() < 1e-12 * p.ekin.max()
+    psi_in = p.psi
+    if cutoff > 0:
+        psi_in = O.rs_filter(p.grid, p.volume, p.ekin, cutoff, p.psi)
+    assert np.abs(psi_after - psi_in).max() < 1e-12 * np.abs(psi_in).max()
     ref = O.rs_mult_spinor(p.grid, p.grid_fine, p.volume, p.ekin, p.vtot, p.npts, p.nproj, p.idx, p.val,
-                           p.wgt, p.psi)
+                           p.wgt, psi_in)
+    if cutoff > 0:
+        ref = O.rs_filter(p.grid, p.volume, p.ekin, cutoff, ref)
     assert np.abs(a3 - ref).max() < 1e-12 * np.abs(ref).max()
+    assert np.abs(a3b - ref).max() < 1e-12 * np.abs(ref).max()

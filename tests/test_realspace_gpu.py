@@ -151,6 +151,21 @@ def test_rs_sic_large_all_bands_linear():
     ctx.close()
 
 
+@pytest.mark.parametrize("name,cutoff,fft", [("sic8_rs", 12.0, None), ("box_rs", 8.0, None),
+                                             ("sic8_rs", 12.0, "rocfft")])
+def test_rs_wavefunction_filter(name, cutoff, fft, monkeypatch):
+    import torch
+    if fft:
+        monkeypatch.setenv("PW_RS_FFT", fft)
+    p = _prob(name, 5)
+    ref = O.rs_filter(p.grid, p.volume, p.ekin, cutoff, p.psi)
+    ctx = R.RealSpaceContext(p)
+    x = torch.from_numpy(p.psi.copy()).to("cuda:0")
+    got = ctx.filter(x, cutoff).cpu().numpy()
+    ctx.close()
+    assert _rel(got, ref) < TOL
+
+
 def test_rs_errors():
     import torch  # noqa: F401  (the HIP runtime)
     L = _lib.lib()

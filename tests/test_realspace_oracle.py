@@ -161,3 +161,26 @@ def test_workload_shapes():
     assert q.grid == (22, 22, 22) and q.grid_fine == (44, 44, 44)
     m = R.mos2_rs()
     assert m.grid == (88, 102, 80) and m.grid_fine == (176, 204, 160) and len(m.atoms) == 108
+
+
+def test_wavefunction_filter_is_the_cutoff_projector():
+    """or_rs_filter (hamiltonian.cpp:3090-3116, 3240-3265): plane waves with
+    |G|^2/2 at or below the cutoff pass unchanged, those above vanish; the
+    filter is a symmetric idempotent projector."""
+    nc = (8, 6, 10)
+    L = np.array([5.0, 4.0, 6.0])
+    sc = np.diag(L)
+    vol = float(np.prod(L))
+    gkk = R.ekin_r2c(nc, sc)
+    k = 2 * np.pi / L
+    x, y, z = _grid_points(nc, L)
+    low = np.cos(k[0] * x + k[1] * y)                     # |G|^2/2 = 1.72
+    high = np.cos(3 * k[0] * x - 2 * k[2] * z)            # |G|^2/2 = 15.5
+    cut = 0.5 * ((k[0]) ** 2 + (k[1]) ** 2) + 1.0
+    out = O.rs_filter(nc, vol, gkk, cut, np.stack([low.reshape(-1), high.reshape(-1)]))
+    assert np.abs(out[0] - low.reshape(-1)).max() < 1e-13
+    assert np.abs(out[1]).max() < 1e-13
+    N = int(np.prod(nc))
+    P = O.rs_filter(nc, vol, gkk, 6.0, np.eye(N))
+    assert np.abs(P - P.T).max() < 1e-13
+    assert np.abs(P @ P - P).max() < 1e-12
