@@ -312,6 +312,15 @@ int pw_rs_mult_spinor(pw_rs* rs, int nb, const double* psi, double* a3);
  * (the handle's ekin, i.e. Fourier::gkkR2C) exceeds cutoff. */
 int pw_rs_filter(pw_rs* rs, double cutoff, int nb, double* x);
 
+/* KohnSham::CalculateDensity, real-space branch, one density component
+ * (src/SCF/hamiltonian.cpp:2453-2627): rho = sum_b num_spin occ_b psiF_b^2 on
+ * the fine grid (psiF the fine-grid interpolant of band b), renormalised to
+ * rho *= num_spin num_occupied Nf / (vol sum rho) (:2599-2600); *val (may be
+ * NULL) = sum(rho) vol / Nf after it.  psi: device, nb x ntot; occ: host,
+ * nb; rho: device, ntotFine doubles. */
+int pw_rs_calculate_density(pw_rs* rs, int nb, const double* psi, const double* occ, int num_spin,
+                            int num_occupied, double* rho, double* val);
+
 #ifdef __cplusplus
 }
 #endif

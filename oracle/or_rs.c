@@ -14,6 +14,8 @@
  *   or_rs_filter               the wavefunction ecut filter of that
  *                              MultSpinor (hamiltonian.cpp:3090-3116,
  *                              3240-3265)
+ *   or_rs_density              KohnSham::CalculateDensity, real-space
+ *                              branch (hamiltonian.cpp:2453-2627)
  *
  * Scalings are FFTWExecute's (src/SCF/fourier.cpp:695-760): forward
  * transforms are multiplied by vol/N, backward ones by 1/vol.
@@ -218,5 +220,51 @@ int or_rs_filter(const int* Nc, double vol, const double* gkk, double cutoff, in
     for (long r = 0; r < ntot; r++) y[r] = tmp[r] * (1.0 / vol);
   }
   free(outc); free(tmp);
+  return rc;
+}
+
+/* KohnSham::CalculateDensity, real-space branch, one density component
+ * (hamiltonian.cpp:2453-2627): rho = sum_b num_spin occ_b psiF_b^2 on the
+ * fine grid, then rho *= num_spin num_occ Nf / (vol sum rho) and *val =
+ * sum(rho) vol / Nf.  The reference interpolates with a C2C transform and
+ * the idxFineGrid padding (:2526-2534) and squares the real part; that real
+ * part is sqrt(Nc/Nf) times the IP_c2f interpolant used here, a factor the
+ * renormalisation removes. */
+int or_rs_density(const int* Nc, const int* Nf, double vol, int nb, const double* psi, const double* occ,
+                  int num_spin, int num_occ, double* rho, double* val) {
+  const long ntot = (long)Nc[0] * Nc[1] * Nc[2], ntotF = (long)Nf[0] * Nf[1] * Nf[2];
+  const long nr2c = (long)(Nc[0] / 2 + 1) * Nc[1] * Nc[2];
+  const long nr2cF = (long)(Nf[0] / 2 + 1) * Nf[1] * Nf[2];
+  double* outc = malloc(sizeof(double) * 2 * nr2c);
+  double* outf = malloc(sizeof(double) * 2 * nr2cF);
+  double* psiF = malloc(sizeof(double) * ntotF);
+  if (!outc || !outf || !psiF) {
+    free(outc); free(outf); free(psiF);
+    return -1;
+  }
+  int rc = 0;
+  memset(rho, 0, sizeof(double) * ntotF);
+  for (int b = 0; b < nb && rc == 0; b++) {
+    rc |= or_fft_r2c(Nc, psi + (long)b * ntot, outc);
+    for (long g = 0; g < 2 * nr2c; g++) outc[g] *= vol / (double)ntot;
+    memset(outf, 0, sizeof(double) * 2 * nr2cF);
+    or_ip_c2f(Nc, Nf, outc, outf);
+    rc |= or_fft_c2r(Nf, outf, psiF);
+    const double fac = num_spin * occ[b];
+    for (long r = 0; r < ntotF; r++) {
+      const double v = psiF[r] * (1.0 / vol);
+      rho[r] += v * v * fac;
+    }
+  }
+  double s = 0.0;
+  for (long r = 0; r < ntotF; r++) s += rho[r];
+  const double scale = (num_spin * (double)num_occ * (double)ntotF) / (vol * s);
+  double s2 = 0.0;
+  for (long r = 0; r < ntotF; r++) {
+    rho[r] *= scale;
+    s2 += rho[r] * vol / ntotF;
+  }
+  if (val) *val = s2;
+  free(outc); free(outf); free(psiF);
   return rc;
 }

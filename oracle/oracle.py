@@ -77,6 +77,8 @@ def lib():
         L.or_rs_mult_spinor.argtypes = [_ip, _ip, C.c_double, _dp, _dp, C.c_int, _ip, _ip, _ip, _dp,
                                         _dp, C.c_int, _dp, _dp]
         L.or_rs_filter.argtypes = [_ip, C.c_double, _dp, C.c_double, C.c_int, _dp]
+        L.or_rs_density.argtypes = [_ip, _ip, C.c_double, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp,
+                                    C.POINTER(C.c_double)]
         _LIB = L
     return _LIB
 
@@ -392,3 +394,16 @@ def rs_filter(nc, vol, gkk, cutoff, psi):
     _chk(lib().or_rs_filter(nc, float(vol), np.ascontiguousarray(gkk, np.float64), float(cutoff),
                             out.shape[0], out.reshape(-1)), "rs_filter")
     return out
+
+
+def rs_density(nc, nf, vol, psi, occ, num_spin, num_occ):
+    """Real-space CalculateDensity (hamiltonian.cpp:2453-2627): (rho on the
+    fine grid, val)."""
+    nc, nf = np.asarray(nc, np.int32), np.asarray(nf, np.int32)
+    psi = np.ascontiguousarray(psi, np.float64)
+    rho = np.zeros(int(np.prod(nf)))
+    val = C.c_double(0.0)
+    _chk(lib().or_rs_density(nc, nf, float(vol), psi.shape[0], psi.reshape(-1),
+                             np.ascontiguousarray(occ, np.float64), int(num_spin), int(num_occ), rho,
+                             C.byref(val)), "rs_density")
+    return rho, val.value

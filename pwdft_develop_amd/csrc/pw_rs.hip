@@ -101,6 +101,8 @@ struct pw_rs {
   double2* d_cv = nullptr;      // coarse [z][y][i] scratch
   double2* d_cw = nullptr;
   double* d_pv = nullptr;       // V psi on the fine grid (fused local term)
+  double* d_f = nullptr;        // density weights of a batch, then sums
+  double* d_part = nullptr;     // partial sums (kSumBlocks)
   int* d_ymap = nullptr;        // fine y -> live row or -1 (f1)
   int* d_zmap = nullptr;        // fine z -> live plane or -1 (f2)
   int* d_lmap = nullptr;        // live row -> fine y (ny)
@@ -876,6 +878,60 @@ __global__ __launch_bounds__(256) void k_rs_cutmask(double2* __restrict__ xc, co
   xc[b * Hc + g] = v;
 }
 
+// Density on the fine grid (KohnSham::CalculateDensity, real-space branch,
+// hamiltonian.cpp:2453-2627): rho (+)= sum_b f_b psiF_b^2, bands in order.
+__global__ __launch_bounds__(256) void k_rs_rho_acc(const double* __restrict__ pf, const double* __restrict__ f,
+                                                   double* __restrict__ rho, int nbc, long Nf, int first) {
+  const long r = (long)blockIdx.x * 256 + threadIdx.x;
+  if (r >= Nf) return;
+  double acc = first ? 0.0 : rho[r];
+  for (int b = 0; b < nbc; b++) {
+    const double v = pf[(long)b * Nf + r];
+    acc += v * v * f[b];
+  }
+  rho[r] = acc;
+}
+
+// Deterministic sum of n doubles: each of the 1024 workgroups sums its
+// stride-1024 share (fixed order, DPP wavefront sums), then one workgroup
+// adds the partials in order.  scale != 0: x *= scale[0] first (in place).
+constexpr int kSumBlocks = 1024;
+__global__ __launch_bounds__(256) void k_rs_sum_partial(double* __restrict__ x, long n,
+                                                       const double* __restrict__ scale, double* __restrict__ part) {
+  __shared__ double sw[4];
+  const double sc = scale ? scale[0] : 1.0;
+  double acc = 0.0;
+  for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)kSumBlocks * 256) {
+    double v = x[i];
+    if (scale) {
+      v *= sc;
+      x[i] = v;
+    }
+    acc += v;
+  }
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = (sw[0] + sw[1]) + (sw[2] + sw[3]);
+}
+
+// out[0] = sum of the partials; with want_scale: out[1] = c / out[0] (the
+// renormalisation factor), else out[1] = out[0] * c.
+__global__ __launch_bounds__(256) void k_rs_sum_final(const double* __restrict__ part, double c, int want_scale,
+                                                     double* __restrict__ out) {
+  __shared__ double sw[4];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < kSumBlocks; i += 256) acc += part[i];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double s = (sw[0] + sw[1]) + (sw[2] + sw[3]);
+    out[0] = s;
+    out[1] = want_scale ? c / s : s * c;
+  }
+}
+
 // ---------------------------------------------------------------- host
 
 namespace {
@@ -1172,6 +1228,8 @@ extern "C" int pw_rs_destroy(pw_rs* rs) {
   hipFree(rs->d_cv);
   hipFree(rs->d_cw);
   hipFree(rs->d_pv);
+  hipFree(rs->d_f);
+  hipFree(rs->d_part);
   hipFree(rs->d_ymap);
   hipFree(rs->d_zmap);
   hipFree(rs->d_lmap);
@@ -1541,5 +1599,59 @@ extern "C" int pw_rs_filter(pw_rs* rs, double cutoff, int nb, double* x) {
       PW_TRY(exec(rs, p->c_inv, rs->d_xc, xb));
     }
   }
+  return PW_OK;
+}
+
+extern "C" int pw_rs_calculate_density(pw_rs* rs, int nb, const double* psi, const double* occ, int num_spin,
+                                       int num_occupied, double* rho, double* val) {
+  if (!rs || nb < 1 || !psi || !occ || !rho || num_spin < 1 || num_occupied < 1)
+    return fail(PW_ERR_ARG, "pw_rs_calculate_density: bad argument");
+  int dev;
+  PW_HIP(hipGetDevice(&dev));
+  if (dev != rs->device) return fail(PW_ERR_STATE, "pw_rs_calculate_density: handle belongs to device %d", rs->device);
+  if (!rs->d_f) PW_HIP(hipMalloc((void**)&rs->d_f, sizeof(double) * (size_t)std::max(rs->chunk, 4)));
+  if (!rs->d_part) PW_HIP(hipMalloc((void**)&rs->d_part, sizeof(double) * kSumBlocks));
+  const int* nc = rs->nc;
+  const int* nf = rs->nf;
+  std::vector<double> f((size_t)rs->chunk);
+  for (int b0 = 0; b0 < nb; b0 += rs->chunk) {
+    const int nbc = std::min(rs->chunk, nb - b0);
+    const double* xb = psi + (long)b0 * rs->Nc;
+    if (rs->own_fft) {
+      PW_TRY(own_to_fine(rs, xb, nbc, false));
+    } else {
+      RsPlans* p;
+      PW_TRY(get_plans(rs, nbc, &p));
+      PW_TRY(exec(rs, p->c_fwd, xb, rs->d_xc));
+      hipLaunchKernelGGL(k_rs_c2f, dim3(nf[1] * nf[2], nbc), dim3(kRowThreads), 0, rs->stream, rs->d_xc,
+                         rs->d_ff, nc[0], nc[1], nc[2], nf[1], nf[2], rs->hc0, rs->hf0, rs->Hc, rs->Hf,
+                         1.0 / (double)rs->Nc);
+      PW_HIP(hipGetLastError());
+      PW_TRY(exec(rs, p->f_inv, rs->d_ff, rs->d_pf));
+    }
+    for (int b = 0; b < nbc; b++) f[(size_t)b] = num_spin * occ[b0 + b];
+    // the previous batch's accumulation reads d_f: copy in stream order
+    PW_HIP(hipMemcpyAsync(rs->d_f, f.data(), sizeof(double) * nbc, hipMemcpyHostToDevice, rs->stream));
+    PW_HIP(hipStreamSynchronize(rs->stream));
+    hipLaunchKernelGGL(k_rs_rho_acc, dim3((unsigned)((rs->Nf + 255) / 256)), dim3(256), 0, rs->stream, rs->d_pf,
+                       rs->d_f, rho, nbc, rs->Nf, b0 == 0 ? 1 : 0);
+    PW_HIP(hipGetLastError());
+  }
+  // rho *= num_spin num_occ Nf / (vol sum rho) (hamiltonian.cpp:2599-2600),
+  // then val = sum rho vol / Nf (:2607-2610)
+  const double c = (double)num_spin * (double)num_occupied * (double)rs->Nf / rs->vol;
+  double* sums = rs->d_f;  // [sum, scale, sum2, val]
+  hipLaunchKernelGGL(k_rs_sum_partial, dim3(kSumBlocks), dim3(256), 0, rs->stream, rho, rs->Nf,
+                     (const double*)nullptr, rs->d_part);
+  hipLaunchKernelGGL(k_rs_sum_final, dim3(1), dim3(256), 0, rs->stream, (const double*)rs->d_part, c, 1, sums);
+  hipLaunchKernelGGL(k_rs_sum_partial, dim3(kSumBlocks), dim3(256), 0, rs->stream, rho, rs->Nf,
+                     (const double*)(sums + 1), rs->d_part);
+  hipLaunchKernelGGL(k_rs_sum_final, dim3(1), dim3(256), 0, rs->stream, (const double*)rs->d_part,
+                     rs->vol / (double)rs->Nf, 0, sums + 2);
+  PW_HIP(hipGetLastError());
+  double h[4];
+  PW_HIP(hipMemcpyAsync(h, sums, sizeof(double) * 4, hipMemcpyDeviceToHost, rs->stream));
+  PW_HIP(hipStreamSynchronize(rs->stream));
+  if (val) *val = h[3];
   return PW_OK;
 }

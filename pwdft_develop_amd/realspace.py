@@ -281,6 +281,16 @@ class RealSpaceContext:
         check(_lib.lib().pw_rs_filter(self.h, float(cutoff), x.shape[0], C.c_void_p(x.data_ptr())))
         return x
 
+    def calculate_density(self, psi, occ, num_spin, num_occupied):
+        """Fine-grid density (pw_rs_calculate_density): (rho tensor, val)."""
+        t = self.torch
+        rho = t.empty(self.prob.ntot_fine, dtype=t.float64, device=self.device)
+        val = C.c_double(0.0)
+        check(_lib.lib().pw_rs_calculate_density(self.h, psi.shape[0], C.c_void_p(psi.data_ptr()),
+                                                 np.ascontiguousarray(occ, np.float64), int(num_spin),
+                                                 int(num_occupied), C.c_void_p(rho.data_ptr()), C.byref(val)))
+        return rho, val.value
+
     def close(self):
         if getattr(self, "h", None):
             _lib.lib().pw_rs_destroy(self.h)

@@ -166,6 +166,22 @@ def test_rs_wavefunction_filter(name, cutoff, fft, monkeypatch):
     assert _rel(got, ref) < TOL
 
 
+@pytest.mark.parametrize("name,fft", [("sic8_rs", None), ("box_rs", None), ("sic8_rs", "rocfft")])
+def test_rs_calculate_density(name, fft, monkeypatch):
+    import torch
+    if fft:
+        monkeypatch.setenv("PW_RS_FFT", fft)
+    p = _prob(name, 5)
+    occ = np.array([1.0, 1.0, 0.9, 0.4, 0.1])
+    rho_ref, val_ref = O.rs_density(p.grid, p.grid_fine, p.volume, p.psi, occ, 2, 5)
+    ctx = R.RealSpaceContext(p, max_bands=2)     # three batches
+    rho, val = ctx.calculate_density(torch.from_numpy(p.psi).to("cuda:0"), occ, 2, 5)
+    got = rho.cpu().numpy()
+    ctx.close()
+    assert _rel(got, rho_ref) < TOL
+    assert val == pytest.approx(val_ref, rel=1e-13) and val == pytest.approx(10.0, rel=1e-12)
+
+
 def test_rs_errors():
     import torch  # noqa: F401  (the HIP runtime)
     L = _lib.lib()

@@ -184,3 +184,28 @@ def test_wavefunction_filter_is_the_cutoff_projector():
     P = O.rs_filter(nc, vol, gkk, 6.0, np.eye(N))
     assert np.abs(P - P.T).max() < 1e-13
     assert np.abs(P @ P - P).max() < 1e-12
+
+
+def test_density_matches_the_references_c2c_padding():
+    """or_rs_density interpolates with IP_c2f; the reference pads the C2C
+    spectrum with idxFineGrid (fourier.cpp:315-339) and squares the real
+    part (hamiltonian.cpp:2500-2534).  Restated here with numpy's FFTs, the
+    two agree after the renormalisation, Nyquist planes included."""
+    nc, nf = (6, 8, 4), (10, 16, 8)
+    vol = 120.0
+    rng = np.random.default_rng(4)
+    psi = rng.standard_normal((3, int(np.prod(nc))))
+    occ = np.array([1.0, 0.75, 0.5])
+    rho, val = O.rs_density(nc, nf, vol, psi, occ, 2, 3)
+    # the reference's path
+    acc = np.zeros((nf[2], nf[1], nf[0]))
+    for b in range(3):
+        X = np.fft.fftn(psi[b].reshape(nc[2], nc[1], nc[0])) * (vol / np.prod(nc))
+        F = np.zeros((nf[2], nf[1], nf[0]), complex)
+        idx = [np.where(np.arange(n) <= n // 2, np.arange(n), m - n + np.arange(n)) for n, m in zip(nc, nf)]
+        F[np.ix_(idx[2], idx[1], idx[0])] = X * np.sqrt(np.prod(nc) / np.prod(nf))
+        pf = np.fft.ifftn(F) * np.prod(nf) / vol
+        acc += np.abs(pf.real) ** 2 * 2 * occ[b]
+    ref = acc.reshape(-1) * (2 * 3 * np.prod(nf)) / (vol * acc.sum())
+    assert np.abs(rho - ref).max() < 1e-13 * ref.max()
+    assert val == pytest.approx(6.0, rel=1e-13)
