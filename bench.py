@@ -42,6 +42,16 @@ def slab_bytes(prob, nstick, nb, p):
     return nb * prob.grid[2] * nstick * 2 * p * 2 + prob.ngrid * p
 
 
+def slab_flops(prob, kxmax, nb):
+    """Nominal FFT flops (5 n log2 n per complex length-n transform) of one
+    xy-slab launch: per z plane the y transforms of the kx <= kxmax columns
+    and the x transforms of the row pairs, both ways, and the V multiply."""
+    import math
+    n1, n2, n3 = prob.grid
+    f = lambda n: 5.0 * n * math.log2(n)
+    return nb * n3 * (2 * (kxmax + 1) * f(n2) + 2 * (n2 // 2) * f(n1) + n1 * n2)
+
+
 def staged_bytes(prob, p):
     """Per-band bytes of the dense-FFT staged model p*(9 npw + 6 Nh + 5 N)."""
     n1, n2, n3 = prob.grid
@@ -220,6 +230,7 @@ def main():
     # model over the three local launches is reported beside it.
     slab_ms = stages["f32"][1]
     slab_b = slab_bytes(prob, info["nstick"], nb, 4)
+    slab_fl = slab_flops(prob, info["kxmax"], nb)
     achieved = slab_b / (slab_ms * 1e-3) / 1e9
     # FP32 square grids of the register-FFT sizes take k_slab_xy_reg (8 bands
     # per z plane), others the LDS kernel k_slab_xy
@@ -261,7 +272,15 @@ def main():
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": slab_b, "launch_ms": round(slab_ms, 4),
-                     "model": "per launch: nb x (n3 x nstick complex sticks read + written) + V read"},
+                     "model": "per launch: nb x (n3 x nstick complex sticks read + written) + V read",
+                     # the kernel's own bound: FP32 VALU (packed FP32 is off in this build, so
+                     # 78.6 TFLOP/s is the FMA peak it can reach)
+                     "valu": {"achieved_tflops": round(slab_fl / (slab_ms * 1e-3) / 1e12, 2),
+                              "peak_tflops_nonpacked_fp32": 78.6,
+                              "frac": round(slab_fl / (slab_ms * 1e-3) / 1e12 / 78.6, 4),
+                              "flops_per_launch": slab_fl,
+                              "model": "5 n log2 n per complex transform: y on kx <= kxmax columns, "
+                                       "x on row pairs, both ways, plus V multiply"}},
         "staged_model": {"kernels": f"k_sticks_inverse_z + {kname.split('<')[0]} + k_sticks_forward_z",
                          "bytes_per_band": "p (9 npw + 6 Nh + 5 N)",
                          "f32": {"GBs": round(st32, 1), "frac": round(st32 / HBM_PEAK_GBS, 4),
