@@ -312,6 +312,12 @@ int pw_rs_mult_spinor(pw_rs* rs, int nb, const double* psi, double* a3);
  * (the handle's ekin, i.e. Fourier::gkkR2C) exceeds cutoff. */
 int pw_rs_filter(pw_rs* rs, double cutoff, int nb, double* x);
 
+/* Spinor::AddTeterPrecond, real version (src/SCF/spinor.cpp:309-345):
+ * a3 += C2R(teter .* R2C(psi)) with the reference's scalings; teter is
+ * KohnSham::CalculateEkin's b / (b + 16 a^4), a = 2 ekin, from the handle's
+ * ekin (src/SCF/hamiltonian.cpp:2659-2663).  psi, a3: device, nb x ntot. */
+int pw_rs_teter_precond(pw_rs* rs, int nb, const double* psi, double* a3);
+
 /* KohnSham::CalculateDensity, real-space branch, one density component
  * (src/SCF/hamiltonian.cpp:2453-2627): rho = sum_b num_spin occ_b psiF_b^2 on
  * the fine grid (psiF the fine-grid interpolant of band b), renormalised to

@@ -16,6 +16,8 @@
  *                              3240-3265)
  *   or_rs_density              KohnSham::CalculateDensity, real-space
  *                              branch (hamiltonian.cpp:2453-2627)
+ *   or_rs_teter_precond        Spinor::AddTeterPrecond, real version
+ *                              (spinor.cpp:309-345)
  *
  * Scalings are FFTWExecute's (src/SCF/fourier.cpp:695-760): forward
  * transforms are multiplied by vol/N, backward ones by 1/vol.
@@ -266,5 +268,32 @@ int or_rs_density(const int* Nc, const int* Nf, double vol, int nb, const double
   }
   if (val) *val = s2;
   free(outc); free(outf); free(psiF);
+  return rc;
+}
+
+/* Spinor::AddTeterPrecond, real version (spinor.cpp:309-345): per band
+ * R2C (x vol/ntot), times teter on the R2C grid, C2R (x 1/vol), a3 += . */
+int or_rs_teter_precond(const int* Nc, double vol, const double* teter, int nb, const double* psi, double* a3) {
+  const long ntot = (long)Nc[0] * Nc[1] * Nc[2];
+  const long nr2c = (long)(Nc[0] / 2 + 1) * Nc[1] * Nc[2];
+  double* outc = malloc(sizeof(double) * 2 * nr2c);
+  double* tmp = malloc(sizeof(double) * ntot);
+  if (!outc || !tmp) {
+    free(outc); free(tmp);
+    return -1;
+  }
+  int rc = 0;
+  for (int b = 0; b < nb && rc == 0; b++) {
+    rc |= or_fft_r2c(Nc, psi + (long)b * ntot, outc);
+    for (long g = 0; g < nr2c; g++) {
+      const double f = vol / (double)ntot * teter[g];
+      outc[2 * g] *= f;
+      outc[2 * g + 1] *= f;
+    }
+    rc |= or_fft_c2r(Nc, outc, tmp);
+    double* y = a3 + (long)b * ntot;
+    for (long r = 0; r < ntot; r++) y[r] += tmp[r] * (1.0 / vol);
+  }
+  free(outc); free(tmp);
   return rc;
 }

@@ -66,6 +66,7 @@ int or_rs_mult_spinor(const int* Nc, const int* Nf, double vol, const double* ek
                       const int* idx, const double* val, const double* wgt, int nb,
                       const double* psi, double* a3);
 int or_rs_filter(const int* Nc, double vol, const double* gkk, double cutoff, int nb, double* x);
+int or_rs_teter_precond(const int* Nc, double vol, const double* teter, int nb, const double* psi, double* a3);
 int or_rs_density(const int* Nc, const int* Nf, double vol, int nb, const double* psi, const double* occ,
                   int num_spin, int num_occ, double* rho, double* val);
 

@@ -77,6 +77,7 @@ def lib():
         L.or_rs_mult_spinor.argtypes = [_ip, _ip, C.c_double, _dp, _dp, C.c_int, _ip, _ip, _ip, _dp,
                                         _dp, C.c_int, _dp, _dp]
         L.or_rs_filter.argtypes = [_ip, C.c_double, _dp, C.c_double, C.c_int, _dp]
+        L.or_rs_teter_precond.argtypes = [_ip, C.c_double, _dp, C.c_int, _dp, _dp]
         L.or_rs_density.argtypes = [_ip, _ip, C.c_double, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp,
                                     C.POINTER(C.c_double)]
         _LIB = L
@@ -407,3 +408,13 @@ def rs_density(nc, nf, vol, psi, occ, num_spin, num_occ):
                              np.ascontiguousarray(occ, np.float64), int(num_spin), int(num_occ), rho,
                              C.byref(val)), "rs_density")
     return rho, val.value
+
+
+def rs_teter_precond(nc, vol, teter, psi, a3):
+    """Real Spinor::AddTeterPrecond (spinor.cpp:309-345): returns a3 + T psi."""
+    nc = np.asarray(nc, np.int32)
+    psi = np.ascontiguousarray(psi, np.float64)
+    out = np.array(a3, np.float64, copy=True, order="C")
+    _chk(lib().or_rs_teter_precond(nc, float(vol), np.ascontiguousarray(teter, np.float64), psi.shape[0],
+                                   psi.reshape(-1), out.reshape(-1)), "rs_teter_precond")
+    return out

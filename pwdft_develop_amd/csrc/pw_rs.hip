@@ -70,6 +70,7 @@ struct pw_rs {
   hipStream_t stream;
   int device;
   double* d_ekin = nullptr;
+  double* d_teter = nullptr;    // b / (b + 16 a^4), a = 2 ekin (CalculateEkin)
   double* d_vtot = nullptr;
   // projectors
   int natom = 0, nproj_tot = 0, npmax = 0;
@@ -932,6 +933,26 @@ __global__ __launch_bounds__(256) void k_rs_sum_final(const double* __restrict__
   }
 }
 
+// Teter preconditioner on the coarse spectrum (Spinor::AddTeterPrecond,
+// real version, spinor.cpp:309-345): xc *= teter / Nc.
+__global__ __launch_bounds__(256) void k_rs_teter_spec(double2* __restrict__ xc, const double* __restrict__ teter,
+                                                      long Hc, double scale) {
+  const long g = (long)blockIdx.x * 256 + threadIdx.x;
+  if (g >= Hc) return;
+  const long b = blockIdx.y;
+  const double f = teter[g] * scale;
+  double2 v = xc[b * Hc + g];
+  v.x *= f;
+  v.y *= f;
+  xc[b * Hc + g] = v;
+}
+
+// y += x over n doubles (the a3 accumulation of AddTeterPrecond)
+__global__ __launch_bounds__(256) void k_rs_axpy(double* __restrict__ y, const double* __restrict__ x, long n) {
+  const long i = (long)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) y[i] += x[i];
+}
+
 // ---------------------------------------------------------------- host
 
 namespace {
@@ -1188,6 +1209,15 @@ extern "C" int pw_rs_create(pw_rs** out, const int grid[3], const int grid_fine[
   };
   if (hipGetDevice(&rs->device) != hipSuccess) return bail(fail(PW_ERR_HIP, "pw_rs_create: hipGetDevice failed"));
   if ((rc = upload_rs(&rs->d_ekin, ekin, (size_t)rs->Hc)) != PW_OK) return bail(rc);
+  {
+    // KohnSham::CalculateEkin's teter_ from the same ekin (hamiltonian.cpp:2659-2663)
+    std::vector<double> t((size_t)rs->Hc);
+    for (long g = 0; g < rs->Hc; g++) {
+      const double a = ekin[g] * 2.0, b = 27.0 + a * (18.0 + a * (12.0 + a * 8.0));
+      t[(size_t)g] = b / (b + 16.0 * pow(a, 4.0));
+    }
+    if ((rc = upload_rs(&rs->d_teter, t.data(), t.size())) != PW_OK) return bail(rc);
+  }
   if (hipMalloc((void**)&rs->d_xc, sizeof(double2) * rs->Hc * rs->chunk) != hipSuccess ||
       hipMalloc((void**)&rs->d_pf, sizeof(double) * rs->Nf * rs->chunk) != hipSuccess)
     return bail(fail(PW_ERR_NOMEM, "pw_rs_create: cannot allocate %d-band chunk buffers", rs->chunk));
@@ -1217,6 +1247,7 @@ extern "C" int pw_rs_destroy(pw_rs* rs) {
   if (rs->info) rocfft_execution_info_destroy(rs->info);
   free_nonlocal(rs);
   hipFree(rs->d_ekin);
+  hipFree(rs->d_teter);
   hipFree(rs->d_vtot);
   hipFree(rs->d_xc);
   hipFree(rs->d_ff);
@@ -1653,5 +1684,37 @@ extern "C" int pw_rs_calculate_density(pw_rs* rs, int nb, const double* psi, con
   PW_HIP(hipMemcpyAsync(h, sums, sizeof(double) * 4, hipMemcpyDeviceToHost, rs->stream));
   PW_HIP(hipStreamSynchronize(rs->stream));
   if (val) *val = h[3];
+  return PW_OK;
+}
+
+extern "C" int pw_rs_teter_precond(pw_rs* rs, int nb, const double* psi, double* a3) {
+  if (!rs || nb < 0 || (nb > 0 && (!psi || !a3))) return fail(PW_ERR_ARG, "pw_rs_teter_precond: bad argument");
+  int dev;
+  PW_HIP(hipGetDevice(&dev));
+  if (dev != rs->device) return fail(PW_ERR_STATE, "pw_rs_teter_precond: handle belongs to device %d", rs->device);
+  for (int b0 = 0; b0 < nb; b0 += rs->chunk) {
+    const int nbc = std::min(rs->chunk, nb - b0);
+    const double* xb = psi + (long)b0 * rs->Nc;
+    RsPlans* p = nullptr;
+    if (rs->own_fft) {
+      PW_TRY(own_coarse_fwd(rs, xb, nbc));
+    } else {
+      PW_TRY(get_plans(rs, nbc, &p));
+      PW_TRY(exec(rs, p->c_fwd, xb, rs->d_xc));
+    }
+    hipLaunchKernelGGL(k_rs_teter_spec, dim3((unsigned)((rs->Hc + 255) / 256), nbc), dim3(256), 0, rs->stream,
+                       rs->d_xc, rs->d_teter, rs->Hc, 1.0 / (double)rs->Nc);
+    PW_HIP(hipGetLastError());
+    // the fine-grid scratch (Nf >= Nc per band) takes T psi before the add
+    if (rs->own_fft) {
+      PW_TRY(own_coarse_inv(rs, rs->d_pf, nbc));
+    } else {
+      PW_TRY(exec(rs, p->c_inv, rs->d_xc, rs->d_pf));
+    }
+    const long n = (long)nbc * rs->Nc;
+    hipLaunchKernelGGL(k_rs_axpy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, rs->stream,
+                       a3 + (long)b0 * rs->Nc, (const double*)rs->d_pf, n);
+    PW_HIP(hipGetLastError());
+  }
   return PW_OK;
 }

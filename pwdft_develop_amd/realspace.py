@@ -61,6 +61,14 @@ def ekin_r2c(grid, supercell):
     return (0.5 * np.sum(G * G, axis=-1)).reshape(-1)
 
 
+def teter_of(ekin):
+    """Teter preconditioner b / (b + 16 a^4), a = 2 ekin, b = 27 + a (18 + a
+    (12 + 8 a)) (KohnSham::CalculateEkin, hamiltonian.cpp:2659-2663)."""
+    a = 2.0 * np.asarray(ekin, np.float64)
+    b = 27.0 + a * (18.0 + a * (12.0 + a * 8.0))
+    return b / (b + 16.0 * a ** 4)
+
+
 def sphere_points(pos_frac, rzero, nf, supercell):
     """Fine-grid indices within rzero of an atom and their Cartesian offsets,
     in the order of periodtable.cpp:1063-1101 (k, then j, then i)."""
@@ -273,6 +281,12 @@ class RealSpaceContext:
         check(_lib.lib().pw_rs_mult_spinor(self.h, psi.shape[0], C.c_void_p(psi.data_ptr()),
                                            C.c_void_p(out.data_ptr())))
         return out
+
+    def teter_precond(self, psi, a3):
+        """a3 += T psi (pw_rs_teter_precond), in place on a3."""
+        check(_lib.lib().pw_rs_teter_precond(self.h, psi.shape[0], C.c_void_p(psi.data_ptr()),
+                                             C.c_void_p(a3.data_ptr())))
+        return a3
 
     def filter(self, x, cutoff):
         """In-place wavefunction ecut filter (pw_rs_filter) on (nb, ntot)."""

@@ -182,6 +182,22 @@ def test_rs_calculate_density(name, fft, monkeypatch):
     assert val == pytest.approx(val_ref, rel=1e-13) and val == pytest.approx(10.0, rel=1e-12)
 
 
+@pytest.mark.parametrize("name,fft", [("sic8_rs", None), ("box_rs", None), ("sic8_rs", "rocfft")])
+def test_rs_teter_precond(name, fft, monkeypatch):
+    import torch
+    if fft:
+        monkeypatch.setenv("PW_RS_FFT", fft)
+    p = _prob(name, 5)
+    a3 = np.random.default_rng(2).standard_normal(p.psi.shape)
+    ref = O.rs_teter_precond(p.grid, p.volume, R.teter_of(p.ekin), p.psi, a3)
+    ctx = R.RealSpaceContext(p, max_bands=2)
+    y = torch.from_numpy(a3.copy()).to("cuda:0")
+    ctx.teter_precond(torch.from_numpy(p.psi).to("cuda:0"), y)
+    got = y.cpu().numpy()
+    ctx.close()
+    assert _rel(got, ref) < TOL
+
+
 def test_rs_errors():
     import torch  # noqa: F401  (the HIP runtime)
     L = _lib.lib()

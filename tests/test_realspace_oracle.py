@@ -209,3 +209,22 @@ def test_density_matches_the_references_c2c_padding():
     ref = acc.reshape(-1) * (2 * 3 * np.prod(nf)) / (vol * acc.sum())
     assert np.abs(rho - ref).max() < 1e-13 * ref.max()
     assert val == pytest.approx(6.0, rel=1e-13)
+
+
+def test_teter_preconditioner_on_plane_waves():
+    """or_rs_teter_precond (spinor.cpp:309-345) multiplies a plane wave by
+    the Teter factor b / (b + 16 a^4) of its own a = |G|^2 and adds it."""
+    nc = (8, 6, 10)
+    L = np.array([5.0, 4.0, 6.0])
+    vol = float(np.prod(L))
+    ekin = R.ekin_r2c(nc, np.diag(L))
+    teter = R.teter_of(ekin)
+    k = 2 * np.pi / L
+    x, y, z = _grid_points(nc, L)
+    G = np.array([2 * k[0], -k[1], k[2]])
+    pw = np.cos(G[0] * x + G[1] * y + G[2] * z).reshape(1, -1)
+    a = float(G @ G)
+    b = 27 + a * (18 + a * (12 + 8 * a))
+    base = np.full_like(pw, 0.25)
+    out = O.rs_teter_precond(nc, vol, teter, pw, base)
+    assert np.abs(out - (0.25 + b / (b + 16 * a ** 4) * pw)).max() < 1e-14
