@@ -397,7 +397,7 @@ __global__ __launch_bounds__(256) void k_rs_nl_apply(double* __restrict__ pf, co
 // ------------------------------------------------- dual-grid transforms
 //
 // Batched 1D transforms in LDS along one axis of a [band][a2][a1][a0]
-// array, P pencils per workgroup (row stride n + 1), radix 4, 2, 3, 5 from
+// array, P pencils per workgroup (row stride n), radix 4, 2, 3, 5 from
 // pw_fft.h and unrolled odd-prime radices 7..31 (the decks' 62 = 2*31 and
 // 124 = 4*31 points).  The fine transforms are pruned to the coarse band:
 // on the way in only the kx <= Nc0/2 half-columns, the coarse rows (plus the
@@ -584,7 +584,7 @@ __global__ __launch_bounds__(256) void k_rs_zinv_c2f(const double2* __restrict__
                                                     int f2, int hc, int ny, int P, long Hc, long Tzb,
                                                     double scale) {
   extern __shared__ double2 sm[];
-  const int n = f2, ld = n + 1;
+  const int n = f2, ld = n;
   double2* tw = sm;
   double2* A = sm + n;
   double2* B = A + P * ld;
@@ -656,7 +656,7 @@ struct PenArgs {
 template <int PR>
 __global__ __launch_bounds__(256) void k_rs_pencil(PenArgs a, const double* __restrict__ cs) {
   extern __shared__ double2 sm[];
-  const int n = a.n, ld = n + 1, P = a.P;
+  const int n = a.n, ld = n, P = a.P;
   double2* tw = sm;
   double2* A = sm + n;
   double2* B = A + P * ld;
@@ -704,7 +704,7 @@ __global__ __launch_bounds__(256) void k_rs_x_c2r(const double2* __restrict__ in
                                                  int n, int hw, int npair, long in_bs, long out_bs, int P,
                                                  const double* __restrict__ vloc, double* __restrict__ out2) {
   extern __shared__ double2 sm[];
-  const int ld = n + 1;
+  const int ld = n;
   double2* tw = sm;
   double2* A = sm + n;
   double2* B = A + P * ld;
@@ -768,7 +768,7 @@ __global__ __launch_bounds__(256) void k_rs_x_r2c(const double* __restrict__ in,
                                                  const double2* __restrict__ twg, const double* __restrict__ cs,
                                                  int n, int hw, int npair, long in_bs, long out_bs, int P) {
   extern __shared__ double2 sm[];
-  const int ld = n + 1;
+  const int ld = n;
   double2* tw = sm;
   double2* A = sm + n;
   double2* B = A + P * ld;
@@ -1088,10 +1088,12 @@ bool own_length(int n) { return big_prime(n) >= 0; }
     default: return fail(PW_ERR_UNSUPPORTED, "dual-grid transform: length without a kernel"); \
   }
 
-// pencils per workgroup: two P x (n + 1) buffers and the twiddles in 64 KiB
-int pencils(int n) { return std::max(1, std::min(64, (4096 - n) / (2 * (n + 1)))); }
+// pencils per workgroup: two P x n buffers (unpadded rows, so that a
+// workgroup's pencils cover whole 256-byte runs of the half rows) and the
+// twiddles in 64 KiB
+int pencils(int n) { return std::max(1, std::min(64, (4096 - n) / (2 * n))); }
 
-size_t pencil_lds(int n, int P) { return sizeof(double2) * ((size_t)n + 2 * (size_t)P * (n + 1)); }
+size_t pencil_lds(int n, int P) { return sizeof(double2) * ((size_t)n + 2 * (size_t)P * n); }
 
 void live_map(int c, int f, std::vector<int>& live, std::vector<int>& fmap) {
   // live index -> fine index, and fine index -> live index (or -1)
