@@ -58,6 +58,7 @@ struct AtomRec {
 };
 
 constexpr int kRowThreads = 64;
+constexpr int kNlBands = 16;  // bands per k_rs_nl_apply workgroup
 
 }  // namespace
 
@@ -346,7 +347,7 @@ __global__ __launch_bounds__(256) void k_rs_vmul(double* __restrict__ pf, const 
 // ... then, one launch per colour, each atom's projectors added at its own
 // points: pf[b, idx_i] += sum_p val[i, p] coef[p, b].  Atoms of one colour
 // share no point, so the read-modify-writes never meet.  Grid (atoms of the
-// colour, groups of 32 bands); the atom's coefficient block sits in LDS.
+// colour, groups of kNlBands bands); the atom's coefficient block sits in LDS.
 template <int NPMAX>
 __global__ __launch_bounds__(256) void k_rs_nl_apply(double* __restrict__ pf, const int* __restrict__ clist,
                                                     const AtomRec* __restrict__ atab,
@@ -354,7 +355,7 @@ __global__ __launch_bounds__(256) void k_rs_nl_apply(double* __restrict__ pf, co
                                                     const int* __restrict__ idx,
                                                     const double* __restrict__ val,
                                                     const double* __restrict__ coef, int nbc, long Nf) {
-  constexpr int BG = 32;
+  constexpr int BG = kNlBands;
   __shared__ double cl[NPMAX * BG];
   const int a = clist[blockIdx.x];
   const AtomRec rec = atab[a];
@@ -1549,7 +1550,7 @@ int local_terms(pw_rs* rs, int nbc, bool fused) {
     for (int c = 0; nl && c < rs->ncolor; c++) {
       const int na = rs->color_off[c + 1] - rs->color_off[c];
       const int* cl = rs->d_clist + rs->color_off[c];
-      const dim3 g(na, (nbc + 31) / 32);
+      const dim3 g(na, (nbc + kNlBands - 1) / kNlBands);
       if (rs->npmax <= 8)
         hipLaunchKernelGGL(k_rs_nl_apply<8>, g, dim3(256), 0, rs->stream, u, cl, rs->d_atab, rs->d_apt_off,
                            rs->d_idx, rs->d_val, rs->d_coef, nbc, rs->Nf);
