@@ -318,6 +318,14 @@ int pw_rs_filter(pw_rs* rs, double cutoff, int nb, double* x);
  * ekin (src/SCF/hamiltonian.cpp:2659-2663).  psi, a3: device, nb x ntot. */
 int pw_rs_teter_precond(pw_rs* rs, int nb, const double* psi, double* a3);
 
+/* EigenSolver::PPCGSolveReal, real-space branch (src/EigenSolver/ppcg.cpp:
+ * 683-1246): the pw_ppcg iteration on the handle's real coarse-grid bands X
+ * (device, nb x ntot, overwritten with the Ritz vectors) with
+ * pw_rs_mult_spinor as H and pw_rs_teter_precond as the preconditioner;
+ * host eigval, resval (nb), iterations in *iters. */
+int pw_rs_ppcg(pw_rs* rs, int nb, int numEig, int scfIter, int eigMaxIter, double eigMinTolerance,
+               double eigTolerance, int sbSize, double* X, double* eigval, double* resval, int* iters);
+
 /* KohnSham::CalculateDensity, real-space branch, one density component
  * (src/SCF/hamiltonian.cpp:2453-2627): rho = sum_b num_spin occ_b psiF_b^2 on
  * the fine grid (psiF the fine-grid interpolant of band b), renormalised to

@@ -287,12 +287,13 @@ static void pp_block_gram(int m, int sb, const double* a, const double* b, doubl
   free(T);
 }
 
-int or_ppcg(or_ham* H, int nb, int numEig, int scfIter, int eigMaxIter, double eigMinTolerance,
-            double eigTolerance, int sbSize, double* psi, double* eigval, double* resval, int* iters_out) {
-  (void)eigMinTolerance;
-  extern int or_ham_npw(const or_ham*);
-  const int npw = or_ham_npw(H);
-  const int m = 2 * npw, width = nb;
+/* The PPCG iteration on real m x nb blocks; hop(ctx, nb, x, y): y = H x;
+ * pop(ctx, nb, r, w): w += T r.  The sphere branch runs it on the (2 npw)-row
+ * real view, the real-space branch (ppcg.cpp:683-1246, the same steps on
+ * DblNumMat) on the ntot-row coarse-grid bands. */
+int or_ppcg_op(int m, or_op hop, or_op pop, void* ctx, int nb, int numEig, int scfIter, int eigMaxIter,
+               double eigTolerance, int sbSize, double* psi, double* eigval, double* resval, int* iters_out) {
+  const int width = nb;
   if (numEig > width) return -5; /* ppcg.cpp:105-112 */
   if (sbSize < 1) return -1;
   const size_t blk = (size_t)m * width;
@@ -314,10 +315,10 @@ int or_ppcg(or_ham* H, int nb, int numEig, int scfIter, int eigMaxIter, double e
 
   memcpy(X, psi, sizeof(double) * blk);
   if (scfIter == 1) {
-    rc = or_orthogonalize(npw, width, X);
+    rc = or_orthogonalize(m / 2, width, X);  /* Cholesky QR on the m rows (m even) */
     if (rc) goto done;
   }
-  or_apply_h(H, width, X, AX);
+  hop(ctx, width, X, AX);
   do {
     iter++;
     numSet = (iter == 1) ? 2 : 3;
@@ -334,8 +335,8 @@ int or_ppcg(or_ham* H, int nb, int numEig, int scfIter, int eigMaxIter, double e
       if (resNorm[k] > resMax) resMax = resNorm[k];
     /* W = T R, AW = A W */
     memset(W, 0, sizeof(double) * blk);
-    or_teter_precond(H, width, Xtemp, W);
-    or_apply_h(H, width, W, AW);
+    pop(ctx, width, Xtemp, W);
+    hop(ctx, width, W, AW);
     /* W -= X (X'W), AW -= AX (X'W); normalise */
     or_gemm_tn(m, width, width, X, W, XTX);
     or_gemm_nn(m, width, width, -1.0, X, m, XTX, width, 1.0, W, m);
@@ -437,4 +438,15 @@ done:
   free(S); free(AS); free(XTX); free(Xtemp); free(tmp); free(resNorm); free(eigValS);
   free(AMat); free(BMat); free(eigs);
   return rc;
+}
+
+static void ham_hop(void* ctx, int nb, const double* x, double* y) { or_apply_h((or_ham*)ctx, nb, x, y); }
+static void ham_pop(void* ctx, int nb, const double* r, double* w) { or_teter_precond((or_ham*)ctx, nb, r, w); }
+
+int or_ppcg(or_ham* H, int nb, int numEig, int scfIter, int eigMaxIter, double eigMinTolerance,
+            double eigTolerance, int sbSize, double* psi, double* eigval, double* resval, int* iters_out) {
+  (void)eigMinTolerance;
+  extern int or_ham_npw(const or_ham*);
+  return or_ppcg_op(2 * or_ham_npw(H), ham_hop, ham_pop, H, nb, numEig, scfIter, eigMaxIter, eigTolerance,
+                    sbSize, psi, eigval, resval, iters_out);
 }

@@ -18,6 +18,9 @@
  *                              branch (hamiltonian.cpp:2453-2627)
  *   or_rs_teter_precond        Spinor::AddTeterPrecond, real version
  *                              (spinor.cpp:309-345)
+ *   or_rs_ppcg                 EigenSolver::PPCGSolveReal, real-space branch
+ *                              (ppcg.cpp:683-1246): the PPCG iteration of
+ *                              or_eig.c over the operators above
  *
  * Scalings are FFTWExecute's (src/SCF/fourier.cpp:695-760): forward
  * transforms are multiplied by vol/N, backward ones by 1/vol.
@@ -295,5 +298,45 @@ int or_rs_teter_precond(const int* Nc, double vol, const double* teter, int nb, 
     for (long r = 0; r < ntot; r++) y[r] += tmp[r] * (1.0 / vol);
   }
   free(outc); free(tmp);
+  return rc;
+}
+
+typedef struct {
+  const int *Nc, *Nf;
+  double vol;
+  const double *ekin, *vtot, *teter;
+  int natom;
+  const int *npts, *nproj, *idx;
+  const double *val, *wgt;
+} rs_problem;
+
+static void rs_hop(void* ctx, int nb, const double* x, double* y) {
+  const rs_problem* p = (const rs_problem*)ctx;
+  or_rs_mult_spinor(p->Nc, p->Nf, p->vol, p->ekin, p->vtot, p->natom, p->npts, p->nproj, p->idx, p->val, p->wgt,
+                    nb, x, y);
+}
+
+static void rs_pop(void* ctx, int nb, const double* r, double* w) {
+  const rs_problem* p = (const rs_problem*)ctx;
+  or_rs_teter_precond(p->Nc, p->vol, p->teter, nb, r, w);
+}
+
+int or_rs_ppcg(const int* Nc, const int* Nf, double vol, const double* ekin, const double* vtot, int natom,
+               const int* npts, const int* nproj, const int* idx, const double* val, const double* wgt,
+               int nb, int numEig, int scfIter, int eigMaxIter, double eigTolerance, int sbSize, double* psi,
+               double* eigval, double* resval, int* iters) {
+  const long nr2c = (long)(Nc[0] / 2 + 1) * Nc[1] * Nc[2];
+  double* teter = malloc(sizeof(double) * nr2c);
+  if (!teter) return -1;
+  /* KohnSham::CalculateEkin's teter_ (hamiltonian.cpp:2659-2663) */
+  for (long g = 0; g < nr2c; g++) {
+    const double a = ekin[g] * 2.0, b = 27.0 + a * (18.0 + a * (12.0 + a * 8.0));
+    teter[g] = b / (b + 16.0 * a * a * a * a);
+  }
+  rs_problem p = {Nc, Nf, vol, ekin, vtot, teter, natom, npts, nproj, idx, val, wgt};
+  const long ntot = (long)Nc[0] * Nc[1] * Nc[2];
+  const int rc = or_ppcg_op((int)ntot, rs_hop, rs_pop, &p, nb, numEig, scfIter, eigMaxIter, eigTolerance, sbSize,
+                            psi, eigval, resval, iters);
+  free(teter);
   return rc;
 }

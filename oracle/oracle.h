@@ -133,6 +133,13 @@ int or_lobpcg(or_ham* H, int nb, int numEig, int scfIter, int eigMaxIter,
               double* eigval, double* resval, int* iters_out);
 
 /* EigenSolver::PPCGSolveReal sphere branch (ppcg.cpp:119-576). */
+typedef void (*or_op)(void* ctx, int nb, const double* x, double* y);
+int or_ppcg_op(int m, or_op hop, or_op pop, void* ctx, int nb, int numEig, int scfIter, int eigMaxIter,
+               double eigTolerance, int sbSize, double* psi, double* eigval, double* resval, int* iters_out);
+int or_rs_ppcg(const int* Nc, const int* Nf, double vol, const double* ekin, const double* vtot, int natom,
+               const int* npts, const int* nproj, const int* idx, const double* val, const double* wgt,
+               int nb, int numEig, int scfIter, int eigMaxIter, double eigTolerance, int sbSize, double* psi,
+               double* eigval, double* resval, int* iters);
 int or_ppcg(or_ham* H, int nb, int numEig, int scfIter, int eigMaxIter,
             double eigMinTolerance, double eigTolerance, int sbSize, double* psi,
             double* eigval, double* resval, int* iters_out);

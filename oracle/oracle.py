@@ -78,6 +78,9 @@ def lib():
                                         _dp, C.c_int, _dp, _dp]
         L.or_rs_filter.argtypes = [_ip, C.c_double, _dp, C.c_double, C.c_int, _dp]
         L.or_rs_teter_precond.argtypes = [_ip, C.c_double, _dp, C.c_int, _dp, _dp]
+        L.or_rs_ppcg.argtypes = [_ip, _ip, C.c_double, _dp, _dp, C.c_int, _ip, _ip, _ip, _dp, _dp, C.c_int,
+                                 C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _dp, _dp, _dp,
+                                 C.POINTER(C.c_int)]
         L.or_rs_density.argtypes = [_ip, _ip, C.c_double, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp,
                                     C.POINTER(C.c_double)]
         _LIB = L
@@ -418,3 +421,19 @@ def rs_teter_precond(nc, vol, teter, psi, a3):
     _chk(lib().or_rs_teter_precond(nc, float(vol), np.ascontiguousarray(teter, np.float64), psi.shape[0],
                                    psi.reshape(-1), out.reshape(-1)), "rs_teter_precond")
     return out
+
+
+def rs_ppcg(p, psi, num_eig, scf_iter, max_iter, tol, sb):
+    """Real-space PPCGSolveReal (ppcg.cpp:683-1246) on a RealSpaceProblem's
+    arrays: (X, eigenvalues, residual norms, iterations)."""
+    nc, nf = np.asarray(p.grid, np.int32), np.asarray(p.grid_fine, np.int32)
+    X = np.array(psi, np.float64, copy=True, order="C")
+    nb = X.shape[0]
+    ev, rv, it = np.zeros(nb), np.zeros(nb), C.c_int(0)
+    _chk(lib().or_rs_ppcg(nc, nf, float(p.volume), np.ascontiguousarray(p.ekin, np.float64),
+                          np.ascontiguousarray(p.vtot, np.float64), len(p.npts), np.ascontiguousarray(p.npts, np.int32),
+                          np.ascontiguousarray(p.nproj, np.int32), np.ascontiguousarray(p.idx, np.int32),
+                          np.ascontiguousarray(p.val, np.float64), np.ascontiguousarray(p.wgt, np.float64), nb,
+                          int(num_eig), int(scf_iter), int(max_iter), float(tol), int(sb), X.reshape(-1), ev, rv,
+                          C.byref(it)), "rs_ppcg")
+    return X, ev, rv, it.value

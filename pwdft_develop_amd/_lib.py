@@ -93,6 +93,8 @@ def lib():
         "pw_rs_mult_spinor": [_vp, C.c_int, _vp, _vp],
         "pw_rs_filter": [_vp, C.c_double, C.c_int, _vp],
         "pw_rs_teter_precond": [_vp, C.c_int, _vp, _vp],
+        "pw_rs_ppcg": [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, _vp, _dp, _dp,
+                       C.POINTER(C.c_int)],
         "pw_rs_calculate_density": [_vp, C.c_int, _vp, _dp, C.c_int, C.c_int, _vp, C.POINTER(C.c_double)],
     }
     for name, args in sig.items():

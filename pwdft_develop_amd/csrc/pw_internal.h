@@ -73,6 +73,11 @@ int fail(int code, const char* fmt, ...);
     if (rc_ != PW_OK) return rc_; \
   } while (0)
 
+// A linear-algebra-only context (stream, rocBLAS handle, GEMM / eigensolver
+// workspaces) bound to a real-space handle, for the solvers that run on its
+// bands; created on first use, freed with the handle.
+int rs_linalg_ctx(pw_rs* rs, int max_bands, pw_ctx** out);
+
 // ---- grid helpers shared by host setup and context creation ----
 inline int kgrid(int i, int n) { return i <= n / 2 ? i : i - n; }
 void recip_cell(const double* supercell, double* recip);

@@ -1,5 +1,7 @@
-// PPCG on gfx950: EigenSolver::PPCGSolveReal, sphere-cut branch
-// (src/EigenSolver/ppcg.cpp:119-576) on one GPU.
+// PPCG on gfx950: EigenSolver::PPCGSolveReal (src/EigenSolver/ppcg.cpp), the
+// sphere-cut branch (:119-576) on the (2 npw)-row real view of the bands and
+// the real-space branch (:683-1246, the same steps) on the ntot-row
+// coarse-grid bands, one GPU.
 //
 // S = (X | W | P) and AS = (AX | AW | AP) live in HBM as real (2 npw) x w
 // panels.  Per iteration: residual and Teter preconditioner, one H*W, the
@@ -14,6 +16,7 @@
 // of each sub-block eigenvector positive).
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <vector>
 
 #include <rocsolver/rocsolver.h>
@@ -124,14 +127,19 @@ struct PpcgWs {
 
 using namespace pw;
 
-extern "C" int pw_ppcg(pw_ctx* c, int nb, int numEig, int scfIter, int eigMaxIter, double eigMinTolerance,
-                       double eigTolerance, int sbSize, void* Xio, double* eigval, double* resval, int* iters) {
-  (void)eigMinTolerance;
-  if (!c || !Xio || !eigval || !resval || nb < 1 || nb > c->max_bands || sbSize < 1)
-    return fail(PW_ERR_ARG, "pw_ppcg: bad argument");
-  if (numEig > nb) return fail(PW_ERR_ARG, "Number of eigenvalues requested  = %d which is larger than the number "
-                                           "of columns in psi = %d", numEig, nb);
-  const int width = nb, m = 2 * c->npw;
+namespace {
+
+// The operator a PPCG run works with: y = H x, and w += T r (the Teter
+// preconditioner), on real m x nb device blocks.
+struct PpcgOps {
+  std::function<int(int, const double*, double*)> apply_h, precond;
+};
+
+// The PPCG iteration on real m-row blocks with the linear algebra of c
+// (stream, rocBLAS handle, GEMM and eigensolver workspaces).
+int ppcg_core(pw_ctx* c, int m, const PpcgOps& ops, int nb, int numEig, int scfIter, int eigMaxIter,
+              double eigTolerance, int sbSize, void* Xio, double* eigval, double* resval, int* iters) {
+  const int width = nb;
   const int sbs = std::min(sbSize, width);
   const int nsb = (width + sbs - 1) / sbs;
   const int sb3 = 3 * sbs;
@@ -192,7 +200,7 @@ extern "C" int pw_ppcg(pw_ctx* c, int nb, int numEig, int scfIter, int eigMaxIte
     return PW_OK;
   };
   if (scfIter == 1) TRY(chol_qr(false));
-  TRY(pw_apply_h(c, PW_F64, width, X, AX));
+  TRY(ops.apply_h(width, X, AX));
 
   int iter = 0;
   double resMax = 0.0;
@@ -213,8 +221,8 @@ extern "C" int pw_ppcg(pw_ctx* c, int nb, int numEig, int scfIter, int eigMaxIte
     resMax = *std::max_element(hres.begin(), hres.begin() + std::max(numEig, 1));
     // W = T R, AW = A W (ppcg.cpp:272-293)
     HIPTRY(hipMemsetAsync(Wb, 0, sizeof(double) * blk, s));
-    TRY(pw_teter_precond(c, PW_F64, width, W.Xtemp, Wb));
-    TRY(pw_apply_h(c, PW_F64, width, Wb, AW));
+    TRY(ops.precond(width, W.Xtemp, Wb));
+    TRY(ops.apply_h(width, Wb, AW));
     // W -= X (X'W), AW -= AX (X'W), normalise (ppcg.cpp:295-322)
     TRY(gemm_tn<double>(c, m, width, width, X, m, Wb, m, W.XTX, width, nullptr, 0.0));
     TRY(gemm_nn<double>(c, m, width, width, -1.0, X, m, W.XTX, width, 1.0, Wb, m));
@@ -332,4 +340,37 @@ extern "C" int pw_ppcg(pw_ctx* c, int nb, int numEig, int scfIter, int eigMaxIte
 #undef HIPTRY
 #undef BLASTRY
   return cleanup(PW_OK);
+}
+
+}  // namespace
+
+extern "C" int pw_ppcg(pw_ctx* c, int nb, int numEig, int scfIter, int eigMaxIter, double eigMinTolerance,
+                       double eigTolerance, int sbSize, void* Xio, double* eigval, double* resval, int* iters) {
+  (void)eigMinTolerance;
+  if (!c || !Xio || !eigval || !resval || nb < 1 || nb > c->max_bands || sbSize < 1)
+    return fail(PW_ERR_ARG, "pw_ppcg: bad argument");
+  if (numEig > nb) return fail(PW_ERR_ARG, "Number of eigenvalues requested  = %d which is larger than the number "
+                                           "of columns in psi = %d", numEig, nb);
+  PpcgOps ops;
+  ops.apply_h = [c](int w, const double* x, double* y) { return pw_apply_h(c, PW_F64, w, x, y); };
+  ops.precond = [c](int w, const double* r, double* y) { return pw_teter_precond(c, PW_F64, w, r, y); };
+  return ppcg_core(c, 2 * c->npw, ops, nb, numEig, scfIter, eigMaxIter, eigTolerance, sbSize, Xio, eigval, resval,
+                   iters);
+}
+
+extern "C" int pw_rs_ppcg(pw_rs* rs, int nb, int numEig, int scfIter, int eigMaxIter, double eigMinTolerance,
+                          double eigTolerance, int sbSize, double* X, double* eigval, double* resval, int* iters) {
+  (void)eigMinTolerance;
+  if (!rs || !X || !eigval || !resval || nb < 1 || sbSize < 1) return fail(PW_ERR_ARG, "pw_rs_ppcg: bad argument");
+  if (numEig > nb) return fail(PW_ERR_ARG, "Number of eigenvalues requested  = %d which is larger than the number "
+                                           "of columns in psi = %d", numEig, nb);
+  long ntot = 0;
+  PW_TRY(pw_rs_info(rs, &ntot, nullptr, nullptr));
+  pw_ctx* la = nullptr;
+  PW_TRY(rs_linalg_ctx(rs, nb, &la));
+  PpcgOps ops;
+  ops.apply_h = [rs](int w, const double* x, double* y) { return pw_rs_mult_spinor(rs, w, x, y); };
+  ops.precond = [rs](int w, const double* r, double* y) { return pw_rs_teter_precond(rs, w, r, y); };
+  return ppcg_core(la, (int)ntot, ops, nb, numEig, scfIter, eigMaxIter, eigTolerance, sbSize, X, eigval, resval,
+                   iters);
 }

@@ -111,6 +111,7 @@ struct pw_rs {
   int* d_mmap = nullptr;        // live plane -> fine z (nz)
   double* d_cs = nullptr;       // prime-radix cos / sin table
   std::map<int, double2*> d_tw; // exp(+2 pi i m / n), per length
+  pw_ctx* la = nullptr;         // linear algebra of the solvers (rs_linalg_ctx)
   rocfft_execution_info info = nullptr;
   std::map<int, RsPlans> plans;
 };
@@ -1244,6 +1245,7 @@ extern "C" int pw_rs_create(pw_rs** out, const int grid[3], const int grid_fine[
 extern "C" int pw_rs_destroy(pw_rs* rs) {
   if (!rs) return PW_OK;
   hipStreamSynchronize(rs->stream);
+  if (rs->la) pw_ctx_destroy(rs->la);
   for (auto& kv : rs->plans)
     for (rocfft_plan q : {kv.second.c_fwd, kv.second.f_inv, kv.second.f_fwd, kv.second.c_inv})
       if (q) rocfft_plan_destroy(q);
@@ -1279,6 +1281,7 @@ extern "C" int pw_rs_set_stream(pw_rs* rs, void* stream) {
   PW_HIP(hipStreamSynchronize(rs->stream));
   rs->stream = (hipStream_t)stream;
   if (rs->info) PW_FFT(rocfft_execution_info_set_stream(rs->info, rs->stream));
+  if (rs->la) PW_TRY(pw_ctx_set_stream(rs->la, stream));
   return PW_OK;
 }
 
@@ -1719,5 +1722,22 @@ extern "C" int pw_rs_teter_precond(pw_rs* rs, int nb, const double* psi, double*
                        a3 + (long)b0 * rs->Nc, (const double*)rs->d_pf, n);
     PW_HIP(hipGetLastError());
   }
+  return PW_OK;
+}
+
+int pw::rs_linalg_ctx(pw_rs* rs, int max_bands, pw_ctx** out) {
+  if (!rs->la) {
+    pw_ctx* c = new pw_ctx();
+    c->device = rs->device;
+    c->stream = rs->stream;
+    if (rocblas_create_handle(&c->blas) != rocblas_status_success) {
+      delete c;
+      return fail(PW_ERR_HIP, "rocblas_create_handle failed");
+    }
+    rocblas_set_stream(c->blas, c->stream);
+    rs->la = c;
+  }
+  rs->la->max_bands = std::max(rs->la->max_bands, max_bands);
+  *out = rs->la;
   return PW_OK;
 }

@@ -305,6 +305,15 @@ class RealSpaceContext:
                                                  int(num_occupied), C.c_void_p(rho.data_ptr()), C.byref(val)))
         return rho, val.value
 
+    def ppcg(self, X, num_eig, scf_iter, max_iter, tol, sb):
+        """Real-space PPCG (pw_rs_ppcg) in place on X: (eigenvalues,
+        residual norms, iterations)."""
+        nb = X.shape[0]
+        ev, rv, it = np.zeros(nb), np.zeros(nb), C.c_int(0)
+        check(_lib.lib().pw_rs_ppcg(self.h, nb, int(num_eig), int(scf_iter), int(max_iter), 0.0, float(tol),
+                                    int(sb), C.c_void_p(X.data_ptr()), ev, rv, C.byref(it)))
+        return ev, rv, it.value
+
     def close(self):
         if getattr(self, "h", None):
             _lib.lib().pw_rs_destroy(self.h)

@@ -198,6 +198,38 @@ def test_rs_teter_precond(name, fft, monkeypatch):
     assert _rel(got, ref) < TOL
 
 
+@pytest.mark.parametrize("name,nb,sb,iters", [("box_rs", 8, 4, 4), ("sic8_rs", 12, 4, 3), ("sic8_rs", 10, 4, 3)])
+def test_rs_ppcg_matches_oracle(name, nb, sb, iters):
+    """pw_rs_ppcg against the oracle's real-space PPCG, fixed iteration count
+    (ragged last sub-block for nb = 10)."""
+    import torch
+    print(f"case rs ppcg {name} nb={nb} sb={sb} iters={iters}", flush=True)
+    p = _prob(name, nb)
+    Xo, eo, ro, io = O.rs_ppcg(p, p.psi, nb - 2, 1, iters, 1e-30, sb)
+    ctx = R.RealSpaceContext(p)
+    X = torch.from_numpy(p.psi.copy()).to("cuda:0")
+    e, r, it = ctx.ppcg(X, nb - 2, 1, iters, 1e-30, sb)
+    ctx.close()
+    assert it == io == iters
+    assert np.abs(e - eo).max() < 1e-10 * max(1.0, np.abs(eo).max())
+    assert np.abs(r - ro).max() < 1e-6 * max(1e-3, ro.max())
+
+
+def test_rs_ppcg_converges_to_dense():
+    import torch
+    p = _prob("box_rs", 8)
+    H = O.rs_mult_spinor(p.grid, p.grid_fine, p.volume, p.ekin, p.vtot, p.npts, p.nproj, p.idx, p.val, p.wgt,
+                         np.eye(p.ntot))
+    ev = np.linalg.eigvalsh(0.5 * (H + H.T))[:8]
+    ctx = R.RealSpaceContext(p)
+    X = torch.from_numpy(p.psi.copy()).to("cuda:0")
+    e, r, it = ctx.ppcg(X, 6, 1, 200, 1e-9, 4)
+    ctx.close()
+    assert it < 200
+    assert np.abs(e - ev).max() < 1e-9
+    assert r[:6].max() < 1e-8
+
+
 def test_rs_errors():
     import torch  # noqa: F401  (the HIP runtime)
     L = _lib.lib()

@@ -228,3 +228,19 @@ def test_teter_preconditioner_on_plane_waves():
     base = np.full_like(pw, 0.25)
     out = O.rs_teter_precond(nc, vol, teter, pw, base)
     assert np.abs(out - (0.25 + b / (b + 16 * a ** 4) * pw)).max() < 1e-14
+
+
+def test_realspace_ppcg_converges_to_dense_eigenvalues():
+    """or_rs_ppcg (ppcg.cpp:683-1246 over the real-space H and Teter
+    preconditioner) reaches the lowest eigenvalues of the dense symmetric H
+    assembled column by column from or_rs_mult_spinor."""
+    p = R.box_rs(nb=8).build()
+    N = p.ntot
+    H = O.rs_mult_spinor(p.grid, p.grid_fine, p.volume, p.ekin, p.vtot, p.npts, p.nproj, p.idx, p.val, p.wgt,
+                         np.eye(N))
+    Hs = 0.5 * (H + H.T)
+    ev = np.linalg.eigvalsh(Hs)[:8]
+    X, e, r, it = O.rs_ppcg(p, p.psi, 6, 1, 200, 1e-9, 4)
+    assert it < 200
+    assert np.abs(e[:8] - ev[:8]).max() < 1e-9
+    assert r[:6].max() < 1e-8
