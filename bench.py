@@ -117,6 +117,21 @@ def realspace_bench(args):
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "model_bytes_per_band": model},
     }
+    # one PPCGSolveReal call with the deck's settings (tests/SiC_large/
+    # config.yaml: PPCG_sbSize 32, Eig_MaxIter 20, Eig_Tolerance 1e-13, so
+    # all 20 iterations run; 432 occupied of 442 bands), from the same block
+    X = x.clone()
+    ctx.ppcg(X, 432, 2, 1, 1e-13, 32)   # warm-up: one iteration
+    torch.cuda.synchronize()
+    X = x.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev, rv, it = ctx.ppcg(X, 432, 1, 20, 1e-13, 32)
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    res["ppcg"] = {"s_per_solve": round(tp, 4), "iterations": it, "sbSize": 32, "num_eig": 432,
+                   "ms_per_iteration": round(tp / max(it, 1) * 1e3, 3),
+                   "lowest_eigenvalue": float(ev[0]), "res_max_occupied": float(rv[:432].max())}
     ctx.close()
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
