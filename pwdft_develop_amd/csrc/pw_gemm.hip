@@ -302,7 +302,11 @@ int gemm_tn(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, i
   }
   const int ti = (n1 + BM - 1) / BM, tj = (n2 + BN - 1) / BN;
   const int tiles = ti * tj;
-  int nsplit = std::max(1, std::min((1024 + tiles - 1) / tiles, (K + 255) / 256));
+  // about 1024 partial workgroups, but at most 256 K-splits for one or two
+  // output tiles (the sb x sb Gram blocks of PPCG over a long K): the
+  // reduction then reads 256 partials per entry, not ~1000
+  const int cap = tiles <= 2 ? 256 : 1024;
+  int nsplit = std::max(1, std::min((cap + tiles - 1) / tiles, (K + 255) / 256));
   int kc = (K + nsplit - 1) / nsplit;
   kc = (kc + BK - 1) / BK * BK;
   nsplit = (K + kc - 1) / kc;
