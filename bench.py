@@ -19,7 +19,9 @@ FP64 sweep time, the mixed/FP64 speedup, the Ritz-value error of the mixed
 sweep against the FP64 sweep, plain H*psi throughput (FP64 and FP32), the
 roofline of the dominant kernel (the xy slab FFT kernel, timed with HIP
 events on its launch stream inside the timed region) and the FP64 CPU oracle
-timed on this host's cores.  A "realspace" object carries the same kind
+timed on this host's cores.  A "ppcg" object times one PPCGSolveReal call
+(sbSize 32, 20 iterations) from the warm block.  A "realspace" object
+carries the same kind
 of line for the real-space dual-grid H*psi on the SiC_large deck's shape
 (62^3 / 124^3 grids, 216 atoms, 442 bands).
 """
@@ -303,6 +305,21 @@ def main():
                          "f64": {"GBs": round(st64, 1), "frac": round(st64 / HBM_PEAK_GBS, 4),
                                  "launch_ms": round(local_ms64, 4)}},
     })
+
+    # PPCG (the reference decks' PW_Solver) on the same warm block: one
+    # PPCGSolveReal call with sbSize 32 and 20 iterations (tolerance 1e-13,
+    # so all run), the setting of tests/*/config.yaml
+    Xp = Xw.clone()
+    ctx.ppcg(Xp, num_eig=3 * nb // 4, scf_iter=2, max_iter=1, tol=1e-13, sb_size=32)   # warm-up
+    Xp = Xw.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, e_pp, r_pp, it_pp = ctx.ppcg(Xp, num_eig=3 * nb // 4, scf_iter=2, max_iter=20, tol=1e-13, sb_size=32)
+    torch.cuda.synchronize()
+    t_pp = time.perf_counter() - t0
+    out["ppcg"] = {"s_per_solve": round(t_pp, 4), "iterations": it_pp, "sbSize": 32, "num_eig": 3 * nb // 4,
+                   "ms_per_iteration": round(t_pp / max(it_pp, 1) * 1e3, 3),
+                   "res_max_wanted": float(r_pp[:3 * nb // 4].max())}
 
     if not args.no_cpu:
         # FP64 oracle sweep on the same warm block, OpenMP over bands
