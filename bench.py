@@ -3,6 +3,7 @@ MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload si64_60]
                     [--no-cpu] [--copy-probe] [--no-realspace | --realspace-only]
+                    [--large-cell]
 
 Workload si64_60: 64-atom diamond supercell, 60^3 grid, Ecut 10 Ha,
 npw 6567, 256 bands, 256 projectors; degree-10 Chebyshev filter with bounds
@@ -23,7 +24,8 @@ timed on this host's cores.  A "ppcg" object times one PPCGSolveReal call
 (sbSize 32, 20 iterations) from the warm block.  A "realspace" object
 carries the same kind
 of line for the real-space dual-grid H*psi on the SiC_large deck's shape
-(62^3 / 124^3 grids, 216 atoms, 442 bands).
+(62^3 / 124^3 grids, 216 atoms, 442 bands).  --large-cell times the
+180 x 180 x 72, 432-atom, 1024-band cell on its own.
 """
 import argparse
 import json
@@ -149,6 +151,87 @@ def realspace_bench(args):
     return res
 
 
+def large_cell_bench(args):
+    """The large single cell (180 x 180 x 72 grid, 432 atoms in three
+    layers, 1728 projectors, npw 76241, 1024 bands): H*psi throughput in
+    both precisions, then one mixed and one FP64 ChebFSI+RR sweep timed from
+    the same block (warmed by W FP64 sweeps, as in an SCF), with the
+    Ritz-value difference between them.  Screened exchange on this cell is not part of
+    the line."""
+    import numpy as np
+    import torch
+    from pwdft_develop_amd import workload as W
+    from pwdft_develop_amd.ops import Context
+
+    def note(msg):
+        print(f"large cell: {msg}", file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    prob = W.layered432_180().build()
+    nb = prob.nb
+    note(f"inputs built in {time.perf_counter() - t0:.1f} s")
+    ctx = Context(prob)
+    info = ctx.info()
+    steps = max(1, min(args.steps, 3))
+    hpsi = {}
+    for dt, tag in ((torch.complex128, "f64"), (torch.complex64, "f32")):
+        X = ctx.to_device(prob.psi, dt)
+        HX = torch.empty_like(X)
+        ctx.apply_h(X, HX)
+        torch.cuda.synchronize()
+        ctx.set_timing(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.apply_h(X, HX)
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / steps
+        ms, ncall = ctx.timing()
+        ctx.set_timing(False)
+        hpsi[tag] = {"band_applications_per_s": round(nb / t, 1), "ms_per_apply": round(t * 1e3, 3),
+                     "stages_ms": [round(x / max(ncall, 1), 3) for x in ms]}
+        note(f"H*psi {tag}: {t * 1e3:.1f} ms per {nb}-band apply")
+        del X, HX
+    v0 = ctx.to_device(prob.lanczos_start())
+    X = ctx.to_device(prob.psi)
+    prev = None
+    for _ in range(max(1, args.warmup)):
+        _, prev, _, _ = ctx.chebfsi_sweep(X, DEGREE, LANCZOS, v0, prev)
+    torch.cuda.synchronize()
+    note(f"{max(1, args.warmup)} FP64 warm-up sweeps done")
+    sweeps = {}
+    for mixed in (True, False):
+        tag = "mixed" if mixed else "fp64"
+        # an untimed call of the mode first (its lazily sized scratch), then
+        # the timed one, both from the warm block
+        Xs = X.clone()
+        ctx.chebfsi_sweep(Xs, DEGREE, LANCZOS, v0, prev, mixed=mixed)
+        Xs.copy_(X)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, e, r, _ = ctx.chebfsi_sweep(Xs, DEGREE, LANCZOS, v0, prev, mixed=mixed)
+        torch.cuda.synchronize()
+        sweeps[tag] = (time.perf_counter() - t0, e, r)
+        note(f"{tag} sweep: {sweeps[tag][0]:.3f} s")
+        del Xs
+    ctx.close()
+    napp = LANCZOS + DEGREE * nb + nb
+    nwant = 3 * nb // 4
+    err = np.abs(sweeps["mixed"][1] - sweeps["fp64"][1])
+    return {"metric": "ChebFSI-RR sweep band-applications/s (mixed precision), Ritz err vs FP64",
+            "value": round(napp / sweeps["mixed"][0], 1), "unit": "band-applications/s",
+            "dtype": "f32 filter / f64 Cholesky-QR + Rayleigh-Ritz",
+            "config": {"workload": prob.name, "grid": list(prob.grid), "npw": prob.npw, "nbands": nb,
+                       "natom": len(prob.atoms), "nbeta": prob.nbeta, "nstick": info["nstick"],
+                       "kxmax": info["kxmax"], "degree": DEGREE, "lanczos": LANCZOS,
+                       "band_applications_per_sweep": napp},
+            "sweep_s": {k: round(v[0], 4) for k, v in sweeps.items()},
+            "speedup_mixed_vs_fp64": round(sweeps["fp64"][0] / sweeps["mixed"][0], 3),
+            "ritz_err_ha": {"wanted_max": float(err[:nwant].max()), "all_max": float(err.max()),
+                            "nwanted": nwant},
+            "res_max_wanted": {k: float(v[2][:nwant].max()) for k, v in sweeps.items()},
+            "hpsi": hpsi}
+
+
 def host_cores():
     try:
         n = len(os.sched_getaffinity(0))
@@ -169,6 +252,9 @@ def main():
                     help="skip the real-space dual-grid H*psi line (sic216_rs, SiC_large deck shape)")
     ap.add_argument("--realspace-only", action="store_true",
                     help="time only the real-space dual-grid H*psi and print its line")
+    ap.add_argument("--large-cell", action="store_true",
+                    help="time only the 180x180x72, 432-atom, 1024-band cell (H*psi and one mixed and "
+                         "one FP64 sweep) and print its line")
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with 3 device copies of 256 MiB (the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -185,6 +271,9 @@ def main():
 
     if args.realspace_only:
         print(json.dumps({"realspace": realspace_bench(args)}))
+        return
+    if args.large_cell:
+        print(json.dumps({"large_cell": large_cell_bench(args)}))
         return
 
     prob = W.WORKLOADS[args.workload]().build()

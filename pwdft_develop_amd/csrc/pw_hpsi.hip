@@ -73,13 +73,35 @@ k_sticks_inverse_z(const typename cx<T>::t* __restrict__ psi, int ldpsi,
 }
 
 // ---------------------------------------------------------------- kernel 2
+// y transforms of the nk columns held in Y (column stride n2).  cg == 0:
+// one batched transform ping-ponging with Yalt, result in either buffer.
+// cg > 0: groups of cg columns through the scratch S (cg * n2 entries),
+// copied back, so the result is always in Y.
+template <typename T, int N2>
+__device__ __forceinline__ typename cx<T>::t* slab_y_fft(typename cx<T>::t* Y, typename cx<T>::t* Yalt,
+                                                         typename cx<T>::t* S, int n2, int nk, int cg,
+                                                         const typename cx<T>::t* tw2, T sg) {
+  typedef typename cx<T>::t C;
+  if (cg == 0) return lds_fft_n<T, N2>(Y, Yalt, n2, nk, n2, tw2, sg);
+  for (int g0 = 0; g0 < nk; g0 += cg) {
+    const int nc = min(cg, nk - g0);
+    C* Yg = Y + g0 * n2;
+    C* R = lds_fft_n<T, N2>(Yg, S, n2, nc, n2, tw2, sg);
+    if (R != Yg) {
+      for (int t = threadIdx.x; t < nc * n2; t += blockDim.x) Yg[t] = R[t];
+      __syncthreads();
+    }
+  }
+  return Y;
+}
+
 template <typename T, int N1, int N2>
 __global__ void __launch_bounds__(256)
 k_slab_xy(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restrict__ t2, int n1_,
           int n2_, int n3, int nstick, int kxmax, const int* __restrict__ pos,
           const int* __restrict__ mpos, const T* __restrict__ vtot,
           const typename cx<T>::t* __restrict__ tw1g, const typename cx<T>::t* __restrict__ tw2g,
-          int rp) {
+          int rp, int cg) {
   typedef typename cx<T>::t C;
   const int n1 = N1 > 0 ? N1 : n1_;  // compile-time when the grid is one of the dispatched sizes
   const int n2 = N2 > 0 ? N2 : n2_;
@@ -88,8 +110,12 @@ k_slab_xy(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restric
   C* tw1 = reinterpret_cast<C*>(smem_raw);
   C* tw2 = tw1 + n1;
   C* Y0 = tw2 + n2;
-  C* Y1 = Y0 + nk * n2;
-  C* Z0 = Y1 + nk * n2;
+  // cg == 0: two y buffers (ping-pong over all nk columns) and two x-row
+  // buffers.  cg > 0 (planes whose double y-space does not fit in LDS): one
+  // y buffer, transformed cg columns at a time through a scratch that the
+  // x pass reuses for its two row buffers.
+  C* Y1 = cg == 0 ? Y0 + nk * n2 : nullptr;
+  C* Z0 = cg == 0 ? Y1 + nk * n2 : Y0 + nk * n2;
   C* Z1 = Z0 + rp * n1;
   const int z = blockIdx.x, band = blockIdx.y;
   for (int t = threadIdx.x; t < n1; t += blockDim.x) tw1[t] = tw1g[t];
@@ -105,7 +131,7 @@ k_slab_xy(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restric
   }
   __syncthreads();
   // inverse y on the kx <= kxmax columns
-  C* Y = lds_fft_n<T, N2>(Y0, Y1, n2, nk, n2, tw2, (T)1);
+  C* Y = slab_y_fft<T, N2>(Y0, Y1, Z0, n2, nk, cg, tw2, (T)1);
   const int npair = (n2 + 1) / 2;
   const T* vz = vtot + (size_t)z * n1 * n2;
   const int h = n1 / 2;
@@ -153,8 +179,7 @@ k_slab_xy(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restric
     }
     __syncthreads();
   }
-  C* Yo = (Y == Y0) ? Y1 : Y0;
-  C* Yf = lds_fft_n<T, N2>(Y, Yo, n2, nk, n2, tw2, (T)-1);
+  C* Yf = slab_y_fft<T, N2>(Y, (Y == Y0) ? Y1 : Y0, Z0, n2, nk, cg, tw2, (T)-1);
   C* dst = t2 + ((size_t)band * n3 + z) * nstick;
   for (int s = threadIdx.x; s < nstick; s += blockDim.x) dst[s] = Yf[pos[s]];
 }
@@ -560,7 +585,14 @@ extern "C" int pw_ctx_create(pw_ctx** out, const int grid[3], const double super
   pw_ctx* c = new pw_ctx();
   {
     const char* sl = getenv("PW_SLAB");
-    c->use_reg_slab = !(sl && std::string(sl) == "lds");
+    c->use_reg_slab = !(sl && std::string(sl).compare(0, 3, "lds") == 0);
+    // "lds:<g>[:<r>]": the LDS kernel with its large-plane layout, g y columns
+    // per group (and r x row pairs per step)
+    if (sl && std::string(sl).compare(0, 4, "lds:") == 0) {
+      c->slab_cg_force = std::max(1, atoi(sl + 4));
+      const char* r = strchr(sl + 4, ':');
+      if (r) c->slab_rp_force = std::max(1, std::min(16, atoi(r + 1)));
+    }
   }
   c->n1 = grid[0]; c->n2 = grid[1]; c->n3 = grid[2];
   c->h1 = grid[0] / 2 + 1;
@@ -768,9 +800,12 @@ static int mark(pw_ctx* c, int k) {
   return PW_OK;
 }
 
-static size_t slab_lds(const pw_ctx* c, int esz, int rp) {
-  const int nk = c->sp.kxmax + 1;
-  return (size_t)esz * (c->n1 + c->n2 + 2 * (size_t)nk * c->n2 + 2 * (size_t)rp * c->n1);
+// LDS bytes of k_slab_xy: twiddles, y-space (two copies when cg == 0, one
+// plus a cg-column scratch shared with the x rows otherwise), x rows.
+static size_t slab_lds(const pw_ctx* c, int esz, int rp, int cg) {
+  const size_t nk = c->sp.kxmax + 1, n1 = c->n1, n2 = c->n2;
+  if (cg == 0) return esz * (n1 + n2 + 2 * nk * n2 + 2 * (size_t)rp * n1);
+  return esz * (n1 + n2 + nk * n2 + std::max((size_t)cg * n2, 2 * (size_t)rp * n1));
 }
 
 template <typename T>
@@ -787,9 +822,29 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
   C* t1 = (C*)c->d_t1;
   C* t2 = (C*)c->d_t2;
   // x-pass row-pair chunk: as many pairs as fit next to the y buffers in 64 KiB
-  int rp = 8;
-  while (rp > 1 && slab_lds(c, esz, rp) > 64 * 1024) rp /= 2;
-  const size_t lds2 = slab_lds(c, esz, rp);
+  int rp = 8, cg = 0;
+  while (rp > 1 && slab_lds(c, esz, rp, 0) > 64 * 1024) rp /= 2;
+  if (c->slab_cg_force > 0) {
+    cg = std::min(c->sp.kxmax + 1, c->slab_cg_force);
+    if (c->slab_rp_force > 0) rp = c->slab_rp_force;
+  } else if (slab_lds(c, esz, rp, 0) > 64 * 1024) {
+    // large planes: the double-buffered layout would hold one workgroup per
+    // CU with one x row pair per step.  One y buffer and a column-group
+    // scratch (the x rows' space) instead: the most row pairs (<= 8) that
+    // leave room for two workgroups per CU, else for one.
+    const int nk = c->sp.kxmax + 1;
+    bool found = false;
+    for (size_t budget : {(size_t)80 * 1024, (size_t)160 * 1024}) {
+      for (rp = 8; rp >= 1 && !found; rp /= 2) {
+        cg = std::min(nk, std::max(1, 2 * rp * c->n1 / c->n2));
+        found = slab_lds(c, esz, rp, cg) <= budget;
+        if (found) break;
+      }
+      if (found) break;
+    }
+    if (!found) { rp = 1; cg = 1; }
+  }
+  const size_t lds2 = slab_lds(c, esz, rp, cg);
   if (lds2 > 160 * 1024)
     return fail(PW_ERR_UNSUPPORTED, "H*psi slab needs %zu bytes of LDS (> 160 KiB)", lds2);
   const size_t lds1 = (size_t)esz * (c->n3 + 2 * (size_t)sc * c->n3);
@@ -813,7 +868,7 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
     auto kslab = slab_kernel<T>(c->n1, c->n2);
     PW_TRY(ensure_lds((const void*)kslab, lds2));
     hipLaunchKernelGGL(kslab, dim3(c->n3, nb), dim3(256), lds2, c->stream, (const C*)t1, t2, c->n1,
-                       c->n2, c->n3, c->sp.nstick, c->sp.kxmax, c->d_pos, c->d_mpos, vt, tw1, tw2, rp);
+                       c->n2, c->n3, c->sp.nstick, c->sp.kxmax, c->d_pos, c->d_mpos, vt, tw1, tw2, rp, cg);
   }
   PW_HIP(hipGetLastError());
   PW_TRY(mark(c, 2));

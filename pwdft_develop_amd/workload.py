@@ -244,4 +244,21 @@ def rand16_48(nb=128):
                    random_atoms(16, L, 3.5, MASTER_SEED), nb)
 
 
-WORKLOADS = {"si8_32": si8_32, "si64_60": si64_60, "rand16_48": rand16_48}
+def layered432_180(nb=1024):
+    """The large single cell: orthorhombic 63.22 x 63.22 x 25.29 bohr, 180 x
+    180 x 72 grid (Ecut 10 Ha, npw 76241), 432 atoms in three layers of a
+    12 x 12 square net (z = 1/6, 1/2, 5/6 of the cell, the middle layer
+    shifted by half a site), 1024 bands."""
+    Lx, Lz = 63.22, 25.29
+    a = Lx / 12
+    atoms = []
+    for layer, zf in enumerate((1 / 6, 1 / 2, 5 / 6)):
+        sh = 0.5 * a if layer % 2 else 0.0
+        for i in range(12):
+            for j in range(12):
+                atoms.append([i * a + sh, j * a + sh, zf * Lz])
+    return Problem("layered432_180", (180, 180, 72), np.diag([Lx, Lx, Lz]).astype(np.float64), 10.0,
+                   np.array(atoms), nb)
+
+
+WORKLOADS = {"si8_32": si8_32, "si64_60": si64_60, "rand16_48": rand16_48, "layered432_180": layered432_180}
