@@ -3,7 +3,7 @@ MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload si64_60]
                     [--no-cpu] [--copy-probe] [--no-realspace | --realspace-only]
-                    [--large-cell]
+                    [--large-cell | --large-exx]
 
 Workload si64_60: 64-atom diamond supercell, 60^3 grid, Ecut 10 Ha,
 npw 6567, 256 bands, 256 projectors; degree-10 Chebyshev filter with bounds
@@ -25,7 +25,8 @@ timed on this host's cores.  A "ppcg" object times one PPCGSolveReal call
 carries the same kind
 of line for the real-space dual-grid H*psi on the SiC_large deck's shape
 (62^3 / 124^3 grids, 216 atoms, 442 bands).  --large-cell times the
-180 x 180 x 72, 432-atom, 1024-band cell on its own.
+180 x 180 x 72, 432-atom, 1024-band cell on its own, --large-exx its
+screened exchange on the 96 x 96 x 40 subgrid.
 """
 import argparse
 import json
@@ -232,6 +233,57 @@ def large_cell_bench(args):
             "hpsi": hpsi}
 
 
+def large_exx_bench(args):
+    """Screened exchange of the layered 432-atom cell on its 96 x 96 x 40
+    subgrid (Ecut 2.8 Ha, npw 11340): V_x applied to 432 bands with all 432
+    bands occupied (186624 pair densities, each through one forward and one
+    inverse pair transform), FP64 and FP32, then the ACE build from the same
+    block.  The FP32 result is compared with the FP64 one."""
+    import numpy as np
+    import torch
+    from pwdft_develop_amd import workload as W
+    from pwdft_develop_amd.ops import Context
+
+    def note(msg):
+        print(f"exchange: {msg}", file=sys.stderr, flush=True)
+
+    prob = W.layered432_96().build()
+    nb = prob.nb
+    ctx = Context(prob)
+    mu, frac = 0.106, 0.25
+    occ = np.ones(nb)
+    X = ctx.to_device(prob.psi)
+    ctx.set_exx(prob.ecut, mu, frac, X, occ)
+    out = {}
+    res = {}
+    for dt, tag in ((torch.complex128, "f64"), (torch.complex64, "f32")):
+        Xd = ctx.to_device(prob.psi, dt)
+        Y = torch.zeros_like(Xd)
+        ctx.add_exx(Xd, Y)            # accumulates: Y += V_x X
+        Y.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ctx.add_exx(Xd, Y)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        res[tag] = Y.cpu().numpy()
+        out[tag] = {"s_per_apply": round(t, 4), "pairs_per_s": round(nb * nb / t, 1),
+                    "band_applications_per_s": round(nb / t, 2)}
+        note(f"{tag} apply of {nb} bands x {nb} occupied: {t:.3f} s")
+        del Xd, Y
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.build_ace(X)
+    torch.cuda.synchronize()
+    t_ace = time.perf_counter() - t0
+    note(f"ACE build: {t_ace:.3f} s")
+    ctx.close()
+    d = res["f32"] - res["f64"]
+    return {"workload": prob.name, "grid": list(prob.grid), "npw": prob.npw, "nbands": nb, "nocc": nb,
+            "mu": mu, "exx_fraction": frac, "pairs": nb * nb, "apply": out, "ace_build_s": round(t_ace, 4),
+            "f32_vs_f64_rel_max": float(np.abs(d).max() / np.abs(res["f64"]).max())}
+
+
 def host_cores():
     try:
         n = len(os.sched_getaffinity(0))
@@ -255,6 +307,9 @@ def main():
     ap.add_argument("--large-cell", action="store_true",
                     help="time only the 180x180x72, 432-atom, 1024-band cell (H*psi and one mixed and "
                          "one FP64 sweep) and print its line")
+    ap.add_argument("--large-exx", action="store_true",
+                    help="time only the screened exchange of the 432-atom cell on its 96x96x40 subgrid "
+                         "(432 bands x 432 occupied) and print its line")
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with 3 device copies of 256 MiB (the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -274,6 +329,9 @@ def main():
         return
     if args.large_cell:
         print(json.dumps({"large_cell": large_cell_bench(args)}))
+        return
+    if args.large_exx:
+        print(json.dumps({"large_exx": large_exx_bench(args)}))
         return
 
     prob = W.WORKLOADS[args.workload]().build()

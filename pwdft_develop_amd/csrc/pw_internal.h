@@ -214,6 +214,7 @@ struct pw_ctx {
   bool use_reg_slab = true;  // FP32 register slab kernel where compiled (env PW_SLAB=lds: the LDS kernel)
   int slab_cg_force = 0;     // env PW_SLAB=lds:<g>[:<r>]: LDS kernel, one y buffer, g columns per transform group
   int slab_rp_force = 0;     //   (and r x row pairs per step)
+  int slab_threads_force = 0;  // PW_SLAB=lds:<g>:<r>:<t>: t threads per workgroup
   hipStream_t gstream = nullptr;
   hipEvent_t gev[2] = {nullptr, nullptr};
   hipGraphExec_t fg_exec = nullptr;

@@ -244,12 +244,14 @@ def rand16_48(nb=128):
                    random_atoms(16, L, 3.5, MASTER_SEED), nb)
 
 
-def layered432_180(nb=1024):
-    """The large single cell: orthorhombic 63.22 x 63.22 x 25.29 bohr, 180 x
-    180 x 72 grid (Ecut 10 Ha, npw 76241), 432 atoms in three layers of a
-    12 x 12 square net (z = 1/6, 1/2, 5/6 of the cell, the middle layer
-    shifted by half a site), 1024 bands."""
-    Lx, Lz = 63.22, 25.29
+LAYERED_CELL = (63.22, 63.22, 25.29)
+
+
+def layered432_atoms():
+    """432 atoms in three layers of a 12 x 12 square net at z = 1/6, 1/2,
+    5/6 of the 63.22 x 63.22 x 25.29 bohr cell, the middle layer shifted by
+    half a site."""
+    Lx, _, Lz = LAYERED_CELL
     a = Lx / 12
     atoms = []
     for layer, zf in enumerate((1 / 6, 1 / 2, 5 / 6)):
@@ -257,8 +259,23 @@ def layered432_180(nb=1024):
         for i in range(12):
             for j in range(12):
                 atoms.append([i * a + sh, j * a + sh, zf * Lz])
-    return Problem("layered432_180", (180, 180, 72), np.diag([Lx, Lx, Lz]).astype(np.float64), 10.0,
-                   np.array(atoms), nb)
+    return np.array(atoms)
 
 
-WORKLOADS = {"si8_32": si8_32, "si64_60": si64_60, "rand16_48": rand16_48, "layered432_180": layered432_180}
+def layered432_180(nb=1024):
+    """The large single cell: orthorhombic 63.22 x 63.22 x 25.29 bohr, 180 x
+    180 x 72 grid (Ecut 10 Ha, npw 76241), 432 atoms, 1024 bands."""
+    return Problem("layered432_180", (180, 180, 72), np.diag(LAYERED_CELL).astype(np.float64), 10.0,
+                   layered432_atoms(), nb)
+
+
+def layered432_96(nb=432):
+    """The same cell on the 96 x 96 x 40 exchange subgrid: Ecut 2.8 Ha, the
+    largest whose Fock sphere (|G|^2/2 < 4 Ecut) the subgrid still holds
+    along x and y; 432 bands (the occupied count of the exchange apply)."""
+    return Problem("layered432_96", (96, 96, 40), np.diag(LAYERED_CELL).astype(np.float64), 2.8,
+                   layered432_atoms(), nb)
+
+
+WORKLOADS = {"si8_32": si8_32, "si64_60": si64_60, "rand16_48": rand16_48, "layered432_180": layered432_180,
+             "layered432_96": layered432_96}

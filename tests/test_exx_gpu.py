@@ -40,6 +40,28 @@ def test_add_exx_matches_oracle(name, nocc):
     ctx.close()
 
 
+def test_add_exx_layered_subgrid():
+    """The 432-atom layered cell on its 96 x 96 x 40 exchange subgrid (Ecut
+    2.8 Ha, the Fock sphere reaching the grid's x and y Nyquist planes), 24
+    occupied bands: two bands against the oracle."""
+    import torch
+    from pwdft_develop_amd.ops import Context
+    prob = W.layered432_96(nb=24).build()
+    nocc = 24
+    print("case: layered432_96 exx nocc=24", flush=True)
+    occ = np.linspace(1.0, 0.5, nocc)
+    H = O.Hamiltonian(prob)
+    H.set_exx(prob.ecut, MU, FRAC, prob.psi[:nocc], occ)
+    ref = H.add_exx(prob.psi[:2])
+    ctx = Context(prob)
+    ctx.set_exx(prob.ecut, MU, FRAC, ctx.to_device(prob.psi[:nocc]), occ)
+    got = ctx.add_exx(ctx.to_device(prob.psi)).cpu().numpy()[:2]
+    assert _rel(got, ref) < 1e-11
+    got32 = ctx.add_exx(ctx.to_device(prob.psi, torch.complex64)).cpu().numpy()[:2]
+    assert _rel(got32, ref) < 1e-5
+    ctx.close()
+
+
 def test_exx_divergence_matches_oracle():
     import ctypes as C
     from pwdft_develop_amd import _lib
