@@ -126,8 +126,10 @@ __device__ void x_r2c_rows(typename cx<T>::t* Zr, typename cx<T>::t* Zo, typenam
   __syncthreads();
 }
 
+// cg == 0: two y buffers and two x-row buffers; cg > 0: one y buffer whose
+// transforms go cg columns at a time through the x rows' scratch
 struct SlabGeom {
-  int n1, n2, n3, nstick, kxmax, rp;
+  int n1, n2, n3, nstick, kxmax, rp, cg;
 };
 
 template <typename T>
@@ -141,7 +143,7 @@ __device__ void load_tw(typename cx<T>::t* tw1, typename cx<T>::t* tw2, const ty
 template <typename T>
 __device__ typename cx<T>::t* sticks_to_yspace(const typename cx<T>::t* src, const int* pos, const int* mpos,
                                                const SlabGeom& g, typename cx<T>::t* Y0, typename cx<T>::t* Y1,
-                                               const typename cx<T>::t* tw2) {
+                                               typename cx<T>::t* S, const typename cx<T>::t* tw2) {
   typedef typename cx<T>::t C;
   const int nk = g.kxmax + 1;
   for (int t = threadIdx.x; t < nk * g.n2; t += blockDim.x) Y0[t] = C{0, 0};
@@ -155,7 +157,7 @@ __device__ typename cx<T>::t* sticks_to_yspace(const typename cx<T>::t* src, con
     }
   }
   __syncthreads();
-  return lds_fft<T>(Y0, Y1, g.n2, nk, g.n2, tw2, (T)1);
+  return slab_y_fft<T, 0>(Y0, Y1, S, g.n2, nk, g.cg, tw2, (T)1);
 }
 
 #define SLAB_LDS_SETUP                                      \
@@ -165,21 +167,21 @@ __device__ typename cx<T>::t* sticks_to_yspace(const typename cx<T>::t* src, con
   C* tw1 = reinterpret_cast<C*>(smem_raw);                  \
   C* tw2 = tw1 + g.n1;                                      \
   C* Y0 = tw2 + g.n2;                                       \
-  C* Y1 = Y0 + nk * g.n2;                                   \
-  C* Z0 = Y1 + nk * g.n2;                                   \
+  C* Y1 = g.cg == 0 ? Y0 + nk * g.n2 : nullptr;            \
+  C* Z0 = g.cg == 0 ? Y1 + nk * g.n2 : Y0 + nk * g.n2;      \
   C* Z1 = Z0 + g.rp * g.n1;                                 \
   (void)Z1;
 
 // wavefunction sticks -> real grid, scaled
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 k_exx_slab_inv(const typename cx<T>::t* __restrict__ st, T* __restrict__ out, const int* __restrict__ pos,
                const int* __restrict__ mpos, SlabGeom g, T scale, const typename cx<T>::t* tw1g,
                const typename cx<T>::t* tw2g) {
   SLAB_LDS_SETUP
   const int z = blockIdx.x, b = blockIdx.y;
   load_tw<T>(tw1, tw2, tw1g, tw2g, g.n1, g.n2);
-  C* Y = sticks_to_yspace<T>(st + ((size_t)b * g.n3 + z) * g.nstick, pos, mpos, g, Y0, Y1, tw2);
+  C* Y = sticks_to_yspace<T>(st + ((size_t)b * g.n3 + z) * g.nstick, pos, mpos, g, Y0, Y1, Z0, tw2);
   T* o = out + ((size_t)b * g.n3 + z) * g.n1 * g.n2;
   const int npair = (g.n2 + 1) / 2;
   for (int q0 = 0; q0 < npair; q0 += g.rp) {
@@ -198,7 +200,7 @@ k_exx_slab_inv(const typename cx<T>::t* __restrict__ st, T* __restrict__ out, co
 
 // pair densities psi_k phi_j -> Fock sticks; pair p = k + nb * jl
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 k_exx_pair_fwd(const T* __restrict__ psir, const T* __restrict__ phir, int nb, int j0,
                typename cx<T>::t* __restrict__ fst, const int* __restrict__ pos, SlabGeom g,
                const typename cx<T>::t* tw1g, const typename cx<T>::t* tw2g) {
@@ -223,7 +225,7 @@ k_exx_pair_fwd(const T* __restrict__ psir, const T* __restrict__ phir, int nb, i
     __syncthreads();
     x_r2c_rows<T>(Z0, Z1, Y0, g.kxmax, g.n1, g.n2, q0, nq, tw1);
   }
-  C* Yf = lds_fft<T>(Y0, Y1, g.n2, nk, g.n2, tw2, (T)-1);
+  C* Yf = slab_y_fft<T, 0>(Y0, Y1, Z0, g.n2, nk, g.cg, tw2, (T)-1);
   C* dst = fst + ((size_t)p * g.n3 + z) * g.nstick;
   for (int s = threadIdx.x; s < g.nstick; s += blockDim.x) dst[s] = Yf[pos[s]];
 }
@@ -264,7 +266,7 @@ k_exx_fock_z(typename cx<T>::t* __restrict__ fst, const T* __restrict__ K, int n
 
 // acc_k(r) += sum_jl fac_j phi_j(r) c2r(Fock sticks of pair (k, jl))(r)
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 k_exx_pair_acc(const typename cx<T>::t* __restrict__ fst, const T* __restrict__ phir, const T* __restrict__ fac,
                int nb, int j0, int nj, T* __restrict__ acc, const int* __restrict__ pos, SlabGeom g,
                const typename cx<T>::t* tw1g, const typename cx<T>::t* tw2g) {
@@ -279,7 +281,7 @@ k_exx_pair_acc(const typename cx<T>::t* __restrict__ fst, const T* __restrict__ 
     const T fj = fac[j];
     const T* ph = phir + ((size_t)j * g.n3 + z) * plane;
     const int p = k + nb * jl;
-    C* Y = sticks_to_yspace<T>(fst + ((size_t)p * g.n3 + z) * g.nstick, pos, nullptr, g, Y0, Y1, tw2);
+    C* Y = sticks_to_yspace<T>(fst + ((size_t)p * g.n3 + z) * g.nstick, pos, nullptr, g, Y0, Y1, Z0, tw2);
     for (int q0 = 0; q0 < npair; q0 += g.rp) {
       const int nq = min(g.rp, npair - q0);
       C* Zr = x_c2r_rows<T>(Y, g.kxmax, g.n1, g.n2, q0, nq, Z0, Z1, tw1);
@@ -297,7 +299,7 @@ k_exx_pair_acc(const typename cx<T>::t* __restrict__ fst, const T* __restrict__ 
 
 // real grid acc_k -> x r2c -> y FFT -> wavefunction sticks
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 k_exx_slab_fwd(const T* __restrict__ in, typename cx<T>::t* __restrict__ st, const int* __restrict__ pos,
                SlabGeom g, const typename cx<T>::t* tw1g, const typename cx<T>::t* tw2g) {
   SLAB_LDS_SETUP
@@ -316,7 +318,7 @@ k_exx_slab_fwd(const T* __restrict__ in, typename cx<T>::t* __restrict__ st, con
     __syncthreads();
     x_r2c_rows<T>(Z0, Z1, Y0, g.kxmax, g.n1, g.n2, q0, nq, tw1);
   }
-  C* Yf = lds_fft<T>(Y0, Y1, g.n2, nk, g.n2, tw2, (T)-1);
+  C* Yf = slab_y_fft<T, 0>(Y0, Y1, Z0, g.n2, nk, g.cg, tw2, (T)-1);
   C* dst = st + ((size_t)b * g.n3 + z) * g.nstick;
   for (int s = threadIdx.x; s < g.nstick; s += blockDim.x) dst[s] = Yf[pos[s]];
 }
@@ -326,8 +328,31 @@ __global__ void k_fill_zero(T* __restrict__ p, long n) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = (T)0;
 }
 
-static size_t slab_bytes(const pw_ctx* c, int esz, int kxmax, int rp) {
-  return (size_t)esz * (c->n1 + c->n2 + 2 * (size_t)(kxmax + 1) * c->n2 + 2 * (size_t)rp * c->n1);
+static size_t slab_bytes(const pw_ctx* c, int esz, int kxmax, int rp, int cg = 0) {
+  const size_t nk = kxmax + 1, n1 = c->n1, n2 = c->n2;
+  if (cg == 0) return esz * (n1 + n2 + 2 * nk * n2 + 2 * (size_t)rp * n1);
+  return esz * (n1 + n2 + nk * n2 + std::max((size_t)cg * n2, 2 * (size_t)rp * n1));
+}
+
+// Layout of the pair kernels: planes whose double-buffered y-space exceeds
+// 64 KiB take the grouped layout at 1024 threads per workgroup, with the
+// most x row pairs that leave room for two workgroups per CU when that is
+// at least 4, else the most that fit one workgroup per CU.
+static int pick_pair_layout(const pw_ctx* c, int esz, SlabGeom& g) {
+  g.cg = 0;
+  g.rp = 8;
+  while (g.rp > 1 && slab_bytes(c, esz, g.kxmax, g.rp) > 96 * 1024) g.rp /= 2;
+  if (slab_bytes(c, esz, g.kxmax, g.rp) <= 64 * 1024) return 256;
+  const int nk = g.kxmax + 1;
+  auto cg_of = [&](int rp) { return std::min(nk, std::max(1, 2 * rp * c->n1 / c->n2)); };
+  for (size_t budget : {(size_t)80 * 1024, (size_t)160 * 1024})
+    for (int rp = 8; rp >= (budget == 80 * 1024 ? 4 : 1); rp /= 2)
+      if (slab_bytes(c, esz, g.kxmax, rp, cg_of(rp)) <= budget) {
+        g.rp = rp;
+        g.cg = cg_of(rp);
+        return 1024;
+      }
+  return 256;  // keeps the double-buffered layout (set_lds reports it when too large)
 }
 
 static int pick_rp(const pw_ctx* c, int esz, int kxmax) {
@@ -348,7 +373,7 @@ static int sphere_to_real(pw_ctx* c, int nb, const void* psi, T* out, T scale) {
   typedef typename cx<T>::t C;
   PW_TRY(sphere_to_sticks<T>(c, nb, psi, c->d_t1));
   const C* tw = (const C*)(sizeof(T) == 8 ? (const void*)c->d_tw64 : (const void*)c->d_tw32);
-  SlabGeom g{c->n1, c->n2, c->n3, c->sp.nstick, c->sp.kxmax, 0};
+  SlabGeom g{c->n1, c->n2, c->n3, c->sp.nstick, c->sp.kxmax, 0, 0};
   g.rp = pick_rp(c, sizeof(C), g.kxmax);
   const size_t lds = slab_bytes(c, sizeof(C), g.kxmax, g.rp);
   PW_TRY(set_lds(k_exx_slab_inv<T>, lds));
@@ -375,26 +400,26 @@ int exx_add(pw_ctx* c, int nb, const void* psi, void* a3) {
   // filter graphs
   hipLaunchKernelGGL(k_fill_zero<T>, dim3((int)std::min<long>((N * nb + 255) / 256, 8192)), dim3(256), 0, c->stream,
                      acc, N * (long)nb);
-  SlabGeom gf{c->n1, c->n2, c->n3, E->nstick, E->kxmax, 0};
-  gf.rp = pick_rp(c, sizeof(C), gf.kxmax);
-  const size_t ldsf = slab_bytes(c, sizeof(C), gf.kxmax, gf.rp);
+  SlabGeom gf{c->n1, c->n2, c->n3, E->nstick, E->kxmax, 0, 0};
+  const int thr = pick_pair_layout(c, sizeof(C), gf);
+  const size_t ldsf = slab_bytes(c, sizeof(C), gf.kxmax, gf.rp, gf.cg);
   PW_TRY(set_lds(k_exx_pair_fwd<T>, ldsf));
   PW_TRY(set_lds(k_exx_pair_acc<T>, ldsf));
   const int J = std::max(1, std::min(E->nocc, E->npair_max / nb));
   const size_t ldsz = sizeof(C) * (c->n3 + 2 * (size_t)E->sc * c->n3);
   for (int j0 = 0; j0 < E->nocc; j0 += J) {
     const int nj = std::min(J, E->nocc - j0);
-    hipLaunchKernelGGL(k_exx_pair_fwd<T>, dim3(c->n3, nb * nj), dim3(256), ldsf, c->stream, (const T*)psir, phir,
+    hipLaunchKernelGGL(k_exx_pair_fwd<T>, dim3(c->n3, nb * nj), dim3(thr), ldsf, c->stream, (const T*)psir, phir,
                        nb, j0, (C*)E->d_fst, E->d_pos, gf, tw, tw + c->n1);
     PW_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_exx_fock_z<T>, dim3(E->nchunk, nb * nj), dim3(256), ldsz, c->stream, (C*)E->d_fst, K,
                        c->n3, E->nstick, E->sc, tw + c->n1 + c->n2);
     PW_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_exx_pair_acc<T>, dim3(c->n3, nb), dim3(256), ldsf, c->stream, (const C*)E->d_fst, phir,
+    hipLaunchKernelGGL(k_exx_pair_acc<T>, dim3(c->n3, nb), dim3(thr), ldsf, c->stream, (const C*)E->d_fst, phir,
                        fac, nb, j0, nj, acc, E->d_pos, gf, tw, tw + c->n1);
     PW_HIP(hipGetLastError());
   }
-  SlabGeom g{c->n1, c->n2, c->n3, c->sp.nstick, c->sp.kxmax, 0};
+  SlabGeom g{c->n1, c->n2, c->n3, c->sp.nstick, c->sp.kxmax, 0, 0};
   g.rp = pick_rp(c, sizeof(C), g.kxmax);
   const size_t lds = slab_bytes(c, sizeof(C), g.kxmax, g.rp);
   PW_TRY(set_lds(k_exx_slab_fwd<T>, lds));
@@ -427,7 +452,7 @@ k_density_acc(const typename cx<T>::t* __restrict__ st, const double* __restrict
   const int npair = (g.n2 + 1) / 2;
   for (int bl = sp; bl < nbc; bl += nsplit) {
     const double fb = f[b0 + bl] * invN;
-    C* Y = sticks_to_yspace<T>(st + ((size_t)bl * g.n3 + z) * g.nstick, pos, mpos, g, Y0, Y1, tw2);
+    C* Y = sticks_to_yspace<T>(st + ((size_t)bl * g.n3 + z) * g.nstick, pos, mpos, g, Y0, Y1, Z0, tw2);
     for (int q0 = 0; q0 < npair; q0 += g.rp) {
       const int nq = min(g.rp, npair - q0);
       C* Zr = x_c2r_rows<T>(Y, g.kxmax, g.n1, g.n2, q0, nq, Z0, Z1, tw1);
@@ -764,7 +789,7 @@ static int density_impl(pw_ctx* c, int nb, const void* psi, const double* f, dou
   int rc = sphere_to_sticks<T>(c, nb, psi, c->d_t1);
   if (rc) return done(rc);
   const C* tw = (const C*)(sizeof(T) == 8 ? (const void*)c->d_tw64 : (const void*)c->d_tw32);
-  SlabGeom g{c->n1, c->n2, c->n3, c->sp.nstick, c->sp.kxmax, 0};
+  SlabGeom g{c->n1, c->n2, c->n3, c->sp.nstick, c->sp.kxmax, 0, 0};
   g.rp = pick_rp(c, sizeof(C), g.kxmax);
   const size_t lds = slab_bytes(c, sizeof(C), g.kxmax, g.rp);
   if ((rc = set_lds(k_density_acc<T>, lds)) != PW_OK) return done(rc);

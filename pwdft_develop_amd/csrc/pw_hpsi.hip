@@ -73,28 +73,6 @@ k_sticks_inverse_z(const typename cx<T>::t* __restrict__ psi, int ldpsi,
 }
 
 // ---------------------------------------------------------------- kernel 2
-// y transforms of the nk columns held in Y (column stride n2).  cg == 0:
-// one batched transform ping-ponging with Yalt, result in either buffer.
-// cg > 0: groups of cg columns through the scratch S (cg * n2 entries),
-// copied back, so the result is always in Y.
-template <typename T, int N2>
-__device__ __forceinline__ typename cx<T>::t* slab_y_fft(typename cx<T>::t* Y, typename cx<T>::t* Yalt,
-                                                         typename cx<T>::t* S, int n2, int nk, int cg,
-                                                         const typename cx<T>::t* tw2, T sg) {
-  typedef typename cx<T>::t C;
-  if (cg == 0) return lds_fft_n<T, N2>(Y, Yalt, n2, nk, n2, tw2, sg);
-  for (int g0 = 0; g0 < nk; g0 += cg) {
-    const int nc = min(cg, nk - g0);
-    C* Yg = Y + g0 * n2;
-    C* R = lds_fft_n<T, N2>(Yg, S, n2, nc, n2, tw2, sg);
-    if (R != Yg) {
-      for (int t = threadIdx.x; t < nc * n2; t += blockDim.x) Yg[t] = R[t];
-      __syncthreads();
-    }
-  }
-  return Y;
-}
-
 template <typename T, int N1, int N2>
 __global__ void __launch_bounds__(1024)  // 256 threads, or up to 1024 for planes that hold one workgroup per CU
 k_slab_xy(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restrict__ t2, int n1_,
