@@ -296,7 +296,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed FP64 sweeps before the timed ones (default 3; 6 with --large-cell)")
     ap.add_argument("--workload", default="si64_60")
     ap.add_argument("--hpsi-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
@@ -324,10 +325,14 @@ def main():
     from pwdft_develop_amd import workload as W
     from pwdft_develop_amd.ops import Context
 
+    if args.warmup is None and not args.large_cell:
+        args.warmup = 3
     if args.realspace_only:
         print(json.dumps({"realspace": realspace_bench(args)}))
         return
     if args.large_cell:
+        if args.warmup is None:
+            args.warmup = 6
         print(json.dumps({"large_cell": large_cell_bench(args)}))
         return
     if args.large_exx:
