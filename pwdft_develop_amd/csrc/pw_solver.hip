@@ -252,11 +252,12 @@ static inline int grid_for(long n) { return (int)std::min<long>((n + 255) / 256,
 // dsyevd beyond, or when PW_EIGEN=rocsolver.
 int sym_eig(pw_ctx* c, int n, double* A, int lda, double* w, double* work, int* info) {
   hipStream_t s = c->stream;
-  static const bool use_rocsolver = [] {
-    const char* e = getenv("PW_EIGEN");
-    return e && std::string(e) == "rocsolver";
-  }();
-  if (use_rocsolver || n > 1024) {
+  // own single-workgroup solver up to n = 512; rocSOLVER's multi-workgroup
+  // dsyevd beyond (n = 768 / 1024: 17 / 23 ms against 37 / 79 ms).
+  // PW_EIGEN=rocsolver / own moves the switch to 0 / 1024.
+  const char* e = getenv("PW_EIGEN");
+  const int own_max = (e && std::string(e) == "rocsolver") ? 0 : (e && std::string(e) == "own") ? 1024 : 512;
+  if (n > own_max) {
     PW_BLAS(rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_upper, n, A, lda, w, work, info));
     int h = 0;
     PW_HIP(hipMemcpyAsync(&h, info, sizeof(int), hipMemcpyDeviceToHost, s));

@@ -55,6 +55,18 @@ def test_random_symmetric(ctx, n):
     _check(A, w, V)
 
 
+@pytest.mark.parametrize("n", [768, 1024])
+def test_own_solver_beyond_512(ctx, n, monkeypatch):
+    """Beyond n = 512 the default is rocSOLVER's dsyevd; PW_EIGEN=own keeps
+    the blocked single-workgroup tridiagonalisation, checked to the same
+    bars."""
+    print(f"case: syevd own solver n={n}", flush=True)
+    monkeypatch.setenv("PW_EIGEN", "own")
+    A = _rand_sym(n, n + 1)
+    w, V = _solve(ctx, A)
+    _check(A, w, V)
+
+
 def test_matches_oracle_vectors(ctx):
     print("case: syevd vs oracle n=200", flush=True)
     A = _rand_sym(200, 11)
