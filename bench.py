@@ -218,6 +218,25 @@ def large_cell_bench(args):
     napp = LANCZOS + DEGREE * nb + nb
     nwant = 3 * nb // 4
     err = np.abs(sweeps["mixed"][1] - sweeps["fp64"][1])
+    # slab stage (HIP events on its launch stream) against HBM and VALU; the
+    # projector GEMM stage (beta^T X then Y += (beta w) W: 4 m nbeta nb flops)
+    # against the dense MFMA peaks
+    roof = {}
+    for tag, p, valu_peak, mfma_peak in (("f32", 4, 78.6, 157.3), ("f64", 8, 78.6, 78.6)):
+        sl_ms = hpsi[tag]["stages_ms"][1]
+        nl_ms = hpsi[tag]["stages_ms"][3]
+        b = slab_bytes(prob, info["nstick"], nb, p)
+        fl = slab_flops(prob, info["kxmax"], nb)
+        gf = 4.0 * (2 * prob.npw) * prob.nbeta * nb
+        roof[tag] = {"slab": {"launch_ms": sl_ms, "algorithmic_bytes": b,
+                              "GBs": round(b / (sl_ms * 1e-3) / 1e9, 1),
+                              "hbm_frac": round(b / (sl_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                              "valu_tflops": round(fl / (sl_ms * 1e-3) / 1e12, 2),
+                              "valu_frac": round(fl / (sl_ms * 1e-3) / 1e12 / valu_peak, 4)},
+                     "projector_gemms": {"ms": nl_ms, "flops": gf,
+                                         "tflops": round(gf / (nl_ms * 1e-3) / 1e12, 1),
+                                         "mfma_peak_tflops": mfma_peak,
+                                         "mfma_frac": round(gf / (nl_ms * 1e-3) / 1e12 / mfma_peak, 4)}}
     return {"metric": "ChebFSI-RR sweep band-applications/s (mixed precision), Ritz err vs FP64",
             "value": round(napp / sweeps["mixed"][0], 1), "unit": "band-applications/s",
             "dtype": "f32 filter / f64 Cholesky-QR + Rayleigh-Ritz",
@@ -230,7 +249,7 @@ def large_cell_bench(args):
             "ritz_err_ha": {"wanted_max": float(err[:nwant].max()), "all_max": float(err.max()),
                             "nwanted": nwant},
             "res_max_wanted": {k: float(v[2][:nwant].max()) for k, v in sweeps.items()},
-            "hpsi": hpsi}
+            "hpsi": hpsi, "roofline": roof}
 
 
 def large_exx_bench(args):
