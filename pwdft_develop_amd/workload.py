@@ -270,9 +270,10 @@ def layered432_180(nb=1024):
 
 
 def layered432_96(nb=432):
-    """The same cell on the 96 x 96 x 40 exchange subgrid: Ecut 2.8 Ha, the
-    largest whose Fock sphere (|G|^2/2 < 4 Ecut) the subgrid still holds
-    along x and y; 432 bands (the occupied count of the exchange apply)."""
+    """The same cell on the 96 x 96 x 40 exchange subgrid at Ecut 2.8 Ha:
+    the Fock sphere (|G|^2/2 < 4 Ecut) reaches |kx|, |ky| = 47 and |kz| =
+    19, one index short of the subgrid's Nyquist planes (from 2.85 Ha on it
+    reaches them); 432 bands (the occupied count of the exchange apply)."""
     return Problem("layered432_96", (96, 96, 40), np.diag(LAYERED_CELL).astype(np.float64), 2.8,
                    layered432_atoms(), nb)
 

@@ -42,8 +42,9 @@ def test_add_exx_matches_oracle(name, nocc):
 
 def test_add_exx_layered_subgrid():
     """The 432-atom layered cell on its 96 x 96 x 40 exchange subgrid (Ecut
-    2.8 Ha, the Fock sphere reaching the grid's x and y Nyquist planes), 24
-    occupied bands: two bands against the oracle."""
+    2.8 Ha, the Fock sphere one index short of the x and y Nyquist planes;
+    the pair kernels take the grouped y layout there), 24 occupied bands:
+    two bands against the oracle."""
     import torch
     from pwdft_develop_amd.ops import Context
     prob = W.layered432_96(nb=24).build()

@@ -60,6 +60,28 @@ def test_workload_sizes_match_survey():
     assert W.rand16_48().build(with_psi=False).npw == 3577
 
 
+def test_layered_cell_shapes():
+    """The 432-atom layered cell: three 12 x 12 layers, periodic minimum
+    separation one net spacing (63.22 / 12 bohr); npw of the 180 x 180 x 72
+    grid at 10 Ha and of the 96 x 96 x 40 subgrid at 2.8 Ha; the subgrid's
+    Fock sphere stays one index short of the kx Nyquist plane at 2.8 Ha and
+    reaches it at 2.85 Ha (brute force over the grid)."""
+    a = W.layered432_atoms()
+    L = np.array(W.LAYERED_CELL)
+    assert a.shape == (432, 3)
+    assert sorted(np.unique(np.round(a[:, 2] / L[2], 6))) == [round(1 / 6, 6), 0.5, round(5 / 6, 6)]
+    d = a[:, None, :] - a[None, :, :]
+    d -= L * np.round(d / L)
+    r = np.linalg.norm(d, axis=2) + np.eye(432) * 1e9
+    assert abs(r.min() - L[0] / 12) < 1e-9
+    assert W.layered432_180(nb=2).build(with_psi=False).npw == 76241
+    assert W.layered432_96(nb=2).build(with_psi=False).npw == 11340
+    for ec, kmax in ((2.8, 47), (2.85, 48)):
+        p = W.Problem("sub", (96, 96, 40), np.diag(L), ec, a, 2).build(with_psi=False)
+        fidx, _ = O.fock_sphere(p.grid, p.supercell, p.ecut)
+        assert (fidx % (49 * 96) % 49).max() == kmax
+
+
 def test_psi_recipe(si8):
     p = si8
     assert np.all(p.psi[:, 0].imag == 0)
