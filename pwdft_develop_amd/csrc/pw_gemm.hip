@@ -46,32 +46,36 @@ __device__ __forceinline__ int acc_row(int lane, int r) {
 }
 
 // Partial C tile of A^T B over K rows [kb, ke): P[split](n1 x n2, ld n1).
+// TM = 1: 64 x 64 tile, each wave 32 x 32; TM = 2: 128 x 128 tile, each wave
+// 64 x 64 (twice the flops per loaded byte, for large outputs).
 // Software-pipelined: the next K chunk is loaded into registers while the
-// MFMAs run on the current one in LDS.  Chunk loads are column runs: load q
-// of wave w reads column 16 w + 2 q + lane/32, rows lane%32 (one 32-row
-// segment per half wave), for A and for B.
-template <typename T>
+// MFMAs run on the current one in LDS (dynamic LDS: 2 x 64 TM x (BK + 1)).
+// Chunk loads are column runs: load q of wave w reads column 16 TM w + 2 q +
+// lane/32, rows lane%32 (one 32-row segment per half wave), for A and for B.
+template <typename T, int TM>
 __global__ void __launch_bounds__(256)
 k_gemm_tn_partial(const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb, int K, int n1, int n2,
                   int kc, T* __restrict__ P) {
   typedef typename acc4<T>::t V;
-  __shared__ T As[BM][BK + 1];
-  __shared__ T Bs[BN][BK + 1];
-  const int i0 = blockIdx.x * BM, j0 = blockIdx.y * BN, split = blockIdx.z;
+  constexpr int TB = BM * TM, NQ = 8 * TM, NA = 2 * TM;
+  extern __shared__ __align__(16) unsigned char gemm_smem[];
+  T (*As)[BK + 1] = reinterpret_cast<T (*)[BK + 1]>(gemm_smem);
+  T (*Bs)[BK + 1] = As + TB;
+  const int i0 = blockIdx.x * TB, j0 = blockIdx.y * TB, split = blockIdx.z;
   const int kb = split * kc, ke = min(K, kb + kc);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int wi = (w & 1) * 32, wj = (w >> 1) * 32;
-  V acc[2][2];
+  const int wi = (w & 1) * 32 * TM, wj = (w >> 1) * 32 * TM;
+  V acc[NA][NA];
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < NA; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++) acc[a][b] = V{0, 0, 0, 0};
-  const int lr = lane & 31, lcol = 16 * w + (lane >> 5);  // column of load q: lcol + 2 q
-  T ra[8], rb[8];
+    for (int b = 0; b < NA; b++) acc[a][b] = V{0, 0, 0, 0};
+  const int lr = lane & 31, lcol = 16 * TM * w + (lane >> 5);  // column of load q: lcol + 2 q
+  T ra[NQ], rb[NQ];
   auto load = [&](int k0) {
     const int r = k0 + lr;
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < NQ; q++) {
       const int ci = i0 + lcol + 2 * q, cj = j0 + lcol + 2 * q;
       ra[q] = (ci < n1 && r < ke) ? A[(size_t)ci * lda + r] : (T)0;
       rb[q] = (cj < n2 && r < ke) ? B[(size_t)cj * ldb + r] : (T)0;
@@ -80,7 +84,7 @@ k_gemm_tn_partial(const T* __restrict__ A, int lda, const T* __restrict__ B, int
   load(kb);
   for (int k0 = kb; k0 < ke; k0 += BK) {
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < NQ; q++) {
       As[lcol + 2 * q][lr] = ra[q];
       Bs[lcol + 2 * q][lr] = rb[q];
     }
@@ -88,23 +92,23 @@ k_gemm_tn_partial(const T* __restrict__ A, int lda, const T* __restrict__ B, int
     if (k0 + BK < ke) load(k0 + BK);  // in flight during the MFMAs below and the barrier after them
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      T av[2], bv[2];
+      T av[NA], bv[NA];
 #pragma unroll
-      for (int a = 0; a < 2; a++) av[a] = As[wi + a * 16 + (lane & 15)][kk + (lane >> 4)];
+      for (int a = 0; a < NA; a++) av[a] = As[wi + a * 16 + (lane & 15)][kk + (lane >> 4)];
 #pragma unroll
-      for (int b = 0; b < 2; b++) bv[b] = Bs[wj + b * 16 + (lane & 15)][kk + (lane >> 4)];
+      for (int b = 0; b < NA; b++) bv[b] = Bs[wj + b * 16 + (lane & 15)][kk + (lane >> 4)];
 #pragma unroll
-      for (int a = 0; a < 2; a++)
+      for (int a = 0; a < NA; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++) acc[a][b] = mfma16(av[a], bv[b], acc[a][b]);
+        for (int b = 0; b < NA; b++) acc[a][b] = mfma16(av[a], bv[b], acc[a][b]);
     }
     lds_barrier();
   }
   T* Ps = P + (size_t)split * n1 * n2;
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < NA; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++) {
+    for (int b = 0; b < NA; b++) {
       const int col = j0 + wj + b * 16 + (lane & 15);
 #pragma unroll
       for (int r = 0; r < 4; r++) {
@@ -300,7 +304,10 @@ int gemm_tn(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, i
     PW_HIP(hipGetLastError());
     return PW_OK;
   }
-  const int ti = (n1 + BM - 1) / BM, tj = (n2 + BN - 1) / BN;
+  // outputs of 256 or more 64 x 64 tiles (1024 x 1024 and up) take
+  // 128 x 128 tiles
+  const int tm = ((n1 + BM - 1) / BM) * ((n2 + BN - 1) / BN) >= 256 ? 2 : 1;
+  const int ti = (n1 + BM * tm - 1) / (BM * tm), tj = (n2 + BN * tm - 1) / (BN * tm);
   const int tiles = ti * tj;
   // about 1024 partial workgroups, but at most 256 K-splits for one or two
   // output tiles (the sb x sb Gram blocks of PPCG over a long K): the
@@ -320,8 +327,15 @@ int gemm_tn(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, i
     c->gemm_ws_bytes = need;
   }
   T* P = (T*)c->d_gemm_ws;
-  hipLaunchKernelGGL(k_gemm_tn_partial<T>, dim3(ti, tj, nsplit), dim3(256), 0, c->stream, A, lda, B, ldb, K, n1, n2,
-                     kc, P);
+  const size_t lds = sizeof(T) * 2 * (size_t)(BM * tm) * (BK + 1);
+  if (tm == 2) {
+    PW_TRY(ensure_lds((const void*)k_gemm_tn_partial<T, 2>, lds));
+    hipLaunchKernelGGL((k_gemm_tn_partial<T, 2>), dim3(ti, tj, nsplit), dim3(256), lds, c->stream, A, lda, B, ldb,
+                       K, n1, n2, kc, P);
+  } else {
+    hipLaunchKernelGGL((k_gemm_tn_partial<T, 1>), dim3(ti, tj, nsplit), dim3(256), lds, c->stream, A, lda, B, ldb,
+                       K, n1, n2, kc, P);
+  }
   const long total = (long)n1 * n2;
   const int g = (int)std::min<long>((total + 255) / 256, 4096);
   hipLaunchKernelGGL(k_gemm_tn_reduce<T>, dim3(g), dim3(256), 0, c->stream, (const T*)P, nsplit, n1, n2, C, ldc,

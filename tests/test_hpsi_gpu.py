@@ -130,6 +130,31 @@ def test_apply_h_large_cell():
     ctx.close()
 
 
+@pytest.mark.parametrize("name,nb", [("layered432_180", 1024), ("si64_60", 1100)])
+def test_nonlocal_large_outputs(name, nb):
+    """Projector products whose outputs span 256 or more 64 x 64 tiles take
+    the 128 x 128 split-K tiles: W = beta^T X (nbeta x nb over the real
+    2 npw rows), Y = (beta w) W, against a torch FP64 matmul of the same
+    real-view formula (FP64 1e-12, FP32 2e-5 relative); seeded random
+    blocks, ragged tile edges at 1728 and 1100."""
+    import torch
+    from pwdft_develop_amd.ops import Context
+    prob = W.WORKLOADS[name](nb=nb).build(with_psi=False)
+    print(f"case: {name} nonlocal nb={nb}", flush=True)
+    ctx = Context(prob)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    X = torch.randn((nb, 2 * prob.npw), dtype=torch.float64, device="cuda", generator=g)
+    Xc = X.view(torch.complex128)
+    beta = torch.from_numpy(prob.beta.view(np.float64).reshape(prob.nbeta, -1)).cuda()
+    wgt = torch.from_numpy(prob.wgt).cuda()
+    ref = (beta @ X.T).T @ (beta * wgt[:, None])
+    got = ctx.nonlocal_multx(Xc).view(torch.float64)
+    assert float((got - ref).abs().max() / ref.abs().max()) < 1e-12
+    got32 = ctx.nonlocal_multx(Xc.to(torch.complex64)).view(torch.float32).double()
+    assert float((got32 - ref).abs().max() / ref.abs().max()) < 2e-5
+    ctx.close()
+
+
 def test_teter(ctx8, si8):
     X = ctx8.to_device(si8.psi[:3])
     T = ctx8.teter_precond(X).cpu().numpy()
