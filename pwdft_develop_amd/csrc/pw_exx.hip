@@ -17,6 +17,7 @@
 //   k_exx_pair_acc   Fock sticks -> y IFFT -> x c2r -> acc_k += f_j phi_j u_kj, over j
 //   k_exx_slab_fwd   acc_k(r) -> x r2c -> y FFT -> wavefunction sticks -> a3 += gather
 #include <algorithm>
+#include <cstdio>
 #include <cmath>
 #include <vector>
 
@@ -335,15 +336,39 @@ static size_t slab_bytes(const pw_ctx* c, int esz, int kxmax, int rp, int cg = 0
 }
 
 // Layout of the pair kernels: planes whose double-buffered y-space exceeds
-// 64 KiB take the grouped layout at 1024 threads per workgroup, with the
-// most x row pairs that leave room for two workgroups per CU when that is
-// at least 4, else the most that fit one workgroup per CU.
+// 64 KiB take the grouped layout (one group of all columns when it fits,
+// else the most x row pairs that leave room for two workgroups per CU when
+// that is at least 4, else the most that fit one workgroup per CU).
 static int pick_pair_layout(const pw_ctx* c, int esz, SlabGeom& g) {
+  // PW_EXX_LAYOUT=<cg>:<rp>:<threads> forces the grouped layout's shape
+  if (const char* fl = getenv("PW_EXX_LAYOUT")) {
+    int cg = 0, rp = 0, th = 0;
+    if (sscanf(fl, "%d:%d:%d", &cg, &rp, &th) == 3 && cg > 0 && rp > 0 && (th == 256 || th == 512 || th == 1024)) {
+      g.cg = std::min(cg, g.kxmax + 1);
+      g.rp = std::min(rp, 16);
+      return th;
+    }
+  }
   g.cg = 0;
   g.rp = 8;
   while (g.rp > 1 && slab_bytes(c, esz, g.kxmax, g.rp) > 96 * 1024) g.rp /= 2;
   if (slab_bytes(c, esz, g.kxmax, g.rp) <= 64 * 1024) return 256;
   const int nk = g.kxmax + 1;
+  {
+    // first choice: all nk columns in one group (the scratch is a second y
+    // buffer) and as many x row pairs (<= 16) as that scratch holds; 1024
+    // threads when one workgroup fills a CU's LDS, else 512 (96 x 96 x 40,
+    // 432 x 64 pairs: FP32 339 -> 192 ms, FP64 399 -> 311 ms against 16
+    // columns per group, 8 row pairs, 1024 threads)
+    int rp = 16;
+    while (rp > 1 && 2 * rp * c->n1 > nk * c->n2) rp /= 2;
+    const size_t b = slab_bytes(c, esz, g.kxmax, rp, nk);
+    if (b <= 160 * 1024) {
+      g.rp = rp;
+      g.cg = nk;
+      return b > 80 * 1024 ? 1024 : 512;
+    }
+  }
   auto cg_of = [&](int rp) { return std::min(nk, std::max(1, 2 * rp * c->n1 / c->n2)); };
   for (size_t budget : {(size_t)80 * 1024, (size_t)160 * 1024})
     for (int rp = 8; rp >= (budget == 80 * 1024 ? 4 : 1); rp /= 2)
