@@ -132,8 +132,8 @@ int pw_orthogonalize(pw_ctx* ctx, int nb, void* X);
  * SolveGenEig call uses.  A (device FP64, upper triangle read) <- the
  * eigenvectors, each with its largest-magnitude component positive (first
  * index on ties); w (device, n) <- eigenvalues ascending.  Householder
- * tridiagonalisation, multisection and inverse iteration for n <= 1024,
- * rocSOLVER's dsyevd beyond. */
+ * tridiagonalisation, multisection and inverse iteration for n <= 512
+ * (environment PW_EIGEN=own: n <= 1024), rocSOLVER's dsyevd beyond. */
 int pw_syevd(pw_ctx* ctx, int n, double* A, int lda, double* w);
 
 /* k-step Lanczos on the device vector v0 (one band, FP64): Ritz values of
