@@ -345,7 +345,7 @@ static int pick_pair_layout(const pw_ctx* c, int esz, SlabGeom& g) {
     int cg = 0, rp = 0, th = 0;
     if (sscanf(fl, "%d:%d:%d", &cg, &rp, &th) == 3 && cg > 0 && rp > 0 && (th == 256 || th == 512 || th == 1024)) {
       g.cg = std::min(cg, g.kxmax + 1);
-      g.rp = std::min(rp, 16);
+      g.rp = std::min(rp, 64);
       return th;
     }
   }
@@ -356,12 +356,11 @@ static int pick_pair_layout(const pw_ctx* c, int esz, SlabGeom& g) {
   const int nk = g.kxmax + 1;
   {
     // first choice: all nk columns in one group (the scratch is a second y
-    // buffer) and as many x row pairs (<= 16) as that scratch holds; 1024
-    // threads when one workgroup fills a CU's LDS, else 512 (96 x 96 x 40,
-    // 432 x 64 pairs: FP32 339 -> 192 ms, FP64 399 -> 311 ms against 16
-    // columns per group, 8 row pairs, 1024 threads)
-    int rp = 16;
-    while (rp > 1 && 2 * rp * c->n1 > nk * c->n2) rp /= 2;
+    // buffer) and as many x row pairs as that scratch holds; 1024 threads
+    // when one workgroup fills a CU's LDS, else 512 (96 x 96 x 40, 432 x 64
+    // pairs, 24 row pairs: FP32 185 ms, FP64 294 ms; 16 row pairs 192 / 311;
+    // 16 columns per group, 8 row pairs, 1024 threads 339 / 399)
+    const int rp = std::max(1, std::min((c->n2 + 1) / 2, nk * c->n2 / (2 * c->n1)));
     const size_t b = slab_bytes(c, esz, g.kxmax, rp, nk);
     if (b <= 160 * 1024) {
       g.rp = rp;

@@ -569,7 +569,7 @@ extern "C" int pw_ctx_create(pw_ctx** out, const int grid[3], const double super
     if (sl && std::string(sl).compare(0, 4, "lds:") == 0) {
       c->slab_cg_force = std::max(1, atoi(sl + 4));
       const char* r = strchr(sl + 4, ':');
-      if (r) c->slab_rp_force = std::max(1, std::min(16, atoi(r + 1)));
+      if (r) c->slab_rp_force = std::max(1, std::min(64, atoi(r + 1)));
       const char* th = r ? strchr(r + 1, ':') : nullptr;  // ":<threads>" (256, 512 or 1024)
       if (th) {
         const int t = atoi(th + 1);
@@ -819,10 +819,9 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
     const int nk = c->sp.kxmax + 1;
     bool found = false;
     // first choice: all nk columns in one group (the scratch is a second y
-    // buffer) and as many x row pairs (<= 16) as that scratch holds (FP32
-    // 180 x 180: 13.3 ms against 14.8 for 8-column groups with 4 row pairs)
-    int r = 16;
-    while (r > 1 && 2 * r * c->n1 > nk * c->n2) r /= 2;
+    // buffer) and as many x row pairs as that scratch holds (FP32 180 x 180,
+    // 22 row pairs: 13.1 ms; 16: 13.3; 8-column groups with 4 row pairs: 14.8)
+    const int r = std::max(1, std::min((c->n2 + 1) / 2, nk * c->n2 / (2 * c->n1)));
     if (slab_lds(c, esz, r, nk) <= 160 * 1024) {
       rp = r;
       cg = nk;
