@@ -3,7 +3,7 @@ MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload si64_60]
                     [--no-cpu] [--copy-probe] [--no-realspace | --realspace-only]
-                    [--large-cell | --large-exx]
+                    [--large-cell | --large-exx] [--no-batch | --batch-only]
 
 Workload si64_60: 64-atom diamond supercell, 60^3 grid, Ecut 10 Ha,
 npw 6567, 256 bands, 256 projectors; degree-10 Chebyshev filter with bounds
@@ -26,7 +26,10 @@ carries the same kind
 of line for the real-space dual-grid H*psi on the SiC_large deck's shape
 (62^3 / 124^3 grids, 216 atoms, 442 bands).  --large-cell times the
 180 x 180 x 72, 432-atom, 1024-band cell on its own, --large-exx its
-screened exchange on the 96 x 96 x 40 subgrid.
+screened exchange on the 96 x 96 x 40 subgrid.  A "batch" object times the
+48-cell uneven workload (cubic grids 24..72, 16..128 bands) through a
+degree-8 FP32 Chebyshev filter in one persistent launch against the same
+filter run cell by cell.
 """
 import argparse
 import json
@@ -150,6 +153,58 @@ def realspace_bench(args):
                                "sample": f"2 bands of {prob.name} through the C oracle, one thread: "
                                          f"{tc:.2f} s per band"}
     return res
+
+
+def batch_bench(args):
+    """Many uneven cells in one persistent launch: the 48-cell workload
+    (cubic grids 24..72, 2..32 atoms, 16..128 bands, FP32) through a
+    degree-8 Chebyshev filter, once as one batched launch (graph-replayed)
+    and once cell by cell through each cell's own context.  A step is one
+    filter over all cells (sum of nb x 8 band-applications)."""
+    import numpy as np
+    import torch
+    from pwdft_develop_amd import workload as W
+    from pwdft_develop_amd.ops import Batch
+    deg = 8
+    cells = W.batch_cells()
+    for c in cells:
+        c.build()
+    B = Batch(cells)
+    bounds = np.array([(1.0, float(c.ekin.max()) + 6.0, -2.5) for c in cells])
+    X0 = B.to_device([c.psi for c in cells])
+    X = torch.empty_like(X0)
+    napp = deg * sum(c.nb for c in cells)
+    res = {}
+    for tag, serial in (("batched", False), ("one_by_one", True)):
+        for _ in range(3):
+            X.copy_(X0)
+            B.cheb_filter(X, deg, bounds, serial=serial)
+        torch.cuda.synchronize()
+        steps = max(args.steps, 1)
+        t = 0.0
+        for _ in range(steps):
+            X.copy_(X0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            B.cheb_filter(X, deg, bounds, serial=serial)
+            torch.cuda.synchronize()
+            t += time.perf_counter() - t0
+        res[tag] = t / steps
+    info = B.info()
+    B.close()
+    grids = {}
+    for c in cells:
+        grids[c.grid[0]] = grids.get(c.grid[0], 0) + 1
+    return {
+        "metric": "batched FP32 Chebyshev filter band-applications/s (48 uneven cells, one persistent launch)",
+        "value": round(napp / res["batched"], 1), "unit": "band-applications/s",
+        "ms_per_step": round(res["batched"] * 1e3, 3), "steps": max(args.steps, 1), "dtype": "f32",
+        "one_by_one_ms": round(res["one_by_one"] * 1e3, 3),
+        "speedup_vs_one_by_one": round(res["one_by_one"] / res["batched"], 3),
+        "config": {"workload": "batch48", "cells": len(cells), "grids": grids,
+                   "bands": sum(c.nb for c in cells), "degree": deg, "band_applications": napp,
+                   "items": info["items"], "workgroups": info["workgroups"], "target": info["target"]},
+    }
 
 
 def large_cell_bench(args):
@@ -330,6 +385,10 @@ def main():
     ap.add_argument("--large-exx", action="store_true",
                     help="time only the screened exchange of the 432-atom cell on its 96x96x40 subgrid "
                          "(432 bands x 432 occupied) and print its line")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched 48-cell filter line")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="time only the batched 48-cell filter and print its line")
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with 3 device copies of 256 MiB (the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -348,6 +407,9 @@ def main():
         args.warmup = 3
     if args.realspace_only:
         print(json.dumps({"realspace": realspace_bench(args)}))
+        return
+    if args.batch_only:
+        print(json.dumps({"batch": batch_bench(args)}))
         return
     if args.large_cell:
         if args.warmup is None:
@@ -512,6 +574,8 @@ def main():
     ctx.close()
     if not args.no_realspace:
         out["realspace"] = realspace_bench(args)
+    if not args.no_batch:
+        out["batch"] = batch_bench(args)
     if args.copy_probe:
         # plain device copies of a known size: the scale for FETCH_SIZE / WRITE_SIZE
         src = torch.empty(256 << 20, dtype=torch.uint8, device="cuda:0")

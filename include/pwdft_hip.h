@@ -246,16 +246,35 @@ int pw_density(pw_ctx* ctx, int dtype, int nb, const void* psi, const double* f,
 int pw_calculate_density(pw_ctx* ctx, int dtype, int nb, const void* psi, const double* occ,
                          int num_spin, int num_occupied, double* rho, double* val);
 
-/* E = sum_i f_i Re<x_i|H x_i> (FP64 block, host f and E). */
-int pw_band_energy(pw_ctx* ctx, int nb, const void* X, const double* f, double* E);
+/* E = sum_i f_i Re<x_i|H x_i> for a dtype block (H x in that precision,
+ * the dot products summed in FP64), host f and E.  The band-energy sum of
+ * KohnSham::CalculateHarrisEnergy / CalculateKSEnergy
+ * (src/SCF/scf.cpp:1473-1486) for a given block. */
+int pw_band_energy(pw_ctx* ctx, int dtype, int nb, const void* X, const double* f, double* E);
 
 /* Nonlocal force F_a = -dE/dR_a through the projectors, d beta/dR = -iG beta:
  * F_a = -2 sum_i f_i sum_{p in a} w_p Re<beta_p|x_i> Re<beta_p|iG x_i>, the
  * G-space form of KohnSham::CalculateForce's nonlocal term
- * (src/SCF/hamiltonian.cpp:2907-3006).  Projectors laid out atom-major, npa
- * per atom (vnlc_ layout); force: host 3*natom. */
-int pw_nonlocal_force(pw_ctx* ctx, int nb, const void* X, const double* f,
+ * (src/SCF/hamiltonian.cpp:2907-3006).  X: dtype block (PW_F64 or PW_F32:
+ * the projections run in that precision, the force sums in FP64).
+ * Projectors laid out atom-major, npa per atom (vnlc_ layout); force: host
+ * 3*natom. */
+int pw_nonlocal_force(pw_ctx* ctx, int dtype, int nb, const void* X, const double* f,
                       int natom, int npa, double* force);
+
+/* Autotuner, in place of FFTW's planner (Fourier::Initialize plans with
+ * FFTW_MEASURE, src/SCF/fourier.cpp:62, 112-184: time candidate algorithms
+ * per transform shape once, keep the fastest).  For the context's grid and a
+ * dtype block of nb bands it times the xy-slab transform layouts (register
+ * FFT kernel; LDS kernel with x row pairs per step, y column groups and
+ * threads per workgroup) on pw_mult_spinor, and the split-K tilings (64 or
+ * 128 square tiles, 256..2048 partial workgroups) of the projector product
+ * beta^T X, applies the fastest of each to the context and appends them to
+ * the text file cache_path (NULL: no file).  Shapes already in the file are
+ * applied without timing.  summary (may be NULL) receives a one-line account
+ * (at most len bytes).  Results do not depend on the choice beyond
+ * rounding. */
+int pw_autotune(pw_ctx* ctx, int dtype, int nb, const char* cache_path, char* summary, int len);
 
 /* Stage timing of pw_mult_spinor / pw_apply_h calls, measured with HIP
  * events recorded on the context's stream around each kernel (no host
@@ -265,6 +284,54 @@ int pw_nonlocal_force(pw_ctx* ctx, int nb, const void* X, const double* f,
  * [3] nonlocal GEMMs; *ncalls = number of timed calls (<= 1024 per read). */
 int pw_set_timing(pw_ctx* ctx, int enable);
 int pw_get_timing(pw_ctx* ctx, double* ms4, int* ncalls);
+
+/* ------------------------------------------------------------------ */
+/* Many independent cells in one persistent launch.  No single counterpart
+ * in the reference, which runs one cell per MPI job and applies H band by
+ * band (Spinor::AddMultSpinorFineR2C, src/SCF/spinor.cpp:562-677, with
+ * EigenSolver::NonlocalMultX, src/EigenSolver/eigensolver.cpp:471-515): a
+ * batch stands for ncell such jobs.  Each cell c has its own grid
+ * (grid[3c..3c+2]), cell (supercell[9c..9c+8]), sphere (npw[c] entries of
+ * idx, ekin, teter), padding pairs (npad[c] of pad_src / pad_dst), local
+ * potential (N_c entries of vtot), projectors (nbeta[c] columns of 2 npw[c]
+ * reals in beta, nbeta[c] weights in wgt) and band count nb[c]; every array
+ * holds the cells' parts one after another.  nbeta[c] <= 256.
+ * The band blocks of all cells live in one device buffer of complex floats
+ * (FP32, the PW_F32 layout per cell): cell c's npw[c] x nb[c] block starts
+ * at complex offset xoff[c] = sum_{c' < c} npw[c'] nb[c'].
+ * Work is split into (cell, band chunk) items taken from a queue by a
+ * persistent grid; the reset of the queue and the launch are captured into a
+ * hipGraph on the second call of a shape and replayed afterwards. */
+typedef struct pw_batch pw_batch;
+int pw_batch_create(pw_batch** batch, int ncell, const int* grid, const double* supercell,
+                    const int* npw, const int* idx, const int* npad, const int* pad_src,
+                    const int* pad_dst, const double* ekin, const double* teter,
+                    const double* vtot, const int* nbeta, const double* beta,
+                    const double* wgt, const int* nb, void* stream);
+int pw_batch_destroy(pw_batch* batch);
+int pw_batch_set_stream(pw_batch* batch, void* stream);
+/* Total complex elements of the block buffer, number of items, workgroups
+ * of the persistent grid, target work per item (bands x grid points). */
+int pw_batch_info(const pw_batch* batch, long* xtotal, int* nitems, int* nwg, long* target);
+/* Re-plan the items: target work per item (bands x grid points), x row
+ * pairs per slab step, 16-stick chunks per z pass, workgroups (0 = the
+ * default of each).  The autotuner's knobs. */
+int pw_batch_tune(pw_batch* batch, long target, int rp, int zc, int nwg);
+int pw_batch_set_graphs(pw_batch* batch, int enable);
+/* The batch's autotuner (as pw_autotune): times work-per-item, row-pair
+ * and z-chunk splits on pw_batch_apply_h over every cell, applies the
+ * fastest and records it in cache_path (NULL: no file). */
+int pw_batch_autotune(pw_batch* batch, const char* cache_path, char* summary, int len);
+/* HX = H X (FP32, local + kinetic + nonlocal) for every cell in one launch;
+ * X and HX: device, xtotal complex floats each, not aliased. */
+int pw_batch_apply_h(pw_batch* batch, const void* X, void* HX);
+/* Scaled Chebyshev filter of degree m (as pw_cheb_filter, FP32) applied in
+ * place to every cell's block in one launch; bounds: host, (a, b, aL) per
+ * cell. */
+int pw_batch_cheb_filter(pw_batch* batch, int m, const double* bounds, void* X);
+/* The same filter cell by cell through pw_cheb_filter on each cell's own
+ * context: the one-by-one reference point. */
+int pw_batch_cheb_filter_serial(pw_batch* batch, int m, const double* bounds, void* X);
 
 /* ------------------------------------------------------------------ */
 /* Real-space dual-grid Gamma-point path (Use_Real_Space = 1, FFT_Number_Type

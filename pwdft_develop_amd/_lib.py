@@ -70,10 +70,10 @@ def lib():
         "pw_set_graphs": [_vp, C.c_int],
         "pw_syevd": [_vp, C.c_int, _vp, C.c_int, _vp],
         "pw_density": [_vp, C.c_int, C.c_int, _vp, _dp, _vp, C.c_int],
-        "pw_band_energy": [_vp, C.c_int, _vp, _dp, C.POINTER(C.c_double)],
+        "pw_band_energy": [_vp, C.c_int, C.c_int, _vp, _dp, C.POINTER(C.c_double)],
         "pw_calculate_density": [_vp, C.c_int, C.c_int, _vp, _dp, C.c_int, C.c_int, _vp,
                                  C.POINTER(C.c_double)],
-        "pw_nonlocal_force": [_vp, C.c_int, _vp, _dp, C.c_int, C.c_int, _dp],
+        "pw_nonlocal_force": [_vp, C.c_int, C.c_int, _vp, _dp, C.c_int, C.c_int, _dp],
         "pw_set_exx": [_vp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, _vp, _dp],
         "pw_add_exx": [_vp, C.c_int, C.c_int, _vp, _vp],
         "pw_build_ace": [_vp, C.c_int, _vp],
@@ -84,6 +84,18 @@ def lib():
                     C.POINTER(C.c_int)],
         "pw_set_timing": [_vp, C.c_int],
         "pw_get_timing": [_vp, _dp, C.POINTER(C.c_int)],
+        "pw_batch_create": [C.POINTER(_vp), C.c_int, _ip, _dp, _ip, _ip, _ip, _ip, _ip, _dp, _dp, _dp, _ip, _dp,
+                            _dp, _ip, _vp],
+        "pw_batch_destroy": [_vp],
+        "pw_batch_set_stream": [_vp, _vp],
+        "pw_batch_info": [_vp, C.POINTER(C.c_long), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_long)],
+        "pw_batch_tune": [_vp, C.c_long, C.c_int, C.c_int, C.c_int],
+        "pw_batch_set_graphs": [_vp, C.c_int],
+        "pw_batch_autotune": [_vp, C.c_char_p, C.c_char_p, C.c_int],
+        "pw_autotune": [_vp, C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_int],
+        "pw_batch_apply_h": [_vp, _vp, _vp],
+        "pw_batch_cheb_filter": [_vp, C.c_int, _dp, _vp],
+        "pw_batch_cheb_filter_serial": [_vp, C.c_int, _dp, _vp],
         "pw_rs_create": [C.POINTER(_vp), _ip, _ip, C.c_double, _dp, C.c_int, _vp],
         "pw_rs_destroy": [_vp],
         "pw_rs_set_stream": [_vp, _vp],

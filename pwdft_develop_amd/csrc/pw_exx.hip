@@ -80,53 +80,6 @@ __global__ void k_exx_negate(T* __restrict__ p, long n) {
 
 static inline int grid_of(long n) { return (int)std::max<long>(1, std::min<long>((n + 255) / 256, 8192)); }
 
-// ------------------------------------------------------------ x-pass helpers
-// Rows y1 = 2q, y2 = 2q+1 (q in [q0, q0+nq)) of the y-transformed spectrum
-// Y[kx * n2 + y] (kx <= kxmax) -> real rows packed as Z = u1 + i u2.
-template <typename T>
-__device__ typename cx<T>::t* x_c2r_rows(const typename cx<T>::t* Y, int kxmax, int n1, int n2, int q0, int nq,
-                                         typename cx<T>::t* Z0, typename cx<T>::t* Z1,
-                                         const typename cx<T>::t* tw1) {
-  typedef typename cx<T>::t C;
-  const int h = n1 / 2;
-  for (int t = threadIdx.x; t < nq * n1; t += blockDim.x) {
-    const int ql = t / n1, k = t - ql * n1;
-    const int y1 = 2 * (q0 + ql), y2 = y1 + 1;
-    C a1 = C{0, 0}, a2 = C{0, 0};
-    int kk = k;
-    bool cj = false;
-    if (k > h) { kk = n1 - k; cj = true; }
-    if (kk <= kxmax) {
-      a1 = Y[kk * n2 + y1];
-      if (y2 < n2) a2 = Y[kk * n2 + y2];
-      if (cj) { a1.y = -a1.y; a2.y = -a2.y; }
-      if (k == 0 || (2 * k == n1)) { a1.y = 0; a2.y = 0; }
-    }
-    Z0[ql * n1 + k] = C{a1.x - a2.y, a1.y + a2.x};
-  }
-  __syncthreads();
-  return lds_fft<T>(Z0, Z1, n1, nq, n1, tw1, (T)1);
-}
-
-// Real rows packed in Zr -> forward x FFT -> spectra of both rows for
-// kx <= kxmax written to Y[kx * n2 + y].
-template <typename T>
-__device__ void x_r2c_rows(typename cx<T>::t* Zr, typename cx<T>::t* Zo, typename cx<T>::t* Y, int kxmax, int n1,
-                           int n2, int q0, int nq, const typename cx<T>::t* tw1) {
-  typedef typename cx<T>::t C;
-  const int nk = kxmax + 1;
-  C* Zf = lds_fft<T>(Zr, Zo, n1, nq, n1, tw1, (T)-1);
-  for (int t = threadIdx.x; t < nq * nk; t += blockDim.x) {
-    const int ql = t / nk, k = t - ql * nk;
-    const int y1 = 2 * (q0 + ql), y2 = y1 + 1;
-    const C w = Zf[ql * n1 + k];
-    const C wc = Zf[ql * n1 + (k == 0 ? 0 : n1 - k)];
-    Y[k * n2 + y1] = C{(T)0.5 * (w.x + wc.x), (T)0.5 * (w.y - wc.y)};
-    if (y2 < n2) Y[k * n2 + y2] = C{(T)0.5 * (w.y + wc.y), (T)0.5 * (wc.x - w.x)};
-  }
-  __syncthreads();
-}
-
 // cg == 0: two y buffers and two x-row buffers; cg > 0: one y buffer whose
 // transforms go cg columns at a time through the x rows' scratch
 struct SlabGeom {
@@ -385,6 +338,45 @@ static int pick_rp(const pw_ctx* c, int esz, int kxmax) {
   return rp;
 }
 
+// Layout of the wavefunction-side slab kernels (k_exx_slab_inv,
+// k_exx_slab_fwd, k_density_acc): the double-buffered layout at 256 threads
+// whenever it fits the LDS; planes beyond that (180 x 180 in FP64: 265 KiB)
+// take the grouped layout of the pair kernels, picked the same way.
+// PW_WFN_LAYOUT=<cg>:<rp>:<threads> forces a grouped shape.  Returns the
+// threads per workgroup.
+static int pick_wfn_layout(const pw_ctx* c, int esz, SlabGeom& g) {
+  if (const char* fl = getenv("PW_WFN_LAYOUT")) {
+    int cg = 0, rp = 0, th = 0;
+    if (sscanf(fl, "%d:%d:%d", &cg, &rp, &th) == 3 && cg > 0 && rp > 0 && (th == 256 || th == 512 || th == 1024)) {
+      g.cg = std::min(cg, g.kxmax + 1);
+      g.rp = std::min(rp, 64);
+      return th;
+    }
+  }
+  g.cg = 0;
+  g.rp = pick_rp(c, esz, g.kxmax);
+  if (slab_bytes(c, esz, g.kxmax, g.rp) <= 160 * 1024) return 256;
+  const int nk = g.kxmax + 1;
+  const int rp = std::max(1, std::min((c->n2 + 1) / 2, nk * c->n2 / (2 * c->n1)));
+  const size_t b = slab_bytes(c, esz, g.kxmax, rp, nk);
+  if (b <= 160 * 1024) {
+    g.rp = rp;
+    g.cg = nk;
+    return b > 80 * 1024 ? 1024 : 512;
+  }
+  auto cg_of = [&](int r) { return std::min(nk, std::max(1, 2 * r * c->n1 / c->n2)); };
+  for (size_t budget : {(size_t)80 * 1024, (size_t)160 * 1024})
+    for (int r = 8; r >= (budget == 80 * 1024 ? 4 : 1); r /= 2)
+      if (slab_bytes(c, esz, g.kxmax, r, cg_of(r)) <= budget) {
+        g.rp = r;
+        g.cg = cg_of(r);
+        return 1024;
+      }
+  g.cg = 0;
+  g.rp = 1;
+  return 256;  // set_lds reports the plane as too large
+}
+
 template <typename K>
 static int set_lds(K kern, size_t bytes) {
   if (bytes > 160 * 1024) return fail(PW_ERR_UNSUPPORTED, "exchange slab needs %zu bytes of LDS (> 160 KiB)", bytes);
@@ -398,10 +390,10 @@ static int sphere_to_real(pw_ctx* c, int nb, const void* psi, T* out, T scale) {
   PW_TRY(sphere_to_sticks<T>(c, nb, psi, c->d_t1));
   const C* tw = (const C*)(sizeof(T) == 8 ? (const void*)c->d_tw64 : (const void*)c->d_tw32);
   SlabGeom g{c->n1, c->n2, c->n3, c->sp.nstick, c->sp.kxmax, 0, 0};
-  g.rp = pick_rp(c, sizeof(C), g.kxmax);
-  const size_t lds = slab_bytes(c, sizeof(C), g.kxmax, g.rp);
+  const int thr = pick_wfn_layout(c, sizeof(C), g);
+  const size_t lds = slab_bytes(c, sizeof(C), g.kxmax, g.rp, g.cg);
   PW_TRY(set_lds(k_exx_slab_inv<T>, lds));
-  hipLaunchKernelGGL(k_exx_slab_inv<T>, dim3(c->n3, nb), dim3(256), lds, c->stream, (const C*)c->d_t1, out,
+  hipLaunchKernelGGL(k_exx_slab_inv<T>, dim3(c->n3, nb), dim3(thr), lds, c->stream, (const C*)c->d_t1, out,
                      c->d_pos, c->d_mpos, g, scale, tw, tw + c->n1);
   PW_HIP(hipGetLastError());
   return PW_OK;
@@ -444,10 +436,10 @@ int exx_add(pw_ctx* c, int nb, const void* psi, void* a3) {
     PW_HIP(hipGetLastError());
   }
   SlabGeom g{c->n1, c->n2, c->n3, c->sp.nstick, c->sp.kxmax, 0, 0};
-  g.rp = pick_rp(c, sizeof(C), g.kxmax);
-  const size_t lds = slab_bytes(c, sizeof(C), g.kxmax, g.rp);
+  const int thw = pick_wfn_layout(c, sizeof(C), g);
+  const size_t lds = slab_bytes(c, sizeof(C), g.kxmax, g.rp, g.cg);
   PW_TRY(set_lds(k_exx_slab_fwd<T>, lds));
-  hipLaunchKernelGGL(k_exx_slab_fwd<T>, dim3(c->n3, nb), dim3(256), lds, c->stream, (const T*)acc, (C*)c->d_t2,
+  hipLaunchKernelGGL(k_exx_slab_fwd<T>, dim3(c->n3, nb), dim3(thw), lds, c->stream, (const T*)acc, (C*)c->d_t2,
                      c->d_pos, g, tw, tw + c->n1);
   PW_HIP(hipGetLastError());
   return sticks_to_sphere_add<T>(c, nb, c->d_t2, a3);
@@ -464,7 +456,7 @@ template int exx_add<float>(pw_ctx*, int, const void*, void*);
 // stored.  Block (z, split) sums the bands b = split, split + nsplit, ...
 // of the chunk into its own partial plane; partials are reduced in order.
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 k_density_acc(const typename cx<T>::t* __restrict__ st, const double* __restrict__ f, int b0, int nbc, int nsplit,
               double* __restrict__ part, const int* __restrict__ pos, const int* __restrict__ mpos, SlabGeom g,
               double invN, const typename cx<T>::t* tw1g, const typename cx<T>::t* tw2g) {
@@ -520,16 +512,18 @@ __global__ void k_scale(double* __restrict__ x, long n, double s) {
   for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) x[i] *= s;
 }
 
-// F[a][d] = -2 sum_i f_i sum_{p in a} w_p W[p,i] D_d[p,i]; one block per (a, d)
-__global__ void k_force_reduce(const double* __restrict__ W, const double* __restrict__ D, const double* __restrict__ w,
+// F[a][d] = -2 sum_i f_i sum_{p in a} w_p W[p,i] D_d[p,i]; one block per
+// (a, d), summed in FP64 whatever the precision of W and D
+template <typename T>
+__global__ void k_force_reduce(const T* __restrict__ W, const T* __restrict__ D, const double* __restrict__ w,
                                const double* __restrict__ f, int nbeta, int nb, int npa, double* __restrict__ F) {
   const int a = blockIdx.x, d = blockIdx.y;
-  const double* Dd = D + (size_t)d * nbeta * nb;
+  const T* Dd = D + (size_t)d * nbeta * nb;
   double s = 0.0;
   for (int t = threadIdx.x; t < npa * nb; t += blockDim.x) {
     const int q = t % npa, i = t / npa;
     const int p = a * npa + q;
-    s += f[i] * w[p] * W[p + (size_t)nbeta * i] * Dd[p + (size_t)nbeta * i];
+    s += f[i] * w[p] * (double)W[p + (size_t)nbeta * i] * (double)Dd[p + (size_t)nbeta * i];
   }
   __shared__ double red[256];
   red[threadIdx.x] = s;
@@ -541,13 +535,14 @@ __global__ void k_force_reduce(const double* __restrict__ W, const double* __res
   if (threadIdx.x == 0) F[3 * a + d] = -2.0 * red[0];
 }
 
-// E = sum_i f_i Re<x_i| h_i>: per-band dots reduced on the host in FP64
-__global__ void k_band_dots(const double* __restrict__ X, const double* __restrict__ HX, long m, double* __restrict__ out) {
+// E = sum_i f_i Re<x_i| h_i>: per-band dots (FP64 sums) reduced on the host
+template <typename T>
+__global__ void k_band_dots(const T* __restrict__ X, const T* __restrict__ HX, long m, double* __restrict__ out) {
   const int j = blockIdx.x;
-  const double* x = X + (size_t)j * m;
-  const double* h = HX + (size_t)j * m;
+  const T* x = X + (size_t)j * m;
+  const T* h = HX + (size_t)j * m;
   double s = 0.0;
-  for (long i = threadIdx.x; i < m; i += blockDim.x) s += x[i] * h[i];
+  for (long i = threadIdx.x; i < m; i += blockDim.x) s += (double)x[i] * (double)h[i];
   __shared__ double red[256];
   red[threadIdx.x] = s;
   __syncthreads();
@@ -814,10 +809,10 @@ static int density_impl(pw_ctx* c, int nb, const void* psi, const double* f, dou
   if (rc) return done(rc);
   const C* tw = (const C*)(sizeof(T) == 8 ? (const void*)c->d_tw64 : (const void*)c->d_tw32);
   SlabGeom g{c->n1, c->n2, c->n3, c->sp.nstick, c->sp.kxmax, 0, 0};
-  g.rp = pick_rp(c, sizeof(C), g.kxmax);
-  const size_t lds = slab_bytes(c, sizeof(C), g.kxmax, g.rp);
+  const int thr = pick_wfn_layout(c, sizeof(C), g);
+  const size_t lds = slab_bytes(c, sizeof(C), g.kxmax, g.rp, g.cg);
   if ((rc = set_lds(k_density_acc<T>, lds)) != PW_OK) return done(rc);
-  hipLaunchKernelGGL(k_density_acc<T>, dim3(c->n3, nsplit), dim3(256), lds, c->stream, (const C*)c->d_t1,
+  hipLaunchKernelGGL(k_density_acc<T>, dim3(c->n3, nsplit), dim3(thr), lds, c->stream, (const C*)c->d_t1,
                      (const double*)df, 0, nb, nsplit, part, c->d_pos, c->d_mpos, g, 1.0 / (double)N, tw,
                      tw + c->n1);
   hipLaunchKernelGGL(k_sum_partials, dim3((int)std::min<long>((N + 255) / 256, 4096)), dim3(256), 0, c->stream,
@@ -827,15 +822,16 @@ static int density_impl(pw_ctx* c, int nb, const void* psi, const double* f, dou
   return done(PW_OK);
 }
 
-__global__ void k_make_dbeta(const double2* __restrict__ beta, const double* __restrict__ G, int npw, int nbeta,
-                             double2* __restrict__ out) {
+template <typename C>
+__global__ void k_make_dbeta(const C* __restrict__ beta, const double* __restrict__ G, int npw, int nbeta,
+                             C* __restrict__ out) {
   const long total = (long)npw * nbeta;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
     const int i = (int)(e % npw);
-    const double2 b = beta[e];
+    const C b = beta[e];
     for (int d = 0; d < 3; d++) {
       const double gd = G[3 * i + d];
-      out[(size_t)d * total + e] = double2{gd * b.y, -gd * b.x};  // -i G_d beta
+      out[(size_t)d * total + e] = C{(decltype(b.x))(gd * b.y), (decltype(b.x))(-gd * b.x)};  // -i G_d beta
     }
   }
 }
@@ -893,19 +889,21 @@ extern "C" int pw_calculate_density(pw_ctx* c, int dtype, int nb, const void* ps
   return PW_OK;
 }
 
-extern "C" int pw_band_energy(pw_ctx* c, int nb, const void* X, const double* f, double* E) {
-  if (!c || !X || !f || !E || nb < 1 || nb > c->max_bands) return fail(PW_ERR_ARG, "pw_band_energy: bad argument");
+namespace pw {
+template <typename T>
+static int band_energy_impl(pw_ctx* c, int dtype, int nb, const void* X, const double* f, double* E) {
   const long m = 2L * c->npw;
-  double *HX = nullptr, *dots = nullptr;
-  if (hipMalloc((void**)&HX, sizeof(double) * m * nb) != hipSuccess ||
+  T* HX = nullptr;
+  double* dots = nullptr;
+  if (hipMalloc((void**)&HX, sizeof(T) * m * nb) != hipSuccess ||
       hipMalloc((void**)&dots, sizeof(double) * nb) != hipSuccess) {
     if (HX) (void)hipFree(HX);
     return fail(PW_ERR_NOMEM, "pw_band_energy: scratch allocation failed");
   }
-  int rc = pw_apply_h(c, PW_F64, nb, X, HX);
+  int rc = pw_apply_h(c, dtype, nb, X, HX);
   std::vector<double> h(nb);
   if (rc == PW_OK) {
-    hipLaunchKernelGGL(k_band_dots, dim3(nb), dim3(256), 0, c->stream, (const double*)X, (const double*)HX, m, dots);
+    hipLaunchKernelGGL(k_band_dots<T>, dim3(nb), dim3(256), 0, c->stream, (const T*)X, (const T*)HX, m, dots);
     if (hipMemcpyAsync(h.data(), dots, sizeof(double) * nb, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
       rc = fail(PW_ERR_HIP, "pw_band_energy: reduction failed");
@@ -919,11 +917,16 @@ extern "C" int pw_band_energy(pw_ctx* c, int nb, const void* X, const double* f,
   return PW_OK;
 }
 
-extern "C" int pw_nonlocal_force(pw_ctx* c, int nb, const void* X, const double* f, int natom, int npa,
-                                 double* force) {
-  if (!c || !X || !f || !force || nb < 1 || nb > c->max_bands) return fail(PW_ERR_ARG, "pw_nonlocal_force: bad argument");
-  if (natom * npa != c->nbeta) return fail(PW_ERR_ARG, "pw_nonlocal_force: natom*npa = %d != nbeta = %d", natom * npa, c->nbeta);
+// Nonlocal force in the precision of the block (T): the projections
+// W = Re(beta^H X) and D_d = Re((-i G_d beta)^H X) are T GEMMs on MFMA, the
+// force sums are FP64.
+template <typename T>
+static int nonlocal_force_impl(pw_ctx* c, int nb, const void* X, const double* f, int natom, int npa,
+                               double* force) {
+  typedef typename cx<T>::t Cx;
   const int npw = c->npw, nbeta = c->nbeta, m = 2 * npw;
+  const T* beta = sizeof(T) == 8 ? (const T*)(const void*)c->d_beta : (const T*)(const void*)c->d_beta_f;
+  if (!beta) return fail(PW_ERR_STATE, "pw_nonlocal_force: no projectors set");
   // G vectors of the sphere
   std::vector<double> G(3 * (size_t)npw);
   for (int i = 0; i < npw; i++) {
@@ -932,8 +935,8 @@ extern "C" int pw_nonlocal_force(pw_ctx* c, int nb, const void* X, const double*
     const int m0 = kgrid(kx, c->n1), m1 = kgrid(ky, c->n2), m2 = kgrid(kz, c->n3);
     for (int d = 0; d < 3; d++) G[3 * i + d] = c->recip[d] * m0 + c->recip[3 + d] * m1 + c->recip[6 + d] * m2;
   }
-  std::vector<double> wh(nbeta);
-  double *dG = nullptr, *dB = nullptr, *dW = nullptr, *dD = nullptr, *dw = nullptr, *dfv = nullptr, *dF = nullptr;
+  double *dG = nullptr, *dw = nullptr, *dfv = nullptr, *dF = nullptr;
+  T *dB = nullptr, *dW = nullptr, *dD = nullptr;
   auto done = [&](int rc) {
     void* p[] = {dG, dB, dW, dD, dw, dfv, dF};
     for (void* q : p)
@@ -942,29 +945,45 @@ extern "C" int pw_nonlocal_force(pw_ctx* c, int nb, const void* X, const double*
   };
   const size_t blk = (size_t)m * nbeta;
   if (hipMalloc((void**)&dG, sizeof(double) * 3 * npw) != hipSuccess ||
-      hipMalloc((void**)&dB, sizeof(double) * 3 * blk) != hipSuccess ||
-      hipMalloc((void**)&dW, sizeof(double) * (size_t)nbeta * nb) != hipSuccess ||
-      hipMalloc((void**)&dD, sizeof(double) * 3 * (size_t)nbeta * nb) != hipSuccess ||
+      hipMalloc((void**)&dB, sizeof(T) * 3 * blk) != hipSuccess ||
+      hipMalloc((void**)&dW, sizeof(T) * (size_t)nbeta * nb) != hipSuccess ||
+      hipMalloc((void**)&dD, sizeof(T) * 3 * (size_t)nbeta * nb) != hipSuccess ||
       hipMalloc((void**)&dw, sizeof(double) * nbeta) != hipSuccess ||
       hipMalloc((void**)&dfv, sizeof(double) * nb) != hipSuccess ||
       hipMalloc((void**)&dF, sizeof(double) * 3 * natom) != hipSuccess)
     return done(fail(PW_ERR_NOMEM, "pw_nonlocal_force: scratch allocation failed"));
-  // weights: betaw = beta * w column-wise -> recover w from the FP64 copies on the host side
   if (hipMemcpy(dG, G.data(), sizeof(double) * 3 * npw, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(dw, c->h_wgt.data(), sizeof(double) * nbeta, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(dfv, f, sizeof(double) * nb, hipMemcpyHostToDevice) != hipSuccess)
     return done(fail(PW_ERR_HIP, "pw_nonlocal_force: upload failed"));
-  hipLaunchKernelGGL(k_make_dbeta, dim3(4096), dim3(256), 0, c->stream, (const double2*)c->d_beta,
-                     (const double*)dG, npw, nbeta, (double2*)dB);
-  int rc = gemm_tn<double>(c, m, nbeta, nb, c->d_beta, m, (const double*)X, m, dW, nbeta, nullptr, 0.0);
+  hipLaunchKernelGGL(k_make_dbeta<Cx>, dim3(4096), dim3(256), 0, c->stream, (const Cx*)beta, (const double*)dG, npw,
+                     nbeta, (Cx*)dB);
+  int rc = gemm_tn<T>(c, m, nbeta, nb, beta, m, (const T*)X, m, dW, nbeta, nullptr, (T)0);
   for (int d = 0; d < 3 && rc == PW_OK; d++)
-    rc = gemm_tn<double>(c, m, nbeta, nb, dB + d * blk, m, (const double*)X, m, dD + (size_t)d * nbeta * nb, nbeta,
-                         nullptr, 0.0);
+    rc = gemm_tn<T>(c, m, nbeta, nb, dB + d * blk, m, (const T*)X, m, dD + (size_t)d * nbeta * nb, nbeta, nullptr,
+                    (T)0);
   if (rc != PW_OK) return done(rc);
-  hipLaunchKernelGGL(k_force_reduce, dim3(natom, 3), dim3(256), 0, c->stream, (const double*)dW, (const double*)dD,
+  hipLaunchKernelGGL(k_force_reduce<T>, dim3(natom, 3), dim3(256), 0, c->stream, (const T*)dW, (const T*)dD,
                      (const double*)dw, (const double*)dfv, nbeta, nb, npa, dF);
   if (hipMemcpyAsync(force, dF, sizeof(double) * 3 * natom, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
       hipStreamSynchronize(c->stream) != hipSuccess)
     return done(fail(PW_ERR_HIP, "pw_nonlocal_force: reduction failed"));
   return done(PW_OK);
+}
+}  // namespace pw
+
+extern "C" int pw_band_energy(pw_ctx* c, int dtype, int nb, const void* X, const double* f, double* E) {
+  if (!c || !X || !f || !E || nb < 1 || nb > c->max_bands) return fail(PW_ERR_ARG, "pw_band_energy: bad argument");
+  if (dtype == PW_F64) return band_energy_impl<double>(c, dtype, nb, X, f, E);
+  if (dtype == PW_F32) return band_energy_impl<float>(c, dtype, nb, X, f, E);
+  return fail(PW_ERR_ARG, "unknown dtype %d", dtype);
+}
+
+extern "C" int pw_nonlocal_force(pw_ctx* c, int dtype, int nb, const void* X, const double* f, int natom, int npa,
+                                 double* force) {
+  if (!c || !X || !f || !force || nb < 1 || nb > c->max_bands) return fail(PW_ERR_ARG, "pw_nonlocal_force: bad argument");
+  if (natom * npa != c->nbeta) return fail(PW_ERR_ARG, "pw_nonlocal_force: natom*npa = %d != nbeta = %d", natom * npa, c->nbeta);
+  if (dtype == PW_F64) return nonlocal_force_impl<double>(c, nb, X, f, natom, npa, force);
+  if (dtype == PW_F32) return nonlocal_force_impl<float>(c, nb, X, f, natom, npa, force);
+  return fail(PW_ERR_ARG, "unknown dtype %d", dtype);
 }

@@ -186,6 +186,53 @@ __device__ __forceinline__ typename cx<T>::t* slab_y_fft(typename cx<T>::t* Y, t
   return Y;
 }
 
+// x-pass helpers of the slab transforms (runtime lengths)
+// Rows y1 = 2q, y2 = 2q+1 (q in [q0, q0+nq)) of the y-transformed spectrum
+// Y[kx * n2 + y] (kx <= kxmax) -> real rows packed as Z = u1 + i u2.
+template <typename T>
+__device__ typename cx<T>::t* x_c2r_rows(const typename cx<T>::t* Y, int kxmax, int n1, int n2, int q0, int nq,
+                                         typename cx<T>::t* Z0, typename cx<T>::t* Z1,
+                                         const typename cx<T>::t* tw1) {
+  typedef typename cx<T>::t C;
+  const int h = n1 / 2;
+  for (int t = threadIdx.x; t < nq * n1; t += blockDim.x) {
+    const int ql = t / n1, k = t - ql * n1;
+    const int y1 = 2 * (q0 + ql), y2 = y1 + 1;
+    C a1 = C{0, 0}, a2 = C{0, 0};
+    int kk = k;
+    bool cj = false;
+    if (k > h) { kk = n1 - k; cj = true; }
+    if (kk <= kxmax) {
+      a1 = Y[kk * n2 + y1];
+      if (y2 < n2) a2 = Y[kk * n2 + y2];
+      if (cj) { a1.y = -a1.y; a2.y = -a2.y; }
+      if (k == 0 || (2 * k == n1)) { a1.y = 0; a2.y = 0; }
+    }
+    Z0[ql * n1 + k] = C{a1.x - a2.y, a1.y + a2.x};
+  }
+  __syncthreads();
+  return lds_fft<T>(Z0, Z1, n1, nq, n1, tw1, (T)1);
+}
+
+// Real rows packed in Zr -> forward x FFT -> spectra of both rows for
+// kx <= kxmax written to Y[kx * n2 + y].
+template <typename T>
+__device__ void x_r2c_rows(typename cx<T>::t* Zr, typename cx<T>::t* Zo, typename cx<T>::t* Y, int kxmax, int n1,
+                           int n2, int q0, int nq, const typename cx<T>::t* tw1) {
+  typedef typename cx<T>::t C;
+  const int nk = kxmax + 1;
+  C* Zf = lds_fft<T>(Zr, Zo, n1, nq, n1, tw1, (T)-1);
+  for (int t = threadIdx.x; t < nq * nk; t += blockDim.x) {
+    const int ql = t / nk, k = t - ql * nk;
+    const int y1 = 2 * (q0 + ql), y2 = y1 + 1;
+    const C w = Zf[ql * n1 + k];
+    const C wc = Zf[ql * n1 + (k == 0 ? 0 : n1 - k)];
+    Y[k * n2 + y1] = C{(T)0.5 * (w.x + wc.x), (T)0.5 * (w.y - wc.y)};
+    if (y2 < n2) Y[k * n2 + y2] = C{(T)0.5 * (w.y + wc.y), (T)0.5 * (wc.x - w.x)};
+  }
+  __syncthreads();
+}
+
 }  // namespace pw
 
 namespace pw {

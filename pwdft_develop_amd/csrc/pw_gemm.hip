@@ -7,10 +7,9 @@
 //        buffer; a second kernel sums the partials in a fixed order, so the
 //        result is deterministic.
 //   NN:  Y(m x n) = alpha A(m x k) B(k x n) + beta Y  (subspace rotations,
-//        beta * W; blas.cpp:1031-1067, lobpcg.cpp:445-526).  Tall products
-//        with n, k >= 8 go to rocBLAS, which is faster at these shapes; the
-//        kernel below serves the rest.  (rocBLAS is far slower on the TN
-//        shape, 256 x 256 x 13134: 2.8 ms against 41 us split-K.)
+//        beta * W; blas.cpp:1031-1067, lobpcg.cpp:445-526), 64 x 64 tiles;
+//        the projector apply inside H Y takes an epilogue that adds H's
+//        other terms or forms a Chebyshev step.
 //
 // Both use 64x64 workgroup tiles, 4 waves of 32x32, and the 16x16x4 MFMA
 // (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32).  Lane layout of the
@@ -118,6 +117,105 @@ k_gemm_tn_partial(const T* __restrict__ A, int lda, const T* __restrict__ B, int
     }
 }
 
+// ---- split BF16 products (FP32 data, FP32 accumulation) ----
+// x = hi + lo with hi = bf16(x) and lo = bf16(x - hi), both rounded to
+// nearest even: hi + lo carries 16 significant bits of x.  A product a b is
+// taken as a_hi b_hi + a_hi b_lo + a_lo b_hi (the a_lo b_lo term, 2^-16
+// relative, is dropped) on v_mfma_f32_16x16x32_bf16, whose operand lane l
+// holds row (or column) l % 16 and k = 8 (l / 16) .. 8 (l / 16) + 7 and whose
+// accumulator is laid out as v_mfma_f32_16x16x4f32's.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ unsigned short bf16_rne(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ void bf16_split(float x, unsigned short& hi, unsigned short& lo) {
+  hi = bf16_rne(x);
+  lo = bf16_rne(x - __uint_as_float((unsigned)hi << 16));
+}
+__device__ __forceinline__ f4v mfma_bf3(const unsigned short* ah, const unsigned short* al, const unsigned short* bh,
+                                        const unsigned short* bl, f4v acc) {
+  const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(ah), a1 = *reinterpret_cast<const bf16x8*>(al);
+  const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(bh), b1 = *reinterpret_cast<const bf16x8*>(bl);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc, 0, 0, 0);
+}
+constexpr int BKP = 40;  // LDS row stride of the bf16 tiles (80 bytes: 16-byte aligned rows)
+
+// k_gemm_tn_partial with FP32 inputs split into BF16 hi + lo: the same tiles,
+// loads and split-K partials; each 32-row K chunk is one 16x16x32 product per
+// (hi, lo) pairing instead of eight 16x16x4 FP32 products.
+template <int TM>
+__global__ void __launch_bounds__(256)
+k_gemm_tn_partial_bf3(const float* __restrict__ A, int lda, const float* __restrict__ B, int ldb, int K, int n1,
+                      int n2, int kc, float* __restrict__ P) {
+  constexpr int TB = BM * TM, NQ = 8 * TM, NA = 2 * TM;
+  extern __shared__ __align__(16) unsigned char gemm_smem[];
+  unsigned short* Ah = reinterpret_cast<unsigned short*>(gemm_smem);
+  unsigned short* Al = Ah + TB * BKP;
+  unsigned short* Bh = Al + TB * BKP;
+  unsigned short* Bl = Bh + TB * BKP;
+  const int i0 = blockIdx.x * TB, j0 = blockIdx.y * TB, split = blockIdx.z;
+  const int kb = split * kc, ke = min(K, kb + kc);
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wi = (w & 1) * 32 * TM, wj = (w >> 1) * 32 * TM;
+  f4v acc[NA][NA];
+#pragma unroll
+  for (int a = 0; a < NA; a++)
+#pragma unroll
+    for (int b = 0; b < NA; b++) acc[a][b] = f4v{0, 0, 0, 0};
+  const int lr = lane & 31, lcol = 16 * TM * w + (lane >> 5);
+  float ra[NQ], rb[NQ];
+  auto load = [&](int k0) {
+    const int r = k0 + lr;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      const int ci = i0 + lcol + 2 * q, cj = j0 + lcol + 2 * q;
+      ra[q] = (ci < n1 && r < ke) ? A[(size_t)ci * lda + r] : 0.f;
+      rb[q] = (cj < n2 && r < ke) ? B[(size_t)cj * ldb + r] : 0.f;
+    }
+  };
+  load(kb);
+  const int fr = lane & 15, fk = 8 * (lane >> 4);
+  for (int k0 = kb; k0 < ke; k0 += BK) {
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      unsigned short h, l;
+      bf16_split(ra[q], h, l);
+      Ah[(lcol + 2 * q) * BKP + lr] = h;
+      Al[(lcol + 2 * q) * BKP + lr] = l;
+      bf16_split(rb[q], h, l);
+      Bh[(lcol + 2 * q) * BKP + lr] = h;
+      Bl[(lcol + 2 * q) * BKP + lr] = l;
+    }
+    lds_barrier();
+    if (k0 + BK < ke) load(k0 + BK);
+#pragma unroll
+    for (int a = 0; a < NA; a++)
+#pragma unroll
+      for (int b = 0; b < NA; b++) {
+        const int ra_ = (wi + a * 16 + fr) * BKP + fk, rb_ = (wj + b * 16 + fr) * BKP + fk;
+        acc[a][b] = mfma_bf3(Ah + ra_, Al + ra_, Bh + rb_, Bl + rb_, acc[a][b]);
+      }
+    lds_barrier();
+  }
+  float* Ps = P + (size_t)split * n1 * n2;
+#pragma unroll
+  for (int a = 0; a < NA; a++)
+#pragma unroll
+    for (int b = 0; b < NA; b++) {
+      const int col = j0 + wj + b * 16 + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = i0 + wi + a * 16 + acc_row<float>(lane, r);
+        if (row < n1 && col < n2) Ps[row + (size_t)n1 * col] = acc[a][b][r];
+      }
+    }
+}
+
 // C[i + ldc j] = rowscale[i] * sum_s P[s][i + n1 j] (+ beta C), rowscale may
 // be null.  The partials are summed in split order (deterministic), eight
 // loads in flight at a time.
@@ -213,6 +311,191 @@ k_gemm_nn(const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb, T*
     }
 }
 
+// Epilogue of the projector apply inside H Y (acc = (beta w) W for one
+// output element e = row + ld col):
+//   kCheb == false: out[e] = H[e] + acc                    (H Y += beta w W)
+//   kCheb == true:  out[e] = s1 (H[e] + acc - c Yc[e]) - s2 Yp[e]
+// with (c, s1, s2) = coef[3 step ..] and Yp == null meaning s2 = 0: one step
+// of the scaled Chebyshev three-term recurrence fused with H Y's last term.
+// out may alias H or Yp, not Yc.
+// The plain product uses the same kernels with out[e] = alpha acc + beta H[e]
+// (H read only when beta != 0; H may alias out).
+template <typename T>
+struct NnEpilogue {
+  const T* H;
+  const T* Yc;
+  const T* Yp;
+  T* out;
+  int ld;
+  const T* coef;
+  int step;
+  T alpha, beta;
+};
+
+template <typename T, bool kCheb>
+__device__ __forceinline__ void nn_epilogue(const NnEpilogue<T>& ep, size_t e, T acc) {
+  if constexpr (kCheb) {
+    const T h = ep.H[e] + acc;
+    const T c = ep.coef[3 * ep.step], s1 = ep.coef[3 * ep.step + 1], s2 = ep.coef[3 * ep.step + 2];
+    T v = (h - c * ep.Yc[e]) * s1;
+    if (ep.Yp) v -= s2 * ep.Yp[e];
+    ep.out[e] = v;
+  } else {
+    const T v = ep.alpha * acc;
+    ep.out[e] = ep.beta == (T)0 ? v : v + ep.beta * ep.H[e];
+  }
+}
+
+// Y-tile product A (m x k) B (k x n) with FP32 inputs split into BF16
+// hi + lo, FP32 accumulation, and the epilogue above.  64 x 64 tiles, four
+// waves of 32 x 32, 32-deep K chunks: A chunk loads are 64-row runs (lane =
+// row, 8 k per thread), B chunk loads 8 contiguous k of one column per
+// thread; both are split and stored as k-contiguous BF16 rows so that one
+// 16-byte LDS read is one MFMA operand.  The finished tile goes through LDS
+// so that the epilogue reads and writes 16 contiguous rows per thread.
+template <bool kCheb>
+__global__ void __launch_bounds__(256)
+k_gemm_nn_bf3(const float* __restrict__ A, int lda, const float* __restrict__ B, int ldb, int m, int n, int k,
+              NnEpilogue<float> ep) {
+  __shared__ __align__(16) unsigned short Sh[4 * 64 * BKP];
+  unsigned short* Ah = Sh;
+  unsigned short* Al = Ah + 64 * BKP;
+  unsigned short* Bh = Al + 64 * BKP;
+  unsigned short* Bl = Bh + 64 * BKP;
+  const int i0 = blockIdx.x * BM, j0 = blockIdx.y * BN;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wi = (w & 1) * 32, wj = (w >> 1) * 32;
+  f4v acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) acc[a][b] = f4v{0, 0, 0, 0};
+  const int arow = i0 + (t & 63), ak = 8 * (t >> 6);
+  const int bcol = j0 + (t >> 2), bk = 8 * (t & 3);
+  const int fr = lane & 15, fk = 8 * (lane >> 4);
+  for (int k0 = 0; k0 < k; k0 += BK) {
+    unsigned short ah[8], al[8], bh[8], bl[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      const int ka = k0 + ak + j, kb2 = k0 + bk + j;
+      const float x = (arow < m && ka < k) ? A[(size_t)ka * lda + arow] : 0.f;
+      const float y = (bcol < n && kb2 < k) ? B[(size_t)bcol * ldb + kb2] : 0.f;
+      bf16_split(x, ah[j], al[j]);
+      bf16_split(y, bh[j], bl[j]);
+    }
+    auto pack = [](const unsigned short* v) {
+      return uint4{v[0] | ((unsigned)v[1] << 16), v[2] | ((unsigned)v[3] << 16), v[4] | ((unsigned)v[5] << 16),
+                   v[6] | ((unsigned)v[7] << 16)};
+    };
+    *reinterpret_cast<uint4*>(Ah + (t & 63) * BKP + ak) = pack(ah);
+    *reinterpret_cast<uint4*>(Al + (t & 63) * BKP + ak) = pack(al);
+    *reinterpret_cast<uint4*>(Bh + (t >> 2) * BKP + bk) = pack(bh);
+    *reinterpret_cast<uint4*>(Bl + (t >> 2) * BKP + bk) = pack(bl);
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        const int oa = (wi + a * 16 + fr) * BKP + fk, ob = (wj + b * 16 + fr) * BKP + fk;
+        acc[a][b] = mfma_bf3(Ah + oa, Al + oa, Bh + ob, Bl + ob, acc[a][b]);
+      }
+    __syncthreads();
+  }
+  // tile (64 x 64 floats, column stride 65) through LDS
+  float* Ct = reinterpret_cast<float*>(Sh);
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        Ct[(wj + b * 16 + (lane & 15)) * 65 + wi + a * 16 + acc_row<float>(lane, i)] = acc[a][b][i];
+  __syncthreads();
+  const int col = j0 + (t >> 2), r0 = 16 * (t & 3);
+  if (col < n) {
+    for (int rr = 0; rr < 16; rr++) {
+      const int row = i0 + r0 + rr;
+      if (row < m) nn_epilogue<float, kCheb>(ep, (size_t)col * ep.ld + row, Ct[(t >> 2) * 65 + r0 + rr]);
+    }
+  }
+}
+
+// The same product on v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32
+// (pipelined like k_gemm_nn), with the same epilogue.
+template <typename T, bool kCheb>
+__global__ void __launch_bounds__(256)
+k_gemm_nn_ep(const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb, int m, int n, int k,
+             NnEpilogue<T> ep) {
+  typedef typename acc4<T>::t V;
+  __shared__ T As[BK][BM + 1];
+  __shared__ T Bs[BN][BK + 1];
+  const int i0 = blockIdx.x * BM, j0 = blockIdx.y * BN;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int wi = (w & 1) * 32, wj = (w >> 1) * 32;
+  V acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) acc[a][b] = V{0, 0, 0, 0};
+  const int bk = lane & 31, bcol = 16 * w + (lane >> 5);
+  const int ar = i0 + lane;
+  T ra[8], rb[8];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int kk = k0 + 8 * w + q;
+      ra[q] = (ar < m && kk < k) ? A[(size_t)kk * lda + ar] : (T)0;
+      const int cj = j0 + bcol + 2 * q, kq = k0 + bk;
+      rb[q] = (cj < n && kq < k) ? B[(size_t)cj * ldb + kq] : (T)0;
+    }
+  };
+  load(0);
+  for (int k0 = 0; k0 < k; k0 += BK) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      As[8 * w + q][lane] = ra[q];
+      Bs[bcol + 2 * q][bk] = rb[q];
+    }
+    lds_barrier();
+    if (k0 + BK < k) load(k0 + BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T av[2], bv[2];
+#pragma unroll
+      for (int a = 0; a < 2; a++) av[a] = As[kk + (lane >> 4)][wi + a * 16 + (lane & 15)];
+#pragma unroll
+      for (int b = 0; b < 2; b++) bv[b] = Bs[wj + b * 16 + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+      for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) acc[a][b] = mfma16(av[a], bv[b], acc[a][b]);
+    }
+    lds_barrier();
+  }
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const int col = j0 + wj + b * 16 + (lane & 15);
+      if (col >= n) continue;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int row = i0 + wi + a * 16 + acc_row<T>(lane, r);
+        if (row < m) nn_epilogue<T, kCheb>(ep, (size_t)col * ep.ld + row, acc[a][b][r]);
+      }
+    }
+}
+
+// Elementwise form of the epilogue (no projectors: acc = 0).
+template <typename T, bool kCheb>
+__global__ void k_nn_epilogue_only(long m, int n, NnEpilogue<T> ep) {
+  const long total = m * n;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    const long col = e / m, row = e - col * m;
+    nn_epilogue<T, kCheb>(ep, (size_t)col * ep.ld + row, (T)0);
+  }
+}
+
 // Single-column products (the one-band H x of Lanczos): GEMV kernels.
 // C(n1) = A^T b over K: workgroup (i, split) sums column i of A against b
 // over one K range, 256 threads with 8 loads in flight each; partials go
@@ -282,7 +565,7 @@ k_gemv_n(const T* __restrict__ A, int lda, const T* __restrict__ x, T* __restric
 
 template <typename T>
 int gemm_tn(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, int ldb, T* C, int ldc,
-            const T* rowscale, T beta) {
+            const T* rowscale, T beta, bool split) {
   if (n1 <= 0 || n2 <= 0) return PW_OK;
   if (n2 == 1) {
     const int nsplit = std::max(1, std::min(64, (K + 2047) / 2048));
@@ -306,13 +589,13 @@ int gemm_tn(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, i
   }
   // outputs of 256 or more 64 x 64 tiles (1024 x 1024 and up) take
   // 128 x 128 tiles
-  const int tm = ((n1 + BM - 1) / BM) * ((n2 + BN - 1) / BN) >= 256 ? 2 : 1;
+  const int tm = c->tn_tm > 0 ? c->tn_tm : (((n1 + BM - 1) / BM) * ((n2 + BN - 1) / BN) >= 256 ? 2 : 1);
   const int ti = (n1 + BM * tm - 1) / (BM * tm), tj = (n2 + BN * tm - 1) / (BN * tm);
   const int tiles = ti * tj;
   // about 1024 partial workgroups, but at most 256 K-splits for one or two
   // output tiles (the sb x sb Gram blocks of PPCG over a long K): the
   // reduction then reads 256 partials per entry, not ~1000
-  const int cap = tiles <= 2 ? 256 : 1024;
+  const int cap = c->tn_cap > 0 ? c->tn_cap : (tiles <= 2 ? 256 : 1024);
   int nsplit = std::max(1, std::min((cap + tiles - 1) / tiles, (K + 255) / 256));
   int kc = (K + nsplit - 1) / nsplit;
   kc = (kc + BK - 1) / BK * BK;
@@ -327,6 +610,25 @@ int gemm_tn(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, i
     c->gemm_ws_bytes = need;
   }
   T* P = (T*)c->d_gemm_ws;
+  if constexpr (sizeof(T) == 4) {
+    if (split) {
+      const size_t ldsb = sizeof(unsigned short) * 4 * (size_t)(BM * tm) * BKP;
+      if (tm == 2) {
+        PW_TRY(ensure_lds((const void*)k_gemm_tn_partial_bf3<2>, ldsb));
+        hipLaunchKernelGGL(k_gemm_tn_partial_bf3<2>, dim3(ti, tj, nsplit), dim3(256), ldsb, c->stream, A, lda, B,
+                           ldb, K, n1, n2, kc, P);
+      } else {
+        hipLaunchKernelGGL(k_gemm_tn_partial_bf3<1>, dim3(ti, tj, nsplit), dim3(256), ldsb, c->stream, A, lda, B,
+                           ldb, K, n1, n2, kc, P);
+      }
+      const long total = (long)n1 * n2;
+      const int g = (int)std::min<long>((total + 255) / 256, 4096);
+      hipLaunchKernelGGL(k_gemm_tn_reduce<T>, dim3(g), dim3(256), 0, c->stream, (const T*)P, nsplit, n1, n2, C, ldc,
+                         rowscale, beta);
+      PW_HIP(hipGetLastError());
+      return PW_OK;
+    }
+  }
   const size_t lds = sizeof(T) * 2 * (size_t)(BM * tm) * (BK + 1);
   if (tm == 2) {
     PW_TRY(ensure_lds((const void*)k_gemm_tn_partial<T, 2>, lds));
@@ -353,36 +655,62 @@ int gemm_nn(pw_ctx* c, int m, int n, int k, T alpha, const T* A, int lda, const 
     PW_HIP(hipGetLastError());
     return PW_OK;
   }
-  if (n >= 8 && k >= 8 && m >= 1024) {
-    // tall A times a small square: rocBLAS (Tensile) runs these plain
-    // products at 1.6-1.8x the MFMA kernel below (13134 x 256 x 256: 24 us
-    // FP32, 37 us FP64).  The handle takes the context's stream for this
-    // call (a capture stream while a graph is recorded) and gets its
-    // previous stream back.
-    hipStream_t prev = nullptr;
-    rocblas_get_stream(c->blas, &prev);
-    rocblas_set_stream(c->blas, c->stream);
-    rocblas_status st;
-    if constexpr (sizeof(T) == 4)
-      st = rocblas_sgemm(c->blas, rocblas_operation_none, rocblas_operation_none, m, n, k, (const float*)&alpha,
-                         (const float*)A, lda, (const float*)B, ldb, (const float*)&beta, (float*)Y, ldy);
-    else
-      st = rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, m, n, k, (const double*)&alpha,
-                         (const double*)A, lda, (const double*)B, ldb, (const double*)&beta, (double*)Y, ldy);
-    rocblas_set_stream(c->blas, prev);
-    if (st != rocblas_status_success) return fail(PW_ERR_HIP, "rocblas gemm failed (%d)", (int)st);
-    return PW_OK;
-  }
   hipLaunchKernelGGL(k_gemm_nn<T>, dim3((m + BM - 1) / BM, (n + BN - 1) / BN), dim3(256), 0, c->stream, A, lda, B,
                      ldb, Y, ldy, m, n, k, alpha, beta);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }
 
+// out (m x n) = epilogue(H, A B) for the projector apply of H Y: FP32 on
+// split-BF16 MFMA (FP32 MFMA when the context's split products are off), FP64
+// on FP64 MFMA; A = beta w (m x k), B = W (k x n).
+template <typename T>
+int gemm_nn_ep(pw_ctx* c, int m, int n, int k, const T* A, int lda, const T* B, int ldb, const T* H, const T* Yc,
+               const T* Yp, T* out, int ld, const T* coef, int step, bool cheb, bool split) {
+  if (m <= 0 || n <= 0) return PW_OK;
+  NnEpilogue<T> ep{H, Yc, Yp, out, ld, coef, step, (T)1, (T)1};
+  const dim3 grid((m + BM - 1) / BM, (n + BN - 1) / BN);
+  bool done = false;
+  if (!cheb && n == 1 && k > 0 && H == out) {
+    // one column (the Lanczos vector): out += A b
+    hipLaunchKernelGGL(k_gemv_n<T>, dim3((m + 63) / 64), dim3(256), 0, c->stream, A, lda, B, out, m, k, (T)1, (T)1);
+    done = true;
+  }
+  if (k <= 0) {
+    const int g = (int)std::min<long>(((long)m * n + 255) / 256, 8192);
+    if (cheb)
+      hipLaunchKernelGGL((k_nn_epilogue_only<T, true>), dim3(g), dim3(256), 0, c->stream, (long)m, n, ep);
+    else
+      hipLaunchKernelGGL((k_nn_epilogue_only<T, false>), dim3(g), dim3(256), 0, c->stream, (long)m, n, ep);
+    done = true;
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (!done && split) {
+      if (cheb)
+        hipLaunchKernelGGL(k_gemm_nn_bf3<true>, grid, dim3(256), 0, c->stream, A, lda, B, ldb, m, n, k, ep);
+      else
+        hipLaunchKernelGGL(k_gemm_nn_bf3<false>, grid, dim3(256), 0, c->stream, A, lda, B, ldb, m, n, k, ep);
+      done = true;
+    }
+  }
+  if (!done) {
+    if (cheb)
+      hipLaunchKernelGGL((k_gemm_nn_ep<T, true>), grid, dim3(256), 0, c->stream, A, lda, B, ldb, m, n, k, ep);
+    else
+      hipLaunchKernelGGL((k_gemm_nn_ep<T, false>), grid, dim3(256), 0, c->stream, A, lda, B, ldb, m, n, k, ep);
+  }
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+template int gemm_nn_ep<double>(pw_ctx*, int, int, int, const double*, int, const double*, int, const double*,
+                                const double*, const double*, double*, int, const double*, int, bool, bool);
+template int gemm_nn_ep<float>(pw_ctx*, int, int, int, const float*, int, const float*, int, const float*,
+                               const float*, const float*, float*, int, const float*, int, bool, bool);
 template int gemm_tn<double>(pw_ctx*, int, int, int, const double*, int, const double*, int, double*, int,
-                             const double*, double);
+                             const double*, double, bool);
 template int gemm_tn<float>(pw_ctx*, int, int, int, const float*, int, const float*, int, float*, int,
-                            const float*, float);
+                            const float*, float, bool);
 template int gemm_nn<double>(pw_ctx*, int, int, int, double, const double*, int, const double*, int, double,
                              double*, int);
 template int gemm_nn<float>(pw_ctx*, int, int, int, float, const float*, int, const float*, int, float, float*,

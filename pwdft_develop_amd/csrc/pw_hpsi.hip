@@ -562,6 +562,8 @@ extern "C" int pw_ctx_create(pw_ctx** out, const int grid[3], const double super
   if (grid[2] >= 32768) return fail(PW_ERR_UNSUPPORTED, "pw_ctx_create: n3 too large");
   pw_ctx* c = new pw_ctx();
   {
+    const char* bf = getenv("PW_BF16");
+    c->bf16_split = !(bf && atoi(bf) == 0);
     const char* sl = getenv("PW_SLAB");
     c->use_reg_slab = !(sl && std::string(sl).compare(0, 3, "lds") == 0);
     // "lds:<g>[:<r>]": the LDS kernel with its large-plane layout, g y columns
@@ -807,7 +809,10 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
   // x-pass row-pair chunk: as many pairs as fit next to the y buffers in 64 KiB
   int rp = 8, cg = 0;
   while (rp > 1 && slab_lds(c, esz, rp, 0) > 64 * 1024) rp /= 2;
-  if (c->slab_cg_force > 0) {
+  if (c->slab_cg_force == 0 && c->slab_rp_force > 0) {
+    // double-buffered layout with a chosen row-pair count (the autotuner's)
+    rp = std::min(c->slab_rp_force, (c->n2 + 1) / 2);
+  } else if (c->slab_cg_force > 0) {
     cg = std::min(c->sp.kxmax + 1, c->slab_cg_force);
     if (c->slab_rp_force > 0) rp = c->slab_rp_force;
   } else if (slab_lds(c, esz, rp, 0) > 64 * 1024) {
@@ -881,6 +886,12 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
   return PW_OK;
 }
 
+bool has_reg_slab(const pw_ctx* c, int dtype) {
+  return dtype == PW_F64 ? slab_reg_kernel<double>(c->n1, c->n2) != nullptr
+                         : slab_reg_kernel<float>(c->n1, c->n2) != nullptr;
+}
+size_t slab_lds_bytes(const pw_ctx* c, int esz, int rp, int cg) { return slab_lds(c, esz, rp, cg); }
+
 // Sphere coefficients -> z-transformed sticks T[band][z][stick] (kernel 1).
 template <typename T>
 int sphere_to_sticks(pw_ctx* c, int nb, const void* psi, void* t1) {
@@ -927,7 +938,39 @@ static int nonlocal(pw_ctx* c, int nb, const void* X, void* Y, bool accumulate) 
   const T* betaw = (const T*)(sizeof(T) == 8 ? (const void*)c->d_betaw : (const void*)c->d_betaw_f);
   T* W = (T*)(sizeof(T) == 8 ? (void*)c->d_W : (void*)c->d_W_f);
   PW_TRY(gemm_tn<T>(c, m, c->nbeta, nb, beta, m, (const T*)X, m, W, c->nbeta, nullptr, zero));
-  PW_TRY(gemm_nn<T>(c, m, nb, c->nbeta, one, betaw, m, W, c->nbeta, b2, (T*)Y, m));
+  if (!accumulate) PW_HIP(hipMemsetAsync(Y, 0, sizeof(T) * (size_t)m * nb, c->stream));
+  // Y += (beta w) W on the MFMA kernel (FP64 / FP32)
+  (void)one;
+  (void)b2;
+  PW_TRY(gemm_nn_ep<T>(c, m, nb, c->nbeta, betaw, m, W, c->nbeta, (const T*)Y, nullptr, nullptr, (T*)Y, m, nullptr,
+                       0, false));
+  return PW_OK;
+}
+
+// One scaled Chebyshev step around H: Ynew = s1 (H Y - c Y) - s2 Yp with
+// (c, s1, s2) = coef[3 step ..] on the device (Yp == null: s2 = 0).  The
+// local, kinetic and exchange terms go into HY; the projector product's
+// epilogue adds its term and forms the recurrence, so H Y is never
+// completed in memory.
+template <typename T>
+static int cheb_step(pw_ctx* c, int nb, const void* Y, void* HY, const void* Yp, void* Ynew, const T* coef,
+                     int step) {
+  PW_TRY(local_kinetic<T>(c, nb, Y, HY));
+  if (c->exx) PW_TRY(exchange_add<T>(c, nb, Y, HY));
+  const int m = 2 * c->npw;
+  const T* betaw = (const T*)(sizeof(T) == 8 ? (const void*)c->d_betaw : (const void*)c->d_betaw_f);
+  if (c->nbeta > 0) {
+    const T* beta = (const T*)(sizeof(T) == 8 ? (const void*)c->d_beta : (const void*)c->d_beta_f);
+    T* W = (T*)(sizeof(T) == 8 ? (void*)c->d_W : (void*)c->d_W_f);
+    const bool split = c->bf16_split;
+    PW_TRY(gemm_tn<T>(c, m, c->nbeta, nb, beta, m, (const T*)Y, m, W, c->nbeta, nullptr, (T)0, split));
+    PW_TRY(gemm_nn_ep<T>(c, m, nb, c->nbeta, betaw, m, W, c->nbeta, (const T*)HY, (const T*)Y, (const T*)Yp,
+                         (T*)Ynew, m, coef, step, true, split));
+  } else {
+    PW_TRY(gemm_nn_ep<T>(c, m, nb, 0, nullptr, m, nullptr, 0, (const T*)HY, (const T*)Y, (const T*)Yp, (T*)Ynew, m,
+                         coef, step, true));
+  }
+  PW_TRY(mark(c, 4));
   return PW_OK;
 }
 
@@ -941,6 +984,13 @@ static int check_call(pw_ctx* c, int dtype, int nb, const void* a, const void* b
 }
 
 }  // namespace pw
+
+int pw::cheb_step_apply(pw_ctx* c, int dtype, int nb, const void* Y, void* HY, const void* Yp, void* Ynew,
+                        int step) {
+  PW_TRY(check_call(c, dtype, nb, Y, HY));
+  if (dtype == PW_F64) return cheb_step<double>(c, nb, Y, HY, Yp, Ynew, c->d_coef, step);
+  return cheb_step<float>(c, nb, Y, HY, Yp, Ynew, c->d_coef_f, step);
+}
 
 extern "C" int pw_mult_spinor(pw_ctx* c, int dtype, int nb, const void* psi, void* a3) {
   PW_TRY(check_call(c, dtype, nb, psi, a3));

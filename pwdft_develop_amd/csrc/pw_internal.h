@@ -124,12 +124,22 @@ struct pw_ctx;
 namespace pw {
 // MFMA GEMMs (pw_gemm.hip).  C = A^T B (+ beta C), rows scaled by rowscale
 // when non-null; Y = alpha A B + beta Y.  Column-major, leading dims given.
+// split: FP32 inputs as split-BF16 products (hi hi + hi lo + lo hi on
+// v_mfma_f32_16x16x32_bf16, FP32 accumulation); ignored for FP64.
 template <typename T>
 int gemm_tn(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, int ldb, T* C, int ldc,
-            const T* rowscale, T beta);
+            const T* rowscale, T beta, bool split = false);
 template <typename T>
 int gemm_nn(pw_ctx* c, int m, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T beta, T* Y,
             int ldy);
+// out = H + A B (cheb false), or out = s1 (H + A B - c Yc) - s2 Yp with (c,
+// s1, s2) = coef[3 step ..] on the device and Yp == null meaning s2 = 0
+// (cheb true): the projector term of H Y fused with a Chebyshev step.  FP64
+// on FP64 MFMA; FP32 on FP32 MFMA, or split-BF16 MFMA when split; k = 0
+// leaves acc = 0.
+template <typename T>
+int gemm_nn_ep(pw_ctx* c, int m, int n, int k, const T* A, int lda, const T* B, int ldb, const T* H, const T* Yc,
+               const T* Yp, T* out, int ld, const T* coef, int step, bool cheb, bool split = false);
 // Dense symmetric eigensolvers (pw_eig.hip, pw_solver.hip), in place on the
 // upper triangle of A (A <- eigenvectors, w (device) <- eigenvalues
 // ascending); sym_eig adds the sign convention of the oracle and takes
@@ -143,11 +153,25 @@ int trsm_right_upper(pw_ctx* c, int m, int n, double* X, int ldx, const double* 
 int chol_inv_max_n();
 size_t chol_dinv_doubles(int n);
 int sym_eig(pw_ctx* c, int n, double* A, int lda, double* w, double* work, int* info);
+// Scaled Chebyshev recurrence coefficients of a degree-m filter damping
+// [a, b] and scaled at aL: coef[3 (k-1) ..] = (c, s1, s2) of step k, with
+// Y_1 = s1 (H - c) X and Y_k = s1 (H - c) Y_{k-1} - s2 Y_{k-2}.
+void cheb_coef(double a, double b, double aL, int m, double* coef);
 // Stick transforms of the wavefunction sphere (pw_hpsi.hip).
 template <typename T>
 int sphere_to_sticks(pw_ctx* c, int nb, const void* psi, void* t1);
 template <typename T>
 int sticks_to_sphere_add(pw_ctx* c, int nb, const void* t2, void* out);
+// Slab-kernel facts for the autotuner (pw_hpsi.hip): whether the
+// register-FFT kernel exists for the context's planes in dtype, and the LDS
+// bytes of the LDS kernel at (rp, cg) for complex elements of esz bytes.
+bool has_reg_slab(const pw_ctx* c, int dtype);
+size_t slab_lds_bytes(const pw_ctx* c, int esz, int rp, int cg);
+// One Chebyshev step Ynew = s1 (H Y - c Y) - s2 Yp (pw_hpsi.hip), the
+// step's (c, s1, s2) read from the context's device coefficient table
+// (d_coef / d_coef_f) at 3 step; Yp == null for the first step.  HY is
+// scratch (the local, kinetic and exchange terms).
+int cheb_step_apply(pw_ctx* c, int dtype, int nb, const void* Y, void* HY, const void* Yp, void* Ynew, int step);
 // Exact exchange (pw_exx.hip): a3 += V_x psi when the context holds an
 // exchange operator; state freed with the context.
 struct ExxState;
@@ -211,10 +235,17 @@ struct pw_ctx {
   double* d_coef = nullptr;
   float* d_coef_f = nullptr;
   bool use_graphs = true;
+  // FP32 projector products inside the Chebyshev filter as split-BF16 MFMA
+  // products (hi hi + hi lo + lo hi, FP32 accumulation); PW_BF16=0 keeps
+  // them on FP32 MFMA
+  bool bf16_split = true;
   bool use_reg_slab = true;  // FP32 register slab kernel where compiled (env PW_SLAB=lds: the LDS kernel)
   int slab_cg_force = 0;     // env PW_SLAB=lds:<g>[:<r>]: LDS kernel, one y buffer, g columns per transform group
   int slab_rp_force = 0;     //   (and r x row pairs per step)
   int slab_threads_force = 0;  // PW_SLAB=lds:<g>:<r>:<t>: t threads per workgroup
+  // split-K TN products: tile (1: 64 x 64, 2: 128 x 128) and the number of
+  // partial workgroups aimed at; 0 = the built-in rule (pw_autotune sets them)
+  int tn_tm = 0, tn_cap = 0;
   hipStream_t gstream = nullptr;
   hipEvent_t gev[2] = {nullptr, nullptr};
   hipGraphExec_t fg_exec = nullptr;

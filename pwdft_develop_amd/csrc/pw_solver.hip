@@ -248,8 +248,9 @@ static inline int grid_for(long n) { return (int)std::min<long>((n + 255) / 256,
 
 // Symmetric eigen-decomposition in place: A (n x n, upper triangle, lda)
 // <- eigenvectors (largest |component| positive), w (device) <- eigenvalues
-// ascending.  The tridiagonal solver of pw_eig.hip up to n = 1024; rocSOLVER
-// dsyevd beyond, or when PW_EIGEN=rocsolver.
+// ascending.  The tridiagonal solver of pw_eig.hip up to n = 512, rocSOLVER
+// dsyevd beyond; PW_EIGEN=rocsolver sends every n to rocSOLVER, PW_EIGEN=own
+// keeps the own solver up to n = 1024.
 int sym_eig(pw_ctx* c, int n, double* A, int lda, double* w, double* work, int* info) {
   hipStream_t s = c->stream;
   // own single-workgroup solver up to n = 512; rocSOLVER's multi-workgroup
@@ -508,22 +509,20 @@ extern "C" int pw_lanczos(pw_ctx* c, int k, const void* v0, double* theta, doubl
   return lanczos_run(c, PW_F64, k, v0, theta, b_up);
 }
 
-// Y = (HX - c X) s with (c, s) = coef[0..1]
-template <typename T>
-__global__ void k_cheb_first_c(T* __restrict__ Y, const T* __restrict__ HX, const T* __restrict__ X,
-                               const T* __restrict__ coef, long n) {
-  const T c = coef[0], s = coef[1];
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    Y[i] = (HX[i] - c * X[i]) * s;
-}
-
-// Yn = (HY - c Y) s1 - s2 Xp with (c, s1, s2) = coef[3 step ..]
-template <typename T>
-__global__ void k_cheb_step_c(T* __restrict__ Yn, const T* __restrict__ HY, const T* __restrict__ Y,
-                              const T* __restrict__ Xp, const T* __restrict__ coef, int step, long n) {
-  const T c = coef[3 * step], s1 = coef[3 * step + 1], s2 = coef[3 * step + 2];
-  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-    Yn[i] = (HY[i] - c * Y[i]) * s1 - s2 * Xp[i];
+void pw::cheb_coef(double a, double b, double aL, int m, double* coef) {
+  const double e = (b - a) / 2.0, cc = (b + a) / 2.0;
+  double sigma = e / (aL - cc);
+  const double tau = 2.0 / sigma;
+  coef[0] = cc;
+  coef[1] = sigma / e;
+  coef[2] = 0.0;
+  for (int it = 2; it <= m; it++) {
+    const double sn = 1.0 / (tau - sigma);
+    coef[3 * (it - 1)] = cc;
+    coef[3 * (it - 1) + 1] = 2.0 * sn / e;
+    coef[3 * (it - 1) + 2] = sigma * sn;
+    sigma = sn;
+  }
 }
 
 // Filter body on the fixed buffers: input d_ws[0], three rotating buffers
@@ -537,23 +536,15 @@ static int cheb_body(pw_ctx* c, int dtype, int nb, int m, void** result) {
   hipStream_t s = c->stream;
   void* Xp = X;       // X_{k-1}
   void* Y = bufs[0];  // X_k
-  PW_TRY(pw_apply_h(c, dtype, nb, X, HY));
-  if (dtype == PW_F64)
-    hipLaunchKernelGGL(k_cheb_first_c<double>, dim3(grid_for(n)), dim3(256), 0, s, (double*)Y, (const double*)HY,
-                       (const double*)X, (const double*)c->d_coef, n);
-  else
-    hipLaunchKernelGGL(k_cheb_first_c<float>, dim3(grid_for(n)), dim3(256), 0, s, (float*)Y, (const float*)HY,
-                       (const float*)X, (const float*)c->d_coef_f, n);
+  (void)n;
+  (void)s;
+  // each step: the local terms into HY, then the projector product whose
+  // epilogue forms the recurrence (no separate pass over the block)
+  PW_TRY(cheb_step_apply(c, dtype, nb, X, HY, nullptr, Y, 0));
   int nxt = 1;
   for (int it = 2; it <= m; it++) {
     void* Yn = bufs[nxt];
-    PW_TRY(pw_apply_h(c, dtype, nb, Y, HY));
-    if (dtype == PW_F64)
-      hipLaunchKernelGGL(k_cheb_step_c<double>, dim3(grid_for(n)), dim3(256), 0, s, (double*)Yn, (const double*)HY,
-                         (const double*)Y, (const double*)Xp, (const double*)c->d_coef, it - 1, n);
-    else
-      hipLaunchKernelGGL(k_cheb_step_c<float>, dim3(grid_for(n)), dim3(256), 0, s, (float*)Yn, (const float*)HY,
-                         (const float*)Y, (const float*)Xp, (const float*)c->d_coef_f, it - 1, n);
+    PW_TRY(cheb_step_apply(c, dtype, nb, Y, HY, Xp, Yn, it - 1));
     // rotate: Xp <- Y, Y <- Yn over the three scratch buffers
     Xp = Y;
     Y = Yn;
@@ -577,20 +568,8 @@ extern "C" int pw_cheb_filter(pw_ctx* c, int dtype, int nb, int m, double a, dou
   const size_t bytes = (dtype == PW_F64 ? 8 : 4) * (size_t)n;
   // scaled-recurrence coefficients (c, s1, s2) per step, uploaded before the
   // (possibly graph-replayed) filter body reads them
-  {
-    const double e = (b - a) / 2.0, cc = (b + a) / 2.0;
-    double sigma = e / (aL - cc);
-    const double tau = 2.0 / sigma;
-    c->h_coef[0] = cc; c->h_coef[1] = sigma / e; c->h_coef[2] = 0.0;
-    for (int it = 2; it <= m; it++) {
-      const double sn = 1.0 / (tau - sigma);
-      c->h_coef[3 * (it - 1)] = cc;
-      c->h_coef[3 * (it - 1) + 1] = 2.0 * sn / e;
-      c->h_coef[3 * (it - 1) + 2] = sigma * sn;
-      sigma = sn;
-    }
-    for (int i = 0; i < 3 * m; i++) c->h_coef_f[i] = (float)c->h_coef[i];
-  }
+  cheb_coef(a, b, aL, m, c->h_coef);
+  for (int i = 0; i < 3 * m; i++) c->h_coef_f[i] = (float)c->h_coef[i];
   hipStream_t s = c->stream;
   PW_HIP(hipMemcpyAsync(c->d_coef, c->h_coef, sizeof(double) * 3 * m, hipMemcpyHostToDevice, s));
   PW_HIP(hipMemcpyAsync(c->d_coef_f, c->h_coef_f, sizeof(float) * 3 * m, hipMemcpyHostToDevice, s));

@@ -1,0 +1,253 @@
+// Autotuner of the H*psi plans: measures candidate xy-slab FFT layouts and
+// split-K tilings of the projector product on the device and keeps the
+// fastest, with a text cache of the choices per shape.
+//
+// The reference's only tuning is FFTW's planner: Fourier::Initialize builds
+// its plans with FFTW_MEASURE (src/SCF/fourier.cpp:62, 112-184), i.e. it
+// times candidate algorithms for each transform shape once and keeps the
+// fastest.  pw_autotune plays that part for this library's transforms and
+// products; pw_batch_autotune does it for the batched persistent launch's
+// work split.
+//
+// Cache file: one line per tuned shape, "<key> <choice...>", keys made of the
+// device name and CU count, grid, sphere size, band count, precision and the
+// stage ("slab" or "tn<nbeta>").  Lines are appended; the last line of a key
+// wins.  A shape found in the cache is applied without timing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "pw_internal.h"
+
+namespace pw {
+
+__global__ void k_tune_fill(float* __restrict__ x, long n, float scale) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    x[i] = scale * __sinf(0.7548776662f * (float)(i % 100003) + 0.1f);
+}
+__global__ void k_tune_fill_d(double* __restrict__ x, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    x[i] = sin(0.7548776662 * (double)(i % 100003) + 0.1);
+}
+
+static std::map<std::string, std::string> read_cache(const char* path) {
+  std::map<std::string, std::string> m;
+  if (!path) return m;
+  std::ifstream f(path);
+  std::string line;
+  while (std::getline(f, line)) {
+    const size_t sp = line.find(' ');
+    if (sp == std::string::npos || line.empty() || line[0] == '#') continue;
+    m[line.substr(0, sp)] = line.substr(sp + 1);
+  }
+  return m;
+}
+
+static void append_cache(const char* path, const std::string& key, const std::string& val) {
+  if (!path) return;
+  std::ofstream f(path, std::ios::app);
+  f << key << " " << val << "\n";
+}
+
+static std::string device_tag(int device) {
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, device) != hipSuccess) return "dev";
+  std::string name = p.gcnArchName;
+  const size_t colon = name.find(':');
+  if (colon != std::string::npos) name = name.substr(0, colon);
+  return name + "-" + std::to_string(p.multiProcessorCount) + "cu";
+}
+
+// Median-free timing: the fastest of `reps` runs of body() after one warm run,
+// by HIP events on the context's stream (ms).
+template <typename Body>
+static int time_min(hipStream_t s, int reps, Body body, double* ms) {
+  hipEvent_t e0, e1;
+  PW_HIP(hipEventCreate(&e0));
+  PW_HIP(hipEventCreate(&e1));
+  int rc = body();
+  double best = 1e30;
+  for (int r = 0; r < reps && rc == PW_OK; r++) {
+    (void)hipEventRecord(e0, s);
+    rc = body();
+    (void)hipEventRecord(e1, s);
+    if (hipEventSynchronize(e1) != hipSuccess) rc = fail(PW_ERR_HIP, "autotune: event wait failed");
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    best = std::min(best, (double)t);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = best;
+  return rc;
+}
+
+struct SlabChoice {
+  bool reg;
+  int cg, rp, threads;
+  std::string str() const {
+    if (reg) return "reg";
+    return "lds " + std::to_string(cg) + " " + std::to_string(rp) + " " + std::to_string(threads);
+  }
+};
+
+static void apply_slab(pw_ctx* c, const SlabChoice& ch) {
+  c->use_reg_slab = ch.reg;
+  c->slab_cg_force = ch.reg ? 0 : ch.cg;
+  c->slab_rp_force = ch.reg ? 0 : ch.rp;
+  c->slab_threads_force = ch.reg ? 0 : ch.threads;
+}
+
+static bool parse_slab(const std::string& v, SlabChoice* ch) {
+  if (v == "reg") {
+    *ch = SlabChoice{true, 0, 0, 0};
+    return true;
+  }
+  int cg = 0, rp = 0, th = 0;
+  if (sscanf(v.c_str(), "lds %d %d %d", &cg, &rp, &th) == 3 && cg >= 0 && rp > 0 &&
+      (th == 256 || th == 512 || th == 1024)) {
+    *ch = SlabChoice{false, cg, rp, th};
+    return true;
+  }
+  return false;
+}
+
+}  // namespace pw
+
+using namespace pw;
+
+extern "C" int pw_autotune(pw_ctx* c, int dtype, int nb, const char* cache_path, char* summary, int len) {
+  if (!c || nb < 1 || nb > c->max_bands) return fail(PW_ERR_ARG, "pw_autotune: bad argument");
+  if (dtype != PW_F64 && dtype != PW_F32) return fail(PW_ERR_ARG, "unknown dtype %d", dtype);
+  if (!c->have_vtot) return fail(PW_ERR_STATE, "vtot has not been set");
+  const int esz = dtype == PW_F64 ? 16 : 8;
+  const std::string base = device_tag(c->device) + "|" + std::to_string(c->n1) + "x" + std::to_string(c->n2) + "x" +
+                           std::to_string(c->n3) + "|npw" + std::to_string(c->npw) + "|nb" + std::to_string(nb) +
+                           (dtype == PW_F64 ? "|f64" : "|f32");
+  const std::string kslab = base + "|slab", ktn = base + "|tn" + std::to_string(c->nbeta);
+  auto cache = read_cache(cache_path);
+  std::string out;
+  // blocks to time on (deterministic content)
+  const long m = 2L * c->npw * nb;
+  void *X = nullptr, *Y = nullptr, *W = nullptr;
+  auto done = [&](int rc) {
+    if (X) (void)hipFree(X);
+    if (Y) (void)hipFree(Y);
+    if (W) (void)hipFree(W);
+    if (rc == PW_OK && summary && len > 0) snprintf(summary, (size_t)len, "%s", out.c_str());
+    return rc;
+  };
+  const size_t es = dtype == PW_F64 ? 8 : 4;
+  if (hipMalloc(&X, es * m) != hipSuccess || hipMalloc(&Y, es * m) != hipSuccess)
+    return done(fail(PW_ERR_NOMEM, "pw_autotune: cannot allocate the timing blocks"));
+  const int g = (int)std::min<long>((m + 255) / 256, 8192);
+  if (dtype == PW_F64)
+    hipLaunchKernelGGL(k_tune_fill_d, dim3(g), dim3(256), 0, c->stream, (double*)X, m);
+  else
+    hipLaunchKernelGGL(k_tune_fill, dim3(g), dim3(256), 0, c->stream, (float*)X, m, 1.f);
+  // ---- xy slab layout ----
+  SlabChoice best{false, 0, 0, 256};
+  auto it = cache.find(kslab);
+  if (it != cache.end() && parse_slab(it->second, &best)) {
+    apply_slab(c, best);
+    out += "slab " + best.str() + " (cached)";
+  } else {
+    std::vector<SlabChoice> cand;
+    if (has_reg_slab(c, dtype)) cand.push_back({true, 0, 0, 0});
+    const int nk = c->sp.kxmax + 1, npair = (c->n2 + 1) / 2;
+    for (int rp : {1, 2, 4, 8, 16})
+      if (rp <= npair && slab_lds_bytes(c, esz, rp, 0) <= 160 * 1024)
+        for (int th : {256, 512}) cand.push_back({false, 0, rp, th});
+    for (int cg : {nk, std::max(1, nk / 2), 8, 4}) {
+      if (cg > nk) continue;
+      for (int rp : {4, 8, 16, std::max(1, std::min(npair, nk * c->n2 / (2 * c->n1)))})
+        if (rp <= npair && slab_lds_bytes(c, esz, rp, cg) <= 160 * 1024)
+          for (int th : {512, 1024}) cand.push_back({false, cg, rp, th});
+    }
+    double tbest = 1e30;
+    const bool saved_reg = c->use_reg_slab;
+    const int saved[3] = {c->slab_cg_force, c->slab_rp_force, c->slab_threads_force};
+    int tried = 0;
+    for (const SlabChoice& ch : cand) {
+      apply_slab(c, ch);
+      double ms = 0;
+      const int rc = time_min(c->stream, 3, [&] { return pw_mult_spinor(c, dtype, nb, X, Y); }, &ms);
+      if (rc != PW_OK) {
+        // a layout this device cannot launch: skip it
+        continue;
+      }
+      tried++;
+      if (ms < tbest) {
+        tbest = ms;
+        best = ch;
+      }
+    }
+    if (tried == 0) {
+      c->use_reg_slab = saved_reg;
+      c->slab_cg_force = saved[0];
+      c->slab_rp_force = saved[1];
+      c->slab_threads_force = saved[2];
+      return done(fail(PW_ERR_HIP, "pw_autotune: no slab layout ran"));
+    }
+    apply_slab(c, best);
+    append_cache(cache_path, kslab, best.str());
+    char b[96];
+    snprintf(b, sizeof(b), " (%.4f ms over %d layouts)", tbest, tried);
+    out += "slab " + best.str() + b;
+  }
+  // ---- projector product W = beta^T X: split-K tiling ----
+  if (c->nbeta > 0) {
+    int tm = 0, cap = 0;
+    auto jt = cache.find(ktn);
+    if (jt != cache.end() && sscanf(jt->second.c_str(), "%d %d", &tm, &cap) == 2 && (tm == 1 || tm == 2) && cap > 0) {
+      c->tn_tm = tm;
+      c->tn_cap = cap;
+      out += "; tn " + std::to_string(tm) + " " + std::to_string(cap) + " (cached)";
+    } else {
+      if (hipMalloc(&W, es * (size_t)c->nbeta * nb) != hipSuccess)
+        return done(fail(PW_ERR_NOMEM, "pw_autotune: cannot allocate the projection block"));
+      const int K = 2 * c->npw;
+      double tbest = 1e30;
+      int btm = 1, bcap = 1024;
+      for (int t : {1, 2})
+        for (int cp : {256, 512, 1024, 2048}) {
+          c->tn_tm = t;
+          c->tn_cap = cp;
+          double ms = 0;
+          int rc;
+          if (dtype == PW_F64)
+            rc = time_min(c->stream, 3, [&] {
+              return gemm_tn<double>(c, K, c->nbeta, nb, c->d_beta, K, (const double*)X, K, (double*)W, c->nbeta,
+                                     nullptr, 0.0);
+            }, &ms);
+          else
+            rc = time_min(c->stream, 3, [&] {
+              return gemm_tn<float>(c, K, c->nbeta, nb, c->d_beta_f, K, (const float*)X, K, (float*)W, c->nbeta,
+                                    nullptr, 0.f);
+            }, &ms);
+          if (rc == PW_OK && ms < tbest) {
+            tbest = ms;
+            btm = t;
+            bcap = cp;
+          }
+        }
+      c->tn_tm = btm;
+      c->tn_cap = bcap;
+      append_cache(cache_path, ktn, std::to_string(btm) + " " + std::to_string(bcap));
+      char b[96];
+      snprintf(b, sizeof(b), "; tn %d %d (%.4f ms)", btm, bcap, tbest);
+      out += b;
+    }
+  }
+  invalidate_graph(c);
+  if (hipStreamSynchronize(c->stream) != hipSuccess) return done(fail(PW_ERR_HIP, "pw_autotune: device error"));
+  return done(PW_OK);
+}

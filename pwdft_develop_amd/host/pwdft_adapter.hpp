@@ -11,8 +11,10 @@
 //   KohnSham          include/hamiltonian.hpp:383-501 (MultSpinor, CalculateEkin,
 //                     Teter, Vnlc, EigVal, density)
 //   EigenSolver       include/eigensolver.hpp:59-219 (Setup, LOBPCGSolveReal,
-//                     NonlocalMultX, FirstChebyStep, GeneralChebyStep, EigVal,
-//                     ResVal)
+//                     PPCGSolveReal, NonlocalMultX, FirstChebyStep,
+//                     GeneralChebyStep, EigVal, ResVal)
+//   esdfParam         include/esdf.hpp: the two keys the solvers read
+//                     (PPCGsbSize, isUseRealSpace)
 //   ErrorHandling     src/global.cpp:71-91: throws std::runtime_error
 //   SparseVec, NonlocalPP, PseudoPot   include/utility.hpp:120-152 (the
 //                     real-space projectors, vnlList only)
@@ -22,7 +24,9 @@
 // MultSpinor overload (hamiltonian.cpp:3058-3125).
 //
 // Host arrays stay host arrays (as in the reference); each call moves the
-// band block to the device, runs the kernels there and copies results back.
+// band block to the device, runs the kernels there and copies results back
+// through staging blocks that the Hamiltonian and the EigenSolver keep and
+// only grow (no device allocation per call once the largest block was seen).
 // A Fourier owns one device context, so (like the reference's Fourier with its
 // shared scratch) it is not re-entrant.
 #pragma once
@@ -45,6 +49,15 @@ typedef double Real;
 typedef std::complex<double> Complex;
 
 inline void ErrorHandling(const std::string& msg) { throw std::runtime_error(msg); }
+
+// The input keys of esdfParam (include/esdf.hpp:473 and its neighbours)
+// that EigenSolver reads: PPCG's sub-block size (src/esdf.cpp:2722, default
+// 1) and the real-space switch (ppcg.cpp:75).
+struct ESDFInputParam {
+  Int PPCGsbSize = 1;
+  bool isUseRealSpace = false;
+};
+inline ESDFInputParam esdfParam;
 inline void Check(int rc) {
   if (rc != PW_OK) ErrorHandling(pw_last_error());
 }
@@ -337,11 +350,28 @@ class Spinor {
 };
 
 // ---- Hamiltonian / KohnSham (include/hamiltonian.hpp) ----
+// The base class holds what the solvers read (hamiltonian.hpp:71-376: the
+// device context in place of the host operator data, ekin_/teter_, vnlc_,
+// eigVal_) so that EigenSolver::Setup takes a Hamiltonian&, as the
+// reference's does.
 class Hamiltonian {
  public:
   virtual ~Hamiltonian() = default;
   virtual void MultSpinor(Spinor& psi, NumTns<Complex>& a3, Fourier& fft) = 0;
   virtual void MultSpinor(Spinor& psi, NumTns<Real>& a3, Fourier& fft) = 0;
+  DblNumVec& Teter() { return teter_; }
+  DblNumVec& Ekin() { return ekin_; }
+  std::pair<CpxNumMat, DblNumVec>& Vnlc() { return vnlc_; }
+  DblNumVec& EigVal() { return eigVal_; }
+  pw_ctx* Ctx() const { return ctx_; }
+  pw_rs* RsCtx() const { return rs_; }
+
+ protected:
+  pw_ctx* ctx_ = nullptr;
+  pw_rs* rs_ = nullptr;
+  DblNumVec ekin_, teter_, vtot_, eigVal_;
+  std::pair<CpxNumMat, DblNumVec> vnlc_;
+  DeviceBlock dx_, dy_;  // staging of MultSpinor's blocks, grown on demand
 };
 
 class KohnSham : public Hamiltonian {
@@ -373,11 +403,10 @@ class KohnSham : public Hamiltonian {
     if (fft.domain.numGridSphere != ntot) ErrorHandling("Domain size does not match3.");
     if (psi.NumComponent() != 1)
       ErrorHandling("Component number of spinor should be one in spin-restricted and spin-unrestricted calculations.");
-    DeviceBlock x, y;
-    x.Upload(psi.WavefunG().Data(), (size_t)ntot * nb);
-    y.Alloc((size_t)ntot * nb);
-    Check(pw_mult_spinor(ctx_, PW_F64, nb, x.Ptr(), y.Ptr()));
-    y.Download(a3.Data(), (size_t)ntot * nb);
+    dx_.Upload(psi.WavefunG().Data(), (size_t)ntot * nb);
+    dy_.Alloc((size_t)ntot * nb);
+    Check(pw_mult_spinor(ctx_, PW_F64, nb, dx_.Ptr(), dy_.Ptr()));
+    dy_.Download(a3.Data(), (size_t)ntot * nb);
   }
   // CalculateEkin without sphere cut (hamiltonian.cpp:2629-2667 with
   // isUseSphereCut false): |G|^2/2 and Teter over the coarse R2C grid, G from
@@ -460,38 +489,29 @@ class KohnSham : public Hamiltonian {
     if (psi.Wavefun().n() != 1)
       ErrorHandling("Component number of spinor should be one in spin-restricted and spin-unrestricted calculations.");
     pw_rs* rs = fft.RsCtx();
-    DeviceBlock x, y;
-    x.UploadReal(psi.Wavefun().Data(), (size_t)ntot * nb);
+    dx_.UploadReal(psi.Wavefun().Data(), (size_t)ntot * nb);
     if (apply_filter_ && apply_first_) {
       apply_first_ = 0;
-      Check(pw_rs_filter(rs, wfn_cutoff_, nb, static_cast<double*>(x.Ptr())));
-      x.DownloadReal(psi.Wavefun().Data(), (size_t)ntot * nb);
+      Check(pw_rs_filter(rs, wfn_cutoff_, nb, static_cast<double*>(dx_.Ptr())));
+      dx_.DownloadReal(psi.Wavefun().Data(), (size_t)ntot * nb);
     }
-    y.Alloc(((size_t)ntot * nb + 1) / 2);
-    Check(pw_rs_mult_spinor(rs, nb, static_cast<const double*>(x.Ptr()), static_cast<double*>(y.Ptr())));
-    if (apply_filter_) Check(pw_rs_filter(rs, wfn_cutoff_, nb, static_cast<double*>(y.Ptr())));
-    y.DownloadReal(a3.Data(), (size_t)ntot * nb);
+    dy_.Alloc(((size_t)ntot * nb + 1) / 2);
+    Check(pw_rs_mult_spinor(rs, nb, static_cast<const double*>(dx_.Ptr()), static_cast<double*>(dy_.Ptr())));
+    if (apply_filter_) Check(pw_rs_filter(rs, wfn_cutoff_, nb, static_cast<double*>(dy_.Ptr())));
+    dy_.DownloadReal(a3.Data(), (size_t)ntot * nb);
   }
   DblNumVec& EkinR2C() { return ekinR2C_; }
-  DblNumVec& Teter() { return teter_; }
-  DblNumVec& Ekin() { return ekin_; }
-  std::pair<CpxNumMat, DblNumVec>& Vnlc() { return vnlc_; }
-  DblNumVec& EigVal() { return eigVal_; }
-  pw_ctx* Ctx() const { return ctx_; }
 
  private:
-  pw_ctx* ctx_ = nullptr;
-  pw_rs* rs_ = nullptr;
   bool apply_filter_ = false, apply_first_ = false;
   Real wfn_cutoff_ = 0.0;
-  DblNumVec ekin_, teter_, vtot_, eigVal_, ekinR2C_, teterR2C_;
-  std::pair<CpxNumMat, DblNumVec> vnlc_;
+  DblNumVec ekinR2C_, teterR2C_;
 };
 
 // ---- EigenSolver (include/eigensolver.hpp) ----
 class EigenSolver {
  public:
-  void Setup(KohnSham& ham, Spinor& psi, Fourier& fft) {
+  void Setup(Hamiltonian& ham, Spinor& psi, Fourier& fft) {
     hamPtr_ = &ham;
     psiPtr_ = &psi;
     fftPtr_ = &fft;
@@ -501,28 +521,50 @@ class EigenSolver {
   DblNumVec& EigVal() { return eigVal_; }
   DblNumVec& ResVal() { return resVal_; }
   Spinor& Psi() { return *psiPtr_; }
-  KohnSham& Ham() { return *hamPtr_; }
+  Hamiltonian& Ham() { return *hamPtr_; }
   Fourier& FFT() { return *fftPtr_; }
 
   // NonlocalMultX (eigensolver.cpp:471-515): VnlX = beta (w .* Re(beta^H X))
   void NonlocalMultX(Int heightLocal, Int width, CpxNumMat& X, CpxNumMat& VnlX) {
-    DeviceBlock x, y;
-    x.Upload(X.Data(), (size_t)heightLocal * width);
-    y.Alloc((size_t)heightLocal * width);
-    Check(pw_nonlocal_multx(hamPtr_->Ctx(), PW_F64, width, x.Ptr(), y.Ptr()));
-    y.Download(VnlX.Data(), (size_t)heightLocal * width);
+    dx_.Upload(X.Data(), (size_t)heightLocal * width);
+    dy_.Alloc((size_t)heightLocal * width);
+    Check(pw_nonlocal_multx(hamPtr_->Ctx(), PW_F64, width, dx_.Ptr(), dy_.Ptr()));
+    dy_.Download(VnlX.Data(), (size_t)heightLocal * width);
   }
 
   // LOBPCGSolveReal (lobpcg.cpp:59-576)
   void LOBPCGSolveReal(Int numEig, Int scfIter, Int eigMaxIter, Real eigMinTolerance, Real eigTolerance) {
     Spinor& psi = *psiPtr_;
     const Int ntot = psi.NumGridTotal(), nb = psi.NumStateTotal();
-    DeviceBlock x;
-    x.Upload(psi.WavefunG().Data(), (size_t)ntot * nb);
+    dx_.Upload(psi.WavefunG().Data(), (size_t)ntot * nb);
     int iters = 0;
-    Check(pw_lobpcg(hamPtr_->Ctx(), nb, numEig, scfIter, eigMaxIter, eigMinTolerance, eigTolerance, x.Ptr(),
+    Check(pw_lobpcg(hamPtr_->Ctx(), nb, numEig, scfIter, eigMaxIter, eigMinTolerance, eigTolerance, dx_.Ptr(),
                     eigVal_.Data(), resVal_.Data(), &iters));
-    x.Download(psi.WavefunG().Data(), (size_t)ntot * nb);
+    dx_.Download(psi.WavefunG().Data(), (size_t)ntot * nb);
+    hamPtr_->EigVal() = eigVal_;
+  }
+
+  // PPCGSolveReal (ppcg.cpp:59-1246): the sphere branch on WavefunG or, with
+  // esdfParam.isUseRealSpace, the real-space branch on Wavefun (ppcg.cpp:75,
+  // 683); sub-blocks of esdfParam.PPCGsbSize (ppcg.cpp:335).
+  void PPCGSolveReal(Int numEig, Int scfIter, Int eigMaxIter, Real eigMinTolerance, Real eigTolerance) {
+    Spinor& psi = *psiPtr_;
+    int iters = 0;
+    if (esdfParam.isUseRealSpace) {
+      const Int ntot = psi.Wavefun().m(), nb = psi.Wavefun().p();
+      pw_rs* rs = hamPtr_->RsCtx();
+      if (!rs) ErrorHandling("Fourier is not prepared.");
+      dx_.UploadReal(psi.Wavefun().Data(), (size_t)ntot * nb);
+      Check(pw_rs_ppcg(rs, nb, numEig, scfIter, eigMaxIter, eigMinTolerance, eigTolerance, esdfParam.PPCGsbSize,
+                       static_cast<double*>(dx_.Ptr()), eigVal_.Data(), resVal_.Data(), &iters));
+      dx_.DownloadReal(psi.Wavefun().Data(), (size_t)ntot * nb);
+    } else {
+      const Int ntot = psi.NumGridTotal(), nb = psi.NumStateTotal();
+      dx_.Upload(psi.WavefunG().Data(), (size_t)ntot * nb);
+      Check(pw_ppcg(hamPtr_->Ctx(), nb, numEig, scfIter, eigMaxIter, eigMinTolerance, eigTolerance,
+                    esdfParam.PPCGsbSize, dx_.Ptr(), eigVal_.Data(), resVal_.Data(), &iters));
+      dx_.Download(psi.WavefunG().Data(), (size_t)ntot * nb);
+    }
     hamPtr_->EigVal() = eigVal_;
   }
 
@@ -534,21 +576,19 @@ class EigenSolver {
     (void)numEig;
     Spinor& psi = *psiPtr_;
     const Int ntot = psi.NumGridTotal(), nb = psi.NumStateTotal();
-    DeviceBlock x;
-    x.Upload(psi.WavefunG().Data(), (size_t)ntot * nb);
-    Check(pw_rayleigh_ritz(hamPtr_->Ctx(), PW_F64, nb, x.Ptr(), eigVal_.Data(), resVal_.Data()));
-    for (Int c = 0; c < cycleNum; c++) Sweep(x, filterOrder, mixed);
-    x.Download(psi.WavefunG().Data(), (size_t)ntot * nb);
+    dx_.Upload(psi.WavefunG().Data(), (size_t)ntot * nb);
+    Check(pw_rayleigh_ritz(hamPtr_->Ctx(), PW_F64, nb, dx_.Ptr(), eigVal_.Data(), resVal_.Data()));
+    for (Int c = 0; c < cycleNum; c++) Sweep(dx_, filterOrder, mixed);
+    dx_.Download(psi.WavefunG().Data(), (size_t)ntot * nb);
     hamPtr_->EigVal() = eigVal_;
   }
   void GeneralChebyStep(Int numEig, Int filterOrder, bool mixed = false) {
     (void)numEig;
     Spinor& psi = *psiPtr_;
     const Int ntot = psi.NumGridTotal(), nb = psi.NumStateTotal();
-    DeviceBlock x;
-    x.Upload(psi.WavefunG().Data(), (size_t)ntot * nb);
-    Sweep(x, filterOrder, mixed);
-    x.Download(psi.WavefunG().Data(), (size_t)ntot * nb);
+    dx_.Upload(psi.WavefunG().Data(), (size_t)ntot * nb);
+    Sweep(dx_, filterOrder, mixed);
+    dx_.Download(psi.WavefunG().Data(), (size_t)ntot * nb);
     hamPtr_->EigVal() = eigVal_;
   }
   // Lanczos start vector of the spectral-bound estimate (one band).
@@ -561,7 +601,7 @@ class EigenSolver {
   void Sweep(DeviceBlock& x, Int filterOrder, bool mixed) {
     const Int nb = psiPtr_->NumStateTotal();
     if (lanczosStart_.empty()) ErrorHandling("EigenSolver: no Lanczos start vector (SetLanczosStart)");
-    DeviceBlock v0;
+    DeviceBlock& v0 = dv0_;
     v0.Upload(lanczosStart_.data(), lanczosStart_.size());
     std::vector<double> prev(eigVal_.Data(), eigVal_.Data() + nb);
     const bool havePrev = !(prev.empty() || (prev.front() == 0.0 && prev.back() == 0.0));
@@ -570,8 +610,9 @@ class EigenSolver {
                            havePrev ? prev.data() : nullptr, x.Ptr(), eigVal_.Data(), resVal_.Data(), bounds));
   }
 
-  KohnSham* hamPtr_ = nullptr;
+  Hamiltonian* hamPtr_ = nullptr;
   Spinor* psiPtr_ = nullptr;
+  DeviceBlock dx_, dy_, dv0_;  // staging blocks, grown on demand
   Fourier* fftPtr_ = nullptr;
   DblNumVec eigVal_, resVal_;
   std::vector<Complex> lanczosStart_;

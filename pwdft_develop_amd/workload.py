@@ -24,7 +24,7 @@ import numpy as np
 from . import _lib
 
 MASTER_SEED = 20261202
-STREAM_PSI, STREAM_VNOISE, STREAM_WGT, STREAM_ATOMS, STREAM_LANCZOS = 0, 1, 2, 3, 4
+STREAM_PSI, STREAM_VNOISE, STREAM_WGT, STREAM_ATOMS, STREAM_LANCZOS, STREAM_BATCH = 0, 1, 2, 3, 4, 5
 
 _M0, _M1 = np.uint64(0xD2511F53), np.uint64(0xCD9E8D57)
 _W0, _W1 = np.uint64(0x9E3779B9), np.uint64(0xBB67AE85)
@@ -276,6 +276,29 @@ def layered432_96(nb=432):
     reaches them); 432 bands (the occupied count of the exchange apply)."""
     return Problem("layered432_96", (96, 96, 40), np.diag(LAYERED_CELL).astype(np.float64), 2.8,
                    layered432_atoms(), nb)
+
+
+BATCH_GRIDS = (24, 30, 36, 40, 48, 60, 72)
+
+
+def batch_cells(ncell=48, seed=MASTER_SEED, min_sep=2.0):
+    """Uneven cells of the batched workload: per cell a cubic grid n drawn
+    from BATCH_GRIDS, side L = n pi / sqrt(80) bohr (so that Ecut 10 Ha
+    gives that grid by the reference's rule), 2..32 atoms at random
+    positions (minimum separation min_sep), 16..128 bands; the potential,
+    projector and wavefunction recipe of the other workloads with a seed of
+    its own per cell.  Returns unbuilt Problems."""
+    u = random_uniform(seed, STREAM_BATCH, 3 * ncell).reshape(ncell, 3)
+    cells = []
+    for i in range(ncell):
+        n = BATCH_GRIDS[min(len(BATCH_GRIDS) - 1, int(u[i, 0] * len(BATCH_GRIDS)))]
+        natom = 2 + min(30, int(u[i, 1] * 31))
+        nb = 16 + min(112, int(u[i, 2] * 113))
+        L = n * np.pi / np.sqrt(80.0)
+        cseed = seed + 1000 + i
+        cells.append(Problem(f"batch{i:02d}_{n}", (n, n, n), cubic(L), 10.0,
+                             random_atoms(natom, L, min_sep, cseed), nb, seed=cseed))
+    return cells
 
 
 WORKLOADS = {"si8_32": si8_32, "si64_60": si64_60, "rand16_48": rand16_48, "layered432_180": layered432_180,

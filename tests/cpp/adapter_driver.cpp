@@ -1,7 +1,7 @@
 // Drives the C++ adapter (pwdft_develop_amd/host/pwdft_adapter.hpp) the way
 // the reference's SCF driver calls its classes: Fourier::Initialize,
 // KohnSham::CalculateEkin / MultSpinor, EigenSolver::NonlocalMultX,
-// LOBPCGSolveReal and GeneralChebyStep.
+// LOBPCGSolveReal, GeneralChebyStep and PPCGSolveReal.
 //
 //   adapter_driver <input.bin> <output.bin>
 //
@@ -9,7 +9,8 @@
 //        beta[2 npw nbeta]; wgt[nbeta]; psi[2 npw nb]; v0[2 npw]
 // output: float64 Hpsi[2 npw nb] (MultSpinor + NonlocalMultX), LOBPCG
 //         eigenvalues[nb] (3 iterations), ChebFSI eigenvalues[nb] (one
-//         GeneralChebyStep after the LOBPCG, degree 10, 8 Lanczos steps)
+//         GeneralChebyStep after the LOBPCG, degree 10, 8 Lanczos steps),
+//         PPCG eigenvalues[nb] (3 iterations from the input block, sbSize 8)
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -128,6 +129,7 @@ int main(int argc, char** argv) {
     rd(f, reinterpret_cast<double*>(psi.data()), 2 * psi.size());
     rd(f, reinterpret_cast<double*>(v0.data()), 2 * v0.size());
     fclose(f);
+    const std::vector<Complex> psi0 = psi;
 
     Fourier fft;
     fft.Initialize(dm, ecut, nb);
@@ -156,6 +158,18 @@ int main(int argc, char** argv) {
     eig.SetLanczosStart(v0, 8);
     eig.GeneralChebyStep(nb, 10);
     for (int i = 0; i < nb; i++) out.push_back(eig.EigVal()[i]);
+    // PPCGSolveReal through a solver set up on the Hamiltonian base class,
+    // on a fresh copy of the input block (sub-blocks of 8, 3 iterations)
+    {
+      std::vector<Complex> psi2 = psi0;
+      Spinor spn2(fft.domain, 1, nb, false, psi2.data());
+      Hamiltonian& hb = ham;
+      EigenSolver eig2;
+      eig2.Setup(hb, spn2, fft);
+      esdfParam.PPCGsbSize = 8;
+      eig2.PPCGSolveReal(nb, 1, 3, 0.0, 1e-20);
+      for (int i = 0; i < nb; i++) out.push_back(eig2.EigVal()[i]);
+    }
     FILE* g = fopen(argv[2], "wb");
     if (!g || fwrite(out.data(), sizeof(double), out.size(), g) != out.size()) ErrorHandling("cannot write output");
     fclose(g);

@@ -1,5 +1,6 @@
 """The C++ adapter (reference class interfaces over the C-ABI) driven from a
-C++ program: H*psi, LOBPCG and a GeneralChebyStep against the oracle."""
+C++ program: H*psi, LOBPCG, a GeneralChebyStep and PPCGSolveReal (on a
+solver set up through the Hamiltonian base class) against the oracle."""
 import os
 import subprocess
 
@@ -37,7 +38,8 @@ def test_adapter_driver(tmp_path):
     n = 2 * prob.npw * prob.nb
     hpsi = res[:n].view(np.complex128).reshape(prob.nb, prob.npw)
     e_lob = res[n:n + prob.nb]
-    e_cheb = res[n + prob.nb:]
+    e_cheb = res[n + prob.nb:n + 2 * prob.nb]
+    e_ppcg = res[n + 2 * prob.nb:]
     H = O.Hamiltonian(prob)
     ref = H.apply_h(prob.psi)
     assert np.abs(hpsi - ref).max() < 1e-11 * np.abs(ref).max()
@@ -45,6 +47,8 @@ def test_adapter_driver(tmp_path):
     assert np.abs(e_lob - eo).max() < 1e-10
     _, ec, _, _ = H.chebfsi_sweep(Xo, 10, 8, v0, eo)
     assert np.abs(e_cheb - ec).max() < 1e-9
+    _, ep, _, _ = H.ppcg(prob.psi, max_iter=3, sb_size=8)
+    assert e_ppcg.shape == ep.shape and np.abs(e_ppcg - ep).max() < 1e-10
 
 
 def test_adapter_driver_reports_errors(tmp_path):

@@ -145,3 +145,25 @@ def test_chebfsi_si64_sweeps():
     print(f"si64 warm mixed-vs-fp64 Ritz err: wanted {err[:24].max():.2e} all {err.max():.2e}")
     assert err[:24].max() < 1e-8
     ctx.close()
+
+
+@pytest.mark.parametrize("split", ["1", "0"])
+def test_cheb_filter_fp32_projector_products(si8, split, monkeypatch):
+    """The FP32 degree-10 filter against the FP64 oracle filter: the
+    projector products inside the filter as split-BF16 MFMA products
+    (hi hi + hi lo + lo hi, FP32 accumulation; the default) or FP32 MFMA
+    products (PW_BF16=0).  The split products carry 16 significant bits per
+    operand (the dropped lo lo term is 2^-16 relative), the FP32 ones 24:
+    relative errors within 5e-5 and 2e-5 of the largest output entry."""
+    import torch
+    from pwdft_develop_amd.ops import Context
+    print(f"case: si8_32 fp32 filter split={split}", flush=True)
+    monkeypatch.setenv("PW_BF16", split)
+    ctx = Context(si8)
+    X = ctx.to_device(si8.psi, torch.complex64)
+    ctx.cheb_filter(X, 10, 0.3, 12.0, -1.0)
+    ref = O.Hamiltonian(si8).cheb_filter(si8.psi, 10, 0.3, 12.0, -1.0)
+    err = float(np.abs(X.cpu().numpy() - ref).max() / np.abs(ref).max())
+    print(f"fp32 filter (split {split}) relative error {err:.2e}")
+    assert err < (5e-5 if split == "1" else 2e-5)
+    ctx.close()

@@ -63,6 +63,58 @@ def test_add_exx_layered_subgrid():
     ctx.close()
 
 
+@pytest.mark.parametrize("pair,wfn", [("5:3:512", "3:5:1024"), ("7:5:1024", "4:3:256")])
+def test_add_exx_forced_grouped_layouts(pair, wfn, monkeypatch):
+    """The grouped y layouts of the pair kernels (PW_EXX_LAYOUT) and of the
+    wavefunction-side slab kernels (PW_WFN_LAYOUT) forced on the si8_32
+    grid with column groups and row-pair counts that are not powers of two
+    and leave ragged last groups (16 row pairs in steps of 3 or 5): FP64
+    against the oracle."""
+    from pwdft_develop_amd.ops import Context
+    prob = W.si8_32(nb=6).build()
+    nocc = 5
+    print(f"case: si8_32 exx layouts pair={pair} wfn={wfn}", flush=True)
+    occ = np.linspace(1.0, 0.4, nocc)
+    H = O.Hamiltonian(prob)
+    H.set_exx(prob.ecut, MU, FRAC, prob.psi[:nocc], occ)
+    ref = H.add_exx(prob.psi)
+    monkeypatch.setenv("PW_EXX_LAYOUT", pair)
+    monkeypatch.setenv("PW_WFN_LAYOUT", wfn)
+    ctx = Context(prob)
+    ctx.set_exx(prob.ecut, MU, FRAC, ctx.to_device(prob.psi[:nocc]), occ)
+    got = ctx.add_exx(ctx.to_device(prob.psi)).cpu().numpy()
+    assert _rel(got, ref) < 1e-11
+    f = np.linspace(2.0, 0.5, prob.nb)
+    rho = ctx.density(ctx.to_device(prob.psi), f).cpu().numpy()
+    rho_o = H.density(prob.psi, f)
+    assert _rel(rho, rho_o) < 1e-12
+    ctx.close()
+
+
+def test_density_large_cell():
+    """CalculateDensity on the 180 x 180 x 72 cell: the FP64 plane's
+    double-buffered y-space (265 KiB) exceeds the LDS, so the density kernel
+    takes the grouped layout; FP64 and FP32 accumulation of four bands and
+    the renormalised density against the oracle."""
+    import torch
+    from pwdft_develop_amd.ops import Context
+    prob = W.layered432_180(nb=4).build()
+    print("case: layered432_180 density fp64/fp32 nb=4", flush=True)
+    f = np.array([2.0, 2.0, 1.5, 0.5])
+    H = O.Hamiltonian(prob)
+    rho_o = H.density(prob.psi, f)
+    ctx = Context(prob)
+    rho = ctx.density(ctx.to_device(prob.psi), f).cpu().numpy()
+    assert _rel(rho, rho_o) < 1e-11
+    rho32 = ctx.density(ctx.to_device(prob.psi, torch.complex64), f).cpu().numpy()
+    assert _rel(rho32, rho_o) < 1e-5
+    rho_n, val = ctx.calculate_density(ctx.to_device(prob.psi), f / 2, num_spin=2, num_occupied=3)
+    ref = rho_o * (2 * 3 * prob.ngrid / (prob.volume * rho_o.sum()))
+    assert _rel(rho_n.cpu().numpy(), ref) < 1e-11
+    assert abs(val - 6.0) < 1e-10
+    ctx.close()
+
+
 def test_exx_divergence_matches_oracle():
     import ctypes as C
     from pwdft_develop_amd import _lib

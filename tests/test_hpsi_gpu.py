@@ -97,7 +97,8 @@ def test_fp64_register_slab_kernel(name, nb, monkeypatch):
     clds.close()
 
 
-@pytest.mark.parametrize("name,nb,groups", [("si8_32", 7, "3"), ("rand16_48", 5, "1"), ("si64_60", 3, "4")])
+@pytest.mark.parametrize("name,nb,groups", [("si8_32", 7, "3"), ("rand16_48", 5, "1"), ("si64_60", 3, "4"),
+                                            ("si8_32", 5, "5:3:512"), ("rand16_48", 3, "7:5:1024")])
 def test_grouped_y_slab_layout(name, nb, groups, monkeypatch):
     """The LDS slab kernel's large-plane layout (one y buffer, y transforms
     in groups of g columns through the x rows' scratch), forced on small
@@ -115,10 +116,11 @@ def test_grouped_y_slab_layout(name, nb, groups, monkeypatch):
 
 
 def test_apply_h_large_cell():
-    """180 x 180 x 72 grid, 432 atoms (1728 projectors), npw 76241: the FP64
-    plane does not fit LDS twice, so the slab kernel takes the grouped
-    layout; FP32 keeps the double-buffered one.  Four bands against the
-    oracle."""
+    """180 x 180 x 72 grid, 432 atoms (1728 projectors), npw 76241: neither
+    precision's y-space fits LDS twice (FP64 265 KiB, FP32 135 KiB), so the
+    slab kernel takes the grouped layout -- FP32 one group of all 45 columns
+    with 22 row pairs at 1024 threads, FP64 groups of 8 columns with 4 row
+    pairs.  Four bands against the oracle."""
     import torch
     from pwdft_develop_amd.ops import Context
     prob = W.layered432_180(nb=4).build()

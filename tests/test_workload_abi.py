@@ -121,3 +121,28 @@ def test_error_status_without_device_calls():
                                   np.zeros(1, np.int32), 1, np.zeros(3), 0.8, 2, np.zeros(3), np.zeros(2),
                                   np.zeros(1))
     assert rc == -7
+
+
+def test_batch_workload_recipe():
+    """The 48-cell batched workload: cubic grids from {24, 30, 36, 40, 48, 60,
+    72}, side n pi / sqrt(80) bohr, 2..32 atoms, 16..128 bands; the half
+    sphere at Ecut 10 Ha holds 448, 896, 1485, 2070, 3562, 6999 and 12152
+    coefficients for those grids (the survey's count of the reference's
+    rule), and generation is deterministic."""
+    from pwdft_develop_amd import workload as W
+    cells = W.batch_cells()
+    again = W.batch_cells()
+    assert len(cells) == 48
+    npw_of = {24: 448, 30: 896, 36: 1485, 40: 2070, 48: 3562, 60: 6999, 72: 12152}
+    seen = set()
+    for c, d in zip(cells, again):
+        n = c.grid[0]
+        assert c.grid == (n, n, n) and n in npw_of
+        assert abs(c.supercell[0, 0] - n * np.pi / np.sqrt(80.0)) < 1e-12
+        assert 2 <= len(c.atoms) <= 32 and 16 <= c.nb <= 128
+        assert np.array_equal(c.atoms, d.atoms) and c.nb == d.nb
+        if n not in seen:
+            seen.add(n)
+            c.build(with_psi=False)
+            assert c.npw == npw_of[n]
+    assert len(seen) == 7
