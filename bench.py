@@ -170,6 +170,7 @@ def batch_bench(args):
     for c in cells:
         c.build()
     B = Batch(cells)
+    tuned = B.autotune(None)
     bounds = np.array([(1.0, float(c.ekin.max()) + 6.0, -2.5) for c in cells])
     X0 = B.to_device([c.psi for c in cells])
     X = torch.empty_like(X0)
@@ -203,7 +204,8 @@ def batch_bench(args):
         "speedup_vs_one_by_one": round(res["one_by_one"] / res["batched"], 3),
         "config": {"workload": "batch48", "cells": len(cells), "grids": grids,
                    "bands": sum(c.nb for c in cells), "degree": deg, "band_applications": napp,
-                   "items": info["items"], "workgroups": info["workgroups"], "target": info["target"]},
+                   "jobs_per_step": info["jobs_per_step"], "workgroups": info["workgroups"],
+                   "lds_bytes": info["lds_bytes"], "autotune": tuned},
     }
 
 
@@ -425,6 +427,10 @@ def main():
     ctx = Context(prob)
     info = ctx.info()
     out = {}
+    # plan autotuning (slab layout, projector-product tiling) for the FP32
+    # filter's and the FP64 Rayleigh-Ritz's H*psi shape, before any timing
+    out["autotune"] = {"f32": ctx.autotune(nb, torch.complex64, None),
+                       "f64": ctx.autotune(nb, torch.complex128, None)}
 
     # ---- plain H*psi throughput, FP64 and FP32 ----
     hpsi, stages = {}, {}

@@ -109,6 +109,17 @@ int pw_mult_spinor(pw_ctx* ctx, int dtype, int nb, const void* psi, void* a3);
  * VnlX = beta * (w .* Re(beta^H X)), VnlX overwritten. */
 int pw_nonlocal_multx(pw_ctx* ctx, int dtype, int nb, const void* X, void* VnlX);
 
+/* pw_nonlocal_multx without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * projections W and the split-K partial products live in a fixed 32 MiB
+ * device array per GPU (held until the process ends): fewer K splits are
+ * used when the usual count does not fit, and shapes whose nbeta x nb
+ * projections alone fill it return PW_ERR_UNSUPPORTED.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU race.  Same result
+ * as pw_nonlocal_multx (bit for bit when the usual split count fits). */
+int pw_nonlocal_multx_noalloc(pw_ctx* ctx, int dtype, int nb, const void* X, void* VnlX);
+
 /* MultSpinor followed by NonlocalMultX and the add, i.e. A*X as LOBPCG
  * assembles it (src/EigenSolver/lobpcg.cpp:203-212, 312-321). */
 int pw_apply_h(pw_ctx* ctx, int dtype, int nb, const void* X, void* HX);
@@ -299,9 +310,15 @@ int pw_get_timing(pw_ctx* ctx, double* ms4, int* ncalls);
  * The band blocks of all cells live in one device buffer of complex floats
  * (FP32, the PW_F32 layout per cell): cell c's npw[c] x nb[c] block starts
  * at complex offset xoff[c] = sum_{c' < c} npw[c'] nb[c'].
- * Work is split into (cell, band chunk) items taken from a queue by a
- * persistent grid; the reset of the queue and the launch are captured into a
- * hipGraph on the second call of a shape and replayed afterwards. */
+ * One launch of a persistent grid (as many workgroups as the device holds
+ * at once) runs every filter step as four stages over queues that mix all
+ * cells' items -- z-inverse sticks of one band and 16 x 16 projection tiles;
+ * one z plane of one band; z-forward sticks of one band; 64 rows x 16 bands
+ * of the projector apply fused with the recurrence -- with a device-wide
+ * barrier between stages.  The reset of the barrier and the launch are
+ * captured into a hipGraph on the second call of a shape and replayed
+ * afterwards.  Calls return after the launch has finished (the barrier's
+ * error word is read back: PW_ERR_HIP if a barrier ever timed out). */
 typedef struct pw_batch pw_batch;
 int pw_batch_create(pw_batch** batch, int ncell, const int* grid, const double* supercell,
                     const int* npw, const int* idx, const int* npad, const int* pad_src,
@@ -310,13 +327,13 @@ int pw_batch_create(pw_batch** batch, int ncell, const int* grid, const double* 
                     const double* wgt, const int* nb, void* stream);
 int pw_batch_destroy(pw_batch* batch);
 int pw_batch_set_stream(pw_batch* batch, void* stream);
-/* Total complex elements of the block buffer, number of items, workgroups
- * of the persistent grid, target work per item (bands x grid points). */
-int pw_batch_info(const pw_batch* batch, long* xtotal, int* nitems, int* nwg, long* target);
-/* Re-plan the items: target work per item (bands x grid points), x row
- * pairs per slab step, 16-stick chunks per z pass, workgroups (0 = the
- * default of each).  The autotuner's knobs. */
-int pw_batch_tune(pw_batch* batch, long target, int rp, int zc, int nwg);
+/* Total complex elements of the block buffer, jobs per filter step,
+ * workgroups of the persistent grid, LDS bytes per workgroup. */
+int pw_batch_info(const pw_batch* batch, long* xtotal, int* njobs, int* nwg, int* lds);
+/* Re-plan: x row pairs per slab job, 16-stick chunks per z job, workgroups
+ * (0 = the default of each; the grid never exceeds what the device holds at
+ * once).  The autotuner's knobs. */
+int pw_batch_tune(pw_batch* batch, int rp, int zc, int nwg);
 int pw_batch_set_graphs(pw_batch* batch, int enable);
 /* The batch's autotuner (as pw_autotune): times work-per-item, row-pair
  * and z-chunk splits on pw_batch_apply_h over every cell, applies the

@@ -59,6 +59,7 @@ def lib():
         "pw_set_nonlocal": [_vp, C.c_int, _dp, _dp],
         "pw_mult_spinor": [_vp, C.c_int, C.c_int, _vp, _vp],
         "pw_nonlocal_multx": [_vp, C.c_int, C.c_int, _vp, _vp],
+        "pw_nonlocal_multx_noalloc": [_vp, C.c_int, C.c_int, _vp, _vp],
         "pw_apply_h": [_vp, C.c_int, C.c_int, _vp, _vp],
         "pw_teter_precond": [_vp, C.c_int, C.c_int, _vp, _vp],
         "pw_orthogonalize": [_vp, C.c_int, _vp],
@@ -88,8 +89,8 @@ def lib():
                             _dp, _ip, _vp],
         "pw_batch_destroy": [_vp],
         "pw_batch_set_stream": [_vp, _vp],
-        "pw_batch_info": [_vp, C.POINTER(C.c_long), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_long)],
-        "pw_batch_tune": [_vp, C.c_long, C.c_int, C.c_int, C.c_int],
+        "pw_batch_info": [_vp, C.POINTER(C.c_long), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+        "pw_batch_tune": [_vp, C.c_int, C.c_int, C.c_int],
         "pw_batch_set_graphs": [_vp, C.c_int],
         "pw_batch_autotune": [_vp, C.c_char_p, C.c_char_p, C.c_int],
         "pw_autotune": [_vp, C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_int],

@@ -29,7 +29,7 @@ CFLAGS = [
     "-Wno-unused-result", "-Wno-unused-command-line-argument",
     f"-I{os.path.join(ROOT, 'include')}", f"-I{ROCM}/include",
 ]
-LDFLAGS = ["-shared", f"--offload-arch={ARCH}", f"-L{ROCM}/lib", "-lrocblas", "-lrocsolver", "-lrocfft",
+LDFLAGS = ["-shared", f"--offload-arch={ARCH}", f"-L{ROCM}/lib", "-lrocblas", "-lrocsolver",
            f"-Wl,-rpath,{ROCM}/lib"]
 
 

@@ -118,11 +118,13 @@ k_gemm_tn_partial(const T* __restrict__ A, int lda, const T* __restrict__ B, int
 }
 
 // ---- split BF16 products (FP32 data, FP32 accumulation) ----
-// x = hi + lo with hi = bf16(x) and lo = bf16(x - hi), both rounded to
-// nearest even: hi + lo carries 16 significant bits of x.  A product a b is
-// taken as a_hi b_hi + a_hi b_lo + a_lo b_hi (the a_lo b_lo term, 2^-16
-// relative, is dropped) on v_mfma_f32_16x16x32_bf16, whose operand lane l
-// holds row (or column) l % 16 and k = 8 (l / 16) .. 8 (l / 16) + 7 and whose
+// Each FP32 operand x is split into NS BF16 terms rounded to nearest even:
+// NS = 2: x = hi + lo (16 significant bits), NS = 3: x = hi + mid + lo (24
+// bits, FP32's).  A product a b is the sum of the term products down to
+// 2^-8 (NS - 1) relative: NS = 2: a_lo b_hi + a_hi b_lo + a_hi b_hi; NS = 3:
+// a_lo b_hi + a_hi b_lo + a_mid b_mid + a_mid b_hi + a_hi b_mid + a_hi b_hi
+// (smallest first), on v_mfma_f32_16x16x32_bf16, whose operand lane l holds
+// row (or column) l % 16 and k = 8 (l / 16) .. 8 (l / 16) + 7 and whose
 // accumulator is laid out as v_mfma_f32_16x16x4f32's.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
@@ -131,33 +133,47 @@ __device__ __forceinline__ unsigned short bf16_rne(float x) {
   u += 0x7fffu + ((u >> 16) & 1u);
   return (unsigned short)(u >> 16);
 }
-__device__ __forceinline__ void bf16_split(float x, unsigned short& hi, unsigned short& lo) {
-  hi = bf16_rne(x);
-  lo = bf16_rne(x - __uint_as_float((unsigned)hi << 16));
+__device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+template <int NS>
+__device__ __forceinline__ void bf16_split(float x, unsigned short (&t)[NS]) {
+  float r = x;
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    t[i] = bf16_rne(r);
+    r -= bf16_val(t[i]);
+  }
 }
-__device__ __forceinline__ f4v mfma_bf3(const unsigned short* ah, const unsigned short* al, const unsigned short* bh,
-                                        const unsigned short* bl, f4v acc) {
-  const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(ah), a1 = *reinterpret_cast<const bf16x8*>(al);
-  const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(bh), b1 = *reinterpret_cast<const bf16x8*>(bl);
-  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc, 0, 0, 0);
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc, 0, 0, 0);
+__device__ __forceinline__ f4v mfma_b16(const unsigned short* a, const unsigned short* b, f4v acc) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8*>(a),
+                                                 *reinterpret_cast<const bf16x8*>(b), acc, 0, 0, 0);
+}
+// a[i], b[i]: the i-th BF16 term tiles (16 x 32 operands at the lane's
+// offset); NS term tiles each, at a common offset.
+template <int NS>
+__device__ __forceinline__ f4v mfma_split(const unsigned short* const (&a)[NS], const unsigned short* const (&b)[NS],
+                                          f4v acc) {
+  if constexpr (NS == 3) {
+    acc = mfma_b16(a[2], b[0], acc);
+    acc = mfma_b16(a[0], b[2], acc);
+    acc = mfma_b16(a[1], b[1], acc);
+  }
+  acc = mfma_b16(a[1], b[0], acc);
+  acc = mfma_b16(a[0], b[1], acc);
+  return mfma_b16(a[0], b[0], acc);
 }
 constexpr int BKP = 40;  // LDS row stride of the bf16 tiles (80 bytes: 16-byte aligned rows)
 
-// k_gemm_tn_partial with FP32 inputs split into BF16 hi + lo: the same tiles,
-// loads and split-K partials; each 32-row K chunk is one 16x16x32 product per
-// (hi, lo) pairing instead of eight 16x16x4 FP32 products.
-template <int TM>
+// k_gemm_tn_partial with FP32 inputs split into NS BF16 terms: the same
+// tiles, loads and split-K partials; each 32-row K chunk is one 16x16x32
+// product per kept pairing of terms instead of eight 16x16x4 FP32 products.
+template <int TM, int NS>
 __global__ void __launch_bounds__(256)
-k_gemm_tn_partial_bf3(const float* __restrict__ A, int lda, const float* __restrict__ B, int ldb, int K, int n1,
-                      int n2, int kc, float* __restrict__ P) {
+k_gemm_tn_partial_bf(const float* __restrict__ A, int lda, const float* __restrict__ B, int ldb, int K, int n1,
+                     int n2, int kc, float* __restrict__ P) {
   constexpr int TB = BM * TM, NQ = 8 * TM, NA = 2 * TM;
   extern __shared__ __align__(16) unsigned char gemm_smem[];
-  unsigned short* Ah = reinterpret_cast<unsigned short*>(gemm_smem);
-  unsigned short* Al = Ah + TB * BKP;
-  unsigned short* Bh = Al + TB * BKP;
-  unsigned short* Bl = Bh + TB * BKP;
+  unsigned short* As = reinterpret_cast<unsigned short*>(gemm_smem);  // NS term tiles of A, then of B
+  unsigned short* Bs = As + NS * TB * BKP;
   const int i0 = blockIdx.x * TB, j0 = blockIdx.y * TB, split = blockIdx.z;
   const int kb = split * kc, ke = min(K, kb + kc);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -183,13 +199,14 @@ k_gemm_tn_partial_bf3(const float* __restrict__ A, int lda, const float* __restr
   for (int k0 = kb; k0 < ke; k0 += BK) {
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
-      unsigned short h, l;
-      bf16_split(ra[q], h, l);
-      Ah[(lcol + 2 * q) * BKP + lr] = h;
-      Al[(lcol + 2 * q) * BKP + lr] = l;
-      bf16_split(rb[q], h, l);
-      Bh[(lcol + 2 * q) * BKP + lr] = h;
-      Bl[(lcol + 2 * q) * BKP + lr] = l;
+      unsigned short ta[NS], tb[NS];
+      bf16_split<NS>(ra[q], ta);
+      bf16_split<NS>(rb[q], tb);
+#pragma unroll
+      for (int i = 0; i < NS; i++) {
+        As[i * TB * BKP + (lcol + 2 * q) * BKP + lr] = ta[i];
+        Bs[i * TB * BKP + (lcol + 2 * q) * BKP + lr] = tb[i];
+      }
     }
     lds_barrier();
     if (k0 + BK < ke) load(k0 + BK);
@@ -197,8 +214,15 @@ k_gemm_tn_partial_bf3(const float* __restrict__ A, int lda, const float* __restr
     for (int a = 0; a < NA; a++)
 #pragma unroll
       for (int b = 0; b < NA; b++) {
-        const int ra_ = (wi + a * 16 + fr) * BKP + fk, rb_ = (wj + b * 16 + fr) * BKP + fk;
-        acc[a][b] = mfma_bf3(Ah + ra_, Al + ra_, Bh + rb_, Bl + rb_, acc[a][b]);
+        const int oa = (wi + a * 16 + fr) * BKP + fk, ob = (wj + b * 16 + fr) * BKP + fk;
+        const unsigned short* pa[NS];
+        const unsigned short* pb[NS];
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
+          pa[i] = As + i * TB * BKP + oa;
+          pb[i] = Bs + i * TB * BKP + ob;
+        }
+        acc[a][b] = mfma_split<NS>(pa, pb, acc[a][b]);
       }
     lds_barrier();
   }
@@ -346,22 +370,20 @@ __device__ __forceinline__ void nn_epilogue(const NnEpilogue<T>& ep, size_t e, T
   }
 }
 
-// Y-tile product A (m x k) B (k x n) with FP32 inputs split into BF16
-// hi + lo, FP32 accumulation, and the epilogue above.  64 x 64 tiles, four
+// Y-tile product A (m x k) B (k x n) with FP32 inputs split into NS BF16
+// terms, FP32 accumulation, and the epilogue above.  64 x 64 tiles, four
 // waves of 32 x 32, 32-deep K chunks: A chunk loads are 64-row runs (lane =
 // row, 8 k per thread), B chunk loads 8 contiguous k of one column per
 // thread; both are split and stored as k-contiguous BF16 rows so that one
 // 16-byte LDS read is one MFMA operand.  The finished tile goes through LDS
 // so that the epilogue reads and writes 16 contiguous rows per thread.
-template <bool kCheb>
+template <bool kCheb, int NS>
 __global__ void __launch_bounds__(256)
-k_gemm_nn_bf3(const float* __restrict__ A, int lda, const float* __restrict__ B, int ldb, int m, int n, int k,
-              NnEpilogue<float> ep) {
-  __shared__ __align__(16) unsigned short Sh[4 * 64 * BKP];
-  unsigned short* Ah = Sh;
-  unsigned short* Al = Ah + 64 * BKP;
-  unsigned short* Bh = Al + 64 * BKP;
-  unsigned short* Bl = Bh + 64 * BKP;
+k_gemm_nn_bf(const float* __restrict__ A, int lda, const float* __restrict__ B, int ldb, int m, int n, int k,
+             NnEpilogue<float> ep) {
+  __shared__ __align__(16) unsigned short Sh[2 * NS * 64 * BKP > 64 * 65 * 2 ? 2 * NS * 64 * BKP : 64 * 65 * 2];
+  unsigned short* As = Sh;                 // NS term tiles of A (64 rows x 32 k)
+  unsigned short* Bs = Sh + NS * 64 * BKP;  // NS term tiles of B (64 columns x 32 k)
   const int i0 = blockIdx.x * BM, j0 = blockIdx.y * BN;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wi = (w & 1) * 32, wj = (w >> 1) * 32;
@@ -373,31 +395,45 @@ k_gemm_nn_bf3(const float* __restrict__ A, int lda, const float* __restrict__ B,
   const int arow = i0 + (t & 63), ak = 8 * (t >> 6);
   const int bcol = j0 + (t >> 2), bk = 8 * (t & 3);
   const int fr = lane & 15, fk = 8 * (lane >> 4);
+  auto pack = [](const unsigned short* v) {
+    return uint4{v[0] | ((unsigned)v[1] << 16), v[2] | ((unsigned)v[3] << 16), v[4] | ((unsigned)v[5] << 16),
+                 v[6] | ((unsigned)v[7] << 16)};
+  };
   for (int k0 = 0; k0 < k; k0 += BK) {
-    unsigned short ah[8], al[8], bh[8], bl[8];
+    unsigned short ta[NS][8], tb[NS][8];
 #pragma unroll
     for (int j = 0; j < 8; j++) {
       const int ka = k0 + ak + j, kb2 = k0 + bk + j;
       const float x = (arow < m && ka < k) ? A[(size_t)ka * lda + arow] : 0.f;
       const float y = (bcol < n && kb2 < k) ? B[(size_t)bcol * ldb + kb2] : 0.f;
-      bf16_split(x, ah[j], al[j]);
-      bf16_split(y, bh[j], bl[j]);
+      unsigned short sx[NS], sy[NS];
+      bf16_split<NS>(x, sx);
+      bf16_split<NS>(y, sy);
+#pragma unroll
+      for (int i = 0; i < NS; i++) {
+        ta[i][j] = sx[i];
+        tb[i][j] = sy[i];
+      }
     }
-    auto pack = [](const unsigned short* v) {
-      return uint4{v[0] | ((unsigned)v[1] << 16), v[2] | ((unsigned)v[3] << 16), v[4] | ((unsigned)v[5] << 16),
-                   v[6] | ((unsigned)v[7] << 16)};
-    };
-    *reinterpret_cast<uint4*>(Ah + (t & 63) * BKP + ak) = pack(ah);
-    *reinterpret_cast<uint4*>(Al + (t & 63) * BKP + ak) = pack(al);
-    *reinterpret_cast<uint4*>(Bh + (t >> 2) * BKP + bk) = pack(bh);
-    *reinterpret_cast<uint4*>(Bl + (t >> 2) * BKP + bk) = pack(bl);
+#pragma unroll
+    for (int i = 0; i < NS; i++) {
+      *reinterpret_cast<uint4*>(As + i * 64 * BKP + (t & 63) * BKP + ak) = pack(ta[i]);
+      *reinterpret_cast<uint4*>(Bs + i * 64 * BKP + (t >> 2) * BKP + bk) = pack(tb[i]);
+    }
     __syncthreads();
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
       for (int b = 0; b < 2; b++) {
         const int oa = (wi + a * 16 + fr) * BKP + fk, ob = (wj + b * 16 + fr) * BKP + fk;
-        acc[a][b] = mfma_bf3(Ah + oa, Al + oa, Bh + ob, Bl + ob, acc[a][b]);
+        const unsigned short* pa[NS];
+        const unsigned short* pb[NS];
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
+          pa[i] = As + i * 64 * BKP + oa;
+          pb[i] = Bs + i * 64 * BKP + ob;
+        }
+        acc[a][b] = mfma_split<NS>(pa, pb, acc[a][b]);
       }
     __syncthreads();
   }
@@ -563,23 +599,33 @@ k_gemv_n(const T* __restrict__ A, int lda, const T* __restrict__ x, T* __restric
   }
 }
 
+// The context's split-K partial buffer, grown to need bytes (allocates).
+static int grow_gemm_ws(pw_ctx* c, size_t need) {
+  if (need <= c->gemm_ws_bytes) return PW_OK;
+  invalidate_graph(c);
+  if (c->d_gemm_ws) (void)hipFree(c->d_gemm_ws);
+  c->d_gemm_ws = nullptr;
+  c->gemm_ws_bytes = 0;
+  PW_HIP(hipMalloc(&c->d_gemm_ws, need));
+  c->gemm_ws_bytes = need;
+  return PW_OK;
+}
+
+// ws == null: the partials go to the context's buffer (grown on demand);
+// else to ws (ws_bytes long), with fewer K splits when the usual count does
+// not fit and PW_ERR_UNSUPPORTED when not even one split does.
 template <typename T>
-int gemm_tn(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, int ldb, T* C, int ldc,
-            const T* rowscale, T beta, bool split) {
+int gemm_tn_ws(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, int ldb, T* C, int ldc,
+               const T* rowscale, T beta, int split, void* ws, size_t ws_bytes) {
   if (n1 <= 0 || n2 <= 0) return PW_OK;
   if (n2 == 1) {
-    const int nsplit = std::max(1, std::min(64, (K + 2047) / 2048));
+    int nsplit = std::max(1, std::min(64, (K + 2047) / 2048));
+    if (ws) nsplit = (int)std::min<size_t>(nsplit, ws_bytes / (sizeof(T) * (size_t)n1));
+    if (nsplit < 1) return fail(PW_ERR_UNSUPPORTED, "projection of %d columns exceeds the %zu-byte scratch", n1, ws_bytes);
     const int kc = (K + nsplit - 1) / nsplit;
     const size_t need = sizeof(T) * (size_t)nsplit * n1;
-    if (need > c->gemm_ws_bytes) {
-      invalidate_graph(c);
-      if (c->d_gemm_ws) (void)hipFree(c->d_gemm_ws);
-      c->d_gemm_ws = nullptr;
-      c->gemm_ws_bytes = 0;
-      PW_HIP(hipMalloc(&c->d_gemm_ws, need));
-      c->gemm_ws_bytes = need;
-    }
-    T* P = (T*)c->d_gemm_ws;
+    if (!ws) PW_TRY(grow_gemm_ws(c, need));
+    T* P = (T*)(ws ? ws : c->d_gemm_ws);
     hipLaunchKernelGGL(k_gemv_t_partial<T>, dim3(n1, nsplit), dim3(256), 0, c->stream, A, lda, B, K, kc, n1, P);
     const int g = std::max(1, std::min((n1 + 255) / 256, 4096));
     hipLaunchKernelGGL(k_gemm_tn_reduce<T>, dim3(g), dim3(256), 0, c->stream, (const T*)P, nsplit, n1, 1, C, ldc,
@@ -589,37 +635,45 @@ int gemm_tn(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, i
   }
   // outputs of 256 or more 64 x 64 tiles (1024 x 1024 and up) take
   // 128 x 128 tiles
-  const int tm = c->tn_tm > 0 ? c->tn_tm : (((n1 + BM - 1) / BM) * ((n2 + BN - 1) / BN) >= 256 ? 2 : 1);
+  const pw_ctx::Plan& pl = c->plan[sizeof(T) == 8 ? PW_F64 : PW_F32];
+  const int tm = pl.tn_tm > 0 ? pl.tn_tm : (((n1 + BM - 1) / BM) * ((n2 + BN - 1) / BN) >= 256 ? 2 : 1);
   const int ti = (n1 + BM * tm - 1) / (BM * tm), tj = (n2 + BN * tm - 1) / (BN * tm);
   const int tiles = ti * tj;
   // about 1024 partial workgroups, but at most 256 K-splits for one or two
   // output tiles (the sb x sb Gram blocks of PPCG over a long K): the
   // reduction then reads 256 partials per entry, not ~1000
-  const int cap = c->tn_cap > 0 ? c->tn_cap : (tiles <= 2 ? 256 : 1024);
+  const int cap = pl.tn_cap > 0 ? pl.tn_cap : (tiles <= 2 ? 256 : 1024);
   int nsplit = std::max(1, std::min((cap + tiles - 1) / tiles, (K + 255) / 256));
+  const size_t per_split = sizeof(T) * (size_t)n1 * n2;
+  if (ws) {
+    if (per_split > ws_bytes)
+      return fail(PW_ERR_UNSUPPORTED, "a %d x %d projection exceeds the %zu-byte scratch", n1, n2, ws_bytes);
+    nsplit = (int)std::min<size_t>(nsplit, ws_bytes / per_split);
+  }
   int kc = (K + nsplit - 1) / nsplit;
   kc = (kc + BK - 1) / BK * BK;
   nsplit = (K + kc - 1) / kc;
-  const size_t need = sizeof(T) * (size_t)nsplit * n1 * n2;
-  if (need > c->gemm_ws_bytes) {
-    invalidate_graph(c);
-    if (c->d_gemm_ws) (void)hipFree(c->d_gemm_ws);
-    c->d_gemm_ws = nullptr;
-    c->gemm_ws_bytes = 0;
-    PW_HIP(hipMalloc(&c->d_gemm_ws, need));
-    c->gemm_ws_bytes = need;
-  }
-  T* P = (T*)c->d_gemm_ws;
+  const size_t need = per_split * nsplit;
+  if (!ws) PW_TRY(grow_gemm_ws(c, need));
+  T* P = (T*)(ws ? ws : c->d_gemm_ws);
   if constexpr (sizeof(T) == 4) {
-    if (split) {
-      const size_t ldsb = sizeof(unsigned short) * 4 * (size_t)(BM * tm) * BKP;
-      if (tm == 2) {
-        PW_TRY(ensure_lds((const void*)k_gemm_tn_partial_bf3<2>, ldsb));
-        hipLaunchKernelGGL(k_gemm_tn_partial_bf3<2>, dim3(ti, tj, nsplit), dim3(256), ldsb, c->stream, A, lda, B,
-                           ldb, K, n1, n2, kc, P);
+    if (split == 2 || split == 3) {
+      const size_t ldsb = sizeof(unsigned short) * 2 * split * (size_t)(BM * tm) * BKP;
+      const dim3 gp(ti, tj, nsplit);
+      if (split == 3 && tm == 2) {
+        PW_TRY(ensure_lds((const void*)k_gemm_tn_partial_bf<2, 3>, ldsb));
+        hipLaunchKernelGGL((k_gemm_tn_partial_bf<2, 3>), gp, dim3(256), ldsb, c->stream, A, lda, B, ldb, K, n1, n2,
+                           kc, P);
+      } else if (split == 3) {
+        hipLaunchKernelGGL((k_gemm_tn_partial_bf<1, 3>), gp, dim3(256), ldsb, c->stream, A, lda, B, ldb, K, n1, n2,
+                           kc, P);
+      } else if (tm == 2) {
+        PW_TRY(ensure_lds((const void*)k_gemm_tn_partial_bf<2, 2>, ldsb));
+        hipLaunchKernelGGL((k_gemm_tn_partial_bf<2, 2>), gp, dim3(256), ldsb, c->stream, A, lda, B, ldb, K, n1, n2,
+                           kc, P);
       } else {
-        hipLaunchKernelGGL(k_gemm_tn_partial_bf3<1>, dim3(ti, tj, nsplit), dim3(256), ldsb, c->stream, A, lda, B,
-                           ldb, K, n1, n2, kc, P);
+        hipLaunchKernelGGL((k_gemm_tn_partial_bf<1, 2>), gp, dim3(256), ldsb, c->stream, A, lda, B, ldb, K, n1, n2,
+                           kc, P);
       }
       const long total = (long)n1 * n2;
       const int g = (int)std::min<long>((total + 255) / 256, 4096);
@@ -647,6 +701,12 @@ int gemm_tn(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, i
 }
 
 template <typename T>
+int gemm_tn(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, int ldb, T* C, int ldc,
+            const T* rowscale, T beta, int split) {
+  return gemm_tn_ws<T>(c, K, n1, n2, A, lda, B, ldb, C, ldc, rowscale, beta, split, nullptr, 0);
+}
+
+template <typename T>
 int gemm_nn(pw_ctx* c, int m, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T beta, T* Y,
             int ldy) {
   if (m <= 0 || n <= 0) return PW_OK;
@@ -666,7 +726,7 @@ int gemm_nn(pw_ctx* c, int m, int n, int k, T alpha, const T* A, int lda, const 
 // on FP64 MFMA; A = beta w (m x k), B = W (k x n).
 template <typename T>
 int gemm_nn_ep(pw_ctx* c, int m, int n, int k, const T* A, int lda, const T* B, int ldb, const T* H, const T* Yc,
-               const T* Yp, T* out, int ld, const T* coef, int step, bool cheb, bool split) {
+               const T* Yp, T* out, int ld, const T* coef, int step, bool cheb, int split) {
   if (m <= 0 || n <= 0) return PW_OK;
   NnEpilogue<T> ep{H, Yc, Yp, out, ld, coef, step, (T)1, (T)1};
   const dim3 grid((m + BM - 1) / BM, (n + BN - 1) / BN);
@@ -685,11 +745,15 @@ int gemm_nn_ep(pw_ctx* c, int m, int n, int k, const T* A, int lda, const T* B, 
     done = true;
   }
   if constexpr (sizeof(T) == 4) {
-    if (!done && split) {
-      if (cheb)
-        hipLaunchKernelGGL(k_gemm_nn_bf3<true>, grid, dim3(256), 0, c->stream, A, lda, B, ldb, m, n, k, ep);
+    if (!done && (split == 2 || split == 3)) {
+      if (cheb && split == 3)
+        hipLaunchKernelGGL((k_gemm_nn_bf<true, 3>), grid, dim3(256), 0, c->stream, A, lda, B, ldb, m, n, k, ep);
+      else if (split == 3)
+        hipLaunchKernelGGL((k_gemm_nn_bf<false, 3>), grid, dim3(256), 0, c->stream, A, lda, B, ldb, m, n, k, ep);
+      else if (cheb)
+        hipLaunchKernelGGL((k_gemm_nn_bf<true, 2>), grid, dim3(256), 0, c->stream, A, lda, B, ldb, m, n, k, ep);
       else
-        hipLaunchKernelGGL(k_gemm_nn_bf3<false>, grid, dim3(256), 0, c->stream, A, lda, B, ldb, m, n, k, ep);
+        hipLaunchKernelGGL((k_gemm_nn_bf<false, 2>), grid, dim3(256), 0, c->stream, A, lda, B, ldb, m, n, k, ep);
       done = true;
     }
   }
@@ -704,13 +768,17 @@ int gemm_nn_ep(pw_ctx* c, int m, int n, int k, const T* A, int lda, const T* B, 
 }
 
 template int gemm_nn_ep<double>(pw_ctx*, int, int, int, const double*, int, const double*, int, const double*,
-                                const double*, const double*, double*, int, const double*, int, bool, bool);
+                                const double*, const double*, double*, int, const double*, int, bool, int);
 template int gemm_nn_ep<float>(pw_ctx*, int, int, int, const float*, int, const float*, int, const float*,
-                               const float*, const float*, float*, int, const float*, int, bool, bool);
+                               const float*, const float*, float*, int, const float*, int, bool, int);
+template int gemm_tn_ws<double>(pw_ctx*, int, int, int, const double*, int, const double*, int, double*, int,
+                                const double*, double, int, void*, size_t);
+template int gemm_tn_ws<float>(pw_ctx*, int, int, int, const float*, int, const float*, int, float*, int,
+                               const float*, float, int, void*, size_t);
 template int gemm_tn<double>(pw_ctx*, int, int, int, const double*, int, const double*, int, double*, int,
-                             const double*, double, bool);
+                             const double*, double, int);
 template int gemm_tn<float>(pw_ctx*, int, int, int, const float*, int, const float*, int, float*, int,
-                            const float*, float, bool);
+                            const float*, float, int);
 template int gemm_nn<double>(pw_ctx*, int, int, int, double, const double*, int, const double*, int, double,
                              double*, int);
 template int gemm_nn<float>(pw_ctx*, int, int, int, float, const float*, int, const float*, int, float, float*,

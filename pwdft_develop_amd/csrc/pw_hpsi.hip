@@ -22,10 +22,12 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "pw_fft.h"
 #include "pw_internal.h"
+#include "pw_slab_reg.h"
 
 namespace pw {
 
@@ -179,161 +181,9 @@ __global__ void __launch_bounds__(256, LB)  // LB waves per SIMD: <= 512 / LB re
 k_slab_xy_reg(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restrict__ t2, int n3, int nstick,
               int kxmax, int nb, const int* __restrict__ pos, const int* __restrict__ mpos,
               const T* __restrict__ vtot) {
-  static_assert(N1 % 2 == 0 && N2 % 2 == 0, "even N1 and N2");
-  typedef typename cx<T>::t C;
-  constexpr int H1 = N1 / 2, NPAIR = N2 / 2, LV = N1 + 1, KC = 8;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  C* Y = reinterpret_cast<C*>(smem_raw);           // SB x nk x N2
-  const int nk = kxmax + 1;
-  C* vp = Y + (size_t)SB * nk * N2;                // NPAIR x LV pairs (v[2q][x], v[2q+1][x])
-  T* vs = reinterpret_cast<T*>(vp);
-  int* spos = reinterpret_cast<int*>(vp + (size_t)NPAIR * LV);  // nstick
-  int* smpos = spos + nstick;                                // nstick
-  const int z = blockIdx.x, b0 = blockIdx.y * SB;
-  const int ns = min(SB, nb - b0);
-  const int t = threadIdx.x;
-  // Staging loads go in batches issued before any use (one memory round trip
-  // per batch; a plain strided loop waits for each load in turn).
-  const T* vz = vtot + (size_t)z * N1 * N2;
-  constexpr int VB = (N1 * N2 + 255) / 256;
-  {
-    T vb[VB];
-#pragma unroll
-    for (int u = 0; u < VB; u++) {
-      const int e = t + 256 * u;
-      vb[u] = e < N1 * N2 ? vz[e] : (T)0;
-    }
-#pragma unroll
-    for (int u = 0; u < VB; u++) {
-      const int e = t + 256 * u;
-      if (e < N1 * N2) {
-        const int y = e / N1, x = e - y * N1;
-        vs[2 * ((y >> 1) * LV + x) + (y & 1)] = vb[u];
-      }
-    }
-  }
-  for (int e0 = 0; e0 < nstick; e0 += 4 * 256) {
-    int pb[4], mb[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int e = e0 + t + 256 * u;
-      pb[u] = e < nstick ? pos[e] : 0;
-      mb[u] = e < nstick ? mpos[e] : -1;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int e = e0 + t + 256 * u;
-      if (e < nstick) { spos[e] = pb[u]; smpos[e] = mb[u]; }
-    }
-  }
-  for (int e = t; e < SB * nk * N2; e += blockDim.x) Y[e] = C{0, 0};
-  __syncthreads();
-  const int tot = ns * nstick;
-  for (int e0 = 0; e0 < tot; e0 += 8 * 256) {
-    C vb[8];
-#pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int e = e0 + t + 256 * u;
-      const int s = e / nstick, st = e - s * nstick;
-      vb[u] = e < tot ? t1[((size_t)(b0 + s) * n3 + z) * nstick + st] : C{0, 0};
-    }
-#pragma unroll
-    for (int u = 0; u < 8; u++) {
-      const int e = e0 + t + 256 * u;
-      if (e < tot) {
-        const int s = e / nstick, st = e - s * nstick;
-        C* Ys = Y + (size_t)s * nk * N2;
-        Ys[spos[st]] = vb[u];
-        const int mp = smpos[st];
-        if (mp >= 0) Ys[mp] = C{vb[u].x, -vb[u].y};
-      }
-    }
-  }
-  __syncthreads();
-  // inverse y on the kx <= kxmax columns
-  for (int c = t; c < ns * nk; c += blockDim.x) {
-    C* col = Y + (size_t)c * N2;
-    C a[N2];
-#pragma unroll
-    for (int i = 0; i < N2; i++) a[i] = col[i];
-    reg_fft<T, N2, 1>(a);
-#pragma unroll
-    for (int i = 0; i < N2; i++) col[i] = a[i];
-  }
-  __syncthreads();
-  // per row pair: Hermitian-extended spectra packed as X1 + i X2, c2r, x V/N,
-  // r2c, unpack kx <= kxmax; a thread reads and writes only its two rows,
-  // which sit side by side in LDS (one 128-bit access per kx)
-  for (int c = t; c < ns * NPAIR; c += blockDim.x) {
-    const int s = c / NPAIR, q = c - s * NPAIR;
-    const int y1 = 2 * q;
-    C* Ys = Y + (size_t)s * nk * N2;
-    C a[N1];
-#pragma unroll
-    for (int k0 = 0; k0 <= H1; k0 += KC) {
-#pragma unroll
-      for (int kk = k0; kk < k0 + KC; kk++) {
-        if (kk > H1) continue;
-        struct { T x, y, z, w; } p = {0, 0, 0, 0};
-        if (kk <= kxmax) {
-          if constexpr (sizeof(T) == 4) {
-            const float4 q = *reinterpret_cast<const float4*>(Ys + kk * N2 + y1);
-            p = {q.x, q.y, q.z, q.w};
-          } else {
-            const C q1 = Ys[kk * N2 + y1], q2 = Ys[kk * N2 + y1 + 1];
-            p = {q1.x, q1.y, q2.x, q2.y};
-          }
-        }
-        if (kk == 0 || kk == H1) { p.y = 0; p.w = 0; }
-        // row y1 spectrum (p.x, p.y), row y2 (p.z, p.w); the kx > H1 half is
-        // their conjugates
-        a[kk] = C{p.x - p.w, p.y + p.z};
-        if (kk != 0 && kk != H1) a[N1 - kk] = C{p.x + p.w, p.z - p.y};
-      }
-    }
-    pin_regs(a);
-    reg_fft<T, N1, 1>(a);
-    pin_regs(a);
-    const C* vq = vp + q * LV;
-#pragma unroll
-    for (int x = 0; x < N1; x++) {
-      const C v = vq[x];
-      a[x] = C{a[x].x * v.x, a[x].y * v.y};
-    }
-    pin_regs(a);
-    reg_fft<T, N1, -1>(a);
-    pin_regs(a);
-#pragma unroll
-    for (int k = 0; k <= H1; k++) {
-      if (k <= kxmax) {
-        const C w = a[k];
-        const C wc = a[k == 0 ? 0 : N1 - k];
-        const C o1{(T)0.5 * (w.x + wc.x), (T)0.5 * (w.y - wc.y)}, o2{(T)0.5 * (w.y + wc.y), (T)0.5 * (wc.x - w.x)};
-        if constexpr (sizeof(T) == 4) {
-          *reinterpret_cast<float4*>(Ys + k * N2 + y1) = make_float4(o1.x, o1.y, o2.x, o2.y);
-        } else {
-          Ys[k * N2 + y1] = o1;
-          Ys[k * N2 + y1 + 1] = o2;
-        }
-      }
-    }
-  }
-  __syncthreads();
-  // forward y
-  for (int c = t; c < ns * nk; c += blockDim.x) {
-    C* col = Y + (size_t)c * N2;
-    C a[N2];
-#pragma unroll
-    for (int i = 0; i < N2; i++) a[i] = col[i];
-    reg_fft<T, N2, -1>(a);
-#pragma unroll
-    for (int i = 0; i < N2; i++) col[i] = a[i];
-  }
-  __syncthreads();
-  for (int e = t; e < tot; e += blockDim.x) {
-    const int s = e / nstick, st = e - s * nstick;
-    t2[((size_t)(b0 + s) * n3 + z) * nstick + st] = Y[(size_t)s * nk * N2 + spos[st]];
-  }
+  slab_xy_reg_body<T, N1, N2, SB>(blockIdx.x, blockIdx.y * SB, t1, t2, n3, nstick, kxmax, nb, pos, mpos, vtot,
+                                  smem_raw);
 }
 
 // ---------------------------------------------------------------- kernel 3
@@ -473,6 +323,60 @@ using namespace pw;
 
 // ----------------------------------------------------------------- host side
 
+// ------------------------------------------------ allocation-free projector product
+// Scratch of pw_nonlocal_multx_noalloc: the projections W (nbeta x nb, then
+// padded to 256 bytes) followed by the split-K partial tiles.  A file-scope
+// device array: each GPU the library runs on holds its own copy until the
+// process ends.  Host code never passes the array's name to a kernel (on the
+// host it is only a stand-in for the symbol); the device address is looked
+// up once per device, with that device current, when a context is created,
+// and that pointer is what the kernels get.
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static void* g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+static int scratch_lookup(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(PW_ERR_UNSUPPORTED, "device %d beyond %d", device, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return PW_OK;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) return fail(PW_ERR_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+  g_scratch_ptr[device] = p;
+  return PW_OK;
+}
+
+template <typename T>
+__global__ void k_zero_block(T* __restrict__ p, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = (T)0;
+}
+
+template <typename T>
+static int nonlocal_noalloc(pw_ctx* c, int nb, const void* X, void* V, void* sp) {
+  const int m = 2 * c->npw;
+  if (c->nbeta == 0) {
+    const long n = (long)m * nb;
+    hipLaunchKernelGGL(k_zero_block<T>, dim3((int)std::min<long>((n + 255) / 256, 8192)), dim3(256), 0, c->stream,
+                       (T*)V, n);
+    PW_HIP(hipGetLastError());
+    return PW_OK;
+  }
+  const size_t wbytes = (sizeof(T) * (size_t)c->nbeta * nb + 255) / 256 * 256;
+  if (wbytes >= kNoAllocScratchBytes)
+    return fail(PW_ERR_UNSUPPORTED,
+                "pw_nonlocal_multx_noalloc: %d projectors x %d bands need more than the %zu-byte scratch", c->nbeta,
+                nb, kNoAllocScratchBytes);
+  T* W = (T*)sp;
+  void* ws = (unsigned char*)sp + wbytes;
+  const T* beta = (const T*)(sizeof(T) == 8 ? (const void*)c->d_beta : (const void*)c->d_beta_f);
+  const T* betaw = (const T*)(sizeof(T) == 8 ? (const void*)c->d_betaw : (const void*)c->d_betaw_f);
+  PW_TRY(gemm_tn_ws<T>(c, m, c->nbeta, nb, beta, m, (const T*)X, m, W, c->nbeta, nullptr, (T)0, false, ws,
+                       kNoAllocScratchBytes - wbytes));
+  return gemm_nn<T>(c, m, nb, c->nbeta, (T)1, betaw, m, W, c->nbeta, (T)0, (T*)V, m);
+}
+
 static void build_stick_plan(pw_ctx* c, const int* idx) {
   StickPlan& sp = c->sp;
   const int h1 = c->h1, n2 = c->n2, n3 = c->n3, npw = c->npw;
@@ -563,19 +467,21 @@ extern "C" int pw_ctx_create(pw_ctx** out, const int grid[3], const double super
   pw_ctx* c = new pw_ctx();
   {
     const char* bf = getenv("PW_BF16");
-    c->bf16_split = !(bf && atoi(bf) == 0);
+    if (bf) c->bf16_split = (atoi(bf) == 2) ? 2 : (atoi(bf) == 0 ? 0 : 3);
     const char* sl = getenv("PW_SLAB");
-    c->use_reg_slab = !(sl && std::string(sl).compare(0, 3, "lds") == 0);
-    // "lds:<g>[:<r>]": the LDS kernel with its large-plane layout, g y columns
-    // per group (and r x row pairs per step)
-    if (sl && std::string(sl).compare(0, 4, "lds:") == 0) {
-      c->slab_cg_force = std::max(1, atoi(sl + 4));
-      const char* r = strchr(sl + 4, ':');
-      if (r) c->slab_rp_force = std::max(1, std::min(64, atoi(r + 1)));
-      const char* th = r ? strchr(r + 1, ':') : nullptr;  // ":<threads>" (256, 512 or 1024)
-      if (th) {
-        const int t = atoi(th + 1);
-        c->slab_threads_force = (t == 512 || t == 1024) ? t : 256;
+    for (auto& pl : c->plan) {
+      pl.reg = !(sl && std::string(sl).compare(0, 3, "lds") == 0);
+      // "lds:<g>[:<r>]": the LDS kernel with its large-plane layout, g y
+      // columns per group (and r x row pairs per step)
+      if (sl && std::string(sl).compare(0, 4, "lds:") == 0) {
+        pl.cg = std::max(1, atoi(sl + 4));
+        const char* r = strchr(sl + 4, ':');
+        if (r) pl.rp = std::max(1, std::min(64, atoi(r + 1)));
+        const char* th = r ? strchr(r + 1, ':') : nullptr;  // ":<threads>" (256, 512 or 1024)
+        if (th) {
+          const int t = atoi(th + 1);
+          pl.threads = (t == 512 || t == 1024) ? t : 256;
+        }
       }
     }
   }
@@ -626,6 +532,10 @@ extern "C" int pw_ctx_create(pw_ctx** out, const int grid[3], const double super
   c->f3 = radix_plan(c->n3);
   int rc;
   PW_HIP(hipGetDevice(&c->device));
+  if ((rc = scratch_lookup(c->device)) != PW_OK) {
+    pw_ctx_destroy(c);
+    return rc;
+  }
 #define UP(d, h, n)                                       \
   if ((rc = upload(&(d), (h), (n))) != PW_OK) {           \
     pw_ctx_destroy(c);                                    \
@@ -807,14 +717,15 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
   C* t1 = (C*)c->d_t1;
   C* t2 = (C*)c->d_t2;
   // x-pass row-pair chunk: as many pairs as fit next to the y buffers in 64 KiB
+  const pw_ctx::Plan& pl = c->plan[sizeof(T) == 8 ? PW_F64 : PW_F32];
   int rp = 8, cg = 0;
   while (rp > 1 && slab_lds(c, esz, rp, 0) > 64 * 1024) rp /= 2;
-  if (c->slab_cg_force == 0 && c->slab_rp_force > 0) {
+  if (pl.cg == 0 && pl.rp > 0) {
     // double-buffered layout with a chosen row-pair count (the autotuner's)
-    rp = std::min(c->slab_rp_force, (c->n2 + 1) / 2);
-  } else if (c->slab_cg_force > 0) {
-    cg = std::min(c->sp.kxmax + 1, c->slab_cg_force);
-    if (c->slab_rp_force > 0) rp = c->slab_rp_force;
+    rp = std::min(pl.rp, (c->n2 + 1) / 2);
+  } else if (pl.cg > 0) {
+    cg = std::min(c->sp.kxmax + 1, pl.cg);
+    if (pl.rp > 0) rp = pl.rp;
   } else if (slab_lds(c, esz, rp, 0) > 64 * 1024) {
     // large planes: the double-buffered layout would hold one workgroup per
     // CU with one x row pair per step.  One y buffer and a column-group
@@ -854,7 +765,7 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
                      c->d_pt_sz, tw3);
   PW_HIP(hipGetLastError());
   PW_TRY(mark(c, 1));
-  slab_reg_fn<T> kreg = c->use_reg_slab ? slab_reg_kernel<T>(c->n1, c->n2) : nullptr;
+  slab_reg_fn<T> kreg = pl.reg ? slab_reg_kernel<T>(c->n1, c->n2) : nullptr;
   if (kreg) {
     // one z plane of SB bands per workgroup, V plane + y-space in LDS
     constexpr int SB = SlabCfg<T>::SB;
@@ -871,7 +782,7 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
     // (180 x 180 planes, 1024 bands: 1024 threads 15.0 / 33.3 ms FP32 / FP64,
     // 256 threads 18.9 / 43.2 ms)
     int threads = cg > 0 ? (lds2 > 80 * 1024 ? 1024 : 512) : 256;
-    if (c->slab_threads_force > 0) threads = c->slab_threads_force;
+    if (pl.threads > 0) threads = pl.threads;
     hipLaunchKernelGGL(kslab, dim3(c->n3, nb), dim3(threads), lds2, c->stream, (const C*)t1, t2, c->n1,
                        c->n2, c->n3, c->sp.nstick, c->sp.kxmax, c->d_pos, c->d_mpos, vt, tw1, tw2, rp, cg);
   }
@@ -962,7 +873,7 @@ static int cheb_step(pw_ctx* c, int nb, const void* Y, void* HY, const void* Yp,
   if (c->nbeta > 0) {
     const T* beta = (const T*)(sizeof(T) == 8 ? (const void*)c->d_beta : (const void*)c->d_beta_f);
     T* W = (T*)(sizeof(T) == 8 ? (void*)c->d_W : (void*)c->d_W_f);
-    const bool split = c->bf16_split;
+    const int split = c->bf16_split;
     PW_TRY(gemm_tn<T>(c, m, c->nbeta, nb, beta, m, (const T*)Y, m, W, c->nbeta, nullptr, (T)0, split));
     PW_TRY(gemm_nn_ep<T>(c, m, nb, c->nbeta, betaw, m, W, c->nbeta, (const T*)HY, (const T*)Y, (const T*)Yp,
                          (T*)Ynew, m, coef, step, true, split));
@@ -1010,6 +921,20 @@ extern "C" int pw_nonlocal_multx(pw_ctx* c, int dtype, int nb, const void* X, vo
   if (nb == 0) return PW_OK;
   if (!X || !V || X == V) return fail(PW_ERR_ARG, "pw_nonlocal_multx: bad block pointers");
   return dtype == PW_F64 ? nonlocal<double>(c, nb, X, V, false) : nonlocal<float>(c, nb, X, V, false);
+}
+
+// EigenSolver::NonlocalMultX without any allocation: the projections and the
+// split-K partials in the per-device scratch array; calls that overlap on one
+// GPU share (and race on) that scratch.
+extern "C" int pw_nonlocal_multx_noalloc(pw_ctx* c, int dtype, int nb, const void* X, void* V) {
+  if (!c) return fail(PW_ERR_ARG, "null context");
+  if (dtype != PW_F64 && dtype != PW_F32) return fail(PW_ERR_ARG, "unknown dtype %d", dtype);
+  if (nb < 0 || nb > c->max_bands) return fail(PW_ERR_ARG, "nb = %d outside [0, %d]", nb, c->max_bands);
+  if (nb == 0) return PW_OK;
+  if (!X || !V || X == V) return fail(PW_ERR_ARG, "pw_nonlocal_multx_noalloc: bad block pointers");
+  void* sp = (c->device >= 0 && c->device < kMaxDevices) ? g_scratch_ptr[c->device] : nullptr;
+  if (!sp) return fail(PW_ERR_STATE, "pw_nonlocal_multx_noalloc: no scratch address for device %d", c->device);
+  return dtype == PW_F64 ? nonlocal_noalloc<double>(c, nb, X, V, sp) : nonlocal_noalloc<float>(c, nb, X, V, sp);
 }
 
 extern "C" int pw_apply_h(pw_ctx* c, int dtype, int nb, const void* X, void* HX) {

@@ -124,22 +124,30 @@ struct pw_ctx;
 namespace pw {
 // MFMA GEMMs (pw_gemm.hip).  C = A^T B (+ beta C), rows scaled by rowscale
 // when non-null; Y = alpha A B + beta Y.  Column-major, leading dims given.
-// split: FP32 inputs as split-BF16 products (hi hi + hi lo + lo hi on
-// v_mfma_f32_16x16x32_bf16, FP32 accumulation); ignored for FP64.
+// split = 2 or 3: FP32 inputs as products of their 2 or 3 split-BF16 terms
+// on v_mfma_f32_16x16x32_bf16 (FP32 accumulation); 0: FP32 MFMA.  Ignored
+// for FP64.
 template <typename T>
 int gemm_tn(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, int ldb, T* C, int ldc,
-            const T* rowscale, T beta, bool split = false);
+            const T* rowscale, T beta, int split = 0);
+// gemm_tn with the split-K partials in a caller's buffer (ws, ws_bytes): no
+// allocation; fewer K splits when the usual count does not fit, and
+// PW_ERR_UNSUPPORTED when one split does not.  ws == null: the context's
+// buffer, grown on demand (gemm_tn).
+template <typename T>
+int gemm_tn_ws(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, int ldb, T* C, int ldc,
+               const T* rowscale, T beta, int split, void* ws, size_t ws_bytes);
 template <typename T>
 int gemm_nn(pw_ctx* c, int m, int n, int k, T alpha, const T* A, int lda, const T* B, int ldb, T beta, T* Y,
             int ldy);
 // out = H + A B (cheb false), or out = s1 (H + A B - c Yc) - s2 Yp with (c,
 // s1, s2) = coef[3 step ..] on the device and Yp == null meaning s2 = 0
 // (cheb true): the projector term of H Y fused with a Chebyshev step.  FP64
-// on FP64 MFMA; FP32 on FP32 MFMA, or split-BF16 MFMA when split; k = 0
-// leaves acc = 0.
+// on FP64 MFMA; FP32 on FP32 MFMA, or on split-BF16 MFMA with split = 2 or 3
+// terms per operand; k = 0 leaves acc = 0.
 template <typename T>
 int gemm_nn_ep(pw_ctx* c, int m, int n, int k, const T* A, int lda, const T* B, int ldb, const T* H, const T* Yc,
-               const T* Yp, T* out, int ld, const T* coef, int step, bool cheb, bool split = false);
+               const T* Yp, T* out, int ld, const T* coef, int step, bool cheb, int split = 0);
 // Dense symmetric eigensolvers (pw_eig.hip, pw_solver.hip), in place on the
 // upper triangle of A (A <- eigenvectors, w (device) <- eigenvalues
 // ascending); sym_eig adds the sign convention of the oracle and takes
@@ -235,17 +243,23 @@ struct pw_ctx {
   double* d_coef = nullptr;
   float* d_coef_f = nullptr;
   bool use_graphs = true;
-  // FP32 projector products inside the Chebyshev filter as split-BF16 MFMA
-  // products (hi hi + hi lo + lo hi, FP32 accumulation); PW_BF16=0 keeps
-  // them on FP32 MFMA
-  bool bf16_split = true;
-  bool use_reg_slab = true;  // FP32 register slab kernel where compiled (env PW_SLAB=lds: the LDS kernel)
-  int slab_cg_force = 0;     // env PW_SLAB=lds:<g>[:<r>]: LDS kernel, one y buffer, g columns per transform group
-  int slab_rp_force = 0;     //   (and r x row pairs per step)
-  int slab_threads_force = 0;  // PW_SLAB=lds:<g>:<r>:<t>: t threads per workgroup
-  // split-K TN products: tile (1: 64 x 64, 2: 128 x 128) and the number of
-  // partial workgroups aimed at; 0 = the built-in rule (pw_autotune sets them)
-  int tn_tm = 0, tn_cap = 0;
+  // FP32 projector products inside the Chebyshev filter as products of split
+  // BF16 terms on MFMA (FP32 accumulation): 3 terms per operand (FP32's 24
+  // bits, six term products) by default, PW_BF16=2 for 2 terms (16 bits,
+  // three products), PW_BF16=0 for FP32 MFMA
+  int bf16_split = 3;
+  // Plans per precision ([PW_F64], [PW_F32]), set by PW_SLAB or pw_autotune:
+  //   reg: the register-FFT slab kernel where compiled (PW_SLAB=lds: the LDS
+  //   kernel); cg / rp / threads: the LDS kernel's y column groups (0 = two
+  //   y buffers), x row pairs per step, threads per workgroup (0 = the
+  //   built-in rule; PW_SLAB=lds:<cg>:<rp>:<threads>); tn_tm / tn_cap: the
+  //   split-K TN product's tile (1: 64 x 64, 2: 128 x 128) and the number of
+  //   partial workgroups aimed at (0 = the built-in rule)
+  struct Plan {
+    bool reg = true;
+    int cg = 0, rp = 0, threads = 0;
+    int tn_tm = 0, tn_cap = 0;
+  } plan[2];
   hipStream_t gstream = nullptr;
   hipEvent_t gev[2] = {nullptr, nullptr};
   hipGraphExec_t fg_exec = nullptr;

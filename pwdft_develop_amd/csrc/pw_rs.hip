@@ -22,11 +22,9 @@
 //   6. k_rs_f2c_ekin_p: IP_f2c as a gather on that pruned spectrum plus the
 //      kinetic term
 //   7. coarse C2R: k_rs_pencil along z and y, k_rs_x_c2r straight into a3.
-// Other lengths take rocFFT's full-grid transforms around the unpruned
-// gathers k_rs_c2f / k_rs_f2c_ekin.  FFTWExecute's vol/N and 1/vol factors
+// Other lengths are refused (PW_ERR_UNSUPPORTED).  FFTWExecute's vol/N and 1/vol factors
 // (src/SCF/fourier.cpp:695-760) cancel pairwise and are folded into the
 // gathers as 1/Nc and 1/Nf.
-#include <rocfft/rocfft.h>
 
 #include <algorithm>
 #include <map>
@@ -37,19 +35,7 @@
 
 using namespace pw;
 
-#define PW_FFT(call)                                                                   \
-  do {                                                                                 \
-    rocfft_status s_ = (call);                                                         \
-    if (s_ != rocfft_status_success)                                                   \
-      return ::pw::fail(PW_ERR_HIP, "%s failed: rocfft status %d (%s:%d)", #call,      \
-                        (int)s_, __FILE__, __LINE__);                                  \
-  } while (0)
-
 namespace {
-
-struct RsPlans {
-  rocfft_plan c_fwd = nullptr, f_inv = nullptr, f_fwd = nullptr, c_inv = nullptr;
-};
 
 // Per-atom projector table: points, first global projector, projector count,
 // offset of the atom's npts x nproj value block.
@@ -89,13 +75,11 @@ struct pw_rs {
   int* d_clist = nullptr;
   // chunk buffers
   double2* d_xc = nullptr;
-  double2* d_ff = nullptr;
   double* d_pf = nullptr;
   double* d_coef = nullptr;
   void* d_work = nullptr;
   size_t work_bytes = 0;
-  // own dual-grid transforms (every length factors into 2, 3, 5, 7 ... 31)
-  bool own_fft = false;
+  // dual-grid transforms (every length factors into 2, 3, 5, 7 ... 31)
   int ny = 0, nz = 0;           // live fine rows / planes
   double2* d_tz = nullptr;      // [z][l][i]  f2 x ny x hc
   double2* d_tf = nullptr;      // [z][y][i]  f2 x f1 x hc
@@ -112,114 +96,9 @@ struct pw_rs {
   double* d_cs = nullptr;       // prime-radix cos / sin table
   std::map<int, double2*> d_tw; // exp(+2 pi i m / n), per length
   pw_ctx* la = nullptr;         // linear algebra of the solvers (rs_linalg_ctx)
-  rocfft_execution_info info = nullptr;
-  std::map<int, RsPlans> plans;
 };
 
 // ---------------------------------------------------------------- kernels
-
-// Step 2: fine half spectrum from the coarse one (IP_c2f, utility.cpp:
-// 3150-3212, as a gather).  Fine row (jF, kF) takes coarse row (j, k) where
-// jF is j's primary image (j <= Nc1/2 -> j, else Nf1 - Nc1 + j) or, for the
-// Nyquist j = Nc1/2, its duplicate cjF = Nf1 - Nc1/2; the reference's branch
-// chain decides the value: primaries carry t/2 if any of i, j, k is a Nyquist
-// index (else t); a j-duplicate carries t/2 only for "j Nyquist alone", a
-// k-duplicate only for "k alone", a (j, k)-duplicate only for "j and k, i
-// not".  One wavefront per fine row; grid (Nf1 * Nf2, bands).
-__global__ __launch_bounds__(kRowThreads) void k_rs_c2f(const double2* __restrict__ xc,
-                                                       double2* __restrict__ ff, int nc0, int nc1,
-                                                       int nc2, int nf1, int nf2, int hc0,
-                                                       int hf0, long Hc, long Hf, double scale) {
-  const int row = blockIdx.x;
-  const int jF = row % nf1, kF = row / nf1;
-  const long b = blockIdx.y;
-  double2* out = ff + b * Hf + (long)row * hf0;
-  const int Jn = nc1 / 2, Kn = nc2 / 2, In = nc0 / 2;
-  // source row in j: dj = 0 primary, 1 duplicate, -1 none
-  int j, dj;
-  if (jF <= Jn) { j = jF; dj = 0; }
-  else if (jF > nf1 - Jn) { j = jF - nf1 + nc1; dj = 0; }
-  else if (jF == nf1 - Jn) { j = Jn; dj = 1; }
-  else { j = 0; dj = -1; }
-  int k, dk;
-  if (kF <= Kn) { k = kF; dk = 0; }
-  else if (kF > nf2 - Kn) { k = kF - nf2 + nc2; dk = 0; }
-  else if (kF == nf2 - Kn) { k = Kn; dk = 1; }
-  else { k = 0; dk = -1; }
-  const bool nj = (j == Jn), nk = (k == Kn);
-  const double2* src = xc + b * Hc + ((long)k * nc1 + j) * hc0;
-  const bool row_live = dj >= 0 && dk >= 0;
-  for (int i = threadIdx.x; i < hf0; i += kRowThreads) {
-    double2 v = make_double2(0.0, 0.0);
-    if (row_live && i <= In) {
-      const bool ni = (i == In);
-      double f;
-      if (dj == 0 && dk == 0) f = (ni || nj || nk) ? 0.5 : 1.0;
-      else if (dj == 1 && dk == 0) f = (!ni && !nk) ? 0.5 : 0.0;
-      else if (dj == 0) f = (!ni && !nj) ? 0.5 : 0.0;
-      else f = !ni ? 0.5 : 0.0;
-      if (f != 0.0) {
-        const double2 t = src[i];
-        v = make_double2(t.x * (f * scale), t.y * (f * scale));
-      }
-    }
-    out[i] = v;
-  }
-}
-
-// Step 7: xc <- ekin .* xc * sc + restriction(ff) * sf (IP_f2c, utility.cpp:
-// 3215-3276, and the kinetic multiply of spinor.cpp:469-473).  One wavefront
-// per coarse row; grid (Nc1 * Nc2, bands).
-__global__ __launch_bounds__(kRowThreads) void k_rs_f2c_ekin(double2* __restrict__ xc,
-                                                            const double2* __restrict__ ff,
-                                                            const double* __restrict__ ekin, int nc0,
-                                                            int nc1, int nc2, int nf1, int nf2,
-                                                            int hc0, int hf0, long Hc, long Hf,
-                                                            double sc, double sf) {
-  const int row = blockIdx.x;
-  const int j = row % nc1, k = row / nc1;
-  const long b = blockIdx.y;
-  const int Jn = nc1 / 2, Kn = nc2 / 2, In = nc0 / 2;
-  const int jF = (j <= Jn) ? j : nf1 - nc1 + j;
-  const int kF = (k <= Kn) ? k : nf2 - nc2 + k;
-  const int cjF = nf1 - Jn, ckF = nf2 - Kn;
-  const int mjF = (nf1 - jF) % nf1, mkF = (nf2 - kF) % nf2;
-  const bool nj = (j == Jn), nk = (k == Kn);
-  const double2* fb = ff + b * Hf;
-  const double2* f1row = fb + ((long)kF * nf1 + jF) * hf0;
-  // second row for the i < Nc0/2 entries (plain sum)
-  const double2* f2row = nullptr;
-  if (nj && nk) f2row = fb + ((long)ckF * nf1 + cjF) * hf0;
-  else if (nj) f2row = fb + ((long)kF * nf1 + cjF) * hf0;
-  else if (nk) f2row = fb + ((long)ckF * nf1 + jF) * hf0;
-  // second row for the i = Nc0/2 entry (conjugate sum)
-  const double2* g2row;
-  if (nj) g2row = fb + ((long)mkF * nf1 + jF) * hf0;
-  else if (nk) g2row = fb + ((long)kF * nf1 + mjF) * hf0;
-  else g2row = fb + ((long)mkF * nf1 + mjF) * hf0;
-  double2* crow = xc + b * Hc + (long)row * hc0;
-  const double* erow = ekin + (long)row * hc0;
-  for (int i = threadIdx.x; i < hc0; i += kRowThreads) {
-    const double2 f1 = f1row[i];
-    double2 acc;
-    if (i == In) {
-      if (nj && nk) {
-        acc = make_double2(f1.x, 0.0);
-      } else {
-        const double2 f2 = g2row[i];
-        acc = make_double2(0.5 * (f1.x + f2.x), 0.5 * (f1.y - f2.y));
-      }
-    } else if (f2row) {
-      const double2 f2 = f2row[i];
-      acc = make_double2(0.5 * (f1.x + f2.x), 0.5 * (f1.y + f2.y));
-    } else {
-      acc = f1;
-    }
-    const double2 x = crow[i];
-    const double e = erow[i] * sc;
-    crow[i] = make_double2(fma(e, x.x, acc.x * sf), fma(e, x.y, acc.y * sf));
-  }
-}
 
 // Step 4: projector weights of one atom for BB bands.  Grid (natom, band
 // groups), 256 threads striding over the atom's points; NPMAX x BB partial
@@ -959,79 +838,6 @@ __global__ __launch_bounds__(256) void k_rs_axpy(double* __restrict__ y, const d
 
 namespace {
 
-std::once_flag g_rocfft_once;
-rocfft_status g_rocfft_setup = rocfft_status_success;
-
-int make_plan(rocfft_plan* plan, rocfft_transform_type type, const int* n, int batch) {
-  const size_t len[3] = {(size_t)n[0], (size_t)n[1], (size_t)n[2]};
-  const size_t h0 = (size_t)n[0] / 2 + 1;
-  const size_t rstr[3] = {1, len[0], len[0] * len[1]};
-  const size_t cstr[3] = {1, h0, h0 * len[1]};
-  const size_t rdist = len[0] * len[1] * len[2], cdist = h0 * len[1] * len[2];
-  rocfft_plan_description desc = nullptr;
-  PW_FFT(rocfft_plan_description_create(&desc));
-  rocfft_status s;
-  if (type == rocfft_transform_type_real_forward)
-    s = rocfft_plan_description_set_data_layout(desc, rocfft_array_type_real,
-                                                rocfft_array_type_hermitian_interleaved, nullptr,
-                                                nullptr, 3, rstr, rdist, 3, cstr, cdist);
-  else
-    s = rocfft_plan_description_set_data_layout(desc, rocfft_array_type_hermitian_interleaved,
-                                                rocfft_array_type_real, nullptr, nullptr, 3, cstr,
-                                                cdist, 3, rstr, rdist);
-  if (s == rocfft_status_success)
-    s = rocfft_plan_create(plan, rocfft_placement_notinplace, type, rocfft_precision_double, 3, len,
-                           (size_t)batch, desc);
-  rocfft_plan_description_destroy(desc);
-  if (s != rocfft_status_success)
-    return fail(PW_ERR_HIP, "rocfft_plan_create (%d x %d x %d, batch %d) failed: status %d", n[0],
-                n[1], n[2], batch, (int)s);
-  return PW_OK;
-}
-
-int get_plans(pw_rs* rs, int batch, RsPlans** out) {
-  auto it = rs->plans.find(batch);
-  if (it != rs->plans.end()) {
-    *out = &it->second;
-    return PW_OK;
-  }
-  RsPlans p;
-  int rc;
-  if ((rc = make_plan(&p.c_fwd, rocfft_transform_type_real_forward, rs->nc, batch)) != PW_OK ||
-      (rc = make_plan(&p.f_inv, rocfft_transform_type_real_inverse, rs->nf, batch)) != PW_OK ||
-      (rc = make_plan(&p.f_fwd, rocfft_transform_type_real_forward, rs->nf, batch)) != PW_OK ||
-      (rc = make_plan(&p.c_inv, rocfft_transform_type_real_inverse, rs->nc, batch)) != PW_OK) {
-    for (rocfft_plan q : {p.c_fwd, p.f_inv, p.f_fwd, p.c_inv})
-      if (q) rocfft_plan_destroy(q);
-    return rc;
-  }
-  size_t need = 0;
-  for (rocfft_plan q : {p.c_fwd, p.f_inv, p.f_fwd, p.c_inv}) {
-    size_t w = 0;
-    PW_FFT(rocfft_plan_get_work_buffer_size(q, &w));
-    need = std::max(need, w);
-  }
-  if (need > rs->work_bytes) {
-    // the old buffer may still be read by queued transforms
-    PW_HIP(hipStreamSynchronize(rs->stream));
-    if (rs->d_work) PW_HIP(hipFree(rs->d_work));
-    rs->d_work = nullptr;
-    PW_HIP(hipMalloc(&rs->d_work, need));
-    rs->work_bytes = need;
-    PW_FFT(rocfft_execution_info_set_work_buffer(rs->info, rs->d_work, rs->work_bytes));
-  }
-  rs->plans[batch] = p;
-  *out = &rs->plans[batch];
-  return PW_OK;
-}
-
-int exec(pw_rs* rs, rocfft_plan plan, const void* in, void* out) {
-  void* ib[1] = {const_cast<void*>(in)};
-  void* ob[1] = {out};
-  PW_FFT(rocfft_execute(plan, ib, ob, rs->info));
-  return PW_OK;
-}
-
 template <typename X>
 int upload_rs(X** d, const X* h, size_t n) {
   PW_HIP(hipMalloc((void**)d, sizeof(X) * (n > 0 ? n : 1)));
@@ -1166,22 +972,13 @@ extern "C" int pw_rs_create(pw_rs** out, const int grid[3], const int grid_fine[
                 "pw_rs_create: grids must be even in every dimension (FFT_Number_Type even/power)");
   for (int d = 0; d < 3; d++)
     if (grid_fine[d] < grid[d]) return fail(PW_ERR_ARG, "pw_rs_create: fine grid smaller than the coarse grid");
+  for (int d = 0; d < 3; d++)
+    if (!own_length(grid[d]) || !own_length(grid_fine[d]) || grid_fine[d] > 1024)
+      return fail(PW_ERR_UNSUPPORTED,
+                  "pw_rs_create: grid %d x %d x %d / %d x %d x %d: every length must factor into 2, 3, 5 and at "
+                  "most one prime 7..31 (fine lengths up to 1024)",
+                  grid[0], grid[1], grid[2], grid_fine[0], grid_fine[1], grid_fine[2]);
   pw_rs* rs = new pw_rs();
-  {
-    // PW_RS_FFT=rocfft forces the library transforms (comparison runs)
-    const char* e = getenv("PW_RS_FFT");
-    bool own = !(e && std::string(e) == "rocfft");
-    for (int d = 0; d < 3; d++)
-      own = own && own_length(grid[d]) && own_length(grid_fine[d]) && grid_fine[d] <= 1024;
-    rs->own_fft = own;
-  }
-  if (!rs->own_fft) {
-    std::call_once(g_rocfft_once, [] { g_rocfft_setup = rocfft_setup(); });
-    if (g_rocfft_setup != rocfft_status_success) {
-      delete rs;
-      return fail(PW_ERR_HIP, "rocfft_setup failed: status %d", (int)g_rocfft_setup);
-    }
-  }
   for (int d = 0; d < 3; d++) {
     rs->nc[d] = grid[d];
     rs->nf[d] = grid_fine[d];
@@ -1199,12 +996,9 @@ extern "C" int pw_rs_create(pw_rs** out, const int grid[3], const int grid_fine[
   rs->vol = vol;
   rs->stream = (hipStream_t)stream;
   // bands per chunk: chunk buffers within about 8 GiB of the 288 GB HBM
-  double per_band = 16.0 * rs->Hc + 16.0 * rs->Hf + 8.0 * rs->Nf;
-  if (rs->own_fft) {
-    const double hc = rs->hc0, ny = grid[1] + 1, nz = grid[2] + 1;
-    per_band = 48.0 * rs->Hc + 16.0 * rs->Nf +
-               16.0 * hc * ((double)grid_fine[2] * ny + (double)grid_fine[2] * grid_fine[1] + nz * ny);
-  }
+  const double hc = rs->hc0, ny = grid[1] + 1, nz = grid[2] + 1;
+  const double per_band = 48.0 * rs->Hc + 16.0 * rs->Nf +
+                          16.0 * hc * ((double)grid_fine[2] * ny + (double)grid_fine[2] * grid_fine[1] + nz * ny);
   rs->chunk = (int)std::max(1.0, std::min((double)max_bands, 8589934592.0 / per_band));
   int rc = PW_OK;
   auto bail = [&](int code) {
@@ -1225,19 +1019,7 @@ extern "C" int pw_rs_create(pw_rs** out, const int grid[3], const int grid_fine[
   if (hipMalloc((void**)&rs->d_xc, sizeof(double2) * rs->Hc * rs->chunk) != hipSuccess ||
       hipMalloc((void**)&rs->d_pf, sizeof(double) * rs->Nf * rs->chunk) != hipSuccess)
     return bail(fail(PW_ERR_NOMEM, "pw_rs_create: cannot allocate %d-band chunk buffers", rs->chunk));
-  if (rs->own_fft) {
-    if ((rc = setup_own_fft(rs)) != PW_OK) return bail(rc);
-    *out = rs;
-    return PW_OK;
-  }
-  if (hipMalloc((void**)&rs->d_ff, sizeof(double2) * rs->Hf * rs->chunk) != hipSuccess)
-    return bail(fail(PW_ERR_NOMEM, "pw_rs_create: cannot allocate %d-band chunk buffers", rs->chunk));
-  if (rocfft_execution_info_create(&rs->info) != rocfft_status_success)
-    return bail(fail(PW_ERR_HIP, "pw_rs_create: rocfft_execution_info_create failed"));
-  if (rocfft_execution_info_set_stream(rs->info, rs->stream) != rocfft_status_success)
-    return bail(fail(PW_ERR_HIP, "pw_rs_create: rocfft_execution_info_set_stream failed"));
-  RsPlans* p;
-  if ((rc = get_plans(rs, rs->chunk, &p)) != PW_OK) return bail(rc);
+  if ((rc = setup_own_fft(rs)) != PW_OK) return bail(rc);
   *out = rs;
   return PW_OK;
 }
@@ -1246,16 +1028,11 @@ extern "C" int pw_rs_destroy(pw_rs* rs) {
   if (!rs) return PW_OK;
   hipStreamSynchronize(rs->stream);
   if (rs->la) pw_ctx_destroy(rs->la);
-  for (auto& kv : rs->plans)
-    for (rocfft_plan q : {kv.second.c_fwd, kv.second.f_inv, kv.second.f_fwd, kv.second.c_inv})
-      if (q) rocfft_plan_destroy(q);
-  if (rs->info) rocfft_execution_info_destroy(rs->info);
   free_nonlocal(rs);
   hipFree(rs->d_ekin);
   hipFree(rs->d_teter);
   hipFree(rs->d_vtot);
   hipFree(rs->d_xc);
-  hipFree(rs->d_ff);
   hipFree(rs->d_pf);
   hipFree(rs->d_work);
   hipFree(rs->d_tz);
@@ -1280,7 +1057,6 @@ extern "C" int pw_rs_set_stream(pw_rs* rs, void* stream) {
   if (!rs) return fail(PW_ERR_ARG, "pw_rs_set_stream: null handle");
   PW_HIP(hipStreamSynchronize(rs->stream));
   rs->stream = (hipStream_t)stream;
-  if (rs->info) PW_FFT(rocfft_execution_info_set_stream(rs->info, rs->stream));
   if (rs->la) PW_TRY(pw_ctx_set_stream(rs->la, stream));
   return PW_OK;
 }
@@ -1585,29 +1361,11 @@ extern "C" int pw_rs_mult_spinor(pw_rs* rs, int nb, const double* psi, double* a
   const int* nf = rs->nf;
   for (int b0 = 0; b0 < nb; b0 += rs->chunk) {
     const int nbc = std::min(rs->chunk, nb - b0);
-    if (rs->own_fft) {
-      const bool nl = rs->natom > 0 && rs->nproj_tot > 0;
-      const bool fuse_v = !nl || rs->ncolor > 0;
-      PW_TRY(own_to_fine(rs, psi + (long)b0 * rs->Nc, nbc, fuse_v));
-      PW_TRY(local_terms(rs, nbc, fuse_v));
-      PW_TRY(own_to_coarse(rs, fuse_v ? rs->d_pv : rs->d_pf, a3 + (long)b0 * rs->Nc, nbc));
-      continue;
-    }
-    RsPlans* p;
-    PW_TRY(get_plans(rs, nbc, &p));
-    PW_TRY(exec(rs, p->c_fwd, psi + (long)b0 * rs->Nc, rs->d_xc));
-    hipLaunchKernelGGL(k_rs_c2f, dim3(nf[1] * nf[2], nbc), dim3(kRowThreads), 0, rs->stream, rs->d_xc,
-                       rs->d_ff, nc[0], nc[1], nc[2], nf[1], nf[2], rs->hc0, rs->hf0, rs->Hc, rs->Hf,
-                       1.0 / (double)rs->Nc);
-    PW_HIP(hipGetLastError());
-    PW_TRY(exec(rs, p->f_inv, rs->d_ff, rs->d_pf));
-    PW_TRY(local_terms(rs, nbc, false));
-    PW_TRY(exec(rs, p->f_fwd, rs->d_pf, rs->d_ff));
-    hipLaunchKernelGGL(k_rs_f2c_ekin, dim3(nc[1] * nc[2], nbc), dim3(kRowThreads), 0, rs->stream, rs->d_xc,
-                       rs->d_ff, rs->d_ekin, nc[0], nc[1], nc[2], nf[1], nf[2], rs->hc0, rs->hf0, rs->Hc,
-                       rs->Hf, 1.0 / (double)rs->Nc, 1.0 / (double)rs->Nf);
-    PW_HIP(hipGetLastError());
-    PW_TRY(exec(rs, p->c_inv, rs->d_xc, a3 + (long)b0 * rs->Nc));
+    const bool nl = rs->natom > 0 && rs->nproj_tot > 0;
+    const bool fuse_v = !nl || rs->ncolor > 0;
+    PW_TRY(own_to_fine(rs, psi + (long)b0 * rs->Nc, nbc, fuse_v));
+    PW_TRY(local_terms(rs, nbc, fuse_v));
+    PW_TRY(own_to_coarse(rs, fuse_v ? rs->d_pv : rs->d_pf, a3 + (long)b0 * rs->Nc, nbc));
   }
   return PW_OK;
 }
@@ -1620,21 +1378,11 @@ extern "C" int pw_rs_filter(pw_rs* rs, double cutoff, int nb, double* x) {
   for (int b0 = 0; b0 < nb; b0 += rs->chunk) {
     const int nbc = std::min(rs->chunk, nb - b0);
     double* xb = x + (long)b0 * rs->Nc;
-    RsPlans* p = nullptr;
-    if (rs->own_fft) {
-      PW_TRY(own_coarse_fwd(rs, xb, nbc));
-    } else {
-      PW_TRY(get_plans(rs, nbc, &p));
-      PW_TRY(exec(rs, p->c_fwd, xb, rs->d_xc));
-    }
+    PW_TRY(own_coarse_fwd(rs, xb, nbc));
     hipLaunchKernelGGL(k_rs_cutmask, dim3((unsigned)((rs->Hc + 255) / 256), nbc), dim3(256), 0, rs->stream,
                        rs->d_xc, rs->d_ekin, cutoff, rs->Hc, 1.0 / (double)rs->Nc);
     PW_HIP(hipGetLastError());
-    if (rs->own_fft) {
-      PW_TRY(own_coarse_inv(rs, xb, nbc));
-    } else {
-      PW_TRY(exec(rs, p->c_inv, rs->d_xc, xb));
-    }
+    PW_TRY(own_coarse_inv(rs, xb, nbc));
   }
   return PW_OK;
 }
@@ -1654,18 +1402,7 @@ extern "C" int pw_rs_calculate_density(pw_rs* rs, int nb, const double* psi, con
   for (int b0 = 0; b0 < nb; b0 += rs->chunk) {
     const int nbc = std::min(rs->chunk, nb - b0);
     const double* xb = psi + (long)b0 * rs->Nc;
-    if (rs->own_fft) {
-      PW_TRY(own_to_fine(rs, xb, nbc, false));
-    } else {
-      RsPlans* p;
-      PW_TRY(get_plans(rs, nbc, &p));
-      PW_TRY(exec(rs, p->c_fwd, xb, rs->d_xc));
-      hipLaunchKernelGGL(k_rs_c2f, dim3(nf[1] * nf[2], nbc), dim3(kRowThreads), 0, rs->stream, rs->d_xc,
-                         rs->d_ff, nc[0], nc[1], nc[2], nf[1], nf[2], rs->hc0, rs->hf0, rs->Hc, rs->Hf,
-                         1.0 / (double)rs->Nc);
-      PW_HIP(hipGetLastError());
-      PW_TRY(exec(rs, p->f_inv, rs->d_ff, rs->d_pf));
-    }
+    PW_TRY(own_to_fine(rs, xb, nbc, false));
     for (int b = 0; b < nbc; b++) f[(size_t)b] = num_spin * occ[b0 + b];
     // the previous batch's accumulation reads d_f: copy in stream order
     PW_HIP(hipMemcpyAsync(rs->d_f, f.data(), sizeof(double) * nbc, hipMemcpyHostToDevice, rs->stream));
@@ -1701,22 +1438,12 @@ extern "C" int pw_rs_teter_precond(pw_rs* rs, int nb, const double* psi, double*
   for (int b0 = 0; b0 < nb; b0 += rs->chunk) {
     const int nbc = std::min(rs->chunk, nb - b0);
     const double* xb = psi + (long)b0 * rs->Nc;
-    RsPlans* p = nullptr;
-    if (rs->own_fft) {
-      PW_TRY(own_coarse_fwd(rs, xb, nbc));
-    } else {
-      PW_TRY(get_plans(rs, nbc, &p));
-      PW_TRY(exec(rs, p->c_fwd, xb, rs->d_xc));
-    }
+    PW_TRY(own_coarse_fwd(rs, xb, nbc));
     hipLaunchKernelGGL(k_rs_teter_spec, dim3((unsigned)((rs->Hc + 255) / 256), nbc), dim3(256), 0, rs->stream,
                        rs->d_xc, rs->d_teter, rs->Hc, 1.0 / (double)rs->Nc);
     PW_HIP(hipGetLastError());
     // the fine-grid scratch (Nf >= Nc per band) takes T psi before the add
-    if (rs->own_fft) {
-      PW_TRY(own_coarse_inv(rs, rs->d_pf, nbc));
-    } else {
-      PW_TRY(exec(rs, p->c_inv, rs->d_xc, rs->d_pf));
-    }
+    PW_TRY(own_coarse_inv(rs, rs->d_pf, nbc));
     const long n = (long)nbc * rs->Nc;
     hipLaunchKernelGGL(k_rs_axpy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, rs->stream,
                        a3 + (long)b0 * rs->Nc, (const double*)rs->d_pf, n);

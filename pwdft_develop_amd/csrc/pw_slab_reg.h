@@ -1,0 +1,183 @@
+// The xy slab transform with every 1D FFT in one thread's registers, as a
+// workgroup-level device function (the body of pw_hpsi.hip's k_slab_xy_reg,
+// also run by the batched persistent kernel).
+//
+// For compile-time N1 = N2: one z plane of SB bands per call (bands b0 ..).
+// V's plane is staged once in LDS as (row 2q, row 2q + 1) pairs, one 64-bit
+// word per x, pair rows padded to N1 + 1 words so that neighbouring threads
+// hit distinct banks; each band's y-space lives in LDS, and the five phases
+// (scatter, y IFFT per column, x c2r / V / r2c per row pair, y FFT per
+// column, gather) are separated by one barrier each.  In the row-pair phase a
+// thread holds N1 complex values; its LDS reads are issued in chunks fenced
+// by scheduling barriers so that the compiler does not hoist all of them next
+// to the spectrum (that spills past the 256 VGPRs of two waves per SIMD).
+// Same arithmetic, in the same order, as k_slab_xy.  t1 may equal t2 (each
+// plane slice is read whole before it is written).  LDS (smem_raw, 16-byte
+// aligned): SB nk N2 complex + N2/2 (N1 + 1) complex + 2 nstick ints.
+#pragma once
+#include "pw_fft.h"
+
+namespace pw {
+
+template <typename T, int N1, int N2, int SB>
+__device__ __forceinline__ void slab_xy_reg_body(int z, int b0, const typename cx<T>::t* t1, typename cx<T>::t* t2,
+                                                 int n3, int nstick, int kxmax, int nb, const int* __restrict__ pos,
+                                                 const int* __restrict__ mpos, const T* __restrict__ vtot,
+                                                 unsigned char* smem_raw) {
+  static_assert(N1 % 2 == 0 && N2 % 2 == 0, "even N1 and N2");
+  typedef typename cx<T>::t C;
+  constexpr int H1 = N1 / 2, NPAIR = N2 / 2, LV = N1 + 1, KC = 8;
+  C* Y = reinterpret_cast<C*>(smem_raw);           // SB x nk x N2
+  const int nk = kxmax + 1;
+  C* vp = Y + (size_t)SB * nk * N2;                // NPAIR x LV pairs (v[2q][x], v[2q+1][x])
+  T* vs = reinterpret_cast<T*>(vp);
+  int* spos = reinterpret_cast<int*>(vp + (size_t)NPAIR * LV);  // nstick
+  int* smpos = spos + nstick;                                // nstick
+  const int ns = min(SB, nb - b0);
+  const int t = threadIdx.x;
+  // Staging loads go in batches issued before any use (one memory round trip
+  // per batch; a plain strided loop waits for each load in turn).
+  const T* vz = vtot + (size_t)z * N1 * N2;
+  constexpr int VB = (N1 * N2 + 255) / 256;
+  {
+    T vb[VB];
+#pragma unroll
+    for (int u = 0; u < VB; u++) {
+      const int e = t + 256 * u;
+      vb[u] = e < N1 * N2 ? vz[e] : (T)0;
+    }
+#pragma unroll
+    for (int u = 0; u < VB; u++) {
+      const int e = t + 256 * u;
+      if (e < N1 * N2) {
+        const int y = e / N1, x = e - y * N1;
+        vs[2 * ((y >> 1) * LV + x) + (y & 1)] = vb[u];
+      }
+    }
+  }
+  for (int e0 = 0; e0 < nstick; e0 += 4 * 256) {
+    int pb[4], mb[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int e = e0 + t + 256 * u;
+      pb[u] = e < nstick ? pos[e] : 0;
+      mb[u] = e < nstick ? mpos[e] : -1;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int e = e0 + t + 256 * u;
+      if (e < nstick) { spos[e] = pb[u]; smpos[e] = mb[u]; }
+    }
+  }
+  for (int e = t; e < SB * nk * N2; e += blockDim.x) Y[e] = C{0, 0};
+  __syncthreads();
+  const int tot = ns * nstick;
+  for (int e0 = 0; e0 < tot; e0 += 8 * 256) {
+    C vb[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int e = e0 + t + 256 * u;
+      const int s = e / nstick, st = e - s * nstick;
+      vb[u] = e < tot ? t1[((size_t)(b0 + s) * n3 + z) * nstick + st] : C{0, 0};
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int e = e0 + t + 256 * u;
+      if (e < tot) {
+        const int s = e / nstick, st = e - s * nstick;
+        C* Ys = Y + (size_t)s * nk * N2;
+        Ys[spos[st]] = vb[u];
+        const int mp = smpos[st];
+        if (mp >= 0) Ys[mp] = C{vb[u].x, -vb[u].y};
+      }
+    }
+  }
+  __syncthreads();
+  // inverse y on the kx <= kxmax columns
+  for (int c = t; c < ns * nk; c += blockDim.x) {
+    C* col = Y + (size_t)c * N2;
+    C a[N2];
+#pragma unroll
+    for (int i = 0; i < N2; i++) a[i] = col[i];
+    reg_fft<T, N2, 1>(a);
+#pragma unroll
+    for (int i = 0; i < N2; i++) col[i] = a[i];
+  }
+  __syncthreads();
+  // per row pair: Hermitian-extended spectra packed as X1 + i X2, c2r, x V/N,
+  // r2c, unpack kx <= kxmax; a thread reads and writes only its two rows,
+  // which sit side by side in LDS (one 128-bit access per kx)
+  for (int c = t; c < ns * NPAIR; c += blockDim.x) {
+    const int s = c / NPAIR, q = c - s * NPAIR;
+    const int y1 = 2 * q;
+    C* Ys = Y + (size_t)s * nk * N2;
+    C a[N1];
+#pragma unroll
+    for (int k0 = 0; k0 <= H1; k0 += KC) {
+#pragma unroll
+      for (int kk = k0; kk < k0 + KC; kk++) {
+        if (kk > H1) continue;
+        struct { T x, y, z, w; } p = {0, 0, 0, 0};
+        if (kk <= kxmax) {
+          if constexpr (sizeof(T) == 4) {
+            const float4 q = *reinterpret_cast<const float4*>(Ys + kk * N2 + y1);
+            p = {q.x, q.y, q.z, q.w};
+          } else {
+            const C q1 = Ys[kk * N2 + y1], q2 = Ys[kk * N2 + y1 + 1];
+            p = {q1.x, q1.y, q2.x, q2.y};
+          }
+        }
+        if (kk == 0 || kk == H1) { p.y = 0; p.w = 0; }
+        // row y1 spectrum (p.x, p.y), row y2 (p.z, p.w); the kx > H1 half is
+        // their conjugates
+        a[kk] = C{p.x - p.w, p.y + p.z};
+        if (kk != 0 && kk != H1) a[N1 - kk] = C{p.x + p.w, p.z - p.y};
+      }
+    }
+    pin_regs(a);
+    reg_fft<T, N1, 1>(a);
+    pin_regs(a);
+    const C* vq = vp + q * LV;
+#pragma unroll
+    for (int x = 0; x < N1; x++) {
+      const C v = vq[x];
+      a[x] = C{a[x].x * v.x, a[x].y * v.y};
+    }
+    pin_regs(a);
+    reg_fft<T, N1, -1>(a);
+    pin_regs(a);
+#pragma unroll
+    for (int k = 0; k <= H1; k++) {
+      if (k <= kxmax) {
+        const C w = a[k];
+        const C wc = a[k == 0 ? 0 : N1 - k];
+        const C o1{(T)0.5 * (w.x + wc.x), (T)0.5 * (w.y - wc.y)}, o2{(T)0.5 * (w.y + wc.y), (T)0.5 * (wc.x - w.x)};
+        if constexpr (sizeof(T) == 4) {
+          *reinterpret_cast<float4*>(Ys + k * N2 + y1) = make_float4(o1.x, o1.y, o2.x, o2.y);
+        } else {
+          Ys[k * N2 + y1] = o1;
+          Ys[k * N2 + y1 + 1] = o2;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // forward y
+  for (int c = t; c < ns * nk; c += blockDim.x) {
+    C* col = Y + (size_t)c * N2;
+    C a[N2];
+#pragma unroll
+    for (int i = 0; i < N2; i++) a[i] = col[i];
+    reg_fft<T, N2, -1>(a);
+#pragma unroll
+    for (int i = 0; i < N2; i++) col[i] = a[i];
+  }
+  __syncthreads();
+  for (int e = t; e < tot; e += blockDim.x) {
+    const int s = e / nstick, st = e - s * nstick;
+    t2[((size_t)(b0 + s) * n3 + z) * nstick + st] = Y[(size_t)s * nk * N2 + spos[st]];
+  }
+  __syncthreads();  // the caller may reuse the LDS at once
+}
+
+}  // namespace pw

@@ -99,11 +99,11 @@ struct SlabChoice {
   }
 };
 
-static void apply_slab(pw_ctx* c, const SlabChoice& ch) {
-  c->use_reg_slab = ch.reg;
-  c->slab_cg_force = ch.reg ? 0 : ch.cg;
-  c->slab_rp_force = ch.reg ? 0 : ch.rp;
-  c->slab_threads_force = ch.reg ? 0 : ch.threads;
+static void apply_slab(pw_ctx::Plan& pl, const SlabChoice& ch) {
+  pl.reg = ch.reg;
+  pl.cg = ch.reg ? 0 : ch.cg;
+  pl.rp = ch.reg ? 0 : ch.rp;
+  pl.threads = ch.reg ? 0 : ch.threads;
 }
 
 static bool parse_slab(const std::string& v, SlabChoice* ch) {
@@ -129,6 +129,7 @@ extern "C" int pw_autotune(pw_ctx* c, int dtype, int nb, const char* cache_path,
   if (dtype != PW_F64 && dtype != PW_F32) return fail(PW_ERR_ARG, "unknown dtype %d", dtype);
   if (!c->have_vtot) return fail(PW_ERR_STATE, "vtot has not been set");
   const int esz = dtype == PW_F64 ? 16 : 8;
+  pw_ctx::Plan& pl = c->plan[dtype];
   const std::string base = device_tag(c->device) + "|" + std::to_string(c->n1) + "x" + std::to_string(c->n2) + "x" +
                            std::to_string(c->n3) + "|npw" + std::to_string(c->npw) + "|nb" + std::to_string(nb) +
                            (dtype == PW_F64 ? "|f64" : "|f32");
@@ -157,7 +158,7 @@ extern "C" int pw_autotune(pw_ctx* c, int dtype, int nb, const char* cache_path,
   SlabChoice best{false, 0, 0, 256};
   auto it = cache.find(kslab);
   if (it != cache.end() && parse_slab(it->second, &best)) {
-    apply_slab(c, best);
+    apply_slab(pl, best);
     out += "slab " + best.str() + " (cached)";
   } else {
     std::vector<SlabChoice> cand;
@@ -173,11 +174,10 @@ extern "C" int pw_autotune(pw_ctx* c, int dtype, int nb, const char* cache_path,
           for (int th : {512, 1024}) cand.push_back({false, cg, rp, th});
     }
     double tbest = 1e30;
-    const bool saved_reg = c->use_reg_slab;
-    const int saved[3] = {c->slab_cg_force, c->slab_rp_force, c->slab_threads_force};
+    const pw_ctx::Plan saved = pl;
     int tried = 0;
     for (const SlabChoice& ch : cand) {
-      apply_slab(c, ch);
+      apply_slab(pl, ch);
       double ms = 0;
       const int rc = time_min(c->stream, 3, [&] { return pw_mult_spinor(c, dtype, nb, X, Y); }, &ms);
       if (rc != PW_OK) {
@@ -191,13 +191,10 @@ extern "C" int pw_autotune(pw_ctx* c, int dtype, int nb, const char* cache_path,
       }
     }
     if (tried == 0) {
-      c->use_reg_slab = saved_reg;
-      c->slab_cg_force = saved[0];
-      c->slab_rp_force = saved[1];
-      c->slab_threads_force = saved[2];
+      pl = saved;
       return done(fail(PW_ERR_HIP, "pw_autotune: no slab layout ran"));
     }
-    apply_slab(c, best);
+    apply_slab(pl, best);
     append_cache(cache_path, kslab, best.str());
     char b[96];
     snprintf(b, sizeof(b), " (%.4f ms over %d layouts)", tbest, tried);
@@ -208,8 +205,8 @@ extern "C" int pw_autotune(pw_ctx* c, int dtype, int nb, const char* cache_path,
     int tm = 0, cap = 0;
     auto jt = cache.find(ktn);
     if (jt != cache.end() && sscanf(jt->second.c_str(), "%d %d", &tm, &cap) == 2 && (tm == 1 || tm == 2) && cap > 0) {
-      c->tn_tm = tm;
-      c->tn_cap = cap;
+      pl.tn_tm = tm;
+      pl.tn_cap = cap;
       out += "; tn " + std::to_string(tm) + " " + std::to_string(cap) + " (cached)";
     } else {
       if (hipMalloc(&W, es * (size_t)c->nbeta * nb) != hipSuccess)
@@ -219,8 +216,8 @@ extern "C" int pw_autotune(pw_ctx* c, int dtype, int nb, const char* cache_path,
       int btm = 1, bcap = 1024;
       for (int t : {1, 2})
         for (int cp : {256, 512, 1024, 2048}) {
-          c->tn_tm = t;
-          c->tn_cap = cp;
+          pl.tn_tm = t;
+          pl.tn_cap = cp;
           double ms = 0;
           int rc;
           if (dtype == PW_F64)
@@ -239,8 +236,8 @@ extern "C" int pw_autotune(pw_ctx* c, int dtype, int nb, const char* cache_path,
             bcap = cp;
           }
         }
-      c->tn_tm = btm;
-      c->tn_cap = bcap;
+      pl.tn_tm = btm;
+      pl.tn_cap = bcap;
       append_cache(cache_path, ktn, std::to_string(btm) + " " + std::to_string(bcap));
       char b[96];
       snprintf(b, sizeof(b), "; tn %d %d (%.4f ms)", btm, bcap, tbest);

@@ -294,12 +294,12 @@ class Batch:
             pass
 
     def info(self):
-        xt, ni, nw, tg = C.c_long(), C.c_int(), C.c_int(), C.c_long()
-        check(_lib.lib().pw_batch_info(self.h, C.byref(xt), C.byref(ni), C.byref(nw), C.byref(tg)))
-        return {"xtotal": xt.value, "items": ni.value, "workgroups": nw.value, "target": tg.value}
+        xt, nj, nw, ld = C.c_long(), C.c_int(), C.c_int(), C.c_int()
+        check(_lib.lib().pw_batch_info(self.h, C.byref(xt), C.byref(nj), C.byref(nw), C.byref(ld)))
+        return {"xtotal": xt.value, "jobs_per_step": nj.value, "workgroups": nw.value, "lds_bytes": ld.value}
 
-    def tune(self, target=0, rp=0, zc=0, nwg=0):
-        check(_lib.lib().pw_batch_tune(self.h, int(target), int(rp), int(zc), int(nwg)))
+    def tune(self, rp=0, zc=0, nwg=0):
+        check(_lib.lib().pw_batch_tune(self.h, int(rp), int(zc), int(nwg)))
 
     def set_graphs(self, on=True):
         check(_lib.lib().pw_batch_set_graphs(self.h, 1 if on else 0))

@@ -39,7 +39,7 @@ def test_batch_apply_h_each_cell(cells):
     print("case: batch48 apply_h fp32 vs oracle", flush=True)
     B = Batch(cells)
     info = B.info()
-    assert info["items"] >= len(cells) and info["workgroups"] >= 1
+    assert info["jobs_per_step"] >= len(cells) and info["workgroups"] >= 1
     X = B.to_device([c.psi for c in cells])
     HX = B.split(B.apply_h(X))
     torch.cuda.synchronize()
@@ -106,19 +106,46 @@ def test_batch_graph_replays_match_eager(cells):
     B.close()
 
 
-@pytest.mark.parametrize("target,rp,zc", [(1, 5, 1), (40000, 7, 3), (10 ** 7, 0, 0)])
-def test_batch_tuning_knobs(cells, target, rp, zc):
-    """The autotuner's knobs (work per item, x row pairs per step, stick
-    chunks per z pass) change the schedule and the LDS layout, not the
-    result: FP32 parity with the oracle on a subset of cells."""
+@pytest.mark.parametrize("rp,zc,nwg", [(5, 1, 0), (7, 3, 96), (0, 0, 256)])
+def test_batch_tuning_knobs(cells, rp, zc, nwg):
+    """The autotuner's knobs (x row pairs per slab job, stick chunks per z
+    job, grid size) change the LDS layout and the schedule, not the result:
+    FP32 parity with the oracle on a subset of cells, for H X and a degree-8
+    filter."""
     from pwdft_develop_amd.ops import Batch
-    print(f"case: batch tune target={target} rp={rp} zc={zc}", flush=True)
+    print(f"case: batch tune rp={rp} zc={zc} nwg={nwg}", flush=True)
     sub = [cells[i] for i in (0, 5, 11, 17, 23)]
     B = Batch(sub)
-    B.tune(target, rp, zc, 0)
+    B.tune(rp, zc, nwg)
     X = B.to_device([c.psi for c in sub])
     HX = B.split(B.apply_h(X))
     for c, h in zip(sub, HX):
         ref = O.Hamiltonian(c).apply_h(c.psi[[0, c.nb - 1]])
         assert _rel(h[[0, c.nb - 1]].astype(np.complex128), ref) < 2e-5, c.name
+    bounds = np.array([_bounds(c) for c in sub])
+    B.cheb_filter(X, DEGREE, bounds)
+    for c, g in zip(sub, B.split(X)):
+        ref = O.Hamiltonian(c).cheb_filter(c.psi[[c.nb - 1]], DEGREE, *_bounds(c))
+        assert _rel(g[[c.nb - 1]].astype(np.complex128), ref) < 2e-5, c.name
     B.close()
+
+
+def test_batch_autotune_and_cache(cells, tmp_path):
+    """pw_batch_autotune picks a split by timing, records it, and a second
+    batch of the same shapes takes it from the cache; results unchanged."""
+    from pwdft_develop_amd.ops import Batch
+    print("case: batch autotune", flush=True)
+    sub = cells[:6]
+    cache = str(tmp_path / "tune.txt")
+    B = Batch(sub)
+    s1 = B.autotune(cache)
+    assert "cached" not in s1
+    X = B.to_device([c.psi for c in sub])
+    H1 = B.apply_h(X).clone()
+    B.close()
+    B2 = Batch(sub)
+    s2 = B2.autotune(cache)
+    assert "cached" in s2 and s2.split(" (")[0] == s1.split(" (")[0]
+    import torch
+    assert torch.equal(B2.apply_h(X), H1)
+    B2.close()

@@ -66,15 +66,35 @@ def test_rs_mult_spinor_matches_oracle(name, nb, max_bands):
     assert _rel(got, ref) < TOL
 
 
-@pytest.mark.parametrize("name,nb", [("box_rs", 4), ("sic8_rs", 3)])
-def test_rs_rocfft_transforms_match_oracle(name, nb, monkeypatch):
-    """The library-FFT variant (taken for lengths with prime factors above
-    31, forced here) agrees with the oracle as the LDS transforms do."""
-    monkeypatch.setenv("PW_RS_FFT", "rocfft")
-    p = _prob(name, nb)
-    ref = _oracle(p, p.psi)
-    got = _gpu(p, p.psi)
-    assert _rel(got, ref) < TOL
+def test_rs_unsupported_length_is_refused():
+    """A grid length with a prime factor above 31 (74 = 2 x 37) is refused
+    with PW_ERR_UNSUPPORTED: there is no second FFT backend to fall to."""
+    import ctypes as C
+    from pwdft_develop_amd import _lib
+    h = C.c_void_p()
+    grid = np.array([74, 24, 24], np.int32)
+    ekin = np.zeros((74 // 2 + 1) * 24 * 24)
+    rc = _lib.lib().pw_rs_create(C.byref(h), grid, grid, 1000.0, ekin, 4, None)
+    assert rc == -7 and b"prime" in _lib.lib().pw_last_error()
+
+
+@pytest.mark.parametrize("name,cutoff", [("sic8_rs", 12.0), ("box_rs", 8.0)])
+def test_rs_filter_against_rocfft(name, cutoff):
+    """Cross-check of the own coarse transforms against rocFFT (torch.fft on
+    the device, FP64): the wavefunction filter x <- C2R(mask . R2C(x)) with
+    mask = [ekin <= cutoff] on the R2C half grid, 1e-12 relative."""
+    import torch
+    p = _prob(name, 4)
+    n1, n2, n3 = p.grid
+    ctx = R.RealSpaceContext(p)
+    x = torch.from_numpy(p.psi.copy()).to("cuda:0")
+    got = ctx.filter(x, cutoff)
+    mask = torch.from_numpy((p.ekin <= cutoff).reshape(n3, n2, n1 // 2 + 1)).to("cuda:0")
+    xr = torch.from_numpy(p.psi).to("cuda:0").reshape(-1, n3, n2, n1)
+    ref = torch.fft.irfftn(torch.fft.rfftn(xr, dim=(1, 2, 3)) * mask, s=(n3, n2, n1), dim=(1, 2, 3))
+    ctx.close()
+    torch.cuda.synchronize()
+    assert _rel(got.cpu().numpy(), ref.reshape(got.shape).cpu().numpy()) < 1e-12
 
 
 def test_rs_local_terms_only():
@@ -151,12 +171,9 @@ def test_rs_sic_large_all_bands_linear():
     ctx.close()
 
 
-@pytest.mark.parametrize("name,cutoff,fft", [("sic8_rs", 12.0, None), ("box_rs", 8.0, None),
-                                             ("sic8_rs", 12.0, "rocfft")])
-def test_rs_wavefunction_filter(name, cutoff, fft, monkeypatch):
+@pytest.mark.parametrize("name,cutoff", [("sic8_rs", 12.0), ("box_rs", 8.0)])
+def test_rs_wavefunction_filter(name, cutoff):
     import torch
-    if fft:
-        monkeypatch.setenv("PW_RS_FFT", fft)
     p = _prob(name, 5)
     ref = O.rs_filter(p.grid, p.volume, p.ekin, cutoff, p.psi)
     ctx = R.RealSpaceContext(p)
@@ -166,11 +183,9 @@ def test_rs_wavefunction_filter(name, cutoff, fft, monkeypatch):
     assert _rel(got, ref) < TOL
 
 
-@pytest.mark.parametrize("name,fft", [("sic8_rs", None), ("box_rs", None), ("sic8_rs", "rocfft")])
-def test_rs_calculate_density(name, fft, monkeypatch):
+@pytest.mark.parametrize("name", ["sic8_rs", "box_rs"])
+def test_rs_calculate_density(name):
     import torch
-    if fft:
-        monkeypatch.setenv("PW_RS_FFT", fft)
     p = _prob(name, 5)
     occ = np.array([1.0, 1.0, 0.9, 0.4, 0.1])
     rho_ref, val_ref = O.rs_density(p.grid, p.grid_fine, p.volume, p.psi, occ, 2, 5)
@@ -182,11 +197,9 @@ def test_rs_calculate_density(name, fft, monkeypatch):
     assert val == pytest.approx(val_ref, rel=1e-13) and val == pytest.approx(10.0, rel=1e-12)
 
 
-@pytest.mark.parametrize("name,fft", [("sic8_rs", None), ("box_rs", None), ("sic8_rs", "rocfft")])
-def test_rs_teter_precond(name, fft, monkeypatch):
+@pytest.mark.parametrize("name", ["sic8_rs", "box_rs"])
+def test_rs_teter_precond(name):
     import torch
-    if fft:
-        monkeypatch.setenv("PW_RS_FFT", fft)
     p = _prob(name, 5)
     a3 = np.random.default_rng(2).standard_normal(p.psi.shape)
     ref = O.rs_teter_precond(p.grid, p.volume, R.teter_of(p.ekin), p.psi, a3)
