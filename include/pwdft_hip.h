@@ -263,6 +263,22 @@ int pw_calculate_density(pw_ctx* ctx, int dtype, int nb, const void* psi, const 
  * (src/SCF/scf.cpp:1473-1486) for a given block. */
 int pw_band_energy(pw_ctx* ctx, int dtype, int nb, const void* X, const double* f, double* E);
 
+/* Harris-Foulkes free energy of the eigenpairs (eig, occ: host, nstate
+ * values) against the input density: SCF::CalculateHarrisEnergy
+ * (src/SCF/scf.cpp:1987-2140, real branch).  Device arrays on the fine grid
+ * of ntot points, stored as the reference's DblNumMat(ntot, ncomp):
+ * density (ntot x nspin; component RHO = 0 first), spindensity (ntot x 2,
+ * UP then DN; read only when nspin = 2, may be NULL otherwise), vxc
+ * (ntot x nspin), vhart and pseudo_charge (ntot).  terms (host, 8 values):
+ * Exc, Eself, EIonSR, EVdw, Eext, fermi, Tbeta, temperature.  EVxc and the
+ * Hartree energy are FP64 device sums in a fixed order.  *efree (host) =
+ * EfreeHarris_: the band sum plus the correction terms at zero temperature
+ * (or when every state is occupied), else the Fermi-Dirac free energy. */
+int pw_harris_energy(pw_ctx* ctx, int ntot, int nspin, double vol, const double* density,
+                     const double* spindensity, const double* vxc, const double* vhart,
+                     const double* pseudo_charge, int nstate, const double* eig, const double* occ,
+                     int num_spin, int num_occupied, const double* terms, double* efree);
+
 /* Nonlocal force F_a = -dE/dR_a through the projectors, d beta/dR = -iG beta:
  * F_a = -2 sum_i f_i sum_{p in a} w_p Re<beta_p|x_i> Re<beta_p|iG x_i>, the
  * G-space form of KohnSham::CalculateForce's nonlocal term
@@ -335,6 +351,16 @@ int pw_batch_info(const pw_batch* batch, long* xtotal, int* njobs, int* nwg, int
  * once).  The autotuner's knobs. */
 int pw_batch_tune(pw_batch* batch, int rp, int zc, int nwg);
 int pw_batch_set_graphs(pw_batch* batch, int enable);
+
+/* Stage timing of the batched launch (as pw_set_timing / pw_get_timing for a
+ * context; no counterpart in the reference): when enabled, the launch stamps
+ * the device wall clock after each of its stage barriers, and
+ * pw_batch_timing returns the last launch's time per stage (ms, 4 values:
+ * z-inverse + projections, xy slabs, z-forward, projector apply +
+ * recurrence), summed over its steps.  Enabling or disabling drops a
+ * captured graph. */
+int pw_batch_set_timing(pw_batch* batch, int enable);
+int pw_batch_timing(pw_batch* batch, double* stage_ms);
 /* The batch's autotuner (as pw_autotune): times work-per-item, row-pair
  * and z-chunk splits on pw_batch_apply_h over every cell, applies the
  * fastest and records it in cache_path (NULL: no file). */

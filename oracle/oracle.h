@@ -177,6 +177,11 @@ double or_band_energy(or_ham* H, int nb, const double* psi, const double* f);
 int or_nonlocal_force(or_ham* H, int nb, const double* psi, const double* f,
                       int natom, int nproj_per_atom, double* force /* 3*natom */);
 
+/* ---------- Harris energy (or_scf.c) ---------- */
+int or_harris_energy(int ntot, int nspin, double vol, const double* density, const double* spindensity,
+                     const double* vxc, const double* vhart, const double* pseudo, int nstate, const double* eig,
+                     const double* occ, int num_spin, int num_occ, const double* terms, double* efree);
+
 #ifdef __cplusplus
 }
 #endif

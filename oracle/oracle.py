@@ -83,8 +83,35 @@ def lib():
                                  C.POINTER(C.c_int)]
         L.or_rs_density.argtypes = [_ip, _ip, C.c_double, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp,
                                     C.POINTER(C.c_double)]
+        L.or_harris_energy.argtypes = [C.c_int, C.c_int, C.c_double, _dp, C.c_void_p, _dp, _dp, _dp, C.c_int,
+                                       _dp, _dp, C.c_int, C.c_int, _dp, C.POINTER(C.c_double)]
         _LIB = L
     return _LIB
+
+
+HARRIS_TERMS = ("Exc", "Eself", "EIonSR", "EVdw", "Eext", "fermi", "Tbeta", "temperature")
+
+
+def harris_energy(density, vxc, vhart, pseudo, vol, eig, occ, num_spin, num_occ, terms, spindensity=None):
+    """SCF::CalculateHarrisEnergy (scf.cpp:1987-2140).  density, vxc:
+    (ncomp, ntot) arrays (component-major: DblNumMat(ntot, ncomp) storage);
+    spindensity (2, ntot) for nspin = 2; terms: dict with HARRIS_TERMS."""
+    density = np.ascontiguousarray(density, dtype=np.float64)
+    vxc = np.ascontiguousarray(vxc, dtype=np.float64)
+    nspin, ntot = vxc.shape
+    sd = None
+    if spindensity is not None:
+        sd = np.ascontiguousarray(spindensity, dtype=np.float64)
+    t = np.array([float(terms[k]) for k in HARRIS_TERMS])
+    e = C.c_double(0.0)
+    _chk(lib().or_harris_energy(ntot, nspin, vol, density.reshape(-1),
+                                sd.ctypes.data_as(C.c_void_p) if sd is not None else None,
+                                vxc.reshape(-1), np.ascontiguousarray(vhart, dtype=np.float64),
+                                np.ascontiguousarray(pseudo, dtype=np.float64), len(eig),
+                                np.ascontiguousarray(eig, dtype=np.float64),
+                                np.ascontiguousarray(occ, dtype=np.float64), int(num_spin), int(num_occ), t,
+                                C.byref(e)), "harris_energy")
+    return e.value
 
 
 def _r(a):

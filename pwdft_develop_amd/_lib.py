@@ -72,6 +72,8 @@ def lib():
         "pw_syevd": [_vp, C.c_int, _vp, C.c_int, _vp],
         "pw_density": [_vp, C.c_int, C.c_int, _vp, _dp, _vp, C.c_int],
         "pw_band_energy": [_vp, C.c_int, C.c_int, _vp, _dp, C.POINTER(C.c_double)],
+        "pw_harris_energy": [_vp, C.c_int, C.c_int, C.c_double, _vp, _vp, _vp, _vp, _vp, C.c_int, _dp, _dp,
+                             C.c_int, C.c_int, _dp, C.POINTER(C.c_double)],
         "pw_calculate_density": [_vp, C.c_int, C.c_int, _vp, _dp, C.c_int, C.c_int, _vp,
                                  C.POINTER(C.c_double)],
         "pw_nonlocal_force": [_vp, C.c_int, C.c_int, _vp, _dp, C.c_int, C.c_int, _dp],
@@ -92,6 +94,8 @@ def lib():
         "pw_batch_info": [_vp, C.POINTER(C.c_long), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
         "pw_batch_tune": [_vp, C.c_int, C.c_int, C.c_int],
         "pw_batch_set_graphs": [_vp, C.c_int],
+        "pw_batch_set_timing": [_vp, C.c_int],
+        "pw_batch_timing": [_vp, _dp],
         "pw_batch_autotune": [_vp, C.c_char_p, C.c_char_p, C.c_int],
         "pw_autotune": [_vp, C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_int],
         "pw_batch_apply_h": [_vp, _vp, _vp],

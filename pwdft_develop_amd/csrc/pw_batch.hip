@@ -349,6 +349,7 @@ struct BatchArgs {
   const float* coefs;
   int degree;
   long xtotal;
+  unsigned long long* stamps;  // null, or one wall-clock stamp per stage end (+ the start)
 };
 
 __global__ void __launch_bounds__(kBatchThreads, 2) k_batch(BatchArgs g) {
@@ -356,6 +357,7 @@ __global__ void __launch_bounds__(kBatchThreads, 2) k_batch(BatchArgs g) {
   float2* lds = reinterpret_cast<float2*>(bsmem);
   unsigned nbar = 0;
   const int steps = g.degree > 0 ? g.degree : 1;
+  if (g.stamps && blockIdx.x == 0 && threadIdx.x == 0) g.stamps[0] = wall_clock64();
   for (int step = 1; step <= steps; step++) {
     // recurrence buffers: step 1 reads X and writes YA; step 2 reads YA and
     // X and writes YB; later steps write over Y_{k-2}
@@ -383,6 +385,7 @@ __global__ void __launch_bounds__(kBatchThreads, 2) k_batch(BatchArgs g) {
         }
       }
       grid_barrier(g.bar, ++nbar);
+      if (g.stamps && blockIdx.x == 0 && threadIdx.x == 0) g.stamps[nbar] = wall_clock64();
     }
   }
   if (g.degree > 0) {
@@ -434,6 +437,8 @@ struct pw_batch {
   hipGraphExec_t gexec = nullptr;
   long gkey = -1, gwarm = -1;
   const void *gX = nullptr, *gHX = nullptr;
+  unsigned long long* d_stamps = nullptr;  // stage timing (pw_batch_set_timing)
+  int stamp_steps = 0;                      // steps of the last timed launch
 };
 
 namespace pw {
@@ -583,6 +588,8 @@ static int batch_launch(pw_batch* b, const void* X, void* HX, int degree) {
   g.coefs = b->d_coef;
   g.degree = degree;
   g.xtotal = b->xtotal;
+  g.stamps = b->d_stamps;
+  if (b->d_stamps) b->stamp_steps = degree > 0 ? degree : 1;
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, s, b->d_bar);
   hipLaunchKernelGGL(k_batch, dim3(b->nwg), dim3(kBatchThreads), b->lds, s, g);
   PW_HIP(hipGetLastError());
@@ -658,6 +665,7 @@ static int batch_run(pw_batch* b, const void* X, void* HX, int degree) {
 extern "C" int pw_batch_destroy(pw_batch* b) {
   if (!b) return PW_OK;
   batch_free_device(b);
+  if (b->d_stamps) (void)hipFree(b->d_stamps);
   for (pw_ctx* c : b->ctx)
     if (c) pw_ctx_destroy(c);
   if (b->gstream) (void)hipStreamDestroy(b->gstream);
@@ -744,6 +752,34 @@ extern "C" int pw_batch_tune(pw_batch* b, int rp, int zc, int nwg) {
   b->zc_force = zc;
   b->nwg_force = nwg;
   return batch_upload(b);
+}
+
+// Stage timing: the launch stamps the device wall clock (100 MHz) at its
+// start and after every stage barrier; pw_batch_timing sums the last
+// launch's stage durations by stage.
+extern "C" int pw_batch_set_timing(pw_batch* b, int enable) {
+  if (!b) return fail(PW_ERR_ARG, "null batch");
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);  // the stamps pointer is baked into a graph
+  b->gexec = nullptr;
+  b->gkey = b->gwarm = -1;
+  if (enable && !b->d_stamps) PW_HIP(hipMalloc((void**)&b->d_stamps, sizeof(unsigned long long) * (4 * kMaxDegree + 1)));
+  if (!enable && b->d_stamps) {
+    (void)hipFree(b->d_stamps);
+    b->d_stamps = nullptr;
+  }
+  b->stamp_steps = 0;
+  return PW_OK;
+}
+
+extern "C" int pw_batch_timing(pw_batch* b, double* stage_ms) {
+  if (!b || !stage_ms) return fail(PW_ERR_ARG, "pw_batch_timing: bad argument");
+  for (int i = 0; i < 4; i++) stage_ms[i] = 0.0;
+  if (!b->d_stamps || b->stamp_steps == 0) return PW_OK;
+  std::vector<unsigned long long> h(4 * (size_t)b->stamp_steps + 1);
+  PW_HIP(hipStreamSynchronize(b->stream));
+  PW_HIP(hipMemcpy(h.data(), b->d_stamps, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost));
+  for (int k = 1; k < (int)h.size(); k++) stage_ms[(k - 1) % 4] += (double)(h[k] - h[k - 1]) * 1e-5;
+  return PW_OK;
 }
 
 extern "C" int pw_batch_set_graphs(pw_batch* b, int enable) {

@@ -261,7 +261,8 @@ k_sytrd(double* __restrict__ A, int n, int lda, int nbk, double* __restrict__ d,
 //   (c) every wave sums A v over its blocks that meet the trailing matrix:
 //       row sums across the 16 lanes of a DPP row, column sums (the
 //       transposed half of an off-diagonal block) across lanes l, l+16,
-//       l+32, l+48; per-wave partials in LDS, added in a fixed order,
+//       l+32, l+48 by v_permlane16_swap / v_permlane32_swap; per-wave
+//       partials in LDS, added in a fixed order,
 //   (d) wave 0: w = p - (tau/2)(p.v) v with p = tau A v,
 //   (e) A -= v w^T + w v^T: one MFMA per 16 x 16 register tile (K = 2 of
 //       the 4), read-modify-write on the LDS diagonal blocks.
@@ -302,6 +303,22 @@ __device__ __forceinline__ double row16_sum8(const double (&r)[8], int lr) {
   const double keep = b1 ? s2[1] : s2[0], send = b1 ? s2[0] : s2[1];
   double s3 = keep + dpp_f64<0x4E, 0xf>(send);  // quad_perm [2,3,0,1]
   return s3 + dpp_f64<0xB1, 0xf>(s3);            // quad_perm [1,0,3,2]
+}
+
+// Sum over the four 16-lane rows of a wavefront, in every lane:
+// v_permlane16_swap pairs rows (0, 1) and (2, 3), v_permlane32_swap the two
+// halves (VALU lane swaps; lane layouts checked on gfx950).
+__device__ __forceinline__ double half_sum(double v) {
+  const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+  auto l32 = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+  auto h32 = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+  return __hiloint2double((int)h32[0], (int)l32[0]) + __hiloint2double((int)h32[1], (int)l32[1]);
+}
+__device__ __forceinline__ double row4_sum(double v) {
+  const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+  auto l16 = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+  auto h16 = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+  return half_sum(__hiloint2double((int)h16[0], (int)l16[0]) + __hiloint2double((int)h16[1], (int)l16[1]));
 }
 
 // Column c, rows r0 .. n-1, of the on-chip matrix into col[] (slot s of
@@ -438,7 +455,7 @@ k_sytrd_reg(double* __restrict__ A, int n, int lda, double* __restrict__ d, doub
         double acc = 0.0;
 #pragma unroll
         for (int j = 0; j < 16; j++) acc += Dw[r * kTr2Ld + 16 * h + j] * sv[32 * w + 16 * h + j];
-        acc += __shfl_xor(acc, 32);
+        acc = half_sum(acc);
         if (h == 0) pw[w * 256 + 32 * w + r] += acc;
         wave_lds_sync();
       }
@@ -466,10 +483,8 @@ k_sytrd_reg(double* __restrict__ A, int n, int lda, double* __restrict__ d, doub
             cp0 += a[s][ti][i] * vr;
             cp1 += a[s][ti + 2][i] * vr;
           }
-        cp0 += __shfl_xor(cp0, 16);
-        cp1 += __shfl_xor(cp1, 16);
-        cp0 += __shfl_xor(cp0, 32);
-        cp1 += __shfl_xor(cp1, 32);
+        cp0 = row4_sum(cp0);
+        cp1 = row4_sum(cp1);
         if (lk == 0) {
           pw[w * 256 + 32 * J + lr] += cp0;
           pw[w * 256 + 32 * J + 16 + lr] += cp1;

@@ -326,6 +326,81 @@ __device__ __forceinline__ void reg_fft(typename cx<T>::t (&a)[N]) {
   reg_fft_rec<T, N, 1, N, SG>(a);
 }
 
+// Value held by the lane whose index differs from this lane's in bit 0
+// (X = 1: DPP quad_perm [1,0,3,2]) or in bit 1 (X = 2: quad_perm [2,3,0,1]).
+template <int X>
+__device__ __forceinline__ float lane_xor(float v) {
+  static_assert(X == 1 || X == 2, "lane bit 0 or 1");
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), X == 1 ? 0xB1 : 0x4E, 0xf, 0xf, true));
+}
+template <int X>
+__device__ __forceinline__ double lane_xor(double v) {
+  static_assert(X == 1 || X == 2, "lane bit 0 or 1");
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), X == 1 ? 0xB1 : 0x4E, 0xf, 0xf, true);
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), X == 1 ? 0xB1 : 0x4E, 0xf, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+
+// v *= w^(h K), w = exp(SG 2 pi i / N), h = 0 .. 3 (a runtime lane index;
+// the three candidate twiddles are compile-time constants).
+template <typename T, int N, int SG, int K>
+__device__ __forceinline__ void lane_twiddle(typename cx<T>::t& v, int h) {
+  typedef typename cx<T>::t C;
+  if constexpr (K == 0) return;
+  constexpr ctw::TwR<N, 4, K> W{};
+  const T c = h == 1 ? (T)W.c[1] : (h == 2 ? (T)W.c[2] : (h == 3 ? (T)W.c[3] : (T)1));
+  const T s = h == 1 ? (T)(SG * W.s[1]) : (h == 2 ? (T)(SG * W.s[2]) : (h == 3 ? (T)(SG * W.s[3]) : (T)0));
+  v = cmul(v, C{c, s});
+}
+template <typename T, int N, int SG, int M, int... K>
+__device__ __forceinline__ void lane_twiddles(typename cx<T>::t (&a)[M], int h, std::integer_sequence<int, K...>) {
+  (lane_twiddle<T, N, SG, K>(a[K], h), ...);
+}
+
+// Length-N FFT of one LDS column (N contiguous complex values at col) by L
+// adjacent lanes, L = 1, 2 or 4: lanes with the same t / L share col and
+// must be active together; h = t % L.  L = 1: the whole transform in one
+// thread's registers.  L > 1, decimation in time: lane h transforms the
+// samples m L + h (length M = N / L) and multiplies its output k by
+// w^(h k), w = exp(SG 2 pi i / N); the L partial spectra then combine by
+// the L-point DFT across the lanes, one radix-2 stage per lane bit (bit 1,
+// then bit 0) with a DPP exchange, the second stage's input on lane 3
+// multiplied by exp(SG 2 pi i / 4).  Lane h = 2 b1 + b0 ends with
+// X[q M + k], k < M, q = b0 (L = 2) or b1 + 2 b0 (L = 4).
+template <typename T, int N, int SG, int L>
+__device__ __forceinline__ void col_fft_lanes(typename cx<T>::t* col, int h) {
+  typedef typename cx<T>::t C;
+  static_assert(L == 1 || L == 2 || L == 4, "1, 2 or 4 lanes per column");
+  static_assert(N % L == 0, "L divides N");
+  constexpr int M = N / L;
+  C a[M];
+#pragma unroll
+  for (int m = 0; m < M; m++) a[m] = col[m * L + h];
+  reg_fft<T, M, SG>(a);
+  int q = 0;
+  if constexpr (L > 1) {
+    lane_twiddles<T, N, SG, M>(a, h, std::make_integer_sequence<int, M>{});
+    if constexpr (L == 4) {
+      const bool up = h & 2;
+#pragma unroll
+      for (int k = 0; k < M; k++) {
+        const C o{lane_xor<2>(a[k].x), lane_xor<2>(a[k].y)};
+        a[k] = up ? csub(o, a[k]) : cadd(a[k], o);
+        if (h == 3) a[k] = cmul_si(a[k], (T)SG);
+      }
+    }
+    const bool odd = h & 1;
+#pragma unroll
+    for (int k = 0; k < M; k++) {
+      const C o{lane_xor<1>(a[k].x), lane_xor<1>(a[k].y)};
+      a[k] = odd ? csub(o, a[k]) : cadd(a[k], o);
+    }
+    q = L == 2 ? (h & 1) : ((h >> 1) + 2 * (h & 1));
+  }
+#pragma unroll
+  for (int k = 0; k < M; k++) col[q * M + k] = a[k];
+}
+
 // Ordering point for a register-held array: every element must be final
 // here (each is an operand of an empty volatile asm), and no memory access
 // moves across.  Without it the scheduler hoists the loads that follow an

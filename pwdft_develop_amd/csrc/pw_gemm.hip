@@ -18,6 +18,8 @@
 // row 4(l/16)+i for the FP32 MFMA but row 4i + l/16 for the FP64 one
 // (checked on gfx950 with a known 16x16x4 product).
 #include <algorithm>
+#include <cstdlib>
+#include <string>
 
 #include "pw_internal.h"
 
@@ -39,9 +41,43 @@ __device__ __forceinline__ f4v mfma16(float a, float b, f4v c) {
 
 constexpr int BM = 64, BN = 64, BK = 32;
 
+// Block -> (row tile, column tile, K split) for the split-K products, which
+// launch ti x tj tiles x nsplit splits as one row of blocks.  Blocks are
+// dispatched round-robin over the 8 XCDs (block b to XCD b % 8), each with
+// its own L2; giving every XCD one contiguous range of (split, tile) pairs
+// puts the ti x tj tiles of a K chunk, which read the same A and B rows, in
+// one L2 instead of spreading them over all eight.
+struct TnBlock {
+  int i, j, split;
+};
+__device__ __forceinline__ TnBlock tn_block(int n1, int n2, int K, int kc, int tb) {
+  const int ti = (n1 + tb - 1) / tb, tj = (n2 + tb - 1) / tb, tiles = ti * tj;
+  const int n = tiles * ((K + kc - 1) / kc), b = blockIdx.x;
+  const int q = n >> 3, r = n & 7, x = b & 7;
+  const int L = x * q + min(x, r) + (b >> 3);
+  const int tile = L % tiles;
+  return TnBlock{tile % ti, tile / ti, L / tiles};
+}
+
 template <typename T>
 __device__ __forceinline__ int acc_row(int lane, int r) {
   return sizeof(T) == 8 ? 4 * r + (lane >> 4) : 4 * (lane >> 4) + r;
+}
+
+// NN products accumulate FP64 tiles transposed (mfma(b, a) = (A B)^T): the
+// 16x16x4f64 accumulator then holds, in register r, 16 consecutive rows of
+// column 4 r + lane / 16 in lanes 16 j .. 16 j + 15, so the epilogue reads
+// and writes 128-byte runs; FP32 tiles keep A B (four consecutive rows of
+// column lane % 16 per lane).  (row, column) of accumulator register r:
+__device__ __forceinline__ d4v mfma_nn(double a, double b, d4v c) { return mfma16(b, a, c); }
+__device__ __forceinline__ f4v mfma_nn(float a, float b, f4v c) { return mfma16(a, b, c); }
+template <typename T>
+__device__ __forceinline__ int nn_row(int lane, int r) {
+  return sizeof(T) == 8 ? (lane & 15) : 4 * (lane >> 4) + r;
+}
+template <typename T>
+__device__ __forceinline__ int nn_col(int lane, int r) {
+  return sizeof(T) == 8 ? 4 * r + (lane >> 4) : (lane & 15);
 }
 
 // Partial C tile of A^T B over K rows [kb, ke): P[split](n1 x n2, ld n1).
@@ -60,7 +96,8 @@ k_gemm_tn_partial(const T* __restrict__ A, int lda, const T* __restrict__ B, int
   extern __shared__ __align__(16) unsigned char gemm_smem[];
   T (*As)[BK + 1] = reinterpret_cast<T (*)[BK + 1]>(gemm_smem);
   T (*Bs)[BK + 1] = As + TB;
-  const int i0 = blockIdx.x * TB, j0 = blockIdx.y * TB, split = blockIdx.z;
+  const TnBlock tb = tn_block(n1, n2, K, kc, TB);
+  const int i0 = tb.i * TB, j0 = tb.j * TB, split = tb.split;
   const int kb = split * kc, ke = min(K, kb + kc);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wi = (w & 1) * 32 * TM, wj = (w >> 1) * 32 * TM;
@@ -128,19 +165,17 @@ k_gemm_tn_partial(const T* __restrict__ A, int lda, const T* __restrict__ B, int
 // accumulator is laid out as v_mfma_f32_16x16x4f32's.
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
-__device__ __forceinline__ unsigned short bf16_rne(float x) {
-  unsigned u = __float_as_uint(x);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (unsigned short)(u >> 16);
-}
-__device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+// Round to nearest even: one v_cvt_pk_bf16_f32 per pair of values on
+// gfx950; x - hi is exact in FP32, so the terms sum to x up to the last
+// term's rounding.
 template <int NS>
 __device__ __forceinline__ void bf16_split(float x, unsigned short (&t)[NS]) {
   float r = x;
 #pragma unroll
   for (int i = 0; i < NS; i++) {
-    t[i] = bf16_rne(r);
-    r -= bf16_val(t[i]);
+    const __bf16 h = (__bf16)r;
+    t[i] = __builtin_bit_cast(unsigned short, h);
+    r -= (float)h;
   }
 }
 __device__ __forceinline__ f4v mfma_b16(const unsigned short* a, const unsigned short* b, f4v acc) {
@@ -174,7 +209,8 @@ k_gemm_tn_partial_bf(const float* __restrict__ A, int lda, const float* __restri
   extern __shared__ __align__(16) unsigned char gemm_smem[];
   unsigned short* As = reinterpret_cast<unsigned short*>(gemm_smem);  // NS term tiles of A, then of B
   unsigned short* Bs = As + NS * TB * BKP;
-  const int i0 = blockIdx.x * TB, j0 = blockIdx.y * TB, split = blockIdx.z;
+  const TnBlock tb = tn_block(n1, n2, K, kc, TB);
+  const int i0 = tb.i * TB, j0 = tb.j * TB, split = tb.split;
   const int kb = split * kc, ke = min(K, kb + kc);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wi = (w & 1) * 32 * TM, wj = (w >> 1) * 32 * TM;
@@ -313,26 +349,23 @@ k_gemm_nn(const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb, T*
 #pragma unroll
       for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++) acc[a][b] = mfma16(av[a], bv[b], acc[a][b]);
+        for (int b = 0; b < 2; b++) acc[a][b] = mfma_nn(av[a], bv[b], acc[a][b]);
     }
     lds_barrier();
   }
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++) {
-      const int col = j0 + wj + b * 16 + (lane & 15);
-      if (col >= n) continue;
+    for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int row = i0 + wi + a * 16 + acc_row<T>(lane, r);
-        if (row < m) {
+        const int row = i0 + wi + a * 16 + nn_row<T>(lane, r), col = j0 + wj + b * 16 + nn_col<T>(lane, r);
+        if (row < m && col < n) {
           T* y = Y + row + (size_t)ldy * col;
           const T v = alpha * acc[a][b][r];
           *y = (beta == (T)0) ? v : v + beta * *y;
         }
       }
-    }
 }
 
 // Epilogue of the projector apply inside H Y (acc = (beta w) W for one
@@ -456,6 +489,163 @@ k_gemm_nn_bf(const float* __restrict__ A, int lda, const float* __restrict__ B, 
   }
 }
 
+// ---- products with operands split ahead of time ----
+// Operand tile layout (both operands): for a 16-row (or 16-column) tile r
+// and a 32-deep K chunk q, the NS_MAX = 3 term blocks of 64 lanes x 8 BF16
+// (one 16-byte v_mfma_f32_16x16x32_bf16 operand per lane: row / column
+// l % 16, k = 32 q + 8 (l / 16) .. + 7) sit at ((r nkc + q) 3 + i) 512
+// BF16 elements, so that a wave loads an operand with one 1 KiB read.  Rows,
+// columns and k beyond the matrix are zero.  The 2-term split is the first
+// two terms of the 3-term one.
+constexpr int kSplitTerms = 3;
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+// One operand group (r, q) per lane-block: A column-major (ld lda): rows
+// of A are the tile dimension (transpose = false), or columns of B
+// (transpose = true, B column-major with ld ldb: B(k, col) at col ldb + k).
+template <bool kCols>
+__global__ void k_split_operand(const float* __restrict__ X, int ldx, int nr, int K, int nkc,
+                                unsigned short* __restrict__ out, long ngroups) {
+  const long g = (long)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (g >= ngroups) return;
+  const int lane = threadIdx.x & 63;
+  const int r = (int)(g / nkc), q = (int)(g - (long)r * nkc);
+  const int idx = 16 * r + (lane & 15), k0 = 32 * q + 8 * (lane >> 4);
+  unsigned short t[kSplitTerms][8];
+#pragma unroll
+  for (int j = 0; j < 8; j++) {
+    const int k = k0 + j;
+    float x = 0.f;
+    if (idx < nr && k < K) x = kCols ? X[(size_t)idx * ldx + k] : X[(size_t)k * ldx + idx];
+    unsigned short v[kSplitTerms];
+    bf16_split<kSplitTerms>(x, v);
+#pragma unroll
+    for (int i = 0; i < kSplitTerms; i++) t[i][j] = v[i];
+  }
+#pragma unroll
+  for (int i = 0; i < kSplitTerms; i++) {
+    u16x8 v;
+#pragma unroll
+    for (int j = 0; j < 8; j++) v[j] = t[i][j];
+    *reinterpret_cast<u16x8*>(out + ((size_t)g * kSplitTerms + i) * 512 + 8 * lane) = v;
+  }
+}
+
+template <int NS>
+__device__ __forceinline__ f4v mfma_split_r(const bf16x8 (&a)[NS], const bf16x8 (&b)[NS], f4v acc) {
+  if constexpr (NS == 3) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], acc, 0, 0, 0);
+  }
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], acc, 0, 0, 0);
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], acc, 0, 0, 0);
+}
+
+// out = epilogue(A B) with both operands split ahead of time (layout above;
+// A: nrt = ceil(m / 16) row tiles, B: ceil(n / 16) column tiles, nkc
+// chunks).  A workgroup takes 64 rows x 256 columns: wave w the 64 x 64
+// block of columns 64 w .., as 4 x 4 MFMA tiles whose operands come
+// straight from global memory (A from HBM, shared by the four waves through
+// the caches; B from L2), the next chunk's operands in flight during the
+// current chunk's products.  (Staging the A slab in LDS measured slower:
+// 42 against 38 us at 13134 x 256 x 256.)  The epilogue works on the
+// accumulator layout: a lane's four rows of one column are contiguous, read
+// and written as two float pairs (m and ld even).
+template <bool kCheb, int NS>
+__global__ void __launch_bounds__(256, 1)
+k_gemm_nn_pre(const unsigned short* __restrict__ Ab, const unsigned short* __restrict__ Bb, int m, int n, int nkc,
+              NnEpilogue<float> ep) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int nrt = (m + 15) >> 4, nct = (n + 15) >> 4;
+  const int rt0 = 4 * blockIdx.x, ct0 = 16 * blockIdx.y + 4 * w;
+  const int na = min(4, nrt - rt0), nbt = min(4, nct - ct0);
+  if (nbt <= 0) return;
+  const bf16x8* A8 = reinterpret_cast<const bf16x8*>(Ab);
+  const bf16x8* B8 = reinterpret_cast<const bf16x8*>(Bb);
+  f4v acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[a][b] = f4v{0, 0, 0, 0};
+  bf16x8 av[4][NS], bv[4][NS];
+  auto load = [&](int q, bf16x8 (&xa)[4][NS], bf16x8 (&xb)[4][NS]) {
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int i = 0; i < NS; i++)
+        xa[a][i] = a < na ? A8[((size_t)((rt0 + a) * nkc + q) * kSplitTerms + i) * 64 + lane] : bf16x8{};
+#pragma unroll
+    for (int b = 0; b < 4; b++)
+#pragma unroll
+      for (int i = 0; i < NS; i++)
+        xb[b][i] = b < nbt ? B8[((size_t)((ct0 + b) * nkc + q) * kSplitTerms + i) * 64 + lane] : bf16x8{};
+  };
+  load(0, av, bv);
+  for (int q = 0; q < nkc; q++) {
+    bf16x8 an[4][NS], bn[4][NS];
+    if (q + 1 < nkc) load(q + 1, an, bn);
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int b = 0; b < 4; b++) acc[a][b] = mfma_split_r<NS>(av[a], bv[b], acc[a][b]);
+    if (q + 1 < nkc) {
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int i = 0; i < NS; i++) {
+          av[a][i] = an[a][i];
+          bv[a][i] = bn[a][i];
+        }
+    }
+  }
+  float cc = 0.f, s1 = 0.f, s2 = 0.f;
+  if constexpr (kCheb) {
+    cc = ep.coef[3 * ep.step];
+    s1 = ep.coef[3 * ep.step + 1];
+    s2 = ep.coef[3 * ep.step + 2];
+  }
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    const int col = 16 * (ct0 + b) + (lane & 15);
+    if (b >= nbt || col >= n) continue;
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const int r0 = 16 * (rt0 + a) + 4 * (lane >> 4);
+      if (a >= na) continue;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int row = r0 + 2 * h;
+        if (row >= m) continue;
+        const size_t e = (size_t)col * ep.ld + row;
+        const float x0 = acc[a][b][2 * h], x1 = acc[a][b][2 * h + 1];
+        float2 o;
+        if constexpr (kCheb) {
+          const float2 hv = *reinterpret_cast<const float2*>(ep.H + e);
+          const float2 yc = *reinterpret_cast<const float2*>(ep.Yc + e);
+          o.x = (hv.x + x0 - cc * yc.x) * s1;
+          o.y = (hv.y + x1 - cc * yc.y) * s1;
+          if (ep.Yp) {
+            const float2 yp = *reinterpret_cast<const float2*>(ep.Yp + e);
+            o.x -= s2 * yp.x;
+            o.y -= s2 * yp.y;
+          }
+        } else {
+          o.x = ep.alpha * x0;
+          o.y = ep.alpha * x1;
+          if (ep.beta != 0.f) {
+            const float2 hv = *reinterpret_cast<const float2*>(ep.H + e);
+            o.x += ep.beta * hv.x;
+            o.y += ep.beta * hv.y;
+          }
+        }
+        *reinterpret_cast<float2*>(ep.out + e) = o;
+      }
+    }
+  }
+}
+
 // The same product on v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32
 // (pipelined like k_gemm_nn), with the same epilogue.
 template <typename T, bool kCheb>
@@ -504,22 +694,19 @@ k_gemm_nn_ep(const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb,
 #pragma unroll
       for (int a = 0; a < 2; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++) acc[a][b] = mfma16(av[a], bv[b], acc[a][b]);
+        for (int b = 0; b < 2; b++) acc[a][b] = mfma_nn(av[a], bv[b], acc[a][b]);
     }
     lds_barrier();
   }
 #pragma unroll
   for (int a = 0; a < 2; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++) {
-      const int col = j0 + wj + b * 16 + (lane & 15);
-      if (col >= n) continue;
+    for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        const int row = i0 + wi + a * 16 + acc_row<T>(lane, r);
-        if (row < m) nn_epilogue<T, kCheb>(ep, (size_t)col * ep.ld + row, acc[a][b][r]);
+        const int row = i0 + wi + a * 16 + nn_row<T>(lane, r), col = j0 + wj + b * 16 + nn_col<T>(lane, r);
+        if (row < m && col < n) nn_epilogue<T, kCheb>(ep, (size_t)col * ep.ld + row, acc[a][b][r]);
       }
-    }
 }
 
 // Elementwise form of the epilogue (no projectors: acc = 0).
@@ -659,7 +846,7 @@ int gemm_tn_ws(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B
   if constexpr (sizeof(T) == 4) {
     if (split == 2 || split == 3) {
       const size_t ldsb = sizeof(unsigned short) * 2 * split * (size_t)(BM * tm) * BKP;
-      const dim3 gp(ti, tj, nsplit);
+      const dim3 gp(ti * tj * nsplit);
       if (split == 3 && tm == 2) {
         PW_TRY(ensure_lds((const void*)k_gemm_tn_partial_bf<2, 3>, ldsb));
         hipLaunchKernelGGL((k_gemm_tn_partial_bf<2, 3>), gp, dim3(256), ldsb, c->stream, A, lda, B, ldb, K, n1, n2,
@@ -686,10 +873,10 @@ int gemm_tn_ws(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B
   const size_t lds = sizeof(T) * 2 * (size_t)(BM * tm) * (BK + 1);
   if (tm == 2) {
     PW_TRY(ensure_lds((const void*)k_gemm_tn_partial<T, 2>, lds));
-    hipLaunchKernelGGL((k_gemm_tn_partial<T, 2>), dim3(ti, tj, nsplit), dim3(256), lds, c->stream, A, lda, B, ldb,
+    hipLaunchKernelGGL((k_gemm_tn_partial<T, 2>), dim3(ti * tj * nsplit), dim3(256), lds, c->stream, A, lda, B, ldb,
                        K, n1, n2, kc, P);
   } else {
-    hipLaunchKernelGGL((k_gemm_tn_partial<T, 1>), dim3(ti, tj, nsplit), dim3(256), lds, c->stream, A, lda, B, ldb,
+    hipLaunchKernelGGL((k_gemm_tn_partial<T, 1>), dim3(ti * tj * nsplit), dim3(256), lds, c->stream, A, lda, B, ldb,
                        K, n1, n2, kc, P);
   }
   const long total = (long)n1 * n2;
@@ -763,6 +950,46 @@ int gemm_nn_ep(pw_ctx* c, int m, int n, int k, const T* A, int lda, const T* B, 
     else
       hipLaunchKernelGGL((k_gemm_nn_ep<T, false>), grid, dim3(256), 0, c->stream, A, lda, B, ldb, m, n, k, ep);
   }
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+// A (m x k, column-major, ld lda) split ahead of time into *out (the tile
+// layout of k_gemm_nn_pre; allocated here, freed by the caller).
+int split_a_operand(pw_ctx* c, const float* A, int lda, int m, int k, unsigned short** out) {
+  *out = nullptr;
+  if (m <= 0 || k <= 0) return PW_OK;
+  const int nrt = (m + 15) / 16, nkc = (k + 31) / 32;
+  const long ng = (long)nrt * nkc;
+  PW_HIP(hipMalloc((void**)out, sizeof(unsigned short) * (size_t)ng * kSplitTerms * 512));
+  hipLaunchKernelGGL(k_split_operand<false>, dim3((unsigned)((ng + 3) / 4)), dim3(256), 0, c->stream, A, lda, m, k,
+                     nkc, *out, ng);
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+size_t split_b_bytes(int n, int k) {
+  return sizeof(unsigned short) * (size_t)((n + 15) / 16) * ((k + 31) / 32) * kSplitTerms * 512;
+}
+
+// out (m x n) = epilogue(A B) on k_gemm_nn_pre: Ab = A split by
+// split_a_operand, B (k x n, ld ldb) split here into Bws (split_b_bytes(n,
+// k) bytes); NS = split terms (2 or 3).  m and ld even, H / Yc / Yp / out
+// 8-byte aligned.
+int gemm_nn_pre(pw_ctx* c, int m, int n, int k, const unsigned short* Ab, const float* B, int ldb,
+                unsigned short* Bws, const float* H, const float* Yc, const float* Yp, float* out, int ld,
+                const float* coef, int step, bool cheb, int split) {
+  if (m <= 0 || n <= 0) return PW_OK;
+  if ((m & 1) || (ld & 1)) return fail(PW_ERR_ARG, "gemm_nn_pre: odd row count or leading dimension");
+  const int nkc = (k + 31) / 32, nct = (n + 15) / 16, nrt = (m + 15) / 16;
+  const long ng = (long)nct * nkc;
+  hipLaunchKernelGGL(k_split_operand<true>, dim3((unsigned)((ng + 3) / 4)), dim3(256), 0, c->stream, B, ldb, n, k,
+                     nkc, Bws, ng);
+  NnEpilogue<float> ep{H, Yc, Yp, out, ld, coef, step, 1.f, 1.f};
+  const dim3 grid((nrt + 3) / 4, (nct + 15) / 16);
+  auto kern = cheb ? (split == 3 ? k_gemm_nn_pre<true, 3> : k_gemm_nn_pre<true, 2>)
+                  : (split == 3 ? k_gemm_nn_pre<false, 3> : k_gemm_nn_pre<false, 2>);
+  hipLaunchKernelGGL(kern, grid, dim3(256), 0, c->stream, Ab, Bws, m, n, nkc, ep);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }

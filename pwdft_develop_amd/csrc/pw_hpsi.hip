@@ -176,14 +176,15 @@ k_slab_xy(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restric
 // by scheduling barriers so that the compiler does not hoist all of them next
 // to the spectrum (that spills past the 256 VGPRs of two waves per SIMD).
 // Same arithmetic, in the same order, as k_slab_xy.
-template <typename T, int N1, int N2, int SB, int LB>
+// LY lanes per y column transform (1, 2 or 4).
+template <typename T, int N1, int N2, int SB, int LB, int LY>
 __global__ void __launch_bounds__(256, LB)  // LB waves per SIMD: <= 512 / LB registers
 k_slab_xy_reg(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restrict__ t2, int n3, int nstick,
               int kxmax, int nb, const int* __restrict__ pos, const int* __restrict__ mpos,
               const T* __restrict__ vtot) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  slab_xy_reg_body<T, N1, N2, SB>(blockIdx.x, blockIdx.y * SB, t1, t2, n3, nstick, kxmax, nb, pos, mpos, vtot,
-                                  smem_raw);
+  slab_xy_reg_body<T, N1, N2, SB, LY>(blockIdx.x, blockIdx.y * SB, t1, t2, n3, nstick, kxmax, nb, pos, mpos, vtot,
+                                      smem_raw);
 }
 
 // ---------------------------------------------------------------- kernel 3
@@ -260,16 +261,33 @@ auto slab_kernel(int n1, int n2) -> decltype(&k_slab_xy<T, 0, 0>) {
 template <typename T> struct SlabCfg { static constexpr int SB = 8; };
 template <> struct SlabCfg<double> { static constexpr int SB = 4; };
 template <typename T, int N> constexpr int slab_lb() { return (sizeof(T) == 4 || N <= 48) ? 2 : 1; }
+// lanes per y column transform when no plan chooses
+template <typename T> static int default_slab_ly(int n) {
+  (void)n;
+  return 1;
+}
 #define PW_REG_SIZES(X) X(24) X(30) X(32) X(36) X(40) X(48) X(60) X(64) X(72)
 template <typename T>
-using slab_reg_fn = decltype(&k_slab_xy_reg<T, 24, 24, SlabCfg<T>::SB, 2>);
+using slab_reg_fn = decltype(&k_slab_xy_reg<T, 24, 24, SlabCfg<T>::SB, 2, 1>);
+template <typename T, int N>
+static slab_reg_fn<T> slab_reg_ly(int ly) {
+  constexpr int SB = SlabCfg<T>::SB, LB = slab_lb<T, N>();
+  if (ly == 1) return &k_slab_xy_reg<T, N, N, SB, LB, 1>;
+  if (ly == 2) return &k_slab_xy_reg<T, N, N, SB, LB, 2>;
+  if constexpr (N % 4 == 0)
+    if (ly == 4) return &k_slab_xy_reg<T, N, N, SB, LB, 4>;
+  return nullptr;
+}
+// the register slab kernel of an n1 x n2 plane with ly lanes per y column
+// transform (ly <= 0: the default for the precision), or null if there is none
 template <typename T>
-static slab_reg_fn<T> slab_reg_kernel(int n1, int n2) {
+static slab_reg_fn<T> slab_reg_kernel(int n1, int n2, int ly = 0) {
   if (n1 != n2) return nullptr;
+  if (ly <= 0) ly = default_slab_ly<T>(n1);
   switch (n1) {
 #define PW_CASE(n) \
   case n:          \
-    return &k_slab_xy_reg<T, n, n, SlabCfg<T>::SB, slab_lb<T, n>()>;
+    return slab_reg_ly<T, n>(ly);
     PW_REG_SIZES(PW_CASE)
 #undef PW_CASE
     default:
@@ -471,6 +489,8 @@ extern "C" int pw_ctx_create(pw_ctx** out, const int grid[3], const double super
     const char* sl = getenv("PW_SLAB");
     for (auto& pl : c->plan) {
       pl.reg = !(sl && std::string(sl).compare(0, 3, "lds") == 0);
+      // "reg:<ly>": the register kernel with ly lanes per y column transform
+      if (sl && std::string(sl).compare(0, 4, "reg:") == 0) pl.ly = std::max(1, atoi(sl + 4));
       // "lds:<g>[:<r>]": the LDS kernel with its large-plane layout, g y
       // columns per group (and r x row pairs per step)
       if (sl && std::string(sl).compare(0, 4, "lds:") == 0) {
@@ -586,7 +606,7 @@ extern "C" int pw_ctx_destroy(pw_ctx* c) {
   void* ptrs[] = {c->d_pt_idx, c->d_pt_sz, c->d_chunk_off, c->d_gpt_idx, c->d_gpt_sz, c->d_gchunk_off,
                   c->d_pos, c->d_mpos, c->d_ekin, c->d_teter, c->d_ekin_f, c->d_teter_f, c->d_tw64,
                   c->d_tw32, c->d_vtot, c->d_vtot_f, c->d_beta, c->d_betaw, c->d_W, c->d_beta_f,
-                  c->d_betaw_f, c->d_W_f, c->d_t1, c->d_t2};
+                  c->d_betaw_f, c->d_W_f, c->d_t1, c->d_t2, c->d_betaw_bf, c->d_W_bf};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->blas) rocblas_destroy_handle(c->blas);
@@ -631,11 +651,12 @@ extern "C" int pw_set_vtot(pw_ctx* c, const double* vtot) {
 extern "C" int pw_set_nonlocal(pw_ctx* c, int nbeta, const double* beta, const double* w) {
   if (!c || nbeta < 0 || (nbeta > 0 && (!beta || !w))) return fail(PW_ERR_ARG, "pw_set_nonlocal: bad argument");
   invalidate_graph(c);
-  void* ptrs[] = {c->d_beta, c->d_betaw, c->d_W, c->d_beta_f, c->d_betaw_f, c->d_W_f};
+  void* ptrs[] = {c->d_beta, c->d_betaw, c->d_W, c->d_beta_f, c->d_betaw_f, c->d_W_f, c->d_betaw_bf, c->d_W_bf};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   c->d_beta = c->d_betaw = c->d_W = nullptr;
   c->d_beta_f = c->d_betaw_f = c->d_W_f = nullptr;
+  c->d_betaw_bf = c->d_W_bf = nullptr;
   c->nbeta = nbeta;
   c->h_wgt.assign(w ? w : (const double*)nullptr, w ? w + nbeta : (const double*)nullptr);
   if (nbeta == 0) return PW_OK;
@@ -651,6 +672,10 @@ extern "C" int pw_set_nonlocal(pw_ctx* c, int nbeta, const double* beta, const d
   if ((rc = upload(&c->d_betaw_f, bwf.data(), m * nbeta)) != PW_OK) return rc;
   PW_HIP(hipMalloc((void**)&c->d_W, sizeof(double) * (size_t)nbeta * c->max_bands));
   PW_HIP(hipMalloc((void**)&c->d_W_f, sizeof(float) * (size_t)nbeta * c->max_bands));
+  // the FP32 filter's projector apply on split BF16 products: betaw split once
+  PW_TRY(split_a_operand(c, c->d_betaw_f, (int)m, (int)m, nbeta, &c->d_betaw_bf));
+  PW_HIP(hipMalloc((void**)&c->d_W_bf, split_b_bytes(c->max_bands, nbeta)));
+  PW_HIP(hipStreamSynchronize(c->stream));
   return PW_OK;
 }
 
@@ -765,7 +790,7 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
                      c->d_pt_sz, tw3);
   PW_HIP(hipGetLastError());
   PW_TRY(mark(c, 1));
-  slab_reg_fn<T> kreg = pl.reg ? slab_reg_kernel<T>(c->n1, c->n2) : nullptr;
+  slab_reg_fn<T> kreg = pl.reg ? slab_reg_kernel<T>(c->n1, c->n2, pl.ly) : nullptr;
   if (kreg) {
     // one z plane of SB bands per workgroup, V plane + y-space in LDS
     constexpr int SB = SlabCfg<T>::SB;
@@ -797,9 +822,9 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
   return PW_OK;
 }
 
-bool has_reg_slab(const pw_ctx* c, int dtype) {
-  return dtype == PW_F64 ? slab_reg_kernel<double>(c->n1, c->n2) != nullptr
-                         : slab_reg_kernel<float>(c->n1, c->n2) != nullptr;
+bool has_reg_slab(const pw_ctx* c, int dtype, int ly) {
+  return dtype == PW_F64 ? slab_reg_kernel<double>(c->n1, c->n2, ly) != nullptr
+                         : slab_reg_kernel<float>(c->n1, c->n2, ly) != nullptr;
 }
 size_t slab_lds_bytes(const pw_ctx* c, int esz, int rp, int cg) { return slab_lds(c, esz, rp, cg); }
 
@@ -858,6 +883,12 @@ static int nonlocal(pw_ctx* c, int nb, const void* X, void* Y, bool accumulate) 
   return PW_OK;
 }
 
+static bool aligned8(const void* p) { return ((uintptr_t)p & 7) == 0; }
+static int finish_step(pw_ctx* c, int rc) {
+  if (rc != PW_OK) return rc;
+  return mark(c, 4);
+}
+
 // One scaled Chebyshev step around H: Ynew = s1 (H Y - c Y) - s2 Yp with
 // (c, s1, s2) = coef[3 step ..] on the device (Yp == null: s2 = 0).  The
 // local, kinetic and exchange terms go into HY; the projector product's
@@ -875,6 +906,13 @@ static int cheb_step(pw_ctx* c, int nb, const void* Y, void* HY, const void* Yp,
     T* W = (T*)(sizeof(T) == 8 ? (void*)c->d_W : (void*)c->d_W_f);
     const int split = c->bf16_split;
     PW_TRY(gemm_tn<T>(c, m, c->nbeta, nb, beta, m, (const T*)Y, m, W, c->nbeta, nullptr, (T)0, split));
+    if constexpr (sizeof(T) == 4) {
+      if ((split == 2 || split == 3) && c->d_betaw_bf && aligned8(HY) && aligned8(Y) && aligned8(Yp) &&
+          aligned8(Ynew))
+        return finish_step(c, gemm_nn_pre(c, m, nb, c->nbeta, c->d_betaw_bf, W, c->nbeta, c->d_W_bf,
+                                          (const float*)HY, (const float*)Y, (const float*)Yp, (float*)Ynew, m,
+                                          coef, step, true, split));
+    }
     PW_TRY(gemm_nn_ep<T>(c, m, nb, c->nbeta, betaw, m, W, c->nbeta, (const T*)HY, (const T*)Y, (const T*)Yp,
                          (T*)Ynew, m, coef, step, true, split));
   } else {

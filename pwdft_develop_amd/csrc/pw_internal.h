@@ -148,6 +148,16 @@ int gemm_nn(pw_ctx* c, int m, int n, int k, T alpha, const T* A, int lda, const 
 template <typename T>
 int gemm_nn_ep(pw_ctx* c, int m, int n, int k, const T* A, int lda, const T* B, int ldb, const T* H, const T* Yc,
                const T* Yp, T* out, int ld, const T* coef, int step, bool cheb, int split = 0);
+// The FP32 projector apply on products split ahead of time (pw_gemm.hip):
+// split_a_operand allocates *out and splits A (m x k) into BF16 terms in the
+// operand-tile layout; gemm_nn_pre splits B (k x n) into Bws
+// (split_b_bytes(n, k) bytes) and forms out = epilogue(A B) like gemm_nn_ep
+// with split (2 or 3) terms.
+int split_a_operand(pw_ctx* c, const float* A, int lda, int m, int k, unsigned short** out);
+size_t split_b_bytes(int n, int k);
+int gemm_nn_pre(pw_ctx* c, int m, int n, int k, const unsigned short* Ab, const float* B, int ldb,
+                unsigned short* Bws, const float* H, const float* Yc, const float* Yp, float* out, int ld,
+                const float* coef, int step, bool cheb, int split);
 // Dense symmetric eigensolvers (pw_eig.hip, pw_solver.hip), in place on the
 // upper triangle of A (A <- eigenvectors, w (device) <- eigenvalues
 // ascending); sym_eig adds the sign convention of the oracle and takes
@@ -171,9 +181,10 @@ int sphere_to_sticks(pw_ctx* c, int nb, const void* psi, void* t1);
 template <typename T>
 int sticks_to_sphere_add(pw_ctx* c, int nb, const void* t2, void* out);
 // Slab-kernel facts for the autotuner (pw_hpsi.hip): whether the
-// register-FFT kernel exists for the context's planes in dtype, and the LDS
-// bytes of the LDS kernel at (rp, cg) for complex elements of esz bytes.
-bool has_reg_slab(const pw_ctx* c, int dtype);
+// register-FFT kernel exists for the context's planes in dtype with ly
+// lanes per y column transform (ly <= 0: the default), and the LDS bytes of
+// the LDS kernel at (rp, cg) for complex elements of esz bytes.
+bool has_reg_slab(const pw_ctx* c, int dtype, int ly = 0);
 size_t slab_lds_bytes(const pw_ctx* c, int esz, int rp, int cg);
 // One Chebyshev step Ynew = s1 (H Y - c Y) - s2 Yp (pw_hpsi.hip), the
 // step's (c, s1, s2) read from the context's device coefficient table
@@ -222,6 +233,9 @@ struct pw_ctx {
   int nbeta = 0;
   double *d_beta = nullptr, *d_betaw = nullptr, *d_W = nullptr;
   float *d_beta_f = nullptr, *d_betaw_f = nullptr, *d_W_f = nullptr;
+  // betaw (FP32) split into BF16 terms ahead of time, and the per-call split
+  // of W (max_bands columns), for the projector apply on split products
+  unsigned short *d_betaw_bf = nullptr, *d_W_bf = nullptr;
   void* d_t1 = nullptr;  // stick buffers: max_bands x n3 x nstick complex double
   void* d_t2 = nullptr;
   size_t tbytes = 0;
@@ -257,6 +271,7 @@ struct pw_ctx {
   //   partial workgroups aimed at (0 = the built-in rule)
   struct Plan {
     bool reg = true;
+    int ly = 0;  // register kernel: lanes per y column transform (0: default)
     int cg = 0, rp = 0, threads = 0;
     int tn_tm = 0, tn_cap = 0;
   } plan[2];

@@ -11,7 +11,10 @@
 // thread holds N1 complex values; its LDS reads are issued in chunks fenced
 // by scheduling barriers so that the compiler does not hoist all of them next
 // to the spectrum (that spills past the 256 VGPRs of two waves per SIMD).
-// Same arithmetic, in the same order, as k_slab_xy.  t1 may equal t2 (each
+// LY lanes share each y column's transform (col_fft_lanes): LY = 1 keeps
+// k_slab_xy's arithmetic and order; LY = 2 or 4 splits every y transform
+// into LY shorter ones combined across lanes, so that SB nk columns occupy
+// LY SB nk threads.  t1 may equal t2 (each
 // plane slice is read whole before it is written).  LDS (smem_raw, 16-byte
 // aligned): SB nk N2 complex + N2/2 (N1 + 1) complex + 2 nstick ints.
 #pragma once
@@ -19,7 +22,7 @@
 
 namespace pw {
 
-template <typename T, int N1, int N2, int SB>
+template <typename T, int N1, int N2, int SB, int LY = 1>
 __device__ __forceinline__ void slab_xy_reg_body(int z, int b0, const typename cx<T>::t* t1, typename cx<T>::t* t2,
                                                  int n3, int nstick, int kxmax, int nb, const int* __restrict__ pos,
                                                  const int* __restrict__ mpos, const T* __restrict__ vtot,
@@ -94,15 +97,7 @@ __device__ __forceinline__ void slab_xy_reg_body(int z, int b0, const typename c
   }
   __syncthreads();
   // inverse y on the kx <= kxmax columns
-  for (int c = t; c < ns * nk; c += blockDim.x) {
-    C* col = Y + (size_t)c * N2;
-    C a[N2];
-#pragma unroll
-    for (int i = 0; i < N2; i++) a[i] = col[i];
-    reg_fft<T, N2, 1>(a);
-#pragma unroll
-    for (int i = 0; i < N2; i++) col[i] = a[i];
-  }
+  for (int c = t / LY; c < ns * nk; c += blockDim.x / LY) col_fft_lanes<T, N2, 1, LY>(Y + (size_t)c * N2, t % LY);
   __syncthreads();
   // per row pair: Hermitian-extended spectra packed as X1 + i X2, c2r, x V/N,
   // r2c, unpack kx <= kxmax; a thread reads and writes only its two rows,
@@ -163,15 +158,7 @@ __device__ __forceinline__ void slab_xy_reg_body(int z, int b0, const typename c
   }
   __syncthreads();
   // forward y
-  for (int c = t; c < ns * nk; c += blockDim.x) {
-    C* col = Y + (size_t)c * N2;
-    C a[N2];
-#pragma unroll
-    for (int i = 0; i < N2; i++) a[i] = col[i];
-    reg_fft<T, N2, -1>(a);
-#pragma unroll
-    for (int i = 0; i < N2; i++) col[i] = a[i];
-  }
+  for (int c = t / LY; c < ns * nk; c += blockDim.x / LY) col_fft_lanes<T, N2, -1, LY>(Y + (size_t)c * N2, t % LY);
   __syncthreads();
   for (int e = t; e < tot; e += blockDim.x) {
     const int s = e / nstick, st = e - s * nstick;

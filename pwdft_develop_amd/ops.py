@@ -202,6 +202,28 @@ class Context:
                                         np.ascontiguousarray(f, np.float64), C.byref(e)))
         return e.value
 
+    HARRIS_TERMS = ("Exc", "Eself", "EIonSR", "EVdw", "Eext", "fermi", "Tbeta", "temperature")
+
+    def harris_energy(self, density, vxc, vhart, pseudo, vol, eig, occ, num_spin, num_occupied, terms,
+                      spindensity=None):
+        """pw_harris_energy (SCF::CalculateHarrisEnergy, scf.cpp:1987-2140).
+        density, vxc: device float64 (ncomp, ntot) tensors (DblNumMat(ntot,
+        ncomp) storage); vhart, pseudo: (ntot,); spindensity: (2, ntot) for
+        nspin = 2; eig, occ: host arrays; terms: dict of HARRIS_TERMS."""
+        nspin, ntot = vxc.shape
+        for a in (density, vxc, vhart, pseudo) + ((spindensity,) if spindensity is not None else ()):
+            if a.dtype != self.torch.float64 or not a.is_contiguous() or a.device.type != "cuda":
+                raise ValueError("harris_energy: device float64 contiguous arrays expected")
+        t = np.array([float(terms[k]) for k in self.HARRIS_TERMS])
+        e = C.c_double()
+        check(_lib.lib().pw_harris_energy(
+            self.h, ntot, nspin, vol, C.c_void_p(density.data_ptr()),
+            C.c_void_p(spindensity.data_ptr()) if spindensity is not None else None,
+            C.c_void_p(vxc.data_ptr()), C.c_void_p(vhart.data_ptr()), C.c_void_p(pseudo.data_ptr()), len(eig),
+            np.ascontiguousarray(eig, np.float64), np.ascontiguousarray(occ, np.float64), int(num_spin),
+            int(num_occupied), t, C.byref(e)))
+        return e.value
+
     def nonlocal_force(self, x, f, natom, npa):
         F = np.zeros(3 * natom)
         check(_lib.lib().pw_nonlocal_force(self.h, _dtype_code(x), x.shape[0], C.c_void_p(x.data_ptr()),
@@ -303,6 +325,16 @@ class Batch:
 
     def set_graphs(self, on=True):
         check(_lib.lib().pw_batch_set_graphs(self.h, 1 if on else 0))
+
+    def set_timing(self, on=True):
+        check(_lib.lib().pw_batch_set_timing(self.h, 1 if on else 0))
+
+    def timing(self):
+        """ms per stage (z-inverse + projections, slabs, z-forward, apply) of
+        the last launch, summed over its steps."""
+        ms = np.zeros(4)
+        check(_lib.lib().pw_batch_timing(self.h, ms))
+        return ms
 
     def autotune(self, cache=None):
         """pw_batch_autotune: time the work splits, keep the fastest."""

@@ -167,3 +167,31 @@ def test_cheb_filter_fp32_projector_products(si8, split, tol, monkeypatch):
     print(f"fp32 filter (split {split}) relative error {err:.2e}")
     assert err < tol
     ctx.close()
+
+
+@pytest.mark.parametrize("name,nb", [("si8_32", 13), ("rand16_48", 70), ("rand16_48", 128)])
+def test_cheb_filter_presplit_products_ragged(name, nb, monkeypatch):
+    """The FP32 filter's projector apply on operands split ahead of time
+    (64-row x 256-column workgroups, partial row and column tiles: 13 and
+    70 bands, 2 npw not a multiple of 64) against the same filter on FP32
+    MFMA products (PW_BF16=0): 3 split terms carry FP32's 24 bits, so the two
+    agree to a few FP32 roundings; and against the FP64 oracle filter."""
+    import torch
+    from pwdft_develop_amd.ops import Context
+    prob = W.WORKLOADS[name](nb=nb).build()
+    print(f"case: {name} presplit filter nb={nb}", flush=True)
+    out = {}
+    for split in ("3", "0"):
+        monkeypatch.setenv("PW_BF16", split)
+        ctx = Context(prob)
+        X = ctx.to_device(prob.psi, torch.complex64)
+        ctx.cheb_filter(X, 10, 0.3, 12.0, -1.0)
+        out[split] = X.cpu().numpy()
+        ctx.close()
+    ref = O.Hamiltonian(prob).cheb_filter(prob.psi, 10, 0.3, 12.0, -1.0)
+    scale = np.abs(ref).max()
+    d = float(np.abs(out["3"] - out["0"]).max() / scale)
+    e = float(np.abs(out["3"] - ref).max() / scale)
+    print(f"split 3 vs FP32 MFMA {d:.2e}, vs oracle {e:.2e}")
+    assert d < 5e-6
+    assert e < 2e-5
