@@ -33,6 +33,7 @@ filter run cell by cell.
 """
 import argparse
 import json
+import re
 import os
 import sys
 import time
@@ -500,10 +501,15 @@ def main():
     # FP32 square grids of the register-FFT sizes take k_slab_xy_reg (8 bands
     # per z plane), others the LDS kernel k_slab_xy
     reg = prob.grid[0] == prob.grid[1] and prob.grid[0] in (24, 30, 32, 36, 40, 48, 60, 64, 72)
-    kname = (f"k_slab_xy_reg<float, {prob.grid[0]}, {prob.grid[1]}," if reg
+    # the autotuner's FP32 choice names the variant ("slab reg <lanes per y
+    # transform>"): k_slab_xy_reg<float, n, n, 8 bands, 2 waves/SIMD, lanes>
+    mly = re.search(r"slab reg (\d)", out["autotune"]["f32"])
+    reg = reg and (mly is not None or "slab lds" not in out["autotune"]["f32"])
+    ly = mly.group(1) if mly else "1"
+    kname = (f"k_slab_xy_reg<float, {prob.grid[0]}, {prob.grid[1]}, 8, 2, {ly}>" if reg
              else f"k_slab_xy<float, {prob.grid[0]}, {prob.grid[1]}>")
     traffic, full = pmc_traffic(os.path.join(ROOT, "profiles", f"pmc_traffic_{prob.name}.md"), kname)
-    kname = full or (kname + " ...>" if reg else kname)
+    kname = full or kname
 
     def staged(tag, p):
         local_ms = sum(stages[tag][:3])

@@ -101,20 +101,34 @@ __device__ void bt_invz_job(const BatchCell& c, const float2* __restrict__ src, 
   const float2* ps = src + (size_t)b * c.npw;
   for (int t = threadIdx.x; t < P * n3; t += blockDim.x) A[t] = float2{0.f, 0.f};
   __syncthreads();
-  for (int j = 0; j < nch; j++) {
-    const int e = c.chunk_off[ch0 + j + 1];
-    for (int t = c.chunk_off[ch0 + j] + threadIdx.x; t < e; t += blockDim.x) {
-      const int i = c.pt_idx[t];
-      const int sz = c.pt_sz[t];
-      float2 v;
-      if (i >= 0) {
-        v = ps[i];
-        if (i != 0) v = cscale(v, is2);
-      } else {
-        const float2 u = ps[-i - 1];
-        v = float2{u.x * is2, -u.y * is2};
+  // the chunks' points are contiguous in pt_idx / pt_sz: gather them in
+  // batches of 8 per thread, every index load and then every coefficient
+  // load in flight before the LDS stores
+  {
+    const int t0 = c.chunk_off[ch0], t1 = c.chunk_off[ch0 + nch];
+    for (int tb = t0 + threadIdx.x; tb < t1; tb += 8 * blockDim.x) {
+      int ii[8], zz[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int t = tb + u * blockDim.x;
+        ii[u] = t < t1 ? c.pt_idx[t] : 0;
+        zz[u] = t < t1 ? c.pt_sz[t] : -1;
       }
-      A[(j * sc + (sz & 0xffff)) * n3 + (sz >> 16)] = v;
+      float2 vv[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) vv[u] = ps[ii[u] >= 0 ? ii[u] : -ii[u] - 1];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int t = tb + u * blockDim.x;
+        if (t >= t1) continue;
+        // chunk j of the group: the first j with chunk_off[ch0 + j + 1] > t
+        int j = 0;
+        while (c.chunk_off[ch0 + j + 1] <= t) j++;
+        float2 v = vv[u];
+        if (ii[u] < 0) v = float2{v.x * is2, -v.y * is2};
+        else if (ii[u] != 0) v = cscale(v, is2);
+        A[(j * sc + (zz[u] & 0xffff)) * n3 + (zz[u] >> 16)] = v;
+      }
     }
   }
   __syncthreads();
@@ -139,24 +153,52 @@ __device__ void bt_fwdz_job(const BatchCell& c, const float2* __restrict__ src, 
   const int ch0 = g * c.zc, nch = min(c.zc, c.nchunk - ch0);
   const int P = nch * sc, s0 = ch0 * sc, nsl = min(P, nstick - s0);
   const float2* T = c.T;
-  for (int t = threadIdx.x; t < n3 * P; t += blockDim.x) {
-    const int z = t / P, sl = t - z * P;
-    A[sl * n3 + z] = (sl < nsl) ? T[((size_t)b * n3 + z) * nstick + s0 + sl] : float2{0.f, 0.f};
+  for (int tb = threadIdx.x; tb < n3 * P; tb += 8 * blockDim.x) {
+    float2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int t = tb + u * blockDim.x, z = t / P, sl = t - z * P;
+      v[u] = (t < n3 * P && sl < nsl) ? T[((size_t)b * n3 + z) * nstick + s0 + sl] : float2{0.f, 0.f};
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int t = tb + u * blockDim.x, z = t / P, sl = t - z * P;
+      if (t < n3 * P) A[sl * n3 + z] = v[u];
+    }
   }
   __syncthreads();
   const float2* R = lds_fft_n<float, N3>(A, Bz, n3, P, n3, tw3, -1.f);
   const float2* ps = src + (size_t)b * c.npw;
   float2* o = HY + (size_t)b * c.npw;
-  for (int j = 0; j < nch; j++) {
-    const int e = c.gchunk_off[ch0 + j + 1];
-    for (int t = c.gchunk_off[ch0 + j] + threadIdx.x; t < e; t += blockDim.x) {
-      const int i = c.gpt_idx[t];
-      const int sz = c.gpt_sz[t];
-      float2 y = R[(j * sc + (sz & 0xffff)) * n3 + (sz >> 16)];
-      if (i != 0) y = cscale(y, s2);
-      const float2 cc = ps[i];
-      const float ek = c.ekin[i];
-      o[i] = float2{ek * cc.x + y.x, ek * cc.y + y.y};
+  // batches of 8 points per thread: index loads, then coefficient and
+  // ekin loads in flight together, then the sums
+  {
+    const int t0 = c.gchunk_off[ch0], t1 = c.gchunk_off[ch0 + nch];
+    for (int tb = t0 + threadIdx.x; tb < t1; tb += 8 * blockDim.x) {
+      int ii[8], zz[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int t = tb + u * blockDim.x;
+        ii[u] = t < t1 ? c.gpt_idx[t] : 0;
+        zz[u] = t < t1 ? c.gpt_sz[t] : 0;
+      }
+      float2 cc[8];
+      float ek[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        cc[u] = ps[ii[u]];
+        ek[u] = c.ekin[ii[u]];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int t = tb + u * blockDim.x;
+        if (t >= t1) continue;
+        int j = 0;
+        while (c.gchunk_off[ch0 + j + 1] <= t) j++;
+        float2 y = R[(j * sc + (zz[u] & 0xffff)) * n3 + (zz[u] >> 16)];
+        if (ii[u] != 0) y = cscale(y, s2);
+        o[ii[u]] = float2{ek[u] * cc[u].x + y.x, ek[u] * cc[u].y + y.y};
+      }
     }
   }
   __syncthreads();
