@@ -86,6 +86,9 @@ struct SlabGeom {
   int n1, n2, n3, nstick, kxmax, rp, cg;
 };
 
+#define PW_EXX_PLANES(X) X(24) X(30) X(32) X(36) X(40) X(48) X(60) X(64) X(72) X(96)
+#define PW_EXX_Z(X) X(24) X(30) X(32) X(36) X(40) X(48) X(60) X(72) X(96)
+
 template <typename T>
 __device__ void load_tw(typename cx<T>::t* tw1, typename cx<T>::t* tw2, const typename cx<T>::t* g1,
                         const typename cx<T>::t* g2, int n1, int n2) {
@@ -94,7 +97,7 @@ __device__ void load_tw(typename cx<T>::t* tw1, typename cx<T>::t* tw2, const ty
 }
 
 // sticks (with optional conj mirrors) -> y IFFT; returns the y-space buffer
-template <typename T>
+template <typename T, int N2 = 0>
 __device__ typename cx<T>::t* sticks_to_yspace(const typename cx<T>::t* src, const int* pos, const int* mpos,
                                                const SlabGeom& g, typename cx<T>::t* Y0, typename cx<T>::t* Y1,
                                                typename cx<T>::t* S, const typename cx<T>::t* tw2) {
@@ -111,7 +114,7 @@ __device__ typename cx<T>::t* sticks_to_yspace(const typename cx<T>::t* src, con
     }
   }
   __syncthreads();
-  return slab_y_fft<T, 0>(Y0, Y1, S, g.n2, nk, g.cg, tw2, (T)1);
+  return slab_y_fft<T, N2>(Y0, Y1, S, g.n2, nk, g.cg, tw2, (T)1);
 }
 
 #define SLAB_LDS_SETUP                                      \
@@ -124,10 +127,12 @@ __device__ typename cx<T>::t* sticks_to_yspace(const typename cx<T>::t* src, con
   C* Y1 = g.cg == 0 ? Y0 + nk * g.n2 : nullptr;            \
   C* Z0 = g.cg == 0 ? Y1 + nk * g.n2 : Y0 + nk * g.n2;      \
   C* Z1 = Z0 + g.rp * g.n1;                                 \
-  (void)Z1;
+  const int n1 = N1 > 0 ? N1 : g.n1;                        \
+  (void)Z1;                                                 \
+  (void)n1;
 
 // wavefunction sticks -> real grid, scaled
-template <typename T>
+template <typename T, int N1 = 0, int N2 = 0>
 __global__ void __launch_bounds__(1024)
 k_exx_slab_inv(const typename cx<T>::t* __restrict__ st, T* __restrict__ out, const int* __restrict__ pos,
                const int* __restrict__ mpos, SlabGeom g, T scale, const typename cx<T>::t* tw1g,
@@ -135,16 +140,16 @@ k_exx_slab_inv(const typename cx<T>::t* __restrict__ st, T* __restrict__ out, co
   SLAB_LDS_SETUP
   const int z = blockIdx.x, b = blockIdx.y;
   load_tw<T>(tw1, tw2, tw1g, tw2g, g.n1, g.n2);
-  C* Y = sticks_to_yspace<T>(st + ((size_t)b * g.n3 + z) * g.nstick, pos, mpos, g, Y0, Y1, Z0, tw2);
+  C* Y = sticks_to_yspace<T, N2>(st + ((size_t)b * g.n3 + z) * g.nstick, pos, mpos, g, Y0, Y1, Z0, tw2);
   T* o = out + ((size_t)b * g.n3 + z) * g.n1 * g.n2;
   const int npair = (g.n2 + 1) / 2;
   for (int q0 = 0; q0 < npair; q0 += g.rp) {
     const int nq = min(g.rp, npair - q0);
-    C* Zr = x_c2r_rows<T>(Y, g.kxmax, g.n1, g.n2, q0, nq, Z0, Z1, tw1);
-    for (int t = threadIdx.x; t < nq * g.n1; t += blockDim.x) {
-      const int ql = t / g.n1, x = t - ql * g.n1;
+    C* Zr = x_c2r_rows<T, N1>(Y, g.kxmax, g.n1, g.n2, q0, nq, Z0, Z1, tw1);
+    for (int t = threadIdx.x; t < nq * n1; t += blockDim.x) {
+      const int ql = t / n1, x = t - ql * n1;
       const int y1 = 2 * (q0 + ql), y2 = y1 + 1;
-      const C u = Zr[ql * g.n1 + x];
+      const C u = Zr[ql * n1 + x];
       o[(size_t)y1 * g.n1 + x] = u.x * scale;
       if (y2 < g.n2) o[(size_t)y2 * g.n1 + x] = u.y * scale;
     }
@@ -153,7 +158,7 @@ k_exx_slab_inv(const typename cx<T>::t* __restrict__ st, T* __restrict__ out, co
 }
 
 // pair densities psi_k phi_j -> Fock sticks; pair p = k + nb * jl
-template <typename T>
+template <typename T, int N1 = 0, int N2 = 0>
 __global__ void __launch_bounds__(1024)
 k_exx_pair_fwd(const T* __restrict__ psir, const T* __restrict__ phir, int nb, int j0,
                typename cx<T>::t* __restrict__ fst, const int* __restrict__ pos, SlabGeom g,
@@ -169,23 +174,23 @@ k_exx_pair_fwd(const T* __restrict__ psir, const T* __restrict__ phir, int nb, i
   const int npair = (g.n2 + 1) / 2;
   for (int q0 = 0; q0 < npair; q0 += g.rp) {
     const int nq = min(g.rp, npair - q0);
-    for (int t = threadIdx.x; t < nq * g.n1; t += blockDim.x) {
-      const int ql = t / g.n1, x = t - ql * g.n1;
+    for (int t = threadIdx.x; t < nq * n1; t += blockDim.x) {
+      const int ql = t / n1, x = t - ql * n1;
       const int y1 = 2 * (q0 + ql), y2 = y1 + 1;
-      const T u1 = a[(size_t)y1 * g.n1 + x] * bph[(size_t)y1 * g.n1 + x];
-      const T u2 = (y2 < g.n2) ? a[(size_t)y2 * g.n1 + x] * bph[(size_t)y2 * g.n1 + x] : (T)0;
-      Z0[ql * g.n1 + x] = C{u1, u2};
+      const T u1 = a[(size_t)y1 * n1 + x] * bph[(size_t)y1 * n1 + x];
+      const T u2 = (y2 < g.n2) ? a[(size_t)y2 * n1 + x] * bph[(size_t)y2 * n1 + x] : (T)0;
+      Z0[ql * n1 + x] = C{u1, u2};
     }
     __syncthreads();
-    x_r2c_rows<T>(Z0, Z1, Y0, g.kxmax, g.n1, g.n2, q0, nq, tw1);
+    x_r2c_rows<T, N1>(Z0, Z1, Y0, g.kxmax, g.n1, g.n2, q0, nq, tw1);
   }
-  C* Yf = slab_y_fft<T, 0>(Y0, Y1, Z0, g.n2, nk, g.cg, tw2, (T)-1);
+  C* Yf = slab_y_fft<T, N2>(Y0, Y1, Z0, g.n2, nk, g.cg, tw2, (T)-1);
   C* dst = fst + ((size_t)p * g.n3 + z) * g.nstick;
   for (int s = threadIdx.x; s < g.nstick; s += blockDim.x) dst[s] = Yf[pos[s]];
 }
 
 // Fock sticks: forward z, x K/N, inverse z (in place); grid (chunk, pair)
-template <typename T>
+template <typename T, int N3 = 0>
 __global__ void __launch_bounds__(256)
 k_exx_fock_z(typename cx<T>::t* __restrict__ fst, const T* __restrict__ K, int n3, int nstick, int sc,
              const typename cx<T>::t* __restrict__ tw3g) {
@@ -204,14 +209,14 @@ k_exx_fock_z(typename cx<T>::t* __restrict__ fst, const T* __restrict__ K, int n
     A[sl * n3 + z] = (sl < nsl) ? base[(size_t)z * nstick + s0 + sl] : C{0, 0};
   }
   __syncthreads();
-  C* R = lds_fft<T>(A, B, n3, sc, n3, tw, (T)-1);
+  C* R = lds_fft_n<T, N3>(A, B, n3, sc, n3, tw, (T)-1);
   for (int t = threadIdx.x; t < n3 * nsl; t += blockDim.x) {
     const int sl = t / n3, z = t - sl * n3;
     R[sl * n3 + z] = cscale(R[sl * n3 + z], K[(size_t)(s0 + sl) * n3 + z]);
   }
   __syncthreads();
   C* O = (R == A) ? B : A;
-  C* Q = lds_fft<T>(R, O, n3, sc, n3, tw, (T)1);
+  C* Q = lds_fft_n<T, N3>(R, O, n3, sc, n3, tw, (T)1);
   for (int t = threadIdx.x; t < n3 * nsl; t += blockDim.x) {
     const int z = t / nsl, sl = t - z * nsl;
     base[(size_t)z * nstick + s0 + sl] = Q[sl * n3 + z];
@@ -219,7 +224,7 @@ k_exx_fock_z(typename cx<T>::t* __restrict__ fst, const T* __restrict__ K, int n
 }
 
 // acc_k(r) += sum_jl fac_j phi_j(r) c2r(Fock sticks of pair (k, jl))(r)
-template <typename T>
+template <typename T, int N1 = 0, int N2 = 0>
 __global__ void __launch_bounds__(1024)
 k_exx_pair_acc(const typename cx<T>::t* __restrict__ fst, const T* __restrict__ phir, const T* __restrict__ fac,
                int nb, int j0, int nj, T* __restrict__ acc, const int* __restrict__ pos, SlabGeom g,
@@ -235,16 +240,16 @@ k_exx_pair_acc(const typename cx<T>::t* __restrict__ fst, const T* __restrict__ 
     const T fj = fac[j];
     const T* ph = phir + ((size_t)j * g.n3 + z) * plane;
     const int p = k + nb * jl;
-    C* Y = sticks_to_yspace<T>(fst + ((size_t)p * g.n3 + z) * g.nstick, pos, nullptr, g, Y0, Y1, Z0, tw2);
+    C* Y = sticks_to_yspace<T, N2>(fst + ((size_t)p * g.n3 + z) * g.nstick, pos, nullptr, g, Y0, Y1, Z0, tw2);
     for (int q0 = 0; q0 < npair; q0 += g.rp) {
       const int nq = min(g.rp, npair - q0);
-      C* Zr = x_c2r_rows<T>(Y, g.kxmax, g.n1, g.n2, q0, nq, Z0, Z1, tw1);
-      for (int t = threadIdx.x; t < nq * g.n1; t += blockDim.x) {
-        const int ql = t / g.n1, x = t - ql * g.n1;
+      C* Zr = x_c2r_rows<T, N1>(Y, g.kxmax, g.n1, g.n2, q0, nq, Z0, Z1, tw1);
+      for (int t = threadIdx.x; t < nq * n1; t += blockDim.x) {
+        const int ql = t / n1, x = t - ql * n1;
         const int y1 = 2 * (q0 + ql), y2 = y1 + 1;
-        const C u = Zr[ql * g.n1 + x];
-        ak[(size_t)y1 * g.n1 + x] += fj * ph[(size_t)y1 * g.n1 + x] * u.x;
-        if (y2 < g.n2) ak[(size_t)y2 * g.n1 + x] += fj * ph[(size_t)y2 * g.n1 + x] * u.y;
+        const C u = Zr[ql * n1 + x];
+        ak[(size_t)y1 * n1 + x] += fj * ph[(size_t)y1 * n1 + x] * u.x;
+        if (y2 < g.n2) ak[(size_t)y2 * n1 + x] += fj * ph[(size_t)y2 * n1 + x] * u.y;
       }
       __syncthreads();
     }
@@ -252,7 +257,7 @@ k_exx_pair_acc(const typename cx<T>::t* __restrict__ fst, const T* __restrict__ 
 }
 
 // real grid acc_k -> x r2c -> y FFT -> wavefunction sticks
-template <typename T>
+template <typename T, int N1 = 0, int N2 = 0>
 __global__ void __launch_bounds__(1024)
 k_exx_slab_fwd(const T* __restrict__ in, typename cx<T>::t* __restrict__ st, const int* __restrict__ pos,
                SlabGeom g, const typename cx<T>::t* tw1g, const typename cx<T>::t* tw2g) {
@@ -264,15 +269,15 @@ k_exx_slab_fwd(const T* __restrict__ in, typename cx<T>::t* __restrict__ st, con
   const int npair = (g.n2 + 1) / 2;
   for (int q0 = 0; q0 < npair; q0 += g.rp) {
     const int nq = min(g.rp, npair - q0);
-    for (int t = threadIdx.x; t < nq * g.n1; t += blockDim.x) {
-      const int ql = t / g.n1, x = t - ql * g.n1;
+    for (int t = threadIdx.x; t < nq * n1; t += blockDim.x) {
+      const int ql = t / n1, x = t - ql * n1;
       const int y1 = 2 * (q0 + ql), y2 = y1 + 1;
-      Z0[ql * g.n1 + x] = C{a[(size_t)y1 * g.n1 + x], (y2 < g.n2) ? a[(size_t)y2 * g.n1 + x] : (T)0};
+      Z0[ql * n1 + x] = C{a[(size_t)y1 * n1 + x], (y2 < g.n2) ? a[(size_t)y2 * n1 + x] : (T)0};
     }
     __syncthreads();
-    x_r2c_rows<T>(Z0, Z1, Y0, g.kxmax, g.n1, g.n2, q0, nq, tw1);
+    x_r2c_rows<T, N1>(Z0, Z1, Y0, g.kxmax, g.n1, g.n2, q0, nq, tw1);
   }
-  C* Yf = slab_y_fft<T, 0>(Y0, Y1, Z0, g.n2, nk, g.cg, tw2, (T)-1);
+  C* Yf = slab_y_fft<T, N2>(Y0, Y1, Z0, g.n2, nk, g.cg, tw2, (T)-1);
   C* dst = st + ((size_t)b * g.n3 + z) * g.nstick;
   for (int s = threadIdx.x; s < g.nstick; s += blockDim.x) dst[s] = Yf[pos[s]];
 }
@@ -384,6 +389,48 @@ static int set_lds(K kern, size_t bytes) {
 }
 
 // psi (sphere, dtype T) -> real grid (scale) for nb bands, via the wavefunction sticks
+// The exchange kernels with compile-time transform lengths for the square
+// planes (and z lengths) of the register-FFT size list, else the
+// runtime-length ones.
+template <typename T>
+struct ExxKernels {
+  decltype(&k_exx_slab_inv<T>) slab_inv;
+  decltype(&k_exx_pair_fwd<T>) pair_fwd;
+  decltype(&k_exx_pair_acc<T>) pair_acc;
+  decltype(&k_exx_slab_fwd<T>) slab_fwd;
+  decltype(&k_exx_fock_z<T>) fock_z;
+};
+template <typename T>
+static ExxKernels<T> exx_kernels(int n1, int n2, int n3) {
+  ExxKernels<T> k{&k_exx_slab_inv<T>, &k_exx_pair_fwd<T>, &k_exx_pair_acc<T>, &k_exx_slab_fwd<T>, &k_exx_fock_z<T>};
+  if (n1 == n2) {
+    switch (n1) {
+#define PW_CASE(n)                                                                                        \
+  case n:                                                                                                 \
+    k.slab_inv = &k_exx_slab_inv<T, n, n>;                                                                \
+    k.pair_fwd = &k_exx_pair_fwd<T, n, n>;                                                                \
+    k.pair_acc = &k_exx_pair_acc<T, n, n>;                                                                \
+    k.slab_fwd = &k_exx_slab_fwd<T, n, n>;                                                                \
+    break;
+      PW_EXX_PLANES(PW_CASE)
+#undef PW_CASE
+      default:
+        break;
+    }
+  }
+  switch (n3) {
+#define PW_CASE(n)                    \
+  case n:                             \
+    k.fock_z = &k_exx_fock_z<T, n>;   \
+    break;
+    PW_EXX_Z(PW_CASE)
+#undef PW_CASE
+    default:
+      break;
+  }
+  return k;
+}
+
 template <typename T>
 static int sphere_to_real(pw_ctx* c, int nb, const void* psi, T* out, T scale) {
   typedef typename cx<T>::t C;
@@ -392,8 +439,9 @@ static int sphere_to_real(pw_ctx* c, int nb, const void* psi, T* out, T scale) {
   SlabGeom g{c->n1, c->n2, c->n3, c->sp.nstick, c->sp.kxmax, 0, 0};
   const int thr = pick_wfn_layout(c, sizeof(C), g);
   const size_t lds = slab_bytes(c, sizeof(C), g.kxmax, g.rp, g.cg);
-  PW_TRY(set_lds(k_exx_slab_inv<T>, lds));
-  hipLaunchKernelGGL(k_exx_slab_inv<T>, dim3(c->n3, nb), dim3(thr), lds, c->stream, (const C*)c->d_t1, out,
+  const auto kinv = exx_kernels<T>(c->n1, c->n2, c->n3).slab_inv;
+  PW_TRY(set_lds(kinv, lds));
+  hipLaunchKernelGGL(kinv, dim3(c->n3, nb), dim3(thr), lds, c->stream, (const C*)c->d_t1, out,
                      c->d_pos, c->d_mpos, g, scale, tw, tw + c->n1);
   PW_HIP(hipGetLastError());
   return PW_OK;
@@ -419,27 +467,28 @@ int exx_add(pw_ctx* c, int nb, const void* psi, void* a3) {
   SlabGeom gf{c->n1, c->n2, c->n3, E->nstick, E->kxmax, 0, 0};
   const int thr = pick_pair_layout(c, sizeof(C), gf);
   const size_t ldsf = slab_bytes(c, sizeof(C), gf.kxmax, gf.rp, gf.cg);
-  PW_TRY(set_lds(k_exx_pair_fwd<T>, ldsf));
-  PW_TRY(set_lds(k_exx_pair_acc<T>, ldsf));
+  const ExxKernels<T> kx = exx_kernels<T>(c->n1, c->n2, c->n3);
+  PW_TRY(set_lds(kx.pair_fwd, ldsf));
+  PW_TRY(set_lds(kx.pair_acc, ldsf));
   const int J = std::max(1, std::min(E->nocc, E->npair_max / nb));
   const size_t ldsz = sizeof(C) * (c->n3 + 2 * (size_t)E->sc * c->n3);
   for (int j0 = 0; j0 < E->nocc; j0 += J) {
     const int nj = std::min(J, E->nocc - j0);
-    hipLaunchKernelGGL(k_exx_pair_fwd<T>, dim3(c->n3, nb * nj), dim3(thr), ldsf, c->stream, (const T*)psir, phir,
+    hipLaunchKernelGGL(kx.pair_fwd, dim3(c->n3, nb * nj), dim3(thr), ldsf, c->stream, (const T*)psir, phir,
                        nb, j0, (C*)E->d_fst, E->d_pos, gf, tw, tw + c->n1);
     PW_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_exx_fock_z<T>, dim3(E->nchunk, nb * nj), dim3(256), ldsz, c->stream, (C*)E->d_fst, K,
+    hipLaunchKernelGGL(kx.fock_z, dim3(E->nchunk, nb * nj), dim3(256), ldsz, c->stream, (C*)E->d_fst, K,
                        c->n3, E->nstick, E->sc, tw + c->n1 + c->n2);
     PW_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_exx_pair_acc<T>, dim3(c->n3, nb), dim3(thr), ldsf, c->stream, (const C*)E->d_fst, phir,
+    hipLaunchKernelGGL(kx.pair_acc, dim3(c->n3, nb), dim3(thr), ldsf, c->stream, (const C*)E->d_fst, phir,
                        fac, nb, j0, nj, acc, E->d_pos, gf, tw, tw + c->n1);
     PW_HIP(hipGetLastError());
   }
   SlabGeom g{c->n1, c->n2, c->n3, c->sp.nstick, c->sp.kxmax, 0, 0};
   const int thw = pick_wfn_layout(c, sizeof(C), g);
   const size_t lds = slab_bytes(c, sizeof(C), g.kxmax, g.rp, g.cg);
-  PW_TRY(set_lds(k_exx_slab_fwd<T>, lds));
-  hipLaunchKernelGGL(k_exx_slab_fwd<T>, dim3(c->n3, nb), dim3(thw), lds, c->stream, (const T*)acc, (C*)c->d_t2,
+  PW_TRY(set_lds(kx.slab_fwd, lds));
+  hipLaunchKernelGGL(kx.slab_fwd, dim3(c->n3, nb), dim3(thw), lds, c->stream, (const T*)acc, (C*)c->d_t2,
                      c->d_pos, g, tw, tw + c->n1);
   PW_HIP(hipGetLastError());
   return sticks_to_sphere_add<T>(c, nb, c->d_t2, a3);
@@ -455,7 +504,7 @@ template int exx_add<float>(pw_ctx*, int, const void*, void*);
 // recomputed from their sticks chunk by chunk; real-space psi is never
 // stored.  Block (z, split) sums the bands b = split, split + nsplit, ...
 // of the chunk into its own partial plane; partials are reduced in order.
-template <typename T>
+template <typename T, int N1 = 0>
 __global__ void __launch_bounds__(1024)
 k_density_acc(const typename cx<T>::t* __restrict__ st, const double* __restrict__ f, int b0, int nbc, int nsplit,
               double* __restrict__ part, const int* __restrict__ pos, const int* __restrict__ mpos, SlabGeom g,

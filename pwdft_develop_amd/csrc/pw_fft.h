@@ -186,14 +186,16 @@ __device__ __forceinline__ typename cx<T>::t* slab_y_fft(typename cx<T>::t* Y, t
   return Y;
 }
 
-// x-pass helpers of the slab transforms (runtime lengths)
+// x-pass helpers of the slab transforms (N1 > 0: compile-time row length
+// N1 = n1; N1 = 0: runtime)
 // Rows y1 = 2q, y2 = 2q+1 (q in [q0, q0+nq)) of the y-transformed spectrum
 // Y[kx * n2 + y] (kx <= kxmax) -> real rows packed as Z = u1 + i u2.
-template <typename T>
-__device__ typename cx<T>::t* x_c2r_rows(const typename cx<T>::t* Y, int kxmax, int n1, int n2, int q0, int nq,
+template <typename T, int N1 = 0>
+__device__ typename cx<T>::t* x_c2r_rows(const typename cx<T>::t* Y, int kxmax, int n1_, int n2, int q0, int nq,
                                          typename cx<T>::t* Z0, typename cx<T>::t* Z1,
                                          const typename cx<T>::t* tw1) {
   typedef typename cx<T>::t C;
+  const int n1 = N1 > 0 ? N1 : n1_;
   const int h = n1 / 2;
   for (int t = threadIdx.x; t < nq * n1; t += blockDim.x) {
     const int ql = t / n1, k = t - ql * n1;
@@ -211,17 +213,18 @@ __device__ typename cx<T>::t* x_c2r_rows(const typename cx<T>::t* Y, int kxmax, 
     Z0[ql * n1 + k] = C{a1.x - a2.y, a1.y + a2.x};
   }
   __syncthreads();
-  return lds_fft<T>(Z0, Z1, n1, nq, n1, tw1, (T)1);
+  return lds_fft_n<T, N1>(Z0, Z1, n1, nq, n1, tw1, (T)1);
 }
 
 // Real rows packed in Zr -> forward x FFT -> spectra of both rows for
 // kx <= kxmax written to Y[kx * n2 + y].
-template <typename T>
-__device__ void x_r2c_rows(typename cx<T>::t* Zr, typename cx<T>::t* Zo, typename cx<T>::t* Y, int kxmax, int n1,
+template <typename T, int N1 = 0>
+__device__ void x_r2c_rows(typename cx<T>::t* Zr, typename cx<T>::t* Zo, typename cx<T>::t* Y, int kxmax, int n1_,
                            int n2, int q0, int nq, const typename cx<T>::t* tw1) {
   typedef typename cx<T>::t C;
+  const int n1 = N1 > 0 ? N1 : n1_;
   const int nk = kxmax + 1;
-  C* Zf = lds_fft<T>(Zr, Zo, n1, nq, n1, tw1, (T)-1);
+  C* Zf = lds_fft_n<T, N1>(Zr, Zo, n1, nq, n1, tw1, (T)-1);
   for (int t = threadIdx.x; t < nq * nk; t += blockDim.x) {
     const int ql = t / nk, k = t - ql * nk;
     const int y1 = 2 * (q0 + ql), y2 = y1 + 1;

@@ -176,15 +176,15 @@ k_slab_xy(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restric
 // by scheduling barriers so that the compiler does not hoist all of them next
 // to the spectrum (that spills past the 256 VGPRs of two waves per SIMD).
 // Same arithmetic, in the same order, as k_slab_xy.
-// LY lanes per y column transform (1, 2 or 4).
-template <typename T, int N1, int N2, int SB, int LB, int LY>
-__global__ void __launch_bounds__(256, LB)  // LB waves per SIMD: <= 512 / LB registers
+// LY lanes per y column transform (1, 2 or 4); NT threads per workgroup.
+template <typename T, int N1, int N2, int SB, int LB, int LY, int NT = 256>
+__global__ void __launch_bounds__(NT, LB)  // LB waves per SIMD: <= 512 / LB registers
 k_slab_xy_reg(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restrict__ t2, int n3, int nstick,
               int kxmax, int nb, const int* __restrict__ pos, const int* __restrict__ mpos,
               const T* __restrict__ vtot) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  slab_xy_reg_body<T, N1, N2, SB, LY>(blockIdx.x, blockIdx.y * SB, t1, t2, n3, nstick, kxmax, nb, pos, mpos, vtot,
-                                      smem_raw);
+  slab_xy_reg_body<T, N1, N2, SB, LY, NT>(blockIdx.x, blockIdx.y * SB, t1, t2, n3, nstick, kxmax, nb, pos, mpos,
+                                          vtot, smem_raw);
 }
 
 // ---------------------------------------------------------------- kernel 3
@@ -259,6 +259,7 @@ auto slab_kernel(int n1, int n2) -> decltype(&k_slab_xy<T, 0, 0>) {
 // second workgroup either), two waves up to 48 points, one beyond (the
 // 60-point double FFT pair needs about 350 registers)
 template <typename T> struct SlabCfg { static constexpr int SB = 8; };
+constexpr int kSlabWideSB = 16;  // the wide FP32 variant: 16 bands per plane, 512 threads
 template <> struct SlabCfg<double> { static constexpr int SB = 4; };
 template <typename T, int N> constexpr int slab_lb() { return (sizeof(T) == 4 || N <= 48) ? 2 : 1; }
 // lanes per y column transform when no plan chooses
@@ -270,8 +271,14 @@ template <typename T> static int default_slab_ly(int n) {
 template <typename T>
 using slab_reg_fn = decltype(&k_slab_xy_reg<T, 24, 24, SlabCfg<T>::SB, 2, 1>);
 template <typename T, int N>
-static slab_reg_fn<T> slab_reg_ly(int ly) {
+static slab_reg_fn<T> slab_reg_ly(int ly, int sb) {
   constexpr int SB = SlabCfg<T>::SB, LB = slab_lb<T, N>();
+  if (sb == kSlabWideSB) {
+    // FP32 planes up to 64 points: 16 bands per z plane, 512 threads
+    if constexpr (sizeof(T) == 4 && N <= 64)
+      if (ly == 1) return &k_slab_xy_reg<T, N, N, kSlabWideSB, 2, 1, 512>;
+    return nullptr;
+  }
   if (ly == 1) return &k_slab_xy_reg<T, N, N, SB, LB, 1>;
   if (ly == 2) return &k_slab_xy_reg<T, N, N, SB, LB, 2>;
   if constexpr (N % 4 == 0)
@@ -279,20 +286,27 @@ static slab_reg_fn<T> slab_reg_ly(int ly) {
   return nullptr;
 }
 // the register slab kernel of an n1 x n2 plane with ly lanes per y column
-// transform (ly <= 0: the default for the precision), or null if there is none
+// transform and sb bands per z plane (<= 0: the defaults for the precision),
+// or null if there is none
 template <typename T>
-static slab_reg_fn<T> slab_reg_kernel(int n1, int n2, int ly = 0) {
+static slab_reg_fn<T> slab_reg_kernel(int n1, int n2, int ly = 0, int sb = 0) {
   if (n1 != n2) return nullptr;
   if (ly <= 0) ly = default_slab_ly<T>(n1);
+  if (sb <= 0) sb = SlabCfg<T>::SB;
   switch (n1) {
 #define PW_CASE(n) \
   case n:          \
-    return slab_reg_ly<T, n>(ly);
+    return slab_reg_ly<T, n>(ly, sb);
     PW_REG_SIZES(PW_CASE)
 #undef PW_CASE
     default:
       return nullptr;
   }
+}
+// LDS bytes of the register slab kernel at sb bands per plane
+static size_t slab_reg_lds(const pw_ctx* c, size_t csz, int sb) {
+  return csz * ((size_t)sb * (c->sp.kxmax + 1) * c->n2) + csz * (size_t)(c->n2 / 2) * (c->n1 + 1) +
+         sizeof(int) * 2 * (size_t)c->sp.nstick;
 }
 
 template <typename T>
@@ -489,8 +503,13 @@ extern "C" int pw_ctx_create(pw_ctx** out, const int grid[3], const double super
     const char* sl = getenv("PW_SLAB");
     for (auto& pl : c->plan) {
       pl.reg = !(sl && std::string(sl).compare(0, 3, "lds") == 0);
-      // "reg:<ly>": the register kernel with ly lanes per y column transform
-      if (sl && std::string(sl).compare(0, 4, "reg:") == 0) pl.ly = std::max(1, atoi(sl + 4));
+      // "reg:<ly>[:<sb>]": the register kernel with ly lanes per y column
+      // transform (and sb bands per z plane; 16: the wide FP32 variant)
+      if (sl && std::string(sl).compare(0, 4, "reg:") == 0) {
+        pl.ly = std::max(1, atoi(sl + 4));
+        const char* b = strchr(sl + 4, ':');
+        if (b) pl.sb = std::max(0, atoi(b + 1));
+      }
       // "lds:<g>[:<r>]": the LDS kernel with its large-plane layout, g y
       // columns per group (and r x row pairs per step)
       if (sl && std::string(sl).compare(0, 4, "lds:") == 0) {
@@ -790,15 +809,18 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
                      c->d_pt_sz, tw3);
   PW_HIP(hipGetLastError());
   PW_TRY(mark(c, 1));
-  slab_reg_fn<T> kreg = pl.reg ? slab_reg_kernel<T>(c->n1, c->n2, pl.ly) : nullptr;
+  // the wide variant only where its y-space fits the LDS
+  const bool wide = pl.sb == kSlabWideSB && slab_reg_lds(c, sizeof(C), kSlabWideSB) <= 160 * 1024;
+  slab_reg_fn<T> kreg = pl.reg ? slab_reg_kernel<T>(c->n1, c->n2, wide ? 1 : pl.ly, wide ? kSlabWideSB : 0)
+                               : nullptr;
   if (kreg) {
     // one z plane of SB bands per workgroup, V plane + y-space in LDS
-    constexpr int SB = SlabCfg<T>::SB;
-    const size_t ldsr = sizeof(C) * ((size_t)SB * (c->sp.kxmax + 1) * c->n2) +
-                        sizeof(C) * (size_t)(c->n2 / 2) * (c->n1 + 1) + sizeof(int) * 2 * (size_t)c->sp.nstick;
+    const int SB = wide ? kSlabWideSB : SlabCfg<T>::SB;
+    const size_t ldsr = slab_reg_lds(c, sizeof(C), SB);
     PW_TRY(ensure_lds((const void*)kreg, ldsr));
-    hipLaunchKernelGGL(kreg, dim3(c->n3, (nb + SB - 1) / SB), dim3(256), ldsr, c->stream, (const C*)t1, (C*)t2,
-                       c->n3, c->sp.nstick, c->sp.kxmax, nb, c->d_pos, c->d_mpos, (const T*)vt);
+    hipLaunchKernelGGL(kreg, dim3(c->n3, (nb + SB - 1) / SB), dim3(wide ? 512 : 256), ldsr, c->stream,
+                       (const C*)t1, (C*)t2, c->n3, c->sp.nstick, c->sp.kxmax, nb, c->d_pos, c->d_mpos,
+                       (const T*)vt);
   } else {
     auto kslab = slab_kernel<T>(c->n1, c->n2);
     PW_TRY(ensure_lds((const void*)kslab, lds2));
@@ -822,9 +844,10 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
   return PW_OK;
 }
 
-bool has_reg_slab(const pw_ctx* c, int dtype, int ly) {
-  return dtype == PW_F64 ? slab_reg_kernel<double>(c->n1, c->n2, ly) != nullptr
-                         : slab_reg_kernel<float>(c->n1, c->n2, ly) != nullptr;
+bool has_reg_slab(const pw_ctx* c, int dtype, int ly, int sb) {
+  if (sb == kSlabWideSB && slab_reg_lds(c, dtype == PW_F64 ? 16 : 8, sb) > 160 * 1024) return false;
+  return dtype == PW_F64 ? slab_reg_kernel<double>(c->n1, c->n2, ly, sb) != nullptr
+                         : slab_reg_kernel<float>(c->n1, c->n2, ly, sb) != nullptr;
 }
 size_t slab_lds_bytes(const pw_ctx* c, int esz, int rp, int cg) { return slab_lds(c, esz, rp, cg); }
 

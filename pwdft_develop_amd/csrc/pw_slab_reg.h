@@ -11,6 +11,7 @@
 // thread holds N1 complex values; its LDS reads are issued in chunks fenced
 // by scheduling barriers so that the compiler does not hoist all of them next
 // to the spectrum (that spills past the 256 VGPRs of two waves per SIMD).
+// NT threads per workgroup (blockDim.x == NT).
 // LY lanes share each y column's transform (col_fft_lanes): LY = 1 keeps
 // k_slab_xy's arithmetic and order; LY = 2 or 4 splits every y transform
 // into LY shorter ones combined across lanes, so that SB nk columns occupy
@@ -22,7 +23,7 @@
 
 namespace pw {
 
-template <typename T, int N1, int N2, int SB, int LY = 1>
+template <typename T, int N1, int N2, int SB, int LY = 1, int NT = 256>
 __device__ __forceinline__ void slab_xy_reg_body(int z, int b0, const typename cx<T>::t* t1, typename cx<T>::t* t2,
                                                  int n3, int nstick, int kxmax, int nb, const int* __restrict__ pos,
                                                  const int* __restrict__ mpos, const T* __restrict__ vtot,
@@ -38,60 +39,99 @@ __device__ __forceinline__ void slab_xy_reg_body(int z, int b0, const typename c
   int* smpos = spos + nstick;                                // nstick
   const int ns = min(SB, nb - b0);
   const int t = threadIdx.x;
-  // Staging loads go in batches issued before any use (one memory round trip
-  // per batch; a plain strided loop waits for each load in turn).
-  const T* vz = vtot + (size_t)z * N1 * N2;
-  constexpr int VB = (N1 * N2 + 255) / 256;
+  // Every global load of the prologue is issued before any use (one memory
+  // round trip: the first KS sticks per thread, the V plane and the stick
+  // positions); a plain strided loop would wait for each load in turn.
+  // Element e of the plane's ns x nstick sticks is (s, st) = (e / nstick,
+  // e % nstick), advanced incrementally (no integer division per element).
+  const int tot = ns * nstick;
+  constexpr int KS = 16;
+  C sk[KS];
   {
-    T vb[VB];
+    int s_ = t / nstick, st_ = t - s_ * nstick;
 #pragma unroll
-    for (int u = 0; u < VB; u++) {
-      const int e = t + 256 * u;
-      vb[u] = e < N1 * N2 ? vz[e] : (T)0;
-    }
-#pragma unroll
-    for (int u = 0; u < VB; u++) {
-      const int e = t + 256 * u;
-      if (e < N1 * N2) {
-        const int y = e / N1, x = e - y * N1;
-        vs[2 * ((y >> 1) * LV + x) + (y & 1)] = vb[u];
-      }
+    for (int u = 0; u < KS; u++) {
+      const int e = t + NT * u;
+      sk[u] = e < tot ? t1[((size_t)(b0 + s_) * n3 + z) * nstick + st_] : C{0, 0};
+      st_ += NT;
+      while (st_ >= nstick) { st_ -= nstick; s_++; }
     }
   }
-  for (int e0 = 0; e0 < nstick; e0 += 4 * 256) {
-    int pb[4], mb[4];
+  const T* vz = vtot + (size_t)z * N1 * N2;
+  constexpr int VB = (N1 * N2 + NT - 1) / NT;
+  T vb[VB];
+#pragma unroll
+  for (int u = 0; u < VB; u++) {
+    const int e = t + NT * u;
+    vb[u] = e < N1 * N2 ? vz[e] : (T)0;
+  }
+  int pb[4], mb[4];
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int e = t + NT * u;
+    pb[u] = e < nstick ? pos[e] : 0;
+    mb[u] = e < nstick ? mpos[e] : -1;
+  }
+#pragma unroll
+  for (int u = 0; u < VB; u++) {
+    const int e = t + NT * u;
+    if (e < N1 * N2) {
+      const int y = e / N1, x = e - y * N1;
+      vs[2 * ((y >> 1) * LV + x) + (y & 1)] = vb[u];
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 4; u++) {
+    const int e = t + NT * u;
+    if (e < nstick) { spos[e] = pb[u]; smpos[e] = mb[u]; }
+  }
+  for (int e0 = 4 * NT; e0 < nstick; e0 += 4 * NT) {  // planes of more than 4 NT sticks
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int e = e0 + t + 256 * u;
+      const int e = e0 + t + NT * u;
       pb[u] = e < nstick ? pos[e] : 0;
       mb[u] = e < nstick ? mpos[e] : -1;
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int e = e0 + t + 256 * u;
+      const int e = e0 + t + NT * u;
       if (e < nstick) { spos[e] = pb[u]; smpos[e] = mb[u]; }
     }
   }
   for (int e = t; e < SB * nk * N2; e += blockDim.x) Y[e] = C{0, 0};
   __syncthreads();
-  const int tot = ns * nstick;
-  for (int e0 = 0; e0 < tot; e0 += 8 * 256) {
-    C vb[8];
+  {
+    int s_ = t / nstick, st_ = t - s_ * nstick;
+#pragma unroll
+    for (int u = 0; u < KS; u++) {
+      const int e = t + NT * u;
+      if (e < tot) {
+        C* Ys = Y + (size_t)s_ * nk * N2;
+        Ys[spos[st_]] = sk[u];
+        const int mp = smpos[st_];
+        if (mp >= 0) Ys[mp] = C{sk[u].x, -sk[u].y};
+      }
+      st_ += NT;
+      while (st_ >= nstick) { st_ -= nstick; s_++; }
+    }
+  }
+  for (int e0 = KS * NT; e0 < tot; e0 += 8 * NT) {  // more than KS sticks per thread
+    C vb8[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-      const int e = e0 + t + 256 * u;
-      const int s = e / nstick, st = e - s * nstick;
-      vb[u] = e < tot ? t1[((size_t)(b0 + s) * n3 + z) * nstick + st] : C{0, 0};
+      const int e = e0 + t + NT * u;
+      const int s_ = e / nstick, st_ = e - s_ * nstick;
+      vb8[u] = e < tot ? t1[((size_t)(b0 + s_) * n3 + z) * nstick + st_] : C{0, 0};
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-      const int e = e0 + t + 256 * u;
+      const int e = e0 + t + NT * u;
       if (e < tot) {
-        const int s = e / nstick, st = e - s * nstick;
-        C* Ys = Y + (size_t)s * nk * N2;
-        Ys[spos[st]] = vb[u];
-        const int mp = smpos[st];
-        if (mp >= 0) Ys[mp] = C{vb[u].x, -vb[u].y};
+        const int s_ = e / nstick, st_ = e - s_ * nstick;
+        C* Ys = Y + (size_t)s_ * nk * N2;
+        Ys[spos[st_]] = vb8[u];
+        const int mp = smpos[st_];
+        if (mp >= 0) Ys[mp] = C{vb8[u].x, -vb8[u].y};
       }
     }
   }
@@ -160,9 +200,24 @@ __device__ __forceinline__ void slab_xy_reg_body(int z, int b0, const typename c
   // forward y
   for (int c = t / LY; c < ns * nk; c += blockDim.x / LY) col_fft_lanes<T, N2, -1, LY>(Y + (size_t)c * N2, t % LY);
   __syncthreads();
-  for (int e = t; e < tot; e += blockDim.x) {
-    const int s = e / nstick, st = e - s * nstick;
-    t2[((size_t)(b0 + s) * n3 + z) * nstick + st] = Y[(size_t)s * nk * N2 + spos[st]];
+  // gather: 8 LDS reads per thread in flight, then the 8 stores
+  {
+    int s_ = t / nstick, st_ = t - s_ * nstick;
+    for (int e0 = t; e0 < tot; e0 += 8 * NT) {
+      C g[8];
+      long idx[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const int e = e0 + NT * u;
+        idx[u] = e < tot ? ((long)(b0 + s_) * n3 + z) * nstick + st_ : -1;
+        g[u] = e < tot ? Y[(size_t)s_ * nk * N2 + spos[st_]] : C{0, 0};
+        st_ += NT;
+        while (st_ >= nstick) { st_ -= nstick; s_++; }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (idx[u] >= 0) t2[idx[u]] = g[u];
+    }
   }
   __syncthreads();  // the caller may reuse the LDS at once
 }

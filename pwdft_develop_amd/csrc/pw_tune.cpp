@@ -90,13 +90,13 @@ static int time_min(hipStream_t s, int reps, Body body, double* ms) {
   return rc;
 }
 
-// reg: the register kernel with ly lanes per y column transform; else the
-// LDS kernel's (cg, rp, threads)
+// reg: the register kernel with ly lanes per y column transform and sb
+// bands per z plane (0: the default); else the LDS kernel's (cg, rp, threads)
 struct SlabChoice {
   bool reg;
-  int cg, rp, threads, ly;
+  int cg, rp, threads, ly, sb;
   std::string str() const {
-    if (reg) return "reg " + std::to_string(ly);
+    if (reg) return "reg " + std::to_string(ly) + " " + std::to_string(sb);
     return "lds " + std::to_string(cg) + " " + std::to_string(rp) + " " + std::to_string(threads);
   }
 };
@@ -104,20 +104,21 @@ struct SlabChoice {
 static void apply_slab(pw_ctx::Plan& pl, const SlabChoice& ch) {
   pl.reg = ch.reg;
   pl.ly = ch.reg ? ch.ly : 0;
+  pl.sb = ch.reg ? ch.sb : 0;
   pl.cg = ch.reg ? 0 : ch.cg;
   pl.rp = ch.reg ? 0 : ch.rp;
   pl.threads = ch.reg ? 0 : ch.threads;
 }
 
 static bool parse_slab(const std::string& v, SlabChoice* ch) {
-  int cg = 0, rp = 0, th = 0, ly = 0;
-  if (sscanf(v.c_str(), "reg %d", &ly) == 1 && (ly == 1 || ly == 2 || ly == 4)) {
-    *ch = SlabChoice{true, 0, 0, 0, ly};
+  int cg = 0, rp = 0, th = 0, ly = 0, sb = 0;
+  if (sscanf(v.c_str(), "reg %d %d", &ly, &sb) == 2 && (ly == 1 || ly == 2 || ly == 4) && sb >= 0) {
+    *ch = SlabChoice{true, 0, 0, 0, ly, sb};
     return true;
   }
   if (sscanf(v.c_str(), "lds %d %d %d", &cg, &rp, &th) == 3 && cg >= 0 && rp > 0 &&
       (th == 256 || th == 512 || th == 1024)) {
-    *ch = SlabChoice{false, cg, rp, th, 0};
+    *ch = SlabChoice{false, cg, rp, th, 0, 0};
     return true;
   }
   return false;
@@ -158,24 +159,26 @@ extern "C" int pw_autotune(pw_ctx* c, int dtype, int nb, const char* cache_path,
   else
     hipLaunchKernelGGL(k_tune_fill, dim3(g), dim3(256), 0, c->stream, (float*)X, m, 1.f);
   // ---- xy slab layout ----
-  SlabChoice best{false, 0, 0, 256, 0};
+  SlabChoice best{false, 0, 0, 256, 0, 0};
   auto it = cache.find(kslab);
-  if (it != cache.end() && parse_slab(it->second, &best) && (!best.reg || has_reg_slab(c, dtype, best.ly))) {
+  if (it != cache.end() && parse_slab(it->second, &best) &&
+      (!best.reg || has_reg_slab(c, dtype, best.ly, best.sb))) {
     apply_slab(pl, best);
     out += "slab " + best.str() + " (cached)";
   } else {
     std::vector<SlabChoice> cand;
     for (int ly : {1, 2, 4})
-      if (has_reg_slab(c, dtype, ly)) cand.push_back({true, 0, 0, 0, ly});
+      if (has_reg_slab(c, dtype, ly)) cand.push_back({true, 0, 0, 0, ly, 0});
+    if (has_reg_slab(c, dtype, 1, 16)) cand.push_back({true, 0, 0, 0, 1, 16});
     const int nk = c->sp.kxmax + 1, npair = (c->n2 + 1) / 2;
     for (int rp : {1, 2, 4, 8, 16})
       if (rp <= npair && slab_lds_bytes(c, esz, rp, 0) <= 160 * 1024)
-        for (int th : {256, 512}) cand.push_back({false, 0, rp, th, 0});
+        for (int th : {256, 512}) cand.push_back({false, 0, rp, th, 0, 0});
     for (int cg : {nk, std::max(1, nk / 2), 8, 4}) {
       if (cg > nk) continue;
       for (int rp : {4, 8, 16, std::max(1, std::min(npair, nk * c->n2 / (2 * c->n1)))})
         if (rp <= npair && slab_lds_bytes(c, esz, rp, cg) <= 160 * 1024)
-          for (int th : {512, 1024}) cand.push_back({false, cg, rp, th, 0});
+          for (int th : {512, 1024}) cand.push_back({false, cg, rp, th, 0, 0});
     }
     double tbest = 1e30;
     const pw_ctx::Plan saved = pl;
