@@ -505,11 +505,13 @@ def main():
     # per z plane), others the LDS kernel k_slab_xy
     reg = prob.grid[0] == prob.grid[1] and prob.grid[0] in (24, 30, 32, 36, 40, 48, 60, 64, 72)
     # the autotuner's FP32 choice names the variant ("slab reg <lanes per y
-    # transform>"): k_slab_xy_reg<float, n, n, 8 bands, 2 waves/SIMD, lanes>
-    mly = re.search(r"slab reg (\d)", out["autotune"]["f32"])
+    # transform> <bands per plane>"): k_slab_xy_reg<float, n, n, bands, 2
+    # waves/SIMD, lanes[, 512 threads for the 16-band variant]>
+    mly = re.search(r"slab reg (\d) (\d+)", out["autotune"]["f32"])
     reg = reg and (mly is not None or "slab lds" not in out["autotune"]["f32"])
-    ly = mly.group(1) if mly else "1"
-    kname = (f"k_slab_xy_reg<float, {prob.grid[0]}, {prob.grid[1]}, 8, 2, {ly}>" if reg
+    ly, sb = (mly.group(1), int(mly.group(2))) if mly else ("1", 0)
+    tail = ", 512>" if sb == 16 else ">"
+    kname = (f"k_slab_xy_reg<float, {prob.grid[0]}, {prob.grid[1]}, {sb or 8}, 2, {ly}{tail}" if reg
              else f"k_slab_xy<float, {prob.grid[0]}, {prob.grid[1]}>")
     traffic, full = pmc_traffic(os.path.join(ROOT, "profiles", f"pmc_traffic_{prob.name}.md"), kname)
     kname = full or kname

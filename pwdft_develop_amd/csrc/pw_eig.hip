@@ -255,15 +255,21 @@ k_sytrd(double* __restrict__ A, int n, int lda, int nbk, double* __restrict__ d,
 // matrix on chip.  The strictly lower 32 x 32 blocks (at most 28) sit in the
 // MFMA accumulator layout of eight wavefronts (block b, numbered row by row,
 // belongs to wave b % 8); diagonal block I lives in LDS (full, kept
-// symmetric) and is handled by wave I.  Per column k:
-//   (a) column k goes to LDS from its owners,
-//   (b) wave 0 forms the reflector v (zero outside k+1 .. n-1) and tau,
+// symmetric) and is handled by wave I.  Per column k, three barriers:
+//   (a) column k goes to LDS from its owners (two buffers by k's parity,
+//       so that the next column's write cannot meet a slow wave's read),
+//   (b) every wave forms the reflector's scalars (norm, beta, tau) itself,
+//       with the same lanes and order, so all hold identical values, and
+//       its own LDS copy of the reflector v (zero outside k+1 .. n-1),
 //   (c) every wave sums A v over its blocks that meet the trailing matrix:
 //       row sums across the 16 lanes of a DPP row, column sums (the
 //       transposed half of an off-diagonal block) across lanes l, l+16,
-//       l+32, l+48 by v_permlane16_swap / v_permlane32_swap; per-wave
-//       partials in LDS, added in a fixed order,
-//   (d) wave 0: w = p - (tau/2)(p.v) v with p = tau A v,
+//       l+32, l+48 by v_permlane16_swap / v_permlane32_swap; each block's
+//       row and column sums go to their own 32-entry rows of a table (no
+//       read-modify-write), with this wave's share of v^T A v,
+//   (d) wave I forms w = p - (tau/2)(p.v) v, p = tau A v, on rows of
+//       block I, adding the table's terms of that row block in a fixed
+//       order,
 //   (e) A -= v w^T + w v^T: one MFMA per 16 x 16 register tile (K = 2 of
 //       the 4), read-modify-write on the LDS diagonal blocks.
 // No global memory traffic after the load except the reflector and d, e,
@@ -272,6 +278,10 @@ constexpr int kTr2MaxNB = 8;
 constexpr int kTr2Waves = 8;  // = kTr2MaxNB: wave I owns diagonal block I
 constexpr int kTr2Slots = (kTr2MaxNB * (kTr2MaxNB - 1) / 2 + kTr2Waves - 1) / kTr2Waves;  // 4
 constexpr int kTr2Ld = 33;    // row stride of an LDS diagonal block
+constexpr int kTr2NBlk = kTr2MaxNB * (kTr2MaxNB - 1) / 2;  // strictly lower blocks (28)
+// partial-sum table rows of 32: diagonal block I, then per strictly lower
+// block b its row sums (rows of I) and its column sums (rows of J)
+constexpr int kTr2TabRows = kTr2MaxNB + 2 * kTr2NBlk;
 
 // strictly lower block b (I > J), numbered row by row: b = I (I - 1) / 2 + J
 __device__ __forceinline__ void tr_block(int b, int& I, int& J) {
@@ -351,13 +361,12 @@ __global__ void __launch_bounds__(512)
 k_sytrd_reg(double* __restrict__ A, int n, int lda, double* __restrict__ d, double* __restrict__ e,
             double* __restrict__ tau) {
   extern __shared__ double sm[];
-  double* xk = sm;                             // 256: column k
-  double* sv = xk + 256;                       // 256: reflector v, zero outside k+1 .. n-1
-  double* sw = sv + 256;                       // 256: w
-  double* pw = sw + 256;                       // kTr2Waves x 256: per-wave partial sums of A v
-  double* Dg = pw + kTr2Waves * 256;           // kTr2MaxNB x 32 x kTr2Ld: diagonal blocks
+  double* xb = sm;                             // 2 x 256: column k (buffer k % 2)
+  double* sw = xb + 512;                       // 256: w
+  double* tab = sw + 256;                      // kTr2TabRows x 32: partial sums of A v
+  double* dots = tab + kTr2TabRows * 32;       // kTr2Waves: partial sums of v^T A v
+  double* Dg = dots + kTr2Waves;               // kTr2MaxNB x 32 x kTr2Ld: diagonal blocks
   double* stg = Dg + kTr2MaxNB * 32 * kTr2Ld;  // load phase: kTr2Waves x 32 x 33 staging
-  __shared__ double stau;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int NB = (n + 31) / 32, nblk = NB * (NB - 1) / 2;
@@ -376,7 +385,7 @@ k_sytrd_reg(double* __restrict__ A, int n, int lda, double* __restrict__ d, doub
   // off-diagonal blocks: the same loads, then through this wave's staging
   // area into the accumulator layout; block coordinates per slot (wave
   // uniform, scalar registers)
-  int bI[kTr2Slots], bJ[kTr2Slots];
+  int bI[kTr2Slots], bJ[kTr2Slots], bB[kTr2Slots];
 #pragma unroll
   for (int s = 0; s < kTr2Slots; s++) {
     const int b = __builtin_amdgcn_readfirstlane(w) + kTr2Waves * s;
@@ -384,6 +393,7 @@ k_sytrd_reg(double* __restrict__ A, int n, int lda, double* __restrict__ d, doub
     if (b < nblk) tr_block(b, I, J);
     bI[s] = I;
     bJ[s] = J;
+    bB[s] = b;
   }
   d4v a[kTr2Slots][4];
   double* st = stg + w * 32 * 33;
@@ -403,159 +413,168 @@ k_sytrd_reg(double* __restrict__ A, int n, int lda, double* __restrict__ d, doub
     wave_lds_sync();
   }
   __syncthreads();
+  double* sv = stg + w * 32 * 33;  // the loop's per-wave copy of v
   for (int k = 0; k < n - 2; k++) {
+    double* x = xb + 256 * (k & 1);  // the other buffer may still be read by (e) of column k - 1
     // (a) column k, rows >= k
-    tr_column(a, bI, bJ, Dg, w, lane, n, k, k, xk);
+    tr_column(a, bI, bJ, Dg, w, lane, n, k, k, x);
     __syncthreads();
-    // (b) reflector (wave 0)
-    if (w == 0) {
-      double ss = 0.0;
+    // (b) the reflector's scalars in every wave (same lanes, same summation
+    // order: identical values); v(r) = 1 at r = k + 1, x[r] / (alpha - beta)
+    // below, 0 above
+    // one batch of LDS reads: lane l holds rows l + 64 u
+    double xr[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int r = k + 2 + lane + 64 * u;
-        const double x = r < n ? xk[r] : 0.0;
-        ss += x * x;
-      }
-      ss = wave_sum(ss);
-      const double alpha = xk[k + 1];
-      double tk = 0.0, beta = alpha, scal = 0.0;
-      if (ss > 0.0) {
-        beta = -copysign(sqrt(alpha * alpha + ss), alpha);
-        tk = (beta - alpha) / beta;
-        scal = 1.0 / (alpha - beta);
-      }
-      if (lane == 0) {
-        d[k] = xk[k];
-        e[k] = beta;
-        tau[k] = tk;
-        stau = tk;
-      }
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int r = lane + 64 * u;
-        double v = 0.0;
-        if (r == k + 1)
-          v = 1.0;
-        else if (r > k + 1 && r < n)
-          v = xk[r] * scal;
-        sv[r] = v;
-        if (r > k && r < n) A[r + (size_t)lda * k] = v;  // reflector storage for the back-transform
-      }
+    for (int u = 0; u < 4; u++) {
+      const int r = lane + 64 * u;
+      xr[u] = r < n ? x[r] : 0.0;
     }
-    __syncthreads();
-    // (c) per-wave partial sums of A v over the blocks meeting the trailing matrix
-    const double tk = stau;
+    const double alpha = x[k + 1], xkk = x[k];
+    double ss = 0.0;
 #pragma unroll
-    for (int u = 0; u < 4; u++) pw[w * 256 + lane + 64 * u] = 0.0;
+    for (int u = 0; u < 4; u++) ss += lane + 64 * u > k + 1 ? xr[u] * xr[u] : 0.0;
+    ss = wave_sum(ss);
+    double tk = 0.0, beta = alpha, scal = 0.0;
+    if (ss > 0.0) {
+      beta = -copysign(sqrt(alpha * alpha + ss), alpha);
+      tk = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    // this wave's copy of v (in its load staging area)
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int r = lane + 64 * u;
+      const double v = r == k + 1 ? 1.0 : ((r > k + 1 && r < n) ? xr[u] * scal : 0.0);
+      sv[r] = v;
+      if (w == 0 && r > k && r < n) A[r + (size_t)lda * k] = v;  // reflector storage for the back-transform
+    }
+    if (w == 0 && lane == 0) {
+      d[k] = xkk;
+      e[k] = beta;
+      tau[k] = tk;
+    }
+    if (tk == 0.0) continue;  // uniform: x is already (alpha, 0, ..., 0) below row k
     wave_lds_sync();
-    if (tk != 0.0) {
-      if (w < NB && 32 * w + 31 > k) {
-        // diagonal block: lane (row, half) sums 16 columns
-        const int r = lane & 31, h = lane >> 5;
-        double acc = 0.0;
+    auto vat = [&](int r) -> double { return sv[r]; };
+    // (c) A v over the blocks meeting the trailing matrix, each block's row
+    // and column sums stored once in the table (no read-modify-write), and
+    // this wave's share of v^T A v
+    double dp = 0.0;
+    if (w < NB && 32 * w + 31 > k) {
+      // diagonal block: lane (row, half) sums 16 columns
+      const int r = lane & 31, h = lane >> 5;
+      double acc = 0.0;
 #pragma unroll
-        for (int j = 0; j < 16; j++) acc += Dw[r * kTr2Ld + 16 * h + j] * sv[32 * w + 16 * h + j];
-        acc = half_sum(acc);
-        if (h == 0) pw[w * 256 + 32 * w + r] += acc;
-        wave_lds_sync();
-      }
-#pragma unroll
-      for (int s = 0; s < kTr2Slots; s++) {
-        const int I = bI[s], J = bJ[s];
-        if (J < 0 || 32 * J + 31 <= k) continue;  // no block, or v is zero on all its columns
-        const double vc0 = sv[32 * J + lr], vc1 = sv[32 * J + 16 + lr];
-        double rp[8];
-#pragma unroll
-        for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-          for (int i = 0; i < 4; i++) rp[4 * ti + i] = a[s][ti][i] * vc0 + a[s][ti + 2][i] * vc1;
-        const double rs = row16_sum8(rp, lr);
-        if ((lr & 1) == 0) {
-          const int j = lr >> 1;  // = 4 ti + i
-          pw[w * 256 + 32 * I + 16 * (j >> 2) + 4 * (j & 3) + lk] += rs;
-        }
-        double cp0 = 0.0, cp1 = 0.0;
-#pragma unroll
-        for (int ti = 0; ti < 2; ti++)
-#pragma unroll
-          for (int i = 0; i < 4; i++) {
-            const double vr = sv[32 * I + 16 * ti + 4 * i + lk];
-            cp0 += a[s][ti][i] * vr;
-            cp1 += a[s][ti + 2][i] * vr;
-          }
-        cp0 = row4_sum(cp0);
-        cp1 = row4_sum(cp1);
-        if (lk == 0) {
-          pw[w * 256 + 32 * J + lr] += cp0;
-          pw[w * 256 + 32 * J + 16 + lr] += cp1;
-        }
-        wave_lds_sync();  // this block's additions land before the next block's
-        // one block at a time: keeps the scheduler from overlapping blocks
-#pragma unroll
-        for (int q = 0; q < 4; q++) asm volatile("" : "+v"(a[s][q]));
+      for (int j = 0; j < 16; j++) acc += Dw[r * kTr2Ld + 16 * h + j] * vat(32 * w + 16 * h + j);
+      acc = half_sum(acc);
+      if (h == 0) {
+        tab[w * 32 + r] = acc;
+        dp += acc * vat(32 * w + r);
       }
     }
+#pragma unroll
+    for (int s = 0; s < kTr2Slots; s++) {
+      const int I = bI[s], J = bJ[s], b = bB[s];
+      if (J < 0 || 32 * J + 31 <= k) continue;  // no block, or v is zero on all its columns
+      const double vc0 = vat(32 * J + lr), vc1 = vat(32 * J + 16 + lr);
+      double rp[8];
+#pragma unroll
+      for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) rp[4 * ti + i] = a[s][ti][i] * vc0 + a[s][ti + 2][i] * vc1;
+      const double rs = row16_sum8(rp, lr);
+      if ((lr & 1) == 0) {
+        const int j = lr >> 1;  // = 4 ti + i
+        const int rl = 16 * (j >> 2) + 4 * (j & 3) + lk;
+        tab[(kTr2MaxNB + b) * 32 + rl] = rs;
+        dp += rs * vat(32 * I + rl);
+      }
+      double cp0 = 0.0, cp1 = 0.0;
+#pragma unroll
+      for (int ti = 0; ti < 2; ti++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const double vr = vat(32 * I + 16 * ti + 4 * i + lk);
+          cp0 += a[s][ti][i] * vr;
+          cp1 += a[s][ti + 2][i] * vr;
+        }
+      cp0 = row4_sum(cp0);
+      cp1 = row4_sum(cp1);
+      if (lk == 0) {
+        tab[(kTr2MaxNB + kTr2NBlk + b) * 32 + lr] = cp0;
+        tab[(kTr2MaxNB + kTr2NBlk + b) * 32 + 16 + lr] = cp1;
+        dp += cp0 * vc0 + cp1 * vc1;
+      }
+      // one block at a time: keeps the scheduler from overlapping blocks
+#pragma unroll
+      for (int q = 0; q < 4; q++) asm volatile("" : "+v"(a[s][q]));
+    }
+    dp = wave_sum(dp);
+    if (lane == 0) dots[w] = dp;
     __syncthreads();
-    // (d) w = p - (tau/2)(p.v) v, p = tau A v (wave 0)
-    if (w == 0) {
-      double pl[4], pv = 0.0;
+    // (d) rows 32 w .. 32 w + 31: p = tau A v (the table's terms of the row
+    // block in a fixed order), w = p - (tau/2)(p.v) v with p.v = tau v^T A v
+    if (w < NB && 32 * w + 31 > k && lane < 32) {
+      double vav = 0.0;
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int r = lane + 64 * u;
-        double p = 0.0;
-#pragma unroll
-        for (int ww = 0; ww < kTr2Waves; ww++) p += pw[ww * 256 + r];
-        p *= tk;
-        pl[u] = p;
-        pv += p * sv[r];
-      }
-      pv = wave_sum(pv);
-      const double h = 0.5 * tk * pv;
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int r = lane + 64 * u;
-        sw[r] = (r > k && r < n) ? pl[u] - h * sv[r] : 0.0;
-      }
+      for (int ww = 0; ww < kTr2Waves; ww++) vav += dots[ww];
+      const double h = 0.5 * tk * (tk * vav);
+      // terms in a fixed order: row sums of blocks (w, J), J = jmin .. w-1,
+      // then column sums of blocks (I, w), I = w+1 .. NB-1
+      double p = tab[w * 32 + lane];
+      for (int J = (k + 1) >> 5; J < w; J++) p += tab[(kTr2MaxNB + w * (w - 1) / 2 + J) * 32 + lane];
+      for (int I = w + 1; I < NB; I++) p += tab[(kTr2MaxNB + kTr2NBlk + I * (I - 1) / 2 + w) * 32 + lane];
+      const int r = 32 * w + lane;
+      sw[r] = (r > k && r < n) ? tk * p - h * vat(r) : 0.0;
     }
     __syncthreads();
     // (e) A -= v w^T + w v^T
-    if (tk != 0.0) {
-      if (w < NB && 32 * w + 31 > k) {
-        const int r = lane & 31, h = lane >> 5;
-        const double vr = sv[32 * w + r], wr = sw[32 * w + r];
+    if (w < NB && 32 * w + 31 > k) {
+      // reads of each 4 columns issued together, then the 4 writes
+      const int r = lane & 31, h = lane >> 5;
+      const double vr = vat(32 * w + r), wr = sw[32 * w + r];
 #pragma unroll
-        for (int j = 0; j < 16; j++) {
-          const int c = 16 * h + j;
-          Dw[r * kTr2Ld + c] -= vr * sw[32 * w + c] + wr * sv[32 * w + c];
-        }
-      }
-      // register tiles: A operand (row, kk) = (v, w, 0, 0)[kk], B operand
-      // (kk, col) = (-w, -v, 0, 0)[kk]
+      for (int j0 = 0; j0 < 16; j0 += 4) {
+        double dv[4], wc[4], vc[4];
 #pragma unroll
-      for (int s = 0; s < kTr2Slots; s++) {
-        const int I = bI[s], J = bJ[s];
-        if (J < 0 || 32 * J + 31 <= k) continue;
-        double ao[2], bo[2];
-#pragma unroll
-        for (int u = 0; u < 2; u++) {
-          const int r = 32 * I + 16 * u + lr, c = 32 * J + 16 * u + lr;
-          ao[u] = lk == 0 ? sv[r] : (lk == 1 ? sw[r] : 0.0);
-          bo[u] = lk == 0 ? -sw[c] : (lk == 1 ? -sv[c] : 0.0);
+        for (int j = 0; j < 4; j++) {
+          const int c = 16 * h + j0 + j;
+          dv[j] = Dw[r * kTr2Ld + c];
+          wc[j] = sw[32 * w + c];
+          vc[j] = vat(32 * w + c);
         }
 #pragma unroll
-        for (int q = 0; q < 4; q++) a[s][q] = mfma_f64(ao[q & 1], bo[q >> 1], a[s][q]);
+        for (int j = 0; j < 4; j++) Dw[r * kTr2Ld + 16 * h + j0 + j] = dv[j] - (vr * wc[j] + wr * vc[j]);
       }
+    }
+    // register tiles: A operand (row, kk) = (v, w, 0, 0)[kk], B operand
+    // (kk, col) = (-w, -v, 0, 0)[kk]
+#pragma unroll
+    for (int s = 0; s < kTr2Slots; s++) {
+      const int I = bI[s], J = bJ[s];
+      if (J < 0 || 32 * J + 31 <= k) continue;
+      double ao[2], bo[2];
+#pragma unroll
+      for (int u = 0; u < 2; u++) {
+        const int r = 32 * I + 16 * u + lr, c = 32 * J + 16 * u + lr;
+        ao[u] = lk == 0 ? vat(r) : (lk == 1 ? sw[r] : 0.0);
+        bo[u] = lk == 0 ? -sw[c] : (lk == 1 ? -vat(c) : 0.0);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; q++) a[s][q] = mfma_f64(ao[q & 1], bo[q >> 1], a[s][q]);
     }
     wave_lds_sync();  // this wave's diagonal-block writes before the next column read
   }
-  // the last 2 x 2: columns n - 2 (rows n-2, n-1) and n - 1
-  tr_column(a, bI, bJ, Dg, w, lane, n, n - 2, n - 2, xk);
-  tr_column(a, bI, bJ, Dg, w, lane, n, n - 1, n - 1, sw);
+  // the last 2 x 2: columns n - 2 (rows n-2, n-1) and n - 1, into buffers
+  // that the last update does not read (x of column n - 2 and the table)
+  double* xl = xb + 256 * ((n - 2) & 1);
+  tr_column(a, bI, bJ, Dg, w, lane, n, n - 2, n - 2, xl);
+  tr_column(a, bI, bJ, Dg, w, lane, n, n - 1, n - 1, tab);
   __syncthreads();
   if (t == 0) {
-    d[n - 2] = xk[n - 2];
-    d[n - 1] = sw[n - 1];
-    e[n - 2] = xk[n - 1];
+    d[n - 2] = xl[n - 2];
+    d[n - 1] = tab[n - 1];
+    e[n - 2] = xl[n - 1];
     tau[n - 2] = 0.0;
   }
 }
@@ -1064,9 +1083,9 @@ int eig_sym(pw_ctx* c, int n, double* A, int lda, double* w) {
   // panel width: the V and W panels (2 nbk n doubles) fit LDS next to v and
   // the partial sums (n = 256: 32; 512: 16; 1024: 8)
   if (n <= 32 * kTr2MaxNB) {
-    const size_t lds_reg = sizeof(double) * (3 * 256 + (size_t)kTr2Waves * 256 + (size_t)kTr2MaxNB * 32 * kTr2Ld +
-                                             (size_t)kTr2Waves * 32 * 33);
-    PW_TRY(ensure_lds((const void*)k_sytrd_reg, lds_reg + 256));
+    const size_t lds_reg = sizeof(double) * (3 * 256 + (size_t)kTr2TabRows * 32 + kTr2Waves +
+                                             (size_t)kTr2MaxNB * 32 * kTr2Ld + (size_t)kTr2Waves * 32 * 33);
+    PW_TRY(ensure_lds((const void*)k_sytrd_reg, lds_reg));
     hipLaunchKernelGGL(k_sytrd_reg, dim3(1), dim3(64 * kTr2Waves), lds_reg, s, A, n, lda, d, e, tau);
   } else {
     const int nbk = std::max(1, std::min(32, 8192 / n));

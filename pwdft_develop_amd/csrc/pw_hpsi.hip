@@ -187,6 +187,17 @@ k_slab_xy_reg(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __res
                                           vtot, smem_raw);
 }
 
+// The same transform as a persistent grid over (plane, band group) items
+// with the next item prefetched (slab_xy_reg_persistent).
+template <typename T, int N1, int N2, int SB, int LB, int LY>
+__global__ void __launch_bounds__(256, LB)
+k_slab_xy_reg_p(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __restrict__ t2, int n3, int nstick,
+                int kxmax, int nb, const int* __restrict__ pos, const int* __restrict__ mpos,
+                const T* __restrict__ vtot) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  slab_xy_reg_persistent<T, N1, N2, SB, LY, 256>(t1, t2, n3, nstick, kxmax, nb, pos, mpos, vtot, smem_raw);
+}
+
 // ---------------------------------------------------------------- kernel 3
 template <typename T, int N3>
 __global__ void __launch_bounds__(256)
@@ -271,8 +282,16 @@ template <typename T> static int default_slab_ly(int n) {
 template <typename T>
 using slab_reg_fn = decltype(&k_slab_xy_reg<T, 24, 24, SlabCfg<T>::SB, 2, 1>);
 template <typename T, int N>
-static slab_reg_fn<T> slab_reg_ly(int ly, int sb) {
+static slab_reg_fn<T> slab_reg_ly(int ly, int sb, bool persist) {
   constexpr int SB = SlabCfg<T>::SB, LB = slab_lb<T, N>();
+  if (persist) {
+    // persistent variant: FP32 only (the FP64 prefetch registers spill),
+    // the default band count, one lane per column
+    if constexpr (sizeof(T) == 4) {
+      if (ly == 1 && sb == SB) return &k_slab_xy_reg_p<T, N, N, SB, LB, 1>;
+    }
+    return nullptr;
+  }
   if (sb == kSlabWideSB) {
     // FP32 planes up to 64 points: 16 bands per z plane, 512 threads
     if constexpr (sizeof(T) == 4 && N <= 64)
@@ -289,14 +308,14 @@ static slab_reg_fn<T> slab_reg_ly(int ly, int sb) {
 // transform and sb bands per z plane (<= 0: the defaults for the precision),
 // or null if there is none
 template <typename T>
-static slab_reg_fn<T> slab_reg_kernel(int n1, int n2, int ly = 0, int sb = 0) {
+static slab_reg_fn<T> slab_reg_kernel(int n1, int n2, int ly = 0, int sb = 0, bool persist = false) {
   if (n1 != n2) return nullptr;
   if (ly <= 0) ly = default_slab_ly<T>(n1);
   if (sb <= 0) sb = SlabCfg<T>::SB;
   switch (n1) {
 #define PW_CASE(n) \
   case n:          \
-    return slab_reg_ly<T, n>(ly, sb);
+    return slab_reg_ly<T, n>(ly, sb, persist);
     PW_REG_SIZES(PW_CASE)
 #undef PW_CASE
     default:
@@ -503,12 +522,14 @@ extern "C" int pw_ctx_create(pw_ctx** out, const int grid[3], const double super
     const char* sl = getenv("PW_SLAB");
     for (auto& pl : c->plan) {
       pl.reg = !(sl && std::string(sl).compare(0, 3, "lds") == 0);
-      // "reg:<ly>[:<sb>]": the register kernel with ly lanes per y column
-      // transform (and sb bands per z plane; 16: the wide FP32 variant)
+      // "reg:<ly>[:<sb>][:p]": the register kernel with ly lanes per y
+      // column transform (sb bands per z plane; 16: the wide FP32 variant;
+      // p: on a persistent grid)
       if (sl && std::string(sl).compare(0, 4, "reg:") == 0) {
         pl.ly = std::max(1, atoi(sl + 4));
         const char* b = strchr(sl + 4, ':');
         if (b) pl.sb = std::max(0, atoi(b + 1));
+        pl.persist = strstr(sl + 4, ":p") != nullptr;  // ":p": the persistent grid
       }
       // "lds:<g>[:<r>]": the LDS kernel with its large-plane layout, g y
       // columns per group (and r x row pairs per step)
@@ -811,16 +832,32 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
   PW_TRY(mark(c, 1));
   // the wide variant only where its y-space fits the LDS
   const bool wide = pl.sb == kSlabWideSB && slab_reg_lds(c, sizeof(C), kSlabWideSB) <= 160 * 1024;
-  slab_reg_fn<T> kreg = pl.reg ? slab_reg_kernel<T>(c->n1, c->n2, wide ? 1 : pl.ly, wide ? kSlabWideSB : 0)
-                               : nullptr;
+  // the persistent grid where that variant exists, else the one-item grid
+  slab_reg_fn<T> kreg = pl.reg && pl.persist && !wide ? slab_reg_kernel<T>(c->n1, c->n2, pl.ly, 0, true) : nullptr;
+  const bool persist = kreg != nullptr;
+  if (pl.reg && !kreg) kreg = slab_reg_kernel<T>(c->n1, c->n2, wide ? 1 : pl.ly, wide ? kSlabWideSB : 0);
   if (kreg) {
-    // one z plane of SB bands per workgroup, V plane + y-space in LDS
+    // one z plane of SB bands per workgroup (or per item of the persistent
+    // grid), V plane + y-space in LDS
     const int SB = wide ? kSlabWideSB : SlabCfg<T>::SB;
     const size_t ldsr = slab_reg_lds(c, sizeof(C), SB);
     PW_TRY(ensure_lds((const void*)kreg, ldsr));
-    hipLaunchKernelGGL(kreg, dim3(c->n3, (nb + SB - 1) / SB), dim3(wide ? 512 : 256), ldsr, c->stream,
-                       (const C*)t1, (C*)t2, c->n3, c->sp.nstick, c->sp.kxmax, nb, c->d_pos, c->d_mpos,
-                       (const T*)vt);
+    dim3 grid(c->n3, (nb + SB - 1) / SB);
+    if (persist) {
+      // as many workgroups as the device holds at once
+      pw_ctx::Plan& pm = c->plan[sizeof(T) == 8 ? PW_F64 : PW_F32];
+      if (pm.pkern != (const void*)kreg || pm.plds != ldsr) {
+        int per_cu = 0, ncu = 0;
+        PW_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kreg, 256, ldsr));
+        PW_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+        pm.pkern = (const void*)kreg;
+        pm.plds = ldsr;
+        pm.pgrid = std::max(1, per_cu) * std::max(1, ncu);
+      }
+      grid = dim3((unsigned)std::min<long>((long)grid.x * grid.y, pm.pgrid));
+    }
+    hipLaunchKernelGGL(kreg, grid, dim3(wide ? 512 : 256), ldsr, c->stream, (const C*)t1, (C*)t2, c->n3,
+                       c->sp.nstick, c->sp.kxmax, nb, c->d_pos, c->d_mpos, (const T*)vt);
   } else {
     auto kslab = slab_kernel<T>(c->n1, c->n2);
     PW_TRY(ensure_lds((const void*)kslab, lds2));
@@ -844,10 +881,10 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
   return PW_OK;
 }
 
-bool has_reg_slab(const pw_ctx* c, int dtype, int ly, int sb) {
+bool has_reg_slab(const pw_ctx* c, int dtype, int ly, int sb, bool persist) {
   if (sb == kSlabWideSB && slab_reg_lds(c, dtype == PW_F64 ? 16 : 8, sb) > 160 * 1024) return false;
-  return dtype == PW_F64 ? slab_reg_kernel<double>(c->n1, c->n2, ly, sb) != nullptr
-                         : slab_reg_kernel<float>(c->n1, c->n2, ly, sb) != nullptr;
+  return dtype == PW_F64 ? slab_reg_kernel<double>(c->n1, c->n2, ly, sb, persist) != nullptr
+                         : slab_reg_kernel<float>(c->n1, c->n2, ly, sb, persist) != nullptr;
 }
 size_t slab_lds_bytes(const pw_ctx* c, int esz, int rp, int cg) { return slab_lds(c, esz, rp, cg); }
 

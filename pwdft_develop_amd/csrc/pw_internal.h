@@ -183,9 +183,10 @@ int sticks_to_sphere_add(pw_ctx* c, int nb, const void* t2, void* out);
 // Slab-kernel facts for the autotuner (pw_hpsi.hip): whether the
 // register-FFT kernel exists for the context's planes in dtype with ly
 // lanes per y column transform and sb bands per plane (<= 0: the defaults;
-// sb = 16: the wide FP32 variant, where its LDS fits), and the LDS bytes of
+// sb = 16: the wide FP32 variant, where its LDS fits; persist: the
+// persistent-grid variant), and the LDS bytes of
 // the LDS kernel at (rp, cg) for complex elements of esz bytes.
-bool has_reg_slab(const pw_ctx* c, int dtype, int ly = 0, int sb = 0);
+bool has_reg_slab(const pw_ctx* c, int dtype, int ly = 0, int sb = 0, bool persist = false);
 size_t slab_lds_bytes(const pw_ctx* c, int esz, int rp, int cg);
 // One Chebyshev step Ynew = s1 (H Y - c Y) - s2 Yp (pw_hpsi.hip), the
 // step's (c, s1, s2) read from the context's device coefficient table
@@ -274,6 +275,10 @@ struct pw_ctx {
     bool reg = true;
     int ly = 0;  // register kernel: lanes per y column transform (0: default)
     int sb = 0;  // register kernel: bands per z plane (0: default; 16: wide FP32 variant)
+    bool persist = false;          // register kernel on a persistent grid (next item prefetched)
+    const void* pkern = nullptr;   // persistent grid size, for this kernel and LDS size
+    size_t plds = 0;
+    int pgrid = 0;
     int cg = 0, rp = 0, threads = 0;
     int tn_tm = 0, tn_cap = 0;
   } plan[2];

@@ -95,8 +95,9 @@ static int time_min(hipStream_t s, int reps, Body body, double* ms) {
 struct SlabChoice {
   bool reg;
   int cg, rp, threads, ly, sb;
+  bool persist = false;
   std::string str() const {
-    if (reg) return "reg " + std::to_string(ly) + " " + std::to_string(sb);
+    if (reg) return "reg " + std::to_string(ly) + " " + std::to_string(sb) + (persist ? " p" : "");
     return "lds " + std::to_string(cg) + " " + std::to_string(rp) + " " + std::to_string(threads);
   }
 };
@@ -105,6 +106,7 @@ static void apply_slab(pw_ctx::Plan& pl, const SlabChoice& ch) {
   pl.reg = ch.reg;
   pl.ly = ch.reg ? ch.ly : 0;
   pl.sb = ch.reg ? ch.sb : 0;
+  pl.persist = ch.reg && ch.persist;
   pl.cg = ch.reg ? 0 : ch.cg;
   pl.rp = ch.reg ? 0 : ch.rp;
   pl.threads = ch.reg ? 0 : ch.threads;
@@ -114,6 +116,7 @@ static bool parse_slab(const std::string& v, SlabChoice* ch) {
   int cg = 0, rp = 0, th = 0, ly = 0, sb = 0;
   if (sscanf(v.c_str(), "reg %d %d", &ly, &sb) == 2 && (ly == 1 || ly == 2 || ly == 4) && sb >= 0) {
     *ch = SlabChoice{true, 0, 0, 0, ly, sb};
+    ch->persist = v.size() > 2 && v.compare(v.size() - 2, 2, " p") == 0;
     return true;
   }
   if (sscanf(v.c_str(), "lds %d %d %d", &cg, &rp, &th) == 3 && cg >= 0 && rp > 0 &&
@@ -162,7 +165,7 @@ extern "C" int pw_autotune(pw_ctx* c, int dtype, int nb, const char* cache_path,
   SlabChoice best{false, 0, 0, 256, 0, 0};
   auto it = cache.find(kslab);
   if (it != cache.end() && parse_slab(it->second, &best) &&
-      (!best.reg || has_reg_slab(c, dtype, best.ly, best.sb))) {
+      (!best.reg || has_reg_slab(c, dtype, best.ly, best.sb, best.persist))) {
     apply_slab(pl, best);
     out += "slab " + best.str() + " (cached)";
   } else {
@@ -170,6 +173,11 @@ extern "C" int pw_autotune(pw_ctx* c, int dtype, int nb, const char* cache_path,
     for (int ly : {1, 2, 4})
       if (has_reg_slab(c, dtype, ly)) cand.push_back({true, 0, 0, 0, ly, 0});
     if (has_reg_slab(c, dtype, 1, 16)) cand.push_back({true, 0, 0, 0, 1, 16});
+    if (has_reg_slab(c, dtype, 1, 0, true)) {
+      SlabChoice p{true, 0, 0, 0, 1, 0};
+      p.persist = true;
+      cand.push_back(p);
+    }
     const int nk = c->sp.kxmax + 1, npair = (c->n2 + 1) / 2;
     for (int rp : {1, 2, 4, 8, 16})
       if (rp <= npair && slab_lds_bytes(c, esz, rp, 0) <= 160 * 1024)

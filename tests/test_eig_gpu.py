@@ -47,7 +47,7 @@ def _rand_sym(n, seed):
     return (r + r.T) / np.sqrt(2 * n)
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 17, 64, 100, 256, 257, 512, 768, 1024])
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 33, 64, 100, 193, 256, 257, 512, 768, 1024])
 def test_random_symmetric(ctx, n):
     print(f"case: syevd random n={n}", flush=True)
     A = _rand_sym(n, n)

@@ -98,6 +98,38 @@ def test_fp32_wide_register_slab_kernel(name, nb, monkeypatch):
     c8.close()
 
 
+@pytest.mark.parametrize("name,nb", [("si8_32", 13), ("rand16_48", 21), ("si64_60", 5), ("si64_60", 203)])
+def test_fp32_persistent_register_slab_kernel(name, nb, monkeypatch):
+    """The FP32 register slab kernel on a persistent grid (PW_SLAB=reg:1:8:p:
+    one workgroup per resident slot, looping over (plane, band group) items
+    with the next item's sticks and V plane prefetched; 203 bands at 60
+    points give several items per workgroup and a ragged last group)
+    against the oracle and against the one-item grid (same arithmetic).
+    FP64 has no persistent variant and keeps the one-item kernel."""
+    import torch
+    from pwdft_develop_amd.ops import Context
+    prob = W.WORKLOADS[name](nb=nb).build()
+    print(f"case: {name} persistent slab nb={nb}", flush=True)
+    ref = O.Hamiltonian(prob).apply_h(prob.psi)
+    monkeypatch.setenv("PW_SLAB", "reg:1:8:p")
+    cp = Context(prob)
+    monkeypatch.setenv("PW_SLAB", "reg:1:8")
+    c1 = Context(prob)
+    monkeypatch.delenv("PW_SLAB", raising=False)
+    X = cp.to_device(prob.psi, torch.complex64)
+    a = cp.apply_h(X).cpu().numpy()
+    b = c1.apply_h(X).cpu().numpy()
+    assert _rel(a, ref) < 2e-5
+    assert np.array_equal(a, b)
+    if nb <= 21:
+        X64 = cp.to_device(prob.psi, torch.complex128)
+        a64 = cp.apply_h(X64).cpu().numpy()
+        assert np.array_equal(a64, c1.apply_h(X64).cpu().numpy())
+        assert _rel(a64, ref) < 1e-12
+    cp.close()
+    c1.close()
+
+
 @pytest.mark.parametrize("name,nb", [("si8_32", 32), ("si8_32", 7), ("rand16_48", 9), ("si64_60", 5)])
 def test_fp64_register_slab_kernel(name, nb, monkeypatch):
     """The FP64 register slab kernel (one z plane of 4 bands per workgroup,
