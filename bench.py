@@ -507,14 +507,17 @@ def main():
     # the autotuner's FP32 choice names the variant ("slab reg <lanes per y
     # transform> <bands per plane>"): k_slab_xy_reg<float, n, n, bands, 2
     # waves/SIMD, lanes[, 512 threads for the 16-band variant]>
-    mly = re.search(r"slab reg (\d) (\d+)", out["autotune"]["f32"])
+    # (" p": the persistent grid, k_slab_xy_reg_p)
+    mly = re.search(r"slab reg (\d) (\d+)( p)?", out["autotune"]["f32"])
     reg = reg and (mly is not None or "slab lds" not in out["autotune"]["f32"])
     ly, sb = (mly.group(1), int(mly.group(2))) if mly else ("1", 0)
-    tail = ", 512>" if sb == 16 else ">"
-    kname = (f"k_slab_xy_reg<float, {prob.grid[0]}, {prob.grid[1]}, {sb or 8}, 2, {ly}{tail}" if reg
-             else f"k_slab_xy<float, {prob.grid[0]}, {prob.grid[1]}>")
+    # (a name prefix: the traced name carries the thread count too)
+    tail = ", 512" if sb == 16 else ""
+    kbase = "k_slab_xy_reg_p" if mly and mly.group(3) else "k_slab_xy_reg"
+    kname = (f"{kbase}<float, {prob.grid[0]}, {prob.grid[1]}, {sb or 8}, 2, {ly}{tail}" if reg
+             else f"k_slab_xy<float, {prob.grid[0]}, {prob.grid[1]}")
     traffic, full = pmc_traffic(os.path.join(ROOT, "profiles", f"pmc_traffic_{prob.name}.md"), kname)
-    kname = full or kname
+    kname = full or kname + ">"
 
     def staged(tag, p):
         local_ms = sum(stages[tag][:3])

@@ -35,7 +35,7 @@ namespace pw {
 template <typename T, int N3>
 __global__ void __launch_bounds__(256)
 k_sticks_inverse_z(const typename cx<T>::t* __restrict__ psi, int ldpsi,
-                   typename cx<T>::t* __restrict__ t1, int n3_, int nstick, int sc,
+                   typename cx<T>::t* __restrict__ t1, int n3_, int nstick, int sc, int nchunk,
                    const int* __restrict__ chunk_off, const int* __restrict__ pt_idx,
                    const int* __restrict__ pt_sz, const typename cx<T>::t* __restrict__ tw3g) {
   typedef typename cx<T>::t C;
@@ -44,7 +44,10 @@ k_sticks_inverse_z(const typename cx<T>::t* __restrict__ psi, int ldpsi,
   C* tw = reinterpret_cast<C*>(smem_raw);
   C* A = tw + n3;
   C* B = A + sc * n3;
-  const int c = blockIdx.x, band = blockIdx.y;
+  // one row of nchunk x nb blocks; the chunks of a band in order on one XCD
+  // (neighbouring chunks share the cache lines at their ends)
+  const int item = xcd_item(blockIdx.x, gridDim.x);
+  const int band = item / nchunk, c = item - band * nchunk;
   const int s0 = c * sc;
   const C* ps = psi + (size_t)band * ldpsi;
   for (int t = threadIdx.x; t < n3; t += blockDim.x) tw[t] = tw3g[t];
@@ -202,7 +205,7 @@ k_slab_xy_reg_p(const typename cx<T>::t* __restrict__ t1, typename cx<T>::t* __r
 template <typename T, int N3>
 __global__ void __launch_bounds__(256)
 k_sticks_forward_z(const typename cx<T>::t* __restrict__ t2, const typename cx<T>::t* __restrict__ psi,
-                   typename cx<T>::t* __restrict__ out, int ld, int n3_, int nstick, int sc,
+                   typename cx<T>::t* __restrict__ out, int ld, int n3_, int nstick, int sc, int nchunk,
                    const int* __restrict__ gchunk_off, const int* __restrict__ gpt_idx,
                    const int* __restrict__ gpt_sz, const T* __restrict__ ekin,
                    const typename cx<T>::t* __restrict__ tw3g) {
@@ -212,7 +215,8 @@ k_sticks_forward_z(const typename cx<T>::t* __restrict__ t2, const typename cx<T
   C* tw = reinterpret_cast<C*>(smem_raw);
   C* A = tw + n3;
   C* B = A + sc * n3;
-  const int c = blockIdx.x, band = blockIdx.y;
+  const int item = xcd_item(blockIdx.x, gridDim.x);  // as in k_sticks_inverse_z
+  const int band = item / nchunk, c = item - band * nchunk;
   const int s0 = c * sc;
   const int nsl = min(sc, nstick - s0);
   for (int t = threadIdx.x; t < n3; t += blockDim.x) tw[t] = tw3g[t];
@@ -825,8 +829,8 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
   const size_t lds1 = (size_t)esz * (c->n3 + 2 * (size_t)sc * c->n3);
   PW_TRY(mark(c, 0));
   auto kin = inv_z_kernel<T>(c->n3);
-  hipLaunchKernelGGL(kin, dim3(c->sp.nchunk, nb), dim3(256), lds1, c->stream,
-                     (const C*)psi, c->npw, t1, c->n3, c->sp.nstick, sc, c->d_chunk_off, c->d_pt_idx,
+  hipLaunchKernelGGL(kin, dim3(c->sp.nchunk * nb), dim3(256), lds1, c->stream,
+                     (const C*)psi, c->npw, t1, c->n3, c->sp.nstick, sc, c->sp.nchunk, c->d_chunk_off, c->d_pt_idx,
                      c->d_pt_sz, tw3);
   PW_HIP(hipGetLastError());
   PW_TRY(mark(c, 1));
@@ -873,8 +877,8 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
   PW_HIP(hipGetLastError());
   PW_TRY(mark(c, 2));
   auto kfw = fwd_z_kernel<T>(c->n3);
-  hipLaunchKernelGGL(kfw, dim3(c->sp.nchunk, nb), dim3(256), lds1, c->stream,
-                     (const C*)t2, (const C*)psi, (C*)a3, c->npw, c->n3, c->sp.nstick, sc,
+  hipLaunchKernelGGL(kfw, dim3(c->sp.nchunk * nb), dim3(256), lds1, c->stream,
+                     (const C*)t2, (const C*)psi, (C*)a3, c->npw, c->n3, c->sp.nstick, sc, c->sp.nchunk,
                      c->d_gchunk_off, c->d_gpt_idx, c->d_gpt_sz, ek, tw3);
   PW_HIP(hipGetLastError());
   PW_TRY(mark(c, 3));
@@ -895,8 +899,8 @@ int sphere_to_sticks(pw_ctx* c, int nb, const void* psi, void* t1) {
   const C* tw3 = (const C*)(sizeof(T) == 8 ? (const void*)c->d_tw64 : (const void*)c->d_tw32) + c->n1 + c->n2;
   const size_t lds1 = sizeof(C) * (c->n3 + 2 * (size_t)c->sp.sc * c->n3);
   auto kin = inv_z_kernel<T>(c->n3);
-  hipLaunchKernelGGL(kin, dim3(c->sp.nchunk, nb), dim3(256), lds1, c->stream,
-                     (const C*)psi, c->npw, (C*)t1, c->n3, c->sp.nstick, c->sp.sc, c->d_chunk_off, c->d_pt_idx,
+  hipLaunchKernelGGL(kin, dim3(c->sp.nchunk * nb), dim3(256), lds1, c->stream,
+                     (const C*)psi, c->npw, (C*)t1, c->n3, c->sp.nstick, c->sp.sc, c->sp.nchunk, c->d_chunk_off, c->d_pt_idx,
                      c->d_pt_sz, tw3);
   PW_HIP(hipGetLastError());
   return PW_OK;
@@ -909,8 +913,8 @@ int sticks_to_sphere_add(pw_ctx* c, int nb, const void* t2, void* out) {
   const C* tw3 = (const C*)(sizeof(T) == 8 ? (const void*)c->d_tw64 : (const void*)c->d_tw32) + c->n1 + c->n2;
   const size_t lds1 = sizeof(C) * (c->n3 + 2 * (size_t)c->sp.sc * c->n3);
   auto kfw = fwd_z_kernel<T>(c->n3);
-  hipLaunchKernelGGL(kfw, dim3(c->sp.nchunk, nb), dim3(256), lds1, c->stream, (const C*)t2,
-                     (const C*)nullptr, (C*)out, c->npw, c->n3, c->sp.nstick, c->sp.sc, c->d_gchunk_off,
+  hipLaunchKernelGGL(kfw, dim3(c->sp.nchunk * nb), dim3(256), lds1, c->stream, (const C*)t2,
+                     (const C*)nullptr, (C*)out, c->npw, c->n3, c->sp.nstick, c->sp.sc, c->sp.nchunk, c->d_gchunk_off,
                      c->d_gpt_idx, c->d_gpt_sz, (const T*)nullptr, tw3);
   PW_HIP(hipGetLastError());
   return PW_OK;

@@ -18,6 +18,16 @@ namespace pw {
 // data goes through LDS.
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
+// Work item of block b in a one-dimensional grid of n blocks.  Blocks are
+// dispatched round-robin over the 8 XCDs (block b on XCD b % 8), each with
+// its own L2; this order gives XCD x the contiguous item range
+// [x q + min(x, r), ..) (q = n / 8, r = n % 8), so that neighbouring items,
+// which share cache lines, share an L2.  A bijection of 0 .. n-1.
+__device__ __forceinline__ int xcd_item(int b, int n) {
+  const int q = n >> 3, r = n & 7, x = b & 7;
+  return x * q + min(x, r) + (b >> 3);
+}
+
 // Data exchanged through LDS between lanes of one wavefront: this
 // wavefront's LDS operations issued so far have completed, and the compiler
 // moves no memory access across (without it, a lane's load of a word another
