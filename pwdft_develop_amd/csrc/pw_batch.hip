@@ -407,7 +407,11 @@ __global__ void __launch_bounds__(kBatchThreads, 2) k_batch(BatchArgs g) {
     float2* prev = step == 1 ? nullptr : (step == 2 ? g.X : ((step & 1) ? g.YA : g.YB));
     float2* dst = g.degree == 0 ? g.HX : ((step & 1) ? g.YA : g.YB);
     for (int s = 0; s < 4; s++) {
-      for (int j = g.joff[s] + blockIdx.x; j < g.joff[s + 1]; j += gridDim.x) {
+      // round r of a stage gives job joff + r G + xcd_item(b, G) to
+      // workgroup b (G workgroups): each XCD a contiguous run of the
+      // round's jobs, so that neighbouring stick chunks of a band (which
+      // share cache lines) meet in one L2
+      for (int j = g.joff[s] + xcd_item(blockIdx.x, gridDim.x); j < g.joff[s + 1]; j += gridDim.x) {
         const int4 jb = g.jobs[j];
         const int type = __builtin_amdgcn_readfirstlane(jb.x), ci = __builtin_amdgcn_readfirstlane(jb.y);
         const int a = __builtin_amdgcn_readfirstlane(jb.z), b = __builtin_amdgcn_readfirstlane(jb.w);

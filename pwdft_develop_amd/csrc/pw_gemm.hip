@@ -299,6 +299,17 @@ __global__ void k_gemm_tn_reduce(const T* __restrict__ P, int nsplit, int n1, in
   }
 }
 
+// Tile origin of block blockIdx.x in the one-row grid of ti x tj BM x BN
+// output tiles of the NN products: the tj column tiles of a row tile, which
+// read the same rows of A, are consecutive items, and xcd_item puts runs of
+// consecutive items on one XCD (one L2).
+__device__ __forceinline__ void nn_tile_origin(int m, int n, int& i0, int& j0) {
+  const int ti = (m + BM - 1) / BM, tj = (n + BN - 1) / BN;
+  const int item = xcd_item(blockIdx.x, ti * tj), i = item / tj;
+  i0 = i * BM;
+  j0 = (item - i * tj) * BN;
+}
+
 // Y = alpha A B + beta Y; A m x k (lda), B k x n (ldb), Y m x n (ldy).
 // Pipelined like k_gemm_tn_partial.  A chunk (64 rows x 32 k): load q of
 // wave w reads k = 8 w + q, rows lane (one 64-row run per wave); B chunk
@@ -310,7 +321,8 @@ k_gemm_nn(const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb, T*
   typedef typename acc4<T>::t V;
   __shared__ T As[BK][BM + 1];
   __shared__ T Bs[BN][BK + 1];
-  const int i0 = blockIdx.x * BM, j0 = blockIdx.y * BN;
+  int i0, j0;
+  nn_tile_origin(m, n, i0, j0);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wi = (w & 1) * 32, wj = (w >> 1) * 32;
   V acc[2][2];
@@ -417,7 +429,8 @@ k_gemm_nn_bf(const float* __restrict__ A, int lda, const float* __restrict__ B, 
   __shared__ __align__(16) unsigned short Sh[2 * NS * 64 * BKP > 64 * 65 * 2 ? 2 * NS * 64 * BKP : 64 * 65 * 2];
   unsigned short* As = Sh;                 // NS term tiles of A (64 rows x 32 k)
   unsigned short* Bs = Sh + NS * 64 * BKP;  // NS term tiles of B (64 columns x 32 k)
-  const int i0 = blockIdx.x * BM, j0 = blockIdx.y * BN;
+  int i0, j0;
+  nn_tile_origin(m, n, i0, j0);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wi = (w & 1) * 32, wj = (w >> 1) * 32;
   f4v acc[2][2];
@@ -655,7 +668,8 @@ k_gemm_nn_ep(const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb,
   typedef typename acc4<T>::t V;
   __shared__ T As[BK][BM + 1];
   __shared__ T Bs[BN][BK + 1];
-  const int i0 = blockIdx.x * BM, j0 = blockIdx.y * BN;
+  int i0, j0;
+  nn_tile_origin(m, n, i0, j0);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int wi = (w & 1) * 32, wj = (w >> 1) * 32;
   V acc[2][2];
@@ -902,7 +916,7 @@ int gemm_nn(pw_ctx* c, int m, int n, int k, T alpha, const T* A, int lda, const 
     PW_HIP(hipGetLastError());
     return PW_OK;
   }
-  hipLaunchKernelGGL(k_gemm_nn<T>, dim3((m + BM - 1) / BM, (n + BN - 1) / BN), dim3(256), 0, c->stream, A, lda, B,
+  hipLaunchKernelGGL(k_gemm_nn<T>, dim3(((m + BM - 1) / BM) * ((n + BN - 1) / BN)), dim3(256), 0, c->stream, A, lda, B,
                      ldb, Y, ldy, m, n, k, alpha, beta);
   PW_HIP(hipGetLastError());
   return PW_OK;
@@ -916,7 +930,7 @@ int gemm_nn_ep(pw_ctx* c, int m, int n, int k, const T* A, int lda, const T* B, 
                const T* Yp, T* out, int ld, const T* coef, int step, bool cheb, int split) {
   if (m <= 0 || n <= 0) return PW_OK;
   NnEpilogue<T> ep{H, Yc, Yp, out, ld, coef, step, (T)1, (T)1};
-  const dim3 grid((m + BM - 1) / BM, (n + BN - 1) / BN);
+  const dim3 grid(((m + BM - 1) / BM) * ((n + BN - 1) / BN));  // nn_tile_origin's order
   bool done = false;
   if (!cheb && n == 1 && k > 0 && H == out) {
     // one column (the Lanczos vector): out += A b
