@@ -145,14 +145,22 @@ def realspace_bench(args):
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle as O
+        from concurrent.futures import ThreadPoolExecutor
+        cores = host_cores()
+        nbs = min(nb, 2 * cores)
+
+        def one_band(b):  # the C call releases the GIL; no state is shared between calls
+            return O.rs_mult_spinor(prob.grid, prob.grid_fine, prob.volume, prob.ekin, prob.vtot, prob.npts,
+                                    prob.nproj, prob.idx, prob.val, prob.wgt, prob.psi[b:b + 1])
+
         t0 = time.perf_counter()
-        O.rs_mult_spinor(prob.grid, prob.grid_fine, prob.volume, prob.ekin, prob.vtot, prob.npts, prob.nproj,
-                         prob.idx, prob.val, prob.wgt, prob.psi[:2])
-        tc = (time.perf_counter() - t0) / 2
-        res["cpu_baseline"] = {"value": round(1.0 / tc, 3), "unit": "band-applications/s", "cores": 1,
+        with ThreadPoolExecutor(cores) as ex:
+            list(ex.map(one_band, range(nbs)))
+        tc = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(nbs / tc, 3), "unit": "band-applications/s", "cores": cores,
                                "kind": "port",
-                               "sample": f"2 bands of {prob.name} through the C oracle, one thread: "
-                                         f"{tc:.2f} s per band"}
+                               "sample": f"{nbs} bands of {prob.name} through the C oracle, one band per call "
+                                         f"on {cores} threads: {tc:.2f} s"}
     return res
 
 

@@ -13,12 +13,16 @@
 //
 //   k_exx_slab_inv   wavefunction sticks -> y IFFT -> x c2r -> real grid (psi_k, phi_j)
 //   k_exx_pair_fwd   psi_k(r) phi_j(r) -> x r2c -> y FFT -> Fock sticks
-//   k_exx_fock_z     Fock sticks: z FFT, x K(G)/N, z IFFT (in place)
+//                    (k_exx_pair_fwd_reg / _acc_reg: the register-FFT forms
+//                    for square planes of compile-time length)
+//   k_exx_fock_z     Fock sticks: z FFT, x K(G)/N, z IFFT (in place);
+//                    k_exx_fock_z_reg for compile-time z lengths
 //   k_exx_pair_acc   Fock sticks -> y IFFT -> x c2r -> acc_k += f_j phi_j u_kj, over j
 //   k_exx_slab_fwd   acc_k(r) -> x r2c -> y FFT -> wavefunction sticks -> a3 += gather
 #include <algorithm>
 #include <cstdio>
 #include <cmath>
+#include <string>
 #include <vector>
 
 #include <rocsolver/rocsolver.h>
@@ -37,6 +41,8 @@ struct ExxState {
   int* d_pos = nullptr;          // kx*n2 + ky per Fock stick
   double* d_K64 = nullptr;       // K(G)/N per (stick, z)
   float* d_K32 = nullptr;
+  double* d_K64t = nullptr;      // the same per (z, stick)
+  float* d_K32t = nullptr;
   double* d_phi64 = nullptr;     // phi_j(r), nocc x N
   float* d_phi32 = nullptr;
   double* d_fac64 = nullptr;     // -frac * occ_j
@@ -223,6 +229,33 @@ k_exx_fock_z(typename cx<T>::t* __restrict__ fst, const T* __restrict__ K, int n
   }
 }
 
+// The same for compile-time N3 with one thread per (pair, stick): the z
+// column in registers (reg_fft), no LDS and no barrier; loads and stores of
+// a z row are coalesced over the sticks, K read per (z, stick).
+template <typename T, int N3>
+__global__ void __launch_bounds__(256)
+k_exx_fock_z_reg(typename cx<T>::t* __restrict__ fst, const T* __restrict__ Kt, int nstick, long nwork) {
+  typedef typename cx<T>::t C;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nwork) return;
+  const long p = e / nstick;
+  const int s = (int)(e - p * nstick);
+  C* base = fst + (size_t)p * N3 * nstick + s;
+  C a[N3];
+#pragma unroll
+  for (int z = 0; z < N3; z++) a[z] = base[(size_t)z * nstick];
+  pin_regs(a);  // phase fences: keep K's loads after the first transform
+  reg_fft<T, N3, -1>(a);
+  pin_regs(a);
+#pragma unroll
+  for (int z = 0; z < N3; z++) a[z] = cscale(a[z], Kt[(size_t)z * nstick + s]);
+  pin_regs(a);
+  reg_fft<T, N3, 1>(a);
+  pin_regs(a);
+#pragma unroll
+  for (int z = 0; z < N3; z++) base[(size_t)z * nstick] = a[z];
+}
+
 // acc_k(r) += sum_jl fac_j phi_j(r) c2r(Fock sticks of pair (k, jl))(r)
 template <typename T, int N1 = 0, int N2 = 0>
 __global__ void __launch_bounds__(1024)
@@ -252,6 +285,189 @@ k_exx_pair_acc(const typename cx<T>::t* __restrict__ fst, const T* __restrict__ 
         if (y2 < g.n2) ak[(size_t)y2 * n1 + x] += fj * ph[(size_t)y2 * n1 + x] * u.y;
       }
       __syncthreads();
+    }
+  }
+}
+
+// Register variants of the pair kernels for square N x N planes: every 1D
+// transform is split over L adjacent lanes (lanes_fft / col_fft_lanes: L =
+// 4 where 4 divides N, else 2), so that each thread holds N / L values and
+// the passes need no LDS round trips or barriers of their own.  LDS holds
+// one y-space (kxmax + 1 <= N / 2 + 1 columns of N), which the pair
+// densities' row pairs share before the x transform; ExxReg<T, N> carries
+// the shape.  One workgroup per (z plane, pair) as in the LDS kernels;
+// the row-pair and column phases run on N / 2 L and (kxmax + 1) L threads.
+template <typename T, int N>
+struct ExxReg {
+  static constexpr int L = N % 4 == 0 ? 4 : 2, M = N / L, NP = N / 2;
+  static constexpr int NT = ((NP * L + 63) / 64) * 64;  // threads per workgroup
+  static constexpr int LDS = (NP + 1) * N * (int)sizeof(typename cx<T>::t);
+  static constexpr int UMAX = (NP * (NP + 1) + NT - 1) / NT;  // unpack outputs per thread
+};
+
+// pair densities psi_k phi_j -> Fock sticks (as k_exx_pair_fwd)
+template <typename T, int N>
+__global__ void __launch_bounds__((ExxReg<T, N>::NT))
+k_exx_pair_fwd_reg(const T* __restrict__ psir, const T* __restrict__ phir, int nb, int j0,
+                   typename cx<T>::t* __restrict__ fst, const int* __restrict__ pos, int n3, int nstick, int kxmax) {
+  typedef typename cx<T>::t C;
+  typedef ExxReg<T, N> R;
+  constexpr int L = R::L, NP = R::NP, NT = R::NT;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* Z = reinterpret_cast<C*>(smem_raw);  // row pairs Z[q][x], then y-space Y[k][y]
+  T* Zr = reinterpret_cast<T*>(smem_raw);
+  const int t = threadIdx.x, z = blockIdx.x, p = blockIdx.y;
+  const int k = p % nb, j = j0 + p / nb;
+  const size_t plane = (size_t)N * N;
+  const T* a = psir + ((size_t)k * n3 + z) * plane;
+  const T* b = phir + ((size_t)j * n3 + z) * plane;
+  // products, rows 2q and 2q + 1 as the real and imaginary parts of Z[q]
+  for (int e0 = 0; e0 < N * N; e0 += 4 * NT) {
+    T u[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int e = e0 + t + NT * i;
+      u[i] = e < N * N ? a[e] * b[e] : (T)0;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int e = e0 + t + NT * i;
+      if (e < N * N) {
+        const int y = e / N, x = e - y * N;
+        Zr[2 * ((y >> 1) * N + x) + (y & 1)] = u[i];
+      }
+    }
+  }
+  __syncthreads();
+  if (t < NP * L) col_fft_lanes<T, N, -1, L>(Z + (t / L) * N, t % L);
+  __syncthreads();
+  // unpack the two rows' spectra for kx <= kxmax (reads, barrier, writes:
+  // the y-space overlays the row pairs)
+  const int nk = kxmax + 1;
+  C o1[R::UMAX], o2[R::UMAX];
+#pragma unroll
+  for (int u = 0; u < R::UMAX; u++) {
+    const int e = t + NT * u;
+    if (e < NP * nk) {
+      const int q = e / nk, kk = e - q * nk;
+      const C w = Z[q * N + kk], wc = Z[q * N + (kk == 0 ? 0 : N - kk)];
+      o1[u] = C{(T)0.5 * (w.x + wc.x), (T)0.5 * (w.y - wc.y)};
+      o2[u] = C{(T)0.5 * (w.y + wc.y), (T)0.5 * (wc.x - w.x)};
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < R::UMAX; u++) {
+    const int e = t + NT * u;
+    if (e < NP * nk) {
+      const int q = e / nk, kk = e - q * nk;
+      Z[kk * N + 2 * q] = o1[u];
+      Z[kk * N + 2 * q + 1] = o2[u];
+    }
+  }
+  __syncthreads();
+  for (int c = t / L; c < nk; c += NT / L) col_fft_lanes<T, N, -1, L>(Z + c * N, t % L);
+  __syncthreads();
+  C* dst = fst + ((size_t)p * n3 + z) * nstick;
+  for (int s0 = 0; s0 < nstick; s0 += 4 * NT) {
+    C g[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int s1 = s0 + t + NT * i;
+      g[i] = s1 < nstick ? Z[pos[s1]] : C{0, 0};
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int s1 = s0 + t + NT * i;
+      if (s1 < nstick) dst[s1] = g[i];
+    }
+  }
+}
+
+// acc_k(r) += sum_jl fac_j phi_j(r) c2r(Fock sticks of pair (k, jl))(r) (as
+// k_exx_pair_acc); thread (q, h) of the row phase owns the same x samples
+// of rows 2q, 2q + 1 for every jl, so acc stays in its registers.
+template <typename T, int N>
+__global__ void __launch_bounds__((ExxReg<T, N>::NT))
+k_exx_pair_acc_reg(const typename cx<T>::t* __restrict__ fst, const T* __restrict__ phir,
+                   const T* __restrict__ fac, int nb, int j0, int nj, T* __restrict__ acc,
+                   const int* __restrict__ pos, int n3, int nstick, int kxmax) {
+  typedef typename cx<T>::t C;
+  typedef ExxReg<T, N> R;
+  constexpr int L = R::L, M = R::M, NP = R::NP, NT = R::NT;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  C* Y = reinterpret_cast<C*>(smem_raw);
+  const int t = threadIdx.x, z = blockIdx.x, k = blockIdx.y;
+  const int nk = kxmax + 1;
+  const size_t plane = (size_t)N * N;
+  T* ak = acc + ((size_t)k * n3 + z) * plane;
+  const bool rowt = t < NP * L;
+  const int q = t / L, h = t % L, y1 = 2 * q, y2 = y1 + 1;
+  // output block of this lane after lanes_fft (the same for every call)
+  const int qo = L == 2 ? (h & 1) : ((h >> 1) + 2 * (h & 1));
+  T r1[M], r2[M];
+#pragma unroll
+  for (int x = 0; x < M; x++) {
+    r1[x] = rowt ? ak[(size_t)y1 * N + qo * M + x] : (T)0;
+    r2[x] = rowt ? ak[(size_t)y2 * N + qo * M + x] : (T)0;
+  }
+  for (int jl = 0; jl < nj; jl++) {
+    const int j = j0 + jl;
+    const T fj = fac[j];
+    const int p = k + nb * jl;
+    for (int e = t; e < nk * N; e += NT) Y[e] = C{0, 0};
+    __syncthreads();
+    const C* src = fst + ((size_t)p * n3 + z) * nstick;
+    for (int s0 = 0; s0 < nstick; s0 += 4 * NT) {
+      C g[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int s1 = s0 + t + NT * i;
+        g[i] = s1 < nstick ? src[s1] : C{0, 0};
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int s1 = s0 + t + NT * i;
+        if (s1 < nstick) Y[pos[s1]] = g[i];
+      }
+    }
+    __syncthreads();
+    for (int c = t / L; c < nk; c += NT / L) col_fft_lanes<T, N, 1, L>(Y + c * N, t % L);
+    __syncthreads();
+    if (rowt) {
+      // Hermitian-extended spectra of rows y1, y2 at x frequencies m L + h,
+      // packed as X1 + i X2 (x_c2r_rows' packing)
+      C a[M];
+#pragma unroll
+      for (int m = 0; m < M; m++) {
+        const int kf = m * L + h;
+        int kk = kf;
+        bool cj = false;
+        if (kf > N / 2) { kk = N - kf; cj = true; }
+        C a1 = C{0, 0}, a2 = C{0, 0};
+        if (kk <= kxmax) {
+          a1 = Y[kk * N + y1];
+          a2 = Y[kk * N + y2];
+          if (cj) { a1.y = -a1.y; a2.y = -a2.y; }
+          if (kf == 0 || 2 * kf == N) { a1.y = 0; a2.y = 0; }
+        }
+        a[m] = C{a1.x - a2.y, a1.y + a2.x};
+      }
+      lanes_fft<T, N, 1, L>(a, h);
+      const T* ph = phir + ((size_t)j * n3 + z) * plane;
+#pragma unroll
+      for (int x = 0; x < M; x++) {
+        r1[x] += fj * ph[(size_t)y1 * N + qo * M + x] * a[x].x;
+        r2[x] += fj * ph[(size_t)y2 * N + qo * M + x] * a[x].y;
+      }
+    }
+    __syncthreads();  // the next pair's zero fill overwrites Y
+  }
+  if (rowt) {
+#pragma unroll
+    for (int x = 0; x < M; x++) {
+      ak[(size_t)y1 * N + qo * M + x] = r1[x];
+      ak[(size_t)y2 * N + qo * M + x] = r2[x];
     }
   }
 }
@@ -399,10 +615,12 @@ struct ExxKernels {
   decltype(&k_exx_pair_acc<T>) pair_acc;
   decltype(&k_exx_slab_fwd<T>) slab_fwd;
   decltype(&k_exx_fock_z<T>) fock_z;
+  decltype(&k_exx_fock_z_reg<T, 24>) fock_z_reg;  // null: the LDS kernel
 };
 template <typename T>
 static ExxKernels<T> exx_kernels(int n1, int n2, int n3) {
-  ExxKernels<T> k{&k_exx_slab_inv<T>, &k_exx_pair_fwd<T>, &k_exx_pair_acc<T>, &k_exx_slab_fwd<T>, &k_exx_fock_z<T>};
+  ExxKernels<T> k{&k_exx_slab_inv<T>, &k_exx_pair_fwd<T>, &k_exx_pair_acc<T>, &k_exx_slab_fwd<T>, &k_exx_fock_z<T>,
+                  nullptr};
   if (n1 == n2) {
     switch (n1) {
 #define PW_CASE(n)                                                                                        \
@@ -419,15 +637,55 @@ static ExxKernels<T> exx_kernels(int n1, int n2, int n3) {
     }
   }
   switch (n3) {
-#define PW_CASE(n)                    \
-  case n:                             \
-    k.fock_z = &k_exx_fock_z<T, n>;   \
+#define PW_CASE(n)                                                   \
+  case n:                                                            \
+    k.fock_z = &k_exx_fock_z<T, n>;                                  \
+    if constexpr (sizeof(T) * n <= 480) /* no spills: FP64 to 60 */  \
+      k.fock_z_reg = &k_exx_fock_z_reg<T, n>;                        \
     break;
     PW_EXX_Z(PW_CASE)
 #undef PW_CASE
     default:
       break;
   }
+  return k;
+}
+
+// the register pair kernels of an n1 x n2 plane (null: none; the LDS
+// kernels then), their block size and LDS bytes
+template <typename T>
+struct ExxRegKernels {
+  decltype(&k_exx_pair_fwd_reg<T, 24>) fwd = nullptr;
+  decltype(&k_exx_pair_acc_reg<T, 24>) acc = nullptr;
+  int threads = 0;
+  size_t lds = 0;
+};
+template <typename T>
+static ExxRegKernels<T> exx_reg_kernels(int n1, int n2) {
+  ExxRegKernels<T> k;
+  // FP32 takes the register kernels, FP64 the LDS ones (config-4 exchange,
+  // 1024 x 432 pairs on 96 x 96 x 40: FP32 1.14 s register against 1.89 s
+  // LDS; FP64 3.34 against 3.32 s, the FP64 register kernels at one wave
+  // per SIMD).  PW_EXX_PAIR (for comparisons): "lds" both LDS kernels,
+  // "reg" both register kernels in either precision, "fwd" the register
+  // forward kernel only.
+  const char* f = getenv("PW_EXX_PAIR");
+  const std::string mode = f ? f : "";
+  if (n1 != n2 || mode == "lds" || (sizeof(T) == 8 && mode != "reg")) return k;
+  switch (n1) {
+#define PW_CASE(n)                              \
+  case n:                                       \
+    k.fwd = &k_exx_pair_fwd_reg<T, n>;          \
+    k.acc = &k_exx_pair_acc_reg<T, n>;          \
+    k.threads = ExxReg<T, n>::NT;               \
+    k.lds = (size_t)ExxReg<T, n>::LDS;          \
+    break;
+    PW_EXX_PLANES(PW_CASE)
+#undef PW_CASE
+    default:
+      break;
+  }
+  if (mode == "fwd") k.acc = nullptr;
   return k;
 }
 
@@ -458,6 +716,7 @@ int exx_add(pw_ctx* c, int nb, const void* psi, void* a3) {
   const T* phir = (const T*)(sizeof(T) == 8 ? (const void*)E->d_phi64 : (const void*)E->d_phi32);
   const T* fac = (const T*)(sizeof(T) == 8 ? (const void*)E->d_fac64 : (const void*)E->d_fac32);
   const T* K = (const T*)(sizeof(T) == 8 ? (const void*)E->d_K64 : (const void*)E->d_K32);
+  const T* Kt = (const T*)(sizeof(T) == 8 ? (const void*)E->d_K64t : (const void*)E->d_K32t);
   const C* tw = (const C*)(sizeof(T) == 8 ? (const void*)c->d_tw64 : (const void*)c->d_tw32);
   PW_TRY(sphere_to_real<T>(c, nb, psi, psir, (T)1));
   // zero fill by a kernel, not a memset node: this runs inside captured
@@ -470,19 +729,47 @@ int exx_add(pw_ctx* c, int nb, const void* psi, void* a3) {
   const ExxKernels<T> kx = exx_kernels<T>(c->n1, c->n2, c->n3);
   PW_TRY(set_lds(kx.pair_fwd, ldsf));
   PW_TRY(set_lds(kx.pair_acc, ldsf));
-  const int J = std::max(1, std::min(E->nocc, E->npair_max / nb));
+  const ExxRegKernels<T> kr = exx_reg_kernels<T>(c->n1, c->n2);
+  if (kr.fwd) PW_TRY(set_lds(kr.fwd, kr.lds));
+  if (kr.acc) PW_TRY(set_lds(kr.acc, kr.lds));
+  // pairs per launch: kc bands x jc occupied bands (kc jc <= npair_max);
+  // up to 64 bands per chunk so that the accumulation kernel's read and
+  // write of acc is shared by jc pairs, and 40 planes x 64 bands still fill
+  // the device
+  const int jc = std::max(1, std::min(E->nocc, E->npair_max / std::min(nb, 64)));
+  const int kc = std::max(1, std::min(nb, E->npair_max / jc));
   const size_t ldsz = sizeof(C) * (c->n3 + 2 * (size_t)E->sc * c->n3);
-  for (int j0 = 0; j0 < E->nocc; j0 += J) {
-    const int nj = std::min(J, E->nocc - j0);
-    hipLaunchKernelGGL(kx.pair_fwd, dim3(c->n3, nb * nj), dim3(thr), ldsf, c->stream, (const T*)psir, phir,
-                       nb, j0, (C*)E->d_fst, E->d_pos, gf, tw, tw + c->n1);
-    PW_HIP(hipGetLastError());
-    hipLaunchKernelGGL(kx.fock_z, dim3(E->nchunk, nb * nj), dim3(256), ldsz, c->stream, (C*)E->d_fst, K,
-                       c->n3, E->nstick, E->sc, tw + c->n1 + c->n2);
-    PW_HIP(hipGetLastError());
-    hipLaunchKernelGGL(kx.pair_acc, dim3(c->n3, nb), dim3(thr), ldsf, c->stream, (const C*)E->d_fst, phir,
-                       fac, nb, j0, nj, acc, E->d_pos, gf, tw, tw + c->n1);
-    PW_HIP(hipGetLastError());
+  const size_t bstride = (size_t)N;  // one band of psir / acc
+  for (int k0 = 0; k0 < nb; k0 += kc) {
+    const int nk = std::min(kc, nb - k0);
+    const T* pk = psir + (size_t)k0 * bstride;
+    T* ak = acc + (size_t)k0 * bstride;
+    for (int j0 = 0; j0 < E->nocc; j0 += jc) {
+      const int nj = std::min(jc, E->nocc - j0);
+      if (kr.fwd)
+        hipLaunchKernelGGL(kr.fwd, dim3(c->n3, nk * nj), dim3(kr.threads), kr.lds, c->stream, pk, phir, nk, j0,
+                           (C*)E->d_fst, E->d_pos, c->n3, E->nstick, E->kxmax);
+      else
+        hipLaunchKernelGGL(kx.pair_fwd, dim3(c->n3, nk * nj), dim3(thr), ldsf, c->stream, pk, phir, nk, j0,
+                           (C*)E->d_fst, E->d_pos, gf, tw, tw + c->n1);
+      PW_HIP(hipGetLastError());
+      if (kx.fock_z_reg) {
+        const long nwork = (long)nk * nj * E->nstick;
+        hipLaunchKernelGGL(kx.fock_z_reg, dim3((unsigned)((nwork + 255) / 256)), dim3(256), 0, c->stream,
+                           (C*)E->d_fst, Kt, E->nstick, nwork);
+      } else {
+        hipLaunchKernelGGL(kx.fock_z, dim3(E->nchunk, nk * nj), dim3(256), ldsz, c->stream, (C*)E->d_fst, K,
+                           c->n3, E->nstick, E->sc, tw + c->n1 + c->n2);
+      }
+      PW_HIP(hipGetLastError());
+      if (kr.acc)
+        hipLaunchKernelGGL(kr.acc, dim3(c->n3, nk), dim3(kr.threads), kr.lds, c->stream, (const C*)E->d_fst, phir,
+                           fac, nk, j0, nj, ak, E->d_pos, c->n3, E->nstick, E->kxmax);
+      else
+        hipLaunchKernelGGL(kx.pair_acc, dim3(c->n3, nk), dim3(thr), ldsf, c->stream, (const C*)E->d_fst, phir,
+                           fac, nk, j0, nj, ak, E->d_pos, gf, tw, tw + c->n1);
+      PW_HIP(hipGetLastError());
+    }
   }
   SlabGeom g{c->n1, c->n2, c->n3, c->sp.nstick, c->sp.kxmax, 0, 0};
   const int thw = pick_wfn_layout(c, sizeof(C), g);
@@ -606,7 +893,7 @@ void exx_free(pw_ctx* c) {
   ExxState* E = c->exx;
   if (!E) return;
   invalidate_graph(c);
-  void* p[] = {E->d_pos, E->d_K64, E->d_K32, E->d_phi64, E->d_phi32, E->d_fac64, E->d_fac32,
+  void* p[] = {E->d_pos, E->d_K64, E->d_K32, E->d_K64t, E->d_K32t, E->d_phi64, E->d_phi32, E->d_fac64, E->d_fac32,
                E->d_psir, E->d_acc, E->d_fst};
   for (void* q : p)
     if (q) (void)hipFree(q);
@@ -704,9 +991,15 @@ extern "C" int pw_set_exx(pw_ctx* c, double ecut, double mu, double exx_fraction
     K[(size_t)sid[col] * n3 + kz] = kv * inv;
   }
   std::vector<float> Kf(K.begin(), K.end());
+  std::vector<double> Kt(K.size());
+  for (int st = 0; st < E->nstick; st++)
+    for (int z = 0; z < n3; z++) Kt[(size_t)z * E->nstick + st] = K[(size_t)st * n3 + z];
+  std::vector<float> Ktf(Kt.begin(), Kt.end());
   PW_TRY(up(&E->d_pos, pos.data(), pos.size()));
   PW_TRY(up(&E->d_K64, K.data(), K.size()));
   PW_TRY(up(&E->d_K32, Kf.data(), Kf.size()));
+  PW_TRY(up(&E->d_K64t, Kt.data(), Kt.size()));
+  PW_TRY(up(&E->d_K32t, Ktf.data(), Ktf.size()));
   std::vector<double> fac(nocc);
   for (int j = 0; j < nocc; j++) fac[j] = -exx_fraction * occ[j];
   std::vector<float> facf(fac.begin(), fac.end());
@@ -717,9 +1010,9 @@ extern "C" int pw_set_exx(pw_ctx* c, double ecut, double mu, double exx_fraction
   PW_HIP(hipMalloc(&E->d_acc, sizeof(double) * N * c->max_bands));
   PW_HIP(hipMalloc((void**)&E->d_phi64, sizeof(double) * N * nocc));
   PW_HIP(hipMalloc((void**)&E->d_phi32, sizeof(float) * N * nocc));
-  // pair stick buffer: up to ~1 GiB, at least one pair per band
+  // pair stick buffer: up to ~4 GiB, at least one pair per band
   const size_t per_pair = sizeof(double) * 2 * (size_t)n3 * E->nstick;
-  size_t npm = std::max<size_t>((size_t)c->max_bands, (size_t)(1ull << 30) / per_pair);
+  size_t npm = std::max<size_t>((size_t)c->max_bands, (size_t)(4ull << 30) / per_pair);
   npm = std::min<size_t>(npm, (size_t)c->max_bands * nocc);
   E->npair_max = (int)npm;
   PW_HIP(hipMalloc(&E->d_fst, per_pair * npm));

@@ -370,15 +370,15 @@ __device__ __forceinline__ void lane_twiddles(typename cx<T>::t (&a)[M], int h, 
 // then bit 0) with a DPP exchange, the second stage's input on lane 3
 // multiplied by exp(SG 2 pi i / 4).  Lane h = 2 b1 + b0 ends with
 // X[q M + k], k < M, q = b0 (L = 2) or b1 + 2 b0 (L = 4).
+//
+// lanes_fft is the register part: on entry lane h holds a[m] = x[m L + h],
+// on exit a[k] = X[q M + k], q returned.
 template <typename T, int N, int SG, int L>
-__device__ __forceinline__ void col_fft_lanes(typename cx<T>::t* col, int h) {
+__device__ __forceinline__ int lanes_fft(typename cx<T>::t (&a)[N / L], int h) {
   typedef typename cx<T>::t C;
   static_assert(L == 1 || L == 2 || L == 4, "1, 2 or 4 lanes per column");
   static_assert(N % L == 0, "L divides N");
   constexpr int M = N / L;
-  C a[M];
-#pragma unroll
-  for (int m = 0; m < M; m++) a[m] = col[m * L + h];
   reg_fft<T, M, SG>(a);
   int q = 0;
   if constexpr (L > 1) {
@@ -400,6 +400,17 @@ __device__ __forceinline__ void col_fft_lanes(typename cx<T>::t* col, int h) {
     }
     q = L == 2 ? (h & 1) : ((h >> 1) + 2 * (h & 1));
   }
+  return q;
+}
+
+template <typename T, int N, int SG, int L>
+__device__ __forceinline__ void col_fft_lanes(typename cx<T>::t* col, int h) {
+  typedef typename cx<T>::t C;
+  constexpr int M = N / L;
+  C a[M];
+#pragma unroll
+  for (int m = 0; m < M; m++) a[m] = col[m * L + h];
+  const int q = lanes_fft<T, N, SG, L>(a, h);
 #pragma unroll
   for (int k = 0; k < M; k++) col[q * M + k] = a[k];
 }

@@ -335,11 +335,14 @@ class Spinor {
     if (fftPtr->domain.numGridSphere != ntot) ErrorHandling("Domain size does not match.");
     (void)teter;
     pw_ctx* ctx = fftPtr->Ctx();
-    DeviceBlock x, y;
-    x.Upload(wavefunG_.Data(), (size_t)ntot * nocc);
-    y.Upload(a3.Data(), (size_t)ntot * nocc);
-    Check(pw_teter_precond(ctx, PW_F64, nocc, x.Ptr(), y.Ptr()));
-    y.Download(a3.Data(), (size_t)ntot * nocc);
+    if (!stx_) {
+      stx_ = std::make_shared<DeviceBlock>();
+      sty_ = std::make_shared<DeviceBlock>();
+    }
+    stx_->Upload(wavefunG_.Data(), (size_t)ntot * nocc);
+    sty_->Upload(a3.Data(), (size_t)ntot * nocc);
+    Check(pw_teter_precond(ctx, PW_F64, nocc, stx_->Ptr(), sty_->Ptr()));
+    sty_->Download(a3.Data(), (size_t)ntot * nocc);
   }
 
  private:
@@ -347,6 +350,9 @@ class Spinor {
   Int numComponent_ = 1, numStateTotal_ = 0;
   NumTns<Complex> wavefunG_;
   NumTns<Real> wavefun_;
+  // device staging of AddTeterPrecond, grown on demand (shared by copies,
+  // which keeps Spinor copyable; calls run in order on one stream)
+  std::shared_ptr<DeviceBlock> stx_, sty_;
 };
 
 // ---- Hamiltonian / KohnSham (include/hamiltonian.hpp) ----

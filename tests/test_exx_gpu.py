@@ -78,6 +78,7 @@ def test_add_exx_forced_grouped_layouts(pair, wfn, monkeypatch):
     H = O.Hamiltonian(prob)
     H.set_exx(prob.ecut, MU, FRAC, prob.psi[:nocc], occ)
     ref = H.add_exx(prob.psi)
+    monkeypatch.setenv("PW_EXX_PAIR", "lds")  # the LDS pair kernels, whose layouts these are
     monkeypatch.setenv("PW_EXX_LAYOUT", pair)
     monkeypatch.setenv("PW_WFN_LAYOUT", wfn)
     ctx = Context(prob)
@@ -89,6 +90,32 @@ def test_add_exx_forced_grouped_layouts(pair, wfn, monkeypatch):
     rho_o = H.density(prob.psi, f)
     assert _rel(rho, rho_o) < 1e-12
     ctx.close()
+
+
+@pytest.mark.parametrize("name,nb,nocc", [("si8_32", 7, 5), ("rand16_48", 5, 3), ("layered432_96", 3, 4)])
+def test_register_pair_kernels(name, nb, nocc, monkeypatch):
+    """The register-FFT pair kernels (k_exx_pair_fwd_reg / _acc_reg: 4 lanes
+    per 1D transform, the accumulator in registers over the occupied bands;
+    PW_EXX_PAIR=reg forces them in FP64 too) against the LDS pair kernels
+    (PW_EXX_PAIR=lds) on square planes of 32, 48 and 96 points, FP64 and
+    FP32; parity of the defaults (FP32 register, FP64 LDS) with the oracle
+    is the other tests'."""
+    import torch
+    from pwdft_develop_amd.ops import Context
+    prob = W.WORKLOADS[name](nb=nb).build()
+    print(f"case: {name} exx register vs lds pair kernels nb={nb} nocc={nocc}", flush=True)
+    occ = np.linspace(1.0, 0.4, nocc)
+    out = {}
+    for mode in ("reg", "lds"):
+        monkeypatch.setenv("PW_EXX_PAIR", mode)
+        ctx = Context(prob)
+        ctx.set_exx(prob.ecut, MU, FRAC, ctx.to_device(prob.psi[:nocc]), occ)
+        out[mode] = (ctx.add_exx(ctx.to_device(prob.psi)).cpu().numpy(),
+                     ctx.add_exx(ctx.to_device(prob.psi, torch.complex64)).cpu().numpy())
+        ctx.close()
+    monkeypatch.delenv("PW_EXX_PAIR", raising=False)
+    assert _rel(out["reg"][0], out["lds"][0]) < 1e-12
+    assert _rel(out["reg"][1], out["lds"][1]) < 2e-6
 
 
 def test_density_large_cell():
