@@ -246,6 +246,90 @@ k_sticks_forward_z(const typename cx<T>::t* __restrict__ t2, const typename cx<T
   }
 }
 
+// Register forms of kernels 1 and 3 for compile-time N3: each stick's z
+// column is split over L adjacent lanes (lanes_fft: L = 4 where 4 divides
+// N3, else 2), lane h holding z = m L + h on input and the block q M ..
+// q M + M - 1 on output; the sphere side is read or written through the
+// (z, stick) table StickPlan::zs.  Thread e = L (band nstick + stick) + h,
+// so that at each step neighbouring lane groups take neighbouring sticks
+// (adjacent t1 / t2 entries; adjacent points of the sphere's (kz, ky, kx)
+// order).  No LDS, no barrier.
+template <int N3>
+struct ZLanes {
+  static constexpr int L = N3 % 4 == 0 ? 4 : 2, M = N3 / L;
+};
+
+template <typename T, int N3>
+__global__ void __launch_bounds__(256)
+k_sticks_inverse_z_reg(const typename cx<T>::t* __restrict__ psi, int ldpsi, typename cx<T>::t* __restrict__ t1,
+                       int nstick, long nwork, const int* __restrict__ zs) {
+  typedef typename cx<T>::t C;
+  constexpr int L = ZLanes<N3>::L, M = ZLanes<N3>::M;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long w = e / L;  // (band, stick); the L lanes of one stick exit together
+  if (w >= nwork) return;
+  const int h = (int)(e - w * L);
+  const int band = (int)(w / nstick), s = (int)(w - (long)band * nstick);
+  const C* ps = psi + (size_t)band * ldpsi;
+  const T is2 = (T)0.70710678118654752440084436210485;
+  C a[M];
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    const int i = zs[(size_t)(m * L + h) * nstick + s];
+    C v = C{0, 0};
+    if (i >= 0) {
+      v = ps[i];
+      if (i != 0) v = cscale(v, is2);
+    } else if (i != kZsAbsent) {
+      const C u = ps[-i - 1];
+      v = C{u.x * is2, -u.y * is2};
+    }
+    a[m] = v;
+  }
+  const int q = lanes_fft<T, N3, 1, L>(a, h);
+  C* o = t1 + ((size_t)band * N3 + q * M) * nstick + s;
+#pragma unroll
+  for (int k = 0; k < M; k++) o[(size_t)k * nstick] = a[k];
+}
+
+template <typename T, int N3>
+__global__ void __launch_bounds__(256)
+k_sticks_forward_z_reg(const typename cx<T>::t* __restrict__ t2, const typename cx<T>::t* __restrict__ psi,
+                       typename cx<T>::t* __restrict__ out, int ld, int nstick, long nwork,
+                       const int* __restrict__ zs, const T* __restrict__ ekin) {
+  typedef typename cx<T>::t C;
+  constexpr int L = ZLanes<N3>::L, M = ZLanes<N3>::M;
+  const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long w = e / L;
+  if (w >= nwork) return;
+  const int h = (int)(e - w * L);
+  const int band = (int)(w / nstick), s = (int)(w - (long)band * nstick);
+  const C* src = t2 + (size_t)band * N3 * nstick + s;
+  C a[M];
+#pragma unroll
+  for (int m = 0; m < M; m++) a[m] = src[(size_t)(m * L + h) * nstick];
+  const int q = lanes_fft<T, N3, -1, L>(a, h);
+  const C* ps = psi + (size_t)band * ld;
+  C* o = out + (size_t)band * ld;
+  const T s2 = (T)1.4142135623730950488016887242097;
+#pragma unroll
+  for (int k = 0; k < M; k++) {
+    const int i = zs[(size_t)(q * M + k) * nstick + s];
+    if (i >= 0) {  // direct points only (mirrors and absent entries are negative)
+      C y = a[k];
+      if (i != 0) y = cscale(y, s2);
+      if (ekin) {
+        const C cc = ps[i];
+        const T ek = ekin[i];
+        o[i] = C{ek * cc.x + y.x, ek * cc.y + y.y};
+      } else {  // accumulate: a3 += gather (spinor.cpp:933-935)
+        const C acc = o[i];
+        o[i] = C{acc.x + y.x, acc.y + y.y};
+      }
+    }
+  }
+}
+
 // Grid sizes whose FFT lengths are compiled in (all lengths of the built
 // workloads); other 2/3/5-smooth sizes use the runtime-length engine.
 #define PW_FFT_SIZES(X) X(24) X(30) X(32) X(36) X(40) X(48) X(60) X(72) X(96) X(180)
@@ -344,6 +428,41 @@ auto inv_z_kernel(int n3) -> decltype(&k_sticks_inverse_z<T, 0>) {
       return &k_sticks_inverse_z<T, 0>;
   }
 }
+
+// register z kernels (even z lengths of the compiled list), else null
+template <typename T>
+auto inv_z_reg_kernel(int n3) -> decltype(&k_sticks_inverse_z_reg<T, 24>) {
+  switch (n3) {
+#define PW_CASE(n)                                                        \
+  case n:                                                                 \
+    if constexpr (n % 2 == 0) return &k_sticks_inverse_z_reg<T, n>; \
+    return nullptr;
+    PW_FFT_SIZES(PW_CASE)
+#undef PW_CASE
+    default:
+      return nullptr;
+  }
+}
+template <typename T>
+auto fwd_z_reg_kernel(int n3) -> decltype(&k_sticks_forward_z_reg<T, 24>) {
+  switch (n3) {
+#define PW_CASE(n)                                                        \
+  case n:                                                                 \
+    if constexpr (n % 2 == 0) return &k_sticks_forward_z_reg<T, n>; \
+    return nullptr;
+    PW_FFT_SIZES(PW_CASE)
+#undef PW_CASE
+    default:
+      return nullptr;
+  }
+}
+
+template <typename T>
+int launch_inv_z(pw_ctx* c, int nb, const typename cx<T>::t* psi, typename cx<T>::t* t1,
+                 const typename cx<T>::t* tw3);
+template <typename T>
+int launch_fwd_z(pw_ctx* c, int nb, const typename cx<T>::t* t2, const typename cx<T>::t* psi,
+                 typename cx<T>::t* out, const T* ek, const typename cx<T>::t* tw3);
 
 template <typename T>
 auto fwd_z_kernel(int n3) -> decltype(&k_sticks_forward_z<T, 0>) {
@@ -492,6 +611,20 @@ static void build_stick_plan(pw_ctx* c, const int* idx) {
   };
   flatten(inv, sp.chunk_off, sp.pt_idx, sp.pt_sz);
   flatten(fw, sp.gchunk_off, sp.gpt_idx, sp.gpt_sz);
+  // (z, stick) table: direct points first, so that a mirror never replaces
+  // one (they could meet only on a kz = n3/2 plane)
+  sp.zs.assign((size_t)n3 * sp.nstick, kZsAbsent);
+  for (int i = 0; i < npw; i++) {
+    const int g = idx[i];
+    sp.zs[(size_t)(g / (h1 * n2)) * sp.nstick + sid[g % (h1 * n2)]] = i;
+  }
+  for (int i = 0; i < npw; i++) {
+    const int g = idx[i], col = g % (h1 * n2), kz = g / (h1 * n2), s = sid[col];
+    if (sp.kx[s] == 0 && sp.ky[s] == 0 && kz != 0) {
+      int& e = sp.zs[(size_t)((n3 - kz) % n3) * sp.nstick + s];
+      if (e == kZsAbsent) e = -(i + 1);
+    }
+  }
 }
 
 template <typename X>
@@ -521,6 +654,11 @@ extern "C" int pw_ctx_create(pw_ctx** out, const int grid[3], const double super
   if (grid[2] >= 32768) return fail(PW_ERR_UNSUPPORTED, "pw_ctx_create: n3 too large");
   pw_ctx* c = new pw_ctx();
   {
+    // register z kernels on request only: at si64_60 (256 bands) they take
+    // 22.8 / 25.9 us (FP32 inverse / forward) against the LDS kernels'
+    // 23.6 / 23.5 us, FP64 35.9 / 43.3 against 37.9 / 36.4 us
+    const char* zr = getenv("PW_ZREG");
+    if (zr && atoi(zr) == 1) c->zreg = true;
     const char* bf = getenv("PW_BF16");
     if (bf) c->bf16_split = (atoi(bf) == 2) ? 2 : (atoi(bf) == 0 ? 0 : 3);
     const char* sl = getenv("PW_SLAB");
@@ -611,6 +749,7 @@ extern "C" int pw_ctx_create(pw_ctx** out, const int grid[3], const double super
   UP(c->d_gpt_idx, c->sp.gpt_idx.data(), c->sp.gpt_idx.size());
   UP(c->d_gpt_sz, c->sp.gpt_sz.data(), c->sp.gpt_sz.size());
   UP(c->d_gchunk_off, c->sp.gchunk_off.data(), c->sp.gchunk_off.size());
+  UP(c->d_zs, c->sp.zs.data(), c->sp.zs.size());
   UP(c->d_pos, c->sp.pos.data(), c->sp.pos.size());
   UP(c->d_mpos, c->sp.mpos.data(), c->sp.mpos.size());
   UP(c->d_ekin, ekin, (size_t)npw);
@@ -647,7 +786,7 @@ extern "C" int pw_ctx_destroy(pw_ctx* c) {
   pw_free_workspace(c);
   exx_free(c);
   if (c->d_gemm_ws) (void)hipFree(c->d_gemm_ws);
-  void* ptrs[] = {c->d_pt_idx, c->d_pt_sz, c->d_chunk_off, c->d_gpt_idx, c->d_gpt_sz, c->d_gchunk_off,
+  void* ptrs[] = {c->d_pt_idx, c->d_pt_sz, c->d_chunk_off, c->d_gpt_idx, c->d_gpt_sz, c->d_gchunk_off, c->d_zs,
                   c->d_pos, c->d_mpos, c->d_ekin, c->d_teter, c->d_ekin_f, c->d_teter_f, c->d_tw64,
                   c->d_tw32, c->d_vtot, c->d_vtot_f, c->d_beta, c->d_betaw, c->d_W, c->d_beta_f,
                   c->d_betaw_f, c->d_W_f, c->d_t1, c->d_t2, c->d_betaw_bf, c->d_W_bf};
@@ -828,11 +967,7 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
     return fail(PW_ERR_UNSUPPORTED, "H*psi slab needs %zu bytes of LDS (> 160 KiB)", lds2);
   const size_t lds1 = (size_t)esz * (c->n3 + 2 * (size_t)sc * c->n3);
   PW_TRY(mark(c, 0));
-  auto kin = inv_z_kernel<T>(c->n3);
-  hipLaunchKernelGGL(kin, dim3(c->sp.nchunk * nb), dim3(256), lds1, c->stream,
-                     (const C*)psi, c->npw, t1, c->n3, c->sp.nstick, sc, c->sp.nchunk, c->d_chunk_off, c->d_pt_idx,
-                     c->d_pt_sz, tw3);
-  PW_HIP(hipGetLastError());
+  PW_TRY(launch_inv_z<T>(c, nb, (const C*)psi, t1, tw3));
   PW_TRY(mark(c, 1));
   // the wide variant only where its y-space fits the LDS
   const bool wide = pl.sb == kSlabWideSB && slab_reg_lds(c, sizeof(C), kSlabWideSB) <= 160 * 1024;
@@ -876,11 +1011,7 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
   }
   PW_HIP(hipGetLastError());
   PW_TRY(mark(c, 2));
-  auto kfw = fwd_z_kernel<T>(c->n3);
-  hipLaunchKernelGGL(kfw, dim3(c->sp.nchunk * nb), dim3(256), lds1, c->stream,
-                     (const C*)t2, (const C*)psi, (C*)a3, c->npw, c->n3, c->sp.nstick, sc, c->sp.nchunk,
-                     c->d_gchunk_off, c->d_gpt_idx, c->d_gpt_sz, ek, tw3);
-  PW_HIP(hipGetLastError());
+  PW_TRY(launch_fwd_z<T>(c, nb, (const C*)t2, (const C*)psi, (C*)a3, ek, tw3));
   PW_TRY(mark(c, 3));
   return PW_OK;
 }
@@ -892,18 +1023,55 @@ bool has_reg_slab(const pw_ctx* c, int dtype, int ly, int sb, bool persist) {
 }
 size_t slab_lds_bytes(const pw_ctx* c, int esz, int rp, int cg) { return slab_lds(c, esz, rp, cg); }
 
+// Kernel 1 over nb bands (psi: npw-strided bands; t1: [band][z][stick]):
+// the register form where the z length has one, else the LDS form.
+template <typename T>
+int launch_inv_z(pw_ctx* c, int nb, const typename cx<T>::t* psi, typename cx<T>::t* t1,
+                 const typename cx<T>::t* tw3) {
+  typedef typename cx<T>::t C;
+  const auto kr = c->zreg ? inv_z_reg_kernel<T>(c->n3) : nullptr;
+  if (kr) {
+    const long nwork = (long)nb * c->sp.nstick, nthr = nwork * (c->n3 % 4 == 0 ? 4 : 2);  // ZLanes<n3>::L
+    hipLaunchKernelGGL(kr, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, c->stream, psi, c->npw, t1,
+                       c->sp.nstick, nwork, c->d_zs);
+  } else {
+    const size_t lds1 = sizeof(C) * (c->n3 + 2 * (size_t)c->sp.sc * c->n3);
+    hipLaunchKernelGGL(inv_z_kernel<T>(c->n3), dim3(c->sp.nchunk * nb), dim3(256), lds1, c->stream, psi, c->npw,
+                       t1, c->n3, c->sp.nstick, c->sp.sc, c->sp.nchunk, c->d_chunk_off, c->d_pt_idx, c->d_pt_sz,
+                       tw3);
+  }
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+// Kernel 3 over nb bands: out = ekin .* psi + gather (ekin non-null), or
+// out += gather.
+template <typename T>
+int launch_fwd_z(pw_ctx* c, int nb, const typename cx<T>::t* t2, const typename cx<T>::t* psi,
+                 typename cx<T>::t* out, const T* ek, const typename cx<T>::t* tw3) {
+  typedef typename cx<T>::t C;
+  const auto kr = c->zreg ? fwd_z_reg_kernel<T>(c->n3) : nullptr;
+  if (kr) {
+    const long nwork = (long)nb * c->sp.nstick, nthr = nwork * (c->n3 % 4 == 0 ? 4 : 2);  // ZLanes<n3>::L
+    hipLaunchKernelGGL(kr, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, c->stream, t2, psi, out, c->npw,
+                       c->sp.nstick, nwork, c->d_zs, ek);
+  } else {
+    const size_t lds1 = sizeof(C) * (c->n3 + 2 * (size_t)c->sp.sc * c->n3);
+    hipLaunchKernelGGL(fwd_z_kernel<T>(c->n3), dim3(c->sp.nchunk * nb), dim3(256), lds1, c->stream, t2, psi, out,
+                       c->npw, c->n3, c->sp.nstick, c->sp.sc, c->sp.nchunk, c->d_gchunk_off, c->d_gpt_idx,
+                       c->d_gpt_sz, ek, tw3);
+  }
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
 // Sphere coefficients -> z-transformed sticks T[band][z][stick] (kernel 1).
 template <typename T>
 int sphere_to_sticks(pw_ctx* c, int nb, const void* psi, void* t1) {
   typedef typename cx<T>::t C;
   const C* tw3 = (const C*)(sizeof(T) == 8 ? (const void*)c->d_tw64 : (const void*)c->d_tw32) + c->n1 + c->n2;
   const size_t lds1 = sizeof(C) * (c->n3 + 2 * (size_t)c->sp.sc * c->n3);
-  auto kin = inv_z_kernel<T>(c->n3);
-  hipLaunchKernelGGL(kin, dim3(c->sp.nchunk * nb), dim3(256), lds1, c->stream,
-                     (const C*)psi, c->npw, (C*)t1, c->n3, c->sp.nstick, c->sp.sc, c->sp.nchunk, c->d_chunk_off, c->d_pt_idx,
-                     c->d_pt_sz, tw3);
-  PW_HIP(hipGetLastError());
-  return PW_OK;
+  return launch_inv_z<T>(c, nb, (const C*)psi, (C*)t1, tw3);
 }
 
 // Sticks -> forward z transform -> out += sqrt(2)-scaled sphere gather.
@@ -912,12 +1080,7 @@ int sticks_to_sphere_add(pw_ctx* c, int nb, const void* t2, void* out) {
   typedef typename cx<T>::t C;
   const C* tw3 = (const C*)(sizeof(T) == 8 ? (const void*)c->d_tw64 : (const void*)c->d_tw32) + c->n1 + c->n2;
   const size_t lds1 = sizeof(C) * (c->n3 + 2 * (size_t)c->sp.sc * c->n3);
-  auto kfw = fwd_z_kernel<T>(c->n3);
-  hipLaunchKernelGGL(kfw, dim3(c->sp.nchunk * nb), dim3(256), lds1, c->stream, (const C*)t2,
-                     (const C*)nullptr, (C*)out, c->npw, c->n3, c->sp.nstick, c->sp.sc, c->sp.nchunk, c->d_gchunk_off,
-                     c->d_gpt_idx, c->d_gpt_sz, (const T*)nullptr, tw3);
-  PW_HIP(hipGetLastError());
-  return PW_OK;
+  return launch_fwd_z<T>(c, nb, (const C*)t2, (const C*)nullptr, (C*)out, (const T*)nullptr, tw3);
 }
 
 template int sphere_to_sticks<double>(pw_ctx*, int, const void*, void*);

@@ -125,7 +125,12 @@ struct StickPlan {
   std::vector<int> gpt_sz;
   // slab positions: kx*n2 + ky, and the mirror position for kx=0,ky>0
   std::vector<int> pos, mpos;     // mpos = -1 when no mirror
+  // per (z, stick), z-major: the sphere index i of that point, -(i+1) for
+  // the conj mirror of c_i (kx = ky = 0 column, kz < 0), kZsAbsent outside
+  // the sphere
+  std::vector<int> zs;
 };
+constexpr int kZsAbsent = (int)0x80000000;
 
 }  // namespace pw
 
@@ -234,6 +239,7 @@ struct pw_ctx {
   // device arrays
   int *d_pt_idx = nullptr, *d_pt_sz = nullptr, *d_chunk_off = nullptr;
   int *d_gpt_idx = nullptr, *d_gpt_sz = nullptr, *d_gchunk_off = nullptr;
+  int* d_zs = nullptr;  // StickPlan::zs
   int *d_pos = nullptr, *d_mpos = nullptr;
   double *d_ekin = nullptr, *d_teter = nullptr;
   float *d_ekin_f = nullptr, *d_teter_f = nullptr;
@@ -274,6 +280,7 @@ struct pw_ctx {
   // bits, six term products) by default, PW_BF16=2 for 2 terms (16 bits,
   // three products), PW_BF16=0 for FP32 MFMA
   int bf16_split = 3;
+  bool zreg = false;  // PW_ZREG=1: the register z kernels where the z length has one
   // Plans per precision ([PW_F64], [PW_F32]), set by PW_SLAB or pw_autotune:
   //   reg: the register-FFT slab kernel where compiled (PW_SLAB=lds: the LDS
   //   kernel); cg / rp / threads: the LDS kernel's y column groups (0 = two

@@ -130,6 +130,30 @@ def test_fp32_persistent_register_slab_kernel(name, nb, monkeypatch):
     c1.close()
 
 
+@pytest.mark.parametrize("name,nb", [("si8_32", 13), ("rand16_48", 5), ("si64_60", 9)])
+def test_register_z_kernels(name, nb, monkeypatch):
+    """The register z-pencil kernels (PW_ZREG=1: each stick's z column over
+    4 lanes, the sphere read and written through the (z, stick) table)
+    against the oracle, FP64 and FP32, and against the LDS z kernels."""
+    import torch
+    from pwdft_develop_amd.ops import Context
+    prob = W.WORKLOADS[name](nb=nb).build()
+    print(f"case: {name} register z kernels nb={nb}", flush=True)
+    ref = O.Hamiltonian(prob).apply_h(prob.psi)
+    monkeypatch.setenv("PW_ZREG", "1")
+    cz = Context(prob)
+    monkeypatch.delenv("PW_ZREG", raising=False)
+    cl = Context(prob)
+    for dt, tol in ((torch.complex128, 1e-12), (torch.complex64, 2e-5)):
+        X = cz.to_device(prob.psi, dt)
+        a = cz.apply_h(X).cpu().numpy()
+        b = cl.apply_h(X).cpu().numpy()
+        assert _rel(a, ref) < tol
+        assert _rel(a, b) < tol
+    cz.close()
+    cl.close()
+
+
 @pytest.mark.parametrize("name,nb", [("si8_32", 32), ("si8_32", 7), ("rand16_48", 9), ("si64_60", 5)])
 def test_fp64_register_slab_kernel(name, nb, monkeypatch):
     """The FP64 register slab kernel (one z plane of 4 bands per workgroup,
