@@ -147,6 +147,18 @@ int pw_orthogonalize(pw_ctx* ctx, int nb, void* X);
  * (environment PW_EIGEN=own: n <= 1024), rocSOLVER's dsyevd beyond. */
 int pw_syevd(pw_ctx* ctx, int n, double* A, int lda, double* w);
 
+/* lapack::Sygvd(itype 1, 'V', 'U') (include/lapack.hpp:118-124) on nbatch
+ * independent problems A x = lambda B x of order 2 <= n <= 256, as PPCGSolveReal
+ * solves one per sub-block (src/EigenSolver/ppcg.cpp:438-460): problem b
+ * has A at A + b strideA (ld lda) and B at B + b strideB (ld ldb), upper
+ * triangles read.  A <- the eigenvectors (x^T B x = 1), B <- the Cholesky
+ * factor U in its upper triangle, w + b strideW <- eigenvalues ascending,
+ * info[b] <- 0 or the failing pivot column of B (device memory
+ * throughout).  Cholesky, C = U^-T A U^-1, the eigensolver of pw_syevd and
+ * x = U^-1 y, each stage one launch for the whole batch. */
+int pw_sygvd_batched(pw_ctx* ctx, int n, double* A, int lda, long strideA, double* B, int ldb, long strideB,
+                     double* w, long strideW, int* info, int nbatch);
+
 /* k-step Lanczos on the device vector v0 (one band, FP64): Ritz values of
  * T_k ascending (host theta[k]) and the upper bound theta_k + |beta_k|.
  * Spectral-bound estimator of the ChebFSI steps the reference only names

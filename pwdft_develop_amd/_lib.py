@@ -70,6 +70,8 @@ def lib():
         "pw_free_workspace": [_vp],
         "pw_set_graphs": [_vp, C.c_int],
         "pw_syevd": [_vp, C.c_int, _vp, C.c_int, _vp],
+        "pw_sygvd_batched": [_vp, C.c_int, _vp, C.c_int, C.c_long, _vp, C.c_int, C.c_long, _vp, C.c_long, _vp,
+                             C.c_int],
         "pw_density": [_vp, C.c_int, C.c_int, _vp, _dp, _vp, C.c_int],
         "pw_band_energy": [_vp, C.c_int, C.c_int, _vp, _dp, C.POINTER(C.c_double)],
         "pw_harris_energy": [_vp, C.c_int, C.c_int, C.c_double, _vp, _vp, _vp, _vp, _vp, C.c_int, _dp, _dp,

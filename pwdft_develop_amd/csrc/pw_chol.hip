@@ -46,8 +46,12 @@ __device__ __forceinline__ void ch_block(int b, int& I, int& J) {
 // (NB blocks of 32 x 32, row-major) <- inverses of U's diagonal blocks
 // (identity beyond n); *info = 0 or the failing column.
 __global__ void __launch_bounds__(512)
-k_potrf(double* __restrict__ S, int n, int ld, double* __restrict__ Dinv, int* __restrict__ info) {
+k_potrf(double* __restrict__ S, int n, int ld, double* __restrict__ Dinv, int* __restrict__ info, long sS, long sD) {
   extern __shared__ double sm[];
+  // one problem per workgroup of a batch (strides sS, sD; info one word each)
+  S += blockIdx.x * sS;
+  Dinv += blockIdx.x * sD;
+  info += blockIdx.x;
   double* Pn = sm;                                // 32 x kChLdp: block row of U (global columns)
   double* Di = Pn + kChB * kChLdp;                // 32 x kChLdd: inverse of the diagonal U block
   double* Dg = Di + kChB * kChLdd;                // kChMaxNB x 32 x kChLdd: diagonal blocks
@@ -283,8 +287,12 @@ k_potrf(double* __restrict__ S, int n, int ld, double* __restrict__ Dinv, int* _
 // one block column at a time.  256 threads = 64 rows of X per workgroup.
 __global__ void __launch_bounds__(256, 2)  // <= 256 registers: two waves per SIMD
 k_trsm_right_upper(double* __restrict__ X, int m, int ldx, int n, const double* __restrict__ U, int ldu,
-                   const double* __restrict__ Dinv) {
+                   const double* __restrict__ Dinv, long sX, long sU, long sD) {
   extern __shared__ double sm[];
+  // problem blockIdx.y of a batch (strides sX, sU, sD)
+  X += blockIdx.y * sX;
+  U += blockIdx.y * sU;
+  Dinv += blockIdx.y * sD;
   double* Us = sm;                                            // (32 J) x kChLdd: U_{<J,J}
   double* Ds = Us + (size_t)kChB * (kChMaxNB - 1) * kChLdd;  // 32 x kChLdd: Dinv_J
   const int NB = (n + kChB - 1) / kChB;
@@ -370,7 +378,7 @@ int chol_upper(pw_ctx* c, int n, double* S, int ld, double* Dinv, int* info) {
   const size_t lds = sizeof(double) * ((size_t)kChB * kChLdp + (size_t)kChB * kChLdd +
                                        (size_t)kChMaxNB * kChB * kChLdd + kChB);
   PW_TRY(ensure_lds((const void*)k_potrf, lds + 256));
-  hipLaunchKernelGGL(k_potrf, dim3(1), dim3(64 * kChWaves), lds, c->stream, S, n, ld, Dinv, info);
+  hipLaunchKernelGGL(k_potrf, dim3(1), dim3(64 * kChWaves), lds, c->stream, S, n, ld, Dinv, info, 0L, 0L);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }
@@ -381,7 +389,29 @@ int trsm_right_upper(pw_ctx* c, int m, int n, double* X, int ldx, const double* 
   const size_t lds = sizeof(double) * ((size_t)kChB * (kChMaxNB - 1) * kChLdd + (size_t)kChB * kChLdd);
   PW_TRY(ensure_lds((const void*)k_trsm_right_upper, lds));
   hipLaunchKernelGGL(k_trsm_right_upper, dim3((m + 63) / 64), dim3(256), lds, c->stream, X, m, ldx, n, U, ldu,
-                     Dinv);
+                     Dinv, 0L, 0L, 0L);
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+// nbatch problems at strides sS (matrices) and sD (Dinv); info[b] per problem.
+int chol_upper_batched(pw_ctx* c, int n, double* S, int ld, long sS, double* Dinv, long sD, int* info, int nbatch) {
+  if (n < 1 || n > kChB * kChMaxNB) return fail(PW_ERR_UNSUPPORTED, "chol_upper: n = %d", n);
+  const size_t lds = sizeof(double) * ((size_t)kChB * kChLdp + (size_t)kChB * kChLdd +
+                                       (size_t)kChMaxNB * kChB * kChLdd + kChB);
+  PW_TRY(ensure_lds((const void*)k_potrf, lds + 256));
+  hipLaunchKernelGGL(k_potrf, dim3(nbatch), dim3(64 * kChWaves), lds, c->stream, S, n, ld, Dinv, info, sS, sD);
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+int trsm_right_upper_batched(pw_ctx* c, int m, int n, double* X, int ldx, long sX, const double* U, int ldu,
+                             long sU, const double* Dinv, long sD, int nbatch) {
+  if (n < 1 || n > kChB * kChMaxNB) return fail(PW_ERR_UNSUPPORTED, "trsm_right_upper: n = %d", n);
+  const size_t lds = sizeof(double) * ((size_t)kChB * (kChMaxNB - 1) * kChLdd + (size_t)kChB * kChLdd);
+  PW_TRY(ensure_lds((const void*)k_trsm_right_upper, lds));
+  hipLaunchKernelGGL(k_trsm_right_upper, dim3((m + 63) / 64, nbatch), dim3(256), lds, c->stream, X, m, ldx, n, U,
+                     ldu, Dinv, sX, sU, sD);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }

@@ -59,7 +59,16 @@ __device__ __forceinline__ double block_sum(double v, double* sred) {
   return s;
 }
 
-__global__ void k_sym_from_upper(double* __restrict__ A, int n, int lda) {
+// Per-matrix strides of a batch of independent problems (all 0 for one
+// problem): A the matrix, v the d, e and tau vectors, w the eigenvalues, T
+// the compact-WY factors, Z the eigenvector blocks.  The batch index is
+// blockIdx.y, or blockIdx.x in the single-workgroup k_sytrd_reg.
+struct EigBatch {
+  long A = 0, v = 0, w = 0, T = 0, Z = 0;
+};
+
+__global__ void k_sym_from_upper(double* __restrict__ A, int n, int lda, long sA) {
+  A += blockIdx.y * sA;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)n * n; e += (long)gridDim.x * blockDim.x) {
     const int i = (int)(e % n), j = (int)(e / n);
     if (i > j) A[i + (size_t)lda * j] = A[j + (size_t)lda * i];
@@ -359,8 +368,12 @@ __device__ __forceinline__ void tr_column(const d4v (&a)[kTr2Slots][4], const in
 
 __global__ void __launch_bounds__(512)
 k_sytrd_reg(double* __restrict__ A, int n, int lda, double* __restrict__ d, double* __restrict__ e,
-            double* __restrict__ tau) {
+            double* __restrict__ tau, EigBatch bs) {
   extern __shared__ double sm[];
+  A += blockIdx.x * bs.A;
+  d += blockIdx.x * bs.v;
+  e += blockIdx.x * bs.v;
+  tau += blockIdx.x * bs.v;
   double* xb = sm;                             // 2 x 256: column k (buffer k % 2)
   double* sw = xb + 512;                       // 256: w
   double* tab = sw + 256;                      // kTr2TabRows x 32: partial sums of A v
@@ -632,8 +645,11 @@ __device__ void tri_bounds(const double* d, const double* e, int n, double* sred
 // Eigenvalue j = (block * waves + wave) of T by multisection: 64 Sturm counts
 // per round narrow the bracket 65-fold.  blockDim = 256.
 __global__ void __launch_bounds__(256)
-k_stebz(const double* __restrict__ dg, const double* __restrict__ eg, int n, double* __restrict__ w) {
+k_stebz(const double* __restrict__ dg, const double* __restrict__ eg, int n, double* __restrict__ w, EigBatch bs) {
   extern __shared__ double smd[];
+  dg += blockIdx.y * bs.v;
+  eg += blockIdx.y * bs.v;
+  w += blockIdx.y * bs.w;
   double* d = smd;
   double* e2 = d + n;
   __shared__ double sred[16];
@@ -865,8 +881,12 @@ k_stein(const double* __restrict__ dg, const double* __restrict__ eg, const doub
 // the n workgroups spread over the CUs.  9 n doubles of LDS.
 __global__ void __launch_bounds__(64)
 k_stein_lds(const double* __restrict__ dg, const double* __restrict__ eg, const double* __restrict__ w, int n,
-            double* __restrict__ Zt) {
+            double* __restrict__ Zt, EigBatch bs) {
   extern __shared__ double sde[];
+  dg += blockIdx.y * bs.v;
+  eg += blockIdx.y * bs.v;
+  w += blockIdx.y * bs.w;
+  Zt += blockIdx.y * bs.Z;
   double* d = sde;
   double* e = d + n;
   double* S = e + n;
@@ -914,8 +934,11 @@ __device__ __forceinline__ void stage_v_chunk(const double* __restrict__ A, int 
 // blockDim = 256.
 __global__ void __launch_bounds__(256)
 k_larft(const double* __restrict__ A, int lda, const double* __restrict__ tau, int n, int nbw,
-        double* __restrict__ Tg) {
+        double* __restrict__ Tg, EigBatch bs) {
   extern __shared__ double sm[];
+  A += blockIdx.y * bs.A;
+  tau += blockIdx.y * bs.v;
+  Tg += blockIdx.y * bs.T;
   const int lvs = nbw + 1;
   double* Vs = sm;                                 // n x nbw (rows kb.. used)
   double* Gs = Vs + (size_t)n * lvs;               // nbw x nbw: V^T V
@@ -965,8 +988,11 @@ k_larft(const double* __restrict__ A, int lda, const double* __restrict__ tau, i
 // in LDS; no per-reflector reductions.  blockDim = 256.
 __global__ void __launch_bounds__(256)
 k_ormtr(const double* __restrict__ A, int lda, const double* __restrict__ Tg, int n, int nbw,
-        double* __restrict__ Z, int ldz) {
+        double* __restrict__ Z, int ldz, EigBatch bs) {
   extern __shared__ double sm[];
+  A += blockIdx.y * bs.A;
+  Tg += blockIdx.y * bs.T;
+  Z += blockIdx.y * bs.Z;
   double* zs = sm;                                 // n x kOrmCols, row-major
   const int lvs = nbw + 1;                         // padded row stride of Vs
   double* Vs = zs + (size_t)n * kOrmCols;          // n x nbw, row-major (row r, reflector j)
@@ -1079,14 +1105,14 @@ int eig_sym(pw_ctx* c, int n, double* A, int lda, double* w) {
   double* S = Zt + nn;
   double* Tw = S + 7 * nn;
   const int g1 = (int)std::min<size_t>((nn + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_sym_from_upper, dim3(g1), dim3(256), 0, s, A, n, lda);
+  hipLaunchKernelGGL(k_sym_from_upper, dim3(g1), dim3(256), 0, s, A, n, lda, 0L);
   // panel width: the V and W panels (2 nbk n doubles) fit LDS next to v and
   // the partial sums (n = 256: 32; 512: 16; 1024: 8)
   if (n <= 32 * kTr2MaxNB) {
     const size_t lds_reg = sizeof(double) * (3 * 256 + (size_t)kTr2TabRows * 32 + kTr2Waves +
                                              (size_t)kTr2MaxNB * 32 * kTr2Ld + (size_t)kTr2Waves * 32 * 33);
     PW_TRY(ensure_lds((const void*)k_sytrd_reg, lds_reg));
-    hipLaunchKernelGGL(k_sytrd_reg, dim3(1), dim3(64 * kTr2Waves), lds_reg, s, A, n, lda, d, e, tau);
+    hipLaunchKernelGGL(k_sytrd_reg, dim3(1), dim3(64 * kTr2Waves), lds_reg, s, A, n, lda, d, e, tau, EigBatch{});
   } else {
     const int nbk = std::max(1, std::min(32, 8192 / n));
     const size_t lds_trd = sizeof(double) * ((size_t)2 * nbk * n + n + 1024 + 2 * (size_t)nbk);
@@ -1097,13 +1123,13 @@ int eig_sym(pw_ctx* c, int n, double* A, int lda, double* w) {
     const size_t lds_t = sizeof(double) * ((size_t)n * (nbw + 1) + 2 * (size_t)nbw * nbw);
     PW_TRY(ensure_lds((const void*)k_larft, lds_t));
     hipLaunchKernelGGL(k_larft, dim3(nchunk), dim3(256), lds_t, s, (const double*)A, lda, (const double*)tau, n,
-                       nbw, Tw);
+                       nbw, Tw, EigBatch{});
   }
   hipLaunchKernelGGL(k_stebz, dim3((n + 3) / 4), dim3(256), sizeof(double) * 2 * n, s, (const double*)d,
-                     (const double*)e, n, w);
+                     (const double*)e, n, w, EigBatch{});
   if (n <= kSteinLdsMaxN)
     hipLaunchKernelGGL(k_stein_lds, dim3(n), dim3(64), sizeof(double) * 9 * n, s, (const double*)d,
-                       (const double*)e, (const double*)w, n, Zt);
+                       (const double*)e, (const double*)w, n, Zt, EigBatch{});
   else
     hipLaunchKernelGGL(k_stein, dim3((n + 63) / 64), dim3(64), sizeof(double) * 2 * n, s, (const double*)d,
                        (const double*)e, (const double*)w, n, Zt, S);
@@ -1112,10 +1138,137 @@ int eig_sym(pw_ctx* c, int n, double* A, int lda, double* w) {
                                            2 * (size_t)nbw * kOrmCols);
   PW_TRY(ensure_lds((const void*)k_ormtr, lds_orm));
   hipLaunchKernelGGL(k_ormtr, dim3((n + kOrmCols - 1) / kOrmCols), dim3(256), lds_orm, s, (const double*)A, lda,
-                     (const double*)Tw, n, nbw, Zt, n);
+                     (const double*)Tw, n, nbw, Zt, n, EigBatch{});
   PW_HIP(hipGetLastError());
   PW_HIP(hipMemcpy2DAsync(A, sizeof(double) * lda, Zt, sizeof(double) * n, sizeof(double) * n, n,
                           hipMemcpyDeviceToDevice, s));
+  return PW_OK;
+}
+
+// ------------------------------------------------------------------------
+// Batched generalized problems A x = lambda B x (Sygvd itype 1, 'V', 'U';
+// the PPCG sub-block problems of ppcg.cpp:438-460), order n <= 256:
+// B = U^T U (k_potrf), C = U^{-T} A U^{-1} by two right triangular solves
+// around a transpose, the eigenpairs of C by the kernels above with a batch
+// index, x = U^{-1} y (k_backsub_upper).  Only the upper triangles of A and
+// B are read.
+
+// T_b = A_b^T, n x n (T with leading dimension n)
+__global__ void k_transpose_batched(const double* __restrict__ A, int lda, long sA, double* __restrict__ T, int n) {
+  A += blockIdx.y * sA;
+  T += (size_t)blockIdx.y * n * n;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < (long)n * n; e += (long)gridDim.x * blockDim.x) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    T[i + (size_t)n * j] = A[j + (size_t)lda * i];
+  }
+}
+
+// Z = U^{-1} V by block back substitution, Z_I = Dinv_I (V_I - sum_{K>I}
+// U_IK Z_K) with Dinv_I = U_II^{-1} (row-major 32 x 32 blocks from
+// k_potrf); one workgroup per (problem, 32-column tile), the tile of Z in
+// LDS (n x 33).  Z may alias V.
+__global__ void __launch_bounds__(256)
+k_backsub_upper(const double* __restrict__ U, int ldu, long sU, const double* __restrict__ Dinv, long sD,
+                const double* V, int ldv, long sV, double* Z, int ldz, long sZ, int n) {
+  extern __shared__ double zs[];  // n x 33 (rows of the tile), then the 32 x 33 block R
+  double* rs = zs + (size_t)n * 33;
+  const int t = threadIdx.x, j0 = blockIdx.x * 32, nc = min(32, n - j0);
+  U += blockIdx.y * sU;
+  Dinv += blockIdx.y * sD;
+  V += blockIdx.y * sV;
+  Z += blockIdx.y * sZ;
+  const int NB = (n + 31) / 32;
+  for (int I = NB - 1; I >= 0; I--) {
+    const int r0 = 32 * I, nr = min(32, n - r0);
+    for (int e = t; e < 32 * 32; e += 256) {
+      const int i = e >> 5, jj = e & 31;
+      double acc = 0.0;
+      if (i < nr && jj < nc) {
+        acc = V[(r0 + i) + (size_t)ldv * (j0 + jj)];
+        for (int k = r0 + 32; k < n; k++) acc -= U[(r0 + i) + (size_t)ldu * k] * zs[k * 33 + jj];
+      }
+      rs[i * 33 + jj] = acc;
+    }
+    __syncthreads();
+    const double* Di = Dinv + (size_t)I * 32 * 32;
+    for (int e = t; e < 32 * 32; e += 256) {
+      const int i = e >> 5, jj = e & 31;
+      double acc = 0.0;
+      for (int q = 0; q < 32; q++) acc += Di[i * 32 + q] * rs[q * 33 + jj];
+      if (i < nr) zs[(r0 + i) * 33 + jj] = acc;
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < n * 32; e += 256) {
+    const int i = e % n, jj = e / n;
+    if (jj < nc) Z[i + (size_t)ldz * (j0 + jj)] = zs[i * 33 + jj];
+  }
+}
+
+int sygvd_batched(pw_ctx* c, int n, double* A, int lda, long sA, double* B, int ldb, long sB, double* w, long sw,
+                  int* info, int nbatch) {
+  // n >= 2: the tridiagonalisation ends on a 2 x 2 block
+  if (n < 2 || n > 32 * kTr2MaxNB) return fail(PW_ERR_UNSUPPORTED, "sygvd_batched: n = %d (2 .. 256)", n);
+  if (nbatch < 1) return PW_OK;
+  hipStream_t s = c->stream;
+  const size_t nn = (size_t)n * n, ndi = chol_dinv_doubles(n);
+  const int nbw = std::max(1, std::min(32, 8192 / n));
+  const int nchunk = (n - 2 + nbw - 1) / nbw;
+  const size_t ntw = (size_t)std::max(1, nchunk) * nbw * nbw;
+  const size_t per = ndi + nn + nn + 3 * (size_t)n + ntw;  // Dinv, C, Zt, d e tau, Tw
+  const size_t need = sizeof(double) * per * nbatch;
+  if (need > c->geig_ws_bytes) {
+    if (c->d_geig_ws) (void)hipFree(c->d_geig_ws);
+    c->d_geig_ws = nullptr;
+    c->geig_ws_bytes = 0;
+    PW_HIP(hipMalloc(&c->d_geig_ws, need));
+    c->geig_ws_bytes = need;
+  }
+  double* Dinv = (double*)c->d_geig_ws;
+  double* C = Dinv + ndi * nbatch;
+  double* Zt = C + nn * nbatch;
+  double* d = Zt + nn * nbatch;
+  double* e = d + (size_t)n * nbatch;
+  double* tau = e + (size_t)n * nbatch;
+  double* Tw = tau + (size_t)n * nbatch;
+  const int g1 = (int)std::min<size_t>((nn + 255) / 256, 4096);
+  PW_TRY(chol_upper_batched(c, n, B, ldb, sB, Dinv, (long)ndi, info, nbatch));
+  hipLaunchKernelGGL(k_sym_from_upper, dim3(g1, nbatch), dim3(256), 0, s, A, n, lda, sA);
+  PW_TRY(trsm_right_upper_batched(c, n, n, A, lda, sA, B, ldb, sB, Dinv, (long)ndi, nbatch));  // A U^{-1}
+  hipLaunchKernelGGL(k_transpose_batched, dim3(g1, nbatch), dim3(256), 0, s, (const double*)A, lda, sA, C, n);
+  PW_TRY(trsm_right_upper_batched(c, n, n, C, n, (long)nn, B, ldb, sB, Dinv, (long)ndi, nbatch));  // U^{-T} A U^{-1}
+  hipLaunchKernelGGL(k_sym_from_upper, dim3(g1, nbatch), dim3(256), 0, s, C, n, n, (long)nn);
+  EigBatch bs;
+  bs.A = (long)nn;
+  bs.v = n;
+  bs.w = sw;
+  bs.T = (long)ntw;
+  bs.Z = (long)nn;
+  const size_t lds_reg = sizeof(double) * (3 * 256 + (size_t)kTr2TabRows * 32 + kTr2Waves +
+                                           (size_t)kTr2MaxNB * 32 * kTr2Ld + (size_t)kTr2Waves * 32 * 33);
+  PW_TRY(ensure_lds((const void*)k_sytrd_reg, lds_reg));
+  hipLaunchKernelGGL(k_sytrd_reg, dim3(nbatch), dim3(64 * kTr2Waves), lds_reg, s, C, n, n, d, e, tau, bs);
+  if (nchunk > 0) {
+    const size_t lds_t = sizeof(double) * ((size_t)n * (nbw + 1) + 2 * (size_t)nbw * nbw);
+    PW_TRY(ensure_lds((const void*)k_larft, lds_t));
+    hipLaunchKernelGGL(k_larft, dim3(nchunk, nbatch), dim3(256), lds_t, s, (const double*)C, n, (const double*)tau,
+                       n, nbw, Tw, bs);
+  }
+  hipLaunchKernelGGL(k_stebz, dim3((n + 3) / 4, nbatch), dim3(256), sizeof(double) * 2 * n, s, (const double*)d,
+                     (const double*)e, n, w, bs);
+  hipLaunchKernelGGL(k_stein_lds, dim3(n, nbatch), dim3(64), sizeof(double) * 9 * n, s, (const double*)d,
+                     (const double*)e, (const double*)w, n, Zt, bs);
+  const size_t lds_orm = sizeof(double) * ((size_t)n * kOrmCols + (size_t)n * (nbw + 1) + (size_t)nbw * nbw +
+                                           2 * (size_t)nbw * kOrmCols);
+  PW_TRY(ensure_lds((const void*)k_ormtr, lds_orm));
+  hipLaunchKernelGGL(k_ormtr, dim3((n + kOrmCols - 1) / kOrmCols, nbatch), dim3(256), lds_orm, s, (const double*)C,
+                     n, (const double*)Tw, n, nbw, Zt, n, bs);
+  // x = U^{-1} y into A
+  const size_t lds_b = sizeof(double) * ((size_t)n * 33 + 32 * 33);
+  PW_TRY(ensure_lds((const void*)k_backsub_upper, lds_b));
+  hipLaunchKernelGGL(k_backsub_upper, dim3((n + 31) / 32, nbatch), dim3(256), lds_b, s, (const double*)B, ldb, sB,
+                     (const double*)Dinv, (long)ndi, (const double*)Zt, n, (long)nn, A, lda, sA, n);
+  PW_HIP(hipGetLastError());
   return PW_OK;
 }
 

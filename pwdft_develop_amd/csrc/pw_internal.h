@@ -186,6 +186,15 @@ int trsm_right_upper(pw_ctx* c, int m, int n, double* X, int ldx, const double* 
 int chol_inv_max_n();
 size_t chol_dinv_doubles(int n);
 int sym_eig(pw_ctx* c, int n, double* A, int lda, double* w, double* work, int* info);
+// batched forms (problem b at the given strides)
+int chol_upper_batched(pw_ctx* c, int n, double* S, int ld, long sS, double* Dinv, long sD, int* info, int nbatch);
+int trsm_right_upper_batched(pw_ctx* c, int m, int n, double* X, int ldx, long sX, const double* U, int ldu,
+                             long sU, const double* Dinv, long sD, int nbatch);
+// A x = lambda B x, n <= 256 (Sygvd itype 1 'V' 'U'): eigenvectors into A,
+// U into B's upper triangle, eigenvalues ascending into w, info[b] 0 or the
+// failing pivot column of B
+int sygvd_batched(pw_ctx* c, int n, double* A, int lda, long sA, double* B, int ldb, long sB, double* w, long sw,
+                  int* info, int nbatch);
 // Scaled Chebyshev recurrence coefficients of a degree-m filter damping
 // [a, b] and scaled at aL: coef[3 (k-1) ..] = (c, s1, s2) of step k, with
 // Y_1 = s1 (H - c) X and Y_k = s1 (H - c) Y_{k-1} - s2 Y_{k-2}.
@@ -265,6 +274,8 @@ struct pw_ctx {
   double* d_lz = nullptr;  // Lanczos vectors v, v_prev, f (3 x 2 npw)
   void* d_eig_ws = nullptr;  // dense eigensolver scratch, grown on demand
   size_t eig_ws_bytes = 0;
+  void* d_geig_ws = nullptr;  // batched generalized eigensolver scratch, grown on demand
+  size_t geig_ws_bytes = 0;
   int* d_info = nullptr;
   double* d_chol = nullptr;  // inverses of the Cholesky factor's diagonal blocks
   int ws_nb = 0;

@@ -114,10 +114,10 @@ static inline int grid1(long n) { return (int)std::max<long>(1, std::min<long>((
 struct PpcgWs {
   double *S = nullptr, *AS = nullptr, *Xtemp = nullptr, *tmp = nullptr;
   double *XTX = nullptr, *AM = nullptr, *BM = nullptr, *eig = nullptr, *E = nullptr;
-  double *nrm = nullptr, *res = nullptr, *w = nullptr;
+  double *nrm = nullptr, *res = nullptr, *w = nullptr, *dinv = nullptr;
   int* info = nullptr;
   void free_all() {
-    void* p[] = {S, AS, Xtemp, tmp, XTX, AM, BM, eig, E, nrm, res, w, info};
+    void* p[] = {S, AS, Xtemp, tmp, XTX, AM, BM, eig, E, nrm, res, w, dinv, info};
     for (void* q : p)
       if (q) (void)hipFree(q);
   }
@@ -163,6 +163,8 @@ int ppcg_core(pw_ctx* c, int m, const PpcgOps& ops, int nb, int numEig, int scfI
   ALLOC(W.res, width);
   ALLOC(W.w, width);
   ALLOC(W.info, nsb + 1);
+  const bool own_chol = width <= chol_inv_max_n();
+  if (own_chol) ALLOC(W.dinv, chol_dinv_doubles(width));
 #undef ALLOC
 #define TRY(call)                          \
   do {                                     \
@@ -184,14 +186,28 @@ int ppcg_core(pw_ctx* c, int m, const PpcgOps& ops, int nb, int numEig, int scfI
   HIPTRY(hipMemsetAsync(W.S, 0, sizeof(double) * 3 * blk, s));
   HIPTRY(hipMemsetAsync(W.AS, 0, sizeof(double) * 3 * blk, s));
   HIPTRY(hipMemcpyAsync(X, Xio, sizeof(double) * blk, hipMemcpyDeviceToDevice, s));
-  // Cholesky QR of X (and of AX by the same factor when it exists): U in XTX
-  auto chol_qr = [&](bool with_ax) -> int {
-    PW_TRY(gemm_tn<double>(c, m, width, width, X, m, X, m, W.XTX, width, nullptr, 0.0));
-    PW_BLAS(rocsolver_dpotrf(c->blas, rocblas_fill_upper, width, W.XTX, width, W.info));
+  // Cholesky QR of X (and of AX by the same factor when it exists): U in
+  // XTX.  Up to 256 bands the factor and the solves are k_potrf and
+  // k_trsm_right_upper, and the pivot check (info in W.info[nsb]) waits for
+  // the next host read (the residual norms of the next iteration, or the
+  // end); beyond, rocSOLVER potrf with an immediate check and rocBLAS trsm.
+  auto chol_check = [&]() -> int {
     int h = 0;
-    PW_HIP(hipMemcpyAsync(&h, W.info, sizeof(int), hipMemcpyDeviceToHost, s));
+    PW_HIP(hipMemcpyAsync(&h, W.info + nsb, sizeof(int), hipMemcpyDeviceToHost, s));
     PW_HIP(hipStreamSynchronize(s));
     if (h != 0) return fail(PW_ERR_LINALG, "Potrf: overlap matrix is not positive definite (info = %d)", h);
+    return PW_OK;
+  };
+  auto chol_qr = [&](bool with_ax) -> int {
+    PW_TRY(gemm_tn<double>(c, m, width, width, X, m, X, m, W.XTX, width, nullptr, 0.0));
+    if (own_chol) {
+      PW_TRY(chol_upper(c, width, W.XTX, width, W.dinv, W.info + nsb));
+      PW_TRY(trsm_right_upper(c, m, width, X, m, W.XTX, width, W.dinv));
+      if (with_ax) PW_TRY(trsm_right_upper(c, m, width, AX, m, W.XTX, width, W.dinv));
+      return PW_OK;
+    }
+    PW_BLAS(rocsolver_dpotrf(c->blas, rocblas_fill_upper, width, W.XTX, width, W.info + nsb));
+    PW_TRY(chol_check());
     PW_BLAS(rocblas_dtrsm(c->blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                           rocblas_diagonal_non_unit, m, width, &one, W.XTX, width, X, m));
     if (with_ax)
@@ -199,7 +215,10 @@ int ppcg_core(pw_ctx* c, int m, const PpcgOps& ops, int nb, int numEig, int scfI
                             rocblas_diagonal_non_unit, m, width, &one, W.XTX, width, AX, m));
     return PW_OK;
   };
-  if (scfIter == 1) TRY(chol_qr(false));
+  if (scfIter == 1) {
+    TRY(chol_qr(false));
+    if (own_chol) TRY(chol_check());
+  }
   TRY(ops.apply_h(width, X, AX));
 
   int iter = 0;
@@ -218,6 +237,7 @@ int ppcg_core(pw_ctx* c, int m, const PpcgOps& ops, int nb, int numEig, int scfI
                        (const double*)W.XTX, width + 1, width, W.res);
     HIPTRY(hipMemcpyAsync(hres.data(), W.res, sizeof(double) * width, hipMemcpyDeviceToHost, s));
     HIPTRY(hipStreamSynchronize(s));
+    if (own_chol && iter > 1) TRY(chol_check());  // the previous iteration's Cholesky QR
     resMax = *std::max_element(hres.begin(), hres.begin() + std::max(numEig, 1));
     // W = T R, AW = A W (ppcg.cpp:272-293)
     HIPTRY(hipMemsetAsync(Wb, 0, sizeof(double) * blk, s));
@@ -272,15 +292,25 @@ int ppcg_core(pw_ctx* c, int m, const PpcgOps& ops, int nb, int numEig, int scfI
     {
       const int nfull = (width % sbs) ? nsb - 1 : nsb;
       const int dimf = (numSet == 3) ? 3 * sbs : 2 * sbs;
-      if (nfull > 0)
-        BLASTRY(rocsolver_dsygvd_strided_batched(c->blas, rocblas_eform_ax, rocblas_evect_original,
-                                                 rocblas_fill_upper, dimf, W.AM, sb3, (rocblas_stride)mat, W.BM,
-                                                 sb3, (rocblas_stride)mat, W.eig, sb3, W.E, sb3, W.info, nfull));
+      // orders up to 256: the batched solver of pw_eig.hip (one launch per
+      // stage for all sub-blocks); beyond, rocSOLVER
+      if (nfull > 0) {
+        if (dimf <= chol_inv_max_n())
+          TRY(sygvd_batched(c, dimf, W.AM, sb3, (long)mat, W.BM, sb3, (long)mat, W.eig, sb3, W.info, nfull));
+        else
+          BLASTRY(rocsolver_dsygvd_strided_batched(c->blas, rocblas_eform_ax, rocblas_evect_original,
+                                                   rocblas_fill_upper, dimf, W.AM, sb3, (rocblas_stride)mat, W.BM,
+                                                   sb3, (rocblas_stride)mat, W.eig, sb3, W.E, sb3, W.info, nfull));
+      }
       if (nfull < nsb) {
         const int sb = width % sbs, dim = (numSet == 3) ? 3 * sb : 2 * sb;
-        BLASTRY(rocsolver_dsygvd(c->blas, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, dim,
-                                 W.AM + mat * nfull, 3 * sb, W.BM + mat * nfull, 3 * sb, W.eig + (size_t)sb3 * nfull,
-                                 W.E + (size_t)sb3 * nfull, W.info + nfull));
+        if (dim <= chol_inv_max_n())
+          TRY(sygvd_batched(c, dim, W.AM + mat * nfull, 3 * sb, 0, W.BM + mat * nfull, 3 * sb, 0,
+                            W.eig + (size_t)sb3 * nfull, 0, W.info + nfull, 1));
+        else
+          BLASTRY(rocsolver_dsygvd(c->blas, rocblas_eform_ax, rocblas_evect_original, rocblas_fill_upper, dim,
+                                   W.AM + mat * nfull, 3 * sb, W.BM + mat * nfull, 3 * sb,
+                                   W.eig + (size_t)sb3 * nfull, W.E + (size_t)sb3 * nfull, W.info + nfull));
       }
       std::vector<int> hinfo(nsb);
       HIPTRY(hipMemcpyAsync(hinfo.data(), W.info, sizeof(int) * nsb, hipMemcpyDeviceToHost, s));
@@ -335,6 +365,7 @@ int ppcg_core(pw_ctx* c, int m, const PpcgOps& ops, int nb, int numEig, int scfI
   HIPTRY(hipMemcpyAsync(eigval, W.w, sizeof(double) * width, hipMemcpyDeviceToHost, s));
   HIPTRY(hipMemcpyAsync(resval, W.res, sizeof(double) * width, hipMemcpyDeviceToHost, s));
   HIPTRY(hipStreamSynchronize(s));
+  if (own_chol) TRY(chol_check());  // the last iteration's Cholesky QR
   if (iters) *iters = iter;
 #undef TRY
 #undef HIPTRY

@@ -311,6 +311,9 @@ extern "C" int pw_free_workspace(pw_ctx* c) {
   if (c->d_eig_ws) (void)hipFree(c->d_eig_ws);
   c->d_eig_ws = nullptr;
   c->eig_ws_bytes = 0;
+  if (c->d_geig_ws) (void)hipFree(c->d_geig_ws);
+  c->d_geig_ws = nullptr;
+  c->geig_ws_bytes = 0;
   // the captured filter graph refers to the workspace: drop it too
   if (c->fg_exec) (void)hipGraphExecDestroy(c->fg_exec);
   c->fg_exec = nullptr;
@@ -610,6 +613,16 @@ extern "C" int pw_syevd(pw_ctx* c, int n, double* A, int lda, double* w) {
   if (hipStreamSynchronize(c->stream) != hipSuccess && rc == PW_OK) rc = fail(PW_ERR_HIP, "pw_syevd: kernel failed");
   (void)hipFree(work);
   (void)hipFree(info);
+  return rc;
+}
+
+extern "C" int pw_sygvd_batched(pw_ctx* c, int n, double* A, int lda, long strideA, double* B, int ldb,
+                                long strideB, double* w, long strideW, int* info, int nbatch) {
+  if (!c || !A || !B || !w || !info || n < 1 || lda < n || ldb < n || nbatch < 1)
+    return fail(PW_ERR_ARG, "pw_sygvd_batched: bad argument");
+  int rc = sygvd_batched(c, n, A, lda, strideA, B, ldb, strideB, w, strideW, info, nbatch);
+  if (hipStreamSynchronize(c->stream) != hipSuccess && rc == PW_OK)
+    rc = fail(PW_ERR_HIP, "pw_sygvd_batched: kernel failed");
   return rc;
 }
 

@@ -241,6 +241,21 @@ class Context:
         check(_lib.lib().pw_syevd(self.h, n, C.c_void_p(Z.data_ptr()), n, C.c_void_p(w.data_ptr())))
         return w, Z
 
+    def sygvd_batched(self, A, B):
+        """lapack::Sygvd(1, 'V', 'U') on a batch of device FP64 problems A[b] x =
+        lambda B[b] x (shape (nbatch, n, n), each column-major as in syevd).
+        Returns (eigenvalues (nbatch, n) ascending, eigenvectors, Cholesky
+        factors, info) as new tensors."""
+        t = self.torch
+        nbatch, n = A.shape[0], A.shape[1]
+        Z = A.detach().clone().contiguous()
+        U = B.detach().clone().contiguous()
+        w = t.empty((nbatch, n), dtype=t.float64, device=self.device)
+        info = t.empty(nbatch, dtype=t.int32, device=self.device)
+        check(_lib.lib().pw_sygvd_batched(self.h, n, C.c_void_p(Z.data_ptr()), n, n * n, C.c_void_p(U.data_ptr()), n,
+                                          n * n, C.c_void_p(w.data_ptr()), n, C.c_void_p(info.data_ptr()), nbatch))
+        return w, Z, U, info
+
     def set_graphs(self, on=True):
         """Capture/replay the Chebyshev filter as a hipGraph (default on)."""
         check(_lib.lib().pw_set_graphs(self.h, 1 if on else 0))

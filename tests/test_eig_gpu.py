@@ -111,3 +111,53 @@ def test_reads_upper_triangle_only(ctx):
     wa, Va = _solve(ctx, A)
     assert np.array_equal(w, wa)
     assert np.array_equal(V, Va)
+
+
+@pytest.mark.parametrize("n,nbatch", [(2, 3), (17, 5), (64, 8), (96, 8), (100, 3), (256, 2)])
+def test_sygvd_batched(ctx, n, nbatch):
+    """pw_sygvd_batched (lapack::Sygvd itype 1, 'V', 'U' on a batch, as the
+    PPCG sub-block problems): eigenvalues against scipy.linalg.eigh(A, B)
+    1e-11 relative to ||A||, x^T B x = I and A x = lambda B x to 1e-10,
+    only the upper triangles read (garbage planted below the diagonal)."""
+    import scipy.linalg as sl
+    import torch
+    print(f"case: sygvd batched n={n} nbatch={nbatch}", flush=True)
+    rng = np.random.default_rng(100 + n)
+    As, Bs = [], []
+    for b in range(nbatch):
+        A = _rand_sym(n, 1000 * n + b)
+        G = rng.standard_normal((n, n))
+        Bm = G @ G.T / n + np.eye(n)
+        As.append(A)
+        Bs.append(Bm)
+    # torch row-major [b, j, i] = column-major element (i, j); plant values
+    # in the strictly lower triangle (i > j), which must not be read
+    At = np.stack([a.T.copy() for a in As])
+    Bt = np.stack([bm.T.copy() for bm in Bs])
+    il = np.tril_indices(n, -1)
+    for arr in (At, Bt):
+        for b in range(nbatch):
+            arr[b][il[1], il[0]] = 7.0
+    w, Z, U, info = ctx.sygvd_batched(torch.from_numpy(At).to(ctx.device), torch.from_numpy(Bt).to(ctx.device))
+    w, Z, info = w.cpu().numpy(), Z.cpu().numpy(), info.cpu().numpy()
+    assert np.all(info == 0)
+    for b in range(nbatch):
+        A, Bm = As[b], Bs[b]
+        ref = sl.eigh(A, Bm, eigvals_only=True)
+        scale = max(1.0, np.abs(A).max())
+        assert np.abs(w[b] - ref).max() < 1e-11 * scale * max(1, n / 64)
+        X = Z[b].T  # columns = eigenvectors
+        assert np.abs(X.T @ Bm @ X - np.eye(n)).max() < 1e-10 * max(1, n / 64)
+        assert np.abs(A @ X - Bm @ X * w[b]).max() < 1e-10 * scale * max(1, n / 64)
+
+
+def test_sygvd_batched_reports_indefinite_b(ctx):
+    """A B that is not positive definite in one problem of the batch:
+    info[b] = the failing pivot column (1-based), 0 for the others."""
+    import torch
+    n = 40
+    As = np.stack([_rand_sym(n, 5 + b).T.copy() for b in range(3)])
+    Bs = np.stack([np.eye(n) for _ in range(3)])
+    Bs[1][12, 12] = -1.0  # column 13 (1-based) fails
+    w, Z, U, info = ctx.sygvd_batched(torch.from_numpy(As).to(ctx.device), torch.from_numpy(Bs).to(ctx.device))
+    assert list(info.cpu().numpy()) == [0, 13, 0]
