@@ -90,8 +90,14 @@ __device__ __forceinline__ int nn_col(int lane, int r) {
 template <typename T, int TM>
 __global__ void __launch_bounds__(256)
 k_gemm_tn_partial(const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb, int K, int n1, int n2,
-                  int kc, T* __restrict__ P) {
+                  int kc, T* __restrict__ P, const T* const* __restrict__ Ab, const T* const* __restrict__ Bb,
+                  long sP) {
   typedef typename acc4<T>::t V;
+  if (Ab) {  // product blockIdx.y of a batch: operands from the pointer tables
+    A = Ab[blockIdx.y];
+    B = Bb[blockIdx.y];
+    P += blockIdx.y * sP;
+  }
   constexpr int TB = BM * TM, NQ = 8 * TM, NA = 2 * TM;
   extern __shared__ __align__(16) unsigned char gemm_smem[];
   T (*As)[BK + 1] = reinterpret_cast<T (*)[BK + 1]>(gemm_smem);
@@ -281,7 +287,11 @@ k_gemm_tn_partial_bf(const float* __restrict__ A, int lda, const float* __restri
 // loads in flight at a time.
 template <typename T>
 __global__ void k_gemm_tn_reduce(const T* __restrict__ P, int nsplit, int n1, int n2, T* __restrict__ C, int ldc,
-                                 const T* __restrict__ rowscale, T beta) {
+                                 const T* __restrict__ rowscale, T beta, T* const* __restrict__ Cb, long sP) {
+  if (Cb) {  // product blockIdx.y of a batch
+    C = Cb[blockIdx.y];
+    P += blockIdx.y * sP;
+  }
   const long total = (long)n1 * n2;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
     const int i = (int)(e % n1), j = (int)(e / n1);
@@ -830,7 +840,7 @@ int gemm_tn_ws(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B
     hipLaunchKernelGGL(k_gemv_t_partial<T>, dim3(n1, nsplit), dim3(256), 0, c->stream, A, lda, B, K, kc, n1, P);
     const int g = std::max(1, std::min((n1 + 255) / 256, 4096));
     hipLaunchKernelGGL(k_gemm_tn_reduce<T>, dim3(g), dim3(256), 0, c->stream, (const T*)P, nsplit, n1, 1, C, ldc,
-                       rowscale, beta);
+                       rowscale, beta, (T* const*)nullptr, 0L);
     PW_HIP(hipGetLastError());
     return PW_OK;
   }
@@ -879,7 +889,7 @@ int gemm_tn_ws(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B
       const long total = (long)n1 * n2;
       const int g = (int)std::min<long>((total + 255) / 256, 4096);
       hipLaunchKernelGGL(k_gemm_tn_reduce<T>, dim3(g), dim3(256), 0, c->stream, (const T*)P, nsplit, n1, n2, C, ldc,
-                         rowscale, beta);
+                         rowscale, beta, (T* const*)nullptr, 0L);
       PW_HIP(hipGetLastError());
       return PW_OK;
     }
@@ -888,15 +898,15 @@ int gemm_tn_ws(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B
   if (tm == 2) {
     PW_TRY(ensure_lds((const void*)k_gemm_tn_partial<T, 2>, lds));
     hipLaunchKernelGGL((k_gemm_tn_partial<T, 2>), dim3(ti * tj * nsplit), dim3(256), lds, c->stream, A, lda, B, ldb,
-                       K, n1, n2, kc, P);
+                       K, n1, n2, kc, P, (const T* const*)nullptr, (const T* const*)nullptr, 0L);
   } else {
     hipLaunchKernelGGL((k_gemm_tn_partial<T, 1>), dim3(ti * tj * nsplit), dim3(256), lds, c->stream, A, lda, B, ldb,
-                       K, n1, n2, kc, P);
+                       K, n1, n2, kc, P, (const T* const*)nullptr, (const T* const*)nullptr, 0L);
   }
   const long total = (long)n1 * n2;
   const int g = (int)std::min<long>((total + 255) / 256, 4096);
   hipLaunchKernelGGL(k_gemm_tn_reduce<T>, dim3(g), dim3(256), 0, c->stream, (const T*)P, nsplit, n1, n2, C, ldc,
-                     rowscale, beta);
+                     rowscale, beta, (T* const*)nullptr, 0L);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }
@@ -1012,6 +1022,35 @@ template int gemm_nn_ep<double>(pw_ctx*, int, int, int, const double*, int, cons
                                 const double*, const double*, double*, int, const double*, int, bool, int);
 template int gemm_nn_ep<float>(pw_ctx*, int, int, int, const float*, int, const float*, int, const float*,
                                const float*, const float*, float*, int, const float*, int, bool, int);
+// nbatch products C_b = A_b^T B_b (n1 x n2, K rows; A_b, B_b, C_b from
+// device pointer tables dA, dB, dC, one leading dimension each): one
+// launch of split-K partial tiles for all products and one fixed-order
+// reduction, each about 2048 workgroups in all.
+template <typename T>
+int gemm_tn_batched(pw_ctx* c, int K, int n1, int n2, const T* const* dA, int lda, const T* const* dB, int ldb,
+                    T* const* dC, int ldc, int nbatch) {
+  if (n1 <= 0 || n2 <= 0 || nbatch <= 0) return PW_OK;
+  const int ti = (n1 + BM - 1) / BM, tj = (n2 + BN - 1) / BN, tiles = ti * tj;
+  int nsplit = std::max(1, std::min((2048 + tiles * nbatch - 1) / (tiles * nbatch), (K + 255) / 256));
+  int kc = (K + nsplit - 1) / nsplit;
+  kc = (kc + BK - 1) / BK * BK;
+  nsplit = (K + kc - 1) / kc;
+  const long per = (long)n1 * n2 * nsplit;  // partials of one product
+  PW_TRY(grow_gemm_ws(c, sizeof(T) * (size_t)per * nbatch));
+  T* P = (T*)c->d_gemm_ws;
+  const size_t lds = sizeof(T) * 2 * (size_t)BM * (BK + 1);
+  hipLaunchKernelGGL((k_gemm_tn_partial<T, 1>), dim3(tiles * nsplit, nbatch), dim3(256), lds, c->stream,
+                     (const T*)nullptr, lda, (const T*)nullptr, ldb, K, n1, n2, kc, P, dA, dB, per);
+  const long total = (long)n1 * n2;
+  const int g = (int)std::max<long>(1, std::min<long>((total + 255) / 256, 4096));
+  hipLaunchKernelGGL(k_gemm_tn_reduce<T>, dim3(g, nbatch), dim3(256), 0, c->stream, (const T*)P, nsplit, n1, n2,
+                     (T*)nullptr, ldc, (const T*)nullptr, (T)0, dC, per);
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+template int gemm_tn_batched<double>(pw_ctx*, int, int, int, const double* const*, int, const double* const*, int,
+                                     double* const*, int, int);
+
 template int gemm_tn_ws<double>(pw_ctx*, int, int, int, const double*, int, const double*, int, double*, int,
                                 const double*, double, int, void*, size_t);
 template int gemm_tn_ws<float>(pw_ctx*, int, int, int, const float*, int, const float*, int, float*, int,

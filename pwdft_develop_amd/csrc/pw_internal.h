@@ -145,6 +145,11 @@ namespace pw {
 template <typename T>
 int gemm_tn(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B, int ldb, T* C, int ldc,
             const T* rowscale, T beta, int split = 0);
+// nbatch products C_b = A_b^T B_b of one shape, operands from device
+// pointer tables (FP64 instantiated): two launches for the whole batch.
+template <typename T>
+int gemm_tn_batched(pw_ctx* c, int K, int n1, int n2, const T* const* dA, int lda, const T* const* dB, int ldb,
+                    T* const* dC, int ldc, int nbatch);
 // gemm_tn with the split-K partials in a caller's buffer (ws, ws_bytes): no
 // allocation; fewer K splits when the usual count does not fit, and
 // PW_ERR_UNSUPPORTED when one split does not.  ws == null: the context's

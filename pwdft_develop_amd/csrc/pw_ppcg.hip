@@ -79,8 +79,8 @@ __global__ void k_pp_resnorm(const double* __restrict__ nrm2, const double* __re
 
 // Sign rule of the oracle: column j's largest-magnitude entry (first index
 // on ties) made positive; one block per column.
-__global__ void k_pp_sign_fix(double* __restrict__ Q, int n, int ld) {
-  double* q = Q + (size_t)blockIdx.x * ld;
+__global__ void k_pp_sign_fix(double* __restrict__ Q, int n, int ld, long sQ) {
+  double* q = Q + blockIdx.y * sQ + (size_t)blockIdx.x * ld;  // column blockIdx.x of matrix blockIdx.y
   double best = -1.0;
   int bi = 0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
@@ -111,13 +111,68 @@ __global__ void k_pp_sign_fix(double* __restrict__ Q, int n, int ld) {
 
 static inline int grid1(long n) { return (int)std::max<long>(1, std::min<long>((n + 255) / 256, 8192)); }
 
+// PPCG block update (ppcg.cpp:462-527) of sub-block blockIdx.y, rows
+// kPpRows blockIdx.x ..: with c_x, c_w, c_p the first sb columns of rows
+// 0, sb and 2 sb of the sub-block's eigenvector matrix Z (ld ldz),
+// p <- p c_p + w c_w (w c_w when numSet = 2), ap likewise, x <- p + x c_x,
+// ax <- ap + ax c_x.  The sub-block's columns start at column off0 + sbs k
+// of each m-row block; the tile's rows and the coefficients are staged in
+// LDS first, so the block is updated in place.  sb <= kPpMaxSb.
+constexpr int kPpRows = 32, kPpMaxSb = 32;
+__global__ void __launch_bounds__(256)
+k_pp_update(double* __restrict__ X, double* __restrict__ Wb, double* __restrict__ P, double* __restrict__ AX,
+            double* __restrict__ AW, double* __restrict__ AP, int m, int sb, int sbs, int col0,
+            const double* __restrict__ Z, long sZ, int ldz, int numSet) {
+  constexpr int LD = kPpMaxSb + 1;
+  __shared__ double cf[3][kPpMaxSb][LD];    // c_x, c_w, c_p (row i, column j)
+  __shared__ double rw[6][kPpRows][LD];     // x, w, p, ax, aw, ap rows of the tile
+  const int t = threadIdx.x, k = blockIdx.y, r0 = blockIdx.x * kPpRows;
+  const int nr = min(kPpRows, m - r0);
+  const double* Zk = Z + k * sZ;
+  for (int e = t; e < 3 * sb * sb; e += 256) {
+    const int q = e / (sb * sb), ij = e - q * sb * sb, j = ij / sb, i = ij - j * sb;
+    cf[q][i][j] = Zk[(size_t)q * sb + i + (size_t)ldz * j];
+  }
+  const size_t off = (size_t)m * (col0 + (size_t)sbs * k);
+  double* arr[6] = {X + off, Wb + off, P + off, AX + off, AW + off, AP + off};
+  for (int e = t; e < 6 * sb * kPpRows; e += 256) {
+    const int a = e / (sb * kPpRows), rem = e - a * sb * kPpRows, j = rem / kPpRows, r = rem - j * kPpRows;
+    rw[a][r][j] = (r < nr && (numSet == 3 || (a != 2 && a != 5))) ? arr[a][r0 + r + (size_t)m * j] : 0.0;
+  }
+  __syncthreads();
+  for (int e = t; e < sb * kPpRows; e += 256) {
+    const int j = e / kPpRows, r = e - j * kPpRows;
+    if (r >= nr) continue;
+    double pn = 0.0, apn = 0.0, xs = 0.0, axs = 0.0;
+    if (numSet == 3)
+      for (int i = 0; i < sb; i++) {
+        pn += rw[2][r][i] * cf[2][i][j];
+        apn += rw[5][r][i] * cf[2][i][j];
+      }
+    for (int i = 0; i < sb; i++) {
+      pn += rw[1][r][i] * cf[1][i][j];
+      apn += rw[4][r][i] * cf[1][i][j];
+    }
+    for (int i = 0; i < sb; i++) {
+      xs += rw[0][r][i] * cf[0][i][j];
+      axs += rw[3][r][i] * cf[0][i][j];
+    }
+    const size_t g = r0 + r + (size_t)m * j;
+    arr[2][g] = pn;
+    arr[5][g] = apn;
+    arr[0][g] = pn + xs;
+    arr[3][g] = apn + axs;
+  }
+}
+
 struct PpcgWs {
   double *S = nullptr, *AS = nullptr, *Xtemp = nullptr, *tmp = nullptr;
   double *XTX = nullptr, *AM = nullptr, *BM = nullptr, *eig = nullptr, *E = nullptr;
   double *nrm = nullptr, *res = nullptr, *w = nullptr, *dinv = nullptr;
   int* info = nullptr;
+  void** gtab = nullptr;  // device pointer tables of the sub-block Gram products
   void free_all() {
-    void* p[] = {S, AS, Xtemp, tmp, XTX, AM, BM, eig, E, nrm, res, w, dinv, info};
+    void* p[] = {S, AS, Xtemp, tmp, XTX, AM, BM, eig, E, nrm, res, w, dinv, info, gtab};
     for (void* q : p)
       if (q) (void)hipFree(q);
   }
@@ -183,6 +238,67 @@ int ppcg_core(pw_ctx* c, int m, const PpcgOps& ops, int nb, int numEig, int scfI
   double *X = W.S, *Wb = W.S + blk, *P = W.S + 2 * blk;
   double *AX = W.AS, *AW = W.AS + blk, *AP = W.AS + 2 * blk;
   const double one = 1.0;
+  // sub-block Gram products (ppcg.cpp:353-436) as pointer tables, built once:
+  // per (numSet 2 or 3) x (full sub-blocks, the ragged last one) a list of
+  // (a, b, C) with C = a^T b written into block (r0, c0) of AM or BM
+  const int nfull = (width % sbs) ? nsb - 1 : nsb, sbr = width % sbs;
+  struct GramList {
+    std::vector<const double*> a, b;
+    std::vector<double*> cc;
+    size_t at = 0;  // offset in the device table
+  };
+  GramList gl[2][2];  // [numSet - 2][ragged]
+  for (int ns = 2; ns <= 3; ns++)
+    for (int k = 0; k < nsb; k++) {
+      const bool rag = k >= nfull;
+      const int sb = rag ? sbr : sbs, ld = 3 * sb;
+      const size_t off = (size_t)m * sbs * k;
+      const double *x = X + off, *w = Wb + off, *p = P + off, *ax = AX + off, *aw = AW + off, *ap = AP + off;
+      double* Ak = W.AM + mat * k;
+      double* Bk = W.BM + mat * k;
+      GramList& L = gl[ns - 2][rag ? 1 : 0];
+      auto add = [&](const double* a, const double* b, double* M, int r0, int c0) {
+        L.a.push_back(a);
+        L.b.push_back(b);
+        L.cc.push_back(M + r0 + (size_t)ld * c0);
+      };
+      add(x, ax, Ak, 0, 0);
+      add(w, aw, Ak, sb, sb);
+      add(x, aw, Ak, 0, sb);
+      add(x, x, Bk, 0, 0);
+      add(w, w, Bk, sb, sb);
+      add(x, w, Bk, 0, sb);
+      if (ns == 3) {
+        add(p, ap, Ak, 2 * sb, 2 * sb);
+        add(x, ap, Ak, 0, 2 * sb);
+        add(w, ap, Ak, sb, 2 * sb);
+        add(p, p, Bk, 2 * sb, 2 * sb);
+        add(x, p, Bk, 0, 2 * sb);
+        add(w, p, Bk, sb, 2 * sb);
+      }
+    }
+  {
+    std::vector<void*> h;
+    for (auto& row : gl)
+      for (GramList& L : row) {
+        L.at = h.size();
+        for (size_t i = 0; i < L.a.size(); i++) h.push_back((void*)L.a[i]);
+        for (size_t i = 0; i < L.a.size(); i++) h.push_back((void*)L.b[i]);
+        for (size_t i = 0; i < L.a.size(); i++) h.push_back((void*)L.cc[i]);
+      }
+    if (hipMalloc((void**)&W.gtab, sizeof(void*) * std::max<size_t>(1, h.size())) != hipSuccess)
+      return cleanup(fail(PW_ERR_NOMEM, "pw_ppcg: cannot allocate the Gram tables"));
+    HIPTRY(hipMemcpy(W.gtab, h.data(), sizeof(void*) * h.size(), hipMemcpyHostToDevice));
+  }
+  auto gram = [&](int ns, int rag) -> int {
+    const GramList& L = gl[ns - 2][rag];
+    const int cnt = (int)L.a.size();
+    if (cnt == 0) return PW_OK;
+    const int sb = rag ? sbr : sbs;
+    void** t = W.gtab + L.at;
+    return gemm_tn_batched<double>(c, m, sb, sb, (const double* const*)t, m, (const double* const*)(t + cnt), m,
+                                   (double* const*)(t + 2 * cnt), 3 * sb, cnt);
+  };
   HIPTRY(hipMemsetAsync(W.S, 0, sizeof(double) * 3 * blk, s));
   HIPTRY(hipMemsetAsync(W.AS, 0, sizeof(double) * 3 * blk, s));
   HIPTRY(hipMemcpyAsync(X, Xio, sizeof(double) * blk, hipMemcpyDeviceToDevice, s));
@@ -263,34 +379,10 @@ int ppcg_core(pw_ctx* c, int m, const PpcgOps& ops, int nb, int numEig, int scfI
     // sub-block Gram matrices (ppcg.cpp:353-436): block k at AM + k mat, ld 3 sb_k
     HIPTRY(hipMemsetAsync(W.AM, 0, sizeof(double) * mat * nsb, s));
     HIPTRY(hipMemsetAsync(W.BM, 0, sizeof(double) * mat * nsb, s));
-    for (int k = 0; k < nsb; k++) {
-      const int sb = (k == nsb - 1 && width % sbs) ? width % sbs : sbs;
-      const int ld = 3 * sb;
-      const size_t off = (size_t)m * sbs * k;
-      double* Ak = W.AM + mat * k;
-      double* Bk = W.BM + mat * k;
-      const double *x = X + off, *w = Wb + off, *p = P + off, *ax = AX + off, *aw = AW + off, *ap = AP + off;
-      auto g = [&](const double* a, const double* b, double* M, int r0, int c0) {
-        return gemm_tn<double>(c, m, sb, sb, a, m, b, m, M + r0 + (size_t)ld * c0, ld, nullptr, 0.0);
-      };
-      TRY(g(x, ax, Ak, 0, 0));
-      TRY(g(w, aw, Ak, sb, sb));
-      TRY(g(x, aw, Ak, 0, sb));
-      TRY(g(x, x, Bk, 0, 0));
-      TRY(g(w, w, Bk, sb, sb));
-      TRY(g(x, w, Bk, 0, sb));
-      if (numSet == 3) {
-        TRY(g(p, ap, Ak, 2 * sb, 2 * sb));
-        TRY(g(x, ap, Ak, 0, 2 * sb));
-        TRY(g(w, ap, Ak, sb, 2 * sb));
-        TRY(g(p, p, Bk, 2 * sb, 2 * sb));
-        TRY(g(x, p, Bk, 0, 2 * sb));
-        TRY(g(w, p, Bk, sb, 2 * sb));
-      }
-    }
+    TRY(gram(numSet, 0));
+    TRY(gram(numSet, 1));
     // all full sub-blocks in one batched Sygvd, the ragged last one alone
     {
-      const int nfull = (width % sbs) ? nsb - 1 : nsb;
       const int dimf = (numSet == 3) ? 3 * sbs : 2 * sbs;
       // orders up to 256: the batched solver of pw_eig.hip (one launch per
       // stage for all sub-blocks); beyond, rocSOLVER
@@ -318,34 +410,51 @@ int ppcg_core(pw_ctx* c, int m, const PpcgOps& ops, int nb, int numEig, int scfI
       for (int k = 0; k < nsb; k++)
         if (hinfo[k] != 0) return cleanup(fail(PW_ERR_LINALG, "sygvd returned with info = %d", hinfo[k]));
     }
-    // block updates (ppcg.cpp:462-527)
-    for (int k = 0; k < nsb; k++) {
-      const int sb = (k == nsb - 1 && width % sbs) ? width % sbs : sbs;
-      const int ld = 3 * sb, dim = (numSet == 3) ? 3 * sb : 2 * sb;
-      const size_t off = (size_t)m * sbs * k;
-      double* Zk = W.AM + mat * k;
-      hipLaunchKernelGGL(k_pp_sign_fix, dim3(sb), dim3(256), 0, s, Zk, dim, ld);
-      const double* cx = Zk;
-      const double* cw = Zk + sb;
-      const double* cp = Zk + 2 * sb;
-      double *x = X + off, *w = Wb + off, *p = P + off, *ax = AX + off, *aw = AW + off, *ap = AP + off;
-      if (numSet == 3) {
-        TRY(gemm_nn<double>(c, m, sb, sb, 1.0, p, m, cp, ld, 0.0, W.tmp, m));
-        TRY(gemm_nn<double>(c, m, sb, sb, 1.0, w, m, cw, ld, 1.0, W.tmp, m));
-        HIPTRY(hipMemcpyAsync(p, W.tmp, sizeof(double) * m * sb, hipMemcpyDeviceToDevice, s));
-        TRY(gemm_nn<double>(c, m, sb, sb, 1.0, ap, m, cp, ld, 0.0, W.tmp, m));
-        TRY(gemm_nn<double>(c, m, sb, sb, 1.0, aw, m, cw, ld, 1.0, W.tmp, m));
-        HIPTRY(hipMemcpyAsync(ap, W.tmp, sizeof(double) * m * sb, hipMemcpyDeviceToDevice, s));
-      } else {
-        TRY(gemm_nn<double>(c, m, sb, sb, 1.0, w, m, cw, ld, 0.0, p, m));
-        TRY(gemm_nn<double>(c, m, sb, sb, 1.0, aw, m, cw, ld, 0.0, ap, m));
+    // block updates (ppcg.cpp:462-527): the sign rule on every sub-block's
+    // eigenvectors, then p, ap, x, ax in one fused pass per sub-block shape
+    if (nfull > 0)
+      hipLaunchKernelGGL(k_pp_sign_fix, dim3(sbs, nfull), dim3(256), 0, s, W.AM, (numSet == 3) ? 3 * sbs : 2 * sbs,
+                         3 * sbs, (long)mat);
+    if (nfull < nsb)
+      hipLaunchKernelGGL(k_pp_sign_fix, dim3(sbr, 1), dim3(256), 0, s, W.AM + mat * nfull,
+                         (numSet == 3) ? 3 * sbr : 2 * sbr, 3 * sbr, 0L);
+    if (sbs <= kPpMaxSb) {
+      const int rt = (m + kPpRows - 1) / kPpRows;
+      if (nfull > 0)
+        hipLaunchKernelGGL(k_pp_update, dim3(rt, nfull), dim3(256), 0, s, X, Wb, P, AX, AW, AP, m, sbs, sbs, 0,
+                           (const double*)W.AM, (long)mat, 3 * sbs, numSet);
+      if (nfull < nsb)
+        hipLaunchKernelGGL(k_pp_update, dim3(rt, 1), dim3(256), 0, s, X, Wb, P, AX, AW, AP, m, sbr, sbs,
+                           sbs * nfull, (const double*)(W.AM + mat * nfull), 0L, 3 * sbr, numSet);
+      HIPTRY(hipGetLastError());
+    } else {
+      for (int k = 0; k < nsb; k++) {
+        const int sb = (k == nsb - 1 && width % sbs) ? width % sbs : sbs;
+        const int ld = 3 * sb;
+        const size_t off = (size_t)m * sbs * k;
+        double* Zk = W.AM + mat * k;
+        const double* cx = Zk;
+        const double* cw = Zk + sb;
+        const double* cp = Zk + 2 * sb;
+        double *x = X + off, *w = Wb + off, *p = P + off, *ax = AX + off, *aw = AW + off, *ap = AP + off;
+        if (numSet == 3) {
+          TRY(gemm_nn<double>(c, m, sb, sb, 1.0, p, m, cp, ld, 0.0, W.tmp, m));
+          TRY(gemm_nn<double>(c, m, sb, sb, 1.0, w, m, cw, ld, 1.0, W.tmp, m));
+          HIPTRY(hipMemcpyAsync(p, W.tmp, sizeof(double) * m * sb, hipMemcpyDeviceToDevice, s));
+          TRY(gemm_nn<double>(c, m, sb, sb, 1.0, ap, m, cp, ld, 0.0, W.tmp, m));
+          TRY(gemm_nn<double>(c, m, sb, sb, 1.0, aw, m, cw, ld, 1.0, W.tmp, m));
+          HIPTRY(hipMemcpyAsync(ap, W.tmp, sizeof(double) * m * sb, hipMemcpyDeviceToDevice, s));
+        } else {
+          TRY(gemm_nn<double>(c, m, sb, sb, 1.0, w, m, cw, ld, 0.0, p, m));
+          TRY(gemm_nn<double>(c, m, sb, sb, 1.0, aw, m, cw, ld, 0.0, ap, m));
+        }
+        HIPTRY(hipMemcpyAsync(W.tmp, p, sizeof(double) * m * sb, hipMemcpyDeviceToDevice, s));
+        TRY(gemm_nn<double>(c, m, sb, sb, 1.0, x, m, cx, ld, 1.0, W.tmp, m));
+        HIPTRY(hipMemcpyAsync(x, W.tmp, sizeof(double) * m * sb, hipMemcpyDeviceToDevice, s));
+        HIPTRY(hipMemcpyAsync(W.tmp, ap, sizeof(double) * m * sb, hipMemcpyDeviceToDevice, s));
+        TRY(gemm_nn<double>(c, m, sb, sb, 1.0, ax, m, cx, ld, 1.0, W.tmp, m));
+        HIPTRY(hipMemcpyAsync(ax, W.tmp, sizeof(double) * m * sb, hipMemcpyDeviceToDevice, s));
       }
-      HIPTRY(hipMemcpyAsync(W.tmp, p, sizeof(double) * m * sb, hipMemcpyDeviceToDevice, s));
-      TRY(gemm_nn<double>(c, m, sb, sb, 1.0, x, m, cx, ld, 1.0, W.tmp, m));
-      HIPTRY(hipMemcpyAsync(x, W.tmp, sizeof(double) * m * sb, hipMemcpyDeviceToDevice, s));
-      HIPTRY(hipMemcpyAsync(W.tmp, ap, sizeof(double) * m * sb, hipMemcpyDeviceToDevice, s));
-      TRY(gemm_nn<double>(c, m, sb, sb, 1.0, ax, m, cx, ld, 1.0, W.tmp, m));
-      HIPTRY(hipMemcpyAsync(ax, W.tmp, sizeof(double) * m * sb, hipMemcpyDeviceToDevice, s));
     }
     TRY(chol_qr(true));  // ppcg.cpp:530-544
   } while (iter < eigMaxIter && resMax > eigTolerance);

@@ -1,8 +1,10 @@
 """PPCG on the GPU against the FP64 oracle's restatement of PPCGSolveReal
 (ppcg.cpp:119-576, same algorithm): Ritz values 1e-10 Ha and residual norms
 1e-8 after the same iterations, with full and ragged sub-blocks; the GPU's
-small Sygvd problems go through rocSOLVER, the oracle's through its own
-Cholesky + Householder/QL solver, with the same eigenvector sign rule."""
+small Sygvd problems go through its batched generalized eigensolver
+(pw_sygvd_batched), the oracle's through its own Cholesky + Householder/QL
+solver, with the same eigenvector sign rule.  Sub-blocks of up to 32
+columns take the fused block update, wider ones the GEMM path."""
 import numpy as np
 import pytest
 
@@ -37,13 +39,15 @@ def test_ppcg_converges_to_dense(si8, golden):
     ctx.close()
 
 
-def test_ppcg_rand16():
+@pytest.mark.parametrize("sb", [16, 24, 40])
+def test_ppcg_rand16(sb):
     from pwdft_develop_amd.ops import Context
     prob = W.rand16_48(nb=40).build()
-    print("case: rand16_48 nb=40 ppcg 3 iterations sb=16", flush=True)
+    print(f"case: rand16_48 nb=40 ppcg 3 iterations sb={sb}", flush=True)
     ctx = Context(prob)
     X = ctx.to_device(prob.psi)
-    _, e, r, it = ctx.ppcg(X, max_iter=3, sb_size=16)
-    _, eo, ro, _ = O.Hamiltonian(prob).ppcg(prob.psi, max_iter=3, sb_size=16)
+    _, e, r, it = ctx.ppcg(X, max_iter=3, sb_size=sb)
+    _, eo, ro, _ = O.Hamiltonian(prob).ppcg(prob.psi, max_iter=3, sb_size=sb)
     assert np.abs(e - eo).max() < 1e-10
+    assert np.abs(r - ro).max() < 1e-8
     ctx.close()
