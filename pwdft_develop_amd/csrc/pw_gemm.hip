@@ -776,38 +776,53 @@ k_gemv_t_partial(const T* __restrict__ A, int lda, const T* __restrict__ b, int 
   if (threadIdx.x == 0) P[(size_t)split * n1 + i] = red[0];
 }
 
-// y = alpha A x + beta y, A m x k: workgroup of 256 threads per 64 rows; wave
-// w sums columns j = w (mod 4) for its 64 rows (16 loads in flight per lane),
-// the four partial rows meet in LDS in a fixed order.
-template <typename T>
-__global__ void __launch_bounds__(256)
+// y = alpha A x + beta y, A m x k: workgroup of NW waves per 64 rows; wave
+// w sums columns j = w (mod NW) for its 64 rows (16 loads in flight per
+// lane), the NW partial rows meet in LDS in a fixed order.  NW = 16 for
+// k >= 128: one round of loads per wave, four times the waves in flight
+// (the single-column products of the Lanczos steps are latency-bound).
+template <typename T, int NW>
+__global__ void __launch_bounds__(64 * NW)
 k_gemv_n(const T* __restrict__ A, int lda, const T* __restrict__ x, T* __restrict__ y, int m, int k, T alpha,
          T beta) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int row = blockIdx.x * 64 + lane;
   T s = 0;
   if (row < m) {
-    for (int j0 = w; j0 < k; j0 += 16 * 4) {
+    for (int j0 = w; j0 < k; j0 += 16 * NW) {
       T a[16];
 #pragma unroll
       for (int u = 0; u < 16; u++) {
-        const int j = j0 + 4 * u;
+        const int j = j0 + NW * u;
         a[u] = j < k ? A[(size_t)j * lda + row] : (T)0;
       }
 #pragma unroll
       for (int u = 0; u < 16; u++) {
-        const int j = j0 + 4 * u;
+        const int j = j0 + NW * u;
         if (j < k) s += a[u] * x[j];
       }
     }
   }
-  __shared__ T red[4][64];
+  __shared__ T red[NW][64];
   red[w][lane] = s;
   __syncthreads();
   if (w == 0 && row < m) {
-    const T v = alpha * (red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane]);
+    T v = 0;
+#pragma unroll
+    for (int q = 0; q < NW; q++) v += red[q][lane];
+    v *= alpha;
     y[row] = (beta == (T)0) ? v : v + beta * y[row];
   }
+}
+
+template <typename T>
+static void launch_gemv_n(pw_ctx* c, const T* A, int lda, const T* x, T* y, int m, int k, T alpha, T beta) {
+  if (k >= 128)
+    hipLaunchKernelGGL((k_gemv_n<T, 16>), dim3((m + 63) / 64), dim3(1024), 0, c->stream, A, lda, x, y, m, k, alpha,
+                       beta);
+  else
+    hipLaunchKernelGGL((k_gemv_n<T, 4>), dim3((m + 63) / 64), dim3(256), 0, c->stream, A, lda, x, y, m, k, alpha,
+                       beta);
 }
 
 // The context's split-K partial buffer, grown to need bytes (allocates).
@@ -922,7 +937,7 @@ int gemm_nn(pw_ctx* c, int m, int n, int k, T alpha, const T* A, int lda, const 
             int ldy) {
   if (m <= 0 || n <= 0) return PW_OK;
   if (n == 1) {
-    hipLaunchKernelGGL(k_gemv_n<T>, dim3((m + 63) / 64), dim3(256), 0, c->stream, A, lda, B, Y, m, k, alpha, beta);
+    launch_gemv_n<T>(c, A, lda, B, Y, m, k, alpha, beta);
     PW_HIP(hipGetLastError());
     return PW_OK;
   }
@@ -944,7 +959,7 @@ int gemm_nn_ep(pw_ctx* c, int m, int n, int k, const T* A, int lda, const T* B, 
   bool done = false;
   if (!cheb && n == 1 && k > 0 && H == out) {
     // one column (the Lanczos vector): out += A b
-    hipLaunchKernelGGL(k_gemv_n<T>, dim3((m + 63) / 64), dim3(256), 0, c->stream, A, lda, B, out, m, k, (T)1, (T)1);
+    launch_gemv_n<T>(c, A, lda, B, out, m, k, (T)1, (T)1);
     done = true;
   }
   if (k <= 0) {

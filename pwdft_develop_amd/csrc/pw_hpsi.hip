@@ -962,6 +962,17 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
     }
     if (!found) { rp = 1; cg = 1; }
   }
+  // one or two bands (the Lanczos steps): the LDS kernel with all y columns
+  // in one group and all x row pairs in one step at 512 threads spreads each
+  // plane's transforms over the workgroup, where the register kernel leaves
+  // most of its threads idle (si64_60, one band: FP32 13.6 us against 24.0,
+  // FP64 15.7 against 29.3)
+  const int nkc = c->sp.kxmax + 1, npr = (c->n2 + 1) / 2;
+  const bool few = nb <= 2 && slab_lds(c, esz, npr, nkc) <= 160 * 1024;
+  if (few) {
+    cg = nkc;
+    rp = npr;
+  }
   const size_t lds2 = slab_lds(c, esz, rp, cg);
   if (lds2 > 160 * 1024)
     return fail(PW_ERR_UNSUPPORTED, "H*psi slab needs %zu bytes of LDS (> 160 KiB)", lds2);
@@ -972,9 +983,10 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
   // the wide variant only where its y-space fits the LDS
   const bool wide = pl.sb == kSlabWideSB && slab_reg_lds(c, sizeof(C), kSlabWideSB) <= 160 * 1024;
   // the persistent grid where that variant exists, else the one-item grid
-  slab_reg_fn<T> kreg = pl.reg && pl.persist && !wide ? slab_reg_kernel<T>(c->n1, c->n2, pl.ly, 0, true) : nullptr;
+  slab_reg_fn<T> kreg =
+      pl.reg && !few && pl.persist && !wide ? slab_reg_kernel<T>(c->n1, c->n2, pl.ly, 0, true) : nullptr;
   const bool persist = kreg != nullptr;
-  if (pl.reg && !kreg) kreg = slab_reg_kernel<T>(c->n1, c->n2, wide ? 1 : pl.ly, wide ? kSlabWideSB : 0);
+  if (pl.reg && !few && !kreg) kreg = slab_reg_kernel<T>(c->n1, c->n2, wide ? 1 : pl.ly, wide ? kSlabWideSB : 0);
   if (kreg) {
     // one z plane of SB bands per workgroup (or per item of the persistent
     // grid), V plane + y-space in LDS
@@ -1006,6 +1018,7 @@ static int local_kinetic(pw_ctx* c, int nb, const void* psi, void* a3) {
     // 256 threads 18.9 / 43.2 ms)
     int threads = cg > 0 ? (lds2 > 80 * 1024 ? 1024 : 512) : 256;
     if (pl.threads > 0) threads = pl.threads;
+    if (few) threads = 512;
     hipLaunchKernelGGL(kslab, dim3(c->n3, nb), dim3(threads), lds2, c->stream, (const C*)t1, t2, c->n1,
                        c->n2, c->n3, c->sp.nstick, c->sp.kxmax, c->d_pos, c->d_mpos, vt, tw1, tw2, rp, cg);
   }
