@@ -793,6 +793,9 @@ extern "C" int pw_ctx_destroy(pw_ctx* c) {
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->blas) rocblas_destroy_handle(c->blas);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+  if (c->side) (void)hipStreamDestroy(c->side);
   for (auto& e : c->ev_pool) (void)hipEventDestroy(e);
   delete c;
   return PW_OK;

@@ -240,6 +240,10 @@ struct pw_ctx {
   size_t gemm_ws_bytes = 0;
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream (default priority) and fork / join events for work that
+  // overlaps a single-workgroup factorisation; created on first use
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   rocblas_handle blas = nullptr;
   int n1, n2, n3, h1;
   long N, Nh;

@@ -663,26 +663,61 @@ extern "C" int pw_orthogonalize(pw_ctx* c, int nb, void* X) {
   return PW_OK;
 }
 
+// HX = H X on c->stream, in FP64 or (hdtype PW_F32) through an FP32 copy.
+static int rr_apply_h(pw_ctx* c, int hdtype, int nb, const void* X, double* HX) {
+  const long n = 2L * c->npw * nb;
+  if (hdtype == PW_F64) return pw_apply_h(c, PW_F64, nb, X, HX);
+  float* Xf = (float*)c->d_ws[1];
+  float* HXf = (float*)c->d_ws[2];
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n)), dim3(256), 0, c->stream, Xf, (const double*)X, n);
+  PW_TRY(pw_apply_h(c, PW_F32, nb, Xf, HXf));
+  hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n)), dim3(256), 0, c->stream, HX, (const float*)HXf, n);
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+static int side_stream(pw_ctx* c) {
+  if (!c->side) PW_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+  if (!c->ev_fork) PW_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+  if (!c->ev_join) PW_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+  return PW_OK;
+}
+
 // Rayleigh-Ritz on an FP64 block: Cholesky QR, H X, X^T H X, eigen-
 // decomposition, rotation; eig (ascending) and residual norms to host.
 // hdtype selects the precision of the H X product (PW_F64 or PW_F32).
+// Up to 256 bands H is applied to the block Y before its orthonormalisation
+// and the factor carried over, H (Y U^{-1}) = (H Y) U^{-1}: after S = Y^T Y
+// the product runs on a second stream while the single-workgroup Cholesky
+// factors S, then X = Y U^{-1} and HX = (H Y) U^{-1} (the factor's pivot
+// check rides on the final host read).
 extern "C" int pw_rayleigh_ritz(pw_ctx* c, int hdtype, int nb, void* X, double* eig, double* res) {
   if (!c || !X || !eig || !res || nb < 1 || nb > c->max_bands) return fail(PW_ERR_ARG, "pw_rayleigh_ritz: bad argument");
-  PW_TRY(pw_orthogonalize(c, nb, X));
   const int m = 2 * c->npw;
   const long n = (long)m * nb;
   const double one = 1.0, zero = 0.0;
   hipStream_t s = c->stream;
+  PW_TRY(ensure_ws(c));
   double* HX = (double*)c->d_ws[4];
   double* T = (double*)c->d_ws[5];
-  if (hdtype == PW_F64) {
-    PW_TRY(pw_apply_h(c, PW_F64, nb, X, HX));
+  const bool fork = nb <= chol_inv_max_n();
+  if (fork) {
+    PW_TRY(gemm_tn<double>(c, m, nb, nb, (const double*)X, m, (const double*)X, m, c->d_S, nb, nullptr, zero));
+    PW_TRY(side_stream(c));
+    PW_HIP(hipEventRecord(c->ev_fork, s));
+    PW_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+    c->stream = c->side;
+    const int rc = rr_apply_h(c, hdtype, nb, X, HX);
+    c->stream = s;
+    PW_TRY(rc);
+    PW_HIP(hipEventRecord(c->ev_join, c->side));
+    PW_TRY(chol_upper(c, nb, c->d_S, nb, c->d_chol, c->d_info + 1));  // word 0 is the eigensolver's
+    PW_HIP(hipStreamWaitEvent(s, c->ev_join, 0));
+    PW_TRY(trsm_right_upper(c, m, nb, (double*)X, m, c->d_S, nb, c->d_chol));
+    PW_TRY(trsm_right_upper(c, m, nb, HX, m, c->d_S, nb, c->d_chol));
   } else {
-    float* Xf = (float*)c->d_ws[1];
-    float* HXf = (float*)c->d_ws[2];
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n)), dim3(256), 0, s, Xf, (const double*)X, n);
-    PW_TRY(pw_apply_h(c, PW_F32, nb, Xf, HXf));
-    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n)), dim3(256), 0, s, HX, (const float*)HXf, n);
+    PW_TRY(pw_orthogonalize(c, nb, X));
+    PW_TRY(rr_apply_h(c, hdtype, nb, X, HX));
   }
   PW_TRY(gemm_tn<double>(c, m, nb, nb, (const double*)X, m, HX, m, c->d_S, nb, nullptr, zero));
   PW_TRY(sym_eig(c, nb, c->d_S, nb, c->d_D, c->d_E, c->d_info));
@@ -692,9 +727,12 @@ extern "C" int pw_rayleigh_ritz(pw_ctx* c, int hdtype, int nb, void* X, double* 
   hipLaunchKernelGGL(k_col_sumsq, dim3(nb), dim3(256), 0, s, (const double*)T, (const double*)X,
                      (const double*)c->d_D, (long)m, c->d_red);
   std::vector<double> r2(nb);
+  int info = 0;
   PW_HIP(hipMemcpyAsync(eig, c->d_D, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
   PW_HIP(hipMemcpyAsync(r2.data(), c->d_red, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  if (fork) PW_HIP(hipMemcpyAsync(&info, c->d_info + 1, sizeof(int), hipMemcpyDeviceToHost, s));
   PW_HIP(hipStreamSynchronize(s));
+  if (info != 0) return fail(PW_ERR_LINALG, "Potrf: overlap matrix is not positive definite (info = %d)", info);
   for (int j = 0; j < nb; j++) res[j] = std::sqrt(r2[j]);
   return PW_OK;
 }
