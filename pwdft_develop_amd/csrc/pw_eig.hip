@@ -1119,11 +1119,18 @@ int eig_sym(pw_ctx* c, int n, double* A, int lda, double* w) {
     PW_TRY(ensure_lds((const void*)k_sytrd, lds_trd + 1024));  // + its static LDS
     hipLaunchKernelGGL(k_sytrd, dim3(1), dim3(kTrdThreads), lds_trd, s, A, n, lda, nbk, d, e, tau);
   }
-  if (nchunk > 0) {
+  // the compact-WY factors (reflectors only) on the second stream while
+  // bisection and inverse iteration (d, e only) run on this one
+  const bool side = nchunk > 0;
+  if (side) {
+    PW_TRY(side_stream(c));
+    PW_HIP(hipEventRecord(c->ev_fork, s));
+    PW_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
     const size_t lds_t = sizeof(double) * ((size_t)n * (nbw + 1) + 2 * (size_t)nbw * nbw);
     PW_TRY(ensure_lds((const void*)k_larft, lds_t));
-    hipLaunchKernelGGL(k_larft, dim3(nchunk), dim3(256), lds_t, s, (const double*)A, lda, (const double*)tau, n,
-                       nbw, Tw, EigBatch{});
+    hipLaunchKernelGGL(k_larft, dim3(nchunk), dim3(256), lds_t, c->side, (const double*)A, lda, (const double*)tau,
+                       n, nbw, Tw, EigBatch{});
+    PW_HIP(hipEventRecord(c->ev_join, c->side));
   }
   hipLaunchKernelGGL(k_stebz, dim3((n + 3) / 4), dim3(256), sizeof(double) * 2 * n, s, (const double*)d,
                      (const double*)e, n, w, EigBatch{});
@@ -1133,6 +1140,7 @@ int eig_sym(pw_ctx* c, int n, double* A, int lda, double* w) {
   else
     hipLaunchKernelGGL(k_stein, dim3((n + 63) / 64), dim3(64), sizeof(double) * 2 * n, s, (const double*)d,
                        (const double*)e, (const double*)w, n, Zt, S);
+  if (side) PW_HIP(hipStreamWaitEvent(s, c->ev_join, 0));
   // compact-WY chunk: V (n x nbw) and T next to the Z block in LDS
   const size_t lds_orm = sizeof(double) * ((size_t)n * kOrmCols + (size_t)n * (nbw + 1) + (size_t)nbw * nbw +
                                            2 * (size_t)nbw * kOrmCols);

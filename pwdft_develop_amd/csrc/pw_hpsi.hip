@@ -1218,6 +1218,15 @@ extern "C" int pw_nonlocal_multx_noalloc(pw_ctx* c, int dtype, int nb, const voi
   return dtype == PW_F64 ? nonlocal_noalloc<double>(c, nb, X, V, sp) : nonlocal_noalloc<float>(c, nb, X, V, sp);
 }
 
+namespace pw {
+int side_stream(pw_ctx* c) {
+  if (!c->side) PW_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+  if (!c->ev_fork) PW_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+  if (!c->ev_join) PW_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+  return PW_OK;
+}
+}  // namespace pw
+
 extern "C" int pw_apply_h(pw_ctx* c, int dtype, int nb, const void* X, void* HX) {
   PW_TRY(check_call(c, dtype, nb, X, HX));
   if (nb == 0) return PW_OK;

@@ -216,6 +216,8 @@ int sticks_to_sphere_add(pw_ctx* c, int nb, const void* t2, void* out);
 // persistent-grid variant), and the LDS bytes of
 // the LDS kernel at (rp, cg) for complex elements of esz bytes.
 bool has_reg_slab(const pw_ctx* c, int dtype, int ly = 0, int sb = 0, bool persist = false);
+// the context's second stream and fork / join events (created on first use)
+int side_stream(pw_ctx* c);
 size_t slab_lds_bytes(const pw_ctx* c, int esz, int rp, int cg);
 // One Chebyshev step Ynew = s1 (H Y - c Y) - s2 Yp (pw_hpsi.hip), the
 // step's (c, s1, s2) read from the context's device coefficient table

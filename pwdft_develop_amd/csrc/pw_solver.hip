@@ -676,21 +676,14 @@ static int rr_apply_h(pw_ctx* c, int hdtype, int nb, const void* X, double* HX) 
   return PW_OK;
 }
 
-static int side_stream(pw_ctx* c) {
-  if (!c->side) PW_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
-  if (!c->ev_fork) PW_HIP(hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
-  if (!c->ev_join) PW_HIP(hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
-  return PW_OK;
-}
-
 // Rayleigh-Ritz on an FP64 block: Cholesky QR, H X, X^T H X, eigen-
 // decomposition, rotation; eig (ascending) and residual norms to host.
 // hdtype selects the precision of the H X product (PW_F64 or PW_F32).
 // Up to 256 bands H is applied to the block Y before its orthonormalisation
-// and the factor carried over, H (Y U^{-1}) = (H Y) U^{-1}: after S = Y^T Y
-// the product runs on a second stream while the single-workgroup Cholesky
-// factors S, then X = Y U^{-1} and HX = (H Y) U^{-1} (the factor's pivot
-// check rides on the final host read).
+// and the factor carried over, H (Y U^{-1}) = (H Y) U^{-1}: the product runs
+// on a second stream while S = Y^T Y is formed and the single-workgroup
+// Cholesky factors it, then X = Y U^{-1} and HX = (H Y) U^{-1} (the
+// factor's pivot check rides on the final host read).
 extern "C" int pw_rayleigh_ritz(pw_ctx* c, int hdtype, int nb, void* X, double* eig, double* res) {
   if (!c || !X || !eig || !res || nb < 1 || nb > c->max_bands) return fail(PW_ERR_ARG, "pw_rayleigh_ritz: bad argument");
   const int m = 2 * c->npw;
@@ -702,7 +695,6 @@ extern "C" int pw_rayleigh_ritz(pw_ctx* c, int hdtype, int nb, void* X, double* 
   double* T = (double*)c->d_ws[5];
   const bool fork = nb <= chol_inv_max_n();
   if (fork) {
-    PW_TRY(gemm_tn<double>(c, m, nb, nb, (const double*)X, m, (const double*)X, m, c->d_S, nb, nullptr, zero));
     PW_TRY(side_stream(c));
     PW_HIP(hipEventRecord(c->ev_fork, s));
     PW_HIP(hipStreamWaitEvent(c->side, c->ev_fork, 0));
@@ -711,6 +703,10 @@ extern "C" int pw_rayleigh_ritz(pw_ctx* c, int hdtype, int nb, void* X, double* 
     c->stream = s;
     PW_TRY(rc);
     PW_HIP(hipEventRecord(c->ev_join, c->side));
+    // S = Y^T Y concurrently, its split-K partials in T (free until the
+    // rotation) since H Y's projector product uses the context's buffer
+    PW_TRY(gemm_tn_ws<double>(c, m, nb, nb, (const double*)X, m, (const double*)X, m, c->d_S, nb, nullptr, zero, 0,
+                              T, sizeof(double) * (size_t)n));
     PW_TRY(chol_upper(c, nb, c->d_S, nb, c->d_chol, c->d_info + 1));  // word 0 is the eigensolver's
     PW_HIP(hipStreamWaitEvent(s, c->ev_join, 0));
     PW_TRY(trsm_right_upper(c, m, nb, (double*)X, m, c->d_S, nb, c->d_chol));
