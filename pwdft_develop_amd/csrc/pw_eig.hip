@@ -13,9 +13,9 @@
 //                     and a read-only symmetric product with the L2-resident
 //                     trailing matrix, per panel one rank-2 nbk update; one
 //                     1024-thread workgroup
-//   k_stebz           all eigenvalues of T: 64-way multisection on Sturm
-//                     counts, one wavefront per eigenvalue (LAPACK dstebz
-//                     tolerances)
+//   k_stebz           all eigenvalues of T: 512-way multisection on Sturm
+//                     counts, one 256-thread workgroup per eigenvalue, two
+//                     interleaved counts per lane (LAPACK dstebz tolerances)
 //   k_stein_lds       eigenvectors of T: inverse iteration on LU with
 //   (k_stein)         partial pivoting of T - lambda I, one workgroup (lane
 //                     0, scratch in LDS) per eigenvalue up to n = 768, one
@@ -592,17 +592,36 @@ k_sytrd_reg(double* __restrict__ A, int n, int lda, double* __restrict__ d, doub
   }
 }
 
-// Number of eigenvalues of T below x (Sturm sequence of the LDL^T pivots).
-__device__ __forceinline__ int sturm_count(const double* d, const double* e2, int n, double x, double pivmin) {
-  double q = d[0] - x;
-  if (fabs(q) < pivmin) q = -pivmin;
-  int c = q < 0.0;
+// 1 / p from v_rcp_f64 and two Newton steps (within an ulp or so of the
+// rounded quotient for the normal p that reach it; callers keep |p| away
+// from zero).  Five dependent operations against eleven for an IEEE divide.
+__device__ __forceinline__ double fast_rcp(double p) {
+  double r = __builtin_amdgcn_rcp(p);
+  r = fma(r, fma(-p, r, 1.0), r);
+  return fma(r, fma(-p, r, 1.0), r);
+}
+
+// Numbers of eigenvalues of T below x0 and below x1 (Sturm sequences of the
+// LDL^T pivots, dstebz's recurrence with pivmin), the two chains
+// interleaved so that each hides the other's latency.
+__device__ __forceinline__ void sturm_count2(const double* d, const double* e2, int n, double x0, double x1,
+                                             double pivmin, int& c0, int& c1) {
+  double q0 = d[0] - x0, q1 = d[0] - x1;
+  if (fabs(q0) < pivmin) q0 = -pivmin;
+  if (fabs(q1) < pivmin) q1 = -pivmin;
+  int n0 = q0 < 0.0, n1 = q1 < 0.0;
+#pragma unroll 4
   for (int i = 1; i < n; i++) {
-    q = (d[i] - x) - e2[i - 1] / q;
-    if (fabs(q) < pivmin) q = -pivmin;
-    c += q < 0.0;
+    const double di = d[i], ei = e2[i - 1];
+    q0 = (di - x0) - ei * fast_rcp(q0);
+    q1 = (di - x1) - ei * fast_rcp(q1);
+    if (fabs(q0) < pivmin) q0 = -pivmin;
+    if (fabs(q1) < pivmin) q1 = -pivmin;
+    n0 += q0 < 0.0;
+    n1 += q1 < 0.0;
   }
-  return c;
+  c0 = n0;
+  c1 = n1;
 }
 
 // Gershgorin interval and pivmin of T, shared by k_stebz and k_stein.
@@ -642,8 +661,13 @@ __device__ void tri_bounds(const double* d, const double* e, int n, double* sred
   *gl = lo; *gu = hi; *onenrm = nrm; *e2max = em;
 }
 
-// Eigenvalue j = (block * waves + wave) of T by multisection: 64 Sturm counts
-// per round narrow the bracket 65-fold.  blockDim = 256.
+// Eigenvalue j = blockIdx.x of T by multisection: the 256 threads count at
+// 512 points per round (two per lane, adjacent), which narrows the bracket
+// 513-fold, about 6 rounds to dstebz's tolerance.  Every thread forms the
+// points from the same bracket with the same expression, so the new bracket
+// needs only the index of the first point above lambda_j (a wavefront
+// ballot, then the minimum over the four wavefronts through LDS).
+constexpr int kBzPts = 512;
 __global__ void __launch_bounds__(256)
 k_stebz(const double* __restrict__ dg, const double* __restrict__ eg, int n, double* __restrict__ w, EigBatch bs) {
   extern __shared__ double smd[];
@@ -653,6 +677,7 @@ k_stebz(const double* __restrict__ dg, const double* __restrict__ eg, int n, dou
   double* d = smd;
   double* e2 = d + n;
   __shared__ double sred[16];
+  __shared__ int sfirst[2][4];
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     d[i] = dg[i];
     if (i < n - 1) e2[i] = eg[i] * eg[i];
@@ -667,24 +692,32 @@ k_stebz(const double* __restrict__ dg, const double* __restrict__ eg, int n, dou
   gl -= 2.0 * n * eps * wid + 2.0 * pivmin;  // dstebz's widening
   gu += 2.0 * n * eps * wid + 2.0 * pivmin;
   const double atol = eps * tnorm, rtol = 2.0 * eps;
-  const int j = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (j >= n) return;  // whole wavefront
+  const int j = blockIdx.x;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   double lo = gl, hi = gu;
-  for (int it = 0; it < 48; it++) {
-    if (hi - lo <= fmax(fmax(atol, pivmin), rtol * fmax(fabs(lo), fabs(hi)))) break;
-    const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
-    const int cnt = sturm_count(d, e2, n, x, pivmin);
-    const unsigned long long mask = __ballot(cnt >= j + 1);  // lambda_j < x
-    const int first = mask ? __ffsll((long long)mask) - 1 : 64;
-    const double xh = __shfl(x, first < 64 ? first : 63, 64);
-    const double xl = __shfl(x, first > 0 ? first - 1 : 0, 64);
-    const double nhi = first < 64 ? xh : hi;
-    const double nlo = first > 0 ? xl : lo;
+  auto point = [&](int p) { return lo + (hi - lo) * (double)(p + 1) / (double)(kBzPts + 1); };
+  for (int it = 0; it < 16; it++) {
+    if (hi - lo <= fmax(fmax(atol, pivmin), rtol * fmax(fabs(lo), fabs(hi)))) break;  // uniform
+    int c0, c1;
+    sturm_count2(d, e2, n, point(2 * t), point(2 * t + 1), pivmin, c0, c1);
+    // first point p with count >= j + 1 (lambda_j < x_p); kBzPts if none
+    const unsigned long long m1 = __ballot(c1 >= j + 1);
+    int first = kBzPts;
+    if (m1) {
+      const int l = __ffsll((long long)m1) - 1;
+      const int a0 = __shfl(c0, l, 64);
+      first = 2 * (64 * wv + l) + (a0 >= j + 1 ? 0 : 1);
+    }
+    if (lane == 0) sfirst[it & 1][wv] = first;
+    __syncthreads();  // the other parity's slots are free: every thread read them before this barrier
+    const int* sf = sfirst[it & 1];
+    const int f = min(min(sf[0], sf[1]), min(sf[2], sf[3]));
+    const double nhi = f < kBzPts ? point(f) : hi;
+    const double nlo = f > 0 ? point(f - 1) : lo;
     lo = nlo;
     hi = nhi;
   }
-  if (lane == 0) w[j] = 0.5 * (lo + hi);
+  if (t == 0) w[j] = 0.5 * (lo + hi);
 }
 
 __device__ __forceinline__ double start_value(int j, int i) {
@@ -711,41 +744,66 @@ struct SteinRows {
   double* __restrict__ y;    // forward-eliminated right-hand side
 };
 
-// LU with partial pivoting of T - lam I (dlagtf's elimination order).
-// Branch-free (selects), so that unrolled iterations can issue their
-// memory operations early.
-__device__ void stein_factor(const double* d, const double* e, int n, double lam, double pert, SteinRows R,
-                             int nthr, int q) {
-  double c = d[0] - lam, s1 = n > 1 ? e[0] : 0.0;
-#pragma unroll 4
-  for (int i = 0; i < n - 1; i++) {
-    const double sub = e[i], dn = d[i + 1] - lam, sup = (i + 1 < n - 1) ? e[i + 1] : 0.0;
-    const size_t o = (size_t)i * nthr + q;
-    const bool sw = fabs(sub) > fabs(c);
-    // without interchange the pivot c is kept away from zero
-    const double cp = (!sw && fabs(c) < pert) ? (c >= 0.0 ? pert : -pert) : c;
-    const double piv = sw ? sub : cp;
-    const double mlt = (sw ? cp : sub) / piv;
-    R.r0[o] = 1.0 / piv;
-    R.u1[o] = sw ? dn : s1;
-    R.u2[o] = sw ? sup : 0.0;
-    R.ml[o] = mlt;
-    R.pv[o] = sw ? 1.0 : 0.0;
-    const double cn = sw ? s1 - mlt * dn : dn - mlt * s1;
-    s1 = sw ? -mlt * sup : sup;
-    c = cn;
+constexpr int kSteinB = 16;  // rows per batch of loads
+
+// LU with partial pivoting of T - lam I (dlagtf's elimination order), fused
+// with the forward elimination of the start vector (start_value(jj, .)):
+// the first inverse iteration's first half.  d and e are read kSteinB rows
+// ahead into registers, so a batch costs one LDS round trip (the row
+// stores of the previous batch may alias them and would otherwise order
+// every load behind them); one reciprocal per row (the multiplier is the
+// numerator times 1 / pivot).  Branch-free selects.
+__device__ void stein_factor(const double* d, const double* e, int n, double lam, double pert, int jj,
+                             SteinRows R, int nthr, int q) {
+  double c = d[0] - lam, s1 = n > 1 ? e[0] : 0.0, sub = s1;
+  double y0 = start_value(jj, 0);
+  for (int i0 = 0; i0 < n - 1; i0 += kSteinB) {
+    double db[kSteinB], eb[kSteinB];
+#pragma unroll
+    for (int u = 0; u < kSteinB; u++) {
+      const int i = i0 + u;
+      db[u] = i + 1 < n ? d[i + 1] : 0.0;
+      eb[u] = i + 2 < n ? e[i + 1] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSteinB; u++) {
+      const int i = i0 + u;
+      if (i < n - 1) {
+        const double dn = db[u] - lam, sup = eb[u];
+        const size_t o = (size_t)i * nthr + q;
+        const bool sw = fabs(sub) > fabs(c);
+        // without interchange the pivot c is kept away from zero
+        const double cp = (!sw && fabs(c) < pert) ? (c >= 0.0 ? pert : -pert) : c;
+        const double piv = sw ? sub : cp;
+        const double rp = fast_rcp(piv);
+        const double mlt = (sw ? cp : sub) * rp;
+        R.r0[o] = rp;
+        R.u1[o] = sw ? dn : s1;
+        R.u2[o] = sw ? sup : 0.0;
+        R.ml[o] = mlt;
+        R.pv[o] = sw ? 1.0 : 0.0;
+        const double cn = sw ? s1 - mlt * dn : dn - mlt * s1;
+        s1 = sw ? -mlt * sup : sup;
+        c = cn;
+        sub = sup;
+        const double y1 = start_value(jj, i + 1);
+        R.y[o] = sw ? y1 : y0;
+        y0 = sw ? y0 - mlt * y1 : y1 - mlt * y0;
+      }
+    }
   }
   if (fabs(c) < pert) c = c >= 0.0 ? pert : -pert;
   const size_t o = (size_t)(n - 1) * nthr + q;
   R.r0[o] = 1.0 / c; R.u1[o] = 0.0; R.u2[o] = 0.0;
+  R.y[o] = y0;
 }
 
-// x <- (T - lam I)^{-1} (x * scale); returns ||x||^2 of the result.  Rows
-// go in batches of kSteinB: all loads of a batch are issued before any of
-// its stores, so a batch costs one memory round trip (the compiler cannot
-// reorder across the previous batch's stores by itself).
-constexpr int kSteinB = 16;
-__device__ double stein_solve(int n, double scale, SteinRows R, int nthr, int q) {
+// Forward elimination x * scale -> y with the factor's multipliers and
+// interchanges.  Rows go in batches of kSteinB: all loads of a batch are
+// issued before any of its stores, so a batch costs one memory round trip
+// (the compiler cannot reorder across the previous batch's stores by
+// itself).
+__device__ void stein_forward(int n, double scale, SteinRows R, int nthr, int q) {
   double y0 = R.x[q] * scale;
   for (int i0 = 0; i0 < n - 1; i0 += kSteinB) {
     double xb[kSteinB], mb[kSteinB], pb[kSteinB];
@@ -770,6 +828,10 @@ __device__ double stein_solve(int n, double scale, SteinRows R, int nthr, int q)
     }
   }
   R.y[(size_t)(n - 1) * nthr + q] = y0;
+}
+
+// Back substitution U x = y; returns ||x||^2.
+__device__ double stein_back(int n, SteinRows R, int nthr, int q) {
   double xn1 = 0.0, xn2 = 0.0, ss = 0.0;
   for (int i0 = n - 1; i0 >= 0; i0 -= kSteinB) {
     double yb[kSteinB], ab[kSteinB], bb[kSteinB], rb[kSteinB];
@@ -818,12 +880,13 @@ __device__ void stein_cluster(const double* d, const double* e, const double* __
     double lam = w[jj];
     if (jj > j && lam - lamprev < 10.0 * eps * fabs(lam)) lam = lamprev + 10.0 * eps * fabs(lam);
     lamprev = lam;
-    stein_factor(d, e, n, lam, pert, R, nthr, q);
-    for (int i = 0; i < n; i++) R.x[(size_t)i * nthr + q] = start_value(jj, i);
+    // the first iteration's forward elimination rides on the factorisation
+    stein_factor(d, e, n, lam, pert, jj, R, nthr, q);
     const int iters = (j1 > j) ? 4 : 2;
     double scale = 1.0;
     for (int it = 0; it < iters; it++) {
-      double ss = stein_solve(n, scale, R, nthr, q);
+      if (it > 0) stein_forward(n, scale, R, nthr, q);
+      double ss = stein_back(n, R, nthr, q);
       if (jj > j) {
         // orthogonalise against the cluster's earlier vectors (unit norm)
         for (int kk = j; kk < jj; kk++) {
@@ -1132,7 +1195,7 @@ int eig_sym(pw_ctx* c, int n, double* A, int lda, double* w) {
                        n, nbw, Tw, EigBatch{});
     PW_HIP(hipEventRecord(c->ev_join, c->side));
   }
-  hipLaunchKernelGGL(k_stebz, dim3((n + 3) / 4), dim3(256), sizeof(double) * 2 * n, s, (const double*)d,
+  hipLaunchKernelGGL(k_stebz, dim3(n), dim3(256), sizeof(double) * 2 * n, s, (const double*)d,
                      (const double*)e, n, w, EigBatch{});
   if (n <= kSteinLdsMaxN)
     hipLaunchKernelGGL(k_stein_lds, dim3(n), dim3(64), sizeof(double) * 9 * n, s, (const double*)d,
@@ -1262,7 +1325,7 @@ int sygvd_batched(pw_ctx* c, int n, double* A, int lda, long sA, double* B, int 
     hipLaunchKernelGGL(k_larft, dim3(nchunk, nbatch), dim3(256), lds_t, s, (const double*)C, n, (const double*)tau,
                        n, nbw, Tw, bs);
   }
-  hipLaunchKernelGGL(k_stebz, dim3((n + 3) / 4, nbatch), dim3(256), sizeof(double) * 2 * n, s, (const double*)d,
+  hipLaunchKernelGGL(k_stebz, dim3(n, nbatch), dim3(256), sizeof(double) * 2 * n, s, (const double*)d,
                      (const double*)e, n, w, bs);
   hipLaunchKernelGGL(k_stein_lds, dim3(n, nbatch), dim3(64), sizeof(double) * 9 * n, s, (const double*)d,
                      (const double*)e, (const double*)w, n, Zt, bs);
