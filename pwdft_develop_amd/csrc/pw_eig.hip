@@ -279,8 +279,8 @@ k_sytrd(double* __restrict__ A, int n, int lda, int nbk, double* __restrict__ d,
 //   (d) wave I forms w = p - (tau/2)(p.v) v, p = tau A v, on rows of
 //       block I, adding the table's terms of that row block in a fixed
 //       order,
-//   (e) A -= v w^T + w v^T: one MFMA per 16 x 16 register tile (K = 2 of
-//       the 4), read-modify-write on the LDS diagonal blocks.
+//   (e) A -= v w^T + w v^T: two VALU FMAs per register element,
+//       read-modify-write on the LDS diagonal blocks.
 // No global memory traffic after the load except the reflector and d, e,
 // tau.  Same arithmetic as k_sytrd up to the summation order.
 constexpr int kTr2MaxNB = 8;
@@ -560,21 +560,40 @@ k_sytrd_reg(double* __restrict__ A, int n, int lda, double* __restrict__ d, doub
         for (int j = 0; j < 4; j++) Dw[r * kTr2Ld + 16 * h + j0 + j] = dv[j] - (vr * wc[j] + wr * vc[j]);
       }
     }
-    // register tiles: A operand (row, kk) = (v, w, 0, 0)[kk], B operand
-    // (kk, col) = (-w, -v, 0, 0)[kk]
+    // register tiles by VALU FMAs, two per element (an FP64 MFMA issues no
+    // faster than the VALU on gfx950, and this product would use K = 2 of
+    // its 4): lane holds rows 32 I + 16 ti + 4 i + lk, columns 32 J + 16 tj
+    // + lr in a[s][ti + 2 tj][i]
 #pragma unroll
     for (int s = 0; s < kTr2Slots; s++) {
       const int I = bI[s], J = bJ[s];
       if (J < 0 || 32 * J + 31 <= k) continue;
-      double ao[2], bo[2];
+      double vc[2], wc[2];
 #pragma unroll
-      for (int u = 0; u < 2; u++) {
-        const int r = 32 * I + 16 * u + lr, c = 32 * J + 16 * u + lr;
-        ao[u] = lk == 0 ? vat(r) : (lk == 1 ? sw[r] : 0.0);
-        bo[u] = lk == 0 ? -sw[c] : (lk == 1 ? -vat(c) : 0.0);
+      for (int tj = 0; tj < 2; tj++) {
+        const int c = 32 * J + 16 * tj + lr;
+        vc[tj] = vat(c);
+        wc[tj] = sw[c];
       }
 #pragma unroll
-      for (int q = 0; q < 4; q++) a[s][q] = mfma_f64(ao[q & 1], bo[q >> 1], a[s][q]);
+      for (int ti = 0; ti < 2; ti++) {  // one row tile at a time (register pressure)
+        double vr[4], wr[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int r = 32 * I + 16 * ti + 4 * i + lk;
+          vr[i] = vat(r);
+          wr[i] = sw[r];
+        }
+#pragma unroll
+        for (int tj = 0; tj < 2; tj++)
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            const double x = fma(-vr[i], wc[tj], a[s][ti + 2 * tj][i]);
+            a[s][ti + 2 * tj][i] = fma(-wr[i], vc[tj], x);
+          }
+#pragma unroll
+        for (int tj = 0; tj < 2; tj++) asm volatile("" : "+v"(a[s][ti + 2 * tj]));
+      }
     }
     wave_lds_sync();  // this wave's diagonal-block writes before the next column read
   }
