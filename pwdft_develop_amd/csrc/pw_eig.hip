@@ -41,7 +41,7 @@ namespace pw {
 
 constexpr int kTrdThreads = 1024;  // k_sytrd workgroup
 constexpr int kTrdWaves = kTrdThreads / 64;
-constexpr int kOrmCols = 8;         // columns per k_ormtr workgroup
+constexpr int kOrmCols = 2;         // columns per k_ormtr workgroup (n / 2 workgroups)
 constexpr int kEigMaxN = 1024;      // LDS bound of k_sytrd
 constexpr int kSteinLdsMaxN = 768;  // 9 n doubles of k_stein_lds within 64 KiB
 
@@ -778,21 +778,24 @@ __device__ void stein_factor(const double* d, const double* e, int n, double lam
   double y0 = start_value(jj, 0);
   for (int i0 = 0; i0 < n - 1; i0 += kSteinB) {
     double db[kSteinB], eb[kSteinB];
+    // unconditional loads at clamped rows (a guarded load becomes a branch
+    // and a wait on every row's stores); rows past the end are not used
 #pragma unroll
     for (int u = 0; u < kSteinB; u++) {
-      const int i = i0 + u;
-      db[u] = i + 1 < n ? d[i + 1] : 0.0;
-      eb[u] = i + 2 < n ? e[i + 1] : 0.0;
+      const int i = min(i0 + u, n - 2);
+      db[u] = d[i + 1];
+      eb[u] = e[min(i + 1, n - 2)];
     }
 #pragma unroll
     for (int u = 0; u < kSteinB; u++) {
       const int i = i0 + u;
       if (i < n - 1) {
-        const double dn = db[u] - lam, sup = eb[u];
+        const double dn = db[u] - lam, sup = i + 2 < n ? eb[u] : 0.0;
         const size_t o = (size_t)i * nthr + q;
         const bool sw = fabs(sub) > fabs(c);
-        // without interchange the pivot c is kept away from zero
-        const double cp = (!sw && fabs(c) < pert) ? (c >= 0.0 ? pert : -pert) : c;
+        // without interchange the pivot c is kept away from zero: +-pert
+        // when |c| < pert (+pert for c = +-0), else c
+        const double cp = sw ? c : (c >= 0.0 ? 1.0 : -1.0) * fmax(fabs(c), pert);
         const double piv = sw ? sub : cp;
         const double rp = fast_rcp(piv);
         const double mlt = (sw ? cp : sub) * rp;
@@ -828,12 +831,10 @@ __device__ void stein_forward(int n, double scale, SteinRows R, int nthr, int q)
     double xb[kSteinB], mb[kSteinB], pb[kSteinB];
 #pragma unroll
     for (int u = 0; u < kSteinB; u++) {
-      const int i = i0 + u;
-      const size_t o = (size_t)i * nthr + q;
-      const bool ok = i < n - 1;
-      xb[u] = ok ? R.x[o + nthr] : 0.0;
-      mb[u] = ok ? R.ml[o] : 0.0;
-      pb[u] = ok ? R.pv[o] : 0.0;
+      const size_t o = (size_t)min(i0 + u, n - 2) * nthr + q;  // clamped: unconditional loads
+      xb[u] = R.x[o + nthr];
+      mb[u] = R.ml[o];
+      pb[u] = R.pv[o];
     }
 #pragma unroll
     for (int u = 0; u < kSteinB; u++) {
@@ -856,13 +857,11 @@ __device__ double stein_back(int n, SteinRows R, int nthr, int q) {
     double yb[kSteinB], ab[kSteinB], bb[kSteinB], rb[kSteinB];
 #pragma unroll
     for (int u = 0; u < kSteinB; u++) {
-      const int i = i0 - u;
-      const size_t o = (size_t)(i >= 0 ? i : 0) * nthr + q;
-      const bool ok = i >= 0;
-      yb[u] = ok ? R.y[o] : 0.0;
-      ab[u] = ok ? R.u1[o] : 0.0;
-      bb[u] = ok ? R.u2[o] : 0.0;
-      rb[u] = ok ? R.r0[o] : 0.0;
+      const size_t o = (size_t)max(i0 - u, 0) * nthr + q;  // clamped: unconditional loads
+      yb[u] = R.y[o];
+      ab[u] = R.u1[o];
+      bb[u] = R.u2[o];
+      rb[u] = R.r0[o];
     }
 #pragma unroll
     for (int u = 0; u < kSteinB; u++) {
@@ -1117,19 +1116,29 @@ k_ormtr(const double* __restrict__ A, int lda, const double* __restrict__ Tg, in
       }
     }
     __syncthreads();
-    // W1 = V^T Z (rows of V below kb + 1 + j are zero in column j)
-    for (int e = t; e < nbw * kOrmCols; e += blockDim.x) {
-      const int j = e / kOrmCols, c = e % kOrmCols;
-      double w0 = 0.0, w1 = 0.0;
-      if (j < b) {
-        int r = kb + 1 + j;
-        for (; r + 1 < n; r += 2) {
-          w0 += Vs[(size_t)r * lvs + j] * zs[r * kOrmCols + c];
-          w1 += Vs[(size_t)(r + 1) * lvs + j] * zs[(r + 1) * kOrmCols + c];
+    // W1 = V^T Z (rows of V above kb + 1 + j are zero in column j), each
+    // of the nbw x kOrmCols dot products split over S consecutive lanes
+    // (rows strided by S), then summed across them
+    {
+      const int D = nbw * kOrmCols;
+      int S = 1;
+      while (S < 16 && 2 * S * D <= (int)blockDim.x) S *= 2;
+      for (int g0 = 0; g0 < D * S; g0 += blockDim.x) {
+        const int g = g0 + t, e = g / S, p = g % S;
+        double w0 = 0.0, w1 = 0.0;
+        const int j = e / kOrmCols, c = e % kOrmCols;
+        if (e < D && j < b) {
+          int r = kb + 1 + j + p;
+          for (; r + S < n; r += 2 * S) {
+            w0 += Vs[(size_t)r * lvs + j] * zs[r * kOrmCols + c];
+            w1 += Vs[(size_t)(r + S) * lvs + j] * zs[(r + S) * kOrmCols + c];
+          }
+          if (r < n) w0 += Vs[(size_t)r * lvs + j] * zs[r * kOrmCols + c];
         }
-        if (r < n) w0 += Vs[(size_t)r * lvs + j] * zs[r * kOrmCols + c];
+        w0 += w1;
+        for (int o = 1; o < S; o <<= 1) w0 += __shfl_xor(w0, o, 64);  // S divides 64: groups stay in a wavefront
+        if (e < D && p == 0) W1[e] = w0;
       }
-      W1[e] = w0 + w1;
     }
     __syncthreads();
     // W2 = T W1
