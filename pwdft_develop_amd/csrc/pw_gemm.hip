@@ -114,21 +114,25 @@ k_gemm_tn_partial(const T* __restrict__ A, int lda, const T* __restrict__ B, int
     for (int b = 0; b < NA; b++) acc[a][b] = V{0, 0, 0, 0};
   const int lr = lane & 31, lcol = 16 * TM * w + (lane >> 5);  // column of load q: lcol + 2 q
   T ra[NQ], rb[NQ];
+  // unconditional loads at clamped indices, the zero fill outside the
+  // operands at the LDS store (a select on a loaded value makes the
+  // compiler wait for the load where it is issued)
   auto load = [&](int k0) {
-    const int r = k0 + lr;
+    const int r = min(k0 + lr, ke - 1);
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
-      const int ci = i0 + lcol + 2 * q, cj = j0 + lcol + 2 * q;
-      ra[q] = (ci < n1 && r < ke) ? A[(size_t)ci * lda + r] : (T)0;
-      rb[q] = (cj < n2 && r < ke) ? B[(size_t)cj * ldb + r] : (T)0;
+      const int ci = min(i0 + lcol + 2 * q, n1 - 1), cj = min(j0 + lcol + 2 * q, n2 - 1);
+      ra[q] = A[(size_t)ci * lda + r];
+      rb[q] = B[(size_t)cj * ldb + r];
     }
   };
   load(kb);
   for (int k0 = kb; k0 < ke; k0 += BK) {
+    const bool okr = k0 + lr < ke;
 #pragma unroll
     for (int q = 0; q < NQ; q++) {
-      As[lcol + 2 * q][lr] = ra[q];
-      Bs[lcol + 2 * q][lr] = rb[q];
+      As[lcol + 2 * q][lr] = (okr && i0 + lcol + 2 * q < n1) ? ra[q] : (T)0;
+      Bs[lcol + 2 * q][lr] = (okr && j0 + lcol + 2 * q < n2) ? rb[q] : (T)0;
     }
     lds_barrier();
     if (k0 + BK < ke) load(k0 + BK);  // in flight during the MFMAs below and the barrier after them
@@ -184,6 +188,27 @@ __device__ __forceinline__ void bf16_split(float x, unsigned short (&t)[NS]) {
     r -= (float)h;
   }
 }
+// Four values (two pairs) into NS packed BF16 term pairs: term i of x0, x1
+// in t[i].x (x0 in the low half), of x2, x3 in t[i].y; the same terms as
+// bf16_split (one v_cvt_pk_bf16_f32 per pair and term).
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+template <int NS>
+__device__ __forceinline__ void bf16_split4(float2 p, float2 q, uint2 (&t)[NS]) {
+  float r0 = p.x, r1 = p.y, r2 = q.x, r3 = q.y;
+#pragma unroll
+  for (int i = 0; i < NS; i++) {
+    const bf16x2 h01 = {(__bf16)r0, (__bf16)r1}, h23 = {(__bf16)r2, (__bf16)r3};
+    const unsigned u = __builtin_bit_cast(unsigned, h01), v = __builtin_bit_cast(unsigned, h23);
+    t[i].x = u;
+    t[i].y = v;
+    if (i + 1 < NS) {  // residuals from the packed words (a BF16 is the top half of its FP32)
+      r0 -= __uint_as_float(u << 16);
+      r1 -= __uint_as_float(u & 0xffff0000u);
+      r2 -= __uint_as_float(v << 16);
+      r3 -= __uint_as_float(v & 0xffff0000u);
+    }
+  }
+}
 __device__ __forceinline__ f4v mfma_b16(const unsigned short* a, const unsigned short* b, f4v acc) {
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8*>(a),
                                                  *reinterpret_cast<const bf16x8*>(b), acc, 0, 0, 0);
@@ -225,33 +250,61 @@ k_gemm_tn_partial_bf(const float* __restrict__ A, int lda, const float* __restri
   for (int a = 0; a < NA; a++)
 #pragma unroll
     for (int b = 0; b < NA; b++) acc[a][b] = f4v{0, 0, 0, 0};
-  const int lr = lane & 31, lcol = 16 * TM * w + (lane >> 5);
-  float ra[NQ], rb[NQ];
-  auto load = [&](int k0) {
-    const int r = k0 + lr;
+  // thread t loads rows 4 (t % 8) .. + 3 of the chunk (two float pairs) of
+  // columns t / 8 + 32 q: unconditional loads at clamped indices (K, kb and
+  // kc are even, so a pair is wholly inside or outside [kb, ke)), zeroed
+  // outside the operands
+  constexpr int NC = 2 * TM;  // columns per thread and operand
+  const int k4 = 4 * (t & 7), lc = t >> 3;
+  float2 ra[2][NC][2], rb[2][NC][2];  // two chunks in flight (loads issued two chunks ahead)
+  auto load = [&](int k0, float2 (&ra)[NC][2], float2 (&rb)[NC][2]) {
 #pragma unroll
-    for (int q = 0; q < NQ; q++) {
-      const int ci = i0 + lcol + 2 * q, cj = j0 + lcol + 2 * q;
-      ra[q] = (ci < n1 && r < ke) ? A[(size_t)ci * lda + r] : 0.f;
-      rb[q] = (cj < n2 && r < ke) ? B[(size_t)cj * ldb + r] : 0.f;
+    for (int q = 0; q < NC; q++) {
+      const int ci = i0 + lc + 32 * q, cj = j0 + lc + 32 * q;
+      const float* pa = A + (size_t)min(ci, n1 - 1) * lda;
+      const float* pb = B + (size_t)min(cj, n2 - 1) * ldb;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int rc = min(k0 + k4 + 2 * h, ke - 2);
+        ra[q][h] = *reinterpret_cast<const float2*>(pa + rc);
+        rb[q][h] = *reinterpret_cast<const float2*>(pb + rc);
+      }
     }
   };
-  load(kb);
-  const int fr = lane & 15, fk = 8 * (lane >> 4);
-  for (int k0 = kb; k0 < ke; k0 += BK) {
+  // the zero fill outside the operands at use, not at load (a select on a
+  // loaded value makes the compiler wait for the load where it is issued)
+  auto mask = [&](int k0, float2 (&ra)[NC][2], float2 (&rb)[NC][2]) {
 #pragma unroll
-    for (int q = 0; q < NQ; q++) {
-      unsigned short ta[NS], tb[NS];
-      bf16_split<NS>(ra[q], ta);
-      bf16_split<NS>(rb[q], tb);
+    for (int q = 0; q < NC; q++) {
+      const int ci = i0 + lc + 32 * q, cj = j0 + lc + 32 * q;
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int r = k0 + k4 + 2 * h;
+        if (!(ci < n1 && r < ke)) ra[q][h] = float2{0.f, 0.f};
+        if (!(cj < n2 && r < ke)) rb[q][h] = float2{0.f, 0.f};
+      }
+    }
+  };
+  load(kb, ra[0], rb[0]);
+  if (kb + BK < ke) load(kb + BK, ra[1], rb[1]);
+  const int fr = lane & 15, fk = 8 * (lane >> 4);
+  // one chunk: its operands split into LDS, the loads of the chunk two
+  // ahead into the same registers, the products
+  auto chunk = [&](int k0, float2 (&ra)[NC][2], float2 (&rb)[NC][2]) {
+    mask(k0, ra, rb);
+#pragma unroll
+    for (int q = 0; q < NC; q++) {
+      uint2 ta[NS], tb[NS];
+      bf16_split4<NS>(ra[q][0], ra[q][1], ta);
+      bf16_split4<NS>(rb[q][0], rb[q][1], tb);
 #pragma unroll
       for (int i = 0; i < NS; i++) {
-        As[i * TB * BKP + (lcol + 2 * q) * BKP + lr] = ta[i];
-        Bs[i * TB * BKP + (lcol + 2 * q) * BKP + lr] = tb[i];
+        *reinterpret_cast<uint2*>(As + i * TB * BKP + (lc + 32 * q) * BKP + k4) = ta[i];
+        *reinterpret_cast<uint2*>(Bs + i * TB * BKP + (lc + 32 * q) * BKP + k4) = tb[i];
       }
     }
     lds_barrier();
-    if (k0 + BK < ke) load(k0 + BK);
+    if (k0 + 2 * BK < ke) load(k0 + 2 * BK, ra, rb);
 #pragma unroll
     for (int a = 0; a < NA; a++)
 #pragma unroll
@@ -267,6 +320,10 @@ k_gemm_tn_partial_bf(const float* __restrict__ A, int lda, const float* __restri
         acc[a][b] = mfma_split<NS>(pa, pb, acc[a][b]);
       }
     lds_barrier();
+  };
+  for (int k0 = kb; k0 < ke; k0 += 2 * BK) {
+    chunk(k0, ra[0], rb[0]);
+    if (k0 + BK < ke) chunk(k0 + BK, ra[1], rb[1]);
   }
   float* Ps = P + (size_t)split * n1 * n2;
 #pragma unroll
@@ -343,21 +400,23 @@ k_gemm_nn(const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb, T*
   const int bk = lane & 31, bcol = 16 * w + (lane >> 5);
   const int ar = i0 + lane;
   T ra[8], rb[8];
+  // unconditional loads at clamped indices, the zero fill at the LDS store
   auto load = [&](int k0) {
+    const int arc = min(ar, m - 1);
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-      const int kk = k0 + 8 * w + q;
-      ra[q] = (ar < m && kk < k) ? A[(size_t)kk * lda + ar] : (T)0;
-      const int cj = j0 + bcol + 2 * q, kq = k0 + bk;
-      rb[q] = (cj < n && kq < k) ? B[(size_t)cj * ldb + kq] : (T)0;
+      const int kk = min(k0 + 8 * w + q, k - 1);
+      ra[q] = A[(size_t)kk * lda + arc];
+      const int cj = min(j0 + bcol + 2 * q, n - 1), kq = min(k0 + bk, k - 1);
+      rb[q] = B[(size_t)cj * ldb + kq];
     }
   };
   load(0);
   for (int k0 = 0; k0 < k; k0 += BK) {
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-      As[8 * w + q][lane] = ra[q];
-      Bs[bcol + 2 * q][bk] = rb[q];
+      As[8 * w + q][lane] = (ar < m && k0 + 8 * w + q < k) ? ra[q] : (T)0;
+      Bs[bcol + 2 * q][bk] = (j0 + bcol + 2 * q < n && k0 + bk < k) ? rb[q] : (T)0;
     }
     lds_barrier();
     if (k0 + BK < k) load(k0 + BK);
@@ -576,26 +635,26 @@ __device__ __forceinline__ f4v mfma_split_r(const bf16x8 (&a)[NS], const bf16x8 
 // 42 against 38 us at 13134 x 256 x 256.)  The epilogue works on the
 // accumulator layout: a lane's four rows of one column are contiguous, read
 // and written as two float pairs (m and ld even).
-template <bool kCheb, int NS>
+template <bool kCheb, int NS, int RT = 4>
 __global__ void __launch_bounds__(256, 1)
 k_gemm_nn_pre(const unsigned short* __restrict__ Ab, const unsigned short* __restrict__ Bb, int m, int n, int nkc,
               NnEpilogue<float> ep) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int nrt = (m + 15) >> 4, nct = (n + 15) >> 4;
-  const int rt0 = 4 * blockIdx.x, ct0 = 16 * blockIdx.y + 4 * w;
-  const int na = min(4, nrt - rt0), nbt = min(4, nct - ct0);
+  const int rt0 = RT * blockIdx.x, ct0 = 16 * blockIdx.y + 4 * w;
+  const int na = min(RT, nrt - rt0), nbt = min(4, nct - ct0);
   if (nbt <= 0) return;
   const bf16x8* A8 = reinterpret_cast<const bf16x8*>(Ab);
   const bf16x8* B8 = reinterpret_cast<const bf16x8*>(Bb);
-  f4v acc[4][4];
+  f4v acc[RT][4];
 #pragma unroll
-  for (int a = 0; a < 4; a++)
+  for (int a = 0; a < RT; a++)
 #pragma unroll
     for (int b = 0; b < 4; b++) acc[a][b] = f4v{0, 0, 0, 0};
-  bf16x8 av[4][NS], bv[4][NS];
-  auto load = [&](int q, bf16x8 (&xa)[4][NS], bf16x8 (&xb)[4][NS]) {
+  bf16x8 av[RT][NS], bv[4][NS];
+  auto load = [&](int q, bf16x8 (&xa)[RT][NS], bf16x8 (&xb)[4][NS]) {
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int a = 0; a < RT; a++)
 #pragma unroll
       for (int i = 0; i < NS; i++)
         xa[a][i] = a < na ? A8[((size_t)((rt0 + a) * nkc + q) * kSplitTerms + i) * 64 + lane] : bf16x8{};
@@ -607,10 +666,10 @@ k_gemm_nn_pre(const unsigned short* __restrict__ Ab, const unsigned short* __res
   };
   load(0, av, bv);
   for (int q = 0; q < nkc; q++) {
-    bf16x8 an[4][NS], bn[4][NS];
+    bf16x8 an[RT][NS], bn[4][NS];
     if (q + 1 < nkc) load(q + 1, an, bn);
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+    for (int a = 0; a < RT; a++)
 #pragma unroll
       for (int b = 0; b < 4; b++) acc[a][b] = mfma_split_r<NS>(av[a], bv[b], acc[a][b]);
     if (q + 1 < nkc) {
@@ -618,7 +677,7 @@ k_gemm_nn_pre(const unsigned short* __restrict__ Ab, const unsigned short* __res
       for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int i = 0; i < NS; i++) {
-          av[a][i] = an[a][i];
+          if (a < RT) av[a][i] = an[a][i];
           bv[a][i] = bn[a][i];
         }
     }
@@ -634,7 +693,7 @@ k_gemm_nn_pre(const unsigned short* __restrict__ Ab, const unsigned short* __res
     const int col = 16 * (ct0 + b) + (lane & 15);
     if (b >= nbt || col >= n) continue;
 #pragma unroll
-    for (int a = 0; a < 4; a++) {
+    for (int a = 0; a < RT; a++) {
       const int r0 = 16 * (rt0 + a) + 4 * (lane >> 4);
       if (a >= na) continue;
 #pragma unroll
@@ -690,21 +749,23 @@ k_gemm_nn_ep(const T* __restrict__ A, int lda, const T* __restrict__ B, int ldb,
   const int bk = lane & 31, bcol = 16 * w + (lane >> 5);
   const int ar = i0 + lane;
   T ra[8], rb[8];
+  // unconditional loads at clamped indices, the zero fill at the LDS store
   auto load = [&](int k0) {
+    const int arc = min(ar, m - 1);
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-      const int kk = k0 + 8 * w + q;
-      ra[q] = (ar < m && kk < k) ? A[(size_t)kk * lda + ar] : (T)0;
-      const int cj = j0 + bcol + 2 * q, kq = k0 + bk;
-      rb[q] = (cj < n && kq < k) ? B[(size_t)cj * ldb + kq] : (T)0;
+      const int kk = min(k0 + 8 * w + q, k - 1);
+      ra[q] = A[(size_t)kk * lda + arc];
+      const int cj = min(j0 + bcol + 2 * q, n - 1), kq = min(k0 + bk, k - 1);
+      rb[q] = B[(size_t)cj * ldb + kq];
     }
   };
   load(0);
   for (int k0 = 0; k0 < k; k0 += BK) {
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-      As[8 * w + q][lane] = ra[q];
-      Bs[bcol + 2 * q][bk] = rb[q];
+      As[8 * w + q][lane] = (ar < m && k0 + 8 * w + q < k) ? ra[q] : (T)0;
+      Bs[bcol + 2 * q][bk] = (j0 + bcol + 2 * q < n && k0 + bk < k) ? rb[q] : (T)0;
     }
     lds_barrier();
     if (k0 + BK < k) load(k0 + BK);
@@ -883,7 +944,8 @@ int gemm_tn_ws(pw_ctx* c, int K, int n1, int n2, const T* A, int lda, const T* B
   if (!ws) PW_TRY(grow_gemm_ws(c, need));
   T* P = (T*)(ws ? ws : c->d_gemm_ws);
   if constexpr (sizeof(T) == 4) {
-    if (split == 2 || split == 3) {
+    // the split kernels load float pairs: K, lda and ldb even
+    if ((split == 2 || split == 3) && !(K & 1) && !(lda & 1) && !(ldb & 1)) {
       const size_t ldsb = sizeof(unsigned short) * 2 * split * (size_t)(BM * tm) * BKP;
       const dim3 gp(ti * tj * nsplit);
       if (split == 3 && tm == 2) {
@@ -1015,6 +1077,7 @@ size_t split_b_bytes(int n, int k) {
 // split_a_operand, B (k x n, ld ldb) split here into Bws (split_b_bytes(n,
 // k) bytes); NS = split terms (2 or 3).  m and ld even, H / Yc / Yp / out
 // 8-byte aligned.
+constexpr int kNnPreRt = 2;  // row tiles per k_gemm_nn_pre workgroup (PW_NNPRE_RT=4 overrides)
 int gemm_nn_pre(pw_ctx* c, int m, int n, int k, const unsigned short* Ab, const float* B, int ldb,
                 unsigned short* Bws, const float* H, const float* Yc, const float* Yp, float* out, int ld,
                 const float* coef, int step, bool cheb, int split) {
@@ -1025,9 +1088,15 @@ int gemm_nn_pre(pw_ctx* c, int m, int n, int k, const unsigned short* Ab, const 
   hipLaunchKernelGGL(k_split_operand<true>, dim3((unsigned)((ng + 3) / 4)), dim3(256), 0, c->stream, B, ldb, n, k,
                      nkc, Bws, ng);
   NnEpilogue<float> ep{H, Yc, Yp, out, ld, coef, step, 1.f, 1.f};
-  const dim3 grid((nrt + 3) / 4, (nct + 15) / 16);
-  auto kern = cheb ? (split == 3 ? k_gemm_nn_pre<true, 3> : k_gemm_nn_pre<true, 2>)
-                  : (split == 3 ? k_gemm_nn_pre<false, 3> : k_gemm_nn_pre<false, 2>);
+  // 16 RT rows per workgroup: RT = 2 doubles the grid of RT = 4 (206 -> 411
+  // workgroups at 13134 rows) at fewer registers per wave
+  const char* rte = std::getenv("PW_NNPRE_RT");
+  const int rt = (rte && std::atoi(rte) == 4) ? 4 : (rte && std::atoi(rte) == 2) ? 2 : kNnPreRt;
+  const dim3 grid((nrt + rt - 1) / rt, (nct + 15) / 16);
+  auto kern = rt == 4 ? (cheb ? (split == 3 ? k_gemm_nn_pre<true, 3, 4> : k_gemm_nn_pre<true, 2, 4>)
+                              : (split == 3 ? k_gemm_nn_pre<false, 3, 4> : k_gemm_nn_pre<false, 2, 4>))
+                      : (cheb ? (split == 3 ? k_gemm_nn_pre<true, 3, 2> : k_gemm_nn_pre<true, 2, 2>)
+                              : (split == 3 ? k_gemm_nn_pre<false, 3, 2> : k_gemm_nn_pre<false, 2, 2>));
   hipLaunchKernelGGL(kern, grid, dim3(256), 0, c->stream, Ab, Bws, m, n, nkc, ep);
   PW_HIP(hipGetLastError());
   return PW_OK;
