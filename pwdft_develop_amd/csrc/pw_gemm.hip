@@ -228,6 +228,12 @@ __device__ __forceinline__ f4v mfma_split(const unsigned short* const (&a)[NS], 
   return mfma_b16(a[0], b[0], acc);
 }
 constexpr int BKP = 40;  // LDS row stride of the bf16 tiles (80 bytes: 16-byte aligned rows)
+// The term pairs of mfma_split in its order (smallest first): NS = 3 uses
+// all six, NS = 2 the last three.  Loops over (pair, tile) in this order
+// give every tile the same sequence of products while consecutive MFMAs
+// write different accumulators.
+__device__ constexpr int kSplitPairA[6] = {2, 0, 1, 1, 0, 0};
+__device__ constexpr int kSplitPairB[6] = {0, 2, 1, 0, 1, 0};
 
 // k_gemm_tn_partial with FP32 inputs split into NS BF16 terms: the same
 // tiles, loads and split-K partials; each 32-row K chunk is one 16x16x32
@@ -305,20 +311,23 @@ k_gemm_tn_partial_bf(const float* __restrict__ A, int lda, const float* __restri
     }
     lds_barrier();
     if (k0 + 2 * BK < ke) load(k0 + 2 * BK, ra, rb);
+    bf16x8 fa[NA][NS], fb[NA][NS];
 #pragma unroll
     for (int a = 0; a < NA; a++)
 #pragma unroll
-      for (int b = 0; b < NA; b++) {
-        const int oa = (wi + a * 16 + fr) * BKP + fk, ob = (wj + b * 16 + fr) * BKP + fk;
-        const unsigned short* pa[NS];
-        const unsigned short* pb[NS];
-#pragma unroll
-        for (int i = 0; i < NS; i++) {
-          pa[i] = As + i * TB * BKP + oa;
-          pb[i] = Bs + i * TB * BKP + ob;
-        }
-        acc[a][b] = mfma_split<NS>(pa, pb, acc[a][b]);
+      for (int i = 0; i < NS; i++) {
+        fa[a][i] = *reinterpret_cast<const bf16x8*>(As + i * TB * BKP + (wi + a * 16 + fr) * BKP + fk);
+        fb[a][i] = *reinterpret_cast<const bf16x8*>(Bs + i * TB * BKP + (wj + a * 16 + fr) * BKP + fk);
       }
+    // term pair outer, tiles inner: mfma_split's order per tile
+#pragma unroll
+    for (int p = 6 - (NS == 3 ? 6 : 3); p < 6; p++)
+#pragma unroll
+      for (int a = 0; a < NA; a++)
+#pragma unroll
+        for (int b = 0; b < NA; b++)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa[a][kSplitPairA[p]], fb[b][kSplitPairB[p]],
+                                                               acc[a][b], 0, 0, 0);
     lds_barrier();
   };
   for (int k0 = kb; k0 < ke; k0 += 2 * BK) {
@@ -613,17 +622,6 @@ __global__ void k_split_operand(const float* __restrict__ X, int ldx, int nr, in
   }
 }
 
-template <int NS>
-__device__ __forceinline__ f4v mfma_split_r(const bf16x8 (&a)[NS], const bf16x8 (&b)[NS], f4v acc) {
-  if constexpr (NS == 3) {
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2], b[0], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[2], acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[1], acc, 0, 0, 0);
-  }
-  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1], b[0], acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[1], acc, 0, 0, 0);
-  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[0], acc, 0, 0, 0);
-}
 
 // out = epilogue(A B) with both operands split ahead of time (layout above;
 // A: nrt = ceil(m / 16) row tiles, B: ceil(n / 16) column tiles, nkc
@@ -668,10 +666,16 @@ k_gemm_nn_pre(const unsigned short* __restrict__ Ab, const unsigned short* __res
   for (int q = 0; q < nkc; q++) {
     bf16x8 an[RT][NS], bn[4][NS];
     if (q + 1 < nkc) load(q + 1, an, bn);
+    // term pair outer, tiles inner: the order of mfma_split per tile, while
+    // consecutive MFMAs write different accumulators
 #pragma unroll
-    for (int a = 0; a < RT; a++)
+    for (int p = 6 - (NS == 3 ? 6 : 3); p < 6; p++)
 #pragma unroll
-      for (int b = 0; b < 4; b++) acc[a][b] = mfma_split_r<NS>(av[a], bv[b], acc[a][b]);
+      for (int a = 0; a < RT; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++)
+          acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a][kSplitPairA[p]], bv[b][kSplitPairB[p]],
+                                                               acc[a][b], 0, 0, 0);
     if (q + 1 < nkc) {
 #pragma unroll
       for (int a = 0; a < 4; a++)
