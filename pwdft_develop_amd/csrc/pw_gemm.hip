@@ -262,7 +262,7 @@ k_gemm_tn_partial_bf(const float* __restrict__ A, int lda, const float* __restri
   // outside the operands
   constexpr int NC = 2 * TM;  // columns per thread and operand
   const int k4 = 4 * (t & 7), lc = t >> 3;
-  float2 ra[2][NC][2], rb[2][NC][2];  // two chunks in flight (loads issued two chunks ahead)
+  float2 ra[NC][2], rb[NC][2];  // the next chunk in flight during the current one's products
   auto load = [&](int k0, float2 (&ra)[NC][2], float2 (&rb)[NC][2]) {
 #pragma unroll
     for (int q = 0; q < NC; q++) {
@@ -291,11 +291,10 @@ k_gemm_tn_partial_bf(const float* __restrict__ A, int lda, const float* __restri
       }
     }
   };
-  load(kb, ra[0], rb[0]);
-  if (kb + BK < ke) load(kb + BK, ra[1], rb[1]);
+  load(kb, ra, rb);
   const int fr = lane & 15, fk = 8 * (lane >> 4);
-  // one chunk: its operands split into LDS, the loads of the chunk two
-  // ahead into the same registers, the products
+  // one chunk: its operands split into LDS, the next chunk's loads into the
+  // same registers, the products
   auto chunk = [&](int k0, float2 (&ra)[NC][2], float2 (&rb)[NC][2]) {
     mask(k0, ra, rb);
 #pragma unroll
@@ -310,7 +309,7 @@ k_gemm_tn_partial_bf(const float* __restrict__ A, int lda, const float* __restri
       }
     }
     lds_barrier();
-    if (k0 + 2 * BK < ke) load(k0 + 2 * BK, ra, rb);
+    if (k0 + BK < ke) load(k0 + BK, ra, rb);
     bf16x8 fa[NA][NS], fb[NA][NS];
 #pragma unroll
     for (int a = 0; a < NA; a++)
@@ -330,10 +329,7 @@ k_gemm_tn_partial_bf(const float* __restrict__ A, int lda, const float* __restri
                                                                acc[a][b], 0, 0, 0);
     lds_barrier();
   };
-  for (int k0 = kb; k0 < ke; k0 += 2 * BK) {
-    chunk(k0, ra[0], rb[0]);
-    if (k0 + BK < ke) chunk(k0 + BK, ra[1], rb[1]);
-  }
+  for (int k0 = kb; k0 < ke; k0 += BK) chunk(k0, ra, rb);
   float* Ps = P + (size_t)split * n1 * n2;
 #pragma unroll
   for (int a = 0; a < NA; a++)
@@ -1081,7 +1077,6 @@ size_t split_b_bytes(int n, int k) {
 // split_a_operand, B (k x n, ld ldb) split here into Bws (split_b_bytes(n,
 // k) bytes); NS = split terms (2 or 3).  m and ld even, H / Yc / Yp / out
 // 8-byte aligned.
-constexpr int kNnPreRt = 2;  // row tiles per k_gemm_nn_pre workgroup (PW_NNPRE_RT=4 overrides)
 int gemm_nn_pre(pw_ctx* c, int m, int n, int k, const unsigned short* Ab, const float* B, int ldb,
                 unsigned short* Bws, const float* H, const float* Yc, const float* Yp, float* out, int ld,
                 const float* coef, int step, bool cheb, int split) {
@@ -1093,9 +1088,14 @@ int gemm_nn_pre(pw_ctx* c, int m, int n, int k, const unsigned short* Ab, const 
                      nkc, Bws, ng);
   NnEpilogue<float> ep{H, Yc, Yp, out, ld, coef, step, 1.f, 1.f};
   // 16 RT rows per workgroup: RT = 2 doubles the grid of RT = 4 (206 -> 411
-  // workgroups at 13134 rows) at fewer registers per wave
+  // workgroups at 13134 rows x 256 columns) at fewer registers per wave, for
+  // grids that would give fewer than two workgroups per CU; larger ones keep
+  // RT = 4 (twice the reuse of each B fragment)
   const char* rte = std::getenv("PW_NNPRE_RT");
-  const int rt = (rte && std::atoi(rte) == 4) ? 4 : (rte && std::atoi(rte) == 2) ? 2 : kNnPreRt;
+  int ncu = 256;
+  PW_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->device));
+  const long g4 = (long)((nrt + 3) / 4) * ((nct + 15) / 16);
+  const int rt = (rte && std::atoi(rte) == 4) ? 4 : (rte && std::atoi(rte) == 2) ? 2 : (g4 < 2L * ncu ? 2 : 4);
   const dim3 grid((nrt + rt - 1) / rt, (nct + 15) / 16);
   auto kern = rt == 4 ? (cheb ? (split == 3 ? k_gemm_nn_pre<true, 3, 4> : k_gemm_nn_pre<true, 2, 4>)
                               : (split == 3 ? k_gemm_nn_pre<false, 3, 4> : k_gemm_nn_pre<false, 2, 4>))
