@@ -709,8 +709,11 @@ extern "C" int pw_rayleigh_ritz(pw_ctx* c, int hdtype, int nb, void* X, double* 
                               T, sizeof(double) * (size_t)n));
     PW_TRY(chol_upper(c, nb, c->d_S, nb, c->d_chol, c->d_info + 1));  // word 0 is the eigensolver's
     PW_HIP(hipStreamWaitEvent(s, c->ev_join, 0));
-    PW_TRY(trsm_right_upper(c, m, nb, (double*)X, m, c->d_S, nb, c->d_chol));
-    PW_TRY(trsm_right_upper(c, m, nb, HX, m, c->d_S, nb, c->d_chol));
+    // X U^-1 and (H Y) U^-1 in one launch (batch index 0: X, 1: HX at the
+    // element offset between the two blocks; the same U and Dinv): twice the
+    // workgroups of one 13134-row solve, which alone leaves CUs idle
+    PW_TRY(trsm_right_upper_batched(c, m, nb, (double*)X, m, (long)(HX - (double*)X), c->d_S, nb, 0L, c->d_chol,
+                                    0L, 2));
   } else {
     PW_TRY(pw_orthogonalize(c, nb, X));
     PW_TRY(rr_apply_h(c, hdtype, nb, X, HX));
