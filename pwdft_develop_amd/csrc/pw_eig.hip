@@ -27,8 +27,8 @@
 //   k_larft           the triangular factor T of every compact-WY chunk of
 //                     reflectors (dlarft), one workgroup per chunk
 //   k_ormtr           Z <- Q Z_T chunk by chunk (dlarfb): every workgroup
-//                     applies all chunks to its own block of 8 columns held
-//                     in LDS
+//                     applies all chunks to its own block of kOrmCols (2)
+//                     columns held in LDS
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -1065,7 +1065,7 @@ k_larft(const double* __restrict__ A, int lda, const double* __restrict__ tau, i
 // Z <- H_0 H_1 ... H_{n-3} Z for the kOrmCols columns of this workgroup, in
 // compact-WY chunks of nbw reflectors (LAPACK dlarfb with the T of k_larft):
 // for the chunk H_kb ... H_{ke-1} = I - V T V^T,  Z <- Z - V (T (V^T Z)),
-// chunks from the last to the first.  V, T and the 8-column block of Z live
+// chunks from the last to the first.  V, T and the kOrmCols-column block of Z live
 // in LDS; no per-reflector reductions.  blockDim = 256.
 __global__ void __launch_bounds__(256)
 k_ormtr(const double* __restrict__ A, int lda, const double* __restrict__ Tg, int n, int nbw,
