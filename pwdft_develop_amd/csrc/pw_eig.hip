@@ -43,7 +43,7 @@ constexpr int kTrdThreads = 1024;  // k_sytrd workgroup
 constexpr int kTrdWaves = kTrdThreads / 64;
 constexpr int kOrmCols = 2;         // columns per k_ormtr workgroup (n / 2 workgroups)
 constexpr int kEigMaxN = 1024;      // LDS bound of k_sytrd
-constexpr int kSteinLdsMaxN = 768;  // 9 n doubles of k_stein_lds within 64 KiB
+constexpr int kSteinLdsMaxN = 768;  // 10 n doubles of k_stein_lds within 64 KiB
 
 // Block-wide sum, identical order on every call (deterministic), returned to
 // all threads.  sred holds one double per wavefront.
@@ -778,13 +778,23 @@ __device__ void stein_factor(const double* d, const double* e, int n, double lam
   double y0 = start_value(jj, 0);
   for (int i0 = 0; i0 < n - 1; i0 += kSteinB) {
     double db[kSteinB], eb[kSteinB];
-    // unconditional loads at clamped rows (a guarded load becomes a branch
-    // and a wait on every row's stores); rows past the end are not used
+    // unconditional loads (a guarded load becomes a branch and a wait on
+    // every row's stores); whole batches at affine rows (one base register
+    // and immediate offsets), the last one at clamped rows (not used past
+    // the end)
+    if (i0 + kSteinB <= n - 2) {
 #pragma unroll
-    for (int u = 0; u < kSteinB; u++) {
-      const int i = min(i0 + u, n - 2);
-      db[u] = d[i + 1];
-      eb[u] = e[min(i + 1, n - 2)];
+      for (int u = 0; u < kSteinB; u++) {
+        db[u] = d[i0 + u + 1];
+        eb[u] = e[i0 + u + 1];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < kSteinB; u++) {
+        const int i = min(i0 + u, n - 2);
+        db[u] = d[i + 1];
+        eb[u] = e[min(i + 1, n - 2)];
+      }
     }
 #pragma unroll
     for (int u = 0; u < kSteinB; u++) {
@@ -829,12 +839,22 @@ __device__ void stein_forward(int n, double scale, SteinRows R, int nthr, int q)
   double y0 = R.x[q] * scale;
   for (int i0 = 0; i0 < n - 1; i0 += kSteinB) {
     double xb[kSteinB], mb[kSteinB], pb[kSteinB];
+    if (i0 + kSteinB - 1 <= n - 2) {  // whole batch: affine rows
 #pragma unroll
-    for (int u = 0; u < kSteinB; u++) {
-      const size_t o = (size_t)min(i0 + u, n - 2) * nthr + q;  // clamped: unconditional loads
-      xb[u] = R.x[o + nthr];
-      mb[u] = R.ml[o];
-      pb[u] = R.pv[o];
+      for (int u = 0; u < kSteinB; u++) {
+        const size_t o = (size_t)(i0 + u) * nthr + q;
+        xb[u] = R.x[o + nthr];
+        mb[u] = R.ml[o];
+        pb[u] = R.pv[o];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < kSteinB; u++) {
+        const size_t o = (size_t)min(i0 + u, n - 2) * nthr + q;  // clamped: unconditional loads
+        xb[u] = R.x[o + nthr];
+        mb[u] = R.ml[o];
+        pb[u] = R.pv[o];
+      }
     }
 #pragma unroll
     for (int u = 0; u < kSteinB; u++) {
@@ -855,13 +875,24 @@ __device__ double stein_back(int n, SteinRows R, int nthr, int q) {
   double xn1 = 0.0, xn2 = 0.0, ss = 0.0;
   for (int i0 = n - 1; i0 >= 0; i0 -= kSteinB) {
     double yb[kSteinB], ab[kSteinB], bb[kSteinB], rb[kSteinB];
+    if (i0 - (kSteinB - 1) >= 0) {  // whole batch: affine rows
 #pragma unroll
-    for (int u = 0; u < kSteinB; u++) {
-      const size_t o = (size_t)max(i0 - u, 0) * nthr + q;  // clamped: unconditional loads
-      yb[u] = R.y[o];
-      ab[u] = R.u1[o];
-      bb[u] = R.u2[o];
-      rb[u] = R.r0[o];
+      for (int u = 0; u < kSteinB; u++) {
+        const size_t o = (size_t)(i0 - u) * nthr + q;
+        yb[u] = R.y[o];
+        ab[u] = R.u1[o];
+        bb[u] = R.u2[o];
+        rb[u] = R.r0[o];
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < kSteinB; u++) {
+        const size_t o = (size_t)max(i0 - u, 0) * nthr + q;  // clamped: unconditional loads
+        yb[u] = R.y[o];
+        ab[u] = R.u1[o];
+        bb[u] = R.u2[o];
+        rb[u] = R.r0[o];
+      }
     }
 #pragma unroll
     for (int u = 0; u < kSteinB; u++) {
@@ -956,10 +987,11 @@ k_stein(const double* __restrict__ dg, const double* __restrict__ eg, const doub
   stein_cluster(d, e, w, n, j, nrm, Zt, R, n, j);
 }
 
-// One eigenvalue per workgroup (one wavefront; lane 0 iterates), its seven
-// scratch rows in LDS next to d and e: every step of the serial
+// One eigenvalue per workgroup (one wavefront; lane 0 iterates), its
+// scratch (seven fields per row, rows interleaved) in LDS next to d and e: every step of the serial
 // factorisation and solves is an LDS round trip instead of an L2 one, and
-// the n workgroups spread over the CUs.  9 n doubles of LDS.
+// the n workgroups spread over the CUs.  10 n doubles of LDS (d, e and
+// eight doubles per row).
 __global__ void __launch_bounds__(64)
 k_stein_lds(const double* __restrict__ dg, const double* __restrict__ eg, const double* __restrict__ w, int n,
             double* __restrict__ Zt, EigBatch bs) {
@@ -979,8 +1011,11 @@ k_stein_lds(const double* __restrict__ dg, const double* __restrict__ eg, const 
   double gl, gu, nrm, em;
   tri_bounds(dg, eg, n, sred, &gl, &gu, &nrm, &em);  // ends with a barrier
   if (threadIdx.x != 0) return;
-  SteinRows R{S, S + n, S + 2 * n, S + 3 * n, S + 4 * n, S + 5 * n, S + 6 * n};
-  stein_cluster(d, e, w, n, blockIdx.x, nrm, Zt, R, 1, 0);
+  // the seven fields of row i at S + 8 i + f (the row stride as the
+  // "thread count" of the shared indexing: every access of a batch is one
+  // base address plus an immediate offset)
+  SteinRows R{S, S + 1, S + 2, S + 3, S + 4, S + 5, S + 6};
+  stein_cluster(d, e, w, n, blockIdx.x, nrm, Zt, R, 8, 0);
 }
 
 // V of the chunk of reflectors kb .. kb + b - 1 into LDS, row-major with the
@@ -1226,7 +1261,7 @@ int eig_sym(pw_ctx* c, int n, double* A, int lda, double* w) {
   hipLaunchKernelGGL(k_stebz, dim3(n), dim3(256), sizeof(double) * 2 * n, s, (const double*)d,
                      (const double*)e, n, w, EigBatch{});
   if (n <= kSteinLdsMaxN)
-    hipLaunchKernelGGL(k_stein_lds, dim3(n), dim3(64), sizeof(double) * 9 * n, s, (const double*)d,
+    hipLaunchKernelGGL(k_stein_lds, dim3(n), dim3(64), sizeof(double) * 10 * n, s, (const double*)d,
                        (const double*)e, (const double*)w, n, Zt, EigBatch{});
   else
     hipLaunchKernelGGL(k_stein, dim3((n + 63) / 64), dim3(64), sizeof(double) * 2 * n, s, (const double*)d,
@@ -1355,7 +1390,7 @@ int sygvd_batched(pw_ctx* c, int n, double* A, int lda, long sA, double* B, int 
   }
   hipLaunchKernelGGL(k_stebz, dim3(n, nbatch), dim3(256), sizeof(double) * 2 * n, s, (const double*)d,
                      (const double*)e, n, w, bs);
-  hipLaunchKernelGGL(k_stein_lds, dim3(n, nbatch), dim3(64), sizeof(double) * 9 * n, s, (const double*)d,
+  hipLaunchKernelGGL(k_stein_lds, dim3(n, nbatch), dim3(64), sizeof(double) * 10 * n, s, (const double*)d,
                      (const double*)e, (const double*)w, n, Zt, bs);
   const size_t lds_orm = sizeof(double) * ((size_t)n * kOrmCols + (size_t)n * (nbw + 1) + (size_t)nbw * nbw +
                                            2 * (size_t)nbw * kOrmCols);
