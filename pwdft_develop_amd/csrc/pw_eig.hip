@@ -623,15 +623,17 @@ __device__ __forceinline__ double fast_rcp(double p) {
 // Numbers of eigenvalues of T below x0 and below x1 (Sturm sequences of the
 // LDL^T pivots, dstebz's recurrence with pivmin), the two chains
 // interleaved so that each hides the other's latency.
-__device__ __forceinline__ void sturm_count2(const double* d, const double* e2, int n, double x0, double x1,
-                                             double pivmin, int& c0, int& c1) {
+// d and e are read at the same index in every lane (scalar loads through
+// the constant cache; e^2 formed here as the LDS copy of k_stebz forms it).
+__device__ __forceinline__ void sturm_count2(const double* __restrict__ d, const double* __restrict__ e, int n,
+                                             double x0, double x1, double pivmin, int& c0, int& c1) {
   double q0 = d[0] - x0, q1 = d[0] - x1;
   if (fabs(q0) < pivmin) q0 = -pivmin;
   if (fabs(q1) < pivmin) q1 = -pivmin;
   int n0 = q0 < 0.0, n1 = q1 < 0.0;
-#pragma unroll 4
+#pragma unroll 8
   for (int i = 1; i < n; i++) {
-    const double di = d[i], ei = e2[i - 1];
+    const double di = d[i], ei = e[i - 1] * e[i - 1];
     q0 = (di - x0) - ei * fast_rcp(q0);
     q1 = (di - x1) - ei * fast_rcp(q1);
     if (fabs(q0) < pivmin) q0 = -pivmin;
@@ -689,18 +691,11 @@ __device__ void tri_bounds(const double* d, const double* e, int n, double* sred
 constexpr int kBzPts = 512;
 __global__ void __launch_bounds__(256)
 k_stebz(const double* __restrict__ dg, const double* __restrict__ eg, int n, double* __restrict__ w, EigBatch bs) {
-  extern __shared__ double smd[];
   dg += blockIdx.y * bs.v;
   eg += blockIdx.y * bs.v;
   w += blockIdx.y * bs.w;
-  double* d = smd;
-  double* e2 = d + n;
   __shared__ double sred[16];
   __shared__ int sfirst[2][4];
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
-    d[i] = dg[i];
-    if (i < n - 1) e2[i] = eg[i] * eg[i];
-  }
   double gl, gu, nrm, em;
   tri_bounds(dg, eg, n, sred, &gl, &gu, &nrm, &em);
   __syncthreads();
@@ -718,7 +713,7 @@ k_stebz(const double* __restrict__ dg, const double* __restrict__ eg, int n, dou
   for (int it = 0; it < 16; it++) {
     if (hi - lo <= fmax(fmax(atol, pivmin), rtol * fmax(fabs(lo), fabs(hi)))) break;  // uniform
     int c0, c1;
-    sturm_count2(d, e2, n, point(2 * t), point(2 * t + 1), pivmin, c0, c1);
+    sturm_count2(dg, eg, n, point(2 * t), point(2 * t + 1), pivmin, c0, c1);
     // first point p with count >= j + 1 (lambda_j < x_p); kBzPts if none
     const unsigned long long m1 = __ballot(c1 >= j + 1);
     int first = kBzPts;
@@ -1258,7 +1253,7 @@ int eig_sym(pw_ctx* c, int n, double* A, int lda, double* w) {
                        n, nbw, Tw, EigBatch{});
     PW_HIP(hipEventRecord(c->ev_join, c->side));
   }
-  hipLaunchKernelGGL(k_stebz, dim3(n), dim3(256), sizeof(double) * 2 * n, s, (const double*)d,
+  hipLaunchKernelGGL(k_stebz, dim3(n), dim3(256), 0, s, (const double*)d,
                      (const double*)e, n, w, EigBatch{});
   if (n <= kSteinLdsMaxN)
     hipLaunchKernelGGL(k_stein_lds, dim3(n), dim3(64), sizeof(double) * 10 * n, s, (const double*)d,
@@ -1388,7 +1383,7 @@ int sygvd_batched(pw_ctx* c, int n, double* A, int lda, long sA, double* B, int 
     hipLaunchKernelGGL(k_larft, dim3(nchunk, nbatch), dim3(256), lds_t, s, (const double*)C, n, (const double*)tau,
                        n, nbw, Tw, bs);
   }
-  hipLaunchKernelGGL(k_stebz, dim3(n, nbatch), dim3(256), sizeof(double) * 2 * n, s, (const double*)d,
+  hipLaunchKernelGGL(k_stebz, dim3(n, nbatch), dim3(256), 0, s, (const double*)d,
                      (const double*)e, n, w, bs);
   hipLaunchKernelGGL(k_stein_lds, dim3(n, nbatch), dim3(64), sizeof(double) * 10 * n, s, (const double*)d,
                      (const double*)e, (const double*)w, n, Zt, bs);
